@@ -1,0 +1,198 @@
+/*
+ * adb_capi.h - C ABI of the MI355X-native ADB Newton contact step.
+ *
+ * This header is the drop-in boundary for the reference's per-timestep contact
+ * step (`ABD::implicit_euler` and everything under it, model/abd.cpp:110-220) and
+ * for the kernel-level functions the reference exposes through its `abdtk`
+ * Python module (python/bindings.cpp:15-40,78-82).  Every entry point takes
+ * plain pointers and sizes; no C++ or torch types cross the boundary.
+ *
+ * Conventions
+ *   - Body generalised coordinates are 12 doubles per body in the reference's
+ *     order q = [p, a1, a2, a3] (model/affine_body.h:24, model/mutils.cpp:44-51);
+ *     a world vertex is x = [a1 a2 a3] * xbar + p.
+ *   - 12x12 matrices are column-major (Eigen's default storage, which the
+ *     reference's CSC values use: model/assemble_matrices.cpp:8-15).
+ *   - Distance arguments are SQUARED distances (model/barrier.cpp:7-8).
+ *   - Enum values equal ipc-toolkit's PointTriangleDistanceType and
+ *     EdgeEdgeDistanceType (python/bindings.cpp:54-76).
+ *   - No entry point ever exits the process: errors are returned as status codes
+ *     (the reference calls exit(1): model/abd.cpp:368-371, model/iaabb.cpp:305-314).
+ */
+#ifndef ADB_CAPI_H
+#define ADB_CAPI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+#define ADB_OK               0
+#define ADB_E_ARG           -1  /* bad argument / size                          */
+#define ADB_E_HIP           -2  /* HIP runtime error                            */
+#define ADB_E_NAN           -3  /* NaN in the Newton direction (abd.cpp:368)    */
+#define ADB_E_TOI_STALL     -4  /* toi < 1e-6 twice in a row (iaabb.cpp:305)    */
+#define ADB_E_CAPACITY      -5  /* a fixed-capacity buffer overflowed           */
+#define ADB_E_NOT_CONVERGED -6  /* Newton hit max_iter without ||dq|| < tol     */
+#define ADB_E_NO_DEVICE     -7  /* no usable gfx950 device                      */
+
+/* ---- distance-type enums (ipc-toolkit numbering) ------------------------- */
+enum adb_pt_type { ADB_P_T0 = 0, ADB_P_T1, ADB_P_T2, ADB_P_E0, ADB_P_E1, ADB_P_E2, ADB_P_T, ADB_PT_AUTO };
+enum adb_ee_type { ADB_EA0_EB0 = 0, ADB_EA0_EB1, ADB_EA1_EB0, ADB_EA1_EB1, ADB_EA_EB0, ADB_EA_EB1,
+                   ADB_EA0_EB, ADB_EA1_EB, ADB_EA_EB, ADB_EE_AUTO };
+
+/* ---- scene ----------------------------------------------------------------
+ * Replaces the reference's std::vector<std::unique_ptr<AffineBody>>
+ * (model/affine_body.h:11-108): rest vertices, triangles, edges, mass, Ic,
+ * q0 and dqdt per body.  Offsets are prefix sums (n_bodies + 1 entries).
+ * Face and edge indices are LOCAL to their body (as AffineBody::indices and
+ * AffineBody::edges).  mass < 0 marks a static body (model/mutils.cpp:55,71). */
+typedef struct adb_scene {
+    int n_bodies;
+    const int*    vert_offset;   /* [n_bodies+1]                      */
+    const int*    face_offset;   /* [n_bodies+1]                      */
+    const int*    edge_offset;   /* [n_bodies+1]                      */
+    const double* rest_verts;    /* [V*3]  xbar                        */
+    const int*    faces;         /* [F*3]  local vertex ids            */
+    const int*    edges;         /* [E*2]  local vertex ids            */
+    const double* mass;          /* [n_bodies]                        */
+    const double* Ic;            /* [n_bodies]                        */
+    const double* q0;            /* [n_bodies*12]                     */
+    const double* dqdt;          /* [n_bodies*12]                     */
+} adb_scene;
+
+/* ---- parameters -----------------------------------------------------------
+ * Every constant the reference reads from config.json, model/settings.h or
+ * hard-codes (model/barrier.cpp:7-8, model/othogonal_energy.cpp:8,
+ * model/geometry.cpp:60) is explicit here; nothing is global or static. */
+typedef struct adb_params {
+    double dt;            /* config.json:5                                        */
+    double d_hat;         /* squared barrier threshold, barrier::d_hat            */
+    double kappa;         /* barrier stiffness, config.json:55                    */
+    double kappa_ortho;   /* orthogonality stiffness, othogonal_energy.cpp:8      */
+    double mu;            /* friction coefficient, config.json:21                 */
+    double evh;           /* friction eps = dt * evh, config.json:51              */
+    double eps_x;         /* EE mollifier scale, config.json:29                   */
+    double tol;           /* Newton tolerance on ||dq||, config.json:57           */
+    double c1;            /* Armijo constant, config.json:60                      */
+    double backoff;       /* CCD backoff factor, config.json:23                   */
+    double max_uk;        /* friction lower branch threshold, config.json:61      */
+    double gravity[3];    /* settings.h:8                                         */
+    double ground_y;      /* ground plane height, geometry.cpp:60 (-0.5)          */
+    int    max_iter;      /* Newton iterations per step                           */
+    int    max_ls_iter;   /* line-search trials per Newton iteration              */
+    int    enable_pt, enable_ee, enable_ground;
+    int    pt_fric, ee_fric, vg_fric;
+    int    psd;           /* PSD projection of contact Hessians, config.json:30   */
+    int    line_search;   /* config.json:14                                       */
+    int    upper_bound;   /* CCD step bound, config.json:20 ("uppper_bound")      */
+    int    broad_phase_fp32; /* 1: FP32 conservative AABBs + exact FP64 filter    */
+    double pcg_rtol;      /* PCG relative residual target                         */
+    int    pcg_max_iter;
+} adb_params;
+
+/* Fills p with the reference's defaults (config.json, model/settings.h). */
+void adb_params_default(adb_params* p);
+
+typedef struct adb_step_stats {
+    int    newton_iters;
+    int    ls_trials;          /* total line-search trials in the step            */
+    int    pcg_iters;          /* total PCG iterations in the step                */
+    int    n_pt, n_ee, n_vg;   /* contact counts at the end of the step           */
+    long long contacts_processed; /* sum over Newton iterations of active contacts */
+    double toi_min;            /* smallest CCD step bound seen                    */
+    double t_ipc, t_solver, t_ccd, t_line_search; /* seconds, the reference's 4
+                                   phase buckets (model/abd.h:36-39)              */
+    double t_total;
+} adb_step_stats;
+
+typedef struct adb_world adb_world;
+
+/* ---- world / step API ----------------------------------------------------
+ * Replaces ABD::ABD (model/abd.h:13-14) and ABD::implicit_euler
+ * (model/abd.cpp:110-220).  The scene is copied; the caller's arrays are not
+ * retained.  One world per (thread, device). */
+int  adb_world_create(const adb_scene* scene, const adb_params* params, adb_world** out);
+void adb_world_destroy(adb_world* w);
+int  adb_step(adb_world* w, adb_step_stats* stats);
+int  adb_get_state(adb_world* w, double* q, double* qdot);               /* [B*12] each */
+int  adb_set_state(adb_world* w, const double* q, const double* qdot);
+const char* adb_last_error(adb_world* w);
+
+/* ---- stage entry points (parity tests) -----------------------------------
+ * adb_collision_set: replaces IAABB::iaabb_brute_force(vtn = 1 or 2)
+ * (model/iaabb.cpp:545-568).  vtn = 1 culls at the current q, vtn = 2 at
+ * q + dq.  Outputs are sorted canonically: PT by (I, v, J, f), EE by
+ * (I, ei, J, ej) with I < J, VG by (I, v).  Capacities are in entries; the
+ * true counts are always written, and ADB_E_CAPACITY is returned when one
+ * exceeds its capacity. */
+int adb_collision_set(adb_world* w, int vtn, const double* dq,
+                      int* pt, int pt_cap, int* n_pt,      /* int32[4] per entry */
+                      int* ee, int ee_cap, int* n_ee,      /* int32[4] per entry */
+                      int* vg, int vg_cap, int* n_vg);     /* int32[2] per entry */
+
+/* Replaces IAABB::iaabb_brute_force(vtn = 3) / step_size_upper_bound
+ * (model/iaabb.cpp:545-568, model/upper_bound.cpp:16): the CCD step bound
+ * along q -> q + dq (dq: [B*12]). */
+int adb_step_size_upper_bound(adb_world* w, const double* dq, double* toi);
+
+/* Contact gradient/Hessian assembly at the current q, with friction lagged on
+ * the step's start positions: replaces ABD::ipc + per-body inertia terms +
+ * ABD::solve's diagonal assembly (model/abd.cpp:148-160,319-340,385-485).
+ * Writes the assembled global gradient [B*12] and the dense 12B x 12B
+ * Hessian (column-major) when hess_dense != NULL (small scenes only). */
+int adb_assemble(adb_world* w, double* grad, double* hess_dense);
+
+/* Newton direction dq = -H^{-1} g at the current q (ABD::solve,
+ * model/abd.cpp:309-383) using block-Jacobi PCG. */
+int adb_newton_direction(adb_world* w, double* dq);
+
+/* ---- kernel-level API (abdtk) ---------------------------------------------
+ * Batched device versions: all pointers are DEVICE pointers, n entries,
+ * launched on `stream` (a hipStream_t, 0 = default).  Constants follow the
+ * reference's frozen abdtk configuration (barrier::d_hat, barrier::kappa,
+ * globals.psd, globals.eps_x) unless set through adb_kernel_config. */
+typedef struct adb_kernel_config {
+    double d_hat, kappa, eps_x;
+    int    psd;
+} adb_kernel_config;
+void adb_kernel_config_default(adb_kernel_config* c);
+
+/* IPC::ipc_hess_pt_12x12 (model/core_ipc.cpp:21-46): pt = [p, t0, t1, t2]
+ * (12 doubles per entry), type per entry, squared distance per entry ->
+ * H (144, column-major) and g (12) per entry. */
+int adb_ipc_hess_pt_12x12_batched(const adb_kernel_config* cfg, int n,
+                                  const double* pt, const int* type, const double* dist,
+                                  double* H, double* g, void* stream);
+/* IPC::ipc_hess_ee_12x12 (model/core_ipc.cpp:47-83) -> H, g, mollifier. */
+int adb_ipc_hess_ee_12x12_batched(const adb_kernel_config* cfg, int n,
+                                  const double* ee, const int* type, const double* dist,
+                                  double* H, double* g, double* mollifier, void* stream);
+/* pt_collision_time / ee_collision_time (model/collision.cpp:299-348):
+ * x = 8 points (24 doubles: 4 at t0 then 4 at t1) per entry -> toi. */
+int adb_pt_collision_time_batched(int n, const double* x, double* toi, void* stream);
+int adb_ee_collision_time_batched(int n, const double* x, double* toi, void* stream);
+
+/* Single-entry host-pointer conveniences with the abdtk signatures
+ * (python/bindings.cpp:15-40).  They run the same device kernels. */
+int adb_ipc_hess_pt_12x12(const adb_kernel_config* cfg, const double pt[12], int pt_type,
+                          double dist, double H[144], double g[12]);
+int adb_ipc_hess_ee_12x12(const adb_kernel_config* cfg, const double ee[12], int ee_type,
+                          double dist, double H[144], double g[12], double* mollifier);
+double adb_pt_collision_time(const double x[24]);
+double adb_ee_collision_time(const double x[24]);
+
+/* Distance type + squared distance (ipc-toolkit point_triangle_distance_type /
+ * point_triangle_distance and the edge-edge pair, called at
+ * model/geometry.cpp:101-112 and model/abd.cpp:433-434), batched on device. */
+int adb_pt_distance_batched(int n, const double* pt, int* type, double* dist, void* stream);
+int adb_ee_distance_batched(int n, const double* ee, int* type, double* dist, void* stream);
+
+/* Library build/version probe: returns the number of gfx950 devices visible,
+ * or a negative status.  Does not allocate device memory. */
+int adb_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ADB_CAPI_H */

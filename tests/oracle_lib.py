@@ -1,0 +1,296 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline use it, always as the checker.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from adb_amd.capi import AdbKernelConfig, AdbParams, AdbScene, AdbStepStats, SceneArrays, dptr, iptr
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(ORACLE_SO):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    L = C.CDLL(ORACLE_SO)
+    vp = C.c_void_p
+    sig = {
+        "oracle_world_create": (C.c_int, [C.POINTER(AdbScene), C.POINTER(AdbParams), C.POINTER(vp)]),
+        "oracle_world_destroy": (None, [vp]),
+        "oracle_step": (C.c_int, [vp, C.POINTER(AdbStepStats)]),
+        "oracle_get_state": (C.c_int, [vp, _dp, _dp]),
+        "oracle_set_state": (C.c_int, [vp, _dp, _dp]),
+        "oracle_collision_set": (C.c_int, [vp, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip, _ip, C.c_int, _ip, _ip, C.c_int, _ip]),
+        "oracle_step_size_upper_bound": (C.c_int, [vp, _dp, C.c_int, _dp]),
+        "oracle_assemble": (C.c_int, [vp, _dp, _dp]),
+        "oracle_newton_direction": (C.c_int, [vp, _dp]),
+        "oracle_energy": (C.c_int, [vp, _dp, _dp]),
+        "oracle_set_threads": (C.c_int, [C.c_int]),
+        "oracle_pt_distance": (None, [_dp, _ip, _dp]),
+        "oracle_ee_distance": (None, [_dp, _ip, _dp]),
+        "oracle_pt_distance_derivs": (None, [_dp, C.c_int, _dp, _dp]),
+        "oracle_ee_distance_derivs": (None, [_dp, C.c_int, _dp, _dp]),
+        "oracle_ee_mollifier": (None, [_dp, C.c_double, _dp, _dp, _dp]),
+        "oracle_barrier": (None, [C.c_double, C.c_double, C.c_double, _dp]),
+        "oracle_sym_eig": (None, [C.c_int, _dp, _dp, _dp]),
+        "oracle_psd_project": (None, [C.c_int, _dp, _dp]),
+        "oracle_ipc_hess_pt_12x12": (C.c_int, [C.POINTER(AdbKernelConfig), _dp, C.c_int, C.c_double, _dp, _dp]),
+        "oracle_ipc_hess_ee_12x12": (C.c_int, [C.POINTER(AdbKernelConfig), _dp, C.c_int, C.c_double, _dp, _dp, _dp]),
+        "oracle_pt_collision_time": (C.c_double, [_dp]),
+        "oracle_ee_collision_time": (C.c_double, [_dp]),
+        "oracle_det_polynomial": (None, [_dp, _dp, _dp]),
+        "oracle_cubic_roots": (C.c_int, [_dp, C.c_double, C.c_double, _dp]),
+        "oracle_friction": (None, [_dp, C.c_double, _dp, C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp]),
+        "oracle_ortho": (None, [_dp, C.c_double, _dp, _dp, _dp]),
+        "oracle_pt_closest": (None, [_dp, C.c_int, _dp, _dp]),
+        "oracle_ee_closest": (None, [_dp, C.c_int, _dp, _dp]),
+        "oracle_f0": (C.c_double, [C.c_double, C.c_double]),
+        "oracle_f1_over_x": (C.c_double, [C.c_double, C.c_double]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ---- kernel level ------------------------------------------------------------
+def pt_distance(x):
+    x = _d(x).reshape(12)
+    t = C.c_int()
+    d = C.c_double()
+    lib().oracle_pt_distance(dptr(x), C.byref(t), C.byref(d))
+    return t.value, d.value
+
+
+def ee_distance(x):
+    x = _d(x).reshape(12)
+    t = C.c_int()
+    d = C.c_double()
+    lib().oracle_ee_distance(dptr(x), C.byref(t), C.byref(d))
+    return t.value, d.value
+
+
+def pt_distance_derivs(x, t):
+    x = _d(x).reshape(12)
+    g = np.zeros(12)
+    H = np.zeros(144)
+    lib().oracle_pt_distance_derivs(dptr(x), t, dptr(g), dptr(H))
+    return g, H.reshape(12, 12).T
+
+
+def ee_distance_derivs(x, t):
+    x = _d(x).reshape(12)
+    g = np.zeros(12)
+    H = np.zeros(144)
+    lib().oracle_ee_distance_derivs(dptr(x), t, dptr(g), dptr(H))
+    return g, H.reshape(12, 12).T
+
+
+def ee_mollifier(x, eps):
+    x = _d(x).reshape(12)
+    m = C.c_double()
+    g = np.zeros(12)
+    H = np.zeros(144)
+    lib().oracle_ee_mollifier(dptr(x), eps, C.byref(m), dptr(g), dptr(H))
+    return m.value, g, H.reshape(12, 12).T
+
+
+def barrier(d, d_hat, kappa):
+    out = np.zeros(3)
+    lib().oracle_barrier(d, d_hat, kappa, dptr(out))
+    return out
+
+
+def sym_eig(A):
+    A = _d(A)
+    n = A.shape[0]
+    a = _d(A.T.reshape(-1))
+    w = np.zeros(n)
+    V = np.zeros(n * n)
+    lib().oracle_sym_eig(n, dptr(a), dptr(w), dptr(V))
+    return w, V.reshape(n, n).T
+
+
+def psd_project(A):
+    A = _d(A)
+    n = A.shape[0]
+    a = _d(A.T.reshape(-1))
+    out = np.zeros(n * n)
+    lib().oracle_psd_project(n, dptr(a), dptr(out))
+    return out.reshape(n, n).T
+
+
+def ipc_hess_pt(cfg, x, t, dist):
+    x = _d(x).reshape(12)
+    H = np.zeros(144)
+    g = np.zeros(12)
+    lib().oracle_ipc_hess_pt_12x12(C.byref(cfg), dptr(x), t, dist, dptr(H), dptr(g))
+    return H.reshape(12, 12).T, g
+
+
+def ipc_hess_ee(cfg, x, t, dist):
+    x = _d(x).reshape(12)
+    H = np.zeros(144)
+    g = np.zeros(12)
+    m = C.c_double()
+    lib().oracle_ipc_hess_ee_12x12(C.byref(cfg), dptr(x), t, dist, dptr(H), dptr(g), C.byref(m))
+    return H.reshape(12, 12).T, g, m.value
+
+
+def pt_collision_time(x):
+    x = _d(x).reshape(24)
+    return lib().oracle_pt_collision_time(dptr(x))
+
+
+def ee_collision_time(x):
+    x = _d(x).reshape(24)
+    return lib().oracle_ee_collision_time(dptr(x))
+
+
+def det_polynomial(a, b):
+    a = _d(np.asarray(a).T.reshape(-1))
+    b = _d(np.asarray(b).T.reshape(-1))
+    out = np.zeros(4)
+    lib().oracle_det_polynomial(dptr(a), dptr(b), dptr(out))
+    return out
+
+
+def cubic_roots(coef, x0=0.0, x1=1.0):
+    c = _d(coef).reshape(4)
+    r = np.zeros(3)
+    n = lib().oracle_cubic_roots(dptr(c), x0, x1, dptr(r))
+    return r[:n]
+
+
+def friction(uk, lam, T, mu, eps, max_uk=0.0):
+    T = _d(T)
+    n = T.shape[0]
+    Tc = _d(T.T.reshape(-1))
+    g = np.zeros(n)
+    H = np.zeros(n * n)
+    lib().oracle_friction(dptr(_d(uk).reshape(2)), lam, dptr(Tc), n, mu, eps, max_uk, dptr(g), dptr(H))
+    return g, H.reshape(n, n).T
+
+
+def ortho(q, kappa):
+    q = _d(q).reshape(12)
+    E = C.c_double()
+    g = np.zeros(12)
+    H = np.zeros(144)
+    lib().oracle_ortho(dptr(q), kappa, C.byref(E), dptr(g), dptr(H))
+    return E.value, g, H.reshape(12, 12).T
+
+
+def pt_closest(x, t):
+    x = _d(x).reshape(12)
+    lam = np.zeros(3)
+    Pk = np.zeros(6)
+    lib().oracle_pt_closest(dptr(x), t, dptr(lam), dptr(Pk))
+    return lam, Pk.reshape(2, 3).T
+
+
+def ee_closest(x, t):
+    x = _d(x).reshape(12)
+    lam = np.zeros(2)
+    Pk = np.zeros(6)
+    lib().oracle_ee_closest(dptr(x), t, dptr(lam), dptr(Pk))
+    return lam, Pk.reshape(2, 3).T
+
+
+def f0(x, eps):
+    return lib().oracle_f0(x, eps)
+
+
+def f1_over_x(x, eps):
+    return lib().oracle_f1_over_x(x, eps)
+
+
+# ---- world level ---------------------------------------------------------------
+class OracleWorld:
+    def __init__(self, scene, params):
+        self.arrays = SceneArrays(scene)
+        self.params = params
+        self.B = len(scene["mass"])
+        h = C.c_void_p()
+        rc = lib().oracle_world_create(C.byref(self.arrays.struct), C.byref(params), C.byref(h))
+        if rc != 0:
+            raise RuntimeError("oracle_world_create failed: %d" % rc)
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.oracle_world_destroy(self.h)
+            self.h = None
+
+    def step(self):
+        st = AdbStepStats()
+        rc = lib().oracle_step(self.h, C.byref(st))
+        return rc, st
+
+    def state(self):
+        q = np.zeros(12 * self.B)
+        qd = np.zeros(12 * self.B)
+        lib().oracle_get_state(self.h, dptr(q), dptr(qd))
+        return q.reshape(-1, 12), qd.reshape(-1, 12)
+
+    def set_state(self, q, qdot=None):
+        q = _d(q).reshape(-1)
+        qd = None if qdot is None else _d(qdot).reshape(-1)
+        lib().oracle_set_state(self.h, dptr(q), dptr(qd) if qd is not None else None)
+
+    def collision_set(self, vtn=1, dq=None, method=1, cap=2_000_000):
+        pt = np.zeros(4 * cap, dtype=np.int32)
+        ee = np.zeros(4 * cap, dtype=np.int32)
+        vg = np.zeros(2 * cap, dtype=np.int32)
+        npt, nee, nvg = C.c_int(), C.c_int(), C.c_int()
+        dqa = None if dq is None else _d(dq).reshape(-1)
+        rc = lib().oracle_collision_set(self.h, vtn, dptr(dqa) if dqa is not None else None, method,
+                                        iptr(pt), cap, C.byref(npt), iptr(ee), cap, C.byref(nee),
+                                        iptr(vg), cap, C.byref(nvg))
+        if rc != 0:
+            raise RuntimeError("oracle_collision_set: %d" % rc)
+        return (pt[:4 * npt.value].reshape(-1, 4).copy(), ee[:4 * nee.value].reshape(-1, 4).copy(),
+                vg[:2 * nvg.value].reshape(-1, 2).copy())
+
+    def toi(self, dq, method=1):
+        t = C.c_double()
+        dqa = _d(dq).reshape(-1)
+        lib().oracle_step_size_upper_bound(self.h, dptr(dqa), method, C.byref(t))
+        return t.value
+
+    def assemble(self, dense=True):
+        n = 12 * self.B
+        g = np.zeros(n)
+        H = np.zeros(n * n) if dense else None
+        lib().oracle_assemble(self.h, dptr(g), dptr(H) if dense else None)
+        return g, (H.reshape(n, n).T if dense else None)
+
+    def energy(self, q):
+        e = C.c_double()
+        qa = _d(q).reshape(-1)
+        lib().oracle_energy(self.h, dptr(qa), C.byref(e))
+        return e.value
+
+    def newton_direction(self):
+        dq = np.zeros(12 * self.B)
+        rc = lib().oracle_newton_direction(self.h, dptr(dq))
+        return rc, dq
