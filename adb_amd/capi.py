@@ -196,6 +196,8 @@ def _declare(lib):
         "adb_pt_distance_batched": (C.c_int, [C.c_int, vp, vp, vp, vp]),
         "adb_ee_distance_batched": (C.c_int, [C.c_int, vp, vp, vp, vp]),
         "adb_device_count": (C.c_int, []),
+        "adb_profile_enable": (C.c_int, [vp, C.c_int]),
+        "adb_profile_read": (C.c_int, [vp, C.c_int, _dp, C.POINTER(C.c_longlong), _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -212,5 +214,10 @@ EXPORTED_SYMBOLS = [
     "adb_pt_collision_time_batched", "adb_ee_collision_time_batched",
     "adb_ipc_hess_pt_12x12", "adb_ipc_hess_ee_12x12", "adb_pt_collision_time",
     "adb_ee_collision_time", "adb_pt_distance_batched", "adb_ee_distance_batched",
-    "adb_device_count",
+    "adb_device_count", "adb_profile_enable", "adb_profile_read",
 ]
+
+# kernel ids of adb_profile_read (enum adb_prof_kernel)
+PROF_KERNELS = ["transform", "body_box", "body_pairs", "narrow_set", "vg_scan", "sort", "narrow_ccd", "vg_toi",
+                "contact", "vg_contact", "inertia", "asm_reduce", "block_chol", "spmv", "pcg_update", "pcg_dir",
+                "energy"]

@@ -1,0 +1,337 @@
+// assemble.hip - per-body inertia terms and block-sparse Hessian assembly.
+//
+// Replaces grad_residue_per_body / hess_inertia_per_body (model/mutils.cpp:53-76,
+// model/othogonal_energy.cpp:16-47), gen_empty_sm (model/sparse.cpp:13-57), the
+// locked scatter of IPC::output_hessian_gradient (model/assemble_matrices.cpp:28-59)
+// and the diagonal += c.hess of ABD::solve (model/abd.cpp:328-341).
+//
+// No atomics: every contribution (4 per PT/EE contact, 1 marker per body) gets
+// a 64-bit key (row << 32 | col); a stable radix sort groups equal keys in a
+// fixed order, unique keys become the blocks of a BSR matrix (12x12 blocks,
+// column-major as the reference's CSC values), and one workgroup per block
+// sums its contributions in sorted order (segmented reduction).  Static
+// bodies (mass < 0) follow the reference: contributions to them are dropped,
+// their diagonal block is I and their gradient 0.
+#include <hipcub/hipcub.hpp>
+#include "dmath.h"
+#include "world.h"
+
+namespace adb {
+
+// inertia + orthogonality per body (H column-major)
+__global__ void k_inertia(int B, double dt, double kap, double ga0, double ga1, double ga2, const double* __restrict__ q,
+                          const double* __restrict__ q0, const double* __restrict__ qdot, const double* __restrict__ mass,
+                          const double* __restrict__ Ic, double* __restrict__ bodyG, double* __restrict__ bodyH) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    double* G = bodyG + 12 * (size_t)b;
+    double* H = bodyH + 144 * (size_t)b;
+    if (mass[b] < 0.0) {
+        for (int i = 0; i < 12; i++) G[i] = 0.0;
+        for (int i = 0; i < 144; i++) H[i] = (i % 13 == 0) ? 1.0 : 0.0;
+        return;
+    }
+    const double* Q = q + 12 * (size_t)b;
+    d3 a[4] = {ld3(Q), ld3(Q + 3), ld3(Q + 6), ld3(Q + 9)};
+    double go[12];
+    go[0] = go[1] = go[2] = 0.0;
+    for (int i = 1; i < 4; i++) {
+        double s0 = 0, s1 = 0, s2 = 0;
+        for (int j = 1; j < 4; j++) {
+            double s = dot(a[i], a[j]) - (i == j ? 1.0 : 0.0);
+            s0 += s * a[j].x; s1 += s * a[j].y; s2 += s * a[j].z;
+        }
+        go[3 * i] = 4 * kap * s0; go[3 * i + 1] = 4 * kap * s1; go[3 * i + 2] = 4 * kap * s2;
+    }
+    const double m = mass[b], ic = Ic[b];
+    for (int i = 0; i < 12; i++) {
+        double qt = q0[12 * (size_t)b + i] + dt * qdot[12 * (size_t)b + i];
+        if (i < 3) qt += dt * dt * (i == 0 ? ga0 : (i == 1 ? ga1 : ga2));
+        G[i] = dt * dt * go[i] + (i < 3 ? m : ic) * (Q[i] - qt);
+    }
+    for (int i = 0; i < 144; i++) H[i] = 0.0;
+    for (int i = 0; i < 12; i++) H[13 * i] = i < 3 ? m : ic;
+    for (int i = 1; i < 4; i++)
+        for (int j = 1; j < 4; j++)
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) {
+                    double h;
+                    if (i == j) {
+                        h = 2 * cmp(a[i], r) * cmp(a[i], c) + (r == c ? dot(a[i], a[i]) - 1 : 0.0);
+                        for (int k = 1; k < 4; k++) if (k != i) h += cmp(a[k], r) * cmp(a[k], c);
+                    } else {
+                        h = (r == c ? dot(a[j], a[i]) : 0.0) + cmp(a[j], r) * cmp(a[i], c);
+                    }
+                    H[(3 * j + c) * 12 + 3 * i + r] += h * 4 * kap * dt * dt;
+                }
+}
+
+// contribution keys: B diagonal markers first, then 4 per PT/EE contact
+__global__ void k_contrib_keys(int B, int n_pt, int n_ee, const int4* __restrict__ pt, const int4* __restrict__ ee,
+                               unsigned long long* __restrict__ keys, int* __restrict__ vals) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = n_pt + n_ee;
+    if (i < B) { keys[i] = ((unsigned long long)i << 32) | (unsigned long long)i; vals[i] = -1 - i; return; }
+    i -= B;
+    if (i >= 4 * nc) return;
+    const int c = i / 4, slot = i % 4;
+    int4 t = c < n_pt ? pt[c] : ee[c - n_pt];
+    const int I = t.x, J = t.z;
+    int r, col;
+    switch (slot) {
+    case 0: r = I; col = I; break;
+    case 1: r = J; col = J; break;
+    case 2: r = I; col = J; break;
+    default: r = J; col = I; break;
+    }
+    keys[B + i] = ((unsigned long long)r << 32) | (unsigned long long)col;
+    vals[B + i] = i;
+}
+__global__ void k_flags(int n, const unsigned long long* __restrict__ keys, int* __restrict__ flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
+}
+// blk_of = inclusive scan of flags - 1
+__global__ void k_block_meta(int n, const unsigned long long* __restrict__ keys, const int* __restrict__ blk_of,
+                             int* __restrict__ blk_start, int* __restrict__ row_ptr, int* __restrict__ col_idx,
+                             int* __restrict__ diag_blk) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (i == 0 || keys[i] != keys[i - 1]) {
+        const int k = blk_of[i] - 1;
+        const int row = (int)(keys[i] >> 32), col = (int)(keys[i] & 0xffffffffull);
+        blk_start[k] = i;
+        col_idx[k] = col;
+        if (row == col) diag_blk[row] = k;
+        if (i == 0 || (int)(keys[i - 1] >> 32) != row) row_ptr[row] = k;
+    }
+}
+__global__ void k_vg_start(int B, int n_vg, const int2* __restrict__ vg, int* __restrict__ start) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > B) return;
+    int lo = 0, hi = n_vg;      // first entry with body >= b
+    while (lo < hi) { int mid = (lo + hi) / 2; if (vg[mid].x < b) lo = mid + 1; else hi = mid; }
+    start[b] = lo;
+}
+
+struct ReduceArgs {
+    int nnzb, n_pt, n_ee, B;
+    const unsigned long long* keys; const int* vals; const int* blk_start;
+    const int4* pt; const int4* ee; const double* mass;
+    const double* cblk; const double* cgrad; const double* bodyH; const double* bodyG;
+    const int2* vg; const int* vg_start; const double* vgK; const int* voff; const double* xbar;
+    double* Hval; double* grad;
+};
+// Pass 1: one 64-lane workgroup per chunk of at most RCH contributions of one
+// block (long segments - the diagonal block of a body with thousands of
+// contacts - are split so every workgroup has bounded work); lane t sums
+// entries t, t + 64, t + 128 (< 144) into the chunk's partial block.
+constexpr int RCH = 16;
+__global__ void k_chunk_count(int nnzb, const int* __restrict__ blk_start, int* __restrict__ nch) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nnzb) nch[k] = (blk_start[k + 1] - blk_start[k] + RCH - 1) / RCH;
+}
+// chunk_off = exclusive scan of nch (nnzb + 1 entries)
+__global__ void k_chunk_owner(int nnzb, const int* __restrict__ chunk_off, int* __restrict__ chunk_blk) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nnzb) for (int c = chunk_off[k]; c < chunk_off[k + 1]; c++) chunk_blk[c] = k;
+}
+__global__ __launch_bounds__(64) void k_reduce_chunks(ReduceArgs a, int nchunks, const int* __restrict__ chunk_off,
+                                                      const int* __restrict__ chunk_blk, double* __restrict__ part) {
+    const int ch = blockIdx.x;
+    if (ch >= nchunks) return;
+    const int k = chunk_blk[ch];
+    const int beg = a.blk_start[k] + (ch - chunk_off[k]) * RCH;
+    const int end = min(beg + RCH, a.blk_start[k + 1]);
+    const int t = threadIdx.x;
+    double acc[3] = {0.0, 0.0, 0.0};
+    double gacc = 0.0;
+    for (int s = beg; s < end; s++) {
+        const int v = a.vals[s];
+        if (v < 0) {     // diagonal marker: inertia terms
+            const int b = -1 - v;
+            for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) acc[u] += a.bodyH[144 * (size_t)b + e]; }
+            if (t < 12) gacc += a.bodyG[12 * (size_t)b + t];
+            continue;
+        }
+        const int c = v / 4, slot = v % 4;
+        const int4 ct = c < a.n_pt ? a.pt[c] : a.ee[c - a.n_pt];
+        const bool di = a.mass[ct.x] > 0.0, dj = a.mass[ct.z] > 0.0;
+        const double* blk = a.cblk + 432 * (size_t)c;
+        if (slot == 0 && di) {
+            for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) acc[u] += blk[e]; }
+            if (t < 12) gacc += a.cgrad[24 * (size_t)c + t];
+        } else if (slot == 1 && dj) {
+            for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) acc[u] += blk[144 + e]; }
+            if (t < 12) gacc += a.cgrad[24 * (size_t)c + 12 + t];
+        } else if (slot == 2 && di && dj) {
+            for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) acc[u] += blk[288 + e]; }
+        } else if (slot == 3 && di && dj) {
+            for (int u = 0; u < 3; u++) {
+                int e = t + 64 * u;
+                if (e < 144) { int cc = e / 12, rr = e % 12; acc[u] += blk[288 + rr * 12 + cc]; }
+            }
+        }
+    }
+    double* P = part + 156 * (size_t)ch;
+    for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) P[e] = acc[u]; }
+    if (t < 12) P[144 + t] = gacc;
+}
+
+// Pass 2: one workgroup per block sums its chunk partials in order, then the
+// body's vertex-ground terms for diagonal blocks.
+__global__ __launch_bounds__(64) void k_reduce_final(ReduceArgs a, const int* __restrict__ chunk_off, const double* __restrict__ part) {
+    const int k = blockIdx.x;
+    if (k >= a.nnzb) return;
+    const unsigned long long key = a.keys[a.blk_start[k]];
+    const int row = (int)(key >> 32), col = (int)(key & 0xffffffffull);
+    const int t = threadIdx.x;
+    double acc[3] = {0.0, 0.0, 0.0};
+    double gacc = 0.0;
+    for (int c = chunk_off[k]; c < chunk_off[k + 1]; c++) {
+        const double* P = part + 156 * (size_t)c;
+        for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) acc[u] += P[e]; }
+        if (t < 12) gacc += P[144 + t];
+    }
+    if (row == col && a.mass[row] > 0.0) {
+        // vertex-ground terms of this body: (w w^T) (x) K, gradient w (x) gx
+        for (int s = a.vg_start[row]; s < a.vg_start[row + 1]; s++) {
+            const int v = a.voff[row] + a.vg[s].y;
+            const double w4[4] = {1.0, a.xbar[3 * (size_t)v], a.xbar[3 * (size_t)v + 1], a.xbar[3 * (size_t)v + 2]};
+            const double* K = a.vgK + 12 * (size_t)s;
+            for (int u = 0; u < 3; u++) {
+                int e = t + 64 * u;
+                if (e < 144) {
+                    int cc = e / 12, rr = e % 12;
+                    acc[u] += w4[rr / 3] * w4[cc / 3] * K[3 * (cc % 3) + (rr % 3)];
+                }
+            }
+            if (t < 12) gacc += w4[t / 3] * K[9 + t % 3];
+        }
+    }
+    for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) a.Hval[144 * (size_t)k + e] = acc[u]; }
+    if (row == col && t < 12) a.grad[12 * (size_t)row + t] = gacc;
+}
+
+// block-Jacobi factors: Cholesky of each diagonal block (one thread per body,
+// matrix staged in LDS); falls back to the diagonal when not positive definite
+constexpr int CH_T = 32;
+__global__ __launch_bounds__(CH_T) void k_block_chol(int B, const int* __restrict__ diag_blk, const double* __restrict__ Hval,
+                                                    double* __restrict__ L) {
+    __shared__ double S[CH_T][144];
+    const int b = blockIdx.x * CH_T + threadIdx.x;
+    if (b >= B) return;
+    double* A = S[threadIdx.x];
+    const double* D = Hval + 144 * (size_t)diag_blk[b];
+    for (int i = 0; i < 144; i++) A[i] = D[i];
+    bool ok = true;
+    for (int j = 0; j < 12 && ok; j++) {
+        double s = A[j * 12 + j];
+        for (int k = 0; k < j; k++) s -= A[k * 12 + j] * A[k * 12 + j];
+        if (!(s > 0.0)) { ok = false; break; }
+        double l = sqrt(s);
+        A[j * 12 + j] = l;
+        for (int i = j + 1; i < 12; i++) {
+            double t = A[j * 12 + i];
+            for (int k = 0; k < j; k++) t -= A[k * 12 + i] * A[k * 12 + j];
+            A[j * 12 + i] = t / l;
+        }
+    }
+    double* out = L + 144 * (size_t)b;
+    if (ok) {
+        for (int c = 0; c < 12; c++) for (int r = 0; r < 12; r++) out[c * 12 + r] = r >= c ? A[c * 12 + r] : 0.0;
+    } else {
+        for (int i = 0; i < 144; i++) out[i] = 0.0;
+        for (int i = 0; i < 12; i++) { double d = D[13 * i]; out[13 * i] = sqrt(d > 0.0 ? d : 1.0); }
+    }
+}
+
+template <class K, class V>
+static void sort_pairs(World& w, K* kin, K* kout, V* vin, V* vout, int n, int end_bit) {
+    size_t tmp = 0;
+    ADB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kin, kout, vin, vout, n, 0, end_bit, w.stream));
+    w.sort_tmp.reserve(tmp + 16);
+    ADB_HIP(hipcub::DeviceRadixSort::SortPairs(w.sort_tmp.p, tmp, kin, kout, vin, vout, n, 0, end_bit, w.stream));
+}
+
+void assemble_system(World& w) {
+    const int B = w.B;
+    w.bodyG.reserve(12 * (size_t)B);
+    w.bodyH.reserve(144 * (size_t)B);
+    hipLaunchKernelGGL(k_inertia, dim3((B + 127) / 128), dim3(128), 0, w.stream, B, w.P.dt, w.P.kappa_ortho, w.P.gravity[0], w.P.gravity[1], w.P.gravity[2],
+                       w.q.p, w.q0.p, w.qdot.p, w.mass.p, w.Ic.p, w.bodyG.p, w.bodyH.p);
+    ADB_HIP(hipGetLastError());
+    const int nc = w.cs.n_pt + w.cs.n_ee;
+    const int n = B + 4 * nc;
+    w.n_contrib = n;
+    w.akeys.reserve(n); w.akeys_s.reserve(n); w.aval.reserve(n); w.aval_s.reserve(n);
+    w.uniq_flag.reserve(n); w.blk_of.reserve(n);
+    hipLaunchKernelGGL(k_contrib_keys, dim3((n + 255) / 256), dim3(256), 0, w.stream, B, w.cs.n_pt, w.cs.n_ee, w.cs.pt.p, w.cs.ee.p,
+                       w.akeys.p, w.aval.p);
+    ADB_HIP(hipGetLastError());
+    int bb = 1; while ((1ll << bb) <= B) bb++;
+    sort_pairs(w, w.akeys.p, w.akeys_s.p, w.aval.p, w.aval_s.p, n, 32 + bb);
+    hipLaunchKernelGGL(k_flags, dim3((n + 255) / 256), dim3(256), 0, w.stream, n, w.akeys_s.p, w.uniq_flag.p);
+    ADB_HIP(hipGetLastError());
+    {
+        size_t tmp = 0;
+        ADB_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tmp, w.uniq_flag.p, w.blk_of.p, n, w.stream));
+        w.sort_tmp.reserve(tmp + 16);
+        ADB_HIP(hipcub::DeviceScan::InclusiveSum(w.sort_tmp.p, tmp, w.uniq_flag.p, w.blk_of.p, n, w.stream));
+    }
+    int last = 0;
+    ADB_HIP(hipMemcpyAsync(&last, w.blk_of.p + n - 1, sizeof(int), hipMemcpyDeviceToHost, w.stream));
+    ADB_HIP(hipStreamSynchronize(w.stream));
+    const int nnzb = last;   // inclusive scan: count of unique keys
+    w.nnzb = nnzb;
+    w.blk_start.reserve(nnzb + 1); w.col_idx.reserve(nnzb); w.row_ptr.reserve(B + 1); w.diag_blk.reserve(B);
+    w.Hval.reserve(144 * (size_t)nnzb);
+    // blk_of currently holds (inclusive count); shift to 0-based block ids in k_block_meta via -1
+    hipLaunchKernelGGL(k_block_meta, dim3((n + 255) / 256), dim3(256), 0, w.stream, n, w.akeys_s.p, w.blk_of.p,
+                       w.blk_start.p, w.row_ptr.p, w.col_idx.p, w.diag_blk.p);
+    ADB_HIP(hipGetLastError());
+    ADB_HIP(hipMemcpyAsync(w.blk_start.p + nnzb, &w.n_contrib, sizeof(int), hipMemcpyHostToDevice, w.stream));
+    ADB_HIP(hipMemcpyAsync(w.row_ptr.p + B, &w.nnzb, sizeof(int), hipMemcpyHostToDevice, w.stream));
+    w.vg_start.reserve(B + 1);
+    hipLaunchKernelGGL(k_vg_start, dim3((B + 1 + 255) / 256), dim3(256), 0, w.stream, B, w.cs.n_vg, w.cs.vg.p, w.vg_start.p);
+    ADB_HIP(hipGetLastError());
+    w.grad.reserve(12 * (size_t)B);
+    ReduceArgs a;
+    a.nnzb = nnzb; a.n_pt = w.cs.n_pt; a.n_ee = w.cs.n_ee; a.B = B;
+    a.keys = w.akeys_s.p; a.vals = w.aval_s.p; a.blk_start = w.blk_start.p;
+    a.pt = w.cs.pt.p; a.ee = w.cs.ee.p; a.mass = w.mass.p;
+    a.cblk = w.cblk.p; a.cgrad = w.cgrad.p; a.bodyH = w.bodyH.p; a.bodyG = w.bodyG.p;
+    a.vg = w.cs.vg.p; a.vg_start = w.vg_start.p; a.vgK = w.vgK.p; a.voff = w.voff.p; a.xbar = w.xbar.p;
+    a.Hval = w.Hval.p; a.grad = w.grad.p;
+    {
+        // chunk layout of the segments
+        w.chunk_n.reserve(nnzb + 1); w.chunk_off.reserve(nnzb + 1);
+        hipLaunchKernelGGL(k_chunk_count, dim3((nnzb + 255) / 256), dim3(256), 0, w.stream, nnzb, w.blk_start.p, w.chunk_n.p);
+        ADB_HIP(hipMemsetAsync(w.chunk_n.p + nnzb, 0, sizeof(int), w.stream));
+        size_t tmp = 0;
+        ADB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, w.chunk_n.p, w.chunk_off.p, nnzb + 1, w.stream));
+        w.sort_tmp.reserve(tmp + 16);
+        ADB_HIP(hipcub::DeviceScan::ExclusiveSum(w.sort_tmp.p, tmp, w.chunk_n.p, w.chunk_off.p, nnzb + 1, w.stream));
+        int nchunks = 0;
+        ADB_HIP(hipMemcpyAsync(&nchunks, w.chunk_off.p + nnzb, sizeof(int), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipStreamSynchronize(w.stream));
+        w.chunk_blk.reserve(nchunks);
+        w.chunk_part.reserve(156 * (size_t)nchunks);
+        hipLaunchKernelGGL(k_chunk_owner, dim3((nnzb + 255) / 256), dim3(256), 0, w.stream, nnzb, w.chunk_off.p, w.chunk_blk.p);
+        ADB_HIP(hipGetLastError());
+        {
+            // every contribution block read once, one partial block written per chunk
+            ProfScope ps(w, PK_ASM_REDUCE, 1152.0 * n + 1248.0 * nchunks);
+            hipLaunchKernelGGL(k_reduce_chunks, dim3(nchunks), dim3(64), 0, w.stream, a, nchunks, w.chunk_off.p, w.chunk_blk.p,
+                               w.chunk_part.p);
+        }
+        hipLaunchKernelGGL(k_reduce_final, dim3(nnzb), dim3(64), 0, w.stream, a, w.chunk_off.p, w.chunk_part.p);
+        ADB_HIP(hipGetLastError());
+    }
+    ADB_HIP(hipGetLastError());
+    w.Lchol.reserve(144 * (size_t)B);
+    hipLaunchKernelGGL(k_block_chol, dim3((B + CH_T - 1) / CH_T), dim3(CH_T), 0, w.stream, B, w.diag_blk.p, w.Hval.p, w.Lchol.p);
+    ADB_HIP(hipGetLastError());
+}
+
+} // namespace adb

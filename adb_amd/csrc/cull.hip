@@ -1,0 +1,590 @@
+// cull.hip - broad phase, narrow-phase contact sets and the CCD step bound.
+//
+// Replaces IAABB::iaabb_brute_force (model/iaabb.cpp:23-568) and its pair
+// tests (model/thread_local_culling.cpp:28-386): vtn = 1, 2 contact sets and
+// the vtn = 3 trajectory step bound.
+//
+// Pipeline per pass (all on the device, one host read of the counters):
+//   1. per-body AABB of its vertices (contact sets: inflated by r >= sqrt(d_hat);
+//      CCD: union of the start and end positions), one workgroup per body;
+//   2. body pairs whose boxes overlap (tiled all-pairs test, wave-aggregated
+//      compaction);
+//   3. one workgroup per body pair: primitives of both bodies that touch the
+//      pair's overlap box are compacted into LDS lists, then every list pair is
+//      tested (conservative box prefilter, exact FP64 predicate) and hits are
+//      appended with one atomic per wavefront;
+//   4. vertex-ground candidates by a vertex-parallel scan;
+//   5. radix sort of 64-bit keys gives the canonical order, then decode.
+// The box prefilters are conservative, so the sets equal the exact predicate
+// over all primitive pairs (d_PT < d_hat for I != J, d_EE < d_hat for I < J).
+#include <hipcub/hipcub.hpp>
+#include <cstdio>
+#include "dmath.h"
+#include "world.h"
+
+namespace adb {
+
+constexpr int NB_THREADS = 256;
+constexpr int LCAP = 2048;      // LDS list capacity per primitive kind and side
+
+// Monotone map of doubles to unsigned 64-bit keys (negative values included),
+// so an integer atomicMin is an exact double min.
+__device__ __forceinline__ unsigned long long order_key(double t) {
+    unsigned long long b = (unsigned long long)__double_as_longlong(t);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+static double order_key_inv(unsigned long long k) {
+    unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    double t;
+    memcpy(&t, &b, 8);
+    return t;
+}
+static unsigned long long order_key_host(double t) {
+    unsigned long long b;
+    memcpy(&b, &t, 8);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+struct Box { double lo[3], hi[3]; };
+__device__ __forceinline__ bool ovl(const double* alo, const double* ahi, const double* blo, const double* bhi) {
+    return alo[0] <= bhi[0] && ahi[0] >= blo[0] && alo[1] <= bhi[1] && ahi[1] >= blo[1] && alo[2] <= bhi[2] && ahi[2] >= blo[2];
+}
+
+// ------------------------------------------------------------------ transforms
+__global__ void k_transform(int V, const int* __restrict__ vbody, const double* __restrict__ xbar,
+                            const double* __restrict__ q, double* __restrict__ X) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const double* Q = q + 12 * (size_t)vbody[v];
+    double x0 = xbar[3 * (size_t)v], x1 = xbar[3 * (size_t)v + 1], x2 = xbar[3 * (size_t)v + 2];
+#pragma unroll
+    for (int i = 0; i < 3; i++) X[3 * (size_t)v + i] = ((Q[3 + i] * x0 + Q[6 + i] * x1) + Q[9 + i] * x2) + Q[i];
+}
+__global__ void k_axpy(int n, const double* __restrict__ q, const double* __restrict__ dq, double alpha, double* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = q[i] + (dq ? dq[i] * alpha : 0.0);
+}
+
+void transform_positions(World& w, const double* q, double* X) {
+    ProfScope ps(w, PK_TRANSFORM, 52.0 * w.V + 96.0 * w.B);   // xbar + vbody in, X out, q
+    hipLaunchKernelGGL(k_transform, dim3((w.V + 255) / 256), dim3(256), 0, w.stream, w.V, w.vbody.p, w.xbar.p, q, X);
+    ADB_HIP(hipGetLastError());
+}
+void q_axpy(World& w, const double* q, const double* dq, double alpha, double* out) {
+    int n = 12 * w.B;
+    hipLaunchKernelGGL(k_axpy, dim3((n + 255) / 256), dim3(256), 0, w.stream, n, q, dq, alpha, out);
+    ADB_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ body boxes
+// one workgroup per body; X1 != null: union of both position sets
+__global__ void k_body_box(const int* __restrict__ voff, const double* __restrict__ X0, const double* __restrict__ X1,
+                           double infl, double* __restrict__ box) {
+    const int b = blockIdx.x;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int v = voff[b] + threadIdx.x; v < voff[b + 1]; v += blockDim.x) {
+        for (int k = 0; k < 3; k++) {
+            double x = X0[3 * (size_t)v + k];
+            lo[k] = fmin(lo[k], x); hi[k] = fmax(hi[k], x);
+            if (X1) { double y = X1[3 * (size_t)v + k]; lo[k] = fmin(lo[k], y); hi[k] = fmax(hi[k], y); }
+        }
+    }
+    __shared__ double s[6][NB_THREADS / 64];
+    for (int k = 0; k < 3; k++) {
+        for (int m = 32; m >= 1; m >>= 1) { lo[k] = fmin(lo[k], __shfl_xor(lo[k], m)); hi[k] = fmax(hi[k], __shfl_xor(hi[k], m)); }
+    }
+    const int wv = threadIdx.x / 64, ln = threadIdx.x % 64;
+    if (ln == 0) for (int k = 0; k < 3; k++) { s[k][wv] = lo[k]; s[3 + k][wv] = hi[k]; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 3; k++) {
+            double l = s[k][0], h = s[3 + k][0];
+            for (int i = 1; i < (int)(blockDim.x / 64); i++) { l = fmin(l, s[k][i]); h = fmax(h, s[3 + k][i]); }
+            box[6 * (size_t)b + k] = l - infl;
+            box[6 * (size_t)b + 3 + k] = h + infl;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ body pairs
+// tiled all-pairs: thread (I, J) over a 2-D grid, only I < J, wave-aggregated append
+__global__ void k_body_pairs(int B, const double* __restrict__ box, int2* __restrict__ pairs, int cap, int* __restrict__ counter) {
+    const int I = blockIdx.y * 16 + threadIdx.y;
+    const int J = blockIdx.x * 16 + threadIdx.x;
+    bool hit = false;
+    if (I < B && J < B && I < J) {
+        const double* a = box + 6 * (size_t)I;
+        const double* b = box + 6 * (size_t)J;
+        hit = ovl(a, a + 3, b, b + 3);
+    }
+    unsigned long long m = __ballot(hit);
+    if (m == 0) return;
+    const int lane = __lane_id();
+    int base = 0;
+    if (lane == __ffsll((long long)m) - 1) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, __ffsll((long long)m) - 1);
+    if (hit) {
+        int off = __popcll(m & ((1ull << lane) - 1));
+        if (base + off < cap) pairs[base + off] = make_int2(I, J);
+    }
+}
+
+// ------------------------------------------------------------------ append helper
+__device__ __forceinline__ void append_key(bool hit, unsigned long long key, unsigned long long* out, int cap, int* counter) {
+    unsigned long long m = __ballot(hit);
+    if (m == 0) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader);
+    if (hit) {
+        int off = __popcll(m & ((1ull << lane) - 1));
+        if (base + off < cap) out[base + off] = key;
+    }
+}
+
+struct NarrowArgs {
+    int n_pairs; const int2* pairs;
+    const int* voff; const int* foff; const int* eoff; const int* face; const int* edge;
+    const double* X0; const double* X1;   // X1 only for CCD
+    const double* box;
+    double r; double d_hat;
+    int enable_pt, enable_ee;
+    unsigned long long* pt_keys; int pt_cap; unsigned long long* ee_keys; int ee_cap;
+    int* counters;                       // [0] pt, [1] ee, [2] overflowed pairs
+    int2* overflow; int ovf_cap;
+    unsigned long long* toi_bits;        // CCD: [1] pt/ee min
+};
+
+template <int CCD>
+__device__ __forceinline__ void prim_box(const double* X0, const double* X1, const int* ids, int n, double lo[3], double hi[3]) {
+    for (int k = 0; k < 3; k++) { lo[k] = 1e300; hi[k] = -1e300; }
+    for (int c = 0; c < n; c++) {
+        const double* p = X0 + 3 * (size_t)ids[c];
+        for (int k = 0; k < 3; k++) { lo[k] = fmin(lo[k], p[k]); hi[k] = fmax(hi[k], p[k]); }
+        if (CCD) {
+            const double* p1 = X1 + 3 * (size_t)ids[c];
+            for (int k = 0; k < 3; k++) { lo[k] = fmin(lo[k], p1[k]); hi[k] = fmax(hi[k], p1[k]); }
+        }
+    }
+}
+
+// binning + tests for one body pair; mode CCD = 0 contact sets, 1 step bound
+template <int CCD>
+__global__ __launch_bounds__(NB_THREADS) void k_pair_narrow(NarrowArgs a) {
+    __shared__ int lv[2][LCAP], lf[2][LCAP], le[2][LCAP];
+    __shared__ int cnt[6];
+    __shared__ double bmin[NB_THREADS / 64];
+    const int pidx = blockIdx.x;
+    if (pidx >= a.n_pairs) return;
+    const int2 pr = a.pairs[pidx];
+    const int body[2] = {pr.x, pr.y};
+    double xlo[3], xhi[3];
+    for (int k = 0; k < 3; k++) {
+        xlo[k] = fmax(a.box[6 * (size_t)pr.x + k], a.box[6 * (size_t)pr.y + k]);
+        xhi[k] = fmin(a.box[6 * (size_t)pr.x + 3 + k], a.box[6 * (size_t)pr.y + 3 + k]);
+    }
+    if (threadIdx.x < 6) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    // ---- binning: lists 0..1 vertices, 2..3 faces, 4..5 edges (side 0 = I, 1 = J)
+    for (int s = 0; s < 2; s++) {
+        const int b = body[s];
+        for (int v = a.voff[b] + threadIdx.x; v < a.voff[b + 1]; v += NB_THREADS) {
+            double lo[3], hi[3];
+            prim_box<CCD>(a.X0, a.X1, &v, 1, lo, hi);
+            if (ovl(lo, hi, xlo, xhi)) { int k = atomicAdd(&cnt[0 + s], 1); if (k < LCAP) lv[s][k] = v; }
+        }
+        for (int f = a.foff[b] + threadIdx.x; f < a.foff[b + 1]; f += NB_THREADS) {
+            double lo[3], hi[3];
+            prim_box<CCD>(a.X0, a.X1, a.face + 3 * (size_t)f, 3, lo, hi);
+            if (ovl(lo, hi, xlo, xhi)) { int k = atomicAdd(&cnt[2 + s], 1); if (k < LCAP) lf[s][k] = f; }
+        }
+        for (int e = a.eoff[b] + threadIdx.x; e < a.eoff[b + 1]; e += NB_THREADS) {
+            double lo[3], hi[3];
+            prim_box<CCD>(a.X0, a.X1, a.edge + 2 * (size_t)e, 2, lo, hi);
+            if (ovl(lo, hi, xlo, xhi)) { int k = atomicAdd(&cnt[4 + s], 1); if (k < LCAP) le[s][k] = e; }
+        }
+    }
+    __syncthreads();
+    bool overflow = false;
+    for (int k = 0; k < 6; k++) overflow |= cnt[k] > LCAP;
+    if (overflow) {
+        if (threadIdx.x == 0) {
+            int o = atomicAdd(&a.counters[2], 1);
+            if (o < a.ovf_cap) a.overflow[o] = pr;
+        }
+        return;
+    }
+    double tmin = 1.0;
+    // ---- PT both directions: vertices of side s against faces of side 1 - s
+    if (a.enable_pt) {
+        for (int s = 0; s < 2; s++) {
+            const int nv = cnt[0 + s], nf = cnt[2 + (1 - s)];
+            const int total = nv * nf;
+            const int nround = (total + NB_THREADS - 1) / NB_THREADS;
+            for (int it = 0; it < nround; it++) {
+                const int t = it * NB_THREADS + threadIdx.x;
+                bool hit = false;
+                unsigned long long key = 0;
+                if (t < total) {
+                    const int v = lv[s][t / nf], f = lf[1 - s][t % nf];
+                    const int* fv = a.face + 3 * (size_t)f;
+                    double vlo[3], vhi[3], flo[3], fhi[3];
+                    prim_box<CCD>(a.X0, a.X1, &v, 1, vlo, vhi);
+                    prim_box<CCD>(a.X0, a.X1, fv, 3, flo, fhi);
+                    if (!CCD) for (int k = 0; k < 3; k++) { vlo[k] -= a.r; vhi[k] += a.r; }
+                    if (ovl(vlo, vhi, flo, fhi)) {
+                        d3 p = ld3(a.X0 + 3 * (size_t)v), t0 = ld3(a.X0 + 3 * (size_t)fv[0]), t1 = ld3(a.X0 + 3 * (size_t)fv[1]), t2 = ld3(a.X0 + 3 * (size_t)fv[2]);
+                        if (!CCD) {
+                            int ty = pt_type(p, t0, t1, t2);
+                            double d = pt_dist(p, t0, t1, t2, ty);
+                            hit = d < a.d_hat;
+                            key = ((unsigned long long)v << 32) | (unsigned long long)f;
+                        } else {
+                            d3 P0[4] = {p, t0, t1, t2};
+                            d3 P1[4] = {ld3(a.X1 + 3 * (size_t)v), ld3(a.X1 + 3 * (size_t)fv[0]), ld3(a.X1 + 3 * (size_t)fv[1]), ld3(a.X1 + 3 * (size_t)fv[2])};
+                            tmin = fmin(tmin, pt_toi(P0, P1));
+                        }
+                    }
+                }
+                if (!CCD) append_key(hit, key, a.pt_keys, a.pt_cap, &a.counters[0]);
+            }
+        }
+    }
+    // ---- EE: edges of I against edges of J
+    if (a.enable_ee) {
+        const int n0 = cnt[4], n1 = cnt[5];
+        const int total = n0 * n1;
+        const int nround = (total + NB_THREADS - 1) / NB_THREADS;
+        for (int it = 0; it < nround; it++) {
+            const int t = it * NB_THREADS + threadIdx.x;
+            bool hit = false;
+            unsigned long long key = 0;
+            if (t < total) {
+                const int ea = le[0][t / n1], eb = le[1][t % n1];
+                const int* va = a.edge + 2 * (size_t)ea;
+                const int* vb = a.edge + 2 * (size_t)eb;
+                double alo[3], ahi[3], blo[3], bhi[3];
+                prim_box<CCD>(a.X0, a.X1, va, 2, alo, ahi);
+                prim_box<CCD>(a.X0, a.X1, vb, 2, blo, bhi);
+                if (!CCD) for (int k = 0; k < 3; k++) { alo[k] -= a.r; ahi[k] += a.r; }
+                if (ovl(alo, ahi, blo, bhi)) {
+                    d3 a0 = ld3(a.X0 + 3 * (size_t)va[0]), a1 = ld3(a.X0 + 3 * (size_t)va[1]);
+                    d3 b0 = ld3(a.X0 + 3 * (size_t)vb[0]), b1 = ld3(a.X0 + 3 * (size_t)vb[1]);
+                    if (!CCD) {
+                        int ty = ee_type(a0, a1, b0, b1);
+                        double d = ee_dist(a0, a1, b0, b1, ty);
+                        hit = d < a.d_hat;
+                        key = ((unsigned long long)ea << 32) | (unsigned long long)eb;
+                    } else {
+                        d3 P0[4] = {a0, a1, b0, b1};
+                        d3 P1[4] = {ld3(a.X1 + 3 * (size_t)va[0]), ld3(a.X1 + 3 * (size_t)va[1]), ld3(a.X1 + 3 * (size_t)vb[0]), ld3(a.X1 + 3 * (size_t)vb[1])};
+                        tmin = fmin(tmin, ee_toi(P0, P1));
+                    }
+                }
+            }
+            if (!CCD) append_key(hit, key, a.ee_keys, a.ee_cap, &a.counters[1]);
+        }
+    }
+    if (CCD) {
+        for (int m = 32; m >= 1; m >>= 1) tmin = fmin(tmin, __shfl_xor(tmin, m));
+        if (__lane_id() == 0) bmin[threadIdx.x / 64] = tmin;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = bmin[0];
+            for (int i = 1; i < NB_THREADS / 64; i++) t = fmin(t, bmin[i]);
+            if (t < 1.0) atomicMin(&a.toi_bits[1], order_key(t));
+        }
+    }
+}
+
+// overflow fallback: the same tests straight from global memory, no LDS lists
+template <int CCD>
+__global__ __launch_bounds__(NB_THREADS) void k_pair_brute(NarrowArgs a, int n_ovf) {
+    const int pidx = blockIdx.x;
+    if (pidx >= n_ovf) return;
+    const int2 pr = a.overflow[pidx];
+    double tmin = 1.0;
+    __shared__ double bmin[NB_THREADS / 64];
+    for (int s = 0; s < 2 && a.enable_pt; s++) {
+        const int A = s == 0 ? pr.x : pr.y, Bb = s == 0 ? pr.y : pr.x;
+        const long long nv = a.voff[A + 1] - a.voff[A], nf = a.foff[Bb + 1] - a.foff[Bb];
+        const long long total = nv * nf;
+        for (long long base = 0; base < total; base += NB_THREADS) {
+            long long t = base + threadIdx.x;
+            bool hit = false; unsigned long long key = 0;
+            if (t < total) {
+                int v = a.voff[A] + (int)(t / nf), f = a.foff[Bb] + (int)(t % nf);
+                const int* fv = a.face + 3 * (size_t)f;
+                double vlo[3], vhi[3], flo[3], fhi[3];
+                prim_box<CCD>(a.X0, a.X1, &v, 1, vlo, vhi);
+                prim_box<CCD>(a.X0, a.X1, fv, 3, flo, fhi);
+                if (!CCD) for (int k = 0; k < 3; k++) { vlo[k] -= a.r; vhi[k] += a.r; }
+                if (ovl(vlo, vhi, flo, fhi)) {
+                    d3 p = ld3(a.X0 + 3 * (size_t)v), t0 = ld3(a.X0 + 3 * (size_t)fv[0]), t1 = ld3(a.X0 + 3 * (size_t)fv[1]), t2 = ld3(a.X0 + 3 * (size_t)fv[2]);
+                    if (!CCD) {
+                        int ty = pt_type(p, t0, t1, t2);
+                        hit = pt_dist(p, t0, t1, t2, ty) < a.d_hat;
+                        key = ((unsigned long long)v << 32) | (unsigned long long)f;
+                    } else {
+                        d3 P0[4] = {p, t0, t1, t2};
+                        d3 P1[4] = {ld3(a.X1 + 3 * (size_t)v), ld3(a.X1 + 3 * (size_t)fv[0]), ld3(a.X1 + 3 * (size_t)fv[1]), ld3(a.X1 + 3 * (size_t)fv[2])};
+                        tmin = fmin(tmin, pt_toi(P0, P1));
+                    }
+                }
+            }
+            if (!CCD) append_key(hit, key, a.pt_keys, a.pt_cap, &a.counters[0]);
+        }
+    }
+    if (a.enable_ee) {
+        const long long n0 = a.eoff[pr.x + 1] - a.eoff[pr.x], n1 = a.eoff[pr.y + 1] - a.eoff[pr.y];
+        const long long total = n0 * n1;
+        for (long long base = 0; base < total; base += NB_THREADS) {
+            long long t = base + threadIdx.x;
+            bool hit = false; unsigned long long key = 0;
+            if (t < total) {
+                int ea = a.eoff[pr.x] + (int)(t / n1), eb = a.eoff[pr.y] + (int)(t % n1);
+                const int* va = a.edge + 2 * (size_t)ea;
+                const int* vb = a.edge + 2 * (size_t)eb;
+                double alo[3], ahi[3], blo[3], bhi[3];
+                prim_box<CCD>(a.X0, a.X1, va, 2, alo, ahi);
+                prim_box<CCD>(a.X0, a.X1, vb, 2, blo, bhi);
+                if (!CCD) for (int k = 0; k < 3; k++) { alo[k] -= a.r; ahi[k] += a.r; }
+                if (ovl(alo, ahi, blo, bhi)) {
+                    d3 a0 = ld3(a.X0 + 3 * (size_t)va[0]), a1 = ld3(a.X0 + 3 * (size_t)va[1]);
+                    d3 b0 = ld3(a.X0 + 3 * (size_t)vb[0]), b1 = ld3(a.X0 + 3 * (size_t)vb[1]);
+                    if (!CCD) {
+                        int ty = ee_type(a0, a1, b0, b1);
+                        hit = ee_dist(a0, a1, b0, b1, ty) < a.d_hat;
+                        key = ((unsigned long long)ea << 32) | (unsigned long long)eb;
+                    } else {
+                        d3 P0[4] = {a0, a1, b0, b1};
+                        d3 P1[4] = {ld3(a.X1 + 3 * (size_t)va[0]), ld3(a.X1 + 3 * (size_t)va[1]), ld3(a.X1 + 3 * (size_t)vb[0]), ld3(a.X1 + 3 * (size_t)vb[1])};
+                        tmin = fmin(tmin, ee_toi(P0, P1));
+                    }
+                }
+            }
+            if (!CCD) append_key(hit, key, a.ee_keys, a.ee_cap, &a.counters[1]);
+        }
+    }
+    if (CCD) {
+        for (int m = 32; m >= 1; m >>= 1) tmin = fmin(tmin, __shfl_xor(tmin, m));
+        if (__lane_id() == 0) bmin[threadIdx.x / 64] = tmin;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = bmin[0];
+            for (int i = 1; i < NB_THREADS / 64; i++) t = fmin(t, bmin[i]);
+            if (t < 1.0) atomicMin(&a.toi_bits[1], order_key(t));
+        }
+    }
+}
+
+// ------------------------------------------------------------------ vertex-ground
+__global__ void k_vg_scan(int V, const double* __restrict__ X, double ground_y, double d_hat,
+                          unsigned long long* keys, int cap, int* counter) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    bool hit = false;
+    if (v < V) {
+        double dd = X[3 * (size_t)v + 1] - ground_y;
+        hit = dd * dd < d_hat;
+    }
+    append_key(hit, (unsigned long long)v, keys, cap, counter);
+}
+// vertex-ground CCD (model/collision.cpp:83-105)
+__global__ void k_vg_toi(int V, const double* __restrict__ X0, const double* __restrict__ X1, double ground_y,
+                         unsigned long long* toi_bits) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    double t = 1.0;
+    if (v < V) {
+        double d2 = X1[3 * (size_t)v + 1] - ground_y;
+        if (d2 < 0) {
+            double d1 = X0[3 * (size_t)v + 1] - ground_y;
+            t = fmin(1.0, d1 / (d1 - d2));
+        }
+    }
+    for (int m = 32; m >= 1; m >>= 1) t = fmin(t, __shfl_xor(t, m));
+    if (__lane_id() == 0 && t < 1.0) atomicMin(&toi_bits[0], order_key(t));
+}
+
+// ------------------------------------------------------------------ decode
+__global__ void k_decode4(int n, const unsigned long long* __restrict__ keys, const int* __restrict__ bodyA,
+                          const int* __restrict__ offA, const int* __restrict__ bodyB, const int* __restrict__ offB, int4* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long k = keys[i];
+    int a = (int)(k >> 32), b = (int)(k & 0xffffffffull);
+    int I = bodyA[a], J = bodyB[b];
+    out[i] = make_int4(I, a - offA[I], J, b - offB[J]);
+}
+__global__ void k_decode2(int n, const unsigned long long* __restrict__ keys, const int* __restrict__ vbody,
+                          const int* __restrict__ voff, int2* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int v = (int)keys[i];
+    int I = vbody[v];
+    out[i] = make_int2(I, v - voff[I]);
+}
+
+static void sort_keys(World& w, unsigned long long* in, unsigned long long* out, int n, int end_bit) {
+    ProfScope ps(w, PK_SORT, 16.0 * n);
+    size_t tmp = 0;
+    ADB_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, in, out, n, 0, end_bit, w.stream));
+    w.sort_tmp.reserve(tmp + 16);
+    ADB_HIP(hipcub::DeviceRadixSort::SortKeys(w.sort_tmp.p, tmp, in, out, n, 0, end_bit, w.stream));
+}
+
+static int bits_for(long long n) { int b = 1; while ((1ll << b) <= n) b++; return b; }
+
+// body boxes + pairs; returns the pair count
+static int body_pairs(World& w, const double* X0, const double* X1, double infl) {
+    w.bbox.reserve(6 * (size_t)w.B);
+    {
+        ProfScope ps(w, PK_BODY_BOX, (X1 ? 48.0 : 24.0) * w.V + 52.0 * w.B);
+        hipLaunchKernelGGL(k_body_box, dim3(w.B), dim3(NB_THREADS), 0, w.stream, w.voff.p, X0, X1, infl, w.bbox.p);
+    }
+    ADB_HIP(hipGetLastError());
+    int npairs = 0;
+    for (int attempt = 0; attempt < 2; attempt++) {
+        size_t cap = std::max<size_t>(w.pairs.cap, 4096);
+        w.pairs.reserve(cap);
+        ADB_HIP(hipMemsetAsync(w.counters.p + 8, 0, sizeof(int), w.stream));
+        dim3 grid((w.B + 15) / 16, (w.B + 15) / 16), blk(16, 16);
+        {
+            ProfScope ps(w, PK_BODY_PAIRS, 48.0 * w.B);
+            hipLaunchKernelGGL(k_body_pairs, grid, blk, 0, w.stream, w.B, w.bbox.p, w.pairs.p, (int)w.pairs.cap, w.counters.p + 8);
+        }
+        ADB_HIP(hipGetLastError());
+        ADB_HIP(hipMemcpyAsync(&npairs, w.counters.p + 8, sizeof(int), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipStreamSynchronize(w.stream));
+        if ((size_t)npairs <= w.pairs.cap) break;
+        w.pairs.reserve(npairs);
+    }
+    return npairs;
+}
+
+void collision_set(World& w, const double* X, ContactSet& out) {
+    const double r = sqrt(w.P.d_hat) * (1.0 + 1e-6) + 1e-12;
+    int npairs = body_pairs(w, X, nullptr, r);
+    w.counters.reserve(16);
+    for (int attempt = 0; attempt < 4; attempt++) {
+        size_t pcap = std::max<size_t>(w.keys_a.cap / 2, 1 << 16);
+        w.keys_a.reserve(2 * pcap);
+        w.keys_b.reserve(2 * pcap);
+        size_t vcap = w.V + 64;
+        ADB_HIP(hipMemsetAsync(w.counters.p, 0, 4 * sizeof(int), w.stream));
+        NarrowArgs a;
+        a.n_pairs = npairs; a.pairs = w.pairs.p;
+        a.voff = w.voff.p; a.foff = w.foff.p; a.eoff = w.eoff.p; a.face = w.face.p; a.edge = w.edge.p;
+        a.X0 = X; a.X1 = nullptr; a.box = w.bbox.p; a.r = r; a.d_hat = w.P.d_hat;
+        a.enable_pt = w.P.enable_pt; a.enable_ee = w.P.enable_ee;
+        a.pt_keys = w.keys_a.p; a.pt_cap = (int)pcap;
+        a.ee_keys = w.keys_a.p + pcap; a.ee_cap = (int)pcap;
+        a.counters = w.counters.p;
+        w.ovf.reserve(std::max(npairs, 1));
+        a.overflow = w.ovf.p; a.ovf_cap = npairs;
+        a.toi_bits = nullptr;
+        if (npairs > 0) {
+            ProfScope ps(w, PK_NARROW_SET, 0.0);
+            hipLaunchKernelGGL(k_pair_narrow<0>, dim3(npairs), dim3(NB_THREADS), 0, w.stream, a);
+            ADB_HIP(hipGetLastError());
+        }
+        int h[4];
+        ADB_HIP(hipMemcpyAsync(h, w.counters.p, 4 * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipStreamSynchronize(w.stream));
+        if (h[2] > 0) {
+            hipLaunchKernelGGL(k_pair_brute<0>, dim3(h[2]), dim3(NB_THREADS), 0, w.stream, a, h[2]);
+            ADB_HIP(hipGetLastError());
+            ADB_HIP(hipMemcpyAsync(h, w.counters.p, 4 * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+            ADB_HIP(hipStreamSynchronize(w.stream));
+        }
+        // vertex-ground
+        w.vg_keys.reserve(vcap);
+        int nvg = 0;
+        if (w.P.enable_ground) {
+            ProfScope ps(w, PK_VG_SCAN, 8.0 * w.V);
+            hipLaunchKernelGGL(k_vg_scan, dim3((w.V + 255) / 256), dim3(256), 0, w.stream, w.V, X, w.P.ground_y, w.P.d_hat,
+                               w.vg_keys.p, (int)vcap, w.counters.p + 3);
+            ADB_HIP(hipGetLastError());
+            ADB_HIP(hipMemcpyAsync(&nvg, w.counters.p + 3, sizeof(int), hipMemcpyDeviceToHost, w.stream));
+            ADB_HIP(hipStreamSynchronize(w.stream));
+        }
+        if ((size_t)h[0] > pcap || (size_t)h[1] > pcap) {
+            size_t need = std::max(h[0], h[1]);
+            w.keys_a.reserve(2 * need); w.keys_b.reserve(2 * need);
+            continue;
+        }
+        const int npt = h[0], nee = h[1];
+        out.pt.reserve(std::max(npt, 1)); out.ee.reserve(std::max(nee, 1)); out.vg.reserve(std::max(nvg, 1));
+        const int vb = bits_for(w.V), fb = bits_for(w.F), eb = bits_for(w.E);
+        if (npt > 0) {
+            sort_keys(w, a.pt_keys, w.keys_b.p, npt, 32 + vb);
+            hipLaunchKernelGGL(k_decode4, dim3((npt + 255) / 256), dim3(256), 0, w.stream, npt, w.keys_b.p, w.vbody.p, w.voff.p,
+                               w.fbody.p, w.foff.p, out.pt.p);
+            ADB_HIP(hipGetLastError());
+        }
+        (void)fb;
+        if (nee > 0) {
+            sort_keys(w, a.ee_keys, w.keys_b.p + pcap, nee, 32 + eb);
+            hipLaunchKernelGGL(k_decode4, dim3((nee + 255) / 256), dim3(256), 0, w.stream, nee, w.keys_b.p + pcap, w.ebody.p, w.eoff.p,
+                               w.ebody.p, w.eoff.p, out.ee.p);
+            ADB_HIP(hipGetLastError());
+        }
+        if (nvg > 0) {
+            sort_keys(w, w.vg_keys.p, w.keys_b.p, nvg, vb);
+            hipLaunchKernelGGL(k_decode2, dim3((nvg + 255) / 256), dim3(256), 0, w.stream, nvg, w.keys_b.p, w.vbody.p, w.voff.p, out.vg.p);
+            ADB_HIP(hipGetLastError());
+        }
+        out.n_pt = npt; out.n_ee = nee; out.n_vg = nvg;
+        return;
+    }
+    throw AdbError(ADB_E_CAPACITY, "contact-set capacity could not be grown");
+}
+
+double ccd_toi(World& w, const double* X0, const double* X1, int* status) {
+    *status = ADB_OK;
+    w.toi_bits.reserve(2);
+    const unsigned long long one = order_key_host(1.0);
+    unsigned long long init[2] = {one, one};
+    ADB_HIP(hipMemcpyAsync(w.toi_bits.p, init, sizeof init, hipMemcpyHostToDevice, w.stream));
+    if (w.P.enable_ground) {
+        ProfScope ps(w, PK_VG_TOI, 16.0 * w.V);
+        hipLaunchKernelGGL(k_vg_toi, dim3((w.V + 255) / 256), dim3(256), 0, w.stream, w.V, X0, X1, w.P.ground_y, w.toi_bits.p);
+        ADB_HIP(hipGetLastError());
+    }
+    int npairs = body_pairs(w, X0, X1, 0.0);
+    w.counters.reserve(16);
+    ADB_HIP(hipMemsetAsync(w.counters.p, 0, 4 * sizeof(int), w.stream));
+    NarrowArgs a;
+    a.n_pairs = npairs; a.pairs = w.pairs.p;
+    a.voff = w.voff.p; a.foff = w.foff.p; a.eoff = w.eoff.p; a.face = w.face.p; a.edge = w.edge.p;
+    a.X0 = X0; a.X1 = X1; a.box = w.bbox.p; a.r = 0.0; a.d_hat = w.P.d_hat;
+    a.enable_pt = w.P.enable_pt; a.enable_ee = w.P.enable_ee;
+    a.pt_keys = nullptr; a.pt_cap = 0; a.ee_keys = nullptr; a.ee_cap = 0;
+    a.counters = w.counters.p;
+    w.ovf.reserve(std::max(npairs, 1));
+    a.overflow = w.ovf.p; a.ovf_cap = npairs;
+    a.toi_bits = w.toi_bits.p;
+    if (npairs > 0) {
+        ProfScope ps(w, PK_NARROW_CCD, 0.0);
+        hipLaunchKernelGGL(k_pair_narrow<1>, dim3(npairs), dim3(NB_THREADS), 0, w.stream, a);
+        ADB_HIP(hipGetLastError());
+    }
+    int h[4];
+    ADB_HIP(hipMemcpyAsync(h, w.counters.p, 4 * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+    ADB_HIP(hipStreamSynchronize(w.stream));
+    if (h[2] > 0) {
+        hipLaunchKernelGGL(k_pair_brute<1>, dim3(h[2]), dim3(NB_THREADS), 0, w.stream, a, h[2]);
+        ADB_HIP(hipGetLastError());
+    }
+    unsigned long long bits[2];
+    ADB_HIP(hipMemcpyAsync(bits, w.toi_bits.p, sizeof bits, hipMemcpyDeviceToHost, w.stream));
+    ADB_HIP(hipStreamSynchronize(w.stream));
+    const double tg = order_key_inv(bits[0]), tp = order_key_inv(bits[1]);
+    // toi < 1e-6 twice in a row is a stall (model/iaabb.cpp:305-314,532-540)
+    if (tg < 1e-6) { if (++w.stall_vg > 1) *status = ADB_E_TOI_STALL; } else w.stall_vg = 0;
+    if (tp < 1e-6) { if (++w.stall_pt > 1) *status = ADB_E_TOI_STALL; } else w.stall_pt = 0;
+    return tg < tp ? tg : tp;
+}
+
+} // namespace adb

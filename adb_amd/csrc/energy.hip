@@ -1,0 +1,210 @@
+// energy.hip - line-search energies and deterministic reductions.
+//
+// Replaces E_barrier_plus_inert, E_fric and E_ground (model/energy.cpp:19-86,
+// 195-270) and the inertia energy E (model/mutils.cpp:197-201).  Every item
+// (body, contact) writes its energy to a buffer which is then summed by a
+// fixed-shape two-level tree, so results do not depend on scheduling.
+#include "dmath.h"
+#include "world.h"
+
+namespace adb {
+
+struct EnergyArgs {
+    int B, n_pt, n_ee, n_vg;
+    double dt, kap_o, d_hat, kappa, eps_x, ground_y, g0, g1, g2;
+    int en_pt, en_ee, en_g;
+    const double* qv; const double* q0; const double* qdot; const double* mass; const double* Ic;
+    const double* X;
+    const int4* pt; const int4* ee; const int2* vg;
+    const int* voff; const int* foff; const int* eoff; const int* face; const int* edge;
+    double* out;
+};
+
+__global__ void k_energy_items(EnergyArgs a) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = a.B + a.n_pt + a.n_ee + a.n_vg;
+    if (i >= n) return;
+    double e = 0.0;
+    if (i < a.B) {
+        if (a.mass[i] >= 0.0) {
+            const double* Q = a.qv + 12 * (size_t)i;
+            d3 c[4] = {ld3(Q), ld3(Q + 3), ld3(Q + 6), ld3(Q + 9)};
+            double eo = 0.0;
+            for (int s = 1; s < 4; s++)
+                for (int t = 1; t < 4; t++) { double v = dot(c[s], c[t]) - 1.0 * (s == t); eo += v * v; }
+            eo = eo * a.kap_o;
+            double pp = 0.0, rr = 0.0;
+            for (int k = 0; k < 12; k++) {
+                double qt = a.q0[12 * (size_t)i + k] + a.dt * a.qdot[12 * (size_t)i + k];
+                if (k < 3) qt += a.dt * a.dt * (k == 0 ? a.g0 : (k == 1 ? a.g1 : a.g2));
+                double d = Q[k] - qt;
+                if (k < 3) pp += d * d; else rr += d * d;
+            }
+            e = eo * a.dt * a.dt + 0.5 * (pp * a.mass[i] + rr * a.Ic[i]);
+        }
+    } else if (i < a.B + a.n_pt) {
+        if (a.en_pt) {
+            int4 c = a.pt[i - a.B];
+            int v = a.voff[c.x] + c.y, f = a.foff[c.z] + c.w;
+            d3 p = ld3(a.X + 3 * (size_t)v), t0 = ld3(a.X + 3 * (size_t)a.face[3 * f]);
+            d3 t1 = ld3(a.X + 3 * (size_t)a.face[3 * f + 1]), t2 = ld3(a.X + 3 * (size_t)a.face[3 * f + 2]);
+            double d = pt_dist(p, t0, t1, t2, pt_type(p, t0, t1, t2));
+            e = barrier(d, a.d_hat, a.kappa);
+        }
+    } else if (i < a.B + a.n_pt + a.n_ee) {
+        if (a.en_ee) {
+            int4 c = a.ee[i - a.B - a.n_pt];
+            int ea = a.eoff[c.x] + c.y, eb = a.eoff[c.z] + c.w;
+            d3 a0 = ld3(a.X + 3 * (size_t)a.edge[2 * ea]), a1 = ld3(a.X + 3 * (size_t)a.edge[2 * ea + 1]);
+            d3 b0 = ld3(a.X + 3 * (size_t)a.edge[2 * eb]), b1 = ld3(a.X + 3 * (size_t)a.edge[2 * eb + 1]);
+            double eps = sqn(a0 - a1) * sqn(b0 - b1) * a.eps_x;
+            double m = ee_mollifier_value(a0, a1, b0, b1, eps);
+            double d = ee_dist(a0, a1, b0, b1, ee_type(a0, a1, b0, b1));
+            e = barrier(d, a.d_hat, a.kappa) * m;
+        }
+    } else {
+        if (a.en_g) {
+            int2 c = a.vg[i - a.B - a.n_pt - a.n_ee];
+            int v = a.voff[c.x] + c.y;
+            double dd = a.X[3 * (size_t)v + 1] - a.ground_y, d = dd * dd;
+            if (d < a.d_hat) e = barrier(d, a.d_hat, a.kappa);
+        }
+    }
+    a.out[i] = e;
+}
+
+struct FricArgs {
+    int n_pt, n_ee, n_vg;
+    double mu, eps_v;
+    int pt_f, ee_f, vg_f, en_pt, en_ee, en_g;
+    const double* X; const double* Xs;
+    const int4* pt; const int4* ee; const int2* vg;
+    const int* voff; const int* foff; const int* eoff; const int* face; const int* edge;
+    const double* tk; const double* lam; const double* vglam;
+    double* out;
+};
+__global__ void k_fric_items(FricArgs a) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = a.n_pt + a.n_ee + a.n_vg;
+    if (i >= n) return;
+    double e = 0.0;
+    if (i < a.n_pt + a.n_ee) {
+        const bool is_ee = i >= a.n_pt;
+        if (is_ee ? (a.en_ee && a.ee_f) : (a.en_pt && a.pt_f)) {
+            int ids[4];
+            if (!is_ee) {
+                int4 c = a.pt[i];
+                int f = a.foff[c.z] + c.w;
+                ids[0] = a.voff[c.x] + c.y; ids[1] = a.face[3 * f]; ids[2] = a.face[3 * f + 1]; ids[3] = a.face[3 * f + 2];
+            } else {
+                int4 c = a.ee[i - a.n_pt];
+                int ea = a.eoff[c.x] + c.y, eb = a.eoff[c.z] + c.w;
+                ids[0] = a.edge[2 * ea]; ids[1] = a.edge[2 * ea + 1]; ids[2] = a.edge[2 * eb]; ids[3] = a.edge[2 * eb + 1];
+            }
+            const double* T = a.tk + 24 * (size_t)i;
+            double u0 = 0.0, u1 = 0.0;
+            for (int k = 0; k < 4; k++)
+                for (int c = 0; c < 3; c++) {
+                    double dv = a.X[3 * (size_t)ids[k] + c] - a.Xs[3 * (size_t)ids[k] + c];
+                    u0 += T[3 * k + c] * dv;
+                    u1 += T[12 + 3 * k + c] * dv;
+                }
+            e = a.mu * a.lam[i] * f0_sf(sqrt(u0 * u0 + u1 * u1), a.eps_v);
+        }
+    } else if (a.en_g && a.vg_f) {
+        int k = i - a.n_pt - a.n_ee;
+        int2 c = a.vg[k];
+        int v = a.voff[c.x] + c.y;
+        double ux = a.X[3 * (size_t)v] - a.Xs[3 * (size_t)v];
+        double uz = a.X[3 * (size_t)v + 2] - a.Xs[3 * (size_t)v + 2];
+        e = a.mu * a.vglam[k] * f0_sf(sqrt(ux * ux + uz * uz), a.eps_v);
+    }
+    a.out[i] = e;
+}
+
+// two-level fixed-shape reduction
+constexpr int RT = 256;
+__global__ __launch_bounds__(RT) void k_sum1(int n, int G, const double* __restrict__ a, const double* __restrict__ b,
+                                            double* __restrict__ part) {
+    __shared__ double sh[RT];
+    double s = 0.0;
+    for (long long i = (long long)blockIdx.x * RT + threadIdx.x; i < n; i += (long long)G * RT) s += b ? a[i] * b[i] : a[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int m = RT / 2; m >= 1; m >>= 1) {
+        if ((int)threadIdx.x < m) sh[threadIdx.x] += sh[threadIdx.x + m];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+__global__ __launch_bounds__(RT) void k_sum2(int G, const double* __restrict__ part, double* __restrict__ out) {
+    __shared__ double sh[RT];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < G; i += RT) s += part[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int m = RT / 2; m >= 1; m >>= 1) {
+        if ((int)threadIdx.x < m) sh[threadIdx.x] += sh[threadIdx.x + m];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+static double reduce(World& w, const double* a, const double* b, int n) {
+    if (n <= 0) return 0.0;
+    int G = (n + RT - 1) / RT;
+    if (G > 1024) G = 1024;
+    w.epart.reserve(G + 2);
+    hipLaunchKernelGGL(k_sum1, dim3(G), dim3(RT), 0, w.stream, n, G, a, b, w.epart.p + 2);
+    hipLaunchKernelGGL(k_sum2, dim3(1), dim3(RT), 0, w.stream, G, w.epart.p + 2, w.epart.p);
+    ADB_HIP(hipGetLastError());
+    double h;
+    ADB_HIP(hipMemcpyAsync(&h, w.epart.p, sizeof(double), hipMemcpyDeviceToHost, w.stream));
+    ADB_HIP(hipStreamSynchronize(w.stream));
+    return h;
+}
+
+double dot_dev(World& w, const double* a, const double* b, int n) { return reduce(w, a, b, n); }
+
+static DBuf<double>& items(World& w) { return w.pcg_Ap; }   // scratch reused outside the solve
+
+double energy_barrier_inertia(World& w, const double* qv, const double* X, const ContactSet& cs) {
+    const int n = w.B + cs.n_pt + cs.n_ee + cs.n_vg;
+    DBuf<double>& buf = items(w);
+    buf.reserve(std::max(n, 12 * w.B));
+    EnergyArgs a;
+    a.B = w.B; a.n_pt = cs.n_pt; a.n_ee = cs.n_ee; a.n_vg = cs.n_vg;
+    a.dt = w.P.dt; a.kap_o = w.P.kappa_ortho; a.d_hat = w.P.d_hat; a.kappa = w.P.kappa; a.eps_x = w.P.eps_x;
+    a.ground_y = w.P.ground_y; a.g0 = w.P.gravity[0]; a.g1 = w.P.gravity[1]; a.g2 = w.P.gravity[2];
+    a.en_pt = w.P.enable_pt; a.en_ee = w.P.enable_ee; a.en_g = w.P.enable_ground;
+    a.qv = qv; a.q0 = w.q0.p; a.qdot = w.qdot.p; a.mass = w.mass.p; a.Ic = w.Ic.p; a.X = X;
+    a.pt = cs.pt.p; a.ee = cs.ee.p; a.vg = cs.vg.p;
+    a.voff = w.voff.p; a.foff = w.foff.p; a.eoff = w.eoff.p; a.face = w.face.p; a.edge = w.edge.p;
+    a.out = buf.p;
+    ProfScope ps(w, PK_ENERGY, 0.0);
+    hipLaunchKernelGGL(k_energy_items, dim3((n + 255) / 256), dim3(256), 0, w.stream, a);
+    ADB_HIP(hipGetLastError());
+    return reduce(w, buf.p, nullptr, n);
+}
+
+double energy_friction(World& w, const double* X) {
+    const ContactSet& cs = w.cs;
+    const int n = cs.n_pt + cs.n_ee + cs.n_vg;
+    if (n == 0) return 0.0;
+    DBuf<double>& buf = items(w);
+    buf.reserve(std::max(n, 12 * w.B));
+    FricArgs a;
+    a.n_pt = cs.n_pt; a.n_ee = cs.n_ee; a.n_vg = cs.n_vg;
+    a.mu = w.P.mu; a.eps_v = w.P.dt * w.P.evh;
+    a.pt_f = w.P.pt_fric; a.ee_f = w.P.ee_fric; a.vg_f = w.P.vg_fric;
+    a.en_pt = w.P.enable_pt; a.en_ee = w.P.enable_ee; a.en_g = w.P.enable_ground;
+    a.X = X; a.Xs = w.Xs.p; a.pt = cs.pt.p; a.ee = cs.ee.p; a.vg = cs.vg.p;
+    a.voff = w.voff.p; a.foff = w.foff.p; a.eoff = w.eoff.p; a.face = w.face.p; a.edge = w.edge.p;
+    a.tk = w.ctk.p; a.lam = w.clam.p; a.vglam = w.vglam.p;
+    a.out = buf.p;
+    hipLaunchKernelGGL(k_fric_items, dim3((n + 255) / 256), dim3(256), 0, w.stream, a);
+    ADB_HIP(hipGetLastError());
+    return reduce(w, buf.p, nullptr, n);
+}
+
+} // namespace adb

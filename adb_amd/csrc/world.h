@@ -1,0 +1,155 @@
+// world.h - host-side state of one ADB world on one gfx950 device, shared by
+// the translation units of libadb.so.  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "../../include/adb_capi.h"
+
+namespace adb {
+
+#define ADB_HIP(call)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) throw HipError(e_, #call, __FILE__, __LINE__);             \
+    } while (0)
+
+struct HipError {
+    hipError_t e; std::string what;
+    HipError(hipError_t e_, const char* call, const char* file, int line) : e(e_) {
+        what = std::string(hipGetErrorString(e_)) + " in " + call + " at " + file + ":" + std::to_string(line);
+    }
+};
+struct AdbError {
+    int code; std::string what;
+    AdbError(int c, std::string w) : code(c), what(std::move(w)) {}
+};
+
+// Grow-only device buffer.
+template <class T> struct DBuf {
+    T* p = nullptr; size_t cap = 0;
+    void reserve(size_t n) {
+        if (n <= cap) return;
+        if (p) ADB_HIP(hipFree(p));
+        size_t c = n + n / 4 + 64;
+        ADB_HIP(hipMalloc((void**)&p, c * sizeof(T)));
+        cap = c;
+    }
+    void free_() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
+};
+
+// Per-kernel HIP-event timing on the world's stream (off unless enabled).
+enum ProfKernel {
+    PK_TRANSFORM = 0, PK_BODY_BOX, PK_BODY_PAIRS, PK_NARROW_SET, PK_VG_SCAN, PK_SORT, PK_NARROW_CCD, PK_VG_TOI,
+    PK_CONTACT, PK_VG_CONTACT, PK_INERTIA, PK_ASM_REDUCE, PK_BLOCK_CHOL, PK_SPMV, PK_PCG_UPDATE, PK_PCG_DIR,
+    PK_ENERGY, PK_COUNT
+};
+struct Prof {
+    bool on = false;
+    struct Rec { int k; hipEvent_t a, b; double bytes; };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    double ms[PK_COUNT] = {0};
+    double bytes[PK_COUNT] = {0};
+    long long count[PK_COUNT] = {0};
+};
+
+// Contact-set buffers (sorted canonical lists).
+struct ContactSet {
+    DBuf<int4> pt, ee;      // (I, v, J, f), (I, ei, J, ej)
+    DBuf<int2> vg;          // (I, v)
+    int n_pt = 0, n_ee = 0, n_vg = 0;
+};
+
+struct World {
+    adb_params P;
+    int B = 0, V = 0, F = 0, E = 0;
+    int max_vpb = 0, max_fpb = 0, max_epb = 0;   // max primitives per body
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // ---- scene (device)
+    DBuf<int> voff, foff, eoff, vbody, fbody, ebody, face, edge;   // face/edge: global vertex ids
+    DBuf<double> xbar, mass, Ic;
+    // ---- state (device)
+    DBuf<double> q, q0, qdot, dq, qtrial;
+    DBuf<double> X, Xs, X1;          // world vertex positions: current, step start, trial
+    // ---- culling scratch
+    DBuf<double> bbox;               // 6 per body
+    DBuf<int2> pairs;                // body pairs (I < J)
+    DBuf<int> counters;              // device counters
+    DBuf<unsigned long long> keys_a, keys_b;  // unsorted / sorted keys
+    DBuf<unsigned char> sort_tmp;
+    DBuf<unsigned long long> toi_bits;
+    DBuf<int2> ovf;                  // body pairs whose LDS lists overflowed
+    DBuf<unsigned long long> vg_keys;
+    ContactSet cs, cs_new;
+    // ---- contact terms (per PT/EE contact, then per VG contact)
+    DBuf<double> cblk;               // 432 per PT/EE contact: hp, ht, off (column-major)
+    DBuf<double> cgrad;              // 24 per PT/EE contact: gp, gt
+    DBuf<double> ctk;                // 24 per PT/EE contact: Tk (2x12 row-major)
+    DBuf<double> clam;               // 1 per PT/EE contact
+    DBuf<double> vgK;                // 12 per VG contact: K (3x3 col-major) + gx (3)
+    DBuf<double> vglam;
+    // ---- assembly / BSR
+    DBuf<double> bodyH, bodyG;       // inertia terms per body (144 + 12)
+    DBuf<unsigned long long> akeys, akeys_s;   // assembly keys (row<<32|col) per contribution
+    DBuf<int> aval, aval_s;          // contribution id: contact * 4 + slot, or -1 - body
+    DBuf<int> blk_of, uniq_flag;     // per sorted contribution
+    DBuf<int> blk_start;             // first contribution of each block
+    DBuf<int> row_ptr, col_idx, diag_blk, vg_start;
+    DBuf<int> chunk_n, chunk_off, chunk_blk;   // segmented-reduction chunks
+    DBuf<double> chunk_part;                   // 156 per chunk (block + gradient)
+    DBuf<double> Hval;               // 144 per block
+    DBuf<double> grad;               // 12 B
+    DBuf<double> Lchol;              // 144 per body (block-Jacobi Cholesky factors)
+    int nnzb = 0, n_contrib = 0;
+    // ---- PCG
+    DBuf<double> pcg_x, pcg_r, pcg_z, pcg_p, pcg_Ap, partial, scal;
+    // ---- energies
+    DBuf<double> epart;
+    // ---- host mirrors
+    std::vector<int> h_voff, h_foff, h_eoff;
+    std::vector<double> h_mass;
+    int stall_pt = 0, stall_vg = 0;
+    int num_cu = 256;
+    Prof prof;
+};
+
+// Brackets a kernel launch with events when profiling is on; `bytes` is the
+// launch's algorithmic HBM traffic.
+struct ProfScope {
+    World& w; int k; double bytes; hipEvent_t a = nullptr;
+    ProfScope(World& w_, int k_, double bytes_) : w(w_), k(k_), bytes(bytes_) {
+        if (!w.prof.on) return;
+        a = get();
+        ADB_HIP(hipEventRecord(a, w.stream));
+    }
+    ~ProfScope() {
+        if (!w.prof.on || !a) return;
+        hipEvent_t b = get();
+        (void)hipEventRecord(b, w.stream);
+        w.prof.pending.push_back({k, a, b, bytes});
+    }
+    hipEvent_t get() {
+        if (!w.prof.pool.empty()) { hipEvent_t e = w.prof.pool.back(); w.prof.pool.pop_back(); return e; }
+        hipEvent_t e; ADB_HIP(hipEventCreate(&e)); return e;
+    }
+};
+void prof_flush(World& w);
+
+// ---- stage launchers (implemented in the .hip files)
+void transform_positions(World& w, const double* q, double* X);                         // cull.hip
+void q_axpy(World& w, const double* q, const double* dq, double alpha, double* out);     // cull.hip
+void collision_set(World& w, const double* X, ContactSet& out);                          // cull.hip
+double ccd_toi(World& w, const double* X0, const double* X1, int* status);              // ccd.hip
+void contact_terms(World& w);                                                            // contact.hip
+void assemble_system(World& w);                                                          // assemble.hip
+int pcg_solve(World& w, double* dq_out, int* iters);                                     // pcg.hip
+double energy_barrier_inertia(World& w, const double* qv, const double* X, const ContactSet& cs); // energy.hip
+double energy_friction(World& w, const double* X);                                       // energy.hip
+double dot_dev(World& w, const double* a, const double* b, int n);                       // energy.hip
+void download_sets(World& w, const ContactSet& cs, std::vector<int>& pt, std::vector<int>& ee, std::vector<int>& vg);
+
+} // namespace adb

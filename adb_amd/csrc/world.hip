@@ -1,0 +1,428 @@
+// world.hip - host driver of the Newton contact step and the C ABI.
+//
+// adb_step replaces ABD::implicit_euler (model/abd.cpp:110-220) with its
+// stages ABD::ipc, ABD::solve, ABD::ccd and ABD::line_search
+// (model/abd.cpp:222-485, model/line_search.cpp:39-136).  The control flow is
+// the reference's: q <- q0; contact set at q; repeat { inertia + contact
+// terms, solve, CCD bound with backoff, Armijo backtracking with a fresh
+// contact set per trial, q += alpha dq } until ||dq|| < tol; then
+// qdot = (q - q0) / dt.  All state stays on the device; the host reads only
+// counters and the scalars that drive the control flow.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <algorithm>
+#include "dmath.h"
+#include "world.h"
+
+struct adb_world { adb::World w; };
+
+namespace adb {
+
+__global__ void k_scale(int n, double* v, double s) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = v[i] * s;
+}
+__global__ void k_velocity(int n, double dt, const double* __restrict__ q, double* __restrict__ q0, double* __restrict__ qdot) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) { qdot[i] = (q[i] - q0[i]) / dt; q0[i] = q[i]; }
+}
+
+static double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+template <class T> static void upload(World& w, DBuf<T>& b, const T* src, size_t n) {
+    b.reserve(std::max<size_t>(n, 1));
+    if (n) ADB_HIP(hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, w.stream));
+}
+
+static void scale(World& w, double* v, int n, double s) {
+    hipLaunchKernelGGL(k_scale, dim3((n + 255) / 256), dim3(256), 0, w.stream, n, v, s);
+    ADB_HIP(hipGetLastError());
+}
+
+static int step(World& w, adb_step_stats* st) {
+    adb_step_stats S;
+    std::memset(&S, 0, sizeof S);
+    S.toi_min = 1.0;
+    const double t_start = now_s();
+    const int n = 12 * w.B;
+    ADB_HIP(hipMemcpyAsync(w.q.p, w.q0.p, n * sizeof(double), hipMemcpyDeviceToDevice, w.stream));
+    transform_positions(w, w.q.p, w.X.p);
+    transform_positions(w, w.q0.p, w.Xs.p);
+    collision_set(w, w.X.p, w.cs);
+    int status = ADB_OK;
+    bool converged = false;
+    for (int iter = 0; iter < w.P.max_iter; iter++) {
+        const double t0 = now_s();
+        contact_terms(w);
+        assemble_system(w);
+        S.contacts_processed += (long long)w.cs.n_pt + w.cs.n_ee + w.cs.n_vg;
+        ADB_HIP(hipStreamSynchronize(w.stream));
+        const double t1 = now_s();
+        S.t_ipc += t1 - t0;
+        int pit = 0;
+        const int prc = pcg_solve(w, w.dq.p, &pit);
+        S.pcg_iters += pit;
+        const double nrm2 = prc == ADB_E_NAN ? NAN : dot_dev(w, w.dq.p, w.dq.p, n);
+        const double sup_dq = std::sqrt(nrm2);
+        const double t2 = now_s();
+        S.t_solver += t2 - t1;
+        if (std::isnan(sup_dq)) { status = ADB_E_NAN; w.err = "solver nan (model/abd.cpp:368-371)"; break; }
+        // CCD step bound (model/abd.cpp:222-276)
+        double toi = 1.0, factor = 1.0;
+        if (w.P.upper_bound) {
+            q_axpy(w, w.q.p, w.dq.p, 1.0, w.qtrial.p);
+            transform_positions(w, w.qtrial.p, w.X1.p);
+            int cs_status = ADB_OK;
+            toi = ccd_toi(w, w.X.p, w.X1.p, &cs_status);
+            if (cs_status != ADB_OK) { status = cs_status; w.err = "toi < 1e-6 twice in a row (model/iaabb.cpp:305-314)"; break; }
+        }
+        if (toi < 1.0) factor = w.P.backoff;
+        S.toi_min = std::min(S.toi_min, toi);
+        scale(w, w.dq.p, n, factor * toi);
+        const double t3 = now_s();
+        S.t_ccd += t3 - t2;
+        // Armijo backtracking (model/line_search.cpp:39-136)
+        double alpha = 1.0;
+        if (w.P.line_search) {
+            const double E0 = energy_barrier_inertia(w, w.q.p, w.X.p, w.cs) + energy_friction(w, w.X.p);
+            const double qdg = dot_dev(w, w.dq.p, w.grad.p, n);
+            const double gn = std::sqrt(dot_dev(w, w.grad.p, w.grad.p, n));
+            int trials = 0;
+            while (true) {
+                q_axpy(w, w.q.p, w.dq.p, alpha, w.qtrial.p);
+                transform_positions(w, w.qtrial.p, w.X1.p);
+                collision_set(w, w.X1.p, w.cs_new);
+                const double E1 = energy_barrier_inertia(w, w.qtrial.p, w.X1.p, w.cs_new) + energy_friction(w, w.X1.p);
+                const bool wolfe = E1 <= E0 + w.P.c1 * alpha * qdg;
+                alpha /= 2;
+                trials++;
+                if (wolfe || !(gn > 1e-3)) break;
+                if (trials >= w.P.max_ls_iter) { status = ADB_E_NOT_CONVERGED; w.err = "line search exhausted"; break; }
+            }
+            S.ls_trials += trials;
+            alpha *= 2;
+            if (status != ADB_OK) break;
+        } else {
+            q_axpy(w, w.q.p, w.dq.p, 1.0, w.qtrial.p);
+            transform_positions(w, w.qtrial.p, w.X1.p);
+            collision_set(w, w.X1.p, w.cs_new);
+        }
+        // q += alpha dq is exactly the last trial's q + dq * alpha
+        std::swap(w.q, w.qtrial);
+        std::swap(w.X, w.X1);
+        std::swap(w.cs, w.cs_new);
+        S.t_line_search += now_s() - t3;
+        S.newton_iters++;
+        if (sup_dq < w.P.tol) { converged = true; break; }
+    }
+    if (status == ADB_OK && !converged) status = ADB_E_NOT_CONVERGED;
+    if (status == ADB_OK || status == ADB_E_NOT_CONVERGED) {
+        hipLaunchKernelGGL(k_velocity, dim3((n + 255) / 256), dim3(256), 0, w.stream, n, w.P.dt, w.q.p, w.q0.p, w.qdot.p);
+        ADB_HIP(hipGetLastError());
+    }
+    ADB_HIP(hipStreamSynchronize(w.stream));
+    S.n_pt = w.cs.n_pt; S.n_ee = w.cs.n_ee; S.n_vg = w.cs.n_vg;
+    S.t_total = now_s() - t_start;
+    if (st) *st = S;
+    return status;
+}
+
+void download_sets(World& w, const ContactSet& cs, std::vector<int>& pt, std::vector<int>& ee, std::vector<int>& vg) {
+    pt.resize(4 * (size_t)cs.n_pt); ee.resize(4 * (size_t)cs.n_ee); vg.resize(2 * (size_t)cs.n_vg);
+    if (cs.n_pt) ADB_HIP(hipMemcpyAsync(pt.data(), cs.pt.p, pt.size() * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+    if (cs.n_ee) ADB_HIP(hipMemcpyAsync(ee.data(), cs.ee.p, ee.size() * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+    if (cs.n_vg) ADB_HIP(hipMemcpyAsync(vg.data(), cs.vg.p, vg.size() * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+    ADB_HIP(hipStreamSynchronize(w.stream));
+}
+
+// assemble at the current q (= q0 between steps)
+static void assemble_now(World& w) {
+    const int n = 12 * w.B;
+    ADB_HIP(hipMemcpyAsync(w.q.p, w.q0.p, n * sizeof(double), hipMemcpyDeviceToDevice, w.stream));
+    transform_positions(w, w.q.p, w.X.p);
+    transform_positions(w, w.q0.p, w.Xs.p);
+    collision_set(w, w.X.p, w.cs);
+    contact_terms(w);
+    assemble_system(w);
+}
+
+} // namespace adb
+
+using namespace adb;
+
+#define ADB_TRY(wptr, ...)                                                    \
+    try { __VA_ARGS__ }                                                       \
+    catch (const HipError& e) { if (wptr) (wptr)->w.err = e.what; return ADB_E_HIP; } \
+    catch (const AdbError& e) { if (wptr) (wptr)->w.err = e.what; return e.code; }    \
+    catch (const std::exception& e) { if (wptr) (wptr)->w.err = e.what(); return ADB_E_ARG; }
+
+extern "C" void adb_params_default(adb_params* p) {
+    std::memset(p, 0, sizeof *p);
+    p->dt = 1e-2; p->d_hat = 1e-4; p->kappa = 1e-4; p->kappa_ortho = 1e9; p->mu = 0.0; p->evh = 1e-3;
+    p->eps_x = 1e-3; p->tol = 1e-2; p->c1 = 1e-4; p->backoff = 0.8; p->max_uk = 0.0;
+    p->gravity[0] = 0.0; p->gravity[1] = -9.8; p->gravity[2] = 0.0; p->ground_y = -0.5;
+    p->max_iter = 1000; p->max_ls_iter = 200;
+    p->enable_pt = p->enable_ee = p->enable_ground = 1;
+    p->pt_fric = p->ee_fric = p->vg_fric = 1;
+    p->psd = 1; p->line_search = 1; p->upper_bound = 1; p->broad_phase_fp32 = 0;
+    p->pcg_rtol = 1e-12; p->pcg_max_iter = 2000;
+}
+
+extern "C" int adb_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return ADB_E_NO_DEVICE;
+    int good = 0;
+    for (int i = 0; i < n; i++) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, i) == hipSuccess && std::strncmp(prop.gcnArchName, "gfx950", 6) == 0) good++;
+    }
+    return good;
+}
+
+extern "C" int adb_world_create(const adb_scene* s, const adb_params* p, adb_world** out) {
+    if (!s || !p || !out || s->n_bodies <= 0) return ADB_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ADB_E_NO_DEVICE;
+    adb_world* W = new adb_world();
+    World& w = W->w;
+    try {
+        w.P = *p;
+        w.B = s->n_bodies;
+        w.h_voff.assign(s->vert_offset, s->vert_offset + w.B + 1);
+        w.h_foff.assign(s->face_offset, s->face_offset + w.B + 1);
+        w.h_eoff.assign(s->edge_offset, s->edge_offset + w.B + 1);
+        w.V = w.h_voff[w.B]; w.F = w.h_foff[w.B]; w.E = w.h_eoff[w.B];
+        if (w.V <= 0) throw AdbError(ADB_E_ARG, "scene has no vertices");
+        ADB_HIP(hipStreamCreate(&w.stream));
+        hipDeviceProp_t prop;
+        int dev = 0;
+        ADB_HIP(hipGetDevice(&dev));
+        ADB_HIP(hipGetDeviceProperties(&prop, dev));
+        w.num_cu = prop.multiProcessorCount;
+        std::vector<int> vbody(w.V), fbody(std::max(w.F, 1)), ebody(std::max(w.E, 1)), face(3 * (size_t)w.F), edge(2 * (size_t)w.E);
+        for (int b = 0; b < w.B; b++) {
+            w.max_vpb = std::max(w.max_vpb, w.h_voff[b + 1] - w.h_voff[b]);
+            for (int v = w.h_voff[b]; v < w.h_voff[b + 1]; v++) vbody[v] = b;
+            for (int f = w.h_foff[b]; f < w.h_foff[b + 1]; f++) {
+                fbody[f] = b;
+                for (int k = 0; k < 3; k++) {
+                    int lv = s->faces[3 * (size_t)f + k];
+                    if (lv < 0 || lv >= w.h_voff[b + 1] - w.h_voff[b]) throw AdbError(ADB_E_ARG, "face index out of range");
+                    face[3 * (size_t)f + k] = w.h_voff[b] + lv;
+                }
+            }
+            for (int e = w.h_eoff[b]; e < w.h_eoff[b + 1]; e++) {
+                ebody[e] = b;
+                for (int k = 0; k < 2; k++) {
+                    int lv = s->edges[2 * (size_t)e + k];
+                    if (lv < 0 || lv >= w.h_voff[b + 1] - w.h_voff[b]) throw AdbError(ADB_E_ARG, "edge index out of range");
+                    edge[2 * (size_t)e + k] = w.h_voff[b] + lv;
+                }
+            }
+        }
+        upload(w, w.voff, w.h_voff.data(), w.B + 1);
+        upload(w, w.foff, w.h_foff.data(), w.B + 1);
+        upload(w, w.eoff, w.h_eoff.data(), w.B + 1);
+        upload(w, w.vbody, vbody.data(), w.V);
+        upload(w, w.fbody, fbody.data(), fbody.size());
+        upload(w, w.ebody, ebody.data(), ebody.size());
+        upload(w, w.face, face.data(), face.size());
+        upload(w, w.edge, edge.data(), edge.size());
+        upload(w, w.xbar, s->rest_verts, 3 * (size_t)w.V);
+        upload(w, w.mass, s->mass, w.B);
+        upload(w, w.Ic, s->Ic, w.B);
+        upload(w, w.q0, s->q0, 12 * (size_t)w.B);
+        upload(w, w.qdot, s->dqdt, 12 * (size_t)w.B);
+        w.h_mass.assign(s->mass, s->mass + w.B);
+        const size_t n = 12 * (size_t)w.B;
+        w.q.reserve(n); w.dq.reserve(n); w.qtrial.reserve(n);
+        ADB_HIP(hipMemcpyAsync(w.q.p, w.q0.p, n * sizeof(double), hipMemcpyDeviceToDevice, w.stream));
+        w.X.reserve(3 * (size_t)w.V); w.Xs.reserve(3 * (size_t)w.V); w.X1.reserve(3 * (size_t)w.V);
+        w.counters.reserve(16);
+        ADB_HIP(hipStreamSynchronize(w.stream));
+    } catch (const HipError& e) {
+        delete W; return ADB_E_HIP;
+    } catch (const AdbError& e) {
+        delete W; return e.code;
+    }
+    *out = W;
+    return ADB_OK;
+}
+
+static void free_world(World& w) {
+    DBuf<int>* ib[] = {&w.voff, &w.foff, &w.eoff, &w.vbody, &w.fbody, &w.ebody, &w.face, &w.edge, &w.counters,
+                       &w.aval, &w.aval_s, &w.blk_of, &w.uniq_flag, &w.blk_start, &w.row_ptr, &w.col_idx, &w.diag_blk, &w.vg_start};
+    for (auto* b : ib) b->free_();
+    DBuf<double>* db[] = {&w.xbar, &w.mass, &w.Ic, &w.q, &w.q0, &w.qdot, &w.dq, &w.qtrial, &w.X, &w.Xs, &w.X1, &w.bbox,
+                          &w.cblk, &w.cgrad, &w.ctk, &w.clam, &w.vgK, &w.vglam, &w.bodyH, &w.bodyG, &w.Hval, &w.grad,
+                          &w.Lchol, &w.pcg_x, &w.pcg_r, &w.pcg_z, &w.pcg_p, &w.pcg_Ap, &w.partial, &w.scal, &w.epart};
+    for (auto* b : db) b->free_();
+    w.pairs.free_(); w.ovf.free_(); w.keys_a.free_(); w.keys_b.free_(); w.sort_tmp.free_(); w.toi_bits.free_(); w.vg_keys.free_();
+    w.akeys.free_(); w.akeys_s.free_();
+    for (ContactSet* c : {&w.cs, &w.cs_new}) { c->pt.free_(); c->ee.free_(); c->vg.free_(); }
+    if (w.stream) (void)hipStreamDestroy(w.stream);
+    w.stream = nullptr;
+}
+
+extern "C" void adb_world_destroy(adb_world* W) {
+    if (!W) return;
+    free_world(W->w);
+    delete W;
+}
+
+extern "C" int adb_step(adb_world* W, adb_step_stats* st) {
+    if (!W) return ADB_E_ARG;
+    ADB_TRY(W, { return step(W->w, st); })
+}
+
+extern "C" int adb_get_state(adb_world* W, double* q, double* qdot) {
+    if (!W) return ADB_E_ARG;
+    ADB_TRY(W, {
+        World& w = W->w;
+        const size_t n = 12 * (size_t)w.B * sizeof(double);
+        if (q) ADB_HIP(hipMemcpyAsync(q, w.q0.p, n, hipMemcpyDeviceToHost, w.stream));
+        if (qdot) ADB_HIP(hipMemcpyAsync(qdot, w.qdot.p, n, hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipStreamSynchronize(w.stream));
+        return ADB_OK;
+    })
+}
+
+extern "C" int adb_set_state(adb_world* W, const double* q, const double* qdot) {
+    if (!W) return ADB_E_ARG;
+    ADB_TRY(W, {
+        World& w = W->w;
+        const size_t n = 12 * (size_t)w.B * sizeof(double);
+        if (q) {
+            ADB_HIP(hipMemcpyAsync(w.q0.p, q, n, hipMemcpyHostToDevice, w.stream));
+            ADB_HIP(hipMemcpyAsync(w.q.p, q, n, hipMemcpyHostToDevice, w.stream));
+        }
+        if (qdot) ADB_HIP(hipMemcpyAsync(w.qdot.p, qdot, n, hipMemcpyHostToDevice, w.stream));
+        ADB_HIP(hipStreamSynchronize(w.stream));
+        return ADB_OK;
+    })
+}
+
+extern "C" const char* adb_last_error(adb_world* W) { return W ? W->w.err.c_str() : "null world"; }
+
+namespace adb {
+void prof_flush(World& w) {
+    if (w.prof.pending.empty()) return;
+    ADB_HIP(hipStreamSynchronize(w.stream));
+    for (auto& r : w.prof.pending) {
+        float ms = 0.0f;
+        ADB_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        w.prof.ms[r.k] += ms;
+        w.prof.bytes[r.k] += r.bytes;
+        w.prof.count[r.k] += 1;
+        w.prof.pool.push_back(r.a);
+        w.prof.pool.push_back(r.b);
+    }
+    w.prof.pending.clear();
+}
+} // namespace adb
+
+extern "C" int adb_profile_enable(adb_world* W, int on) {
+    if (!W) return ADB_E_ARG;
+    ADB_TRY(W, {
+        World& w = W->w;
+        prof_flush(w);
+        w.prof.on = on != 0;
+        for (int k = 0; k < PK_COUNT; k++) { w.prof.ms[k] = 0.0; w.prof.bytes[k] = 0.0; w.prof.count[k] = 0; }
+        return ADB_OK;
+    })
+}
+
+extern "C" int adb_profile_read(adb_world* W, int kernel, double* ms, long long* launches, double* bytes) {
+    if (!W || kernel < 0 || kernel >= PK_COUNT) return ADB_E_ARG;
+    ADB_TRY(W, {
+        World& w = W->w;
+        prof_flush(w);
+        if (ms) *ms = w.prof.ms[kernel];
+        if (launches) *launches = w.prof.count[kernel];
+        if (bytes) *bytes = w.prof.bytes[kernel];
+        return ADB_OK;
+    })
+}
+
+extern "C" int adb_collision_set(adb_world* W, int vtn, const double* dq, int* pt, int pt_cap, int* n_pt, int* ee, int ee_cap,
+                                 int* n_ee, int* vg, int vg_cap, int* n_vg) {
+    if (!W || (vtn != 1 && vtn != 2) || (vtn == 2 && !dq)) return ADB_E_ARG;
+    ADB_TRY(W, {
+        World& w = W->w;
+        const size_t n = 12 * (size_t)w.B;
+        ADB_HIP(hipMemcpyAsync(w.q.p, w.q0.p, n * sizeof(double), hipMemcpyDeviceToDevice, w.stream));
+        if (vtn == 2) {
+            ADB_HIP(hipMemcpyAsync(w.dq.p, dq, n * sizeof(double), hipMemcpyHostToDevice, w.stream));
+            q_axpy(w, w.q.p, w.dq.p, 1.0, w.qtrial.p);
+        } else {
+            q_axpy(w, w.q.p, nullptr, 0.0, w.qtrial.p);
+        }
+        transform_positions(w, w.qtrial.p, w.X1.p);
+        collision_set(w, w.X1.p, w.cs_new);
+        std::vector<int> a, b, c;
+        download_sets(w, w.cs_new, a, b, c);
+        *n_pt = w.cs_new.n_pt; *n_ee = w.cs_new.n_ee; *n_vg = w.cs_new.n_vg;
+        if (pt) std::memcpy(pt, a.data(), sizeof(int) * std::min(a.size(), 4 * (size_t)std::max(pt_cap, 0)));
+        if (ee) std::memcpy(ee, b.data(), sizeof(int) * std::min(b.size(), 4 * (size_t)std::max(ee_cap, 0)));
+        if (vg) std::memcpy(vg, c.data(), sizeof(int) * std::min(c.size(), 2 * (size_t)std::max(vg_cap, 0)));
+        if (*n_pt > pt_cap || *n_ee > ee_cap || *n_vg > vg_cap) return ADB_E_CAPACITY;
+        return ADB_OK;
+    })
+}
+
+extern "C" int adb_step_size_upper_bound(adb_world* W, const double* dq, double* toi) {
+    if (!W || !dq || !toi) return ADB_E_ARG;
+    ADB_TRY(W, {
+        World& w = W->w;
+        const size_t n = 12 * (size_t)w.B;
+        ADB_HIP(hipMemcpyAsync(w.q.p, w.q0.p, n * sizeof(double), hipMemcpyDeviceToDevice, w.stream));
+        ADB_HIP(hipMemcpyAsync(w.dq.p, dq, n * sizeof(double), hipMemcpyHostToDevice, w.stream));
+        transform_positions(w, w.q.p, w.X.p);
+        q_axpy(w, w.q.p, w.dq.p, 1.0, w.qtrial.p);
+        transform_positions(w, w.qtrial.p, w.X1.p);
+        int sp = w.stall_pt, sv = w.stall_vg, status = ADB_OK;
+        *toi = ccd_toi(w, w.X.p, w.X1.p, &status);
+        w.stall_pt = sp; w.stall_vg = sv;
+        return ADB_OK;
+    })
+}
+
+extern "C" int adb_assemble(adb_world* W, double* grad, double* hess) {
+    if (!W) return ADB_E_ARG;
+    ADB_TRY(W, {
+        World& w = W->w;
+        assemble_now(w);
+        const int n = 12 * w.B;
+        if (grad) ADB_HIP(hipMemcpyAsync(grad, w.grad.p, n * sizeof(double), hipMemcpyDeviceToHost, w.stream));
+        if (hess) {
+            std::vector<int> rp(w.B + 1), ci(w.nnzb);
+            std::vector<double> hv(144 * (size_t)w.nnzb);
+            ADB_HIP(hipMemcpyAsync(rp.data(), w.row_ptr.p, rp.size() * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+            ADB_HIP(hipMemcpyAsync(ci.data(), w.col_idx.p, ci.size() * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+            ADB_HIP(hipMemcpyAsync(hv.data(), w.Hval.p, hv.size() * sizeof(double), hipMemcpyDeviceToHost, w.stream));
+            ADB_HIP(hipStreamSynchronize(w.stream));
+            std::memset(hess, 0, sizeof(double) * (size_t)n * n);
+            for (int r = 0; r < w.B; r++)
+                for (int k = rp[r]; k < rp[r + 1]; k++)
+                    for (int c = 0; c < 12; c++)
+                        for (int i = 0; i < 12; i++)
+                            hess[(size_t)(12 * ci[k] + c) * n + 12 * r + i] = hv[144 * (size_t)k + 12 * c + i];
+        }
+        ADB_HIP(hipStreamSynchronize(w.stream));
+        return ADB_OK;
+    })
+}
+
+extern "C" int adb_newton_direction(adb_world* W, double* dq) {
+    if (!W || !dq) return ADB_E_ARG;
+    ADB_TRY(W, {
+        World& w = W->w;
+        assemble_now(w);
+        int it = 0;
+        int rc = pcg_solve(w, w.dq.p, &it);
+        ADB_HIP(hipMemcpyAsync(dq, w.dq.p, 12 * (size_t)w.B * sizeof(double), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipStreamSynchronize(w.stream));
+        return rc;
+    })
+}

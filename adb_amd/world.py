@@ -1,0 +1,109 @@
+"""Python handle on an adb_world (include/adb_capi.h) - the MI355X Newton
+contact step.  Mirrors the reference's ABD driver (model/abd.h:9-80): one
+`step()` is ABD::implicit_euler (model/abd.cpp:110-220)."""
+import ctypes as C
+
+import numpy as np
+
+from .capi import (ADB_OK, STATUS_NAMES, AdbStepStats, SceneArrays, default_params, dptr, iptr,
+                   load_library)
+
+
+class AdbError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (STATUS_NAMES.get(code, "?"), code, msg))
+        self.code = code
+
+
+class World:
+    def __init__(self, scene, params=None):
+        self.lib = load_library()
+        self.params = params if params is not None else default_params()
+        self.arrays = SceneArrays(scene)
+        self.B = len(scene["mass"])
+        h = C.c_void_p()
+        rc = self.lib.adb_world_create(C.byref(self.arrays.struct), C.byref(self.params), C.byref(h))
+        if rc != ADB_OK:
+            raise AdbError(rc, "adb_world_create failed")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None:
+            self.lib.adb_world_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what, allow=()):
+        if rc != ADB_OK and rc not in allow:
+            raise AdbError(rc, "%s: %s" % (what, self.lib.adb_last_error(self.h).decode()))
+        return rc
+
+    def step(self, allow_not_converged=True):
+        st = AdbStepStats()
+        rc = self.lib.adb_step(self.h, C.byref(st))
+        allow = (-6,) if allow_not_converged else ()
+        self._check(rc, "adb_step", allow)
+        return rc, st
+
+    def state(self):
+        q = np.zeros(12 * self.B)
+        qd = np.zeros(12 * self.B)
+        self._check(self.lib.adb_get_state(self.h, dptr(q), dptr(qd)), "adb_get_state")
+        return q.reshape(-1, 12), qd.reshape(-1, 12)
+
+    def set_state(self, q, qdot=None):
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        qd = None if qdot is None else np.ascontiguousarray(qdot, dtype=np.float64).reshape(-1)
+        self._check(self.lib.adb_set_state(self.h, dptr(q), dptr(qd) if qd is not None else None), "adb_set_state")
+
+    def collision_set(self, vtn=1, dq=None, cap=1 << 20):
+        while True:
+            pt = np.zeros(4 * cap, dtype=np.int32)
+            ee = np.zeros(4 * cap, dtype=np.int32)
+            vg = np.zeros(2 * cap, dtype=np.int32)
+            npt, nee, nvg = C.c_int(), C.c_int(), C.c_int()
+            dqa = None if dq is None else np.ascontiguousarray(dq, dtype=np.float64).reshape(-1)
+            rc = self.lib.adb_collision_set(self.h, vtn, dptr(dqa) if dqa is not None else None, iptr(pt), cap,
+                                            C.byref(npt), iptr(ee), cap, C.byref(nee), iptr(vg), cap, C.byref(nvg))
+            if rc == -5:
+                cap = max(npt.value, nee.value, nvg.value) + 16
+                continue
+            self._check(rc, "adb_collision_set")
+            return (pt[:4 * npt.value].reshape(-1, 4).copy(), ee[:4 * nee.value].reshape(-1, 4).copy(),
+                    vg[:2 * nvg.value].reshape(-1, 2).copy())
+
+    def toi(self, dq):
+        t = C.c_double()
+        dqa = np.ascontiguousarray(dq, dtype=np.float64).reshape(-1)
+        self._check(self.lib.adb_step_size_upper_bound(self.h, dptr(dqa), C.byref(t)), "adb_step_size_upper_bound")
+        return t.value
+
+    def assemble(self, dense=True):
+        n = 12 * self.B
+        g = np.zeros(n)
+        H = np.zeros(n * n) if dense else None
+        self._check(self.lib.adb_assemble(self.h, dptr(g), dptr(H) if dense else None), "adb_assemble")
+        return g, (H.reshape(n, n).T if dense else None)
+
+    def profile_enable(self, on=True):
+        self._check(self.lib.adb_profile_enable(self.h, 1 if on else 0), "adb_profile_enable")
+
+    def profile_read(self):
+        from .capi import PROF_KERNELS
+        out = {}
+        for k, name in enumerate(PROF_KERNELS):
+            ms, n, b = C.c_double(), C.c_longlong(), C.c_double()
+            self._check(self.lib.adb_profile_read(self.h, k, C.byref(ms), C.byref(n), C.byref(b)), "adb_profile_read")
+            out[name] = {"ms": ms.value, "launches": n.value, "bytes": b.value}
+        return out
+
+    def newton_direction(self):
+        dq = np.zeros(12 * self.B)
+        rc = self.lib.adb_newton_direction(self.h, dptr(dq))
+        self._check(rc, "adb_newton_direction", allow=(-6,))
+        return rc, dq

@@ -1,0 +1,154 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X Newton contact step (one GPU).
+
+Workload: 2000 icospheres (subdivision 3: 642 vertices, 1280 triangles, 1920
+edges each; radius U(0.05, 0.15)) rejection-packed in a 4 x 4 x 8 box above a
+ground plane, uniform random rotations, v0 ~ N(0, 0.5^2), mu = 0.3, seed 2.
+Parameters: dt = 0.01, d_hat = 1e-3 (squared barrier threshold), barrier
+kappa = 1e-4, orthogonality kappa = 1e9, Newton tol 1e-2, Armijo c1 = 1e-4,
+CCD backoff 0.8 (the reference's config.json / settings.h values).
+
+A step is one full adb_step (Newton loop with CCD and line search) on
+synthetic input already resident in HBM.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MI355X_HBM_GBS = 8000.0
+MI355X_FP64_TFLOPS = 78.6
+
+
+def workload(n_bodies=2000):
+    from adb_amd import scenes
+    sc = scenes.sphere_pack(n_bodies, seed=2, subdiv=3)
+    return sc
+
+
+def params(precision="fp64"):
+    from adb_amd.capi import default_params
+    return default_params(d_hat=1e-3, kappa=1e-4, kappa_ortho=1e9, mu=0.3, dt=0.01, tol=1e-2,
+                          broad_phase_fp32=1 if precision == "mixed" else 0)
+
+
+def cpu_baseline(sc, p, budget_s):
+    """The oracle (FP64 C++/OpenMP restatement of the reference path) timed on
+    the host: as many whole steps from the initial state as fit in budget_s."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    threads = O.lib().oracle_set_threads(min(os.cpu_count() or 1, 16))
+    w = O.OracleWorld(sc, p)
+    t0 = time.perf_counter()
+    steps = 0
+    pairs = 0
+    while True:
+        rc, st = w.step()
+        steps += 1
+        pairs += st.contacts_processed
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 50:
+            break
+    return {"value": steps / el, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": "first %d steps of the same 2000-icosphere scene from its initial state, %.1f s, "
+                      "oracle FP64 restatement with OpenMP (%d threads); contact-pairs/s %.3g"
+                      % (steps, el, threads, pairs / el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--bodies", type=int, default=2000)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-json", default="")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the contact step of one scene runs on one device "
+              "(multi-GPU sharding is a design note, not implemented); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from adb_amd.world import World
+    sc = workload(args.bodies)
+    p = params(args.precision)
+    w = World(sc, p)
+    for _ in range(args.warmup):
+        w.step()
+    w.profile_enable(True)
+    w.state()   # synchronizes the world's stream
+    tot = {"contacts": 0, "newton": 0, "ls": 0, "pcg": 0, "t_ipc": 0.0, "t_solver": 0.0, "t_ccd": 0.0, "t_ls": 0.0}
+    t0 = time.perf_counter()
+    last = None
+    for _ in range(args.steps):
+        rc, st = w.step()
+        last = st
+        tot["contacts"] += st.contacts_processed
+        tot["newton"] += st.newton_iters
+        tot["ls"] += st.ls_trials
+        tot["pcg"] += st.pcg_iters
+        tot["t_ipc"] += st.t_ipc
+        tot["t_solver"] += st.t_solver
+        tot["t_ccd"] += st.t_ccd
+        tot["t_ls"] += st.t_line_search
+    w.state()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    prof = w.profile_read()
+    w.profile_enable(False)
+    if args.profile_json:
+        with open(args.profile_json, "w") as fh:
+            json.dump(prof, fh, indent=1)
+
+    steps_per_s = args.steps / elapsed
+    # dominant kernel by device time among those with an algorithmic byte count
+    ranked = sorted(prof.items(), key=lambda kv: -kv[1]["ms"])
+    dom = next(((k, v) for k, v in ranked if v["bytes"] > 0 and v["launches"] > 0), None)
+    roofline = None
+    if dom:
+        k, v = dom
+        avg_ms = v["ms"] / v["launches"]
+        bytes_per_launch = v["bytes"] / v["launches"]
+        gbs = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        roofline = {"kernel": k, "bound": "hbm", "achieved": round(gbs, 1), "peak": MI355X_HBM_GBS,
+                    "unit": "GB/s", "frac": round(gbs / MI355X_HBM_GBS, 4), "traffic": None,
+                    "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch}
+
+    line = {
+        "metric": "steps/s at 2k bodies (mixed vs FP64, same traj err); contact-pairs/s",
+        "value": round(steps_per_s, 4),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f64 (f32 broad-phase boxes)",
+        "data": "synthetic (seeded PCG32 scene, no dataset)",
+        "config": {"workload": "2000 icospheres (642 v / 1280 t / 1920 e each), 4x4x8 box, ground, mu=0.3, "
+                               "dt=0.01, d_hat=1e-3, seed 2",
+                   "bodies": args.bodies, "precision": args.precision},
+        "contact_pairs_per_s": round(tot["contacts"] / elapsed, 1),
+        "newton_iters": tot["newton"], "ls_trials": tot["ls"], "pcg_iters": tot["pcg"],
+        "phase_s": {"ipc": round(tot["t_ipc"], 4), "solver": round(tot["t_solver"], 4),
+                    "ccd": round(tot["t_ccd"], 4), "line_search": round(tot["t_ls"], 4)},
+        "contacts_last_step": {"pt": last.n_pt, "ee": last.n_ee, "vg": last.n_vg} if last else None,
+        "top_kernels_ms": {k: round(v["ms"], 3) for k, v in ranked[:6]},
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(sc, p, args.cpu_seconds)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

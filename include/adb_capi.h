@@ -192,6 +192,20 @@ int adb_ee_distance_batched(int n, const double* ee, int* type, double* dist, vo
  * or a negative status.  Does not allocate device memory. */
 int adb_device_count(void);
 
+/* ---- measurement hooks -----------------------------------------------------
+ * No counterpart in the reference (its timers are the four std::chrono phase
+ * buckets of model/abd.h:36-39, reported in adb_step_stats).  Per-kernel
+ * HIP-event timing on the world's own stream, for roofline figures:
+ * adb_profile_enable(w, 1) clears and starts, adb_profile_read returns the
+ * accumulated time, launch count and algorithmic bytes of one kernel. */
+enum adb_prof_kernel {
+    ADB_PK_TRANSFORM = 0, ADB_PK_BODY_BOX, ADB_PK_BODY_PAIRS, ADB_PK_NARROW_SET, ADB_PK_VG_SCAN, ADB_PK_SORT,
+    ADB_PK_NARROW_CCD, ADB_PK_VG_TOI, ADB_PK_CONTACT, ADB_PK_VG_CONTACT, ADB_PK_INERTIA, ADB_PK_ASM_REDUCE,
+    ADB_PK_BLOCK_CHOL, ADB_PK_SPMV, ADB_PK_PCG_UPDATE, ADB_PK_PCG_DIR, ADB_PK_ENERGY, ADB_PK_COUNT
+};
+int adb_profile_enable(adb_world* w, int on);
+int adb_profile_read(adb_world* w, int kernel, double* total_ms, long long* launches, double* bytes);
+
 #ifdef __cplusplus
 }
 #endif
