@@ -118,7 +118,7 @@ struct ReduceArgs {
     int nnzb, n_pt, n_ee, B;
     const unsigned long long* keys; const int* vals; const int* blk_start;
     const int4* pt; const int4* ee; const double* mass;
-    const double* cblk; const double* cgrad; const double* bodyH; const double* bodyG;
+    const double* rec; const double* bodyH; const double* bodyG;
     const int2* vg; const int* vg_start; const double* vgK; const int* voff; const double* xbar;
     double* Hval; double* grad;
 };
@@ -157,21 +157,31 @@ __global__ __launch_bounds__(64) void k_reduce_chunks(ReduceArgs a, int nchunks,
         const int c = v / 4, slot = v % 4;
         const int4 ct = c < a.n_pt ? a.pt[c] : a.ee[c - a.n_pt];
         const bool di = a.mass[ct.x] > 0.0, dj = a.mass[ct.z] > 0.0;
-        const double* blk = a.cblk + 432 * (size_t)c;
-        if (slot == 0 && di) {
-            for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) acc[u] += blk[e]; }
-            if (t < 12) gacc += a.cgrad[24 * (size_t)c + t];
-        } else if (slot == 1 && dj) {
-            for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) acc[u] += blk[144 + e]; }
-            if (t < 12) gacc += a.cgrad[24 * (size_t)c + 12 + t];
-        } else if (slot == 2 && di && dj) {
-            for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) acc[u] += blk[288 + e]; }
-        } else if (slot == 3 && di && dj) {
-            for (int u = 0; u < 3; u++) {
-                int e = t + 64 * u;
-                if (e < 144) { int cc = e / 12, rr = e % 12; acc[u] += blk[288 + rr * 12 + cc]; }
+        const bool use = slot == 0 ? di : (slot == 1 ? dj : (di && dj));
+        const double* R = a.rec + REC * (size_t)c;
+        const int D = use ? (int)R[R_D] : 0;
+        if (D == 0) continue;
+        // block (s1, s2): slot 0 = (I, I), 1 = (J, J), 2 = (I, J), 3 = (J, I)
+        const int s1 = (slot == 1 || slot == 3) ? 1 : 0, s2 = (slot == 1 || slot == 2) ? 1 : 0;
+        const int nA = D / 3;
+        for (int u = 0; u < 3; u++) {
+            const int e = t + 64 * u;
+            if (e < 144) {
+                // column (beta, cc), row (alpha, r)
+                const int col = e / 12, row = e % 12;
+                const int be = col / 3, cc = col % 3, al = row / 3, r = row % 3;
+                double val = R[R_G + 4 * s1 + al] * R[R_G + 4 * s2 + be] * R[R_F + 3 * r + cc];
+                for (int aa = 0; aa < nA; aa++) {
+                    const double za = R[R_Z + 8 * aa + 4 * s1 + al];
+                    double sb = 0.0;
+                    for (int bb = 0; bb < nA; bb++)
+                        sb += R[R_Z + 8 * bb + 4 * s2 + be] * R[R_PK + (3 * aa + r) * D + 3 * bb + cc];
+                    val += za * sb;
+                }
+                acc[u] += val;
             }
         }
+        if (slot < 2 && t < 12) gacc += R[R_GRAD + 12 * s1 + t];
     }
     double* P = part + 156 * (size_t)ch;
     for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) P[e] = acc[u]; }
@@ -300,7 +310,7 @@ void assemble_system(World& w) {
     a.nnzb = nnzb; a.n_pt = w.cs.n_pt; a.n_ee = w.cs.n_ee; a.B = B;
     a.keys = w.akeys_s.p; a.vals = w.aval_s.p; a.blk_start = w.blk_start.p;
     a.pt = w.cs.pt.p; a.ee = w.cs.ee.p; a.mass = w.mass.p;
-    a.cblk = w.cblk.p; a.cgrad = w.cgrad.p; a.bodyH = w.bodyH.p; a.bodyG = w.bodyG.p;
+    a.rec = w.crec.p; a.bodyH = w.bodyH.p; a.bodyG = w.bodyG.p;
     a.vg = w.cs.vg.p; a.vg_start = w.vg_start.p; a.vgK = w.vgK.p; a.voff = w.voff.p; a.xbar = w.xbar.p;
     a.Hval = w.Hval.p; a.grad = w.grad.p;
     {
@@ -320,8 +330,10 @@ void assemble_system(World& w) {
         hipLaunchKernelGGL(k_chunk_owner, dim3((nnzb + 255) / 256), dim3(256), 0, w.stream, nnzb, w.chunk_off.p, w.chunk_blk.p);
         ADB_HIP(hipGetLastError());
         {
-            // every contribution block read once, one partial block written per chunk
-            ProfScope ps(w, PK_ASM_REDUCE, 1152.0 * n + 1248.0 * nchunks);
+            // every contact record read once (D = 3: 59 doubles, 6: 94, 9: 147), the
+            // contribution list and inertia terms read, one partial block written per chunk
+            ProfScope ps(w, PK_ASM_REDUCE, 1248.0 * nchunks + 24.0 * n + 1248.0 * B);
+            ps.counts(prof_snapshot(w, w.ccount.p), 472.0, 752.0, 1176.0, 0.0);
             hipLaunchKernelGGL(k_reduce_chunks, dim3(nchunks), dim3(64), 0, w.stream, a, nchunks, w.chunk_off.p, w.chunk_blk.p,
                                w.chunk_part.p);
         }

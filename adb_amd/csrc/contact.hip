@@ -69,9 +69,12 @@ __device__ __forceinline__ double sel4(double a0, double a1, double a2, double a
 }
 
 // ---------------------------------------------------------------- PSD (group)
-// Projects the symmetric vertex-space Hessian in S.H in place (all 64 lanes
-// must call: barriers).  S.H is unchanged when the matrix is already PSD or
-// `on` is false.
+// Input: the symmetric vertex-space Hessian in S.H (all 64 lanes must call:
+// barriers).  HOUT = false: S.H is projected in place, and left unchanged when
+// the matrix is already PSD or `on` is false.  HOUT = true: S.A receives the
+// Helmert-coordinate matrix K = Q H Q^T, replaced by its projection
+// V diag(max(w, 0)) V^T when `on`.
+template <bool HOUT>
 __device__ __forceinline__ void group_psd(GroupLDS& S, int lane, bool on) {
     // K = Q H Q^T: lane u = (b, j) < 9 builds column u
     if (lane < 9) {
@@ -162,7 +165,7 @@ __device__ __forceinline__ void group_psd(GroupLDS& S, int lane, bool on) {
     double wmin = 0.0;
 #pragma unroll
     for (int k = 0; k < 9; k++) { w[k] = S.A[k][k]; wmin = k == 0 ? w[k] : fmin(wmin, w[k]); }
-    const bool project = on && wmin < 0.0;
+    const bool project = HOUT ? on : (on && wmin < 0.0);
     __syncthreads();
     // P(K) = V diag(max(w, 0)) V^T, column `lane`, into A
     if (project && lane < 9) {
@@ -175,6 +178,7 @@ __device__ __forceinline__ void group_psd(GroupLDS& S, int lane, bool on) {
         }
     }
     __syncthreads();
+    if (HOUT) return;
     // H_psd = Q^T P(K) Q: lane v = (l, j) < 12 builds column v
     if (project && lane < 12) {
         const int l = lane / 3, j = lane % 3;
@@ -329,11 +333,11 @@ __device__ __forceinline__ void solve2(double m00, double m01, double m11, doubl
 }
 // coefficients g0..g3 of the relative tangential displacement sum_k g_k x_k
 // and the tangent basis (c0, c1) (model/pt_ipc.cpp:31-103, model/ee_ipc.cpp:38-110)
-__device__ __forceinline__ void friction_frame(const Geo& G, double& g0, double& g1, double& g2, double& g3, d3& c0, d3& c1) {
-    const d3 P0 = G.p0, P1 = G.p1, P2 = G.p2, P3 = G.p3;
-    if (!G.is_ee) {
+__device__ __forceinline__ void friction_frame(int is_ee, int type, d3 P0, d3 P1, d3 P2, d3 P3,
+                                               double& g0, double& g1, double& g2, double& g3, d3& c0, d3& c1) {
+    if (!is_ee) {
         double t0, t1, t2;
-        switch (G.type) {
+        switch (type) {
         case P_T0: basis_pp(P0, P1, c0, c1); t0 = 1; t1 = 0; t2 = 0; break;
         case P_T1: basis_pp(P0, P2, c0, c1); t0 = 0; t1 = 1; t2 = 0; break;
         case P_T2: basis_pp(P0, P3, c0, c1); t0 = 0; t1 = 0; t2 = 1; break;
@@ -351,7 +355,7 @@ __device__ __forceinline__ void friction_frame(const Geo& G, double& g0, double&
         g0 = -1.0; g1 = t0; g2 = t1; g3 = t2;
     } else {
         double l0, l1;
-        switch (G.type) {
+        switch (type) {
         case EA0_EB0: basis_pp(P0, P2, c0, c1); l0 = 0; l1 = 0; break;
         case EA0_EB1: basis_pp(P0, P3, c0, c1); l0 = 0; l1 = 1; break;
         case EA1_EB0: basis_pp(P1, P2, c0, c1); l0 = 1; l1 = 0; break;
@@ -430,7 +434,7 @@ __global__ __launch_bounds__(64) void k_hess_api(KParams kp, int n, const double
         for (int i = 0; i < 12; i++) L.H[i][lane] = 0.0;
     }
     __syncthreads();
-    if (kp.psd) group_psd(L, lane, valid);
+    if (kp.psd) group_psd<false>(L, lane, valid);
     if (valid && lane < 12) {
         for (int i = 0; i < 12; i++) H[144 * (size_t)e + 12 * lane + i] = L.H[i][lane];
         g[12 * (size_t)e + lane] = gj;
@@ -439,130 +443,446 @@ __global__ __launch_bounds__(64) void k_hess_api(KParams kp, int n, const double
 }
 
 // ---------------------------------------------------------------- world contact terms
+// Contact ids: the four world vertex ids of contact `cid` (PT: point, face
+// vertices; EE: edge a, edge b).  Points 0..nside0-1 belong to body I.
 struct WorldArgs {
     KParams kp;
     int n_pt, n_ee;
     const int4* pt; const int4* ee;
     const int* voff; const int* foff; const int* eoff; const int* face; const int* edge;
     const double* X; const double* Xs; const double* xbar;
-    double* blk; double* grad; double* tk; double* lam;
+    int* ctype; double* cdist; int* list; int* list3; int* counts;
+    double* rec; double* tk; double* lam;
 };
 
-__global__ __launch_bounds__(64) void k_contact_world(WorldArgs a) {
-    __shared__ GroupLDS lds[CPW];
-    const int grp = threadIdx.x / GL, lane = threadIdx.x % GL;
-    const int cid = blockIdx.x * CPW + grp;
+__device__ __forceinline__ bool contact_ids(const WorldArgs& a, int cid, int& id0, int& id1, int& id2, int& id3) {
+    if (cid < a.n_pt) {
+        const int4 c = a.pt[cid];
+        const int f = a.foff[c.z] + c.w;
+        id0 = a.voff[c.x] + c.y; id1 = a.face[3 * f]; id2 = a.face[3 * f + 1]; id3 = a.face[3 * f + 2];
+        return false;
+    }
+    const int4 c = a.ee[cid - a.n_pt];
+    const int ea = a.eoff[c.x] + c.y, eb = a.eoff[c.z] + c.w;
+    id0 = a.edge[2 * ea]; id1 = a.edge[2 * ea + 1]; id2 = a.edge[2 * eb]; id3 = a.edge[2 * eb + 1];
+    return true;
+}
+
+// Wave-aggregated append: one atomic per wavefront.  Every lane of the
+// wavefront must call it.
+__device__ __forceinline__ int wave_append(bool pred, int* counter) {
+    const unsigned long long m = __ballot(pred);
+    if (m == 0ull) return -1;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader);
+    return pred ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
+}
+
+// Class of an active contact by the points its barrier depends on:
+// 1 = two points (D = 3), 2 = three points (D = 6), 3 = four points (D = 9,
+// also every mollified edge-edge contact).  Inactive contacts (d >= d_hat)
+// get an empty record here.
+__global__ __launch_bounds__(256) void k_contact_classify(WorldArgs a) {
+    const int cid = blockIdx.x * blockDim.x + threadIdx.x;
     const int total = a.n_pt + a.n_ee;
-    const bool valid = cid < total;
-    GroupLDS& L = lds[grp];
-    Geo G;
-    int id0 = 0, id1 = 0, id2 = 0, id3 = 0;
-    double gj = 0.0;
-    bool active = false;
-    if (valid) {
-        G.is_ee = cid >= a.n_pt;
-        if (!G.is_ee) {
-            const int4 c = a.pt[cid];
-            const int f = a.foff[c.z] + c.w;
-            id0 = a.voff[c.x] + c.y; id1 = a.face[3 * f]; id2 = a.face[3 * f + 1]; id3 = a.face[3 * f + 2];
+    int cls = 0;
+    if (cid < total) {
+        int id0, id1, id2, id3;
+        const bool is_ee = contact_ids(a, cid, id0, id1, id2, id3);
+        const d3 p0 = ld3(a.X + 3 * (size_t)id0), p1 = ld3(a.X + 3 * (size_t)id1);
+        const d3 p2 = ld3(a.X + 3 * (size_t)id2), p3 = ld3(a.X + 3 * (size_t)id3);
+        int type; double d;
+        if (!is_ee) { type = pt_type(p0, p1, p2, p3); d = pt_dist(p0, p1, p2, p3, type); }
+        else { type = ee_type(p0, p1, p2, p3); d = ee_dist(p0, p1, p2, p3, type); }
+        a.ctype[cid] = type; a.cdist[cid] = d;
+        if (d < a.kp.d_hat) {
+            const int kind = is_ee ? ee_map(type).kind : pt_map(type).kind;
+            cls = kind == K_PP ? 1 : (kind == K_PL ? 2 : 3);
+            if (is_ee && cls < 3) {
+                // mollifier active: x = |(p1 - p0) x (p3 - p2)|^2 < eps_x |p0 - p1|^2 |p2 - p3|^2
+                const double eps = a.kp.eps_x * sqn(p0 - p1) * sqn(p2 - p3);
+                const d3 c = cross(p1 - p0, p3 - p2);
+                if (dot(c, c) < eps) cls = 3;
+            }
         } else {
-            const int4 c = a.ee[cid - a.n_pt];
-            const int ea = a.eoff[c.x] + c.y, eb = a.eoff[c.z] + c.w;
-            id0 = a.edge[2 * ea]; id1 = a.edge[2 * ea + 1]; id2 = a.edge[2 * eb]; id3 = a.edge[2 * eb + 1];
-        }
-        G.p0 = ld3(a.X + 3 * (size_t)id0); G.p1 = ld3(a.X + 3 * (size_t)id1);
-        G.p2 = ld3(a.X + 3 * (size_t)id2); G.p3 = ld3(a.X + 3 * (size_t)id3);
-        if (!G.is_ee) { G.type = pt_type(G.p0, G.p1, G.p2, G.p3); G.d = pt_dist(G.p0, G.p1, G.p2, G.p3, G.type); }
-        else { G.type = ee_type(G.p0, G.p1, G.p2, G.p3); G.d = ee_dist(G.p0, G.p1, G.p2, G.p3, G.type); }
-        active = G.d < a.kp.d_hat;
-        if (active) {
-            geo_init(G, a.kp);
-            if (lane < 12) barrier_column(G, lane, L.H, gj);
+            a.rec[REC * (size_t)cid + R_D] = 0.0;
+            double* tk = a.tk + 24 * (size_t)cid;
+            for (int i = 0; i < 24; i++) tk[i] = 0.0;
+            a.lam[cid] = 0.0;
         }
     }
-    if (!active && lane < 12) {
-        for (int i = 0; i < 12; i++) L.H[i][lane] = 0.0;
-    }
-    __syncthreads();
-    if (a.kp.psd) group_psd(L, lane, valid && active);
-    // friction (model/friction.cpp:15-47) with displacement from the step start
-    double lamc = 0.0, gm0 = 0.0, gm1 = 0.0, gm2 = 0.0, gm3 = 0.0;
-    d3 c0 = mk(0, 0, 0), c1 = mk(0, 0, 0);
-    if (valid && active) {
-        friction_frame(G, gm0, gm1, gm2, gm3, c0, c1);
-        const d3 d0 = G.p0 - ld3(a.Xs + 3 * (size_t)id0), d1 = G.p1 - ld3(a.Xs + 3 * (size_t)id1);
-        const d3 d2 = G.p2 - ld3(a.Xs + 3 * (size_t)id2), d3v = G.p3 - ld3(a.Xs + 3 * (size_t)id3);
-        const double u0 = gm0 * dot(c0, d0) + gm1 * dot(c0, d1) + gm2 * dot(c0, d2) + gm3 * dot(c0, d3v);
-        const double u1 = gm0 * dot(c1, d0) + gm1 * dot(c1, d1) + gm2 * dot(c1, d2) + gm3 * dot(c1, d3v);
-        lamc = -G.B1 * 2 * sqrt(G.d);
-        const bool fr = G.is_ee ? a.kp.ee_fric : a.kp.pt_fric;
-        if (fr && lane < 12) {
-            double fs, M[3];
-            friction_M(u0, u1, lamc, a.kp, fs, M);
-            // T = Tk^T (12 x 2): T[(k, i)][r] = gam_k * c_r[i]
-            const int kj = lane / 3, ij = lane % 3;
-            const double gk = sel4(gm0, gm1, gm2, gm3, kj);
-            const double Tj0 = gk * cmp(c0, ij), Tj1 = gk * cmp(c1, ij);
-            gj += fs * (Tj0 * u0 + Tj1 * u1);
-            const double m0 = M[0] * Tj0 + M[1] * Tj1, m1 = M[1] * Tj0 + M[2] * Tj1;
-            const double gmv[4] = {gm0, gm1, gm2, gm3};
+    const int total_n = total;
+    const int p1 = wave_append(cls == 1, a.counts + 0);
+    const int p2 = wave_append(cls == 2, a.counts + 1);
+    const int p3 = wave_append(cls == 3, a.counts + 2);
+    if (cls == 1) a.list[p1] = cid;
+    else if (cls == 2) a.list[total_n - 1 - p2] = cid;
+    else if (cls == 3) a.list3[p3] = cid;
+}
+
+// Friction of one contact (model/friction.cpp:15-47) with the tangential
+// displacement measured from the step start.
+struct Fric {
+    double gm0, gm1, gm2, gm3;
+    d3 c0, c1;
+    double u0, u1, lam, fs, M[3];
+    bool on;
+};
+__device__ __forceinline__ void friction_eval(const WorldArgs& a, int is_ee, int type, double d, double B1,
+                                              d3 P0, d3 P1, d3 P2, d3 P3, int id0, int id1, int id2, int id3, Fric& f) {
+    friction_frame(is_ee, type, P0, P1, P2, P3, f.gm0, f.gm1, f.gm2, f.gm3, f.c0, f.c1);
+    const d3 d0 = P0 - ld3(a.Xs + 3 * (size_t)id0), d1 = P1 - ld3(a.Xs + 3 * (size_t)id1);
+    const d3 d2 = P2 - ld3(a.Xs + 3 * (size_t)id2), d3v = P3 - ld3(a.Xs + 3 * (size_t)id3);
+    f.u0 = f.gm0 * dot(f.c0, d0) + f.gm1 * dot(f.c0, d1) + f.gm2 * dot(f.c0, d2) + f.gm3 * dot(f.c0, d3v);
+    f.u1 = f.gm0 * dot(f.c1, d0) + f.gm1 * dot(f.c1, d1) + f.gm2 * dot(f.c1, d2) + f.gm3 * dot(f.c1, d3v);
+    f.lam = -B1 * 2 * sqrt(d);
+    f.on = is_ee ? a.kp.ee_fric : a.kp.pt_fric;
+    f.fs = 0.0; f.M[0] = 0.0; f.M[1] = 0.0; f.M[2] = 0.0;
+    if (f.on) friction_M(f.u0, f.u1, f.lam, a.kp, f.fs, f.M);
+}
+// F[i][j] = sum_{r, q} M_rq c_r[i] c_q[j]
+__device__ __forceinline__ double fric_F(const Fric& f, int i, int j) {
+    const double c0i = cmp(f.c0, i), c1i = cmp(f.c1, i), c0j = cmp(f.c0, j), c1j = cmp(f.c1, j);
+    return f.M[0] * c0i * c0j + f.M[1] * (c0i * c1j + c1i * c0j) + f.M[2] * c1i * c1j;
+}
+
+// Record fields shared by the per-thread and per-group paths.
+// Friction part: F, G, the lagged Tk (2 x 12, row-major) and lambda.
+__device__ __forceinline__ void write_fric(const WorldArgs& a, int cid, int nside0, const Fric& f, const double w[4][4]) {
+    double* R = a.rec + REC * (size_t)cid;
+    const double gmv[4] = {f.gm0, f.gm1, f.gm2, f.gm3};
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) R[R_F + 3 * i + j] = fric_F(f, i, j);
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+        for (int al = 0; al < 4; al++) {
+            double g = 0.0;
 #pragma unroll
             for (int k = 0; k < 4; k++)
-#pragma unroll
-                for (int i = 0; i < 3; i++) L.H[3 * k + i][lane] += gmv[k] * cmp(c0, i) * m0 + gmv[k] * cmp(c1, i) * m1;
+                if ((k < nside0) == (s == 0)) g += gmv[k] * w[k][al];
+            R[R_G + 4 * s + al] = g;
         }
-    }
-    if (lane < 12) L.g[lane] = gj;
-    __syncthreads();
-    if (valid && lane < 12) {
-        // J^T H J with J = w^T (x) I3, w = (1, xbar): block (alpha, beta) of the
-        // body-space matrix is sum_{a, b} w_a[alpha] w_b[beta] H_ab (Kronecker form)
-        const int beta = lane / 3, c = lane % 3;
-        const d3 xb0 = ld3(a.xbar + 3 * (size_t)id0), xb1 = ld3(a.xbar + 3 * (size_t)id1);
-        const d3 xb2 = ld3(a.xbar + 3 * (size_t)id2), xb3 = ld3(a.xbar + 3 * (size_t)id3);
-        const double wv[4][4] = {{1.0, xb0.x, xb0.y, xb0.z}, {1.0, xb1.x, xb1.y, xb1.z},
-                                 {1.0, xb2.x, xb2.y, xb2.z}, {1.0, xb3.x, xb3.y, xb3.z}};
-        const int nside0 = G.is_ee ? 2 : 1;      // points 0..nside0-1 belong to body I
-        double wb[4];
+    double* tk = a.tk + 24 * (size_t)cid;
 #pragma unroll
-        for (int b = 0; b < 4; b++) wb[b] = sel4(wv[b][0], wv[b][1], wv[b][2], wv[b][3], beta);
-        double* blk = a.blk + 432 * (size_t)cid;
-        // entry (alpha, r) of column (beta, c): hp = (I, I), ht = (J, J), off = (I, J)
+    for (int k = 0; k < 4; k++)
 #pragma unroll
-        for (int alpha = 0; alpha < 4; alpha++)
+        for (int r = 0; r < 3; r++) {
+            tk[3 * k + r] = gmv[k] * cmp(f.c0, r);
+            tk[12 + 3 * k + r] = gmv[k] * cmp(f.c1, r);
+        }
+    a.lam[cid] = f.lam;
+}
+// Body-space gradient from per-point vertex gradients gv_k (friction included).
+__device__ __forceinline__ void write_grad(const WorldArgs& a, int cid, int nside0, const double w[4][4], const d3 gv[4]) {
+    double* R = a.rec + REC * (size_t)cid;
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+        for (int al = 0; al < 4; al++)
 #pragma unroll
             for (int r = 0; r < 3; r++) {
-                double o0 = 0.0, o1 = 0.0, o2 = 0.0;
+                double g = 0.0;
 #pragma unroll
-                for (int b = 0; b < 4; b++) {
-                    double s0 = 0.0, s1 = 0.0;
-#pragma unroll
-                    for (int aa = 0; aa < 4; aa++) {
-                        const double t = wv[aa][alpha] * L.H[3 * aa + r][3 * b + c];
-                        if (aa < nside0) s0 += t; else s1 += t;
-                    }
-                    if (b < nside0) o0 += wb[b] * s0;
-                    else { o1 += wb[b] * s1; o2 += wb[b] * s0; }
-                }
-                blk[12 * lane + 3 * alpha + r] = active ? o0 : 0.0;
-                blk[144 + 12 * lane + 3 * alpha + r] = active ? o1 : 0.0;
-                blk[288 + 12 * lane + 3 * alpha + r] = active ? o2 : 0.0;
+                for (int k = 0; k < 4; k++)
+                    if ((k < nside0) == (s == 0)) g += w[k][al] * cmp(gv[k], r);
+                R[R_GRAD + 12 * s + 3 * al + r] = g;
             }
-        // gradients: dg[(alpha, r)] = sum_a w_a[alpha] g_a[r] over each side
-        const int alpha = lane / 3, r = lane % 3;
-        double gp = 0.0, gt = 0.0;
+}
+__device__ __forceinline__ void load_w(const double* xbar, int id0, int id1, int id2, int id3, double w[4][4]) {
+    const d3 x0 = ld3(xbar + 3 * (size_t)id0), x1 = ld3(xbar + 3 * (size_t)id1);
+    const d3 x2 = ld3(xbar + 3 * (size_t)id2), x3 = ld3(xbar + 3 * (size_t)id3);
+    const d3 xs[4] = {x0, x1, x2, x3};
 #pragma unroll
-        for (int aa = 0; aa < 4; aa++) {
-            const double t = sel4(wv[aa][0], wv[aa][1], wv[aa][2], wv[aa][3], alpha) * L.g[3 * aa + r];
-            if (aa < nside0) gp += t; else gt += t;
+    for (int k = 0; k < 4; k++) { w[k][0] = 1.0; w[k][1] = xs[k].x; w[k][2] = xs[k].y; w[k][3] = xs[k].z; }
+}
+
+// Cyclic Jacobi on a register-resident symmetric N x N matrix (all indices
+// static), eigenvectors accumulated in v.
+template <int N>
+__device__ __forceinline__ void jacobi_reg(double (&A)[N][N], double (&v)[N][N]) {
+#pragma unroll
+    for (int p = 0; p < N; p++)
+#pragma unroll
+        for (int q = 0; q < N; q++) v[p][q] = p == q ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < MAX_SWEEPS; sweep++) {
+        double off = 0.0, tot = 0.0;
+#pragma unroll
+        for (int p = 0; p < N; p++)
+#pragma unroll
+            for (int q = 0; q < N; q++) { const double x = A[p][q] * A[p][q]; tot += x; if (p != q) off += x; }
+        if (off <= 1e-30 * tot || off == 0.0) break;
+#pragma unroll
+        for (int p = 0; p < N - 1; p++)
+#pragma unroll
+            for (int q = p + 1; q < N; q++) {
+                const double apq = A[p][q];
+                const bool nz = apq != 0.0;
+                const double tau = (A[q][q] - A[p][p]) / (2.0 * (nz ? apq : 1.0));
+                double t = fabs(tau) > 1e150 ? 0.5 / tau : (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                t = nz ? t : 0.0;
+                const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+#pragma unroll
+                for (int k = 0; k < N; k++) {
+                    const double ap = A[k][p], aq = A[k][q];
+                    A[k][p] = c * ap - s * aq; A[k][q] = s * ap + c * aq;
+                }
+#pragma unroll
+                for (int k = 0; k < N; k++) {
+                    const double ap = A[p][k], aq = A[q][k];
+                    A[p][k] = c * ap - s * aq; A[q][k] = s * ap + c * aq;
+                }
+#pragma unroll
+                for (int k = 0; k < N; k++) {
+                    const double vp = v[k][p], vq = v[k][q];
+                    v[k][p] = c * vp - s * vq; v[k][q] = s * vp + c * vq;
+                }
+            }
+    }
+}
+
+// Point slot k of the contact as a selector over the four points.
+__device__ __forceinline__ d3 pick(d3 p0, d3 p1, d3 p2, d3 p3, int k) {
+    return k == 0 ? p0 : (k == 1 ? p1 : (k == 2 ? p2 : p3));
+}
+
+// Class 1 and 2 contacts, one thread each.
+//   class 1: d = |r|^2, r = x_a - x_b.  In the Helmert coordinate
+//     y = (x_a - x_b) / sqrt(2) the Hessian is K = 4 B1 I + 8 B2 r r^T with
+//     eigenvalues 4 B1 (twice, normal to r) and 4 B1 + 8 B2 d (along r).
+//   class 2: d = point-line distance of r0 = x_a - x_m, r1 = x_b - x_m;
+//     Helmert rows over the slots (m, a, b) give r = C^T y with
+//     C = [[-sqrt 2, -1 / sqrt 2], [0, -sqrt(3 / 2)]] and K = (C (x) I3) Mred
+//     (C^T (x) I3), Mred = B1 Hess(d) + B2 grad(d) grad(d)^T; 6x6 Jacobi.
+template <int CLS>
+__device__ __forceinline__ void contact_small(const WorldArgs& a, int cid) {
+    int id0, id1, id2, id3;
+    const int is_ee = contact_ids(a, cid, id0, id1, id2, id3);
+    const d3 P0 = ld3(a.X + 3 * (size_t)id0), P1 = ld3(a.X + 3 * (size_t)id1);
+    const d3 P2 = ld3(a.X + 3 * (size_t)id2), P3 = ld3(a.X + 3 * (size_t)id3);
+    const int type = a.ctype[cid];
+    const double d = a.cdist[cid];
+    const RedMap mp = is_ee ? ee_map(type) : pt_map(type);
+    const int nside0 = is_ee ? 2 : 1;
+    const double B1 = barrier_d1(d, a.kp.d_hat, a.kp.kappa);
+    const double B2 = barrier_d2(d, a.kp.d_hat, a.kp.kappa);
+    double* R = a.rec + REC * (size_t)cid;
+    const d3 z = mk(0, 0, 0);
+    // friction first: F, G, Tk, lambda written now; kept: gm_k and
+    // ft = fs (u0 c0 + u1 c1), the friction gradient of point k being gm_k ft
+    double gmv[4];
+    d3 ft = z;
+    {
+        Fric f;
+        friction_eval(a, is_ee, type, d, B1, P0, P1, P2, P3, id0, id1, id2, id3, f);
+        double w[4][4];
+        load_w(a.xbar, id0, id1, id2, id3, w);
+        write_fric(a, cid, nside0, f, w);
+        gmv[0] = f.gm0; gmv[1] = f.gm1; gmv[2] = f.gm2; gmv[3] = f.gm3;
+        if (f.on) {
+            const double t0 = f.fs * f.u0, t1 = f.fs * f.u1;
+            ft = mk(t0 * f.c0.x + t1 * f.c1.x, t0 * f.c0.y + t1 * f.c1.y, t0 * f.c0.z + t1 * f.c1.z);
         }
-        a.grad[24 * (size_t)cid + lane] = active ? gp : 0.0;
-        a.grad[24 * (size_t)cid + 12 + lane] = active ? gt : 0.0;
-        // lagged friction data: Tk (2 x 12, row-major) and lambda
-        const double gk = sel4(gm0, gm1, gm2, gm3, alpha);
-        a.tk[24 * (size_t)cid + lane] = gk * cmp(c0, r);
-        a.tk[24 * (size_t)cid + 12 + lane] = gk * cmp(c1, r);
-        if (lane == 0) a.lam[cid] = lamc;
+    }
+    d3 gv[4];
+    gv[0] = z; gv[1] = z; gv[2] = z; gv[3] = z;
+    double hz[4][2];      // Helmert weight of point slot k in row a (zero off the contact's slots)
+    if (CLS == 1) {
+        const int ka = mp.plus[0], kb = mp.minus[0];
+        const d3 r = pick(P0, P1, P2, P3, ka) - pick(P0, P1, P2, P3, kb);
+        const double lp = 4.0 * B1, ll = 4.0 * B1 + 8.0 * B2 * d;
+        if (a.kp.psd) {
+            const double cp = fmax(lp, 0.0), cl = fmax(ll, 0.0);
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const double rr = cmp(r, i) * cmp(r, j) / d;
+                    R[R_PK + 3 * i + j] = cl * rr + cp * ((i == j ? 1.0 : 0.0) - rr);
+                }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) R[R_PK + 3 * i + j] = (i == j ? lp : 0.0) + 8.0 * B2 * cmp(r, i) * cmp(r, j);
+        }
+        R[R_D] = 3.0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) { hz[k][0] = k == ka ? HS2 : (k == kb ? -HS2 : 0.0); hz[k][1] = 0.0; }
+        const d3 g = mk(2 * r.x * B1, 2 * r.y * B1, 2 * r.z * B1);
+#pragma unroll
+        for (int k = 0; k < 4; k++) gv[k] = k == ka ? g : (k == kb ? mk(-g.x, -g.y, -g.z) : z);
+    } else {
+        const int ka = mp.plus[0], kb = mp.plus[1], km = mp.minus[0];
+        const d3 pm = pick(P0, P1, P2, P3, km);
+        const d3 r0 = pick(P0, P1, P2, P3, ka) - pm, r1 = pick(P0, P1, P2, P3, kb) - pm;
+        RedDist rd;
+        rd.init(K_PL, r0, r1, z);
+        double K[6][6];
+        {
+            double M[6][6];
+#pragma unroll
+            for (int u = 0; u < 6; u++)
+#pragma unroll
+                for (int v = u; v < 6; v++) {
+                    const int m = u / 3, i = u % 3, n = v / 3, j = v % 3;
+                    M[u][v] = B1 * rd.hess(m, i, n, j) + B2 * cmp(rd.g[m], i) * cmp(rd.g[n], j);
+                    M[v][u] = M[u][v];
+                }
+            const double C[2][2] = {{-2.0 * HS2, -HS2}, {0.0, -3.0 * HS6}};
+#pragma unroll
+            for (int aa = 0; aa < 2; aa++)
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int bb = 0; bb < 2; bb++)
+#pragma unroll
+                        for (int j = 0; j < 3; j++) {
+                            double s = 0.0;
+#pragma unroll
+                            for (int m = 0; m < 2; m++)
+#pragma unroll
+                                for (int n = 0; n < 2; n++) s += C[aa][m] * C[bb][n] * M[3 * m + i][3 * n + j];
+                            K[3 * aa + i][3 * bb + j] = s;
+                        }
+        }
+        const d3 g0 = mk(B1 * rd.g[0].x, B1 * rd.g[0].y, B1 * rd.g[0].z);
+        const d3 g1 = mk(B1 * rd.g[1].x, B1 * rd.g[1].y, B1 * rd.g[1].z);
+        if (a.kp.psd) {
+            double V[6][6];
+            jacobi_reg<6>(K, V);
+            double wv[6];
+#pragma unroll
+            for (int k = 0; k < 6; k++) wv[k] = fmax(K[k][k], 0.0);
+#pragma unroll
+            for (int p = 0; p < 6; p++)
+#pragma unroll
+                for (int q = p; q < 6; q++) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 6; k++) s += V[p][k] * wv[k] * V[q][k];
+                    R[R_PK + 6 * p + q] = s;
+                    R[R_PK + 6 * q + p] = s;
+                }
+        } else {
+#pragma unroll
+            for (int p = 0; p < 6; p++)
+#pragma unroll
+                for (int q = 0; q < 6; q++) R[R_PK + 6 * p + q] = K[p][q];
+        }
+        R[R_D] = 6.0;
+        // Helmert rows over the slots (m, a, b): (1, -1, 0) / sqrt 2, (1, 1, -2) / sqrt 6
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            hz[k][0] = k == km ? HS2 : (k == ka ? -HS2 : 0.0);
+            hz[k][1] = k == km ? HS6 : (k == ka ? HS6 : (k == kb ? -2.0 * HS6 : 0.0));
+        }
+        const d3 gm = mk(-(g0.x + g1.x), -(g0.y + g1.y), -(g0.z + g1.z));
+#pragma unroll
+        for (int k = 0; k < 4; k++) gv[k] = k == ka ? g0 : (k == kb ? g1 : (k == km ? gm : z));
+    }
+    double w[4][4];
+    load_w(a.xbar, id0, id1, id2, id3, w);
+    constexpr int NA = CLS == 1 ? 1 : 2;
+#pragma unroll
+    for (int aa = 0; aa < NA; aa++)
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+            for (int al = 0; al < 4; al++) {
+                double zz = 0.0;
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if ((k < nside0) == (s == 0)) zz += hz[k][aa] * w[k][al];
+                R[R_Z + 8 * aa + 4 * s + al] = zz;
+            }
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        gv[k] = mk(gv[k].x + gmv[k] * ft.x, gv[k].y + gmv[k] * ft.y, gv[k].z + gmv[k] * ft.z);
+    write_grad(a, cid, nside0, w, gv);
+}
+// grid-strided over the class list: class 1 at list[0, n1), class 2 at
+// list[total - n2, total) (stored from the back)
+template <int CLS>
+__global__ __launch_bounds__(256) void k_contact_small(WorldArgs a) {
+    const int total = a.n_pt + a.n_ee;
+    const int cnt = a.counts[CLS - 1];
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < cnt; p += gridDim.x * blockDim.x)
+        contact_small<CLS>(a, a.list[CLS == 1 ? p : total - 1 - p]);
+}
+
+// Class 3 contacts: one 16-lane group each (vertex-space barrier Hessian in
+// LDS, Helmert projection to 9x9, parallel Jacobi), grid-strided over the
+// class-3 list.
+__global__ __launch_bounds__(64) void k_contact_group(WorldArgs a) {
+    __shared__ GroupLDS lds[CPW];
+    const int grp = threadIdx.x / GL, lane = threadIdx.x % GL;
+    GroupLDS& L = lds[grp];
+    const int n3 = a.counts[2];
+    for (int base = blockIdx.x * CPW; base < n3; base += gridDim.x * CPW) {
+        const int pos = base + grp;
+        const bool valid = pos < n3;
+        const int cid = valid ? a.list3[pos] : 0;
+        Geo G;
+        int id0 = 0, id1 = 0, id2 = 0, id3 = 0;
+        double gj = 0.0;
+        if (valid) {
+            G.is_ee = contact_ids(a, cid, id0, id1, id2, id3);
+            G.p0 = ld3(a.X + 3 * (size_t)id0); G.p1 = ld3(a.X + 3 * (size_t)id1);
+            G.p2 = ld3(a.X + 3 * (size_t)id2); G.p3 = ld3(a.X + 3 * (size_t)id3);
+            G.type = a.ctype[cid]; G.d = a.cdist[cid];
+            geo_init(G, a.kp);
+            if (lane < 12) barrier_column(G, lane, L.H, gj);
+        } else if (lane < 12) {
+            for (int i = 0; i < 12; i++) L.H[i][lane] = 0.0;
+        }
+        __syncthreads();
+        group_psd<true>(L, lane, valid && a.kp.psd);
+        Fric f;
+        double w[4][4];
+        const int nside0 = G.is_ee ? 2 : 1;
+        if (valid) {
+            friction_eval(a, G.is_ee, G.type, G.d, G.B1, G.p0, G.p1, G.p2, G.p3, id0, id1, id2, id3, f);
+            if (f.on && lane < 12) {
+                const int kj = lane / 3, ij = lane % 3;
+                const double gk = sel4(f.gm0, f.gm1, f.gm2, f.gm3, kj);
+                gj += f.fs * (gk * cmp(f.c0, ij) * f.u0 + gk * cmp(f.c1, ij) * f.u1);
+            }
+            if (lane < 12) L.g[lane] = gj;
+            double* R = a.rec + REC * (size_t)cid;
+            if (lane < 9) {
+#pragma unroll
+                for (int r = 0; r < 9; r++) R[R_PK + 9 * r + lane] = L.A[r][lane];
+            }
+            if (lane == 0) R[R_D] = 9.0;
+            load_w(a.xbar, id0, id1, id2, id3, w);
+            // Z[a][s][alpha] = sum_{k on side s} h4[a][k] w_k[alpha]: lane = (a, alpha) < 12 writes both sides
+            if (lane < 12) {
+                const int aa = lane / 4, al = lane % 4;
+                double z0 = 0.0, z1 = 0.0;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const double t = kHel[aa][k] * sel4(w[k][0], w[k][1], w[k][2], w[k][3], al);
+                    if (k < nside0) z0 += t; else z1 += t;
+                }
+                R[R_Z + 8 * aa + al] = z0;
+                R[R_Z + 8 * aa + 4 + al] = z1;
+            }
+        }
+        __syncthreads();
+        if (valid && lane == 0) {
+            d3 gv[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) gv[k] = mk(L.g[3 * k], L.g[3 * k + 1], L.g[3 * k + 2]);
+            write_fric(a, cid, nside0, f, w);
+            write_grad(a, cid, nside0, w, gv);
+        }
+        __syncthreads();
     }
 }
 
@@ -619,10 +939,13 @@ static KParams make_kp(const adb_params& P) {
 
 void contact_terms(World& w) {
     const int n = w.cs.n_pt + w.cs.n_ee;
-    w.cblk.reserve(432 * (size_t)std::max(n, 1));
-    w.cgrad.reserve(24 * (size_t)std::max(n, 1));
-    w.ctk.reserve(24 * (size_t)std::max(n, 1));
-    w.clam.reserve(std::max(n, 1));
+    const size_t nn = (size_t)std::max(n, 1);
+    w.crec.reserve(REC * nn);
+    w.ctype.reserve(nn); w.cdist.reserve(nn); w.clist.reserve(nn); w.clist3.reserve(nn);
+    w.ccount.reserve(4);
+    w.ctk.reserve(24 * nn);
+    w.clam.reserve(nn);
+    ADB_HIP(hipMemsetAsync(w.ccount.p, 0, 4 * sizeof(int), w.stream));
     if (n > 0) {
         WorldArgs a;
         a.kp = make_kp(w.P);
@@ -630,12 +953,33 @@ void contact_terms(World& w) {
         a.pt = w.cs.pt.p; a.ee = w.cs.ee.p;
         a.voff = w.voff.p; a.foff = w.foff.p; a.eoff = w.eoff.p; a.face = w.face.p; a.edge = w.edge.p;
         a.X = w.X.p; a.Xs = w.Xs.p; a.xbar = w.xbar.p;
-        a.blk = w.cblk.p; a.grad = w.cgrad.p; a.tk = w.ctk.p; a.lam = w.clam.p;
-        const int blocks = (n + CPW - 1) / CPW;
-        // per contact: 3 x 4 points of X, Xs, xbar + ids in; 3 blocks, 2 grads, Tk, lambda out
-        ProfScope ps(w, PK_CONTACT, (double)n * (288.0 + 16.0 + 3456.0 + 192.0 + 192.0 + 8.0));
-        hipLaunchKernelGGL(k_contact_world, dim3(blocks), dim3(64), 0, w.stream, a);
-        ADB_HIP(hipGetLastError());
+        a.ctype = w.ctype.p; a.cdist = w.cdist.p; a.list = w.clist.p; a.list3 = w.clist3.p; a.counts = w.ccount.p;
+        a.rec = w.crec.p; a.tk = w.ctk.p; a.lam = w.clam.p;
+        {
+            // ids and 4 points in; type, distance and list slot out
+            ProfScope ps(w, PK_CONTACT_CLASSIFY, (double)n * 152.0);
+            hipLaunchKernelGGL(k_contact_classify, dim3((n + 255) / 256), dim3(256), 0, w.stream, a);
+            ADB_HIP(hipGetLastError());
+        }
+        const int* snap = prof_snapshot(w, w.ccount.p);
+        {
+            // per contact: list, ids, type, distance, X / Xs / xbar of 4 points in;
+            // record (D = 3: 59 doubles, D = 6: 94), Tk and lambda out
+            ProfScope ps(w, PK_CONTACT, 0.0);
+            ps.counts(snap, 1016.0, 1296.0, 0.0, 0.0);
+            const int grid = std::min((n + 255) / 256, 8 * w.num_cu);
+            hipLaunchKernelGGL(k_contact_small<1>, dim3(grid), dim3(256), 0, w.stream, a);
+            ADB_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_contact_small<2>, dim3(grid), dim3(256), 0, w.stream, a);
+            ADB_HIP(hipGetLastError());
+        }
+        {
+            ProfScope ps(w, PK_CONTACT_GROUP, 0.0);
+            ps.counts(snap, 0.0, 0.0, 1720.0, 0.0);
+            const int grid = std::min((n + CPW - 1) / CPW, 2048);
+            hipLaunchKernelGGL(k_contact_group, dim3(grid), dim3(64), 0, w.stream, a);
+            ADB_HIP(hipGetLastError());
+        }
     }
     const int nv = w.cs.n_vg;
     w.vgK.reserve(12 * (size_t)std::max(nv, 1));

@@ -39,17 +39,34 @@ template <class T> struct DBuf {
     void free_() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
 };
 
+// Compact body-space record of one PT/EE contact (doubles).  With D the
+// dimension of the contact's translation-free point space (3, 6 or 9; 0 when
+// the contact is inactive), n = D / 3 Helmert rows, sides s = 0 (body I) and
+// 1 (body J), and block coordinates (alpha, r) = (affine column, component):
+//   block(s1, s2)[(alpha, r), (beta, c)] =
+//       sum_{a, b < n} Z[a][s1][alpha] Z[b][s2][beta] PK[(3a + r) * D + 3b + c]
+//     + G[s1][alpha] G[s2][beta] F[3r + c]
+//   gradient(s)[(alpha, r)] = GRAD[12 s + 3 alpha + r]
+// PK is the PSD-projected barrier Hessian in Helmert coordinates, F the
+// friction matrix and Z, G the per-side weights of the affine map.
+constexpr int REC = 160;
+constexpr int R_D = 0, R_F = 1, R_G = 10, R_Z = 18, R_GRAD = 42, R_PK = 66;
+
 // Per-kernel HIP-event timing on the world's stream (off unless enabled).
 enum ProfKernel {
     PK_TRANSFORM = 0, PK_BODY_BOX, PK_BODY_PAIRS, PK_NARROW_SET, PK_VG_SCAN, PK_SORT, PK_NARROW_CCD, PK_VG_TOI,
     PK_CONTACT, PK_VG_CONTACT, PK_INERTIA, PK_ASM_REDUCE, PK_BLOCK_CHOL, PK_SPMV, PK_PCG_UPDATE, PK_PCG_DIR,
-    PK_ENERGY, PK_COUNT
+    PK_ENERGY, PK_CONTACT_CLASSIFY, PK_CONTACT_GROUP, PK_COUNT
 };
 struct Prof {
     bool on = false;
-    struct Rec { int k; hipEvent_t a, b; double bytes; };
+    // bytes of a launch = bytes + sum_i per[i] * snap[i], where snap points to
+    // a pinned host copy of device counters taken on the stream before the launch
+    struct Rec { int k; hipEvent_t a, b; double bytes; const int* snap; double per[4]; };
     std::vector<Rec> pending;
     std::vector<hipEvent_t> pool;
+    int* snap = nullptr;             // pinned host ring of counter snapshots
+    int snap_cap = 0, snap_used = 0;
     double ms[PK_COUNT] = {0};
     double bytes[PK_COUNT] = {0};
     long long count[PK_COUNT] = {0};
@@ -86,8 +103,12 @@ struct World {
     DBuf<unsigned long long> vg_keys;
     ContactSet cs, cs_new;
     // ---- contact terms (per PT/EE contact, then per VG contact)
-    DBuf<double> cblk;               // 432 per PT/EE contact: hp, ht, off (column-major)
-    DBuf<double> cgrad;              // 24 per PT/EE contact: gp, gt
+    DBuf<double> crec;               // REC per PT/EE contact: compact body-space record
+    DBuf<int> ctype;                 // distance type per contact
+    DBuf<double> cdist;              // squared distance per contact
+    DBuf<int> clist;                 // class 1 from the front, class 2 from the back
+    DBuf<int> clist3;                // class 3 contacts
+    DBuf<int> ccount;                // 4 counters: class 1, 2, 3 sizes
     DBuf<double> ctk;                // 24 per PT/EE contact: Tk (2x12 row-major)
     DBuf<double> clam;               // 1 per PT/EE contact
     DBuf<double> vgK;                // 12 per VG contact: K (3x3 col-major) + gx (3)
@@ -121,6 +142,8 @@ struct World {
 // launch's algorithmic HBM traffic.
 struct ProfScope {
     World& w; int k; double bytes; hipEvent_t a = nullptr;
+    const int* snap = nullptr; double per[4] = {0, 0, 0, 0};
+    void counts(const int* s, double p0, double p1, double p2, double p3) { snap = s; per[0] = p0; per[1] = p1; per[2] = p2; per[3] = p3; }
     ProfScope(World& w_, int k_, double bytes_) : w(w_), k(k_), bytes(bytes_) {
         if (!w.prof.on) return;
         a = get();
@@ -130,7 +153,7 @@ struct ProfScope {
         if (!w.prof.on || !a) return;
         hipEvent_t b = get();
         (void)hipEventRecord(b, w.stream);
-        w.prof.pending.push_back({k, a, b, bytes});
+        w.prof.pending.push_back({k, a, b, bytes, snap, {per[0], per[1], per[2], per[3]}});
     }
     hipEvent_t get() {
         if (!w.prof.pool.empty()) { hipEvent_t e = w.prof.pool.back(); w.prof.pool.pop_back(); return e; }
@@ -138,6 +161,9 @@ struct ProfScope {
     }
 };
 void prof_flush(World& w);
+// Pinned host copy of 4 device counters (enqueued on the world stream) when
+// profiling is on, else nullptr.
+const int* prof_snapshot(World& w, const int* dev_counts);
 
 // ---- stage launchers (implemented in the .hip files)
 void transform_positions(World& w, const double* q, double* X);                         // cull.hip

@@ -252,15 +252,22 @@ extern "C" int adb_world_create(const adb_scene* s, const adb_params* p, adb_wor
 
 static void free_world(World& w) {
     DBuf<int>* ib[] = {&w.voff, &w.foff, &w.eoff, &w.vbody, &w.fbody, &w.ebody, &w.face, &w.edge, &w.counters,
-                       &w.aval, &w.aval_s, &w.blk_of, &w.uniq_flag, &w.blk_start, &w.row_ptr, &w.col_idx, &w.diag_blk, &w.vg_start};
+                       &w.aval, &w.aval_s, &w.blk_of, &w.uniq_flag, &w.blk_start, &w.row_ptr, &w.col_idx, &w.diag_blk, &w.vg_start,
+                       &w.ctype, &w.clist, &w.clist3, &w.ccount, &w.chunk_n, &w.chunk_off, &w.chunk_blk};
     for (auto* b : ib) b->free_();
     DBuf<double>* db[] = {&w.xbar, &w.mass, &w.Ic, &w.q, &w.q0, &w.qdot, &w.dq, &w.qtrial, &w.X, &w.Xs, &w.X1, &w.bbox,
-                          &w.cblk, &w.cgrad, &w.ctk, &w.clam, &w.vgK, &w.vglam, &w.bodyH, &w.bodyG, &w.Hval, &w.grad,
+                          &w.crec, &w.cdist, &w.chunk_part, &w.ctk, &w.clam, &w.vgK, &w.vglam, &w.bodyH, &w.bodyG, &w.Hval, &w.grad,
                           &w.Lchol, &w.pcg_x, &w.pcg_r, &w.pcg_z, &w.pcg_p, &w.pcg_Ap, &w.partial, &w.scal, &w.epart};
     for (auto* b : db) b->free_();
     w.pairs.free_(); w.ovf.free_(); w.keys_a.free_(); w.keys_b.free_(); w.sort_tmp.free_(); w.toi_bits.free_(); w.vg_keys.free_();
     w.akeys.free_(); w.akeys_s.free_();
     for (ContactSet* c : {&w.cs, &w.cs_new}) { c->pt.free_(); c->ee.free_(); c->vg.free_(); }
+    for (auto& r : w.prof.pending) { w.prof.pool.push_back(r.a); w.prof.pool.push_back(r.b); }
+    w.prof.pending.clear();
+    for (hipEvent_t e : w.prof.pool) (void)hipEventDestroy(e);
+    w.prof.pool.clear();
+    if (w.prof.snap) (void)hipHostFree(w.prof.snap);
+    w.prof.snap = nullptr;
     if (w.stream) (void)hipStreamDestroy(w.stream);
     w.stream = nullptr;
 }
@@ -313,12 +320,27 @@ void prof_flush(World& w) {
         float ms = 0.0f;
         ADB_HIP(hipEventElapsedTime(&ms, r.a, r.b));
         w.prof.ms[r.k] += ms;
-        w.prof.bytes[r.k] += r.bytes;
+        double bytes = r.bytes;
+        if (r.snap) for (int i = 0; i < 4; i++) bytes += r.per[i] * (double)r.snap[i];
+        w.prof.bytes[r.k] += bytes;
         w.prof.count[r.k] += 1;
         w.prof.pool.push_back(r.a);
         w.prof.pool.push_back(r.b);
     }
     w.prof.pending.clear();
+    w.prof.snap_used = 0;
+}
+const int* prof_snapshot(World& w, const int* dev_counts) {
+    if (!w.prof.on) return nullptr;
+    if (!w.prof.snap) {
+        ADB_HIP(hipHostMalloc((void**)&w.prof.snap, 4 * 8192 * sizeof(int), hipHostMallocDefault));
+        w.prof.snap_cap = 4 * 8192;
+    }
+    if (w.prof.snap_used + 4 > w.prof.snap_cap) prof_flush(w);
+    int* dst = w.prof.snap + w.prof.snap_used;
+    w.prof.snap_used += 4;
+    ADB_HIP(hipMemcpyAsync(dst, dev_counts, 4 * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+    return dst;
 }
 } // namespace adb
 
