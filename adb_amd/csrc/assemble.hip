@@ -122,87 +122,116 @@ struct ReduceArgs {
     const int2* vg; const int* vg_start; const double* vgK; const int* voff; const double* xbar;
     double* Hval; double* grad;
 };
-// Pass 1: one 64-lane workgroup per chunk of at most RCH contributions of one
-// block (long segments - the diagonal block of a body with thousands of
-// contacts - are split so every workgroup has bounded work); lane t sums
-// entries t, t + 64, t + 128 (< 144) into the chunk's partial block.
-constexpr int RCH = 16;
+// Pass 1: one 256-lane workgroup per chunk of at most RCH contributions of
+// one block (long segments - the diagonal block of a body with thousands of
+// contacts - are split so every workgroup has bounded work).  Wave w sums
+// contributions w, w + 4, ... of the chunk (lane t: entries t, t + 64,
+// t + 128 < 144), then the four wave sums are combined in a fixed order.
+constexpr int RCH = 64;
+constexpr int RW = 4;            // waves per reduction workgroup
 __global__ void k_chunk_count(int nnzb, const int* __restrict__ blk_start, int* __restrict__ nch) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < nnzb) nch[k] = (blk_start[k + 1] - blk_start[k] + RCH - 1) / RCH;
 }
 // chunk_off = exclusive scan of nch (nnzb + 1 entries)
-__global__ void k_chunk_owner(int nnzb, const int* __restrict__ chunk_off, int* __restrict__ chunk_blk) {
+// (also stores the chunk count in counts[3] for the byte accounting)
+__global__ void k_chunk_owner(int nnzb, const int* __restrict__ chunk_off, int* __restrict__ chunk_blk, int* __restrict__ counts) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < nnzb) for (int c = chunk_off[k]; c < chunk_off[k + 1]; c++) chunk_blk[c] = k;
+    if (k == 0) counts[3] = chunk_off[nnzb];
 }
-__global__ __launch_bounds__(64) void k_reduce_chunks(ReduceArgs a, int nchunks, const int* __restrict__ chunk_off,
-                                                      const int* __restrict__ chunk_blk, double* __restrict__ part) {
+// 144 block entries + 12 gradient entries of one wave, lane t owning t, t + 64, t + 128
+struct Acc156 { double e[3]; double g; };
+__device__ __forceinline__ void add_contribution(const ReduceArgs& a, int v, int t, Acc156& A) {
+    if (v < 0) {     // diagonal marker: inertia terms
+        const int b = -1 - v;
+        for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) A.e[u] += a.bodyH[144 * (size_t)b + e]; }
+        if (t < 12) A.g += a.bodyG[12 * (size_t)b + t];
+        return;
+    }
+    const int c = v / 4, slot = v % 4;
+    const int4 ct = c < a.n_pt ? a.pt[c] : a.ee[c - a.n_pt];
+    const bool di = a.mass[ct.x] > 0.0, dj = a.mass[ct.z] > 0.0;
+    const bool use = slot == 0 ? di : (slot == 1 ? dj : (di && dj));
+    const double* R = a.rec + REC * (size_t)c;
+    const int D = use ? (int)R[R_D] : 0;
+    if (D == 0) return;
+    // block (s1, s2): slot 0 = (I, I), 1 = (J, J), 2 = (I, J), 3 = (J, I)
+    const int s1 = (slot == 1 || slot == 3) ? 1 : 0, s2 = (slot == 1 || slot == 2) ? 1 : 0;
+    const int nA = D / 3;
+    for (int u = 0; u < 3; u++) {
+        const int e = t + 64 * u;
+        if (e < 144) {
+            // column (beta, cc), row (alpha, r)
+            const int col = e / 12, row = e % 12;
+            const int be = col / 3, cc = col % 3, al = row / 3, r = row % 3;
+            double val = R[R_G + 4 * s1 + al] * R[R_G + 4 * s2 + be] * R[R_F + 3 * r + cc];
+            for (int aa = 0; aa < nA; aa++) {
+                const double za = R[R_Z + 8 * aa + 4 * s1 + al];
+                double sb = 0.0;
+                for (int bb = 0; bb < nA; bb++)
+                    sb += R[R_Z + 8 * bb + 4 * s2 + be] * R[R_PK + (3 * aa + r) * D + 3 * bb + cc];
+                val += za * sb;
+            }
+            A.e[u] += val;
+        }
+    }
+    if (slot < 2 && t < 12) A.g += R[R_GRAD + 12 * s1 + t];
+}
+// fixed-order combine of the RW wave accumulators into wave 0
+__device__ __forceinline__ void combine_waves(Acc156& A, double (*sh)[156]) {
+    const int w = threadIdx.x / 64, t = threadIdx.x % 64;
+    if (w > 0) {
+        for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) sh[w - 1][e] = A.e[u]; }
+        if (t < 12) sh[w - 1][144 + t] = A.g;
+    }
+    __syncthreads();
+    if (w == 0) {
+        for (int q = 0; q < RW - 1; q++) {
+            for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) A.e[u] += sh[q][e]; }
+            if (t < 12) A.g += sh[q][144 + t];
+        }
+    }
+}
+__global__ __launch_bounds__(64 * RW) void k_reduce_chunks(ReduceArgs a, const int* __restrict__ chunk_off,
+                                                           const int* __restrict__ chunk_blk, double* __restrict__ part) {
+    __shared__ double sh[RW - 1][156];
     const int ch = blockIdx.x;
-    if (ch >= nchunks) return;
+    if (ch >= chunk_off[a.nnzb]) return;
     const int k = chunk_blk[ch];
     const int beg = a.blk_start[k] + (ch - chunk_off[k]) * RCH;
     const int end = min(beg + RCH, a.blk_start[k + 1]);
-    const int t = threadIdx.x;
-    double acc[3] = {0.0, 0.0, 0.0};
-    double gacc = 0.0;
-    for (int s = beg; s < end; s++) {
-        const int v = a.vals[s];
-        if (v < 0) {     // diagonal marker: inertia terms
-            const int b = -1 - v;
-            for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) acc[u] += a.bodyH[144 * (size_t)b + e]; }
-            if (t < 12) gacc += a.bodyG[12 * (size_t)b + t];
-            continue;
-        }
-        const int c = v / 4, slot = v % 4;
-        const int4 ct = c < a.n_pt ? a.pt[c] : a.ee[c - a.n_pt];
-        const bool di = a.mass[ct.x] > 0.0, dj = a.mass[ct.z] > 0.0;
-        const bool use = slot == 0 ? di : (slot == 1 ? dj : (di && dj));
-        const double* R = a.rec + REC * (size_t)c;
-        const int D = use ? (int)R[R_D] : 0;
-        if (D == 0) continue;
-        // block (s1, s2): slot 0 = (I, I), 1 = (J, J), 2 = (I, J), 3 = (J, I)
-        const int s1 = (slot == 1 || slot == 3) ? 1 : 0, s2 = (slot == 1 || slot == 2) ? 1 : 0;
-        const int nA = D / 3;
-        for (int u = 0; u < 3; u++) {
-            const int e = t + 64 * u;
-            if (e < 144) {
-                // column (beta, cc), row (alpha, r)
-                const int col = e / 12, row = e % 12;
-                const int be = col / 3, cc = col % 3, al = row / 3, r = row % 3;
-                double val = R[R_G + 4 * s1 + al] * R[R_G + 4 * s2 + be] * R[R_F + 3 * r + cc];
-                for (int aa = 0; aa < nA; aa++) {
-                    const double za = R[R_Z + 8 * aa + 4 * s1 + al];
-                    double sb = 0.0;
-                    for (int bb = 0; bb < nA; bb++)
-                        sb += R[R_Z + 8 * bb + 4 * s2 + be] * R[R_PK + (3 * aa + r) * D + 3 * bb + cc];
-                    val += za * sb;
-                }
-                acc[u] += val;
-            }
-        }
-        if (slot < 2 && t < 12) gacc += R[R_GRAD + 12 * s1 + t];
+    const int w = threadIdx.x / 64, t = threadIdx.x % 64;
+    Acc156 A = {{0.0, 0.0, 0.0}, 0.0};
+    for (int s = beg + w; s < end; s += RW) add_contribution(a, a.vals[s], t, A);
+    combine_waves(A, sh);
+    if (w == 0) {
+        double* P = part + 156 * (size_t)ch;
+        for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) P[e] = A.e[u]; }
+        if (t < 12) P[144 + t] = A.g;
     }
-    double* P = part + 156 * (size_t)ch;
-    for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) P[e] = acc[u]; }
-    if (t < 12) P[144 + t] = gacc;
 }
 
-// Pass 2: one workgroup per block sums its chunk partials in order, then the
-// body's vertex-ground terms for diagonal blocks.
-__global__ __launch_bounds__(64) void k_reduce_final(ReduceArgs a, const int* __restrict__ chunk_off, const double* __restrict__ part) {
+// Pass 2: one workgroup per block sums its chunk partials (wave w: chunks
+// w, w + 4, ..., then a fixed-order combine), then the body's vertex-ground
+// terms for diagonal blocks.
+__global__ __launch_bounds__(64 * RW) void k_reduce_final(ReduceArgs a, const int* __restrict__ chunk_off, const double* __restrict__ part) {
+    __shared__ double sh[RW - 1][156];
     const int k = blockIdx.x;
     if (k >= a.nnzb) return;
     const unsigned long long key = a.keys[a.blk_start[k]];
     const int row = (int)(key >> 32), col = (int)(key & 0xffffffffull);
-    const int t = threadIdx.x;
-    double acc[3] = {0.0, 0.0, 0.0};
-    double gacc = 0.0;
-    for (int c = chunk_off[k]; c < chunk_off[k + 1]; c++) {
+    const int w = threadIdx.x / 64, t = threadIdx.x % 64;
+    Acc156 A = {{0.0, 0.0, 0.0}, 0.0};
+    for (int c = chunk_off[k] + w; c < chunk_off[k + 1]; c += RW) {
         const double* P = part + 156 * (size_t)c;
-        for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) acc[u] += P[e]; }
-        if (t < 12) gacc += P[144 + t];
+        for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) A.e[u] += P[e]; }
+        if (t < 12) A.g += P[144 + t];
     }
+    combine_waves(A, sh);
+    if (w != 0) return;
+    double acc[3] = {A.e[0], A.e[1], A.e[2]};
+    double gacc = A.g;
     if (row == col && a.mass[row] > 0.0) {
         // vertex-ground terms of this body: (w w^T) (x) K, gradient w (x) gx
         for (int s = a.vg_start[row]; s < a.vg_start[row + 1]; s++) {
@@ -223,15 +252,19 @@ __global__ __launch_bounds__(64) void k_reduce_final(ReduceArgs a, const int* __
     if (row == col && t < 12) a.grad[12 * (size_t)row + t] = gacc;
 }
 
-// block-Jacobi factors: Cholesky of each diagonal block (one thread per body,
-// matrix staged in LDS); falls back to the diagonal when not positive definite
+// block-Jacobi preconditioner: the explicit inverse of each diagonal block,
+// M^-1 = W^T W with W = L^-1 from the Cholesky factor L (symmetric by
+// construction).  One thread per body, matrices staged in LDS; falls back to
+// the inverse diagonal when the block is not positive definite.
 constexpr int CH_T = 32;
-__global__ __launch_bounds__(CH_T) void k_block_chol(int B, const int* __restrict__ diag_blk, const double* __restrict__ Hval,
-                                                    double* __restrict__ L) {
+__global__ __launch_bounds__(CH_T) void k_block_inv(int B, const int* __restrict__ diag_blk, const double* __restrict__ Hval,
+                                                   double* __restrict__ Minv) {
     __shared__ double S[CH_T][144];
+    __shared__ double T[CH_T][144];
     const int b = blockIdx.x * CH_T + threadIdx.x;
     if (b >= B) return;
-    double* A = S[threadIdx.x];
+    double* A = S[threadIdx.x];   // A[c * 12 + r]: column-major; L(r, c) for r >= c after factorisation
+    double* W = T[threadIdx.x];   // W[c * 12 + r] = (L^-1)(r, c), r >= c
     const double* D = Hval + 144 * (size_t)diag_blk[b];
     for (int i = 0; i < 144; i++) A[i] = D[i];
     bool ok = true;
@@ -247,12 +280,26 @@ __global__ __launch_bounds__(CH_T) void k_block_chol(int B, const int* __restric
             A[j * 12 + i] = t / l;
         }
     }
-    double* out = L + 144 * (size_t)b;
+    double* out = Minv + 144 * (size_t)b;
     if (ok) {
-        for (int c = 0; c < 12; c++) for (int r = 0; r < 12; r++) out[c * 12 + r] = r >= c ? A[c * 12 + r] : 0.0;
+        for (int c = 0; c < 12; c++) {
+            W[c * 12 + c] = 1.0 / A[c * 12 + c];
+            for (int r = c + 1; r < 12; r++) {
+                double s = 0.0;
+                for (int k = c; k < r; k++) s += A[k * 12 + r] * W[c * 12 + k];
+                W[c * 12 + r] = -s / A[r * 12 + r];
+            }
+        }
+        for (int c = 0; c < 12; c++)
+            for (int r = c; r < 12; r++) {
+                double s = 0.0;
+                for (int k = r; k < 12; k++) s += W[r * 12 + k] * W[c * 12 + k];
+                out[c * 12 + r] = s;
+                out[r * 12 + c] = s;
+            }
     } else {
         for (int i = 0; i < 144; i++) out[i] = 0.0;
-        for (int i = 0; i < 12; i++) { double d = D[13 * i]; out[13 * i] = sqrt(d > 0.0 ? d : 1.0); }
+        for (int i = 0; i < 12; i++) { double d = D[13 * i]; out[13 * i] = 1.0 / (d > 0.0 ? d : 1.0); }
     }
 }
 
@@ -322,27 +369,28 @@ void assemble_system(World& w) {
         ADB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, w.chunk_n.p, w.chunk_off.p, nnzb + 1, w.stream));
         w.sort_tmp.reserve(tmp + 16);
         ADB_HIP(hipcub::DeviceScan::ExclusiveSum(w.sort_tmp.p, tmp, w.chunk_n.p, w.chunk_off.p, nnzb + 1, w.stream));
-        int nchunks = 0;
-        ADB_HIP(hipMemcpyAsync(&nchunks, w.chunk_off.p + nnzb, sizeof(int), hipMemcpyDeviceToHost, w.stream));
-        ADB_HIP(hipStreamSynchronize(w.stream));
-        w.chunk_blk.reserve(nchunks);
-        w.chunk_part.reserve(156 * (size_t)nchunks);
-        hipLaunchKernelGGL(k_chunk_owner, dim3((nnzb + 255) / 256), dim3(256), 0, w.stream, nnzb, w.chunk_off.p, w.chunk_blk.p);
+        // each block has at most len / RCH + 1 chunks: launch for the bound, the
+        // kernel reads the exact count
+        const int max_chunks = nnzb + n / RCH;
+        w.chunk_blk.reserve(max_chunks);
+        w.chunk_part.reserve(156 * (size_t)max_chunks);
+        hipLaunchKernelGGL(k_chunk_owner, dim3((nnzb + 255) / 256), dim3(256), 0, w.stream, nnzb, w.chunk_off.p, w.chunk_blk.p,
+                           w.ccount.p);
         ADB_HIP(hipGetLastError());
         {
             // every contact record read once (D = 3: 59 doubles, 6: 94, 9: 147), the
             // contribution list and inertia terms read, one partial block written per chunk
-            ProfScope ps(w, PK_ASM_REDUCE, 1248.0 * nchunks + 24.0 * n + 1248.0 * B);
-            ps.counts(prof_snapshot(w, w.ccount.p), 472.0, 752.0, 1176.0, 0.0);
-            hipLaunchKernelGGL(k_reduce_chunks, dim3(nchunks), dim3(64), 0, w.stream, a, nchunks, w.chunk_off.p, w.chunk_blk.p,
+            ProfScope ps(w, PK_ASM_REDUCE, 24.0 * n + 1248.0 * B);
+            ps.counts(prof_snapshot(w, w.ccount.p), 472.0, 752.0, 1176.0, 1248.0);
+            hipLaunchKernelGGL(k_reduce_chunks, dim3(max_chunks), dim3(64 * RW), 0, w.stream, a, w.chunk_off.p, w.chunk_blk.p,
                                w.chunk_part.p);
         }
-        hipLaunchKernelGGL(k_reduce_final, dim3(nnzb), dim3(64), 0, w.stream, a, w.chunk_off.p, w.chunk_part.p);
+        hipLaunchKernelGGL(k_reduce_final, dim3(nnzb), dim3(64 * RW), 0, w.stream, a, w.chunk_off.p, w.chunk_part.p);
         ADB_HIP(hipGetLastError());
     }
     ADB_HIP(hipGetLastError());
-    w.Lchol.reserve(144 * (size_t)B);
-    hipLaunchKernelGGL(k_block_chol, dim3((B + CH_T - 1) / CH_T), dim3(CH_T), 0, w.stream, B, w.diag_blk.p, w.Hval.p, w.Lchol.p);
+    w.Pinv.reserve(144 * (size_t)B);
+    hipLaunchKernelGGL(k_block_inv, dim3((B + CH_T - 1) / CH_T), dim3(CH_T), 0, w.stream, B, w.diag_blk.p, w.Hval.p, w.Pinv.p);
     ADB_HIP(hipGetLastError());
 }
 

@@ -74,10 +74,11 @@ __device__ __forceinline__ double sel4(double a0, double a1, double a2, double a
 // the matrix is already PSD or `on` is false.  HOUT = true: S.A receives the
 // Helmert-coordinate matrix K = Q H Q^T, replaced by its projection
 // V diag(max(w, 0)) V^T when `on`.
-template <bool HOUT>
+// KIN = true: S.A already holds K (S.H is not read).
+template <bool HOUT, bool KIN = false>
 __device__ __forceinline__ void group_psd(GroupLDS& S, int lane, bool on) {
     // K = Q H Q^T: lane u = (b, j) < 9 builds column u
-    if (lane < 9) {
+    if (!KIN && lane < 9) {
         const int b = lane / 3, j = lane % 3;
         double hb[4];
 #pragma unroll
@@ -101,6 +102,8 @@ __device__ __forceinline__ void group_psd(GroupLDS& S, int lane, bool on) {
                 for (int k = 0; k < 4; k++) s += kHel[a][k] * t[k][i];
                 S.A[3 * a + i][lane] = s;
             }
+    }
+    if (lane < 9) {
 #pragma unroll
         for (int r = 0; r < 9; r++) S.V[r][lane] = (r == lane) ? 1.0 : 0.0;
     }
@@ -314,6 +317,86 @@ __device__ __forceinline__ void barrier_column(const Geo& G, int j, double (*H)[
             }
         }
     gj = G.is_ee ? G.m * gdj * G.B1 + gmj * G.B0 : gdj * G.B1;
+}
+
+// Helmert-coordinate Hessian of a class-3 contact, K = (C (x) I3) Mred
+// (C^T (x) I3) + mollifier terms, with C = h4 S^T (S: signs of the points in
+// the reduced vectors) and Cm the same for the mollifier vectors y = p1 - p0,
+// z = p3 - p2.  Lane (i, j) < 9 writes K[(a, i), (b, j)] for a, b < 3 into
+// S.A; no vertex-space matrix is formed.
+__device__ __forceinline__ void helmert_k9(const Geo& G, int lane, GroupLDS& L) {
+    if (lane >= 9) return;
+    const int i = lane / 3, j = lane % 3;
+    double C[3][3], Cm[3][2];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            double c = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) c += kHel[a][k] * red_sign(G.map, m, k);
+            C[a][m] = m < G.map.nr ? c : 0.0;
+        }
+        Cm[a][0] = kHel[a][1] - kHel[a][0];
+        Cm[a][1] = kHel[a][3] - kHel[a][2];
+    }
+    double K[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) K[a][b] = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; m++)
+#pragma unroll
+        for (int n = 0; n < 3; n++) {
+            if (m < G.map.nr && n < G.map.nr) {
+                const double Mv = G.B1 * G.rd.hess(m, i, n, j) + G.B2 * cmp(G.rd.g[m], i) * cmp(G.rd.g[n], j);
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int b = 0; b < 3; b++) K[a][b] += C[a][m] * C[b][n] * Mv;
+            }
+        }
+    if (G.is_ee) {
+        double gdi[3], gdj[3], gxi[3], gxj[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            double si = 0.0, sj = 0.0;
+#pragma unroll
+            for (int m = 0; m < 3; m++)
+                if (m < G.map.nr) { si += C[a][m] * cmp(G.rd.g[m], i); sj += C[a][m] * cmp(G.rd.g[m], j); }
+            gdi[a] = si; gdj[a] = sj;
+            gxi[a] = G.m2 != 0.0 ? Cm[a][0] * G.mc.grad(0, i) + Cm[a][1] * G.mc.grad(1, i) : 0.0;
+            gxj[a] = G.m2 != 0.0 ? Cm[a][0] * G.mc.grad(0, j) + Cm[a][1] * G.mc.grad(1, j) : 0.0;
+        }
+        double hx[2][2];
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+            for (int r = 0; r < 2; r++) hx[q][r] = G.m2 != 0.0 ? G.mc.hess(q, i, r, j) : 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                double hxk = 0.0;
+#pragma unroll
+                for (int q = 0; q < 2; q++)
+#pragma unroll
+                    for (int r = 0; r < 2; r++) hxk += Cm[a][q] * Cm[b][r] * hx[q][r];
+                const double gmi = G.m1 * gxi[a], gmj = G.m1 * gxj[b];
+                K[a][b] = G.m * K[a][b] + G.B1 * (gmi * gdj[b] + gdi[a] * gmj) + G.B0 * (G.m2 * gxi[a] * gxj[b] + G.m1 * hxk);
+            }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) L.A[3 * a + i][3 * b + j] = K[a][b];
+}
+// vertex-space gradient entry j = (point, component) < 12 (model/core_ipc.cpp:36-43,70-80)
+__device__ __forceinline__ double barrier_grad(const Geo& G, int j) {
+    const int l = j / 3, c = j % 3;
+    const double gdj = grad_d(G, l, c);
+    return G.is_ee ? G.m * gdj * G.B1 + grad_m(G, l, c) * G.B0 : gdj * G.B1;
 }
 
 // ---------------------------------------------------------------- friction setup
@@ -837,12 +920,13 @@ __global__ __launch_bounds__(64) void k_contact_group(WorldArgs a) {
             G.p2 = ld3(a.X + 3 * (size_t)id2); G.p3 = ld3(a.X + 3 * (size_t)id3);
             G.type = a.ctype[cid]; G.d = a.cdist[cid];
             geo_init(G, a.kp);
-            if (lane < 12) barrier_column(G, lane, L.H, gj);
-        } else if (lane < 12) {
-            for (int i = 0; i < 12; i++) L.H[i][lane] = 0.0;
+            helmert_k9(G, lane, L);
+            if (lane < 12) gj = barrier_grad(G, lane);
+        } else if (lane < 9) {
+            for (int r = 0; r < 9; r++) L.A[r][lane] = 0.0;
         }
         __syncthreads();
-        group_psd<true>(L, lane, valid && a.kp.psd);
+        group_psd<true, true>(L, lane, valid && a.kp.psd);
         Fric f;
         double w[4][4];
         const int nside0 = G.is_ee ? 2 : 1;

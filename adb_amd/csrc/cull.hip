@@ -18,6 +18,8 @@
 // The box prefilters are conservative, so the sets equal the exact predicate
 // over all primitive pairs (d_PT < d_hat for I != J, d_EE < d_hat for I < J).
 #include <hipcub/hipcub.hpp>
+#include <algorithm>
+#include <vector>
 #include <cstdio>
 #include "dmath.h"
 #include "world.h"
@@ -25,7 +27,6 @@
 namespace adb {
 
 constexpr int NB_THREADS = 256;
-constexpr int LCAP = 2048;      // LDS list capacity per primitive kind and side
 
 // Monotone map of doubles to unsigned 64-bit keys (negative values included),
 // so an integer atomicMin is an exact double min.
@@ -144,17 +145,32 @@ __device__ __forceinline__ void append_key(bool hit, unsigned long long key, uns
     }
 }
 
+// ------------------------------------------------------------------ narrow phase
+// One workgroup per overlapping body pair, three passes (vertices of I vs
+// faces of J, vertices of J vs faces of I, edges of I vs edges of J):
+//   1. stage: the primitives of each side whose box meets the overlap region
+//      of the two body boxes go to LDS with their boxes (BT = float: rounded
+//      outward, the mixed-precision broad phase; BT = double: exact), at most
+//      CAP per side per round (longer lists are streamed in rounds);
+//   2. box tests: all staged pairs, PAIRS_PT per thread per round, LDS reads
+//      only; overlapping pairs go to an LDS candidate queue;
+//   3. exact tests (FP64, from the positions in HBM) of the queued candidates:
+//      distance type and value against d_hat (sets), or the CCD time of impact.
+// Boxes only cull; the FP64 predicate decides, so the sets equal those of the
+// all-pairs tests (model/thread_local_culling.cpp:209-386 pt/ee_col_set_task).
+constexpr int PAIRS_PT = 4;                    // box tests per thread per round
+constexpr int QCAP = NB_THREADS * PAIRS_PT * 2; // candidate queue
+
 struct NarrowArgs {
     int n_pairs; const int2* pairs;
     const int* voff; const int* foff; const int* eoff; const int* face; const int* edge;
     const double* X0; const double* X1;   // X1 only for CCD
-    const double* box;
+    const double* box;                    // body boxes
     double r; double d_hat;
     int enable_pt, enable_ee;
     unsigned long long* pt_keys; int pt_cap; unsigned long long* ee_keys; int ee_cap;
-    int* counters;                       // [0] pt, [1] ee, [2] overflowed pairs
-    int2* overflow; int ovf_cap;
-    unsigned long long* toi_bits;        // CCD: [1] pt/ee min
+    int* counters;                        // [0] pt, [1] ee
+    unsigned long long* toi_bits;         // CCD: [1] pt/ee min
 };
 
 template <int CCD>
@@ -169,203 +185,168 @@ __device__ __forceinline__ void prim_box(const double* X0, const double* X1, con
         }
     }
 }
+// outward rounding to the box type
+__device__ __forceinline__ float round_down(double x) { float f = (float)x; return (double)f > x ? nextafterf(f, -INFINITY) : f; }
+__device__ __forceinline__ float round_up(double x) { float f = (float)x; return (double)f < x ? nextafterf(f, INFINITY) : f; }
+template <class BT> __device__ __forceinline__ BT lo_of(double x);
+template <class BT> __device__ __forceinline__ BT hi_of(double x);
+template <> __device__ __forceinline__ float lo_of<float>(double x) { return round_down(x); }
+template <> __device__ __forceinline__ float hi_of<float>(double x) { return round_up(x); }
+template <> __device__ __forceinline__ double lo_of<double>(double x) { return x; }
+template <> __device__ __forceinline__ double hi_of<double>(double x) { return x; }
 
-// binning + tests for one body pair; mode CCD = 0 contact sets, 1 step bound
+// primitive `id` of kind 0 (vertex), 1 (face), 2 (edge): its vertex ids
+__device__ __forceinline__ int prim_verts(const NarrowArgs& a, int kind, int id, int v[3]) {
+    if (kind == 0) { v[0] = id; return 1; }
+    if (kind == 1) { v[0] = a.face[3 * (size_t)id]; v[1] = a.face[3 * (size_t)id + 1]; v[2] = a.face[3 * (size_t)id + 2]; return 3; }
+    v[0] = a.edge[2 * (size_t)id]; v[1] = a.edge[2 * (size_t)id + 1]; return 2;
+}
+
+// Stage primitives [*cursor, p1) of one kind into LDS until CAP entries are
+// filled (whole rounds of NB_THREADS); boxes are inflated by `infl`.  Returns
+// the staged count; *cursor advances past the scanned primitives.
+template <int CCD, class BT, int CAP>
+__device__ int stage_prims(const NarrowArgs& a, int kind, int* cursor, int p1, const double* xlo, const double* xhi,
+                           double infl, BT (*bx)[6], int* ids, int* cnt) {
+    __syncthreads();
+    if (threadIdx.x == 0) *cnt = 0;
+    __syncthreads();
+    int c = *cursor;
+    while (c < p1 && *cnt + NB_THREADS <= CAP) {
+        const int p = c + threadIdx.x;
+        if (p < p1) {
+            int v[3];
+            const int nv = prim_verts(a, kind, p, v);
+            double lo[3], hi[3];
+            prim_box<CCD>(a.X0, a.X1, v, nv, lo, hi);
+            if (ovl(lo, hi, xlo, xhi)) {
+                const int k = atomicAdd(cnt, 1);
+                ids[k] = p;
+                for (int i = 0; i < 3; i++) { bx[k][i] = lo_of<BT>(lo[i] - infl); bx[k][3 + i] = hi_of<BT>(hi[i] + infl); }
+            }
+        }
+        c += NB_THREADS;
+        __syncthreads();
+    }
+    *cursor = c;
+    return *cnt;
+}
+
+// exact tests of the queued candidates (LDS index pairs into the staged lists)
 template <int CCD>
+__device__ void flush_candidates(const NarrowArgs& a, int kind, const int* idA, const int* idB, const int2* q, int* qn,
+                                 double& tmin) {
+    __syncthreads();
+    const int n = *qn;
+    const int nround = (n + NB_THREADS - 1) / NB_THREADS;
+    for (int it = 0; it < nround; it++) {
+        const int t = it * NB_THREADS + threadIdx.x;
+        bool hit = false;
+        unsigned long long key = 0;
+        if (t < n) {
+            const int2 c = q[t];
+            const int pa = idA[c.x], pb = idB[c.y];
+            if (kind == 0) {
+                const int v = pa, f = pb;
+                const int* fv = a.face + 3 * (size_t)f;
+                const d3 p = ld3(a.X0 + 3 * (size_t)v), t0 = ld3(a.X0 + 3 * (size_t)fv[0]), t1 = ld3(a.X0 + 3 * (size_t)fv[1]),
+                         t2 = ld3(a.X0 + 3 * (size_t)fv[2]);
+                if (!CCD) {
+                    const int ty = pt_type(p, t0, t1, t2);
+                    hit = pt_dist(p, t0, t1, t2, ty) < a.d_hat;
+                    key = ((unsigned long long)v << 32) | (unsigned long long)f;
+                } else {
+                    d3 P0[4] = {p, t0, t1, t2};
+                    d3 P1[4] = {ld3(a.X1 + 3 * (size_t)v), ld3(a.X1 + 3 * (size_t)fv[0]), ld3(a.X1 + 3 * (size_t)fv[1]),
+                                ld3(a.X1 + 3 * (size_t)fv[2])};
+                    tmin = fmin(tmin, pt_toi(P0, P1));
+                }
+            } else {
+                const int ea = pa, eb = pb;
+                const int* va = a.edge + 2 * (size_t)ea;
+                const int* vb = a.edge + 2 * (size_t)eb;
+                const d3 a0 = ld3(a.X0 + 3 * (size_t)va[0]), a1 = ld3(a.X0 + 3 * (size_t)va[1]);
+                const d3 b0 = ld3(a.X0 + 3 * (size_t)vb[0]), b1 = ld3(a.X0 + 3 * (size_t)vb[1]);
+                if (!CCD) {
+                    const int ty = ee_type(a0, a1, b0, b1);
+                    hit = ee_dist(a0, a1, b0, b1, ty) < a.d_hat;
+                    key = ((unsigned long long)ea << 32) | (unsigned long long)eb;
+                } else {
+                    d3 P0[4] = {a0, a1, b0, b1};
+                    d3 P1[4] = {ld3(a.X1 + 3 * (size_t)va[0]), ld3(a.X1 + 3 * (size_t)va[1]), ld3(a.X1 + 3 * (size_t)vb[0]),
+                                ld3(a.X1 + 3 * (size_t)vb[1])};
+                    tmin = fmin(tmin, ee_toi(P0, P1));
+                }
+            }
+        }
+        if (!CCD) {
+            if (kind == 0) append_key(hit, key, a.pt_keys, a.pt_cap, &a.counters[0]);
+            else append_key(hit, key, a.ee_keys, a.ee_cap, &a.counters[1]);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *qn = 0;
+    __syncthreads();
+}
+
+template <class BT> struct NarrowCap;
+template <> struct NarrowCap<float> { static constexpr int A = 1024, B = 1024; };
+template <> struct NarrowCap<double> { static constexpr int A = 512, B = 512; };
+
+template <int CCD, class BT>
 __global__ __launch_bounds__(NB_THREADS) void k_pair_narrow(NarrowArgs a) {
-    __shared__ int lv[2][LCAP], lf[2][LCAP], le[2][LCAP];
-    __shared__ int cnt[6];
+    constexpr int CA = NarrowCap<BT>::A, CB = NarrowCap<BT>::B;
+    __shared__ BT bxA[CA][6], bxB[CB][6];
+    __shared__ int idA[CA], idB[CB];
+    __shared__ int2 q[QCAP];
+    __shared__ int cntA, cntB, qn;
     __shared__ double bmin[NB_THREADS / 64];
     const int pidx = blockIdx.x;
     if (pidx >= a.n_pairs) return;
     const int2 pr = a.pairs[pidx];
-    const int body[2] = {pr.x, pr.y};
     double xlo[3], xhi[3];
     for (int k = 0; k < 3; k++) {
         xlo[k] = fmax(a.box[6 * (size_t)pr.x + k], a.box[6 * (size_t)pr.y + k]);
         xhi[k] = fmin(a.box[6 * (size_t)pr.x + 3 + k], a.box[6 * (size_t)pr.y + 3 + k]);
     }
-    if (threadIdx.x < 6) cnt[threadIdx.x] = 0;
-    __syncthreads();
-    // ---- binning: lists 0..1 vertices, 2..3 faces, 4..5 edges (side 0 = I, 1 = J)
-    for (int s = 0; s < 2; s++) {
-        const int b = body[s];
-        for (int v = a.voff[b] + threadIdx.x; v < a.voff[b + 1]; v += NB_THREADS) {
-            double lo[3], hi[3];
-            prim_box<CCD>(a.X0, a.X1, &v, 1, lo, hi);
-            if (ovl(lo, hi, xlo, xhi)) { int k = atomicAdd(&cnt[0 + s], 1); if (k < LCAP) lv[s][k] = v; }
-        }
-        for (int f = a.foff[b] + threadIdx.x; f < a.foff[b + 1]; f += NB_THREADS) {
-            double lo[3], hi[3];
-            prim_box<CCD>(a.X0, a.X1, a.face + 3 * (size_t)f, 3, lo, hi);
-            if (ovl(lo, hi, xlo, xhi)) { int k = atomicAdd(&cnt[2 + s], 1); if (k < LCAP) lf[s][k] = f; }
-        }
-        for (int e = a.eoff[b] + threadIdx.x; e < a.eoff[b + 1]; e += NB_THREADS) {
-            double lo[3], hi[3];
-            prim_box<CCD>(a.X0, a.X1, a.edge + 2 * (size_t)e, 2, lo, hi);
-            if (ovl(lo, hi, xlo, xhi)) { int k = atomicAdd(&cnt[4 + s], 1); if (k < LCAP) le[s][k] = e; }
-        }
-    }
-    __syncthreads();
-    bool overflow = false;
-    for (int k = 0; k < 6; k++) overflow |= cnt[k] > LCAP;
-    if (overflow) {
-        if (threadIdx.x == 0) {
-            int o = atomicAdd(&a.counters[2], 1);
-            if (o < a.ovf_cap) a.overflow[o] = pr;
-        }
-        return;
-    }
+    if (threadIdx.x == 0) qn = 0;
     double tmin = 1.0;
-    // ---- PT both directions: vertices of side s against faces of side 1 - s
-    if (a.enable_pt) {
-        for (int s = 0; s < 2; s++) {
-            const int nv = cnt[0 + s], nf = cnt[2 + (1 - s)];
-            const int total = nv * nf;
-            const int nround = (total + NB_THREADS - 1) / NB_THREADS;
-            for (int it = 0; it < nround; it++) {
-                const int t = it * NB_THREADS + threadIdx.x;
-                bool hit = false;
-                unsigned long long key = 0;
-                if (t < total) {
-                    const int v = lv[s][t / nf], f = lf[1 - s][t % nf];
-                    const int* fv = a.face + 3 * (size_t)f;
-                    double vlo[3], vhi[3], flo[3], fhi[3];
-                    prim_box<CCD>(a.X0, a.X1, &v, 1, vlo, vhi);
-                    prim_box<CCD>(a.X0, a.X1, fv, 3, flo, fhi);
-                    if (!CCD) for (int k = 0; k < 3; k++) { vlo[k] -= a.r; vhi[k] += a.r; }
-                    if (ovl(vlo, vhi, flo, fhi)) {
-                        d3 p = ld3(a.X0 + 3 * (size_t)v), t0 = ld3(a.X0 + 3 * (size_t)fv[0]), t1 = ld3(a.X0 + 3 * (size_t)fv[1]), t2 = ld3(a.X0 + 3 * (size_t)fv[2]);
-                        if (!CCD) {
-                            int ty = pt_type(p, t0, t1, t2);
-                            double d = pt_dist(p, t0, t1, t2, ty);
-                            hit = d < a.d_hat;
-                            key = ((unsigned long long)v << 32) | (unsigned long long)f;
-                        } else {
-                            d3 P0[4] = {p, t0, t1, t2};
-                            d3 P1[4] = {ld3(a.X1 + 3 * (size_t)v), ld3(a.X1 + 3 * (size_t)fv[0]), ld3(a.X1 + 3 * (size_t)fv[1]), ld3(a.X1 + 3 * (size_t)fv[2])};
-                            tmin = fmin(tmin, pt_toi(P0, P1));
+    const double infl = CCD ? 0.0 : a.r;
+    for (int pass = 0; pass < 3; pass++) {
+        if (pass < 2 ? !a.enable_pt : !a.enable_ee) continue;
+        const int kind = pass < 2 ? 0 : 1;                        // 0 = PT, 1 = EE
+        const int bA = pass == 1 ? pr.y : pr.x, bB = pass == 1 ? pr.x : pr.y;
+        const int kA = pass < 2 ? 0 : 2, kB = pass < 2 ? 1 : 2;   // primitive kinds
+        const int* offA = kA == 0 ? a.voff : a.eoff;
+        const int* offB = kB == 1 ? a.foff : a.eoff;
+        int curA = offA[bA];
+        const int endA = offA[bA + 1];
+        while (curA < endA) {
+            // the A side is inflated by r (vertex or edge of body I), as the all-pairs test
+            const int nA = stage_prims<CCD, BT, CA>(a, kA, &curA, endA, xlo, xhi, infl, bxA, idA, &cntA);
+            if (nA == 0) continue;
+            int curB = offB[bB];
+            const int endB = offB[bB + 1];
+            while (curB < endB) {
+                const int nB = stage_prims<CCD, BT, CB>(a, kB, &curB, endB, xlo, xhi, 0.0, bxB, idB, &cntB);
+                const int total = nA * nB;
+                for (int t0 = 0; t0 < total; t0 += NB_THREADS * PAIRS_PT) {
+                    __syncthreads();
+                    if (qn + NB_THREADS * PAIRS_PT > QCAP) flush_candidates<CCD>(a, kind, idA, idB, q, &qn, tmin);
+#pragma unroll
+                    for (int u = 0; u < PAIRS_PT; u++) {
+                        const int t = t0 + u * NB_THREADS + threadIdx.x;
+                        if (t < total) {
+                            const int i = t / nB, j = t % nB;
+                            const bool o = bxA[i][0] <= bxB[j][3] && bxB[j][0] <= bxA[i][3] &&
+                                           bxA[i][1] <= bxB[j][4] && bxB[j][1] <= bxA[i][4] &&
+                                           bxA[i][2] <= bxB[j][5] && bxB[j][2] <= bxA[i][5];
+                            if (o) { const int s = atomicAdd(&qn, 1); q[s] = make_int2(i, j); }
                         }
                     }
                 }
-                if (!CCD) append_key(hit, key, a.pt_keys, a.pt_cap, &a.counters[0]);
+                flush_candidates<CCD>(a, kind, idA, idB, q, &qn, tmin);
             }
-        }
-    }
-    // ---- EE: edges of I against edges of J
-    if (a.enable_ee) {
-        const int n0 = cnt[4], n1 = cnt[5];
-        const int total = n0 * n1;
-        const int nround = (total + NB_THREADS - 1) / NB_THREADS;
-        for (int it = 0; it < nround; it++) {
-            const int t = it * NB_THREADS + threadIdx.x;
-            bool hit = false;
-            unsigned long long key = 0;
-            if (t < total) {
-                const int ea = le[0][t / n1], eb = le[1][t % n1];
-                const int* va = a.edge + 2 * (size_t)ea;
-                const int* vb = a.edge + 2 * (size_t)eb;
-                double alo[3], ahi[3], blo[3], bhi[3];
-                prim_box<CCD>(a.X0, a.X1, va, 2, alo, ahi);
-                prim_box<CCD>(a.X0, a.X1, vb, 2, blo, bhi);
-                if (!CCD) for (int k = 0; k < 3; k++) { alo[k] -= a.r; ahi[k] += a.r; }
-                if (ovl(alo, ahi, blo, bhi)) {
-                    d3 a0 = ld3(a.X0 + 3 * (size_t)va[0]), a1 = ld3(a.X0 + 3 * (size_t)va[1]);
-                    d3 b0 = ld3(a.X0 + 3 * (size_t)vb[0]), b1 = ld3(a.X0 + 3 * (size_t)vb[1]);
-                    if (!CCD) {
-                        int ty = ee_type(a0, a1, b0, b1);
-                        double d = ee_dist(a0, a1, b0, b1, ty);
-                        hit = d < a.d_hat;
-                        key = ((unsigned long long)ea << 32) | (unsigned long long)eb;
-                    } else {
-                        d3 P0[4] = {a0, a1, b0, b1};
-                        d3 P1[4] = {ld3(a.X1 + 3 * (size_t)va[0]), ld3(a.X1 + 3 * (size_t)va[1]), ld3(a.X1 + 3 * (size_t)vb[0]), ld3(a.X1 + 3 * (size_t)vb[1])};
-                        tmin = fmin(tmin, ee_toi(P0, P1));
-                    }
-                }
-            }
-            if (!CCD) append_key(hit, key, a.ee_keys, a.ee_cap, &a.counters[1]);
-        }
-    }
-    if (CCD) {
-        for (int m = 32; m >= 1; m >>= 1) tmin = fmin(tmin, __shfl_xor(tmin, m));
-        if (__lane_id() == 0) bmin[threadIdx.x / 64] = tmin;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = bmin[0];
-            for (int i = 1; i < NB_THREADS / 64; i++) t = fmin(t, bmin[i]);
-            if (t < 1.0) atomicMin(&a.toi_bits[1], order_key(t));
-        }
-    }
-}
-
-// overflow fallback: the same tests straight from global memory, no LDS lists
-template <int CCD>
-__global__ __launch_bounds__(NB_THREADS) void k_pair_brute(NarrowArgs a, int n_ovf) {
-    const int pidx = blockIdx.x;
-    if (pidx >= n_ovf) return;
-    const int2 pr = a.overflow[pidx];
-    double tmin = 1.0;
-    __shared__ double bmin[NB_THREADS / 64];
-    for (int s = 0; s < 2 && a.enable_pt; s++) {
-        const int A = s == 0 ? pr.x : pr.y, Bb = s == 0 ? pr.y : pr.x;
-        const long long nv = a.voff[A + 1] - a.voff[A], nf = a.foff[Bb + 1] - a.foff[Bb];
-        const long long total = nv * nf;
-        for (long long base = 0; base < total; base += NB_THREADS) {
-            long long t = base + threadIdx.x;
-            bool hit = false; unsigned long long key = 0;
-            if (t < total) {
-                int v = a.voff[A] + (int)(t / nf), f = a.foff[Bb] + (int)(t % nf);
-                const int* fv = a.face + 3 * (size_t)f;
-                double vlo[3], vhi[3], flo[3], fhi[3];
-                prim_box<CCD>(a.X0, a.X1, &v, 1, vlo, vhi);
-                prim_box<CCD>(a.X0, a.X1, fv, 3, flo, fhi);
-                if (!CCD) for (int k = 0; k < 3; k++) { vlo[k] -= a.r; vhi[k] += a.r; }
-                if (ovl(vlo, vhi, flo, fhi)) {
-                    d3 p = ld3(a.X0 + 3 * (size_t)v), t0 = ld3(a.X0 + 3 * (size_t)fv[0]), t1 = ld3(a.X0 + 3 * (size_t)fv[1]), t2 = ld3(a.X0 + 3 * (size_t)fv[2]);
-                    if (!CCD) {
-                        int ty = pt_type(p, t0, t1, t2);
-                        hit = pt_dist(p, t0, t1, t2, ty) < a.d_hat;
-                        key = ((unsigned long long)v << 32) | (unsigned long long)f;
-                    } else {
-                        d3 P0[4] = {p, t0, t1, t2};
-                        d3 P1[4] = {ld3(a.X1 + 3 * (size_t)v), ld3(a.X1 + 3 * (size_t)fv[0]), ld3(a.X1 + 3 * (size_t)fv[1]), ld3(a.X1 + 3 * (size_t)fv[2])};
-                        tmin = fmin(tmin, pt_toi(P0, P1));
-                    }
-                }
-            }
-            if (!CCD) append_key(hit, key, a.pt_keys, a.pt_cap, &a.counters[0]);
-        }
-    }
-    if (a.enable_ee) {
-        const long long n0 = a.eoff[pr.x + 1] - a.eoff[pr.x], n1 = a.eoff[pr.y + 1] - a.eoff[pr.y];
-        const long long total = n0 * n1;
-        for (long long base = 0; base < total; base += NB_THREADS) {
-            long long t = base + threadIdx.x;
-            bool hit = false; unsigned long long key = 0;
-            if (t < total) {
-                int ea = a.eoff[pr.x] + (int)(t / n1), eb = a.eoff[pr.y] + (int)(t % n1);
-                const int* va = a.edge + 2 * (size_t)ea;
-                const int* vb = a.edge + 2 * (size_t)eb;
-                double alo[3], ahi[3], blo[3], bhi[3];
-                prim_box<CCD>(a.X0, a.X1, va, 2, alo, ahi);
-                prim_box<CCD>(a.X0, a.X1, vb, 2, blo, bhi);
-                if (!CCD) for (int k = 0; k < 3; k++) { alo[k] -= a.r; ahi[k] += a.r; }
-                if (ovl(alo, ahi, blo, bhi)) {
-                    d3 a0 = ld3(a.X0 + 3 * (size_t)va[0]), a1 = ld3(a.X0 + 3 * (size_t)va[1]);
-                    d3 b0 = ld3(a.X0 + 3 * (size_t)vb[0]), b1 = ld3(a.X0 + 3 * (size_t)vb[1]);
-                    if (!CCD) {
-                        int ty = ee_type(a0, a1, b0, b1);
-                        hit = ee_dist(a0, a1, b0, b1, ty) < a.d_hat;
-                        key = ((unsigned long long)ea << 32) | (unsigned long long)eb;
-                    } else {
-                        d3 P0[4] = {a0, a1, b0, b1};
-                        d3 P1[4] = {ld3(a.X1 + 3 * (size_t)va[0]), ld3(a.X1 + 3 * (size_t)va[1]), ld3(a.X1 + 3 * (size_t)vb[0]), ld3(a.X1 + 3 * (size_t)vb[1])};
-                        tmin = fmin(tmin, ee_toi(P0, P1));
-                    }
-                }
-            }
-            if (!CCD) append_key(hit, key, a.ee_keys, a.ee_cap, &a.counters[1]);
         }
     }
     if (CCD) {
@@ -463,6 +444,19 @@ static int body_pairs(World& w, const double* X0, const double* X1, double infl)
     return npairs;
 }
 
+static NarrowArgs narrow_args(World& w, int npairs, const double* X0, const double* X1, double r) {
+    NarrowArgs a;
+    a.n_pairs = npairs; a.pairs = w.pairs.p;
+    a.voff = w.voff.p; a.foff = w.foff.p; a.eoff = w.eoff.p; a.face = w.face.p; a.edge = w.edge.p;
+    a.X0 = X0; a.X1 = X1; a.box = w.bbox.p; a.r = r; a.d_hat = w.P.d_hat;
+    a.enable_pt = w.P.enable_pt && w.F > 0;
+    a.enable_ee = w.P.enable_ee && w.E > 0;
+    a.pt_keys = nullptr; a.pt_cap = 0; a.ee_keys = nullptr; a.ee_cap = 0;
+    a.counters = w.counters.p;
+    a.toi_bits = nullptr;
+    return a;
+}
+
 void collision_set(World& w, const double* X, ContactSet& out) {
     const double r = sqrt(w.P.d_hat) * (1.0 + 1e-6) + 1e-12;
     int npairs = body_pairs(w, X, nullptr, r);
@@ -473,31 +467,18 @@ void collision_set(World& w, const double* X, ContactSet& out) {
         w.keys_b.reserve(2 * pcap);
         size_t vcap = w.V + 64;
         ADB_HIP(hipMemsetAsync(w.counters.p, 0, 4 * sizeof(int), w.stream));
-        NarrowArgs a;
-        a.n_pairs = npairs; a.pairs = w.pairs.p;
-        a.voff = w.voff.p; a.foff = w.foff.p; a.eoff = w.eoff.p; a.face = w.face.p; a.edge = w.edge.p;
-        a.X0 = X; a.X1 = nullptr; a.box = w.bbox.p; a.r = r; a.d_hat = w.P.d_hat;
-        a.enable_pt = w.P.enable_pt; a.enable_ee = w.P.enable_ee;
+        NarrowArgs a = narrow_args(w, npairs, X, nullptr, r);
         a.pt_keys = w.keys_a.p; a.pt_cap = (int)pcap;
         a.ee_keys = w.keys_a.p + pcap; a.ee_cap = (int)pcap;
-        a.counters = w.counters.p;
-        w.ovf.reserve(std::max(npairs, 1));
-        a.overflow = w.ovf.p; a.ovf_cap = npairs;
-        a.toi_bits = nullptr;
         if (npairs > 0) {
             ProfScope ps(w, PK_NARROW_SET, 0.0);
-            hipLaunchKernelGGL(k_pair_narrow<0>, dim3(npairs), dim3(NB_THREADS), 0, w.stream, a);
+            if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<0, float>), dim3(npairs), dim3(NB_THREADS), 0, w.stream, a);
+            else hipLaunchKernelGGL((k_pair_narrow<0, double>), dim3(npairs), dim3(NB_THREADS), 0, w.stream, a);
             ADB_HIP(hipGetLastError());
         }
         int h[4];
         ADB_HIP(hipMemcpyAsync(h, w.counters.p, 4 * sizeof(int), hipMemcpyDeviceToHost, w.stream));
         ADB_HIP(hipStreamSynchronize(w.stream));
-        if (h[2] > 0) {
-            hipLaunchKernelGGL(k_pair_brute<0>, dim3(h[2]), dim3(NB_THREADS), 0, w.stream, a, h[2]);
-            ADB_HIP(hipGetLastError());
-            ADB_HIP(hipMemcpyAsync(h, w.counters.p, 4 * sizeof(int), hipMemcpyDeviceToHost, w.stream));
-            ADB_HIP(hipStreamSynchronize(w.stream));
-        }
         // vertex-ground
         w.vg_keys.reserve(vcap);
         int nvg = 0;
@@ -555,26 +536,12 @@ double ccd_toi(World& w, const double* X0, const double* X1, int* status) {
     int npairs = body_pairs(w, X0, X1, 0.0);
     w.counters.reserve(16);
     ADB_HIP(hipMemsetAsync(w.counters.p, 0, 4 * sizeof(int), w.stream));
-    NarrowArgs a;
-    a.n_pairs = npairs; a.pairs = w.pairs.p;
-    a.voff = w.voff.p; a.foff = w.foff.p; a.eoff = w.eoff.p; a.face = w.face.p; a.edge = w.edge.p;
-    a.X0 = X0; a.X1 = X1; a.box = w.bbox.p; a.r = 0.0; a.d_hat = w.P.d_hat;
-    a.enable_pt = w.P.enable_pt; a.enable_ee = w.P.enable_ee;
-    a.pt_keys = nullptr; a.pt_cap = 0; a.ee_keys = nullptr; a.ee_cap = 0;
-    a.counters = w.counters.p;
-    w.ovf.reserve(std::max(npairs, 1));
-    a.overflow = w.ovf.p; a.ovf_cap = npairs;
+    NarrowArgs a = narrow_args(w, npairs, X0, X1, 0.0);
     a.toi_bits = w.toi_bits.p;
     if (npairs > 0) {
         ProfScope ps(w, PK_NARROW_CCD, 0.0);
-        hipLaunchKernelGGL(k_pair_narrow<1>, dim3(npairs), dim3(NB_THREADS), 0, w.stream, a);
-        ADB_HIP(hipGetLastError());
-    }
-    int h[4];
-    ADB_HIP(hipMemcpyAsync(h, w.counters.p, 4 * sizeof(int), hipMemcpyDeviceToHost, w.stream));
-    ADB_HIP(hipStreamSynchronize(w.stream));
-    if (h[2] > 0) {
-        hipLaunchKernelGGL(k_pair_brute<1>, dim3(h[2]), dim3(NB_THREADS), 0, w.stream, a, h[2]);
+        if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<1, float>), dim3(npairs), dim3(NB_THREADS), 0, w.stream, a);
+        else hipLaunchKernelGGL((k_pair_narrow<1, double>), dim3(npairs), dim3(NB_THREADS), 0, w.stream, a);
         ADB_HIP(hipGetLastError());
     }
     unsigned long long bits[2];

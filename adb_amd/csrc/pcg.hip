@@ -2,197 +2,247 @@
 // Hessian, FP64 throughout.  Replaces the direct PardisoLLT / SimplicialLLT
 // solve of ABD::solve (model/abd.cpp:343-371): dq = -H^{-1} g.
 //
-// Per iteration three launches, no host round trip:
-//   spmv   : y = H p (one 64-lane workgroup per block row, 5 x 12-lane groups
-//            split the row's blocks, fixed-order LDS combine) + row partial p.y
-//   update : every workgroup sums the row partials in the same fixed order
-//            (deterministic), alpha = rz / pAp, x += alpha p, r -= alpha y,
-//            z = L^-T L^-1 r per body, partials r.z and r.r
-//   dir    : beta = rz' / rz, p = z + beta p
-// Scalars are double-buffered by iteration parity.  The host reads r.r every
-// CHECK iterations.
+// Two launches per iteration, no atomics and no host round trip:
+//   spmv(k)  : every wave first sums the update partials of iteration k - 1
+//              (r.z, r.r per update workgroup) in a fixed order; from them the
+//              stop test (r.r <= rtol^2 b.b) and beta = rz_k / rz_{k-1}; then
+//              p = z + beta p_old (on the fly for every column block, written
+//              for the workgroup's own rows into the other p buffer), y = H p,
+//              and one p.y partial per workgroup
+//   update(k): every wave sums the p.y partials in a fixed order, alpha =
+//              rz / pAp, x += alpha p, r -= alpha y, z = M^-1 r (explicit
+//              inverse of the diagonal block, 12 lanes per body), and one
+//              (r.z, r.r) partial per workgroup
+// Every workgroup evaluates the same scalars from the same partials in the
+// same order, so they agree bit for bit; once the stop test holds every kernel
+// returns at once, and the host queues iterations in batches, checking the
+// flag of the previous batch while the current one runs.
 #include <cmath>
 #include "dmath.h"
 #include "world.h"
 
 namespace adb {
 
-constexpr int CHECK = 8;
+// scalars
+enum { S_STOP = 0, S_ITERS, S_BB, S_RZ0, S_RZ1, S_COUNT };
+// stop flag values
+enum { RUN = 0, CONVERGED = 1, BREAKDOWN = 2, NONFINITE = 3 };
 
-__device__ __forceinline__ double sum_fixed(const double* v, int n) {
-    // identical order in every workgroup: 4 interleaved accumulators then combine
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int i = 0;
-    for (; i + 3 < n; i += 4) { s0 += v[i]; s1 += v[i + 1]; s2 += v[i + 2]; s3 += v[i + 3]; }
-    for (; i < n; i++) s0 += v[i];
-    return (s0 + s1) + (s2 + s3);
+constexpr int SP_ROWS = 4;       // block rows per spmv workgroup (one 64-lane team each)
+constexpr int UP_BODIES = 16;    // bodies per update workgroup (12 lanes each)
+constexpr int UP_T = 12 * UP_BODIES;
+constexpr int BATCH = 16;        // iterations queued between host checks
+
+// fixed-order wave sum, broadcast from lane 0
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_down(v, m);
+    return __shfl(v, 0);
 }
-// LDS-cooperative version: each workgroup (256 lanes) computes the same total
-__device__ double block_sum_fixed(const double* v, int n, double* sh) {
-    const int t = threadIdx.x;
+// fixed-order sum of n values (lane l adds l, l + 64, ... in order)
+__device__ __forceinline__ double wave_sum_array(const double* v, int n) {
+    const int l = threadIdx.x % 64;
     double s = 0.0;
-    for (int i = t; i < n; i += blockDim.x) s += v[i];
-    sh[t] = s;
+    for (int i = l; i < n; i += 64) s += v[i];
+    return wave_sum(s);
+}
+// fixed-order workgroup sum (one value per wave combined in wave order)
+__device__ double wg_sum(double v, double* sh) {
+    v = wave_sum(v);
+    const int w = threadIdx.x / 64, nw = (blockDim.x + 63) / 64;
     __syncthreads();
-    for (int m = blockDim.x / 2; m >= 1; m >>= 1) {
-        if (t < m) sh[t] += sh[t + m];
-        __syncthreads();
-    }
-    double r = sh[0];
+    if (threadIdx.x % 64 == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    for (int i = 0; i < nw; i++) r += sh[i];
     __syncthreads();
     return r;
 }
 
-__global__ __launch_bounds__(64) void k_spmv(int B, const int* __restrict__ row_ptr, const int* __restrict__ col_idx,
-                                             const double* __restrict__ Hval, const double* __restrict__ p,
-                                             double* __restrict__ y, double* __restrict__ part) {
-    __shared__ double sh[5][12];
-    const int row = blockIdx.x;
-    if (row >= B) return;
-    const int g = threadIdx.x / 12, r = threadIdx.x % 12;
-    double acc = 0.0;
-    if (g < 5) {
+// initial: x = 0, r = -g, z = M^-1 r, p_old = 0, (r.z, r.r) partials
+__global__ __launch_bounds__(UP_T) void k_pcg_init(int B, const double* __restrict__ g, const double* __restrict__ Minv,
+                                                   double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
+                                                   double* __restrict__ p0, double* __restrict__ partB, double* __restrict__ scal) {
+    __shared__ double rs[UP_BODIES][12];
+    __shared__ double sh[4];
+    const int lb = threadIdx.x / 12, i = threadIdx.x % 12;
+    const int b = blockIdx.x * UP_BODIES + lb;
+    double rz = 0.0, rr = 0.0;
+    if (b < B) {
+        const size_t k = 12 * (size_t)b + i;
+        const double rk = -g[k];
+        x[k] = 0.0; r[k] = rk; p0[k] = 0.0;
+        rs[lb][i] = rk;
+    }
+    __syncthreads();
+    if (b < B) {
+        const double* M = Minv + 144 * (size_t)b;
+        double zi = 0.0;
+#pragma unroll
+        for (int j = 0; j < 12; j++) zi += M[12 * j + i] * rs[lb][j];
+        z[12 * (size_t)b + i] = zi;
+        rz = rs[lb][i] * zi; rr = rs[lb][i] * rs[lb][i];
+    }
+    rz = wg_sum(rz, sh);
+    rr = wg_sum(rr, sh);
+    if (threadIdx.x == 0) {
+        partB[blockIdx.x] = rz; partB[gridDim.x + blockIdx.x] = rr;
+        if (blockIdx.x == 0) { scal[S_STOP] = RUN; scal[S_ITERS] = 0.0; }
+    }
+}
+
+__global__ __launch_bounds__(64 * SP_ROWS) void k_pcg_spmv(int B, int iter, double rtol2, int nbU, const int* __restrict__ row_ptr,
+                                                           const int* __restrict__ col_idx, const double* __restrict__ Hval,
+                                                           const double* __restrict__ z, const double* __restrict__ pold,
+                                                           double* __restrict__ pnew, double* __restrict__ y,
+                                                           const double* __restrict__ partB, double* __restrict__ partA,
+                                                           double* __restrict__ scal) {
+    __shared__ double sh[SP_ROWS];
+    __shared__ double acc_sh[SP_ROWS][5][12];
+    if (scal[S_STOP] != RUN) return;
+    // scalars of iteration iter - 1 (identical in every wave of every workgroup)
+    const double RZ = wave_sum_array(partB, nbU);
+    const double RR = wave_sum_array(partB + nbU, nbU);
+    const double bb = iter == 0 ? RR : scal[S_BB];
+    int stop = RUN;
+    if (!std::isfinite(RR) || !std::isfinite(RZ)) stop = NONFINITE;
+    else if (RR <= rtol2 * bb) stop = CONVERGED;
+    else if (!(RZ > 0.0)) stop = BREAKDOWN;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (iter == 0) scal[S_BB] = RR;
+        scal[S_RZ0 + (iter & 1)] = RZ;
+        if (stop != RUN) { scal[S_STOP] = stop; scal[S_ITERS] = iter; }
+    }
+    if (stop != RUN) return;
+    const double beta = iter == 0 ? 0.0 : RZ / scal[S_RZ0 + ((iter - 1) & 1)];
+    const int team = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int row = blockIdx.x * SP_ROWS + team;
+    const int g = lane / 12, r = lane % 12;
+    if (row < B && g < 5) {
+        double acc = 0.0;
         for (int k = row_ptr[row] + g; k < row_ptr[row + 1]; k += 5) {
             const double* H = Hval + 144 * (size_t)k;
-            const double* x = p + 12 * (size_t)col_idx[k];
+            const size_t cb = 12 * (size_t)col_idx[k];
             double s = 0.0;
 #pragma unroll
-            for (int c = 0; c < 12; c++) s += H[c * 12 + r] * x[c];
+            for (int c = 0; c < 12; c++) s += H[c * 12 + r] * (z[cb + c] + beta * pold[cb + c]);
             acc += s;
         }
-        sh[g][r] = acc;
+        acc_sh[team][g][r] = acc;
     }
     __syncthreads();
-    __shared__ double prod[12];
-    if (threadIdx.x < 12) {
-        const int rr = threadIdx.x;
-        double yr = (((sh[0][rr] + sh[1][rr]) + sh[2][rr]) + sh[3][rr]) + sh[4][rr];
-        y[12 * (size_t)row + rr] = yr;
-        prod[rr] = p[12 * (size_t)row + rr] * yr;
+    double prod = 0.0;
+    if (row < B && lane < 12) {
+        const size_t k = 12 * (size_t)row + lane;
+        const double yr = (((acc_sh[team][0][lane] + acc_sh[team][1][lane]) + acc_sh[team][2][lane]) + acc_sh[team][3][lane]) +
+                          acc_sh[team][4][lane];
+        const double pr = z[k] + beta * pold[k];
+        y[k] = yr;
+        pnew[k] = pr;
+        prod = pr * yr;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int i = 0; i < 12; i++) t += prod[i];
-        part[row] = t;
-    }
+    prod = wg_sum(prod, sh);
+    if (threadIdx.x == 0) partA[blockIdx.x] = prod;
 }
 
-// z_b = L^-T L^-1 r_b
-__device__ void precond_body(const double* L, const double* r, double* z) {
-    double t[12];
-    for (int i = 0; i < 12; i++) {
-        double s = r[i];
-        for (int m = 0; m < i; m++) s -= L[m * 12 + i] * t[m];
-        t[i] = s / L[i * 12 + i];
+__global__ __launch_bounds__(UP_T) void k_pcg_update(int B, int iter, int nbS, const double* __restrict__ Minv,
+                                                     const double* __restrict__ p, const double* __restrict__ y,
+                                                     double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
+                                                     const double* __restrict__ partA, double* __restrict__ partB,
+                                                     double* __restrict__ scal) {
+    __shared__ double rs[UP_BODIES][12];
+    __shared__ double sh[4];
+    if (scal[S_STOP] != RUN) return;
+    const double pAp = wave_sum_array(partA, nbS);
+    const double RZ = scal[S_RZ0 + (iter & 1)];
+    if (!(pAp > 0.0) || !std::isfinite(pAp)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { scal[S_STOP] = std::isfinite(pAp) ? BREAKDOWN : NONFINITE; scal[S_ITERS] = iter; }
+        return;
     }
-    for (int i = 11; i >= 0; i--) {
-        double s = t[i];
-        for (int m = i + 1; m < 12; m++) s -= L[i * 12 + m] * z[m];
-        z[i] = s / L[i * 12 + i];
-    }
-}
-
-// initial: x = 0, r = -g (solve H dq = -g), z = M^-1 r, partials r.z, r.r, b.b
-__global__ void k_pcg_init(int B, const double* __restrict__ g, const double* __restrict__ L, double* __restrict__ x,
-                           double* __restrict__ r, double* __restrict__ z, double* __restrict__ p, double* __restrict__ part) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= B) return;
-    double rb[12], zb[12];
-    for (int i = 0; i < 12; i++) { rb[i] = -g[12 * (size_t)b + i]; x[12 * (size_t)b + i] = 0.0; r[12 * (size_t)b + i] = rb[i]; }
-    precond_body(L + 144 * (size_t)b, rb, zb);
+    const double alpha = RZ / pAp;
+    const int lb = threadIdx.x / 12, i = threadIdx.x % 12;
+    const int b = blockIdx.x * UP_BODIES + lb;
     double rz = 0.0, rr = 0.0;
-    for (int i = 0; i < 12; i++) { z[12 * (size_t)b + i] = zb[i]; p[12 * (size_t)b + i] = zb[i]; rz += rb[i] * zb[i]; rr += rb[i] * rb[i]; }
-    part[B + b] = rz;
-    part[2 * B + b] = rr;
-}
-__global__ __launch_bounds__(256) void k_pcg_init_scal(int B, const double* __restrict__ part, double* __restrict__ scal) {
-    __shared__ double sh[256];
-    double rz = block_sum_fixed(part + B, B, sh);
-    double rr = block_sum_fixed(part + 2 * B, B, sh);
-    if (threadIdx.x == 0) { scal[0] = rz; scal[1] = rz; scal[2] = rr; scal[3] = rr; scal[4] = 0.0; }
-}
-// scal: [0..1] rz by parity, [2] rr, [3] bb, [4] breakdown flag
-__global__ __launch_bounds__(256) void k_pcg_update(int B, int par, const double* __restrict__ L, const double* __restrict__ p,
-                                                    const double* __restrict__ y, double* __restrict__ x, double* __restrict__ r,
-                                                    double* __restrict__ z, double* __restrict__ part, double* __restrict__ scal) {
-    __shared__ double sh[256];
-    const double pAp = block_sum_fixed(part, B, sh);
-    const double rz = scal[par];
-    const bool stop = scal[4] != 0.0 || !(pAp > 0.0) || !(rz > 0.0);
-    const double alpha = stop ? 0.0 : rz / pAp;
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= B) return;
-    double rb[12], zb[12];
-    for (int i = 0; i < 12; i++) {
-        size_t k = 12 * (size_t)b + i;
+    if (b < B) {
+        const size_t k = 12 * (size_t)b + i;
         x[k] += alpha * p[k];
-        rb[i] = r[k] - alpha * y[k];
-        r[k] = rb[i];
+        const double rk = r[k] - alpha * y[k];
+        r[k] = rk;
+        rs[lb][i] = rk;
     }
-    precond_body(L + 144 * (size_t)b, rb, zb);
-    double rzn = 0.0, rr = 0.0;
-    for (int i = 0; i < 12; i++) { z[12 * (size_t)b + i] = zb[i]; rzn += rb[i] * zb[i]; rr += rb[i] * rb[i]; }
-    part[B + b] = rzn;
-    part[2 * B + b] = rr;
-    if (b == 0 && stop) scal[4] = 1.0;
-}
-__global__ __launch_bounds__(256) void k_pcg_dir(int B, int par, const double* __restrict__ z, double* __restrict__ p,
-                                                 const double* __restrict__ part, double* __restrict__ scal) {
-    __shared__ double sh[256];
-    const double rzn = block_sum_fixed(part + B, B, sh);
-    const double rr = block_sum_fixed(part + 2 * B, B, sh);
-    const double rz = scal[par];
-    const bool stop = scal[4] != 0.0 || !(rz > 0.0);
-    const double beta = stop ? 0.0 : rzn / rz;
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < B && !stop)
-        for (int i = 0; i < 12; i++) { size_t k = 12 * (size_t)b + i; p[k] = z[k] + beta * p[k]; }
-    if (blockIdx.x == 0 && threadIdx.x == 0) { scal[par ^ 1] = rzn; scal[2] = rr; }
+    __syncthreads();
+    if (b < B) {
+        const double* M = Minv + 144 * (size_t)b;
+        double zi = 0.0;
+#pragma unroll
+        for (int j = 0; j < 12; j++) zi += M[12 * j + i] * rs[lb][j];
+        z[12 * (size_t)b + i] = zi;
+        rz = rs[lb][i] * zi; rr = rs[lb][i] * rs[lb][i];
+    }
+    rz = wg_sum(rz, sh);
+    rr = wg_sum(rr, sh);
+    if (threadIdx.x == 0) { partB[blockIdx.x] = rz; partB[gridDim.x + blockIdx.x] = rr; }
 }
 
 int pcg_solve(World& w, double* dq_out, int* iters) {
     const int B = w.B, n = 12 * B;
-    w.pcg_x.reserve(n); w.pcg_r.reserve(n); w.pcg_z.reserve(n); w.pcg_p.reserve(n); w.pcg_Ap.reserve(n);
-    w.partial.reserve(3 * (size_t)B); w.scal.reserve(8);
-    const int nb = (B + 255) / 256;
-    hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(256), 0, w.stream, B, w.grad.p, w.Lchol.p, w.pcg_x.p, w.pcg_r.p, w.pcg_z.p, w.pcg_p.p, w.partial.p);
-    hipLaunchKernelGGL(k_pcg_init_scal, dim3(1), dim3(256), 0, w.stream, B, w.partial.p, w.scal.p);
+    const int nbU = (B + UP_BODIES - 1) / UP_BODIES;
+    const int nbS = (B + SP_ROWS - 1) / SP_ROWS;
+    const double rtol2 = w.P.pcg_rtol * w.P.pcg_rtol;
+    w.pcg_x.reserve(n); w.pcg_r.reserve(n); w.pcg_z.reserve(n); w.pcg_p.reserve(n); w.pcg_p2.reserve(n); w.pcg_Ap.reserve(n);
+    w.partial.reserve((size_t)nbS + 2 * (size_t)nbU); w.scal.reserve(S_COUNT);
+    double* partA = w.partial.p;
+    double* partB = w.partial.p + nbS;
+    if (!w.h_scal) ADB_HIP(hipHostMalloc((void**)&w.h_scal, 2 * S_COUNT * sizeof(double), hipHostMallocDefault));
+    for (hipEvent_t& e : w.pcg_ev)
+        if (!e) ADB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    double* pb[2] = {w.pcg_p.p, w.pcg_p2.p};
+    hipLaunchKernelGGL(k_pcg_init, dim3(nbU), dim3(UP_T), 0, w.stream, B, w.grad.p, w.Pinv.p, w.pcg_x.p, w.pcg_r.p, w.pcg_z.p, pb[0],
+                       partB, w.scal.p);
     ADB_HIP(hipGetLastError());
-    double h[5];
-    ADB_HIP(hipMemcpyAsync(h, w.scal.p, 5 * sizeof(double), hipMemcpyDeviceToHost, w.stream));
-    ADB_HIP(hipStreamSynchronize(w.stream));
-    const double bb = h[3];
-    if (!std::isfinite(bb)) {   // NaN / inf in the gradient: the reference aborts (model/abd.cpp:368-371)
-        if (iters) *iters = 0;
-        return ADB_E_NAN;
-    }
-    const double tol2 = w.P.pcg_rtol * w.P.pcg_rtol * bb;
-    int it = 0, par = 0;
-    int status = ADB_OK;
-    if (bb > 0.0) {
-        while (it < w.P.pcg_max_iter) {
-            for (int k = 0; k < CHECK; k++) {
-                {
-                    ProfScope ps(w, PK_SPMV, 1152.0 * w.nnzb + 4.0 * w.nnzb + 2.0 * 96.0 * B);
-                    hipLaunchKernelGGL(k_spmv, dim3(B), dim3(64), 0, w.stream, B, w.row_ptr.p, w.col_idx.p, w.Hval.p, w.pcg_p.p, w.pcg_Ap.p, w.partial.p);
-                }
-                hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(256), 0, w.stream, B, par, w.Lchol.p, w.pcg_p.p, w.pcg_Ap.p,
-                                   w.pcg_x.p, w.pcg_r.p, w.pcg_z.p, w.partial.p, w.scal.p);
-                hipLaunchKernelGGL(k_pcg_dir, dim3(nb), dim3(256), 0, w.stream, B, par, w.pcg_z.p, w.pcg_p.p, w.partial.p, w.scal.p);
-                par ^= 1;
-                it++;
+    int it = 0, slot = 0;
+    bool have_prev = false;
+    const int max_it = std::max(w.P.pcg_max_iter, 0);
+    auto spmv = [&](int k) {
+        // H blocks, their column indices, z and p_old read, p_new and y written
+        ProfScope ps(w, PK_SPMV, 1152.0 * w.nnzb + 4.0 * w.nnzb + 4.0 * 96.0 * B);
+        hipLaunchKernelGGL(k_pcg_spmv, dim3(nbS), dim3(64 * SP_ROWS), 0, w.stream, B, k, rtol2, nbU, w.row_ptr.p, w.col_idx.p,
+                           w.Hval.p, w.pcg_z.p, pb[k & 1], pb[(k & 1) ^ 1], w.pcg_Ap.p, partB, partA, w.scal.p);
+    };
+    while (it < max_it) {
+        const int nb = std::min(BATCH, max_it - it);
+        for (int k = it; k < it + nb; k++) {
+            spmv(k);
+            {
+                // M^-1 blocks read; p, y read; x, r read and written; z written
+                ProfScope ps(w, PK_PCG_UPDATE, 1152.0 * B + 7.0 * 96.0 * B);
+                hipLaunchKernelGGL(k_pcg_update, dim3(nbU), dim3(UP_T), 0, w.stream, B, k, nbS, w.Pinv.p, pb[(k & 1) ^ 1],
+                                   w.pcg_Ap.p, w.pcg_x.p, w.pcg_r.p, w.pcg_z.p, partA, partB, w.scal.p);
             }
-            ADB_HIP(hipGetLastError());
-            ADB_HIP(hipMemcpyAsync(h, w.scal.p, 5 * sizeof(double), hipMemcpyDeviceToHost, w.stream));
-            ADB_HIP(hipStreamSynchronize(w.stream));
-            if (h[2] <= tol2 || h[4] != 0.0) break;
         }
-        if (!(h[2] <= tol2) && h[4] == 0.0) status = ADB_E_NOT_CONVERGED;
+        it += nb;
+        ADB_HIP(hipGetLastError());
+        ADB_HIP(hipMemcpyAsync(w.h_scal + S_COUNT * slot, w.scal.p, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipEventRecord(w.pcg_ev[slot], w.stream));
+        if (have_prev) {
+            // the batch before this one: stopped already?
+            ADB_HIP(hipEventSynchronize(w.pcg_ev[slot ^ 1]));
+            if (w.h_scal[S_COUNT * (slot ^ 1) + S_STOP] != RUN) break;
+        }
+        have_prev = true;
+        slot ^= 1;
     }
-    if (iters) *iters = it;
+    // stop test after the last queued update (no-op when stopped earlier)
+    spmv(it);
+    ADB_HIP(hipGetLastError());
+    ADB_HIP(hipMemcpyAsync(w.h_scal, w.scal.p, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, w.stream));
+    ADB_HIP(hipStreamSynchronize(w.stream));
+    const double* h = w.h_scal;
+    const int stop = (int)h[S_STOP];
+    if (iters) *iters = stop == RUN ? it : (int)h[S_ITERS];
+    if (stop == NONFINITE) return ADB_E_NAN;
     ADB_HIP(hipMemcpyAsync(dq_out, w.pcg_x.p, n * sizeof(double), hipMemcpyDeviceToDevice, w.stream));
-    return status;
+    if (stop == RUN) return ADB_E_NOT_CONVERGED;
+    return ADB_OK;
 }
 
 } // namespace adb

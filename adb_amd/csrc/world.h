@@ -99,7 +99,7 @@ struct World {
     DBuf<unsigned long long> keys_a, keys_b;  // unsorted / sorted keys
     DBuf<unsigned char> sort_tmp;
     DBuf<unsigned long long> toi_bits;
-    DBuf<int2> ovf;                  // body pairs whose LDS lists overflowed
+
     DBuf<unsigned long long> vg_keys;
     ContactSet cs, cs_new;
     // ---- contact terms (per PT/EE contact, then per VG contact)
@@ -124,10 +124,12 @@ struct World {
     DBuf<double> chunk_part;                   // 156 per chunk (block + gradient)
     DBuf<double> Hval;               // 144 per block
     DBuf<double> grad;               // 12 B
-    DBuf<double> Lchol;              // 144 per body (block-Jacobi Cholesky factors)
+    DBuf<double> Pinv;               // 144 per body: inverse diagonal blocks (block-Jacobi)
     int nnzb = 0, n_contrib = 0;
     // ---- PCG
-    DBuf<double> pcg_x, pcg_r, pcg_z, pcg_p, pcg_Ap, partial, scal;
+    DBuf<double> pcg_x, pcg_r, pcg_z, pcg_p, pcg_p2, pcg_Ap, partial, scal;
+    double* h_scal = nullptr;        // pinned host copies of the PCG scalars (2 x 8)
+    hipEvent_t pcg_ev[2] = {nullptr, nullptr};
     // ---- energies
     DBuf<double> epart;
     // ---- host mirrors

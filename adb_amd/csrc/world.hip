@@ -257,9 +257,9 @@ static void free_world(World& w) {
     for (auto* b : ib) b->free_();
     DBuf<double>* db[] = {&w.xbar, &w.mass, &w.Ic, &w.q, &w.q0, &w.qdot, &w.dq, &w.qtrial, &w.X, &w.Xs, &w.X1, &w.bbox,
                           &w.crec, &w.cdist, &w.chunk_part, &w.ctk, &w.clam, &w.vgK, &w.vglam, &w.bodyH, &w.bodyG, &w.Hval, &w.grad,
-                          &w.Lchol, &w.pcg_x, &w.pcg_r, &w.pcg_z, &w.pcg_p, &w.pcg_Ap, &w.partial, &w.scal, &w.epart};
+                          &w.Pinv, &w.pcg_x, &w.pcg_r, &w.pcg_z, &w.pcg_p, &w.pcg_p2, &w.pcg_Ap, &w.partial, &w.scal, &w.epart};
     for (auto* b : db) b->free_();
-    w.pairs.free_(); w.ovf.free_(); w.keys_a.free_(); w.keys_b.free_(); w.sort_tmp.free_(); w.toi_bits.free_(); w.vg_keys.free_();
+    w.pairs.free_(); w.keys_a.free_(); w.keys_b.free_(); w.sort_tmp.free_(); w.toi_bits.free_(); w.vg_keys.free_();
     w.akeys.free_(); w.akeys_s.free_();
     for (ContactSet* c : {&w.cs, &w.cs_new}) { c->pt.free_(); c->ee.free_(); c->vg.free_(); }
     for (auto& r : w.prof.pending) { w.prof.pool.push_back(r.a); w.prof.pool.push_back(r.b); }
@@ -268,6 +268,9 @@ static void free_world(World& w) {
     w.prof.pool.clear();
     if (w.prof.snap) (void)hipHostFree(w.prof.snap);
     w.prof.snap = nullptr;
+    if (w.h_scal) (void)hipHostFree(w.h_scal);
+    w.h_scal = nullptr;
+    for (hipEvent_t& e : w.pcg_ev) { if (e) (void)hipEventDestroy(e); e = nullptr; }
     if (w.stream) (void)hipStreamDestroy(w.stream);
     w.stream = nullptr;
 }

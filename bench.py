@@ -69,6 +69,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-json", default="")
+    ap.add_argument("--copy-mb", type=float, default=0.0,
+                    help="first run one plain device-to-device copy of this many MB (a known-size kernel "
+                         "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the contact step of one scene runs on one device "
@@ -76,6 +79,14 @@ def main():
         sys.exit(2)
 
     from adb_amd.world import World
+    if args.copy_mb > 0:
+        import torch
+        n = int(args.copy_mb * 1e6 / 8)
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     sc = workload(args.bodies)
     p = params(args.precision)
     w = World(sc, p)

@@ -32,23 +32,38 @@ def small_spheres(n=12, seed=5):
     return scenes.sphere_pack(n, seed=seed, subdiv=2, rmin=0.2, rmax=0.3, half=0.8, ytop=1.5, gap=0.02, vsigma=0.3)
 
 
+def big_pair():
+    # two subdivision-3 icospheres (642 / 1280 / 1920) nearly touching, with a
+    # d_hat band wide enough that more primitives per side meet the overlap
+    # region than one LDS staging round holds (exercises the streamed rounds)
+    v, f, e = scenes.icosphere_mesh(3, 0.1)
+    rng = scenes.PCG32(9)
+    bodies = [dict(verts=v, tris=f, edges=e, R=rng.rotation(), p=(0.0, 0.0, 0.0)),
+              dict(verts=v, tris=f, edges=e, R=rng.rotation(), p=(0.004, 0.212, 0.003))]
+    return scenes.build_scene(bodies)
+
+
 SCENES = {
     "random_cubes": (lambda: scenes.random_cubes(50, seed=7, box=2.0), dict(d_hat=0.05)),
     "cube_stack": (lambda: scenes.cube_stack(10, seed=1), dict(d_hat=1e-3, mu=0.5)),
     "spheres": (lambda: small_spheres(), dict(d_hat=4e-3, mu=0.3)),
 }
+# contact sets and step bounds: every scene with FP64 and FP32 (outward-rounded)
+# broad-phase boxes; the FP64 predicate decides in both, so both must be exact
+SET_CASES = [(n, fp32) for n in list(SCENES) + ["big_pair"] for fp32 in (0, 1)]
+SET_SCENES = dict(SCENES, big_pair=(big_pair, dict(d_hat=2e-2)))
 
 
 def keys(a):
     return a.tolist()
 
 
-@pytest.mark.parametrize("name", list(SCENES))
-def test_collision_sets_bit_exact(name):
+@pytest.mark.parametrize("name,fp32", SET_CASES)
+def test_collision_sets_bit_exact(name, fp32):
     _require_gpu()
-    mk, kw = SCENES[name]
+    mk, kw = SET_SCENES[name]
     sc = mk()
-    p = default_params(**kw)
+    p = default_params(broad_phase_fp32=fp32, **kw)
     gw, ow = World(sc, p), O.OracleWorld(sc, p)
     for vtn, dq in [(1, None), (2, np.random.default_rng(3).normal(0, 0.02, size=(len(sc["mass"]), 12)))]:
         g = gw.collision_set(vtn, dq)
@@ -58,12 +73,12 @@ def test_collision_sets_bit_exact(name):
     assert len(o[0]) + len(o[1]) > 0
 
 
-@pytest.mark.parametrize("name", list(SCENES))
-def test_ccd_bit_exact(name):
+@pytest.mark.parametrize("name,fp32", SET_CASES)
+def test_ccd_bit_exact(name, fp32):
     _require_gpu()
-    mk, kw = SCENES[name]
+    mk, kw = SET_SCENES[name]
     sc = mk()
-    p = default_params(**kw)
+    p = default_params(broad_phase_fp32=fp32, **kw)
     gw, ow = World(sc, p), O.OracleWorld(sc, p)
     rng = np.random.default_rng(11)
     hits = 0
