@@ -142,42 +142,6 @@ __global__ void k_chunk_owner(int nnzb, const int* __restrict__ chunk_off, int* 
 }
 // 144 block entries + 12 gradient entries of one wave, lane t owning t, t + 64, t + 128
 struct Acc156 { double e[3]; double g; };
-__device__ __forceinline__ void add_contribution(const ReduceArgs& a, int v, int t, Acc156& A) {
-    if (v < 0) {     // diagonal marker: inertia terms
-        const int b = -1 - v;
-        for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) A.e[u] += a.bodyH[144 * (size_t)b + e]; }
-        if (t < 12) A.g += a.bodyG[12 * (size_t)b + t];
-        return;
-    }
-    const int c = v / 4, slot = v % 4;
-    const int4 ct = c < a.n_pt ? a.pt[c] : a.ee[c - a.n_pt];
-    const bool di = a.mass[ct.x] > 0.0, dj = a.mass[ct.z] > 0.0;
-    const bool use = slot == 0 ? di : (slot == 1 ? dj : (di && dj));
-    const double* R = a.rec + REC * (size_t)c;
-    const int D = use ? (int)R[R_D] : 0;
-    if (D == 0) return;
-    // block (s1, s2): slot 0 = (I, I), 1 = (J, J), 2 = (I, J), 3 = (J, I)
-    const int s1 = (slot == 1 || slot == 3) ? 1 : 0, s2 = (slot == 1 || slot == 2) ? 1 : 0;
-    const int nA = D / 3;
-    for (int u = 0; u < 3; u++) {
-        const int e = t + 64 * u;
-        if (e < 144) {
-            // column (beta, cc), row (alpha, r)
-            const int col = e / 12, row = e % 12;
-            const int be = col / 3, cc = col % 3, al = row / 3, r = row % 3;
-            double val = R[R_G + 4 * s1 + al] * R[R_G + 4 * s2 + be] * R[R_F + 3 * r + cc];
-            for (int aa = 0; aa < nA; aa++) {
-                const double za = R[R_Z + 8 * aa + 4 * s1 + al];
-                double sb = 0.0;
-                for (int bb = 0; bb < nA; bb++)
-                    sb += R[R_Z + 8 * bb + 4 * s2 + be] * R[R_PK + (3 * aa + r) * D + 3 * bb + cc];
-                val += za * sb;
-            }
-            A.e[u] += val;
-        }
-    }
-    if (slot < 2 && t < 12) A.g += R[R_GRAD + 12 * s1 + t];
-}
 // fixed-order combine of the RW wave accumulators into wave 0
 __device__ __forceinline__ void combine_waves(Acc156& A, double (*sh)[156]) {
     const int w = threadIdx.x / 64, t = threadIdx.x % 64;
@@ -193,17 +157,114 @@ __device__ __forceinline__ void combine_waves(Acc156& A, double (*sh)[156]) {
         }
     }
 }
+// LDS writes of one wave visible to its own lanes
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+constexpr int RG = 8;            // contributions staged in LDS per wave at once
+// Each wave sums RCH / RW consecutive contributions of the chunk.  They are
+// taken RG at a time: every lane reads the RG contribution ids and record
+// headers (broadcast loads), the wave copies the RG records (or inertia
+// blocks) into LDS with independent coalesced loads, then every lane expands
+// its entries from LDS:
+//   block(s1, s2)[(al, r), (be, c)] = sum_ab Z[a][s1][al] Z[b][s2][be] PK[(3a + r) D + 3b + c]
+//                                   + G[s1][al] G[s2][be] F[3r + c]
 __global__ __launch_bounds__(64 * RW) void k_reduce_chunks(ReduceArgs a, const int* __restrict__ chunk_off,
                                                            const int* __restrict__ chunk_blk, double* __restrict__ part) {
     __shared__ double sh[RW - 1][156];
+    __shared__ double stage[RW][RG][160];
+    __shared__ int meta[RW][RG][4];     // kind (0 skip, 1 contact, 2 inertia), D, side bits, contact / body id
     const int ch = blockIdx.x;
     if (ch >= chunk_off[a.nnzb]) return;
     const int k = chunk_blk[ch];
     const int beg = a.blk_start[k] + (ch - chunk_off[k]) * RCH;
     const int end = min(beg + RCH, a.blk_start[k + 1]);
     const int w = threadIdx.x / 64, t = threadIdx.x % 64;
+    const int wb = min(beg + w * (RCH / RW), end), we = min(wb + RCH / RW, end);
     Acc156 A = {{0.0, 0.0, 0.0}, 0.0};
-    for (int s = beg + w; s < end; s += RW) add_contribution(a, a.vals[s], t, A);
+    double (*S)[160] = stage[w];
+    int (*M)[4] = meta[w];
+    for (int g0 = wb; g0 < we; g0 += RG) {
+        const int ng = min(RG, we - g0);
+        if (t < RG) {
+            int kind = 0, D = 0, sides = 0, x = 0;
+            if (t < ng) {
+                const int v = a.vals[g0 + t];
+                if (v < 0) { kind = 2; x = -1 - v; }
+                else {
+                    const int c = v / 4, slot = v % 4;
+                    const int code = (int)a.rec[REC * (size_t)c + R_D];
+                    const int Dd = code % 100, di = (code / 100) & 1, dj = (code / 100) >> 1;
+                    const bool use = slot == 0 ? di : (slot == 1 ? dj : (di && dj));
+                    if (use && Dd > 0) {
+                        kind = 1; D = Dd; x = c;
+                        // bit 0: s1, bit 1: s2, bit 2: gradient (slot 0 = (I, I), 1 = (J, J), 2 = (I, J), 3 = (J, I))
+                        sides = ((slot == 1 || slot == 3) ? 1 : 0) | ((slot == 1 || slot == 2) ? 2 : 0) | (slot < 2 ? 4 : 0);
+                    }
+                }
+            }
+            M[t][0] = kind; M[t][1] = D; M[t][2] = sides; M[t][3] = x;
+        }
+        wave_sync();
+        {
+            // all loads first (independent), then the LDS stores
+            double tmp[RG][3];
+#pragma unroll
+            for (int j = 0; j < RG; j++) {
+                const int kind = j < ng ? M[j][0] : 0, x = M[j][3];
+                const int len = kind == 1 ? R_PK + M[j][1] * M[j][1] : (kind == 2 ? 156 : 0);
+#pragma unroll
+                for (int u = 0; u < 3; u++) {
+                    const int e = t + 64 * u;
+                    double v = 0.0;
+                    if (e < len) {
+                        if (kind == 1) v = a.rec[REC * (size_t)x + e];
+                        else v = e < 144 ? a.bodyH[144 * (size_t)x + e] : a.bodyG[12 * (size_t)x + e - 144];
+                    }
+                    tmp[j][u] = v;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < RG; j++)
+#pragma unroll
+                for (int u = 0; u < 3; u++) {
+                    const int e = t + 64 * u;
+                    if (e < 160) S[j][e] = tmp[j][u];
+                }
+        }
+        wave_sync();
+#pragma unroll 1
+        for (int j = 0; j < ng; j++) {
+            const int kind = M[j][0];
+            const double* Rj = S[j];
+            if (kind == 2) {
+                for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) A.e[u] += Rj[e]; }
+                if (t < 12) A.g += Rj[144 + t];
+            } else if (kind == 1) {
+                const int Dj = M[j][1], nA = Dj / 3, sides = M[j][2], a1 = sides & 1, a2 = (sides >> 1) & 1;
+                for (int u = 0; u < 3; u++) {
+                    const int e = t + 64 * u;
+                    if (e < 144) {
+                        // column (be, cc), row (al, r)
+                        const int col = e / 12, row = e % 12;
+                        const int be = col / 3, cc = col % 3, al = row / 3, r = row % 3;
+                        double val = Rj[R_G + 4 * a1 + al] * Rj[R_G + 4 * a2 + be] * Rj[R_F + 3 * r + cc];
+                        for (int aa = 0; aa < nA; aa++) {
+                            double sb = 0.0;
+                            for (int bb = 0; bb < nA; bb++)
+                                sb += Rj[R_Z + 8 * bb + 4 * a2 + be] * Rj[R_PK + (3 * aa + r) * Dj + 3 * bb + cc];
+                            val += Rj[R_Z + 8 * aa + 4 * a1 + al] * sb;
+                        }
+                        A.e[u] += val;
+                    }
+                }
+                if ((sides & 4) && t < 12) A.g += Rj[R_GRAD + 12 * a1 + t];
+            }
+        }
+        wave_sync();
+    }
     combine_waves(A, sh);
     if (w == 0) {
         double* P = part + 156 * (size_t)ch;

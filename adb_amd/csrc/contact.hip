@@ -319,79 +319,6 @@ __device__ __forceinline__ void barrier_column(const Geo& G, int j, double (*H)[
     gj = G.is_ee ? G.m * gdj * G.B1 + gmj * G.B0 : gdj * G.B1;
 }
 
-// Helmert-coordinate Hessian of a class-3 contact, K = (C (x) I3) Mred
-// (C^T (x) I3) + mollifier terms, with C = h4 S^T (S: signs of the points in
-// the reduced vectors) and Cm the same for the mollifier vectors y = p1 - p0,
-// z = p3 - p2.  Lane (i, j) < 9 writes K[(a, i), (b, j)] for a, b < 3 into
-// S.A; no vertex-space matrix is formed.
-__device__ __forceinline__ void helmert_k9(const Geo& G, int lane, GroupLDS& L) {
-    if (lane >= 9) return;
-    const int i = lane / 3, j = lane % 3;
-    double C[3][3], Cm[3][2];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-#pragma unroll
-        for (int m = 0; m < 3; m++) {
-            double c = 0.0;
-#pragma unroll
-            for (int k = 0; k < 4; k++) c += kHel[a][k] * red_sign(G.map, m, k);
-            C[a][m] = m < G.map.nr ? c : 0.0;
-        }
-        Cm[a][0] = kHel[a][1] - kHel[a][0];
-        Cm[a][1] = kHel[a][3] - kHel[a][2];
-    }
-    double K[3][3];
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int b = 0; b < 3; b++) K[a][b] = 0.0;
-#pragma unroll
-    for (int m = 0; m < 3; m++)
-#pragma unroll
-        for (int n = 0; n < 3; n++) {
-            if (m < G.map.nr && n < G.map.nr) {
-                const double Mv = G.B1 * G.rd.hess(m, i, n, j) + G.B2 * cmp(G.rd.g[m], i) * cmp(G.rd.g[n], j);
-#pragma unroll
-                for (int a = 0; a < 3; a++)
-#pragma unroll
-                    for (int b = 0; b < 3; b++) K[a][b] += C[a][m] * C[b][n] * Mv;
-            }
-        }
-    if (G.is_ee) {
-        double gdi[3], gdj[3], gxi[3], gxj[3];
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            double si = 0.0, sj = 0.0;
-#pragma unroll
-            for (int m = 0; m < 3; m++)
-                if (m < G.map.nr) { si += C[a][m] * cmp(G.rd.g[m], i); sj += C[a][m] * cmp(G.rd.g[m], j); }
-            gdi[a] = si; gdj[a] = sj;
-            gxi[a] = G.m2 != 0.0 ? Cm[a][0] * G.mc.grad(0, i) + Cm[a][1] * G.mc.grad(1, i) : 0.0;
-            gxj[a] = G.m2 != 0.0 ? Cm[a][0] * G.mc.grad(0, j) + Cm[a][1] * G.mc.grad(1, j) : 0.0;
-        }
-        double hx[2][2];
-#pragma unroll
-        for (int q = 0; q < 2; q++)
-#pragma unroll
-            for (int r = 0; r < 2; r++) hx[q][r] = G.m2 != 0.0 ? G.mc.hess(q, i, r, j) : 0.0;
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-#pragma unroll
-            for (int b = 0; b < 3; b++) {
-                double hxk = 0.0;
-#pragma unroll
-                for (int q = 0; q < 2; q++)
-#pragma unroll
-                    for (int r = 0; r < 2; r++) hxk += Cm[a][q] * Cm[b][r] * hx[q][r];
-                const double gmi = G.m1 * gxi[a], gmj = G.m1 * gxj[b];
-                K[a][b] = G.m * K[a][b] + G.B1 * (gmi * gdj[b] + gdi[a] * gmj) + G.B0 * (G.m2 * gxi[a] * gxj[b] + G.m1 * hxk);
-            }
-    }
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int b = 0; b < 3; b++) L.A[3 * a + i][3 * b + j] = K[a][b];
-}
 // vertex-space gradient entry j = (point, component) < 12 (model/core_ipc.cpp:36-43,70-80)
 __device__ __forceinline__ double barrier_grad(const Geo& G, int j) {
     const int l = j / 3, c = j % 3;
@@ -533,11 +460,18 @@ struct WorldArgs {
     int n_pt, n_ee;
     const int4* pt; const int4* ee;
     const int* voff; const int* foff; const int* eoff; const int* face; const int* edge;
-    const double* X; const double* Xs; const double* xbar;
+    const double* X; const double* Xs; const double* xbar; const double* mass;
     int* ctype; double* cdist; int* list; int* list3; int* counts;
     double* rec; double* tk; double* lam;
 };
 
+// R[R_D] = D + 100 (dynamic(I) + 2 dynamic(J)): the assembly drops the
+// blocks of static bodies (mass < 0, model/assemble_matrices.cpp:43-58)
+__device__ __forceinline__ double rec_code(const WorldArgs& a, int cid, int D) {
+    const int4 c = cid < a.n_pt ? a.pt[cid] : a.ee[cid - a.n_pt];
+    const int di = a.mass[c.x] > 0.0 ? 1 : 0, dj = a.mass[c.z] > 0.0 ? 1 : 0;
+    return (double)(D + 100 * (di + 2 * dj));
+}
 __device__ __forceinline__ bool contact_ids(const WorldArgs& a, int cid, int& id0, int& id1, int& id2, int& id3) {
     if (cid < a.n_pt) {
         const int4 c = a.pt[cid];
@@ -794,7 +728,7 @@ __device__ __forceinline__ void contact_small(const WorldArgs& a, int cid) {
 #pragma unroll
                 for (int j = 0; j < 3; j++) R[R_PK + 3 * i + j] = (i == j ? lp : 0.0) + 8.0 * B2 * cmp(r, i) * cmp(r, j);
         }
-        R[R_D] = 3.0;
+        R[R_D] = rec_code(a, cid, 3);
 #pragma unroll
         for (int k = 0; k < 4; k++) { hz[k][0] = k == ka ? HS2 : (k == kb ? -HS2 : 0.0); hz[k][1] = 0.0; }
         const d3 g = mk(2 * r.x * B1, 2 * r.y * B1, 2 * r.z * B1);
@@ -858,7 +792,7 @@ __device__ __forceinline__ void contact_small(const WorldArgs& a, int cid) {
 #pragma unroll
                 for (int q = 0; q < 6; q++) R[R_PK + 6 * p + q] = K[p][q];
         }
-        R[R_D] = 6.0;
+        R[R_D] = rec_code(a, cid, 6);
         // Helmert rows over the slots (m, a, b): (1, -1, 0) / sqrt 2, (1, 1, -2) / sqrt 6
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -889,6 +823,184 @@ __device__ __forceinline__ void contact_small(const WorldArgs& a, int cid) {
         gv[k] = mk(gv[k].x + gmv[k] * ft.x, gv[k].y + gmv[k] * ft.y, gv[k].z + gmv[k] * ft.z);
     write_grad(a, cid, nside0, w, gv);
 }
+// Class 3 contacts, one thread each: the 9x9 Helmert-coordinate matrix
+// K = (C (x) I3) Mred (C^T (x) I3) + mollifier terms (C = h4 S^T, Cm the same
+// for y = p1 - p0, z = p3 - p2) is built one component pair at a time; cyclic
+// Jacobi then rotates its packed upper triangle while the eigenvectors
+// accumulate, both held in LDS element-major (X[e][lane]) so lanes never
+// conflict.
+__host__ __device__ constexpr int P9(int p, int q) {
+    return p <= q ? p * 9 - p * (p - 1) / 2 + (q - p) : q * 9 - q * (q - 1) / 2 + (p - q);
+}
+__device__ __forceinline__ void contact_c3(const WorldArgs& a, int cid, double (*V)[64], double (*As)[64], int lane) {
+    double* R = a.rec + REC * (size_t)cid;
+    {
+        Geo G;
+        int id0, id1, id2, id3;
+        G.is_ee = contact_ids(a, cid, id0, id1, id2, id3);
+        G.p0 = ld3(a.X + 3 * (size_t)id0); G.p1 = ld3(a.X + 3 * (size_t)id1);
+        G.p2 = ld3(a.X + 3 * (size_t)id2); G.p3 = ld3(a.X + 3 * (size_t)id3);
+        G.type = a.ctype[cid]; G.d = a.cdist[cid];
+        geo_init(G, a.kp);
+        double C[3][3], Cm[3][2];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+#pragma unroll
+            for (int m = 0; m < 3; m++) {
+                double c = 0.0;
+#pragma unroll
+                for (int k = 0; k < 4; k++) c += kHel[q][k] * red_sign(G.map, m, k);
+                C[q][m] = m < G.map.nr ? c : 0.0;
+            }
+            Cm[q][0] = kHel[q][1] - kHel[q][0];
+            Cm[q][1] = kHel[q][3] - kHel[q][2];
+        }
+        // K[(q, i), (b, j)] for one component pair (i, j) per iteration, into V (LDS)
+#pragma unroll 1
+        for (int ij = 0; ij < 9; ij++) {
+            const int i = ij / 3, j = ij % 3;
+            double K[3][3];
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) K[q][b] = 0.0;
+#pragma unroll
+            for (int m = 0; m < 3; m++)
+#pragma unroll
+                for (int n = 0; n < 3; n++) {
+                    if (m < G.map.nr && n < G.map.nr) {
+                        const double Mv = G.B1 * G.rd.hess(m, i, n, j) + G.B2 * cmp(G.rd.g[m], i) * cmp(G.rd.g[n], j);
+#pragma unroll
+                        for (int q = 0; q < 3; q++)
+#pragma unroll
+                            for (int b = 0; b < 3; b++) K[q][b] += C[q][m] * C[b][n] * Mv;
+                    }
+                }
+            if (G.is_ee) {
+                double gdi[3], gdj[3], gxi[3], gxj[3];
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    double si = 0.0, sj = 0.0;
+#pragma unroll
+                    for (int m = 0; m < 3; m++)
+                        if (m < G.map.nr) { si += C[q][m] * cmp(G.rd.g[m], i); sj += C[q][m] * cmp(G.rd.g[m], j); }
+                    gdi[q] = si; gdj[q] = sj;
+                    gxi[q] = G.m2 != 0.0 ? Cm[q][0] * G.mc.grad(0, i) + Cm[q][1] * G.mc.grad(1, i) : 0.0;
+                    gxj[q] = G.m2 != 0.0 ? Cm[q][0] * G.mc.grad(0, j) + Cm[q][1] * G.mc.grad(1, j) : 0.0;
+                }
+#pragma unroll
+                for (int q = 0; q < 3; q++)
+#pragma unroll
+                    for (int b = 0; b < 3; b++) {
+                        double hxk = 0.0;
+                        if (G.m2 != 0.0) {
+#pragma unroll
+                            for (int u = 0; u < 2; u++)
+#pragma unroll
+                                for (int v = 0; v < 2; v++) hxk += Cm[q][u] * Cm[b][v] * G.mc.hess(u, i, v, j);
+                        }
+                        const double gmi = G.m1 * gxi[q], gmj = G.m1 * gxj[b];
+                        K[q][b] = G.m * K[q][b] + G.B1 * (gmi * gdj[b] + gdi[q] * gmj) + G.B0 * (G.m2 * gxi[q] * gxj[b] + G.m1 * hxk);
+                    }
+            }
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) V[9 * (3 * q + i) + 3 * b + j][lane] = K[q][b];
+        }
+        // everything but PK: Z, friction (F, G, Tk, lambda) and the gradient
+        const int nside0 = G.is_ee ? 2 : 1;
+        double w[4][4];
+        load_w(a.xbar, id0, id1, id2, id3, w);
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+#pragma unroll
+            for (int s = 0; s < 2; s++)
+#pragma unroll
+                for (int al = 0; al < 4; al++) {
+                    double zz = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        if ((k < nside0) == (s == 0)) zz += kHel[q][k] * w[k][al];
+                    R[R_Z + 8 * q + 4 * s + al] = zz;
+                }
+        Fric f;
+        friction_eval(a, G.is_ee, G.type, G.d, G.B1, G.p0, G.p1, G.p2, G.p3, id0, id1, id2, id3, f);
+        write_fric(a, cid, nside0, f, w);
+        const double gmv[4] = {f.gm0, f.gm1, f.gm2, f.gm3};
+        d3 gv[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            double g3[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                double g = barrier_grad(G, 3 * k + c);
+                if (f.on) g += f.fs * (gmv[k] * cmp(f.c0, c) * f.u0 + gmv[k] * cmp(f.c1, c) * f.u1);
+                g3[c] = g;
+            }
+            gv[k] = mk(g3[0], g3[1], g3[2]);
+        }
+        write_grad(a, cid, nside0, w, gv);
+        R[R_D] = rec_code(a, cid, 9);
+    }
+    // PK: packed K into As, V reused for the eigenvectors; runtime loops
+    // keep the register footprint small (both matrices are per-lane LDS columns)
+    for (int p = 0; p < 9; p++)
+        for (int q = p; q < 9; q++) As[P9(p, q)][lane] = V[9 * p + q][lane];
+    if (a.kp.psd) {
+        for (int e = 0; e < 81; e++) V[e][lane] = (e % 10 == 0) ? 1.0 : 0.0;
+        for (int sweep = 0; sweep < MAX_SWEEPS; sweep++) {
+            double off = 0.0, tot = 0.0;
+            for (int p = 0; p < 9; p++)
+                for (int q = p; q < 9; q++) {
+                    const double x = As[P9(p, q)][lane] * As[P9(p, q)][lane];
+                    if (p == q) tot += x; else { off += 2.0 * x; tot += 2.0 * x; }
+                }
+            if (off <= 1e-30 * tot || off == 0.0) break;
+            for (int p = 0; p < 8; p++)
+                for (int q = p + 1; q < 9; q++) {
+                    const double apq = As[P9(p, q)][lane];
+                    if (apq == 0.0) continue;
+                    const double app = As[P9(p, p)][lane], aqq = As[P9(q, q)][lane];
+                    const double tau = (aqq - app) / (2.0 * apq);
+                    const double t = fabs(tau) > 1e150 ? 0.5 / tau : (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                    const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+                    for (int k = 0; k < 9; k++) {
+                        if (k == p || k == q) continue;
+                        const int ip = P9(k, p), iq = P9(k, q);
+                        const double akp = As[ip][lane], akq = As[iq][lane];
+                        As[ip][lane] = c * akp - s * akq;
+                        As[iq][lane] = s * akp + c * akq;
+                    }
+                    As[P9(p, p)][lane] = c * c * app - 2.0 * c * s * apq + s * s * aqq;
+                    As[P9(q, q)][lane] = s * s * app + 2.0 * c * s * apq + c * c * aqq;
+                    As[P9(p, q)][lane] = 0.0;
+                    for (int k = 0; k < 9; k++) {
+                        const double vkp = V[9 * k + p][lane], vkq = V[9 * k + q][lane];
+                        V[9 * k + p][lane] = c * vkp - s * vkq;
+                        V[9 * k + q][lane] = s * vkp + c * vkq;
+                    }
+                }
+        }
+        for (int p = 0; p < 9; p++)
+            for (int q = p; q < 9; q++) {
+                double s = 0.0;
+                for (int k = 0; k < 9; k++) s += V[9 * p + k][lane] * fmax(As[P9(k, k)][lane], 0.0) * V[9 * q + k][lane];
+                R[R_PK + 9 * p + q] = s;
+                R[R_PK + 9 * q + p] = s;
+            }
+    } else {
+        for (int p = 0; p < 9; p++)
+            for (int q = p; q < 9; q++) { R[R_PK + 9 * p + q] = As[P9(p, q)][lane]; R[R_PK + 9 * q + p] = As[P9(p, q)][lane]; }
+    }
+}
+__global__ __launch_bounds__(64) void k_contact_c3(WorldArgs a) {
+    __shared__ double V[81][64];
+    __shared__ double As[45][64];
+    const int n3 = a.counts[2];
+    for (int p = blockIdx.x * 64 + threadIdx.x; p < n3; p += gridDim.x * 64) contact_c3(a, a.list3[p], V, As, threadIdx.x);
+}
+
 // grid-strided over the class list: class 1 at list[0, n1), class 2 at
 // list[total - n2, total) (stored from the back)
 template <int CLS>
@@ -897,77 +1009,6 @@ __global__ __launch_bounds__(256) void k_contact_small(WorldArgs a) {
     const int cnt = a.counts[CLS - 1];
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < cnt; p += gridDim.x * blockDim.x)
         contact_small<CLS>(a, a.list[CLS == 1 ? p : total - 1 - p]);
-}
-
-// Class 3 contacts: one 16-lane group each (vertex-space barrier Hessian in
-// LDS, Helmert projection to 9x9, parallel Jacobi), grid-strided over the
-// class-3 list.
-__global__ __launch_bounds__(64) void k_contact_group(WorldArgs a) {
-    __shared__ GroupLDS lds[CPW];
-    const int grp = threadIdx.x / GL, lane = threadIdx.x % GL;
-    GroupLDS& L = lds[grp];
-    const int n3 = a.counts[2];
-    for (int base = blockIdx.x * CPW; base < n3; base += gridDim.x * CPW) {
-        const int pos = base + grp;
-        const bool valid = pos < n3;
-        const int cid = valid ? a.list3[pos] : 0;
-        Geo G;
-        int id0 = 0, id1 = 0, id2 = 0, id3 = 0;
-        double gj = 0.0;
-        if (valid) {
-            G.is_ee = contact_ids(a, cid, id0, id1, id2, id3);
-            G.p0 = ld3(a.X + 3 * (size_t)id0); G.p1 = ld3(a.X + 3 * (size_t)id1);
-            G.p2 = ld3(a.X + 3 * (size_t)id2); G.p3 = ld3(a.X + 3 * (size_t)id3);
-            G.type = a.ctype[cid]; G.d = a.cdist[cid];
-            geo_init(G, a.kp);
-            helmert_k9(G, lane, L);
-            if (lane < 12) gj = barrier_grad(G, lane);
-        } else if (lane < 9) {
-            for (int r = 0; r < 9; r++) L.A[r][lane] = 0.0;
-        }
-        __syncthreads();
-        group_psd<true, true>(L, lane, valid && a.kp.psd);
-        Fric f;
-        double w[4][4];
-        const int nside0 = G.is_ee ? 2 : 1;
-        if (valid) {
-            friction_eval(a, G.is_ee, G.type, G.d, G.B1, G.p0, G.p1, G.p2, G.p3, id0, id1, id2, id3, f);
-            if (f.on && lane < 12) {
-                const int kj = lane / 3, ij = lane % 3;
-                const double gk = sel4(f.gm0, f.gm1, f.gm2, f.gm3, kj);
-                gj += f.fs * (gk * cmp(f.c0, ij) * f.u0 + gk * cmp(f.c1, ij) * f.u1);
-            }
-            if (lane < 12) L.g[lane] = gj;
-            double* R = a.rec + REC * (size_t)cid;
-            if (lane < 9) {
-#pragma unroll
-                for (int r = 0; r < 9; r++) R[R_PK + 9 * r + lane] = L.A[r][lane];
-            }
-            if (lane == 0) R[R_D] = 9.0;
-            load_w(a.xbar, id0, id1, id2, id3, w);
-            // Z[a][s][alpha] = sum_{k on side s} h4[a][k] w_k[alpha]: lane = (a, alpha) < 12 writes both sides
-            if (lane < 12) {
-                const int aa = lane / 4, al = lane % 4;
-                double z0 = 0.0, z1 = 0.0;
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const double t = kHel[aa][k] * sel4(w[k][0], w[k][1], w[k][2], w[k][3], al);
-                    if (k < nside0) z0 += t; else z1 += t;
-                }
-                R[R_Z + 8 * aa + al] = z0;
-                R[R_Z + 8 * aa + 4 + al] = z1;
-            }
-        }
-        __syncthreads();
-        if (valid && lane == 0) {
-            d3 gv[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) gv[k] = mk(L.g[3 * k], L.g[3 * k + 1], L.g[3 * k + 2]);
-            write_fric(a, cid, nside0, f, w);
-            write_grad(a, cid, nside0, w, gv);
-        }
-        __syncthreads();
-    }
 }
 
 // VG contact: compact body-space form.  The VG block of body I is
@@ -1036,7 +1077,7 @@ void contact_terms(World& w) {
         a.n_pt = w.cs.n_pt; a.n_ee = w.cs.n_ee;
         a.pt = w.cs.pt.p; a.ee = w.cs.ee.p;
         a.voff = w.voff.p; a.foff = w.foff.p; a.eoff = w.eoff.p; a.face = w.face.p; a.edge = w.edge.p;
-        a.X = w.X.p; a.Xs = w.Xs.p; a.xbar = w.xbar.p;
+        a.X = w.X.p; a.Xs = w.Xs.p; a.xbar = w.xbar.p; a.mass = w.mass.p;
         a.ctype = w.ctype.p; a.cdist = w.cdist.p; a.list = w.clist.p; a.list3 = w.clist3.p; a.counts = w.ccount.p;
         a.rec = w.crec.p; a.tk = w.ctk.p; a.lam = w.clam.p;
         {
@@ -1060,8 +1101,8 @@ void contact_terms(World& w) {
         {
             ProfScope ps(w, PK_CONTACT_GROUP, 0.0);
             ps.counts(snap, 0.0, 0.0, 1720.0, 0.0);
-            const int grid = std::min((n + CPW - 1) / CPW, 2048);
-            hipLaunchKernelGGL(k_contact_group, dim3(grid), dim3(64), 0, w.stream, a);
+            const int grid = std::min((n + 63) / 64, 4 * w.num_cu);
+            hipLaunchKernelGGL(k_contact_c3, dim3(grid), dim3(64), 0, w.stream, a);
             ADB_HIP(hipGetLastError());
         }
     }

@@ -61,7 +61,8 @@ __device__ double wg_sum(double v, double* sh) {
 // initial: x = 0, r = -g, z = M^-1 r, p_old = 0, (r.z, r.r) partials
 __global__ __launch_bounds__(UP_T) void k_pcg_init(int B, const double* __restrict__ g, const double* __restrict__ Minv,
                                                    double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
-                                                   double* __restrict__ p0, double* __restrict__ partB, double* __restrict__ scal) {
+                                                   double* __restrict__ p0, double* __restrict__ partB, double* __restrict__ scal,
+                                                   int* __restrict__ kbase) {
     __shared__ double rs[UP_BODIES][12];
     __shared__ double sh[4];
     const int lb = threadIdx.x / 12, i = threadIdx.x % 12;
@@ -88,9 +89,13 @@ __global__ __launch_bounds__(UP_T) void k_pcg_init(int B, const double* __restri
         partB[blockIdx.x] = rz; partB[gridDim.x + blockIdx.x] = rr;
         if (blockIdx.x == 0) { scal[S_STOP] = RUN; scal[S_ITERS] = 0.0; }
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *kbase = 0;
 }
+// iteration base of the next batch
+__global__ void k_pcg_advance(int* kbase, int n) { if (threadIdx.x == 0 && blockIdx.x == 0) *kbase += n; }
 
-__global__ __launch_bounds__(64 * SP_ROWS) void k_pcg_spmv(int B, int iter, double rtol2, int nbU, const int* __restrict__ row_ptr,
+__global__ __launch_bounds__(64 * SP_ROWS) void k_pcg_spmv(int B, const int* __restrict__ kbase, int kloc, double rtol2, int nbU,
+                                                           const int* __restrict__ row_ptr,
                                                            const int* __restrict__ col_idx, const double* __restrict__ Hval,
                                                            const double* __restrict__ z, const double* __restrict__ pold,
                                                            double* __restrict__ pnew, double* __restrict__ y,
@@ -99,6 +104,7 @@ __global__ __launch_bounds__(64 * SP_ROWS) void k_pcg_spmv(int B, int iter, doub
     __shared__ double sh[SP_ROWS];
     __shared__ double acc_sh[SP_ROWS][5][12];
     if (scal[S_STOP] != RUN) return;
+    const int iter = *kbase + kloc;
     // scalars of iteration iter - 1 (identical in every wave of every workgroup)
     const double RZ = wave_sum_array(partB, nbU);
     const double RR = wave_sum_array(partB + nbU, nbU);
@@ -144,7 +150,7 @@ __global__ __launch_bounds__(64 * SP_ROWS) void k_pcg_spmv(int B, int iter, doub
     if (threadIdx.x == 0) partA[blockIdx.x] = prod;
 }
 
-__global__ __launch_bounds__(UP_T) void k_pcg_update(int B, int iter, int nbS, const double* __restrict__ Minv,
+__global__ __launch_bounds__(UP_T) void k_pcg_update(int B, const int* __restrict__ kbase, int kloc, int nbS, const double* __restrict__ Minv,
                                                      const double* __restrict__ p, const double* __restrict__ y,
                                                      double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
                                                      const double* __restrict__ partA, double* __restrict__ partB,
@@ -152,6 +158,7 @@ __global__ __launch_bounds__(UP_T) void k_pcg_update(int B, int iter, int nbS, c
     __shared__ double rs[UP_BODIES][12];
     __shared__ double sh[4];
     if (scal[S_STOP] != RUN) return;
+    const int iter = *kbase + kloc;
     const double pAp = wave_sum_array(partA, nbS);
     const double RZ = scal[S_RZ0 + (iter & 1)];
     if (!(pAp > 0.0) || !std::isfinite(pAp)) {
@@ -183,42 +190,74 @@ __global__ __launch_bounds__(UP_T) void k_pcg_update(int B, int iter, int nbS, c
     if (threadIdx.x == 0) { partB[blockIdx.x] = rz; partB[gridDim.x + blockIdx.x] = rr; }
 }
 
+// One batch = BATCH iterations; kernel k of the batch computes iteration
+// *kbase + k, and the last node advances *kbase.  Because BATCH is even, the
+// p-buffer parity of local iteration k is k & 1 in every batch, so one
+// captured graph serves every batch.
+static void enqueue_batch(World& w, int nb, bool prof) {
+    const int B = w.B;
+    const int nbU = (B + UP_BODIES - 1) / UP_BODIES, nbS = (B + SP_ROWS - 1) / SP_ROWS;
+    const double rtol2 = w.P.pcg_rtol * w.P.pcg_rtol;
+    double* partA = w.partial.p;
+    double* partB = w.partial.p + nbS;
+    double* pb[2] = {w.pcg_p.p, w.pcg_p2.p};
+    for (int k = 0; k < nb; k++) {
+        {
+            // H blocks, their column indices, z and p_old read, p_new and y written
+            ProfScope ps(w, PK_SPMV, prof ? 1152.0 * w.nnzb + 4.0 * w.nnzb + 4.0 * 96.0 * B : 0.0);
+            hipLaunchKernelGGL(k_pcg_spmv, dim3(nbS), dim3(64 * SP_ROWS), 0, w.stream, B, w.pcg_k.p, k, rtol2, nbU, w.row_ptr.p,
+                               w.col_idx.p, w.Hval.p, w.pcg_z.p, pb[k & 1], pb[(k & 1) ^ 1], w.pcg_Ap.p, partB, partA, w.scal.p);
+        }
+        {
+            // M^-1 blocks read; p, y read; x, r read and written; z written
+            ProfScope ps(w, PK_PCG_UPDATE, prof ? 1152.0 * B + 7.0 * 96.0 * B : 0.0);
+            hipLaunchKernelGGL(k_pcg_update, dim3(nbU), dim3(UP_T), 0, w.stream, B, w.pcg_k.p, k, nbS, w.Pinv.p, pb[(k & 1) ^ 1],
+                               w.pcg_Ap.p, w.pcg_x.p, w.pcg_r.p, w.pcg_z.p, partA, partB, w.scal.p);
+        }
+    }
+    hipLaunchKernelGGL(k_pcg_advance, dim3(1), dim3(64), 0, w.stream, w.pcg_k.p, nb);
+}
+
 int pcg_solve(World& w, double* dq_out, int* iters) {
     const int B = w.B, n = 12 * B;
     const int nbU = (B + UP_BODIES - 1) / UP_BODIES;
     const int nbS = (B + SP_ROWS - 1) / SP_ROWS;
-    const double rtol2 = w.P.pcg_rtol * w.P.pcg_rtol;
     w.pcg_x.reserve(n); w.pcg_r.reserve(n); w.pcg_z.reserve(n); w.pcg_p.reserve(n); w.pcg_p2.reserve(n); w.pcg_Ap.reserve(n);
-    w.partial.reserve((size_t)nbS + 2 * (size_t)nbU); w.scal.reserve(S_COUNT);
-    double* partA = w.partial.p;
+    w.partial.reserve((size_t)nbS + 2 * (size_t)nbU); w.scal.reserve(S_COUNT); w.pcg_k.reserve(1);
     double* partB = w.partial.p + nbS;
     if (!w.h_scal) ADB_HIP(hipHostMalloc((void**)&w.h_scal, 2 * S_COUNT * sizeof(double), hipHostMallocDefault));
     for (hipEvent_t& e : w.pcg_ev)
         if (!e) ADB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    double* pb[2] = {w.pcg_p.p, w.pcg_p2.p};
-    hipLaunchKernelGGL(k_pcg_init, dim3(nbU), dim3(UP_T), 0, w.stream, B, w.grad.p, w.Pinv.p, w.pcg_x.p, w.pcg_r.p, w.pcg_z.p, pb[0],
-                       partB, w.scal.p);
+    hipLaunchKernelGGL(k_pcg_init, dim3(nbU), dim3(UP_T), 0, w.stream, B, w.grad.p, w.Pinv.p, w.pcg_x.p, w.pcg_r.p, w.pcg_z.p,
+                       w.pcg_p.p, partB, w.scal.p, w.pcg_k.p);
     ADB_HIP(hipGetLastError());
+    // the batch graph is captured once per set of buffer addresses (not while
+    // per-kernel profiling is on, nor with ADB_PCG_GRAPH=0 for tools that do
+    // not support graph capture)
+    const bool use_graph = !w.prof.on && w.use_graphs;
+    if (use_graph) {
+        const void* key[] = {w.row_ptr.p, w.col_idx.p, w.Hval.p, w.Pinv.p, w.pcg_x.p, w.pcg_r.p, w.pcg_z.p, w.pcg_p.p,
+                             w.pcg_p2.p, w.pcg_Ap.p, w.partial.p, w.scal.p, w.pcg_k.p};
+        bool same = w.pcg_graph != nullptr;
+        for (int i = 0; i < 13; i++) same = same && w.pcg_graph_key[i] == key[i];
+        if (!same) {
+            if (w.pcg_graph) { ADB_HIP(hipGraphExecDestroy(w.pcg_graph)); w.pcg_graph = nullptr; }
+            hipGraph_t g;
+            ADB_HIP(hipStreamBeginCapture(w.stream, hipStreamCaptureModeThreadLocal));
+            enqueue_batch(w, BATCH, false);
+            ADB_HIP(hipStreamEndCapture(w.stream, &g));
+            ADB_HIP(hipGraphInstantiate(&w.pcg_graph, g, nullptr, nullptr, 0));
+            ADB_HIP(hipGraphDestroy(g));
+            for (int i = 0; i < 13; i++) w.pcg_graph_key[i] = key[i];
+        }
+    }
     int it = 0, slot = 0;
     bool have_prev = false;
     const int max_it = std::max(w.P.pcg_max_iter, 0);
-    auto spmv = [&](int k) {
-        // H blocks, their column indices, z and p_old read, p_new and y written
-        ProfScope ps(w, PK_SPMV, 1152.0 * w.nnzb + 4.0 * w.nnzb + 4.0 * 96.0 * B);
-        hipLaunchKernelGGL(k_pcg_spmv, dim3(nbS), dim3(64 * SP_ROWS), 0, w.stream, B, k, rtol2, nbU, w.row_ptr.p, w.col_idx.p,
-                           w.Hval.p, w.pcg_z.p, pb[k & 1], pb[(k & 1) ^ 1], w.pcg_Ap.p, partB, partA, w.scal.p);
-    };
     while (it < max_it) {
         const int nb = std::min(BATCH, max_it - it);
-        for (int k = it; k < it + nb; k++) {
-            spmv(k);
-            {
-                // M^-1 blocks read; p, y read; x, r read and written; z written
-                ProfScope ps(w, PK_PCG_UPDATE, 1152.0 * B + 7.0 * 96.0 * B);
-                hipLaunchKernelGGL(k_pcg_update, dim3(nbU), dim3(UP_T), 0, w.stream, B, k, nbS, w.Pinv.p, pb[(k & 1) ^ 1],
-                                   w.pcg_Ap.p, w.pcg_x.p, w.pcg_r.p, w.pcg_z.p, partA, partB, w.scal.p);
-            }
-        }
+        if (use_graph && nb == BATCH) ADB_HIP(hipGraphLaunch(w.pcg_graph, w.stream));
+        else enqueue_batch(w, nb, w.prof.on);
         it += nb;
         ADB_HIP(hipGetLastError());
         ADB_HIP(hipMemcpyAsync(w.h_scal + S_COUNT * slot, w.scal.p, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, w.stream));
@@ -231,8 +270,13 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
         have_prev = true;
         slot ^= 1;
     }
-    // stop test after the last queued update (no-op when stopped earlier)
-    spmv(it);
+    // stop test after the last queued update (returns at once when stopped earlier)
+    {
+        const double rtol2 = w.P.pcg_rtol * w.P.pcg_rtol;
+        double* partA = w.partial.p;
+        hipLaunchKernelGGL(k_pcg_spmv, dim3(nbS), dim3(64 * SP_ROWS), 0, w.stream, B, w.pcg_k.p, 0, rtol2, nbU, w.row_ptr.p,
+                           w.col_idx.p, w.Hval.p, w.pcg_z.p, w.pcg_p.p, w.pcg_p2.p, w.pcg_Ap.p, partB, partA, w.scal.p);
+    }
     ADB_HIP(hipGetLastError());
     ADB_HIP(hipMemcpyAsync(w.h_scal, w.scal.p, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, w.stream));
     ADB_HIP(hipStreamSynchronize(w.stream));

@@ -41,7 +41,8 @@ template <class T> struct DBuf {
 
 // Compact body-space record of one PT/EE contact (doubles).  With D the
 // dimension of the contact's translation-free point space (3, 6 or 9; 0 when
-// the contact is inactive), n = D / 3 Helmert rows, sides s = 0 (body I) and
+// the contact is inactive; R[R_D] = D + 100 (dynamic(I) + 2 dynamic(J))),
+// n = D / 3 Helmert rows, sides s = 0 (body I) and
 // 1 (body J), and block coordinates (alpha, r) = (affine column, component):
 //   block(s1, s2)[(alpha, r), (beta, c)] =
 //       sum_{a, b < n} Z[a][s1][alpha] Z[b][s2][beta] PK[(3a + r) * D + 3b + c]
@@ -128,6 +129,10 @@ struct World {
     int nnzb = 0, n_contrib = 0;
     // ---- PCG
     DBuf<double> pcg_x, pcg_r, pcg_z, pcg_p, pcg_p2, pcg_Ap, partial, scal;
+    DBuf<int> pcg_k;                 // iteration base of the queued PCG batch
+    bool use_graphs = true;          // ADB_PCG_GRAPH=0 turns graph capture off
+    hipGraphExec_t pcg_graph = nullptr;   // one captured PCG batch
+    const void* pcg_graph_key[13] = {};   // buffer addresses the graph was captured with
     double* h_scal = nullptr;        // pinned host copies of the PCG scalars (2 x 8)
     hipEvent_t pcg_ev[2] = {nullptr, nullptr};
     // ---- energies

@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
 #include <algorithm>
 #include "dmath.h"
 #include "world.h"
@@ -189,6 +190,7 @@ extern "C" int adb_world_create(const adb_scene* s, const adb_params* p, adb_wor
     try {
         w.P = *p;
         w.B = s->n_bodies;
+        if (const char* g = std::getenv("ADB_PCG_GRAPH")) w.use_graphs = std::strcmp(g, "0") != 0;
         w.h_voff.assign(s->vert_offset, s->vert_offset + w.B + 1);
         w.h_foff.assign(s->face_offset, s->face_offset + w.B + 1);
         w.h_eoff.assign(s->edge_offset, s->edge_offset + w.B + 1);
@@ -268,6 +270,9 @@ static void free_world(World& w) {
     w.prof.pool.clear();
     if (w.prof.snap) (void)hipHostFree(w.prof.snap);
     w.prof.snap = nullptr;
+    if (w.pcg_graph) (void)hipGraphExecDestroy(w.pcg_graph);
+    w.pcg_graph = nullptr;
+    w.pcg_k.free_();
     if (w.h_scal) (void)hipHostFree(w.h_scal);
     w.h_scal = nullptr;
     for (hipEvent_t& e : w.pcg_ev) { if (e) (void)hipEventDestroy(e); e = nullptr; }

@@ -59,45 +59,17 @@ def cpu_baseline(sc, p, budget_s):
                       % (steps, el, threads, pairs / el)}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--bodies", type=int, default=2000)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--profile-json", default="")
-    ap.add_argument("--copy-mb", type=float, default=0.0,
-                    help="first run one plain device-to-device copy of this many MB (a known-size kernel "
-                         "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: the contact step of one scene runs on one device "
-              "(multi-GPU sharding is a design note, not implemented); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-
+def run(sc, p, warmup, steps):
+    """warmup untimed steps, then `steps` timed steps; returns (seconds, totals, last stats, world)."""
     from adb_amd.world import World
-    if args.copy_mb > 0:
-        import torch
-        n = int(args.copy_mb * 1e6 / 8)
-        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-    sc = workload(args.bodies)
-    p = params(args.precision)
     w = World(sc, p)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         w.step()
-    w.profile_enable(True)
     w.state()   # synchronizes the world's stream
     tot = {"contacts": 0, "newton": 0, "ls": 0, "pcg": 0, "t_ipc": 0.0, "t_solver": 0.0, "t_ccd": 0.0, "t_ls": 0.0}
-    t0 = time.perf_counter()
     last = None
-    for _ in range(args.steps):
+    t0 = time.perf_counter()
+    for _ in range(steps):
         rc, st = w.step()
         last = st
         tot["contacts"] += st.contacts_processed
@@ -109,13 +81,78 @@ def main():
         tot["t_ccd"] += st.t_ccd
         tot["t_ls"] += st.t_line_search
     w.state()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    prof = w.profile_read()
-    w.profile_enable(False)
+    return time.perf_counter() - t0, tot, last, w
+
+
+def max_vertex_deviation(sc, qa, qb):
+    """max over vertices of |x_a - x_b| for two states (x = A xbar + p)."""
+    import numpy as np
+    vo = sc["vert_offset"]
+    body = np.repeat(np.arange(len(vo) - 1), np.diff(vo))
+    xb = sc["rest_verts"]
+    d = (qa - qb).reshape(-1, 12)
+    dx = d[body, :3] + np.einsum("nki,nk->ni", d[body, 3:].reshape(-1, 3, 3), xb)
+    return float(np.sqrt((dx * dx).sum(1)).max())
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed",
+                    help="precision of the reported value; the other one is timed beside it")
+    ap.add_argument("--bodies", type=int, default=2000)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-compare", action="store_true", help="skip the run in the other precision")
+    ap.add_argument("--profile-steps", type=int, default=2,
+                    help="steps after the timed ones with per-kernel HIP-event timing (roofline figures)")
+    ap.add_argument("--profile-json", default="")
+    ap.add_argument("--copy-mb", type=float, default=0.0,
+                    help="first run one plain device-to-device copy of this many MB (a known-size kernel "
+                         "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the contact step of one scene runs on one device "
+              "(multi-GPU sharding is a design note, not implemented); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    if args.copy_mb > 0:
+        import torch
+        n = int(args.copy_mb * 1e6 / 8)
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    sc = workload(args.bodies)
+    p = params(args.precision)
+    elapsed, tot, last, w = run(sc, p, args.warmup, args.steps)
+    q_main, _ = w.state()
+
+    # per-kernel timing on extra steps (HIP events on the world's stream)
+    prof = {}
+    if args.profile_steps > 0:
+        w.profile_enable(True)
+        for _ in range(args.profile_steps):
+            w.step()
+        prof = w.profile_read()
+        w.profile_enable(False)
+    del w
     if args.profile_json:
         with open(args.profile_json, "w") as fh:
             json.dump(prof, fh, indent=1)
+
+    other = None
+    if not args.no_compare:
+        op = "fp64" if args.precision == "mixed" else "mixed"
+        e2, t2, _, w2 = run(sc, params(op), args.warmup, args.steps)
+        q2, _ = w2.state()
+        del w2
+        other = {"precision": op, "steps_per_s": round(args.steps / e2, 4),
+                 "max_vertex_deviation_vs_value_run": max_vertex_deviation(sc, q_main, q2),
+                 "newton_iters": t2["newton"], "pcg_iters": t2["pcg"]}
 
     steps_per_s = args.steps / elapsed
     # dominant kernel by device time among those with an algorithmic byte count
@@ -142,7 +179,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f64 (f32 broad-phase boxes)",
+        "dtype": "f64" if args.precision == "fp64" else "f64 (f32 outward-rounded broad-phase boxes)",
         "data": "synthetic (seeded PCG32 scene, no dataset)",
         "config": {"workload": "2000 icospheres (642 v / 1280 t / 1920 e each), 4x4x8 box, ground, mu=0.3, "
                                "dt=0.01, d_hat=1e-3, seed 2",
@@ -152,7 +189,8 @@ def main():
         "phase_s": {"ipc": round(tot["t_ipc"], 4), "solver": round(tot["t_solver"], 4),
                     "ccd": round(tot["t_ccd"], 4), "line_search": round(tot["t_ls"], 4)},
         "contacts_last_step": {"pt": last.n_pt, "ee": last.n_ee, "vg": last.n_vg} if last else None,
-        "top_kernels_ms": {k: round(v["ms"], 3) for k, v in ranked[:6]},
+        "other_precision": other,
+        "top_kernels_ms_profiled_steps": {k: round(v["ms"], 3) for k, v in ranked[:8]},
         "roofline": roofline,
         "cpu_baseline": None,
     }
