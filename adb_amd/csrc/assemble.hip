@@ -13,6 +13,8 @@
 // bodies (mass < 0) follow the reference: contributions to them are dropped,
 // their diagonal block is I and their gradient 0.
 #include <hipcub/hipcub.hpp>
+#include <algorithm>
+#include <vector>
 #include "dmath.h"
 #include "world.h"
 
@@ -124,11 +126,10 @@ struct ReduceArgs {
 };
 // Pass 1: one 256-lane workgroup per chunk of at most RCH contributions of
 // one block (long segments - the diagonal block of a body with thousands of
-// contacts - are split so every workgroup has bounded work).  Wave w sums
-// contributions w, w + 4, ... of the chunk (lane t: entries t, t + 64,
-// t + 128 < 144), then the four wave sums are combined in a fixed order.
-constexpr int RCH = 64;
+// contacts - are split so every workgroup has bounded work).
 constexpr int RW = 4;            // waves per reduction workgroup
+constexpr int RS = 5;            // contributions per wave at once (12 lanes each)
+constexpr int RCH = RW * 3 * RS; // contributions per chunk
 __global__ void k_chunk_count(int nnzb, const int* __restrict__ blk_start, int* __restrict__ nch) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < nnzb) nch[k] = (blk_start[k + 1] - blk_start[k] + RCH - 1) / RCH;
@@ -157,119 +158,94 @@ __device__ __forceinline__ void combine_waves(Acc156& A, double (*sh)[156]) {
         }
     }
 }
-// LDS writes of one wave visible to its own lanes
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-constexpr int RG = 8;            // contributions staged in LDS per wave at once
-// Each wave sums RCH / RW consecutive contributions of the chunk.  They are
-// taken RG at a time: every lane reads the RG contribution ids and record
-// headers (broadcast loads), the wave copies the RG records (or inertia
-// blocks) into LDS with independent coalesced loads, then every lane expands
-// its entries from LDS:
-//   block(s1, s2)[(al, r), (be, c)] = sum_ab Z[a][s1][al] Z[b][s2][be] PK[(3a + r) D + 3b + c]
-//                                   + G[s1][al] G[s2][be] F[3r + c]
+// Pass 1 body: each wave takes RCH / RW consecutive contributions of the
+// chunk, RS at a time; 12 lanes per contribution, lane (slot, col) owning
+// column col = (be, cc) of the 12x12 block:
+//   t[(a, r)]       = sum_b PK[(3a + r) D + 3b + cc] Z[b][s2][be]
+//   entry (al, r)  += sum_a Z[a][s1][al] t[(a, r)] + G[s1][al] G[s2][be] F[3r + cc]
+// (lane col 0 of each slot also adds the 12 gradient entries).  The RW x RS
+// slot sums are combined in a fixed order.
 __global__ __launch_bounds__(64 * RW) void k_reduce_chunks(ReduceArgs a, const int* __restrict__ chunk_off,
                                                            const int* __restrict__ chunk_blk, double* __restrict__ part) {
-    __shared__ double sh[RW - 1][156];
-    __shared__ double stage[RW][RG][160];
-    __shared__ int meta[RW][RG][4];     // kind (0 skip, 1 contact, 2 inertia), D, side bits, contact / body id
+    __shared__ double P[RW][RS][156];
     const int ch = blockIdx.x;
     if (ch >= chunk_off[a.nnzb]) return;
     const int k = chunk_blk[ch];
     const int beg = a.blk_start[k] + (ch - chunk_off[k]) * RCH;
     const int end = min(beg + RCH, a.blk_start[k + 1]);
-    const int w = threadIdx.x / 64, t = threadIdx.x % 64;
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    const int slot = l / 12, col = l % 12, be = col / 3, cc = col % 3;
     const int wb = min(beg + w * (RCH / RW), end), we = min(wb + RCH / RW, end);
-    Acc156 A = {{0.0, 0.0, 0.0}, 0.0};
-    double (*S)[160] = stage[w];
-    int (*M)[4] = meta[w];
-    for (int g0 = wb; g0 < we; g0 += RG) {
-        const int ng = min(RG, we - g0);
-        if (t < RG) {
-            int kind = 0, D = 0, sides = 0, x = 0;
-            if (t < ng) {
-                const int v = a.vals[g0 + t];
-                if (v < 0) { kind = 2; x = -1 - v; }
-                else {
-                    const int c = v / 4, slot = v % 4;
-                    const int code = (int)a.rec[REC * (size_t)c + R_D];
-                    const int Dd = code % 100, di = (code / 100) & 1, dj = (code / 100) >> 1;
-                    const bool use = slot == 0 ? di : (slot == 1 ? dj : (di && dj));
-                    if (use && Dd > 0) {
-                        kind = 1; D = Dd; x = c;
-                        // bit 0: s1, bit 1: s2, bit 2: gradient (slot 0 = (I, I), 1 = (J, J), 2 = (I, J), 3 = (J, I))
-                        sides = ((slot == 1 || slot == 3) ? 1 : 0) | ((slot == 1 || slot == 2) ? 2 : 0) | (slot < 2 ? 4 : 0);
-                    }
+    double acc[12], gacc[12];
+#pragma unroll
+    for (int i = 0; i < 12; i++) { acc[i] = 0.0; gacc[i] = 0.0; }
+    for (int r0 = wb; r0 < we; r0 += RS) {
+        const int s = r0 + slot;
+        if (slot < RS && s < we) {
+            const int v = a.vals[s];
+            if (v < 0) {     // diagonal marker: inertia block and gradient of body -1 - v
+                const int b = -1 - v;
+#pragma unroll
+                for (int i = 0; i < 12; i++) acc[i] += a.bodyH[144 * (size_t)b + 12 * col + i];
+                if (col == 0) {
+#pragma unroll
+                    for (int i = 0; i < 12; i++) gacc[i] += a.bodyG[12 * (size_t)b + i];
                 }
-            }
-            M[t][0] = kind; M[t][1] = D; M[t][2] = sides; M[t][3] = x;
-        }
-        wave_sync();
-        {
-            // all loads first (independent), then the LDS stores
-            double tmp[RG][3];
+            } else {
+                const int c = v / 4, cs = v % 4;
+                const double* R = a.rec + REC * (size_t)c;
+                const int code = (int)R[R_D];
+                const int D = code % 100, di = (code / 100) & 1, dj = (code / 100) >> 1;
+                const bool use = cs == 0 ? di : (cs == 1 ? dj : (di && dj));
+                if (use && D > 0) {
+                    // cs 0 = (I, I), 1 = (J, J), 2 = (I, J), 3 = (J, I)
+                    const int s1 = (cs == 1 || cs == 3) ? 1 : 0, s2 = (cs == 1 || cs == 2) ? 1 : 0;
+                    const int nA = D / 3;
+                    double t[3][3];
 #pragma unroll
-            for (int j = 0; j < RG; j++) {
-                const int kind = j < ng ? M[j][0] : 0, x = M[j][3];
-                const int len = kind == 1 ? R_PK + M[j][1] * M[j][1] : (kind == 2 ? 156 : 0);
+                    for (int aa = 0; aa < 3; aa++)
 #pragma unroll
-                for (int u = 0; u < 3; u++) {
-                    const int e = t + 64 * u;
-                    double v = 0.0;
-                    if (e < len) {
-                        if (kind == 1) v = a.rec[REC * (size_t)x + e];
-                        else v = e < 144 ? a.bodyH[144 * (size_t)x + e] : a.bodyG[12 * (size_t)x + e - 144];
-                    }
-                    tmp[j][u] = v;
-                }
-            }
+                        for (int r = 0; r < 3; r++) {
+                            double sum = 0.0;
 #pragma unroll
-            for (int j = 0; j < RG; j++)
-#pragma unroll
-                for (int u = 0; u < 3; u++) {
-                    const int e = t + 64 * u;
-                    if (e < 160) S[j][e] = tmp[j][u];
-                }
-        }
-        wave_sync();
-#pragma unroll 1
-        for (int j = 0; j < ng; j++) {
-            const int kind = M[j][0];
-            const double* Rj = S[j];
-            if (kind == 2) {
-                for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) A.e[u] += Rj[e]; }
-                if (t < 12) A.g += Rj[144 + t];
-            } else if (kind == 1) {
-                const int Dj = M[j][1], nA = Dj / 3, sides = M[j][2], a1 = sides & 1, a2 = (sides >> 1) & 1;
-                for (int u = 0; u < 3; u++) {
-                    const int e = t + 64 * u;
-                    if (e < 144) {
-                        // column (be, cc), row (al, r)
-                        const int col = e / 12, row = e % 12;
-                        const int be = col / 3, cc = col % 3, al = row / 3, r = row % 3;
-                        double val = Rj[R_G + 4 * a1 + al] * Rj[R_G + 4 * a2 + be] * Rj[R_F + 3 * r + cc];
-                        for (int aa = 0; aa < nA; aa++) {
-                            double sb = 0.0;
-                            for (int bb = 0; bb < nA; bb++)
-                                sb += Rj[R_Z + 8 * bb + 4 * a2 + be] * Rj[R_PK + (3 * aa + r) * Dj + 3 * bb + cc];
-                            val += Rj[R_Z + 8 * aa + 4 * a1 + al] * sb;
+                            for (int bb = 0; bb < 3; bb++)
+                                if (aa < nA && bb < nA) sum += R[R_PK + (3 * aa + r) * D + 3 * bb + cc] * R[R_Z + 8 * bb + 4 * s2 + be];
+                            t[aa][r] = sum;
                         }
-                        A.e[u] += val;
+                    const double g2 = R[R_G + 4 * s2 + be];
+                    const double F0 = R[R_F + cc], F1 = R[R_F + 3 + cc], F2 = R[R_F + 6 + cc];
+#pragma unroll
+                    for (int al = 0; al < 4; al++) {
+                        const double gg = R[R_G + 4 * s1 + al] * g2;
+                        double z[3];
+#pragma unroll
+                        for (int aa = 0; aa < 3; aa++) z[aa] = aa < nA ? R[R_Z + 8 * aa + 4 * s1 + al] : 0.0;
+                        acc[3 * al + 0] += (z[0] * t[0][0] + z[1] * t[1][0] + z[2] * t[2][0]) + gg * F0;
+                        acc[3 * al + 1] += (z[0] * t[0][1] + z[1] * t[1][1] + z[2] * t[2][1]) + gg * F1;
+                        acc[3 * al + 2] += (z[0] * t[0][2] + z[1] * t[1][2] + z[2] * t[2][2]) + gg * F2;
+                    }
+                    if (cs < 2 && col == 0) {
+#pragma unroll
+                        for (int i = 0; i < 12; i++) gacc[i] += R[R_GRAD + 12 * s1 + i];
                     }
                 }
-                if ((sides & 4) && t < 12) A.g += Rj[R_GRAD + 12 * a1 + t];
             }
         }
-        wave_sync();
     }
-    combine_waves(A, sh);
-    if (w == 0) {
-        double* P = part + 156 * (size_t)ch;
-        for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) P[e] = A.e[u]; }
-        if (t < 12) P[144 + t] = A.g;
+    if (slot < RS) {
+#pragma unroll
+        for (int i = 0; i < 12; i++) P[w][slot][12 * col + i] = acc[i];
+        if (col == 0) {
+#pragma unroll
+            for (int i = 0; i < 12; i++) P[w][slot][144 + i] = gacc[i];
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 156; e += 64 * RW) {
+        double sum = 0.0;
+        for (int q = 0; q < RW; q++)
+            for (int u = 0; u < RS; u++) sum += P[q][u][e];
+        part[156 * (size_t)ch + e] = sum;
     }
 }
 
@@ -313,55 +289,149 @@ __global__ __launch_bounds__(64 * RW) void k_reduce_final(ReduceArgs a, const in
     if (row == col && t < 12) a.grad[12 * (size_t)row + t] = gacc;
 }
 
-// block-Jacobi preconditioner: the explicit inverse of each diagonal block,
-// M^-1 = W^T W with W = L^-1 from the Cholesky factor L (symmetric by
-// construction).  One thread per body, matrices staged in LDS; falls back to
-// the inverse diagonal when the block is not positive definite.
-constexpr int CH_T = 32;
-__global__ __launch_bounds__(CH_T) void k_block_inv(int B, const int* __restrict__ diag_blk, const double* __restrict__ Hval,
-                                                   double* __restrict__ Minv) {
-    __shared__ double S[CH_T][144];
-    __shared__ double T[CH_T][144];
-    const int b = blockIdx.x * CH_T + threadIdx.x;
-    if (b >= B) return;
-    double* A = S[threadIdx.x];   // A[c * 12 + r]: column-major; L(r, c) for r >= c after factorisation
-    double* W = T[threadIdx.x];   // W[c * 12 + r] = (L^-1)(r, c), r >= c
-    const double* D = Hval + 144 * (size_t)diag_blk[b];
-    for (int i = 0; i < 144; i++) A[i] = D[i];
-    bool ok = true;
-    for (int j = 0; j < 12 && ok; j++) {
-        double s = A[j * 12 + j];
-        for (int k = 0; k < j; k++) s -= A[k * 12 + j] * A[k * 12 + j];
-        if (!(s > 0.0)) { ok = false; break; }
-        double l = sqrt(s);
-        A[j * 12 + j] = l;
-        for (int i = j + 1; i < 12; i++) {
-            double t = A[j * 12 + i];
-            for (int k = 0; k < j; k++) t -= A[k * 12 + i] * A[k * 12 + j];
-            A[j * 12 + i] = t / l;
+// Preconditioner: the explicit inverse of the diagonal sub-matrix of each
+// body group (a connected component of the contact graph, or a BFS-ordered
+// piece of one with at most GMAX bodies).  On groups that are whole
+// components the preconditioned system decouples exactly, so PCG needs only a
+// few iterations where the contact graph is a union of small clusters; in
+// large clusters it is block-Jacobi with blocks of up to GMAX bodies.
+// One workgroup per group: gather the blocks into LDS, Cholesky in place,
+// W = L^-1 column by column (thread per column, stored transposed in the
+// upper triangle), then M^-1 = W^T W.  A group whose matrix is not positive
+// definite falls back to the inverse diagonal.
+constexpr int GMAX = 8;
+constexpr int GN = 12 * GMAX;
+constexpr int GLD = GN + 1;
+__global__ __launch_bounds__(256) void k_group_inv(const int* __restrict__ grp_off, const int* __restrict__ grp_body,
+                                                   const long long* __restrict__ grp_moff, const int* __restrict__ row_ptr,
+                                                   const int* __restrict__ col_idx, const double* __restrict__ Hval,
+                                                   double* __restrict__ Ginv) {
+    __shared__ double A[GN * GLD];
+    __shared__ double wd[GN];
+    __shared__ int blk[GMAX][GMAX];
+    __shared__ int bod[GMAX];
+    __shared__ int ok;
+    const int g = blockIdx.x, t = threadIdx.x;
+    const int m = grp_off[g + 1] - grp_off[g], n = 12 * m;
+    if (t < m) bod[t] = grp_body[grp_off[g] + t];
+    if (t == 0) ok = 1;
+    __syncthreads();
+    if (t < m * m) {
+        const int ia = t / m, ib = t % m;
+        int k = -1;
+        for (int q = row_ptr[bod[ia]]; q < row_ptr[bod[ia] + 1]; q++)
+            if (col_idx[q] == bod[ib]) { k = q; break; }
+        blk[ia][ib] = k;
+    }
+    __syncthreads();
+    for (int e = t; e < n * n; e += blockDim.x) {
+        const int j = e / n, i = e % n;       // A[i][j], column-major in the block
+        const int k = blk[i / 12][j / 12];
+        A[i * GLD + j] = k >= 0 ? Hval[144 * (size_t)k + 12 * (j % 12) + (i % 12)] : 0.0;
+    }
+    __syncthreads();
+    // Cholesky, lower triangle in place
+    for (int j = 0; j < n; j++) {
+        if (t == 0) {
+            const double d = A[j * GLD + j];
+            if (d > 0.0) A[j * GLD + j] = sqrt(d); else ok = 0;
+        }
+        __syncthreads();
+        if (!ok) break;
+        const double ljj = A[j * GLD + j];
+        for (int i = j + 1 + t; i < n; i += blockDim.x) A[i * GLD + j] /= ljj;
+        __syncthreads();
+        const int r = n - j - 1;
+        for (int e = t; e < r * r; e += blockDim.x) {
+            const int i = j + 1 + e / r, k = j + 1 + e % r;
+            if (k <= i) A[i * GLD + k] -= A[i * GLD + j] * A[k * GLD + j];
+        }
+        __syncthreads();
+    }
+    double* out = Ginv + grp_moff[g];
+    if (!ok) {
+        // the Cholesky stopped early; the original diagonal is gone, so take it from Hval
+        for (int e = t; e < n * n; e += blockDim.x) {
+            const int j = e / n, i = e % n;
+            double v = 0.0;
+            if (i == j) {
+                const int k = blk[i / 12][i / 12];
+                const double d = k >= 0 ? Hval[144 * (size_t)k + 13 * (i % 12)] : 1.0;
+                v = 1.0 / (d > 0.0 ? d : 1.0);
+            }
+            out[(size_t)j * n + i] = v;
+        }
+        return;
+    }
+    // W = L^-1: thread c builds column c (W[i][c], i >= c) into A[c][i] (upper) and wd[c]
+    if (t < n) {
+        const int c = t;
+        const double wcc = 1.0 / A[c * GLD + c];
+        wd[c] = wcc;
+        for (int i = c + 1; i < n; i++) {
+            double s = A[i * GLD + c] * wcc;
+            for (int k = c + 1; k < i; k++) s += A[i * GLD + k] * A[c * GLD + k];
+            A[c * GLD + i] = -s / A[i * GLD + i];
         }
     }
-    double* out = Minv + 144 * (size_t)b;
-    if (ok) {
-        for (int c = 0; c < 12; c++) {
-            W[c * 12 + c] = 1.0 / A[c * 12 + c];
-            for (int r = c + 1; r < 12; r++) {
-                double s = 0.0;
-                for (int k = c; k < r; k++) s += A[k * 12 + r] * W[c * 12 + k];
-                W[c * 12 + r] = -s / A[r * 12 + r];
+    __syncthreads();
+    // M^-1 = W^T W: (i, j) = sum_{k >= max(i, j)} W[k][i] W[k][j]
+    for (int e = t; e < n * n; e += blockDim.x) {
+        const int j = e / n, i = e % n;
+        if (i > j) continue;
+        double s = 0.0;
+        for (int k = j; k < n; k++) {
+            const double wki = k == i ? wd[i] : A[i * GLD + k];
+            const double wkj = k == j ? wd[j] : A[j * GLD + k];
+            s += wki * wkj;
+        }
+        out[(size_t)j * n + i] = s;
+        out[(size_t)i * n + j] = s;
+    }
+}
+
+// Connected components of the block graph (host, from the BSR structure) and
+// their split into groups of at most GMAX bodies in BFS order.
+static void build_groups(World& w) {
+    const int B = w.B, nnzb = w.nnzb;
+    w.h_rp.resize(B + 1); w.h_ci.resize(nnzb);
+    ADB_HIP(hipMemcpyAsync(w.h_rp.data(), w.row_ptr.p, (B + 1) * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+    ADB_HIP(hipMemcpyAsync(w.h_ci.data(), w.col_idx.p, nnzb * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+    ADB_HIP(hipStreamSynchronize(w.stream));
+    std::vector<char> seen(B, 0);
+    std::vector<int> queue;
+    queue.reserve(B);
+    w.h_grp_off.assign(1, 0);
+    w.h_grp_body.clear();
+    w.h_grp_moff.assign(1, 0);
+    for (int s = 0; s < B; s++) {
+        if (seen[s]) continue;
+        queue.clear();
+        queue.push_back(s);
+        seen[s] = 1;
+        for (size_t h = 0; h < queue.size(); h++) {
+            const int r = queue[h];
+            for (int k = w.h_rp[r]; k < w.h_rp[r + 1]; k++) {
+                const int c = w.h_ci[k];
+                if (!seen[c]) { seen[c] = 1; queue.push_back(c); }
             }
         }
-        for (int c = 0; c < 12; c++)
-            for (int r = c; r < 12; r++) {
-                double s = 0.0;
-                for (int k = r; k < 12; k++) s += W[r * 12 + k] * W[c * 12 + k];
-                out[c * 12 + r] = s;
-                out[r * 12 + c] = s;
-            }
-    } else {
-        for (int i = 0; i < 144; i++) out[i] = 0.0;
-        for (int i = 0; i < 12; i++) { double d = D[13 * i]; out[13 * i] = 1.0 / (d > 0.0 ? d : 1.0); }
+        for (size_t q = 0; q < queue.size(); q += GMAX) {
+            const int m = (int)std::min<size_t>(GMAX, queue.size() - q);
+            for (int i = 0; i < m; i++) w.h_grp_body.push_back(queue[q + i]);
+            w.h_grp_off.push_back((int)w.h_grp_body.size());
+            w.h_grp_moff.push_back(w.h_grp_moff.back() + (long long)(12 * m) * (12 * m));
+        }
     }
+    w.n_groups = (int)w.h_grp_off.size() - 1;
+    w.grp_off.reserve(w.n_groups + 1); w.grp_body.reserve(B); w.grp_moff.reserve(w.n_groups + 1); w.grp_n.reserve(1);
+    ADB_HIP(hipMemcpyAsync(w.grp_n.p, &w.n_groups, sizeof(int), hipMemcpyHostToDevice, w.stream));
+    ADB_HIP(hipMemcpyAsync(w.grp_off.p, w.h_grp_off.data(), (w.n_groups + 1) * sizeof(int), hipMemcpyHostToDevice, w.stream));
+    ADB_HIP(hipMemcpyAsync(w.grp_body.p, w.h_grp_body.data(), B * sizeof(int), hipMemcpyHostToDevice, w.stream));
+    ADB_HIP(hipMemcpyAsync(w.grp_moff.p, w.h_grp_moff.data(), (w.n_groups + 1) * sizeof(long long), hipMemcpyHostToDevice,
+                           w.stream));
+    // the host vectors stay alive (members of w) until the copies have run
+    w.Ginv.reserve((size_t)w.h_grp_moff.back());
 }
 
 template <class K, class V>
@@ -438,6 +508,10 @@ void assemble_system(World& w) {
         hipLaunchKernelGGL(k_chunk_owner, dim3((nnzb + 255) / 256), dim3(256), 0, w.stream, nnzb, w.chunk_off.p, w.chunk_blk.p,
                            w.ccount.p);
         ADB_HIP(hipGetLastError());
+        if (w.side_pending) {   // records of the 4-point contacts (side stream)
+            ADB_HIP(hipStreamWaitEvent(w.stream, w.ev_side[1], 0));
+            w.side_pending = false;
+        }
         {
             // every contact record read once (D = 3: 59 doubles, 6: 94, 9: 147), the
             // contribution list and inertia terms read, one partial block written per chunk
@@ -450,9 +524,13 @@ void assemble_system(World& w) {
         ADB_HIP(hipGetLastError());
     }
     ADB_HIP(hipGetLastError());
-    w.Pinv.reserve(144 * (size_t)B);
-    hipLaunchKernelGGL(k_block_inv, dim3((B + CH_T - 1) / CH_T), dim3(CH_T), 0, w.stream, B, w.diag_blk.p, w.Hval.p, w.Pinv.p);
-    ADB_HIP(hipGetLastError());
+    build_groups(w);
+    {
+        ProfScope ps(w, PK_BLOCK_CHOL, 0.0);
+        hipLaunchKernelGGL(k_group_inv, dim3(w.n_groups), dim3(256), 0, w.stream, w.grp_off.p, w.grp_body.p, w.grp_moff.p,
+                           w.row_ptr.p, w.col_idx.p, w.Hval.p, w.Ginv.p);
+        ADB_HIP(hipGetLastError());
+    }
 }
 
 } // namespace adb

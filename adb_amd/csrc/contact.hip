@@ -42,6 +42,11 @@ constexpr int CPW = 4;        // contacts per 64-lane workgroup
 constexpr int LDH = 13;       // padded leading dimensions in LDS
 constexpr int LDK = 11;
 constexpr int MAX_SWEEPS = 15;
+// Jacobi stops when the squared off-diagonal Frobenius norm is below this
+// fraction of the total (off-diagonal entries ~1e-12 relative: rounding-level
+// for the projected Hessian, and reachable in a few sweeps, unlike 1e-30,
+// which sits at the FP64 noise floor of the rotations themselves)
+constexpr double JACOBI_TOL = 1e-24;
 
 struct GroupLDS {
     double H[12][LDH];
@@ -115,7 +120,7 @@ __device__ __forceinline__ void group_psd(GroupLDS& S, int lane, bool on) {
             for (int r = 0; r < 9; r++) { double a = S.A[r][lane]; tot += a * a; if (r != lane) off += a * a; }
         }
         for (int m = 8; m >= 1; m >>= 1) { off += __shfl_xor(off, m, GL); tot += __shfl_xor(tot, m, GL); }
-        const bool conv = !on || off <= 1e-30 * tot || off == 0.0;
+        const bool conv = !on || off <= JACOBI_TOL * tot || off == 0.0;
         if (__all(conv)) break;
         for (int rd = 0; rd < 9; rd++) {
             if (lane < 5) {
@@ -633,7 +638,7 @@ __device__ __forceinline__ void jacobi_reg(double (&A)[N][N], double (&v)[N][N])
         for (int p = 0; p < N; p++)
 #pragma unroll
             for (int q = 0; q < N; q++) { const double x = A[p][q] * A[p][q]; tot += x; if (p != q) off += x; }
-        if (off <= 1e-30 * tot || off == 0.0) break;
+        if (off <= JACOBI_TOL * tot || off == 0.0) break;
 #pragma unroll
         for (int p = 0; p < N - 1; p++)
 #pragma unroll
@@ -956,7 +961,7 @@ __device__ __forceinline__ void contact_c3(const WorldArgs& a, int cid, double (
                     const double x = As[P9(p, q)][lane] * As[P9(p, q)][lane];
                     if (p == q) tot += x; else { off += 2.0 * x; tot += 2.0 * x; }
                 }
-            if (off <= 1e-30 * tot || off == 0.0) break;
+            if (off <= JACOBI_TOL * tot || off == 0.0) break;
             for (int p = 0; p < 8; p++)
                 for (int q = p + 1; q < 9; q++) {
                     const double apq = As[P9(p, q)][lane];
@@ -1088,6 +1093,26 @@ void contact_terms(World& w) {
         }
         const int* snap = prof_snapshot(w, w.ccount.p);
         {
+            // 4-point contacts on the side stream: they fill few CUs for long, so
+            // they overlap the 2- and 3-point kernels and the assembly's sort;
+            // the assembly waits for them before its reduction
+            if (!w.stream2) {
+                ADB_HIP(hipStreamCreate(&w.stream2));
+                for (hipEvent_t& e : w.ev_side) ADB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            }
+            ADB_HIP(hipEventRecord(w.ev_side[0], w.stream));
+            ADB_HIP(hipStreamWaitEvent(w.stream2, w.ev_side[0], 0));
+            {
+                ProfScope ps(w, PK_CONTACT_GROUP, 0.0, w.stream2);
+                ps.counts(snap, 0.0, 0.0, 1720.0, 0.0);
+                const int grid = std::min((n + 63) / 64, 4 * w.num_cu);
+                hipLaunchKernelGGL(k_contact_c3, dim3(grid), dim3(64), 0, w.stream2, a);
+                ADB_HIP(hipGetLastError());
+            }
+            ADB_HIP(hipEventRecord(w.ev_side[1], w.stream2));
+            w.side_pending = true;
+        }
+        {
             // per contact: list, ids, type, distance, X / Xs / xbar of 4 points in;
             // record (D = 3: 59 doubles, D = 6: 94), Tk and lambda out
             ProfScope ps(w, PK_CONTACT, 0.0);
@@ -1096,13 +1121,6 @@ void contact_terms(World& w) {
             hipLaunchKernelGGL(k_contact_small<1>, dim3(grid), dim3(256), 0, w.stream, a);
             ADB_HIP(hipGetLastError());
             hipLaunchKernelGGL(k_contact_small<2>, dim3(grid), dim3(256), 0, w.stream, a);
-            ADB_HIP(hipGetLastError());
-        }
-        {
-            ProfScope ps(w, PK_CONTACT_GROUP, 0.0);
-            ps.counts(snap, 0.0, 0.0, 1720.0, 0.0);
-            const int grid = std::min((n + 63) / 64, 4 * w.num_cu);
-            hipLaunchKernelGGL(k_contact_c3, dim3(grid), dim3(64), 0, w.stream, a);
             ADB_HIP(hipGetLastError());
         }
     }

@@ -29,8 +29,6 @@ enum { S_STOP = 0, S_ITERS, S_BB, S_RZ0, S_RZ1, S_COUNT };
 enum { RUN = 0, CONVERGED = 1, BREAKDOWN = 2, NONFINITE = 3 };
 
 constexpr int SP_ROWS = 4;       // block rows per spmv workgroup (one 64-lane team each)
-constexpr int UP_BODIES = 16;    // bodies per update workgroup (12 lanes each)
-constexpr int UP_T = 12 * UP_BODIES;
 constexpr int BATCH = 16;        // iterations queued between host checks
 
 // fixed-order wave sum, broadcast from lane 0
@@ -58,43 +56,47 @@ __device__ double wg_sum(double v, double* sh) {
     return r;
 }
 
-// initial: x = 0, r = -g, z = M^-1 r, p_old = 0, (r.z, r.r) partials
-__global__ __launch_bounds__(UP_T) void k_pcg_init(int B, const double* __restrict__ g, const double* __restrict__ Minv,
-                                                   double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
-                                                   double* __restrict__ p0, double* __restrict__ partB, double* __restrict__ scal,
-                                                   int* __restrict__ kbase) {
-    __shared__ double rs[UP_BODIES][12];
-    __shared__ double sh[4];
-    const int lb = threadIdx.x / 12, i = threadIdx.x % 12;
-    const int b = blockIdx.x * UP_BODIES + lb;
-    double rz = 0.0, rr = 0.0;
-    if (b < B) {
-        const size_t k = 12 * (size_t)b + i;
+// Preconditioner application for one body group (one wave per group):
+// z = M_g^-1 r on the group's rows (n = 12 bodies <= GN), rows i = lane, lane + 64.
+constexpr int GN = 96;
+struct Groups { const int* off; const int* body; const long long* moff; const double* Minv; const int* count; };
+
+// initial: x = 0, r = -g, z = M^-1 r, p_old = 0, (r.z, r.r) partials per group
+__global__ __launch_bounds__(64) void k_pcg_init(int B, Groups G, const double* __restrict__ g, double* __restrict__ x,
+                                                 double* __restrict__ r, double* __restrict__ z, double* __restrict__ p0,
+                                                 double* __restrict__ partB, double* __restrict__ scal, int* __restrict__ kbase) {
+    __shared__ double rs[GN];
+    __shared__ int bod[GN / 12];
+    const int gi = blockIdx.x, l = threadIdx.x;
+    if (gi == 0 && l == 0) { scal[S_STOP] = RUN; scal[S_ITERS] = 0.0; *kbase = 0; }
+    if (gi >= *G.count) return;
+    const int m = G.off[gi + 1] - G.off[gi], n = 12 * m;
+    if (l < m) bod[l] = G.body[G.off[gi] + l];
+    __syncthreads();
+    for (int i = l; i < n; i += 64) {
+        const size_t k = 12 * (size_t)bod[i / 12] + i % 12;
         const double rk = -g[k];
         x[k] = 0.0; r[k] = rk; p0[k] = 0.0;
-        rs[lb][i] = rk;
+        rs[i] = rk;
     }
     __syncthreads();
-    if (b < B) {
-        const double* M = Minv + 144 * (size_t)b;
+    const double* M = G.Minv + G.moff[gi];
+    double rz = 0.0, rr = 0.0;
+    for (int i = l; i < n; i += 64) {
         double zi = 0.0;
-#pragma unroll
-        for (int j = 0; j < 12; j++) zi += M[12 * j + i] * rs[lb][j];
-        z[12 * (size_t)b + i] = zi;
-        rz = rs[lb][i] * zi; rr = rs[lb][i] * rs[lb][i];
+        for (int j = 0; j < n; j++) zi += M[(size_t)j * n + i] * rs[j];
+        z[12 * (size_t)bod[i / 12] + i % 12] = zi;
+        rz += rs[i] * zi; rr += rs[i] * rs[i];
     }
-    rz = wg_sum(rz, sh);
-    rr = wg_sum(rr, sh);
-    if (threadIdx.x == 0) {
-        partB[blockIdx.x] = rz; partB[gridDim.x + blockIdx.x] = rr;
-        if (blockIdx.x == 0) { scal[S_STOP] = RUN; scal[S_ITERS] = 0.0; }
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) *kbase = 0;
+    rz = wave_sum(rz);
+    rr = wave_sum(rr);
+    if (l == 0) { partB[gi] = rz; partB[B + gi] = rr; }
 }
 // iteration base of the next batch
 __global__ void k_pcg_advance(int* kbase, int n) { if (threadIdx.x == 0 && blockIdx.x == 0) *kbase += n; }
 
-__global__ __launch_bounds__(64 * SP_ROWS) void k_pcg_spmv(int B, const int* __restrict__ kbase, int kloc, double rtol2, int nbU,
+__global__ __launch_bounds__(64 * SP_ROWS) void k_pcg_spmv(int B, const int* __restrict__ kbase, int kloc, double rtol2,
+                                                           const int* __restrict__ gcount,
                                                            const int* __restrict__ row_ptr,
                                                            const int* __restrict__ col_idx, const double* __restrict__ Hval,
                                                            const double* __restrict__ z, const double* __restrict__ pold,
@@ -106,8 +108,9 @@ __global__ __launch_bounds__(64 * SP_ROWS) void k_pcg_spmv(int B, const int* __r
     if (scal[S_STOP] != RUN) return;
     const int iter = *kbase + kloc;
     // scalars of iteration iter - 1 (identical in every wave of every workgroup)
+    const int nbU = *gcount;
     const double RZ = wave_sum_array(partB, nbU);
-    const double RR = wave_sum_array(partB + nbU, nbU);
+    const double RR = wave_sum_array(partB + B, nbU);
     const double bb = iter == 0 ? RR : scal[S_BB];
     int stop = RUN;
     if (!std::isfinite(RR) || !std::isfinite(RZ)) stop = NONFINITE;
@@ -150,44 +153,52 @@ __global__ __launch_bounds__(64 * SP_ROWS) void k_pcg_spmv(int B, const int* __r
     if (threadIdx.x == 0) partA[blockIdx.x] = prod;
 }
 
-__global__ __launch_bounds__(UP_T) void k_pcg_update(int B, const int* __restrict__ kbase, int kloc, int nbS, const double* __restrict__ Minv,
-                                                     const double* __restrict__ p, const double* __restrict__ y,
-                                                     double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
-                                                     const double* __restrict__ partA, double* __restrict__ partB,
-                                                     double* __restrict__ scal) {
-    __shared__ double rs[UP_BODIES][12];
-    __shared__ double sh[4];
+__global__ __launch_bounds__(64) void k_pcg_update(int B, const int* __restrict__ kbase, int kloc, int nbS, Groups G,
+                                                   const double* __restrict__ p, const double* __restrict__ y,
+                                                   double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
+                                                   const double* __restrict__ partA, double* __restrict__ partB,
+                                                   double* __restrict__ scal) {
+    __shared__ double rs[GN];
+    __shared__ int bod[GN / 12];
     if (scal[S_STOP] != RUN) return;
+    const int gi = blockIdx.x, l = threadIdx.x;
     const int iter = *kbase + kloc;
     const double pAp = wave_sum_array(partA, nbS);
     const double RZ = scal[S_RZ0 + (iter & 1)];
     if (!(pAp > 0.0) || !std::isfinite(pAp)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) { scal[S_STOP] = std::isfinite(pAp) ? BREAKDOWN : NONFINITE; scal[S_ITERS] = iter; }
+        if (gi == 0 && l == 0) { scal[S_STOP] = std::isfinite(pAp) ? BREAKDOWN : NONFINITE; scal[S_ITERS] = iter; }
         return;
     }
+    if (gi >= *G.count) return;
     const double alpha = RZ / pAp;
-    const int lb = threadIdx.x / 12, i = threadIdx.x % 12;
-    const int b = blockIdx.x * UP_BODIES + lb;
-    double rz = 0.0, rr = 0.0;
-    if (b < B) {
-        const size_t k = 12 * (size_t)b + i;
+    const int m = G.off[gi + 1] - G.off[gi], n = 12 * m;
+    if (l < m) bod[l] = G.body[G.off[gi] + l];
+    __syncthreads();
+    for (int i = l; i < n; i += 64) {
+        const size_t k = 12 * (size_t)bod[i / 12] + i % 12;
         x[k] += alpha * p[k];
         const double rk = r[k] - alpha * y[k];
         r[k] = rk;
-        rs[lb][i] = rk;
+        rs[i] = rk;
     }
     __syncthreads();
-    if (b < B) {
-        const double* M = Minv + 144 * (size_t)b;
+    const double* M = G.Minv + G.moff[gi];
+    double rz = 0.0, rr = 0.0;
+    for (int i = l; i < n; i += 64) {
         double zi = 0.0;
-#pragma unroll
-        for (int j = 0; j < 12; j++) zi += M[12 * j + i] * rs[lb][j];
-        z[12 * (size_t)b + i] = zi;
-        rz = rs[lb][i] * zi; rr = rs[lb][i] * rs[lb][i];
+        for (int j = 0; j < n; j++) zi += M[(size_t)j * n + i] * rs[j];
+        z[12 * (size_t)bod[i / 12] + i % 12] = zi;
+        rz += rs[i] * zi; rr += rs[i] * rs[i];
     }
-    rz = wg_sum(rz, sh);
-    rr = wg_sum(rr, sh);
-    if (threadIdx.x == 0) { partB[blockIdx.x] = rz; partB[gridDim.x + blockIdx.x] = rr; }
+    rz = wave_sum(rz);
+    rr = wave_sum(rr);
+    if (l == 0) { partB[gi] = rz; partB[B + gi] = rr; }
+}
+
+static Groups groups_of(World& w) {
+    Groups G;
+    G.off = w.grp_off.p; G.body = w.grp_body.p; G.moff = w.grp_moff.p; G.Minv = w.Ginv.p; G.count = w.grp_n.p;
+    return G;
 }
 
 // One batch = BATCH iterations; kernel k of the batch computes iteration
@@ -196,22 +207,23 @@ __global__ __launch_bounds__(UP_T) void k_pcg_update(int B, const int* __restric
 // captured graph serves every batch.
 static void enqueue_batch(World& w, int nb, bool prof) {
     const int B = w.B;
-    const int nbU = (B + UP_BODIES - 1) / UP_BODIES, nbS = (B + SP_ROWS - 1) / SP_ROWS;
+    const int nbS = (B + SP_ROWS - 1) / SP_ROWS;
     const double rtol2 = w.P.pcg_rtol * w.P.pcg_rtol;
     double* partA = w.partial.p;
     double* partB = w.partial.p + nbS;
+    const Groups G = groups_of(w);
     double* pb[2] = {w.pcg_p.p, w.pcg_p2.p};
     for (int k = 0; k < nb; k++) {
         {
             // H blocks, their column indices, z and p_old read, p_new and y written
             ProfScope ps(w, PK_SPMV, prof ? 1152.0 * w.nnzb + 4.0 * w.nnzb + 4.0 * 96.0 * B : 0.0);
-            hipLaunchKernelGGL(k_pcg_spmv, dim3(nbS), dim3(64 * SP_ROWS), 0, w.stream, B, w.pcg_k.p, k, rtol2, nbU, w.row_ptr.p,
+            hipLaunchKernelGGL(k_pcg_spmv, dim3(nbS), dim3(64 * SP_ROWS), 0, w.stream, B, w.pcg_k.p, k, rtol2, w.grp_n.p, w.row_ptr.p,
                                w.col_idx.p, w.Hval.p, w.pcg_z.p, pb[k & 1], pb[(k & 1) ^ 1], w.pcg_Ap.p, partB, partA, w.scal.p);
         }
         {
-            // M^-1 blocks read; p, y read; x, r read and written; z written
-            ProfScope ps(w, PK_PCG_UPDATE, prof ? 1152.0 * B + 7.0 * 96.0 * B : 0.0);
-            hipLaunchKernelGGL(k_pcg_update, dim3(nbU), dim3(UP_T), 0, w.stream, B, w.pcg_k.p, k, nbS, w.Pinv.p, pb[(k & 1) ^ 1],
+            // group inverses read; p, y read; x, r read and written; z written
+            ProfScope ps(w, PK_PCG_UPDATE, prof ? 8.0 * (double)w.h_grp_moff.back() + 7.0 * 96.0 * B : 0.0);
+            hipLaunchKernelGGL(k_pcg_update, dim3(B), dim3(64), 0, w.stream, B, w.pcg_k.p, k, nbS, G, pb[(k & 1) ^ 1],
                                w.pcg_Ap.p, w.pcg_x.p, w.pcg_r.p, w.pcg_z.p, partA, partB, w.scal.p);
         }
     }
@@ -220,15 +232,14 @@ static void enqueue_batch(World& w, int nb, bool prof) {
 
 int pcg_solve(World& w, double* dq_out, int* iters) {
     const int B = w.B, n = 12 * B;
-    const int nbU = (B + UP_BODIES - 1) / UP_BODIES;
     const int nbS = (B + SP_ROWS - 1) / SP_ROWS;
     w.pcg_x.reserve(n); w.pcg_r.reserve(n); w.pcg_z.reserve(n); w.pcg_p.reserve(n); w.pcg_p2.reserve(n); w.pcg_Ap.reserve(n);
-    w.partial.reserve((size_t)nbS + 2 * (size_t)nbU); w.scal.reserve(S_COUNT); w.pcg_k.reserve(1);
+    w.partial.reserve((size_t)nbS + 2 * (size_t)B); w.scal.reserve(S_COUNT); w.pcg_k.reserve(1);
     double* partB = w.partial.p + nbS;
     if (!w.h_scal) ADB_HIP(hipHostMalloc((void**)&w.h_scal, 2 * S_COUNT * sizeof(double), hipHostMallocDefault));
     for (hipEvent_t& e : w.pcg_ev)
         if (!e) ADB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    hipLaunchKernelGGL(k_pcg_init, dim3(nbU), dim3(UP_T), 0, w.stream, B, w.grad.p, w.Pinv.p, w.pcg_x.p, w.pcg_r.p, w.pcg_z.p,
+    hipLaunchKernelGGL(k_pcg_init, dim3(B), dim3(64), 0, w.stream, B, groups_of(w), w.grad.p, w.pcg_x.p, w.pcg_r.p, w.pcg_z.p,
                        w.pcg_p.p, partB, w.scal.p, w.pcg_k.p);
     ADB_HIP(hipGetLastError());
     // the batch graph is captured once per set of buffer addresses (not while
@@ -236,10 +247,11 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
     // not support graph capture)
     const bool use_graph = !w.prof.on && w.use_graphs;
     if (use_graph) {
-        const void* key[] = {w.row_ptr.p, w.col_idx.p, w.Hval.p, w.Pinv.p, w.pcg_x.p, w.pcg_r.p, w.pcg_z.p, w.pcg_p.p,
-                             w.pcg_p2.p, w.pcg_Ap.p, w.partial.p, w.scal.p, w.pcg_k.p};
+        const void* key[] = {w.row_ptr.p, w.col_idx.p, w.Hval.p, w.Ginv.p, w.pcg_x.p, w.pcg_r.p, w.pcg_z.p, w.pcg_p.p,
+                             w.pcg_p2.p, w.pcg_Ap.p, w.partial.p, w.scal.p, w.pcg_k.p, w.grp_off.p, w.grp_body.p, w.grp_moff.p,
+                             w.grp_n.p};
         bool same = w.pcg_graph != nullptr;
-        for (int i = 0; i < 13; i++) same = same && w.pcg_graph_key[i] == key[i];
+        for (int i = 0; i < 17; i++) same = same && w.pcg_graph_key[i] == key[i];
         if (!same) {
             if (w.pcg_graph) { ADB_HIP(hipGraphExecDestroy(w.pcg_graph)); w.pcg_graph = nullptr; }
             hipGraph_t g;
@@ -248,7 +260,7 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
             ADB_HIP(hipStreamEndCapture(w.stream, &g));
             ADB_HIP(hipGraphInstantiate(&w.pcg_graph, g, nullptr, nullptr, 0));
             ADB_HIP(hipGraphDestroy(g));
-            for (int i = 0; i < 13; i++) w.pcg_graph_key[i] = key[i];
+            for (int i = 0; i < 17; i++) w.pcg_graph_key[i] = key[i];
         }
     }
     int it = 0, slot = 0;
@@ -274,7 +286,7 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
     {
         const double rtol2 = w.P.pcg_rtol * w.P.pcg_rtol;
         double* partA = w.partial.p;
-        hipLaunchKernelGGL(k_pcg_spmv, dim3(nbS), dim3(64 * SP_ROWS), 0, w.stream, B, w.pcg_k.p, 0, rtol2, nbU, w.row_ptr.p,
+        hipLaunchKernelGGL(k_pcg_spmv, dim3(nbS), dim3(64 * SP_ROWS), 0, w.stream, B, w.pcg_k.p, 0, rtol2, w.grp_n.p, w.row_ptr.p,
                            w.col_idx.p, w.Hval.p, w.pcg_z.p, w.pcg_p.p, w.pcg_p2.p, w.pcg_Ap.p, partB, partA, w.scal.p);
     }
     ADB_HIP(hipGetLastError());

@@ -85,6 +85,9 @@ struct World {
     int B = 0, V = 0, F = 0, E = 0;
     int max_vpb = 0, max_fpb = 0, max_epb = 0;   // max primitives per body
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;   // side stream: 4-point contacts overlap the rest of the stage
+    hipEvent_t ev_side[2] = {nullptr, nullptr};   // [0] fork, [1] join
+    bool side_pending = false;
     std::string err;
 
     // ---- scene (device)
@@ -125,14 +128,23 @@ struct World {
     DBuf<double> chunk_part;                   // 156 per chunk (block + gradient)
     DBuf<double> Hval;               // 144 per block
     DBuf<double> grad;               // 12 B
-    DBuf<double> Pinv;               // 144 per body: inverse diagonal blocks (block-Jacobi)
+    // preconditioner groups: connected components of the body graph, split in
+    // BFS order into groups of at most GMAX bodies; per group the dense inverse
+    // of its diagonal sub-matrix (n x n, n = 12 bodies, column-major)
+    DBuf<int> grp_off, grp_body;     // group g owns grp_body[grp_off[g], grp_off[g + 1])
+    DBuf<long long> grp_moff;        // offset of group g's inverse in Ginv
+    DBuf<int> grp_n;                 // number of groups (device copy)
+    DBuf<double> Ginv;
+    int n_groups = 0;
+    std::vector<int> h_rp, h_ci, h_grp_off, h_grp_body;
+    std::vector<long long> h_grp_moff;
     int nnzb = 0, n_contrib = 0;
     // ---- PCG
     DBuf<double> pcg_x, pcg_r, pcg_z, pcg_p, pcg_p2, pcg_Ap, partial, scal;
     DBuf<int> pcg_k;                 // iteration base of the queued PCG batch
     bool use_graphs = true;          // ADB_PCG_GRAPH=0 turns graph capture off
     hipGraphExec_t pcg_graph = nullptr;   // one captured PCG batch
-    const void* pcg_graph_key[13] = {};   // buffer addresses the graph was captured with
+    const void* pcg_graph_key[17] = {};   // buffer addresses the graph was captured with
     double* h_scal = nullptr;        // pinned host copies of the PCG scalars (2 x 8)
     hipEvent_t pcg_ev[2] = {nullptr, nullptr};
     // ---- energies
@@ -148,18 +160,18 @@ struct World {
 // Brackets a kernel launch with events when profiling is on; `bytes` is the
 // launch's algorithmic HBM traffic.
 struct ProfScope {
-    World& w; int k; double bytes; hipEvent_t a = nullptr;
+    World& w; int k; double bytes; hipEvent_t a = nullptr; hipStream_t st;
     const int* snap = nullptr; double per[4] = {0, 0, 0, 0};
     void counts(const int* s, double p0, double p1, double p2, double p3) { snap = s; per[0] = p0; per[1] = p1; per[2] = p2; per[3] = p3; }
-    ProfScope(World& w_, int k_, double bytes_) : w(w_), k(k_), bytes(bytes_) {
+    ProfScope(World& w_, int k_, double bytes_, hipStream_t s = nullptr) : w(w_), k(k_), bytes(bytes_), st(s ? s : w_.stream) {
         if (!w.prof.on) return;
         a = get();
-        ADB_HIP(hipEventRecord(a, w.stream));
+        ADB_HIP(hipEventRecord(a, st));
     }
     ~ProfScope() {
         if (!w.prof.on || !a) return;
         hipEvent_t b = get();
-        (void)hipEventRecord(b, w.stream);
+        (void)hipEventRecord(b, st);
         w.prof.pending.push_back({k, a, b, bytes, snap, {per[0], per[1], per[2], per[3]}});
     }
     hipEvent_t get() {

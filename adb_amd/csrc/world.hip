@@ -255,11 +255,12 @@ extern "C" int adb_world_create(const adb_scene* s, const adb_params* p, adb_wor
 static void free_world(World& w) {
     DBuf<int>* ib[] = {&w.voff, &w.foff, &w.eoff, &w.vbody, &w.fbody, &w.ebody, &w.face, &w.edge, &w.counters,
                        &w.aval, &w.aval_s, &w.blk_of, &w.uniq_flag, &w.blk_start, &w.row_ptr, &w.col_idx, &w.diag_blk, &w.vg_start,
-                       &w.ctype, &w.clist, &w.clist3, &w.ccount, &w.chunk_n, &w.chunk_off, &w.chunk_blk};
+                       &w.ctype, &w.clist, &w.clist3, &w.ccount, &w.chunk_n, &w.chunk_off, &w.chunk_blk,
+                       &w.grp_off, &w.grp_body};
     for (auto* b : ib) b->free_();
     DBuf<double>* db[] = {&w.xbar, &w.mass, &w.Ic, &w.q, &w.q0, &w.qdot, &w.dq, &w.qtrial, &w.X, &w.Xs, &w.X1, &w.bbox,
                           &w.crec, &w.cdist, &w.chunk_part, &w.ctk, &w.clam, &w.vgK, &w.vglam, &w.bodyH, &w.bodyG, &w.Hval, &w.grad,
-                          &w.Pinv, &w.pcg_x, &w.pcg_r, &w.pcg_z, &w.pcg_p, &w.pcg_p2, &w.pcg_Ap, &w.partial, &w.scal, &w.epart};
+                          &w.Ginv, &w.pcg_x, &w.pcg_r, &w.pcg_z, &w.pcg_p, &w.pcg_p2, &w.pcg_Ap, &w.partial, &w.scal, &w.epart};
     for (auto* b : db) b->free_();
     w.pairs.free_(); w.keys_a.free_(); w.keys_b.free_(); w.sort_tmp.free_(); w.toi_bits.free_(); w.vg_keys.free_();
     w.akeys.free_(); w.akeys_s.free_();
@@ -273,9 +274,14 @@ static void free_world(World& w) {
     if (w.pcg_graph) (void)hipGraphExecDestroy(w.pcg_graph);
     w.pcg_graph = nullptr;
     w.pcg_k.free_();
+    w.grp_moff.free_();
+    w.grp_n.free_();
     if (w.h_scal) (void)hipHostFree(w.h_scal);
     w.h_scal = nullptr;
     for (hipEvent_t& e : w.pcg_ev) { if (e) (void)hipEventDestroy(e); e = nullptr; }
+    for (hipEvent_t& e : w.ev_side) { if (e) (void)hipEventDestroy(e); e = nullptr; }
+    if (w.stream2) (void)hipStreamDestroy(w.stream2);
+    w.stream2 = nullptr;
     if (w.stream) (void)hipStreamDestroy(w.stream);
     w.stream = nullptr;
 }
