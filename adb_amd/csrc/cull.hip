@@ -79,8 +79,11 @@ void q_axpy(World& w, const double* q, const double* dq, double alpha, double* o
 
 // ------------------------------------------------------------------ body boxes
 // one workgroup per body; X1 != null: union of both position sets
+// Body AABB (inflated by `infl`) and bounding sphere about the box centre,
+// both from the vertex positions at X0 (and X1: the swept set, as the
+// segments x0 -> x1 stay inside the ball of their end points).
 __global__ void k_body_box(const int* __restrict__ voff, const double* __restrict__ X0, const double* __restrict__ X1,
-                           double infl, double* __restrict__ box) {
+                           double infl, double* __restrict__ box, double* __restrict__ sph) {
     const int b = blockIdx.x;
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (int v = voff[b] + threadIdx.x; v < voff[b + 1]; v += blockDim.x) {
@@ -97,19 +100,44 @@ __global__ void k_body_box(const int* __restrict__ voff, const double* __restric
     const int wv = threadIdx.x / 64, ln = threadIdx.x % 64;
     if (ln == 0) for (int k = 0; k < 3; k++) { s[k][wv] = lo[k]; s[3 + k][wv] = hi[k]; }
     __syncthreads();
+    __shared__ double cen[3];
     if (threadIdx.x == 0) {
         for (int k = 0; k < 3; k++) {
             double l = s[k][0], h = s[3 + k][0];
             for (int i = 1; i < (int)(blockDim.x / 64); i++) { l = fmin(l, s[k][i]); h = fmax(h, s[3 + k][i]); }
             box[6 * (size_t)b + k] = l - infl;
             box[6 * (size_t)b + 3 + k] = h + infl;
+            cen[k] = 0.5 * (l + h);
         }
+    }
+    __syncthreads();
+    double r2 = 0.0;
+    for (int v = voff[b] + threadIdx.x; v < voff[b + 1]; v += blockDim.x) {
+        const double dx = X0[3 * (size_t)v] - cen[0], dy = X0[3 * (size_t)v + 1] - cen[1], dz = X0[3 * (size_t)v + 2] - cen[2];
+        r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+        if (X1) {
+            const double ex = X1[3 * (size_t)v] - cen[0], ey = X1[3 * (size_t)v + 1] - cen[1], ez = X1[3 * (size_t)v + 2] - cen[2];
+            r2 = fmax(r2, ex * ex + ey * ey + ez * ez);
+        }
+    }
+    for (int m = 32; m >= 1; m >>= 1) r2 = fmax(r2, __shfl_xor(r2, m));
+    __syncthreads();
+    if (ln == 0) s[0][wv] = r2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double rr = s[0][0];
+        for (int i = 1; i < (int)(blockDim.x / 64); i++) rr = fmax(rr, s[0][i]);
+        sph[4 * (size_t)b] = cen[0]; sph[4 * (size_t)b + 1] = cen[1]; sph[4 * (size_t)b + 2] = cen[2];
+        sph[4 * (size_t)b + 3] = sqrt(rr) * (1.0 + 1e-12) + 1e-300;
     }
 }
 
 // ------------------------------------------------------------------ body pairs
 // tiled all-pairs: thread (I, J) over a 2-D grid, only I < J, wave-aggregated append
-__global__ void k_body_pairs(int B, const double* __restrict__ box, int2* __restrict__ pairs, int cap, int* __restrict__ counter) {
+// body pairs whose (inflated) boxes overlap and whose bounding spheres are
+// within `infl` (1e-9 relative slack covers the rounding of the test)
+__global__ void k_body_pairs(int B, const double* __restrict__ box, const double* __restrict__ sph, double infl,
+                             int2* __restrict__ pairs, int cap, int* __restrict__ counter) {
     const int I = blockIdx.y * 16 + threadIdx.y;
     const int J = blockIdx.x * 16 + threadIdx.x;
     bool hit = false;
@@ -117,6 +145,13 @@ __global__ void k_body_pairs(int B, const double* __restrict__ box, int2* __rest
         const double* a = box + 6 * (size_t)I;
         const double* b = box + 6 * (size_t)J;
         hit = ovl(a, a + 3, b, b + 3);
+        if (hit) {
+            const double* sa = sph + 4 * (size_t)I;
+            const double* sb = sph + 4 * (size_t)J;
+            const double dx = sa[0] - sb[0], dy = sa[1] - sb[1], dz = sa[2] - sb[2];
+            const double lim = (sa[3] + sb[3] + infl) * (1.0 + 1e-9);
+            hit = dx * dx + dy * dy + dz * dz <= lim * lim;
+        }
     }
     unsigned long long m = __ballot(hit);
     if (m == 0) return;
@@ -159,6 +194,8 @@ __device__ __forceinline__ void append_key(bool hit, unsigned long long key, uns
 // Boxes only cull; the FP64 predicate decides, so the sets equal those of the
 // all-pairs tests (model/thread_local_culling.cpp:209-386 pt/ee_col_set_task).
 constexpr int PAIRS_PT = 4;                    // box tests per thread per round
+constexpr int EE_SLICES = 4;                   // workgroups per body pair for the EE pass
+constexpr int NARROW_TASKS = 2 + EE_SLICES;    // workgroups per body pair
 constexpr int QCAP = NB_THREADS * PAIRS_PT * 2; // candidate queue
 
 struct NarrowArgs {
@@ -312,15 +349,23 @@ __global__ __launch_bounds__(NB_THREADS) void k_pair_narrow(NarrowArgs a) {
     if (threadIdx.x == 0) qn = 0;
     double tmin = 1.0;
     const double infl = CCD ? 0.0 : a.r;
-    for (int pass = 0; pass < 3; pass++) {
-        if (pass < 2 ? !a.enable_pt : !a.enable_ee) continue;
+    // task (blockIdx.y): 0 = vertices of I vs faces of J, 1 = vertices of J vs
+    // faces of I, 2 + q = edges of I in slice q (of EE_SLICES) vs edges of J
+    const int task = blockIdx.y;
+    const int pass = task < 2 ? task : 2;
+    if (pass < 2 ? a.enable_pt : a.enable_ee) {
         const int kind = pass < 2 ? 0 : 1;                        // 0 = PT, 1 = EE
         const int bA = pass == 1 ? pr.y : pr.x, bB = pass == 1 ? pr.x : pr.y;
         const int kA = pass < 2 ? 0 : 2, kB = pass < 2 ? 1 : 2;   // primitive kinds
         const int* offA = kA == 0 ? a.voff : a.eoff;
         const int* offB = kB == 1 ? a.foff : a.eoff;
         int curA = offA[bA];
-        const int endA = offA[bA + 1];
+        int endA = offA[bA + 1];
+        if (pass == 2) {
+            const int len = endA - curA, q = task - 2;
+            const int s0 = curA + (int)((long long)len * q / EE_SLICES), s1 = curA + (int)((long long)len * (q + 1) / EE_SLICES);
+            curA = s0; endA = s1;
+        }
         while (curA < endA) {
             // the A side is inflated by r (vertex or edge of body I), as the all-pairs test
             const int nA = stage_prims<CCD, BT, CA>(a, kA, &curA, endA, xlo, xhi, infl, bxA, idA, &cntA);
@@ -420,9 +465,10 @@ static int bits_for(long long n) { int b = 1; while ((1ll << b) <= n) b++; retur
 // body boxes + pairs; returns the pair count
 static int body_pairs(World& w, const double* X0, const double* X1, double infl) {
     w.bbox.reserve(6 * (size_t)w.B);
+    w.bsph.reserve(4 * (size_t)w.B);
     {
         ProfScope ps(w, PK_BODY_BOX, (X1 ? 48.0 : 24.0) * w.V + 52.0 * w.B);
-        hipLaunchKernelGGL(k_body_box, dim3(w.B), dim3(NB_THREADS), 0, w.stream, w.voff.p, X0, X1, infl, w.bbox.p);
+        hipLaunchKernelGGL(k_body_box, dim3(w.B), dim3(NB_THREADS), 0, w.stream, w.voff.p, X0, X1, infl, w.bbox.p, w.bsph.p);
     }
     ADB_HIP(hipGetLastError());
     int npairs = 0;
@@ -433,7 +479,8 @@ static int body_pairs(World& w, const double* X0, const double* X1, double infl)
         dim3 grid((w.B + 15) / 16, (w.B + 15) / 16), blk(16, 16);
         {
             ProfScope ps(w, PK_BODY_PAIRS, 48.0 * w.B);
-            hipLaunchKernelGGL(k_body_pairs, grid, blk, 0, w.stream, w.B, w.bbox.p, w.pairs.p, (int)w.pairs.cap, w.counters.p + 8);
+            hipLaunchKernelGGL(k_body_pairs, grid, blk, 0, w.stream, w.B, w.bbox.p, w.bsph.p, infl, w.pairs.p, (int)w.pairs.cap,
+                               w.counters.p + 8);
         }
         ADB_HIP(hipGetLastError());
         ADB_HIP(hipMemcpyAsync(&npairs, w.counters.p + 8, sizeof(int), hipMemcpyDeviceToHost, w.stream));
@@ -472,8 +519,9 @@ void collision_set(World& w, const double* X, ContactSet& out) {
         a.ee_keys = w.keys_a.p + pcap; a.ee_cap = (int)pcap;
         if (npairs > 0) {
             ProfScope ps(w, PK_NARROW_SET, 0.0);
-            if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<0, float>), dim3(npairs), dim3(NB_THREADS), 0, w.stream, a);
-            else hipLaunchKernelGGL((k_pair_narrow<0, double>), dim3(npairs), dim3(NB_THREADS), 0, w.stream, a);
+            const dim3 grid(npairs, NARROW_TASKS);
+            if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<0, float>), grid, dim3(NB_THREADS), 0, w.stream, a);
+            else hipLaunchKernelGGL((k_pair_narrow<0, double>), grid, dim3(NB_THREADS), 0, w.stream, a);
             ADB_HIP(hipGetLastError());
         }
         int h[4];
@@ -540,8 +588,9 @@ double ccd_toi(World& w, const double* X0, const double* X1, int* status) {
     a.toi_bits = w.toi_bits.p;
     if (npairs > 0) {
         ProfScope ps(w, PK_NARROW_CCD, 0.0);
-        if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<1, float>), dim3(npairs), dim3(NB_THREADS), 0, w.stream, a);
-        else hipLaunchKernelGGL((k_pair_narrow<1, double>), dim3(npairs), dim3(NB_THREADS), 0, w.stream, a);
+        const dim3 grid(npairs, NARROW_TASKS);
+        if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<1, float>), grid, dim3(NB_THREADS), 0, w.stream, a);
+        else hipLaunchKernelGGL((k_pair_narrow<1, double>), grid, dim3(NB_THREADS), 0, w.stream, a);
         ADB_HIP(hipGetLastError());
     }
     unsigned long long bits[2];

@@ -98,6 +98,7 @@ struct World {
     DBuf<double> X, Xs, X1;          // world vertex positions: current, step start, trial
     // ---- culling scratch
     DBuf<double> bbox;               // 6 per body
+    DBuf<double> bsph;               // 4 per body: bounding sphere centre, radius
     DBuf<int2> pairs;                // body pairs (I < J)
     DBuf<int> counters;              // device counters
     DBuf<unsigned long long> keys_a, keys_b;  // unsorted / sorted keys

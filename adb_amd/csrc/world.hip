@@ -258,7 +258,7 @@ static void free_world(World& w) {
                        &w.ctype, &w.clist, &w.clist3, &w.ccount, &w.chunk_n, &w.chunk_off, &w.chunk_blk,
                        &w.grp_off, &w.grp_body};
     for (auto* b : ib) b->free_();
-    DBuf<double>* db[] = {&w.xbar, &w.mass, &w.Ic, &w.q, &w.q0, &w.qdot, &w.dq, &w.qtrial, &w.X, &w.Xs, &w.X1, &w.bbox,
+    DBuf<double>* db[] = {&w.xbar, &w.mass, &w.Ic, &w.q, &w.q0, &w.qdot, &w.dq, &w.qtrial, &w.X, &w.Xs, &w.X1, &w.bbox, &w.bsph,
                           &w.crec, &w.cdist, &w.chunk_part, &w.ctk, &w.clam, &w.vgK, &w.vglam, &w.bodyH, &w.bodyG, &w.Hval, &w.grad,
                           &w.Ginv, &w.pcg_x, &w.pcg_r, &w.pcg_z, &w.pcg_p, &w.pcg_p2, &w.pcg_Ap, &w.partial, &w.scal, &w.epart};
     for (auto* b : db) b->free_();
