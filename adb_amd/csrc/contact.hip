@@ -47,6 +47,10 @@ constexpr int MAX_SWEEPS = 15;
 // for the projected Hessian, and reachable in a few sweeps, unlike 1e-30,
 // which sits at the FP64 noise floor of the rotations themselves)
 constexpr double JACOBI_TOL = 1e-24;
+// rotations of off-diagonal entries below JACOBI_SKIP * |A|_F are skipped: 36
+// such entries add at most 36e-28 |A|_F^2 to the off-diagonal sum, far below
+// the stop test, and late sweeps then cost little
+constexpr double JACOBI_SKIP = 1e-14;
 
 struct GroupLDS {
     double H[12][LDH];
@@ -639,12 +643,14 @@ __device__ __forceinline__ void jacobi_reg(double (&A)[N][N], double (&v)[N][N])
 #pragma unroll
             for (int q = 0; q < N; q++) { const double x = A[p][q] * A[p][q]; tot += x; if (p != q) off += x; }
         if (off <= JACOBI_TOL * tot || off == 0.0) break;
+        // entries this small cannot hold the sweep above the stop test: skip them
+        const double thr = JACOBI_SKIP * sqrt(tot);
 #pragma unroll
         for (int p = 0; p < N - 1; p++)
 #pragma unroll
             for (int q = p + 1; q < N; q++) {
                 const double apq = A[p][q];
-                const bool nz = apq != 0.0;
+                const bool nz = fabs(apq) > thr;
                 const double tau = (A[q][q] - A[p][p]) / (2.0 * (nz ? apq : 1.0));
                 double t = fabs(tau) > 1e150 ? 0.5 / tau : (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
                 t = nz ? t : 0.0;
@@ -664,6 +670,7 @@ __device__ __forceinline__ void jacobi_reg(double (&A)[N][N], double (&v)[N][N])
                     const double vp = v[k][p], vq = v[k][q];
                     v[k][p] = c * vp - s * vq; v[k][q] = s * vp + c * vq;
                 }
+                __builtin_amdgcn_sched_barrier(0);
             }
     }
 }
@@ -837,7 +844,7 @@ __device__ __forceinline__ void contact_small(const WorldArgs& a, int cid) {
 __host__ __device__ constexpr int P9(int p, int q) {
     return p <= q ? p * 9 - p * (p - 1) / 2 + (q - p) : q * 9 - q * (q - 1) / 2 + (p - q);
 }
-__device__ __forceinline__ void contact_c3(const WorldArgs& a, int cid, double (*V)[64], double (*As)[64], int lane) {
+__device__ __forceinline__ void contact_c3(const WorldArgs& a, int cid, double (*V)[64], int lane) {
     double* R = a.rec + REC * (size_t)cid;
     {
         Geo G;
@@ -948,62 +955,82 @@ __device__ __forceinline__ void contact_c3(const WorldArgs& a, int cid, double (
         write_grad(a, cid, nside0, w, gv);
         R[R_D] = rec_code(a, cid, 9);
     }
-    // PK: packed K into As, V reused for the eigenvectors; runtime loops
-    // keep the register footprint small (both matrices are per-lane LDS columns)
+    // PK: packed K and V in registers (static indices); a scheduling barrier
+    // after each rotation keeps the live ranges to one rotation
+    double A[45];
+#pragma unroll
     for (int p = 0; p < 9; p++)
-        for (int q = p; q < 9; q++) As[P9(p, q)][lane] = V[9 * p + q][lane];
+#pragma unroll
+        for (int q = p; q < 9; q++) A[P9(p, q)] = V[9 * p + q][lane];
     if (a.kp.psd) {
-        for (int e = 0; e < 81; e++) V[e][lane] = (e % 10 == 0) ? 1.0 : 0.0;
+        double Vr[81];
+#pragma unroll
+        for (int e = 0; e < 81; e++) Vr[e] = (e % 10 == 0) ? 1.0 : 0.0;
         for (int sweep = 0; sweep < MAX_SWEEPS; sweep++) {
             double off = 0.0, tot = 0.0;
+#pragma unroll
             for (int p = 0; p < 9; p++)
+#pragma unroll
                 for (int q = p; q < 9; q++) {
-                    const double x = As[P9(p, q)][lane] * As[P9(p, q)][lane];
+                    const double x = A[P9(p, q)] * A[P9(p, q)];
                     if (p == q) tot += x; else { off += 2.0 * x; tot += 2.0 * x; }
                 }
             if (off <= JACOBI_TOL * tot || off == 0.0) break;
+            const double thr = JACOBI_SKIP * sqrt(tot);
+#pragma unroll
             for (int p = 0; p < 8; p++)
+#pragma unroll
                 for (int q = p + 1; q < 9; q++) {
-                    const double apq = As[P9(p, q)][lane];
-                    if (apq == 0.0) continue;
-                    const double app = As[P9(p, p)][lane], aqq = As[P9(q, q)][lane];
-                    const double tau = (aqq - app) / (2.0 * apq);
-                    const double t = fabs(tau) > 1e150 ? 0.5 / tau : (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                    const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
-                    for (int k = 0; k < 9; k++) {
-                        if (k == p || k == q) continue;
-                        const int ip = P9(k, p), iq = P9(k, q);
-                        const double akp = As[ip][lane], akq = As[iq][lane];
-                        As[ip][lane] = c * akp - s * akq;
-                        As[iq][lane] = s * akp + c * akq;
+                    const double apq = A[P9(p, q)];
+                    if (fabs(apq) > thr) {
+                        const double app = A[P9(p, p)], aqq = A[P9(q, q)];
+                        const double tau = (aqq - app) / (2.0 * apq);
+                        const double t = fabs(tau) > 1e150 ? 0.5 / tau : (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+#pragma unroll
+                        for (int k = 0; k < 9; k++) {
+                            if (k == p || k == q) continue;
+                            const double akp = A[P9(k, p)], akq = A[P9(k, q)];
+                            A[P9(k, p)] = c * akp - s * akq;
+                            A[P9(k, q)] = s * akp + c * akq;
+                        }
+                        A[P9(p, p)] = c * c * app - 2.0 * c * s * apq + s * s * aqq;
+                        A[P9(q, q)] = s * s * app + 2.0 * c * s * apq + c * c * aqq;
+                        A[P9(p, q)] = 0.0;
+#pragma unroll
+                        for (int k = 0; k < 9; k++) {
+                            const double vkp = Vr[9 * k + p], vkq = Vr[9 * k + q];
+                            Vr[9 * k + p] = c * vkp - s * vkq;
+                            Vr[9 * k + q] = s * vkp + c * vkq;
+                        }
                     }
-                    As[P9(p, p)][lane] = c * c * app - 2.0 * c * s * apq + s * s * aqq;
-                    As[P9(q, q)][lane] = s * s * app + 2.0 * c * s * apq + c * c * aqq;
-                    As[P9(p, q)][lane] = 0.0;
-                    for (int k = 0; k < 9; k++) {
-                        const double vkp = V[9 * k + p][lane], vkq = V[9 * k + q][lane];
-                        V[9 * k + p][lane] = c * vkp - s * vkq;
-                        V[9 * k + q][lane] = s * vkp + c * vkq;
-                    }
+                    __builtin_amdgcn_sched_barrier(0);
                 }
         }
+        double wv[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) wv[k] = fmax(A[P9(k, k)], 0.0);
+#pragma unroll
         for (int p = 0; p < 9; p++)
+#pragma unroll
             for (int q = p; q < 9; q++) {
                 double s = 0.0;
-                for (int k = 0; k < 9; k++) s += V[9 * p + k][lane] * fmax(As[P9(k, k)][lane], 0.0) * V[9 * q + k][lane];
+#pragma unroll
+                for (int k = 0; k < 9; k++) s += Vr[9 * p + k] * wv[k] * Vr[9 * q + k];
                 R[R_PK + 9 * p + q] = s;
                 R[R_PK + 9 * q + p] = s;
             }
     } else {
+#pragma unroll
         for (int p = 0; p < 9; p++)
-            for (int q = p; q < 9; q++) { R[R_PK + 9 * p + q] = As[P9(p, q)][lane]; R[R_PK + 9 * q + p] = As[P9(p, q)][lane]; }
+#pragma unroll
+            for (int q = p; q < 9; q++) { R[R_PK + 9 * p + q] = A[P9(p, q)]; R[R_PK + 9 * q + p] = A[P9(p, q)]; }
     }
 }
 __global__ __launch_bounds__(64) void k_contact_c3(WorldArgs a) {
     __shared__ double V[81][64];
-    __shared__ double As[45][64];
     const int n3 = a.counts[2];
-    for (int p = blockIdx.x * 64 + threadIdx.x; p < n3; p += gridDim.x * 64) contact_c3(a, a.list3[p], V, As, threadIdx.x);
+    for (int p = blockIdx.x * 64 + threadIdx.x; p < n3; p += gridDim.x * 64) contact_c3(a, a.list3[p], V, threadIdx.x);
 }
 
 // grid-strided over the class list: class 1 at list[0, n1), class 2 at
