@@ -69,7 +69,11 @@ __global__ void k_inertia(int B, double dt, double kap, double ga0, double ga1, 
 }
 
 // contribution keys: B diagonal markers first, then 4 per PT/EE contact
+// value of a contact contribution: slot (bits 0-1), number of Helmert rows
+// nA = D / 3 (bits 2-3), used (bit 4: the block's bodies are dynamic and the
+// contact active), contact id (bits 5+); body markers are -1 - body
 __global__ void k_contrib_keys(int B, int n_pt, int n_ee, const int4* __restrict__ pt, const int4* __restrict__ ee,
+                               const int* __restrict__ ccls, const double* __restrict__ mass,
                                unsigned long long* __restrict__ keys, int* __restrict__ vals) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int nc = n_pt + n_ee;
@@ -87,7 +91,9 @@ __global__ void k_contrib_keys(int B, int n_pt, int n_ee, const int4* __restrict
     default: r = J; col = I; break;
     }
     keys[B + i] = ((unsigned long long)r << 32) | (unsigned long long)col;
-    vals[B + i] = i;
+    const bool di = mass[I] > 0.0, dj = mass[J] > 0.0;
+    const bool use = (slot == 0 ? di : (slot == 1 ? dj : (di && dj))) && ccls[c] > 0;
+    vals[B + i] = (c << 5) | ((use ? 1 : 0) << 4) | (ccls[c] << 2) | slot;
 }
 __global__ void k_flags(int n, const unsigned long long* __restrict__ keys, int* __restrict__ flag) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -176,9 +182,9 @@ __global__ __launch_bounds__(64 * RW) void k_reduce_chunks(ReduceArgs a, const i
     const int w = threadIdx.x / 64, l = threadIdx.x % 64;
     const int slot = l / 12, col = l % 12, be = col / 3, cc = col % 3;
     const int wb = min(beg + w * (RCH / RW), end), we = min(wb + RCH / RW, end);
-    double acc[12], gacc[12];
+    double acc[12], gacc = 0.0;
 #pragma unroll
-    for (int i = 0; i < 12; i++) { acc[i] = 0.0; gacc[i] = 0.0; }
+    for (int i = 0; i < 12; i++) acc[i] = 0.0;
     for (int r0 = wb; r0 < we; r0 += RS) {
         const int s = r0 + slot;
         if (slot < RS && s < we) {
@@ -187,20 +193,13 @@ __global__ __launch_bounds__(64 * RW) void k_reduce_chunks(ReduceArgs a, const i
                 const int b = -1 - v;
 #pragma unroll
                 for (int i = 0; i < 12; i++) acc[i] += a.bodyH[144 * (size_t)b + 12 * col + i];
-                if (col == 0) {
-#pragma unroll
-                    for (int i = 0; i < 12; i++) gacc[i] += a.bodyG[12 * (size_t)b + i];
-                }
-            } else {
-                const int c = v / 4, cs = v % 4;
+                gacc += a.bodyG[12 * (size_t)b + col];
+            } else if ((v >> 4) & 1) {
+                const int c = v >> 5, cs = v & 3, nA = (v >> 2) & 3, D = 3 * nA;
                 const double* R = a.rec + REC * (size_t)c;
-                const int code = (int)R[R_D];
-                const int D = code % 100, di = (code / 100) & 1, dj = (code / 100) >> 1;
-                const bool use = cs == 0 ? di : (cs == 1 ? dj : (di && dj));
-                if (use && D > 0) {
+                {
                     // cs 0 = (I, I), 1 = (J, J), 2 = (I, J), 3 = (J, I)
                     const int s1 = (cs == 1 || cs == 3) ? 1 : 0, s2 = (cs == 1 || cs == 2) ? 1 : 0;
-                    const int nA = D / 3;
                     double t[3][3];
 #pragma unroll
                     for (int aa = 0; aa < 3; aa++)
@@ -224,10 +223,7 @@ __global__ __launch_bounds__(64 * RW) void k_reduce_chunks(ReduceArgs a, const i
                         acc[3 * al + 1] += (z[0] * t[0][1] + z[1] * t[1][1] + z[2] * t[2][1]) + gg * F1;
                         acc[3 * al + 2] += (z[0] * t[0][2] + z[1] * t[1][2] + z[2] * t[2][2]) + gg * F2;
                     }
-                    if (cs < 2 && col == 0) {
-#pragma unroll
-                        for (int i = 0; i < 12; i++) gacc[i] += R[R_GRAD + 12 * s1 + i];
-                    }
+                    if (cs < 2) gacc += R[R_GRAD + 12 * s1 + col];
                 }
             }
         }
@@ -235,10 +231,7 @@ __global__ __launch_bounds__(64 * RW) void k_reduce_chunks(ReduceArgs a, const i
     if (slot < RS) {
 #pragma unroll
         for (int i = 0; i < 12; i++) P[w][slot][12 * col + i] = acc[i];
-        if (col == 0) {
-#pragma unroll
-            for (int i = 0; i < 12; i++) P[w][slot][144 + i] = gacc[i];
-        }
+        P[w][slot][144 + col] = gacc;
     }
     __syncthreads();
     for (int e = threadIdx.x; e < 156; e += 64 * RW) {
@@ -455,6 +448,7 @@ void assemble_system(World& w) {
     w.akeys.reserve(n); w.akeys_s.reserve(n); w.aval.reserve(n); w.aval_s.reserve(n);
     w.uniq_flag.reserve(n); w.blk_of.reserve(n);
     hipLaunchKernelGGL(k_contrib_keys, dim3((n + 255) / 256), dim3(256), 0, w.stream, B, w.cs.n_pt, w.cs.n_ee, w.cs.pt.p, w.cs.ee.p,
+                       w.ccls.p, w.mass.p,
                        w.akeys.p, w.aval.p);
     ADB_HIP(hipGetLastError());
     int bb = 1; while ((1ll << bb) <= B) bb++;

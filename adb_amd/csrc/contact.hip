@@ -470,7 +470,7 @@ struct WorldArgs {
     const int4* pt; const int4* ee;
     const int* voff; const int* foff; const int* eoff; const int* face; const int* edge;
     const double* X; const double* Xs; const double* xbar; const double* mass;
-    int* ctype; double* cdist; int* list; int* list3; int* counts;
+    int* ctype; double* cdist; int* ccls; int* list; int* list3; int* counts;
     double* rec; double* tk; double* lam;
 };
 
@@ -540,6 +540,7 @@ __global__ __launch_bounds__(256) void k_contact_classify(WorldArgs a) {
             a.lam[cid] = 0.0;
         }
     }
+    if (cid < total) a.ccls[cid] = cls;
     const int total_n = total;
     const int p1 = wave_append(cls == 1, a.counts + 0);
     const int p2 = wave_append(cls == 2, a.counts + 1);
@@ -1098,7 +1099,7 @@ void contact_terms(World& w) {
     const int n = w.cs.n_pt + w.cs.n_ee;
     const size_t nn = (size_t)std::max(n, 1);
     w.crec.reserve(REC * nn);
-    w.ctype.reserve(nn); w.cdist.reserve(nn); w.clist.reserve(nn); w.clist3.reserve(nn);
+    w.ctype.reserve(nn); w.cdist.reserve(nn); w.ccls.reserve(nn); w.clist.reserve(nn); w.clist3.reserve(nn);
     w.ccount.reserve(4);
     w.ctk.reserve(24 * nn);
     w.clam.reserve(nn);
@@ -1110,7 +1111,7 @@ void contact_terms(World& w) {
         a.pt = w.cs.pt.p; a.ee = w.cs.ee.p;
         a.voff = w.voff.p; a.foff = w.foff.p; a.eoff = w.eoff.p; a.face = w.face.p; a.edge = w.edge.p;
         a.X = w.X.p; a.Xs = w.Xs.p; a.xbar = w.xbar.p; a.mass = w.mass.p;
-        a.ctype = w.ctype.p; a.cdist = w.cdist.p; a.list = w.clist.p; a.list3 = w.clist3.p; a.counts = w.ccount.p;
+        a.ctype = w.ctype.p; a.cdist = w.cdist.p; a.ccls = w.ccls.p; a.list = w.clist.p; a.list3 = w.clist3.p; a.counts = w.ccount.p;
         a.rec = w.crec.p; a.tk = w.ctk.p; a.lam = w.clam.p;
         {
             // ids and 4 points in; type, distance and list slot out

@@ -110,6 +110,7 @@ struct World {
     // ---- contact terms (per PT/EE contact, then per VG contact)
     DBuf<double> crec;               // REC per PT/EE contact: compact body-space record
     DBuf<int> ctype;                 // distance type per contact
+    DBuf<int> ccls;                  // class per contact: 0 inactive, 1..3 = D / 3
     DBuf<double> cdist;              // squared distance per contact
     DBuf<int> clist;                 // class 1 from the front, class 2 from the back
     DBuf<int> clist3;                // class 3 contacts
