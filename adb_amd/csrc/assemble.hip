@@ -293,24 +293,26 @@ __global__ __launch_bounds__(64 * RW) void k_reduce_final(ReduceArgs a, const in
 // upper triangle), then M^-1 = W^T W.  A group whose matrix is not positive
 // definite falls back to the inverse diagonal.
 constexpr int GMAX = 8;
-constexpr int GN = 12 * GMAX;
-constexpr int GLD = GN + 1;
-__global__ __launch_bounds__(256) void k_group_inv(const int* __restrict__ grp_off, const int* __restrict__ grp_body,
-                                                   const long long* __restrict__ grp_moff, const int* __restrict__ row_ptr,
-                                                   const int* __restrict__ col_idx, const double* __restrict__ Hval,
-                                                   double* __restrict__ Ginv) {
+// GM: largest group size of the launch (1 for single bodies, GMAX otherwise);
+// list: the groups of this launch
+template <int GM>
+__global__ __launch_bounds__(256) void k_group_inv(const int* __restrict__ list, const int* __restrict__ grp_off,
+                                                   const int* __restrict__ grp_body, const long long* __restrict__ grp_moff,
+                                                   const int* __restrict__ row_ptr, const int* __restrict__ col_idx,
+                                                   const double* __restrict__ Hval, double* __restrict__ Ginv) {
+    constexpr int GN = 12 * GM, GLD = GN + 1;
     __shared__ double A[GN * GLD];
     __shared__ double wd[GN];
-    __shared__ int blk[GMAX][GMAX];
-    __shared__ int bod[GMAX];
+    __shared__ int blk[GM][GM];
+    __shared__ int bod[GM];
     __shared__ int ok;
-    const int g = blockIdx.x, t = threadIdx.x;
+    const int g = list[blockIdx.x], t = threadIdx.x;
     const int m = grp_off[g + 1] - grp_off[g], n = 12 * m;
     if (t < m) bod[t] = grp_body[grp_off[g] + t];
     if (t == 0) ok = 1;
     __syncthreads();
-    if (t < m * m) {
-        const int ia = t / m, ib = t % m;
+    for (int e = t; e < m * m; e += blockDim.x) {
+        const int ia = e / m, ib = e % m;
         int k = -1;
         for (int q = row_ptr[bod[ia]]; q < row_ptr[bod[ia] + 1]; q++)
             if (col_idx[q] == bod[ib]) { k = q; break; }
@@ -357,8 +359,7 @@ __global__ __launch_bounds__(256) void k_group_inv(const int* __restrict__ grp_o
         return;
     }
     // W = L^-1: thread c builds column c (W[i][c], i >= c) into A[c][i] (upper) and wd[c]
-    if (t < n) {
-        const int c = t;
+    for (int c = t; c < n; c += blockDim.x) {
         const double wcc = 1.0 / A[c * GLD + c];
         wd[c] = wcc;
         for (int i = c + 1; i < n; i++) {
@@ -417,6 +418,12 @@ static void build_groups(World& w) {
         }
     }
     w.n_groups = (int)w.h_grp_off.size() - 1;
+    // groups with several bodies first: their inverses take longest
+    w.h_glist.clear();
+    for (int g = 0; g < w.n_groups; g++) if (w.h_grp_off[g + 1] - w.h_grp_off[g] > 1) w.h_glist.push_back(g);
+    for (int g = 0; g < w.n_groups; g++) if (w.h_grp_off[g + 1] - w.h_grp_off[g] == 1) w.h_glist.push_back(g);
+    w.glist.reserve(std::max(w.n_groups, 1));
+    ADB_HIP(hipMemcpyAsync(w.glist.p, w.h_glist.data(), w.n_groups * sizeof(int), hipMemcpyHostToDevice, w.stream));
     w.grp_off.reserve(w.n_groups + 1); w.grp_body.reserve(B); w.grp_moff.reserve(w.n_groups + 1); w.grp_n.reserve(1);
     ADB_HIP(hipMemcpyAsync(w.grp_n.p, &w.n_groups, sizeof(int), hipMemcpyHostToDevice, w.stream));
     ADB_HIP(hipMemcpyAsync(w.grp_off.p, w.h_grp_off.data(), (w.n_groups + 1) * sizeof(int), hipMemcpyHostToDevice, w.stream));
@@ -521,8 +528,8 @@ void assemble_system(World& w) {
     build_groups(w);
     {
         ProfScope ps(w, PK_BLOCK_CHOL, 0.0);
-        hipLaunchKernelGGL(k_group_inv, dim3(w.n_groups), dim3(256), 0, w.stream, w.grp_off.p, w.grp_body.p, w.grp_moff.p,
-                           w.row_ptr.p, w.col_idx.p, w.Hval.p, w.Ginv.p);
+        hipLaunchKernelGGL(k_group_inv<GMAX>, dim3(w.n_groups), dim3(256), 0, w.stream, w.glist.p, w.grp_off.p, w.grp_body.p,
+                           w.grp_moff.p, w.row_ptr.p, w.col_idx.p, w.Hval.p, w.Ginv.p);
         ADB_HIP(hipGetLastError());
     }
 }

@@ -1034,6 +1034,206 @@ __global__ __launch_bounds__(64) void k_contact_c3(WorldArgs a) {
     for (int p = blockIdx.x * 64 + threadIdx.x; p < n3; p += gridDim.x * 64) contact_c3(a, a.list3[p], V, threadIdx.x);
 }
 
+// record fields other than PK of a 3-point contact: Z, friction, gradient, D
+__device__ void scalars_c2(const WorldArgs& a, int cid) {
+    int id0, id1, id2, id3;
+    const int is_ee = contact_ids(a, cid, id0, id1, id2, id3);
+    const d3 P0 = ld3(a.X + 3 * (size_t)id0), P1 = ld3(a.X + 3 * (size_t)id1);
+    const d3 P2 = ld3(a.X + 3 * (size_t)id2), P3 = ld3(a.X + 3 * (size_t)id3);
+    const int type = a.ctype[cid];
+    const double d = a.cdist[cid];
+    const RedMap mp = is_ee ? ee_map(type) : pt_map(type);
+    const int nside0 = is_ee ? 2 : 1;
+    const double B1 = barrier_d1(d, a.kp.d_hat, a.kp.kappa);
+    double* R = a.rec + REC * (size_t)cid;
+    const d3 z = mk(0, 0, 0);
+    Fric f;
+    friction_eval(a, is_ee, type, d, B1, P0, P1, P2, P3, id0, id1, id2, id3, f);
+    double w[4][4];
+    load_w(a.xbar, id0, id1, id2, id3, w);
+    write_fric(a, cid, nside0, f, w);
+    const int ka = mp.plus[0], kb = mp.plus[1], km = mp.minus[0];
+    const d3 pm = pick(P0, P1, P2, P3, km);
+    RedDist rd;
+    rd.init(K_PL, pick(P0, P1, P2, P3, ka) - pm, pick(P0, P1, P2, P3, kb) - pm, z);
+    const d3 g0 = mk(B1 * rd.g[0].x, B1 * rd.g[0].y, B1 * rd.g[0].z);
+    const d3 g1 = mk(B1 * rd.g[1].x, B1 * rd.g[1].y, B1 * rd.g[1].z);
+    const d3 gm = mk(-(g0.x + g1.x), -(g0.y + g1.y), -(g0.z + g1.z));
+    const double gmv[4] = {f.gm0, f.gm1, f.gm2, f.gm3};
+    d3 ft = z;
+    if (f.on) {
+        const double t0 = f.fs * f.u0, t1 = f.fs * f.u1;
+        ft = mk(t0 * f.c0.x + t1 * f.c1.x, t0 * f.c0.y + t1 * f.c1.y, t0 * f.c0.z + t1 * f.c1.z);
+    }
+    d3 gv[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const d3 gb = k == ka ? g0 : (k == kb ? g1 : (k == km ? gm : z));
+        gv[k] = mk(gb.x + gmv[k] * ft.x, gb.y + gmv[k] * ft.y, gb.z + gmv[k] * ft.z);
+    }
+    write_grad(a, cid, nside0, w, gv);
+    // Helmert rows over the slots (m, a, b): (1, -1, 0) / sqrt 2, (1, 1, -2) / sqrt 6
+#pragma unroll
+    for (int aa = 0; aa < 2; aa++)
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+            for (int al = 0; al < 4; al++) {
+                double zz = 0.0;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const double h = aa == 0 ? (k == km ? HS2 : (k == ka ? -HS2 : 0.0))
+                                             : (k == km ? HS6 : (k == ka ? HS6 : (k == kb ? -2.0 * HS6 : 0.0)));
+                    if ((k < nside0) == (s == 0)) zz += h * w[k][al];
+                }
+                R[R_Z + 8 * aa + 4 * s + al] = zz;
+            }
+    R[R_D] = rec_code(a, cid, 6);
+}
+
+// packed upper-triangle index of an N x N symmetric matrix
+template <int N> __host__ __device__ constexpr int PK_(int p, int q) {
+    return p <= q ? p * N - p * (p - 1) / 2 + (q - p) : q * N - q * (q - 1) / 2 + (p - q);
+}
+// Cyclic Jacobi on a packed symmetric matrix A (registers, static indices)
+// with the eigenvectors V (row-major N x N, registers); rotations of entries
+// below JACOBI_SKIP |A|_F are skipped, and a scheduling barrier after each
+// rotation bounds the live ranges.  Returns with the eigenvalues on A's diagonal.
+template <int N>
+__device__ __forceinline__ void jacobi_packed(double (&A)[N * (N + 1) / 2], double (&V)[N * N]) {
+#pragma unroll
+    for (int e = 0; e < N * N; e++) V[e] = (e % (N + 1) == 0) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < MAX_SWEEPS; sweep++) {
+        double off = 0.0, tot = 0.0;
+#pragma unroll
+        for (int p = 0; p < N; p++)
+#pragma unroll
+            for (int q = p; q < N; q++) {
+                const double x = A[PK_<N>(p, q)] * A[PK_<N>(p, q)];
+                if (p == q) tot += x; else { off += 2.0 * x; tot += 2.0 * x; }
+            }
+        if (off <= JACOBI_TOL * tot || off == 0.0) break;
+        const double thr = JACOBI_SKIP * sqrt(tot);
+#pragma unroll
+        for (int p = 0; p < N - 1; p++)
+#pragma unroll
+            for (int q = p + 1; q < N; q++) {
+                const double apq = A[PK_<N>(p, q)];
+                if (fabs(apq) > thr) {
+                    const double app = A[PK_<N>(p, p)], aqq = A[PK_<N>(q, q)];
+                    const double tau = (aqq - app) / (2.0 * apq);
+                    const double t = fabs(tau) > 1e150 ? 0.5 / tau : (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                    const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+#pragma unroll
+                    for (int k = 0; k < N; k++) {
+                        if (k == p || k == q) continue;
+                        const double akp = A[PK_<N>(k, p)], akq = A[PK_<N>(k, q)];
+                        A[PK_<N>(k, p)] = c * akp - s * akq;
+                        A[PK_<N>(k, q)] = s * akp + c * akq;
+                    }
+                    A[PK_<N>(p, p)] = c * c * app - 2.0 * c * s * apq + s * s * aqq;
+                    A[PK_<N>(q, q)] = s * s * app + 2.0 * c * s * apq + c * c * aqq;
+                    A[PK_<N>(p, q)] = 0.0;
+#pragma unroll
+                    for (int k = 0; k < N; k++) {
+                        const double vkp = V[N * k + p], vkq = V[N * k + q];
+                        V[N * k + p] = c * vkp - s * vkq;
+                        V[N * k + q] = s * vkp + c * vkq;
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+    }
+}
+// PK = V diag(max(w, 0)) V^T into R[R_PK], row-major N x N
+template <int N>
+__device__ __forceinline__ void write_projection(const double (&A)[N * (N + 1) / 2], const double (&V)[N * N], double* R) {
+    double wv[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) wv[k] = fmax(A[PK_<N>(k, k)], 0.0);
+#pragma unroll
+    for (int p = 0; p < N; p++)
+#pragma unroll
+        for (int q = p; q < N; q++) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < N; k++) s += V[N * p + k] * wv[k] * V[N * q + k];
+            R[R_PK + N * p + q] = s;
+            R[R_PK + N * q + p] = s;
+        }
+}
+
+// PK of a 3-point contact (point-line distance): K = (C (x) I3) Mred (C^T (x) I3),
+// 6x6 Jacobi in registers
+__device__ __forceinline__ void pk_c2(const WorldArgs& a, int cid) {
+    int id0, id1, id2, id3;
+    const int is_ee = contact_ids(a, cid, id0, id1, id2, id3);
+    const d3 P0 = ld3(a.X + 3 * (size_t)id0), P1 = ld3(a.X + 3 * (size_t)id1);
+    const d3 P2 = ld3(a.X + 3 * (size_t)id2), P3 = ld3(a.X + 3 * (size_t)id3);
+    const int type = a.ctype[cid];
+    const double d = a.cdist[cid];
+    const RedMap mp = is_ee ? ee_map(type) : pt_map(type);
+    const double B1 = barrier_d1(d, a.kp.d_hat, a.kp.kappa);
+    const double B2 = barrier_d2(d, a.kp.d_hat, a.kp.kappa);
+    double* R = a.rec + REC * (size_t)cid;
+    const int ka = mp.plus[0], kb = mp.plus[1], km = mp.minus[0];
+    const d3 pm = pick(P0, P1, P2, P3, km);
+    RedDist rd;
+    rd.init(K_PL, pick(P0, P1, P2, P3, ka) - pm, pick(P0, P1, P2, P3, kb) - pm, mk(0, 0, 0));
+    double A[21];
+    {
+        double M[6][6];
+#pragma unroll
+        for (int u = 0; u < 6; u++)
+#pragma unroll
+            for (int v = u; v < 6; v++) {
+                const int m = u / 3, i = u % 3, n = v / 3, j = v % 3;
+                M[u][v] = B1 * rd.hess(m, i, n, j) + B2 * cmp(rd.g[m], i) * cmp(rd.g[n], j);
+                M[v][u] = M[u][v];
+            }
+        const double C[2][2] = {{-2.0 * HS2, -HS2}, {0.0, -3.0 * HS6}};
+#pragma unroll
+        for (int aa = 0; aa < 2; aa++)
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int bb = 0; bb < 2; bb++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) {
+                        double s = 0.0;
+#pragma unroll
+                        for (int m = 0; m < 2; m++)
+#pragma unroll
+                            for (int n = 0; n < 2; n++) s += C[aa][m] * C[bb][n] * M[3 * m + i][3 * n + j];
+                        if (3 * aa + i <= 3 * bb + j) A[PK_<6>(3 * aa + i, 3 * bb + j)] = s;
+                    }
+    }
+    if (a.kp.psd) {
+        double V[36];
+        jacobi_packed<6>(A, V);
+        write_projection<6>(A, V, R);
+    } else {
+#pragma unroll
+        for (int p = 0; p < 6; p++)
+#pragma unroll
+            for (int q = 0; q < 6; q++) R[R_PK + 6 * p + q] = A[PK_<6>(p, q)];
+    }
+}
+
+// record fields other than PK of 3-point contacts, one thread each
+__global__ __launch_bounds__(256) void k_contact_scalars2(WorldArgs a) {
+    const int total = a.n_pt + a.n_ee;
+    const int cnt = a.counts[1];
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < cnt; p += gridDim.x * blockDim.x)
+        scalars_c2(a, a.list[total - 1 - p]);
+}
+__global__ __launch_bounds__(64) void k_contact_pk2(WorldArgs a) {
+    const int total = a.n_pt + a.n_ee;
+    const int cnt = a.counts[1];
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < cnt; p += gridDim.x * blockDim.x)
+        pk_c2(a, a.list[total - 1 - p]);
+}
+
 // grid-strided over the class list: class 1 at list[0, n1), class 2 at
 // list[total - n2, total) (stored from the back)
 template <int CLS>
@@ -1148,7 +1348,9 @@ void contact_terms(World& w) {
             const int grid = std::min((n + 255) / 256, 8 * w.num_cu);
             hipLaunchKernelGGL(k_contact_small<1>, dim3(grid), dim3(256), 0, w.stream, a);
             ADB_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_contact_small<2>, dim3(grid), dim3(256), 0, w.stream, a);
+            hipLaunchKernelGGL(k_contact_scalars2, dim3(grid), dim3(256), 0, w.stream, a);
+            ADB_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_contact_pk2, dim3(std::min((n + 63) / 64, 16 * w.num_cu)), dim3(64), 0, w.stream, a);
             ADB_HIP(hipGetLastError());
         }
     }

@@ -138,6 +138,8 @@ struct World {
     DBuf<int> grp_n;                 // number of groups (device copy)
     DBuf<double> Ginv;
     int n_groups = 0;
+    DBuf<int> glist;                 // group ids in launch order: multi-body groups first
+    std::vector<int> h_glist;
     std::vector<int> h_rp, h_ci, h_grp_off, h_grp_body;
     std::vector<long long> h_grp_moff;
     int nnzb = 0, n_contrib = 0;

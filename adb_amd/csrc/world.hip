@@ -256,7 +256,7 @@ static void free_world(World& w) {
     DBuf<int>* ib[] = {&w.voff, &w.foff, &w.eoff, &w.vbody, &w.fbody, &w.ebody, &w.face, &w.edge, &w.counters,
                        &w.aval, &w.aval_s, &w.blk_of, &w.uniq_flag, &w.blk_start, &w.row_ptr, &w.col_idx, &w.diag_blk, &w.vg_start,
                        &w.ctype, &w.ccls, &w.clist, &w.clist3, &w.ccount, &w.chunk_n, &w.chunk_off, &w.chunk_blk,
-                       &w.grp_off, &w.grp_body};
+                       &w.grp_off, &w.grp_body, &w.glist};
     for (auto* b : ib) b->free_();
     DBuf<double>* db[] = {&w.xbar, &w.mass, &w.Ic, &w.q, &w.q0, &w.qdot, &w.dq, &w.qtrial, &w.X, &w.Xs, &w.X1, &w.bbox, &w.bsph,
                           &w.crec, &w.cdist, &w.chunk_part, &w.ctk, &w.clam, &w.vgK, &w.vglam, &w.bodyH, &w.bodyG, &w.Hval, &w.grad,
