@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libadb.so")
-SOURCES = ["contact.hip", "cull.hip", "assemble.hip", "pcg.hip", "energy.hip", "ccd.hip", "world.hip"]
+SOURCES = ["contact.hip", "cull.hip", "assemble.hip", "pcg.hip", "energy.hip", "ccd.hip", "hash.hip", "world.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",

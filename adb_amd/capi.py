@@ -53,6 +53,7 @@ class AdbParams(C.Structure):
         ("psd", C.c_int), ("line_search", C.c_int), ("upper_bound", C.c_int),
         ("broad_phase_fp32", C.c_int),
         ("pcg_rtol", C.c_double), ("pcg_max_iter", C.c_int),
+        ("iaabb", C.c_int), ("sh_bits", C.c_int), ("sh_min_xyz", C.c_double), ("sh_dx", C.c_double),
     ]
 
 
@@ -102,6 +103,10 @@ def default_params(**overrides):
     p.broad_phase_fp32 = 0
     p.pcg_rtol = 1e-12
     p.pcg_max_iter = 2000
+    p.iaabb = 3
+    p.sh_bits = 6
+    p.sh_min_xyz = -10.0
+    p.sh_dx = 0.8
     for k, v in overrides.items():
         if k == "gravity":
             for i in range(3):
@@ -220,4 +225,4 @@ EXPORTED_SYMBOLS = [
 # kernel ids of adb_profile_read (enum adb_prof_kernel)
 PROF_KERNELS = ["transform", "body_box", "body_pairs", "narrow_set", "vg_scan", "sort", "narrow_ccd", "vg_toi",
                 "contact", "vg_contact", "inertia", "asm_reduce", "block_chol", "spmv", "pcg_update", "pcg_dir",
-                "energy", "contact_classify", "contact_group"]
+                "energy", "contact_classify", "contact_group", "hash_build", "hash_query"]

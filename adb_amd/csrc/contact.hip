@@ -6,18 +6,30 @@
 // mapping (model/pt_ipc.cpp:153-169, model/ee_ipc.cpp:175-192) and the VG term
 // (model/vg_ipc.cpp:12-30, model/abd.cpp:460-481).
 //
-// Layout: one 16-lane group per PT/EE contact, four contacts per 64-lane
-// wavefront (one wavefront per workgroup).  Lane j < 12 owns column j of the
-// contact's 12x12 vertex-space Hessian.
+// World path (contact_terms): k_contact_classify computes each contact's
+// distance type, squared distance and class, the dimension D = 3 n of the
+// translation-free space of the n + 1 points it involves, and appends it to
+// that class's list; then one thread per contact:
+//   class 1 (two points, D = 3): closed-form projection (k_contact_small<1>);
+//   class 2 (three points, D = 6): record fields (k_contact_scalars2), then
+//     cyclic Jacobi on the packed 6 x 6 matrix in registers (k_contact_pk2);
+//   class 3 (four points, or any mollified EE, D = 9): 9 x 9 matrix staged
+//     through LDS, cyclic Jacobi in registers (k_contact_c3, on a side stream).
+// Each contact writes a compact body-space record (world.h) that the
+// assembly expands into its four 12 x 12 blocks.
 //
-// PSD projection: every barrier and mollifier term depends on the four points
-// only through differences, so the Hessian annihilates rigid translations.
-// With Q = h (x) I3, h the 3x4 Helmert basis of the complement of (1,1,1,1),
-// H = Q^T (Q H Q^T) Q and the projection onto the PSD cone is
-// Q^T P(Q H Q^T) Q: a 9x9 symmetric eigenproblem, solved by parallel cyclic
-// Jacobi (5 disjoint rotations per round, 9 rounds per sweep over 10 slots,
-// slot 9 idle) on the matrix staged in LDS.  As in the reference, a Hessian
-// whose eigenvalues are all >= 0 is returned unchanged.
+// PSD projection: every barrier and mollifier term depends on the points only
+// through differences, so the Hessian annihilates rigid translations.  With
+// Q = h (x) I3, h the Helmert basis of the complement of (1, ..., 1) over the
+// contact's points, H = Q^T (Q H Q^T) Q and the projection onto the PSD cone
+// is Q^T P(Q H Q^T) Q: a D x D symmetric eigenproblem.  As in the reference,
+// a Hessian whose eigenvalues are all >= 0 is returned unchanged.
+//
+// Kernel-level API (k_hess_api, the abdtk functions): one 16-lane group per
+// contact, four contacts per wavefront, lane j < 12 owning column j of the
+// 12 x 12 Hessian; its 9 x 9 projection is parallel cyclic Jacobi (5 disjoint
+// rotations per round, 9 rounds per sweep over 10 slots, slot 9 idle) on the
+// matrix staged in LDS.
 #include <cstdio>
 #include "dmath.h"
 #include "world.h"
@@ -629,66 +641,15 @@ __device__ __forceinline__ void load_w(const double* xbar, int id0, int id1, int
     for (int k = 0; k < 4; k++) { w[k][0] = 1.0; w[k][1] = xs[k].x; w[k][2] = xs[k].y; w[k][3] = xs[k].z; }
 }
 
-// Cyclic Jacobi on a register-resident symmetric N x N matrix (all indices
-// static), eigenvectors accumulated in v.
-template <int N>
-__device__ __forceinline__ void jacobi_reg(double (&A)[N][N], double (&v)[N][N]) {
-#pragma unroll
-    for (int p = 0; p < N; p++)
-#pragma unroll
-        for (int q = 0; q < N; q++) v[p][q] = p == q ? 1.0 : 0.0;
-    for (int sweep = 0; sweep < MAX_SWEEPS; sweep++) {
-        double off = 0.0, tot = 0.0;
-#pragma unroll
-        for (int p = 0; p < N; p++)
-#pragma unroll
-            for (int q = 0; q < N; q++) { const double x = A[p][q] * A[p][q]; tot += x; if (p != q) off += x; }
-        if (off <= JACOBI_TOL * tot || off == 0.0) break;
-        // entries this small cannot hold the sweep above the stop test: skip them
-        const double thr = JACOBI_SKIP * sqrt(tot);
-#pragma unroll
-        for (int p = 0; p < N - 1; p++)
-#pragma unroll
-            for (int q = p + 1; q < N; q++) {
-                const double apq = A[p][q];
-                const bool nz = fabs(apq) > thr;
-                const double tau = (A[q][q] - A[p][p]) / (2.0 * (nz ? apq : 1.0));
-                double t = fabs(tau) > 1e150 ? 0.5 / tau : (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                t = nz ? t : 0.0;
-                const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
-#pragma unroll
-                for (int k = 0; k < N; k++) {
-                    const double ap = A[k][p], aq = A[k][q];
-                    A[k][p] = c * ap - s * aq; A[k][q] = s * ap + c * aq;
-                }
-#pragma unroll
-                for (int k = 0; k < N; k++) {
-                    const double ap = A[p][k], aq = A[q][k];
-                    A[p][k] = c * ap - s * aq; A[q][k] = s * ap + c * aq;
-                }
-#pragma unroll
-                for (int k = 0; k < N; k++) {
-                    const double vp = v[k][p], vq = v[k][q];
-                    v[k][p] = c * vp - s * vq; v[k][q] = s * vp + c * vq;
-                }
-                __builtin_amdgcn_sched_barrier(0);
-            }
-    }
-}
-
 // Point slot k of the contact as a selector over the four points.
 __device__ __forceinline__ d3 pick(d3 p0, d3 p1, d3 p2, d3 p3, int k) {
     return k == 0 ? p0 : (k == 1 ? p1 : (k == 2 ? p2 : p3));
 }
 
-// Class 1 and 2 contacts, one thread each.
-//   class 1: d = |r|^2, r = x_a - x_b.  In the Helmert coordinate
-//     y = (x_a - x_b) / sqrt(2) the Hessian is K = 4 B1 I + 8 B2 r r^T with
-//     eigenvalues 4 B1 (twice, normal to r) and 4 B1 + 8 B2 d (along r).
-//   class 2: d = point-line distance of r0 = x_a - x_m, r1 = x_b - x_m;
-//     Helmert rows over the slots (m, a, b) give r = C^T y with
-//     C = [[-sqrt 2, -1 / sqrt 2], [0, -sqrt(3 / 2)]] and K = (C (x) I3) Mred
-//     (C^T (x) I3), Mred = B1 Hess(d) + B2 grad(d) grad(d)^T; 6x6 Jacobi.
+// Class 1 contacts, one thread each: d = |r|^2, r = x_a - x_b.  In the
+// Helmert coordinate y = (x_a - x_b) / sqrt(2) the Hessian is
+// K = 4 B1 I + 8 B2 r r^T with eigenvalues 4 B1 (twice, normal to r) and
+// 4 B1 + 8 B2 d (along r).
 template <int CLS>
 __device__ __forceinline__ void contact_small(const WorldArgs& a, int cid) {
     int id0, id1, id2, id3;
@@ -721,8 +682,9 @@ __device__ __forceinline__ void contact_small(const WorldArgs& a, int cid) {
     }
     d3 gv[4];
     gv[0] = z; gv[1] = z; gv[2] = z; gv[3] = z;
+    static_assert(CLS == 1, "3-point contacts: scalars_c2 / pk_c2");
     double hz[4][2];      // Helmert weight of point slot k in row a (zero off the contact's slots)
-    if (CLS == 1) {
+    {
         const int ka = mp.plus[0], kb = mp.minus[0];
         const d3 r = pick(P0, P1, P2, P3, ka) - pick(P0, P1, P2, P3, kb);
         const double lp = 4.0 * B1, ll = 4.0 * B1 + 8.0 * B2 * d;
@@ -747,74 +709,6 @@ __device__ __forceinline__ void contact_small(const WorldArgs& a, int cid) {
         const d3 g = mk(2 * r.x * B1, 2 * r.y * B1, 2 * r.z * B1);
 #pragma unroll
         for (int k = 0; k < 4; k++) gv[k] = k == ka ? g : (k == kb ? mk(-g.x, -g.y, -g.z) : z);
-    } else {
-        const int ka = mp.plus[0], kb = mp.plus[1], km = mp.minus[0];
-        const d3 pm = pick(P0, P1, P2, P3, km);
-        const d3 r0 = pick(P0, P1, P2, P3, ka) - pm, r1 = pick(P0, P1, P2, P3, kb) - pm;
-        RedDist rd;
-        rd.init(K_PL, r0, r1, z);
-        double K[6][6];
-        {
-            double M[6][6];
-#pragma unroll
-            for (int u = 0; u < 6; u++)
-#pragma unroll
-                for (int v = u; v < 6; v++) {
-                    const int m = u / 3, i = u % 3, n = v / 3, j = v % 3;
-                    M[u][v] = B1 * rd.hess(m, i, n, j) + B2 * cmp(rd.g[m], i) * cmp(rd.g[n], j);
-                    M[v][u] = M[u][v];
-                }
-            const double C[2][2] = {{-2.0 * HS2, -HS2}, {0.0, -3.0 * HS6}};
-#pragma unroll
-            for (int aa = 0; aa < 2; aa++)
-#pragma unroll
-                for (int i = 0; i < 3; i++)
-#pragma unroll
-                    for (int bb = 0; bb < 2; bb++)
-#pragma unroll
-                        for (int j = 0; j < 3; j++) {
-                            double s = 0.0;
-#pragma unroll
-                            for (int m = 0; m < 2; m++)
-#pragma unroll
-                                for (int n = 0; n < 2; n++) s += C[aa][m] * C[bb][n] * M[3 * m + i][3 * n + j];
-                            K[3 * aa + i][3 * bb + j] = s;
-                        }
-        }
-        const d3 g0 = mk(B1 * rd.g[0].x, B1 * rd.g[0].y, B1 * rd.g[0].z);
-        const d3 g1 = mk(B1 * rd.g[1].x, B1 * rd.g[1].y, B1 * rd.g[1].z);
-        if (a.kp.psd) {
-            double V[6][6];
-            jacobi_reg<6>(K, V);
-            double wv[6];
-#pragma unroll
-            for (int k = 0; k < 6; k++) wv[k] = fmax(K[k][k], 0.0);
-#pragma unroll
-            for (int p = 0; p < 6; p++)
-#pragma unroll
-                for (int q = p; q < 6; q++) {
-                    double s = 0.0;
-#pragma unroll
-                    for (int k = 0; k < 6; k++) s += V[p][k] * wv[k] * V[q][k];
-                    R[R_PK + 6 * p + q] = s;
-                    R[R_PK + 6 * q + p] = s;
-                }
-        } else {
-#pragma unroll
-            for (int p = 0; p < 6; p++)
-#pragma unroll
-                for (int q = 0; q < 6; q++) R[R_PK + 6 * p + q] = K[p][q];
-        }
-        R[R_D] = rec_code(a, cid, 6);
-        // Helmert rows over the slots (m, a, b): (1, -1, 0) / sqrt 2, (1, 1, -2) / sqrt 6
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            hz[k][0] = k == km ? HS2 : (k == ka ? -HS2 : 0.0);
-            hz[k][1] = k == km ? HS6 : (k == ka ? HS6 : (k == kb ? -2.0 * HS6 : 0.0));
-        }
-        const d3 gm = mk(-(g0.x + g1.x), -(g0.y + g1.y), -(g0.z + g1.z));
-#pragma unroll
-        for (int k = 0; k < 4; k++) gv[k] = k == ka ? g0 : (k == kb ? g1 : (k == km ? gm : z));
     }
     double w[4][4];
     load_w(a.xbar, id0, id1, id2, id3, w);
@@ -838,10 +732,10 @@ __device__ __forceinline__ void contact_small(const WorldArgs& a, int cid) {
 }
 // Class 3 contacts, one thread each: the 9x9 Helmert-coordinate matrix
 // K = (C (x) I3) Mred (C^T (x) I3) + mollifier terms (C = h4 S^T, Cm the same
-// for y = p1 - p0, z = p3 - p2) is built one component pair at a time; cyclic
-// Jacobi then rotates its packed upper triangle while the eigenvectors
-// accumulate, both held in LDS element-major (X[e][lane]) so lanes never
-// conflict.
+// for y = p1 - p0, z = p3 - p2) is built one component pair at a time into
+// LDS (element-major, X[e][lane], so lanes never conflict); cyclic Jacobi then
+// rotates its packed upper triangle in registers while the eigenvectors
+// accumulate there too.
 __host__ __device__ constexpr int P9(int p, int q) {
     return p <= q ? p * 9 - p * (p - 1) / 2 + (q - p) : q * 9 - q * (q - 1) / 2 + (p - q);
 }

@@ -23,33 +23,12 @@
 #include <cstdio>
 #include "dmath.h"
 #include "world.h"
+#include "cull.h"
 
 namespace adb {
 
 constexpr int NB_THREADS = 256;
 
-// Monotone map of doubles to unsigned 64-bit keys (negative values included),
-// so an integer atomicMin is an exact double min.
-__device__ __forceinline__ unsigned long long order_key(double t) {
-    unsigned long long b = (unsigned long long)__double_as_longlong(t);
-    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
-static double order_key_inv(unsigned long long k) {
-    unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
-    double t;
-    memcpy(&t, &b, 8);
-    return t;
-}
-static unsigned long long order_key_host(double t) {
-    unsigned long long b;
-    memcpy(&b, &t, 8);
-    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
-
-struct Box { double lo[3], hi[3]; };
-__device__ __forceinline__ bool ovl(const double* alo, const double* ahi, const double* blo, const double* bhi) {
-    return alo[0] <= bhi[0] && ahi[0] >= blo[0] && alo[1] <= bhi[1] && ahi[1] >= blo[1] && alo[2] <= bhi[2] && ahi[2] >= blo[2];
-}
 
 // ------------------------------------------------------------------ transforms
 __global__ void k_transform(int V, const int* __restrict__ vbody, const double* __restrict__ xbar,
@@ -162,21 +141,6 @@ __global__ void k_body_pairs(int B, const double* __restrict__ box, const double
     if (hit) {
         int off = __popcll(m & ((1ull << lane) - 1));
         if (base + off < cap) pairs[base + off] = make_int2(I, J);
-    }
-}
-
-// ------------------------------------------------------------------ append helper
-__device__ __forceinline__ void append_key(bool hit, unsigned long long key, unsigned long long* out, int cap, int* counter) {
-    unsigned long long m = __ballot(hit);
-    if (m == 0) return;
-    const int lane = __lane_id();
-    const int leader = __ffsll((long long)m) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(counter, __popcll(m));
-    base = __shfl(base, leader);
-    if (hit) {
-        int off = __popcll(m & ((1ull << lane) - 1));
-        if (base + off < cap) out[base + off] = key;
     }
 }
 
@@ -506,7 +470,8 @@ static NarrowArgs narrow_args(World& w, int npairs, const double* X0, const doub
 
 void collision_set(World& w, const double* X, ContactSet& out) {
     const double r = sqrt(w.P.d_hat) * (1.0 + 1e-6) + 1e-12;
-    int npairs = body_pairs(w, X, nullptr, r);
+    const bool hashed = !(w.P.iaabb & 1);    // spatial hashing (hash.hip) instead of body overlaps
+    int npairs = hashed ? 0 : body_pairs(w, X, nullptr, r);
     w.counters.reserve(16);
     for (int attempt = 0; attempt < 4; attempt++) {
         size_t pcap = std::max<size_t>(w.keys_a.cap / 2, 1 << 16);
@@ -517,7 +482,9 @@ void collision_set(World& w, const double* X, ContactSet& out) {
         NarrowArgs a = narrow_args(w, npairs, X, nullptr, r);
         a.pt_keys = w.keys_a.p; a.pt_cap = (int)pcap;
         a.ee_keys = w.keys_a.p + pcap; a.ee_cap = (int)pcap;
-        if (npairs > 0) {
+        if (hashed) {
+            hash_sets(w, X, r, a.pt_keys, a.pt_cap, a.ee_keys, a.ee_cap, w.counters.p);
+        } else if (npairs > 0) {
             ProfScope ps(w, PK_NARROW_SET, 0.0);
             const dim3 grid(npairs, NARROW_TASKS);
             if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<0, float>), grid, dim3(NB_THREADS), 0, w.stream, a);
@@ -581,12 +548,15 @@ double ccd_toi(World& w, const double* X0, const double* X1, int* status) {
         hipLaunchKernelGGL(k_vg_toi, dim3((w.V + 255) / 256), dim3(256), 0, w.stream, w.V, X0, X1, w.P.ground_y, w.toi_bits.p);
         ADB_HIP(hipGetLastError());
     }
-    int npairs = body_pairs(w, X0, X1, 0.0);
+    const bool hashed = !(w.P.iaabb & 2);    // spatial hashing (hash.hip) instead of body overlaps
+    int npairs = hashed ? 0 : body_pairs(w, X0, X1, 0.0);
     w.counters.reserve(16);
     ADB_HIP(hipMemsetAsync(w.counters.p, 0, 4 * sizeof(int), w.stream));
     NarrowArgs a = narrow_args(w, npairs, X0, X1, 0.0);
     a.toi_bits = w.toi_bits.p;
-    if (npairs > 0) {
+    if (hashed) {
+        hash_ccd(w, X0, X1, w.toi_bits.p);
+    } else if (npairs > 0) {
         ProfScope ps(w, PK_NARROW_CCD, 0.0);
         const dim3 grid(npairs, NARROW_TASKS);
         if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<1, float>), grid, dim3(NB_THREADS), 0, w.stream, a);

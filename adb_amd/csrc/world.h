@@ -57,7 +57,7 @@ constexpr int R_D = 0, R_F = 1, R_G = 10, R_Z = 18, R_GRAD = 42, R_PK = 66;
 enum ProfKernel {
     PK_TRANSFORM = 0, PK_BODY_BOX, PK_BODY_PAIRS, PK_NARROW_SET, PK_VG_SCAN, PK_SORT, PK_NARROW_CCD, PK_VG_TOI,
     PK_CONTACT, PK_VG_CONTACT, PK_INERTIA, PK_ASM_REDUCE, PK_BLOCK_CHOL, PK_SPMV, PK_PCG_UPDATE, PK_PCG_DIR,
-    PK_ENERGY, PK_CONTACT_CLASSIFY, PK_CONTACT_GROUP, PK_COUNT
+    PK_ENERGY, PK_CONTACT_CLASSIFY, PK_CONTACT_GROUP, PK_HASH_BUILD, PK_HASH_QUERY, PK_COUNT
 };
 struct Prof {
     bool on = false;
@@ -106,6 +106,11 @@ struct World {
     DBuf<unsigned long long> toi_bits;
 
     DBuf<unsigned long long> vg_keys;
+    // spatial-hash table (hash.hip): slot counts / starts (2^(3 bits) + 1),
+    // per registered primitive its cell range, the slot entries, entry total
+    DBuf<int> sh_cnt, sh_start, sh_ent;
+    DBuf<int4> sh_rng;
+    DBuf<unsigned long long> sh_total;
     ContactSet cs, cs_new;
     // ---- contact terms (per PT/EE contact, then per VG contact)
     DBuf<double> crec;               // REC per PT/EE contact: compact body-space record
@@ -192,7 +197,10 @@ const int* prof_snapshot(World& w, const int* dev_counts);
 void transform_positions(World& w, const double* q, double* X);                         // cull.hip
 void q_axpy(World& w, const double* q, const double* dq, double alpha, double* out);     // cull.hip
 void collision_set(World& w, const double* X, ContactSet& out);                          // cull.hip
-double ccd_toi(World& w, const double* X0, const double* X1, int* status);              // ccd.hip
+double ccd_toi(World& w, const double* X0, const double* X1, int* status);              // cull.hip
+void hash_sets(World& w, const double* X, double r, unsigned long long* pt_keys, int pt_cap,
+               unsigned long long* ee_keys, int ee_cap, int* counters);                  // hash.hip
+void hash_ccd(World& w, const double* X0, const double* X1, unsigned long long* toi_bits); // hash.hip
 void contact_terms(World& w);                                                            // contact.hip
 void assemble_system(World& w);                                                          // assemble.hip
 int pcg_solve(World& w, double* dq_out, int* iters);                                     // pcg.hip

@@ -168,6 +168,7 @@ extern "C" void adb_params_default(adb_params* p) {
     p->pt_fric = p->ee_fric = p->vg_fric = 1;
     p->psd = 1; p->line_search = 1; p->upper_bound = 1; p->broad_phase_fp32 = 0;
     p->pcg_rtol = 1e-12; p->pcg_max_iter = 2000;
+    p->iaabb = 3; p->sh_bits = 6; p->sh_min_xyz = -10.0; p->sh_dx = 0.8;
 }
 
 extern "C" int adb_device_count(void) {
@@ -189,6 +190,9 @@ extern "C" int adb_world_create(const adb_scene* s, const adb_params* p, adb_wor
     World& w = W->w;
     try {
         w.P = *p;
+        if ((w.P.iaabb & 3) != 3 && !(w.P.sh_bits >= 1 && w.P.sh_bits <= 8 && w.P.sh_dx > 0.0 && std::isfinite(w.P.sh_dx) &&
+                                       std::isfinite(w.P.sh_min_xyz)))
+            throw AdbError(ADB_E_ARG, "spatial hashing needs 1 <= sh_bits <= 8 and a finite sh_dx > 0");
         w.B = s->n_bodies;
         if (const char* g = std::getenv("ADB_PCG_GRAPH")) w.use_graphs = std::strcmp(g, "0") != 0;
         w.h_voff.assign(s->vert_offset, s->vert_offset + w.B + 1);
@@ -263,6 +267,7 @@ static void free_world(World& w) {
                           &w.Ginv, &w.pcg_x, &w.pcg_r, &w.pcg_z, &w.pcg_p, &w.pcg_p2, &w.pcg_Ap, &w.partial, &w.scal, &w.epart};
     for (auto* b : db) b->free_();
     w.pairs.free_(); w.keys_a.free_(); w.keys_b.free_(); w.sort_tmp.free_(); w.toi_bits.free_(); w.vg_keys.free_();
+    w.sh_cnt.free_(); w.sh_start.free_(); w.sh_ent.free_(); w.sh_rng.free_(); w.sh_total.free_();
     w.akeys.free_(); w.akeys_s.free_();
     for (ContactSet* c : {&w.cs, &w.cs_new}) { c->pt.free_(); c->ee.free_(); c->vg.free_(); }
     for (auto& r : w.prof.pending) { w.prof.pool.push_back(r.a); w.prof.pool.push_back(r.b); }

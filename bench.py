@@ -95,6 +95,57 @@ def max_vertex_deviation(sc, qa, qb):
     return float(np.sqrt((dx * dx).sum(1)).max())
 
 
+KERNEL_OF_SCOPE = {"asm_reduce": "k_reduce_chunks", "narrow_set": "k_pair_narrow<0", "block_chol": "k_group_inv",
+                   "contact_group": "k_contact_c3", "contact": "k_contact_small<1>", "spmv": "k_pcg_spmv",
+                   "pcg_update": "k_pcg_update", "contact_classify": "k_contact_classify"}
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hbm_traffic_2000_icospheres.json")
+
+
+def pmc_traffic(fetch_csv, write_csv, prof_json, scope, copy_mb):
+    """HBM bytes per launch of one profiled kernel from two rocprofv3 --pmc runs
+    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch) of
+    `bench.py --steps 10 --warmup 2 --no-cpu-baseline --no-compare --copy-mb M
+    --profile-json P`.  Each counter is scaled by what it reads for the plain
+    copy of M MB that the run starts with (the largest copyBuffer dispatch);
+    the dispatches used are the kernel's last N, N = the scope's launches in
+    the profiled steps (from P, whose byte counts give the algorithmic bytes)."""
+    import csv
+    with open(prof_json) as fh:
+        prof = json.load(fh)
+    n = prof[scope]["launches"]
+    kname = KERNEL_OF_SCOPE[scope]
+    out = {"kernel_scope": scope, "kernel": kname, "launches": n, "copy_bytes": copy_mb * 1e6,
+           "algorithmic_bytes_per_launch": prof[scope]["bytes"] / n}
+    for tag, path in (("fetch", fetch_csv), ("write", write_csv)):
+        rows = list(csv.DictReader(open(path)))
+        copies = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]]
+        scale = copy_mb * 1e6 / (1024.0 * max(copies))
+        mine = [float(r["Counter_Value"]) for r in rows if kname in r["Kernel_Name"]][-n:]
+        out[tag + "_kib_raw_avg"] = sum(mine) / len(mine)
+        out[tag + "_scale"] = scale
+        out[tag + "_bytes"] = 1024.0 * scale * sum(mine) / len(mine)
+    out["traffic_bytes_per_launch"] = out["fetch_bytes"] + out["write_bytes"]
+    return out
+
+
+def kernel_stats_md(stats_csv, title, cmd, top=25):
+    """Markdown table of a rocprofv3 --stats kernel summary (CSV)."""
+    import csv
+    rows = list(csv.DictReader(open(stats_csv)))
+    tot = sum(float(r["TotalDurationNs"]) for r in rows)
+    lines = ["# " + title, "", "Command (from the repository root, on one MI355X):", "", "    " + cmd, "",
+             "Total kernel time: %.2f ms over all dispatches of the run (warm-up included)." % (tot / 1e6), "",
+             "| kernel | calls | total ms | avg us | min us | max us | % |", "|---|---|---|---|---|---|---|"]
+    for r in rows[:top]:
+        name = r["Name"].replace("|", "/")
+        if len(name) > 90:
+            name = name[:87] + "..."
+        lines.append("| `%s` | %s | %.3f | %.1f | %.1f | %.1f | %.2f |" % (
+            name, r["Calls"], float(r["TotalDurationNs"]) / 1e6, float(r["AverageNs"]) / 1e3,
+            float(r["MinNs"]) / 1e3, float(r["MaxNs"]) / 1e3, float(r["Percentage"])))
+    return "\n".join(lines) + "\n"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,7 +163,27 @@ def main():
     ap.add_argument("--copy-mb", type=float, default=0.0,
                     help="first run one plain device-to-device copy of this many MB (a known-size kernel "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--pmc-traffic", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "PROFILE_JSON", "SCOPE"),
+                    help="no run: summarize two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) into HBM bytes per "
+                         "launch of one kernel scope, scaled by the --copy-mb copy; writes --out")
+    ap.add_argument("--kernel-stats", nargs=3, metavar=("STATS_CSV", "TITLE", "CMD"),
+                    help="no run: write a markdown table of a rocprofv3 --stats kernel summary to --out")
+    ap.add_argument("--out", default="")
     args = ap.parse_args()
+    if args.pmc_traffic:
+        res = pmc_traffic(*args.pmc_traffic, args.copy_mb)
+        if args.out:
+            with open(args.out, "w") as fh:
+                json.dump(res, fh, indent=1)
+        print(json.dumps(res))
+        return
+    if args.kernel_stats:
+        md = kernel_stats_md(*args.kernel_stats)
+        if args.out:
+            with open(args.out, "w") as fh:
+                fh.write(md)
+        print(md)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the contact step of one scene runs on one device "
               "(multi-GPU sharding is a design note, not implemented); run with --gpus 1", file=sys.stderr)
@@ -166,7 +237,14 @@ def main():
         gbs = bytes_per_launch / (avg_ms * 1e-3) / 1e9
         roofline = {"kernel": k, "bound": "hbm", "achieved": round(gbs, 1), "peak": MI355X_HBM_GBS,
                     "unit": "GB/s", "frac": round(gbs / MI355X_HBM_GBS, 4), "traffic": None,
-                    "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch}
+                    "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch, "launches": v["launches"]}
+        # HBM bytes per launch measured by rocprofv3 --pmc on this workload (committed summary)
+        if os.path.exists(TRAFFIC_JSON):
+            with open(TRAFFIC_JSON) as fh:
+                tr = json.load(fh)
+            if tr.get("kernel_scope") == k:
+                roofline["traffic"] = round(tr["traffic_bytes_per_launch"], 0)
+                roofline["traffic_source"] = os.path.relpath(TRAFFIC_JSON, ROOT)
 
     line = {
         "metric": "steps/s at 2k bodies (mixed vs FP64, same traj err); contact-pairs/s",

@@ -89,6 +89,15 @@ typedef struct adb_params {
     int    broad_phase_fp32; /* 1: FP32 conservative AABBs + exact FP64 filter    */
     double pcg_rtol;      /* PCG relative residual target                         */
     int    pcg_max_iter;
+    /* Culling method, config.json:50 "iaabb" (model/abd.cpp:48-51,159-173,
+     * model/line_search.cpp:88-109): bit 0 set = contact sets by iAABB body
+     * overlaps, clear = spatial hashing (model/col_set.cpp:20-339); bit 1 set =
+     * CCD step bound by iAABB, clear = spatial hashing (model/upper_bound.cpp:16-228).
+     * Both methods give the same sets and bounds; default 3.                  */
+    int    iaabb;
+    int    sh_bits;       /* bits per cell coordinate in the hash, config.json:44 */
+    double sh_min_xyz;    /* grid origin, config.json:40                          */
+    double sh_dx;         /* cell size, config.json:42                            */
 } adb_params;
 
 /* Fills p with the reference's defaults (config.json, model/settings.h). */
@@ -202,7 +211,7 @@ enum adb_prof_kernel {
     ADB_PK_TRANSFORM = 0, ADB_PK_BODY_BOX, ADB_PK_BODY_PAIRS, ADB_PK_NARROW_SET, ADB_PK_VG_SCAN, ADB_PK_SORT,
     ADB_PK_NARROW_CCD, ADB_PK_VG_TOI, ADB_PK_CONTACT, ADB_PK_VG_CONTACT, ADB_PK_INERTIA, ADB_PK_ASM_REDUCE,
     ADB_PK_BLOCK_CHOL, ADB_PK_SPMV, ADB_PK_PCG_UPDATE, ADB_PK_PCG_DIR, ADB_PK_ENERGY,
-    ADB_PK_CONTACT_CLASSIFY, ADB_PK_CONTACT_GROUP, ADB_PK_COUNT
+    ADB_PK_CONTACT_CLASSIFY, ADB_PK_CONTACT_GROUP, ADB_PK_HASH_BUILD, ADB_PK_HASH_QUERY, ADB_PK_COUNT
 };
 int adb_profile_enable(adb_world* w, int on);
 int adb_profile_read(adb_world* w, int kernel, double* total_ms, long long* launches, double* bytes);

@@ -18,6 +18,7 @@
 #include "oracle_math.hpp"
 #include <vector>
 #include <map>
+#include <unordered_map>
 #include <set>
 #include <algorithm>
 #include <chrono>
@@ -432,6 +433,180 @@ static std::vector<std::pair<int, int>> body_pairs(const std::vector<Box>& bb) {
     return out;
 }
 
+// ---------------------------------------------------------------------------
+// Spatial hashing, the reference's alternative culling (model/spatial_hashing.cpp:11-211;
+// contact sets model/col_set.cpp:20-339; CCD model/upper_bound.cpp:16-228).
+// Cell coordinates are floor(max(x - min_xyz, 0) / dx) per axis
+// (spatial_hashing.cpp:23-27); the hash concatenates the low `bits` bits of
+// the three coordinates (spatial_hashing.cpp:11-21), so cells 2^bits apart
+// alias.  A registered primitive goes into every cell of its box
+// (register_interval, :43-66); a query gathers the entries of every cell of
+// its box, sorts and de-duplicates them (query_interval, :68-113); its own
+// body is dropped and the exact predicate decides.  Differences from the
+// reference, none of which changes a result:
+//   * cells hold any number of entries (the reference keeps n_buffer per cell
+//     plus 8 overflow slots and loses candidates beyond that);
+//   * a range longer than 2^bits cells stops after 2^bits (the rest alias);
+//   * query boxes of contact sets are inflated by 2 r (r = the culling radius)
+//     and candidates pass the same primitive box test as the iAABB path, so
+//     both methods test exactly the pairs whose boxes overlap;
+//   * an edge's CCD box spans its end points at t0 and t1 (the reference's
+//     register_edge_trajectory reads the t0 end points twice,
+//     spatial_hashing.cpp:150-151).
+// ---------------------------------------------------------------------------
+struct HashGrid {
+    double mn, dx; int bits;
+    std::unordered_map<int, std::vector<int>> cell;
+};
+static HashGrid sh_grid(const World& w) { HashGrid g; g.mn = w.P.sh_min_xyz; g.dx = w.P.sh_dx; g.bits = w.P.sh_bits; return g; }
+static int sh_coord(const HashGrid& g, double x) {
+    double u = std::max(x - g.mn, 0.0) / g.dx;
+    return u < 1073741824.0 ? (int)u : 1073741824;
+}
+template <class Fn> static void sh_cells(const HashGrid& g, const Box& b, Fn fn) {
+    int lo[3], hi[3];
+    for (int k = 0; k < 3; k++) {
+        lo[k] = sh_coord(g, b.lo[k]);
+        hi[k] = std::min(sh_coord(g, b.hi[k]), lo[k] + (1 << g.bits) - 1);
+    }
+    const int m = (1 << g.bits) - 1;
+    for (int i = lo[0]; i <= hi[0]; i++)
+        for (int j = lo[1]; j <= hi[1]; j++)
+            for (int k = lo[2]; k <= hi[2]; k++) fn(((i & m) << (2 * g.bits)) | ((j & m) << g.bits) | (k & m));
+}
+static void sh_register(HashGrid& g, const Box& b, int id) {
+    sh_cells(g, b, [&](int h) { g.cell[h].push_back(id); });
+}
+static void sh_query(const HashGrid& g, const Box& b, std::vector<int>& out) {
+    out.clear();
+    sh_cells(g, b, [&](int h) {
+        auto it = g.cell.find(h);
+        if (it != g.cell.end()) out.insert(out.end(), it->second.begin(), it->second.end());
+    });
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+}
+static int n_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+static int thread_id() {
+#ifdef _OPENMP
+    return omp_get_thread_num();
+#else
+    return 0;
+#endif
+}
+
+// contact sets (model/col_set.cpp:65-152 PT, :188-296 EE)
+static void sh_collision_set(const World& w, const std::vector<double>& X, std::vector<int>& pt, std::vector<int>& ee) {
+    const double r = cull_radius(w), dh = w.P.d_hat;
+    const int nt = n_threads();
+    std::vector<std::vector<int>> tpt(nt), tee(nt);
+    if (w.P.enable_pt && w.F > 0) {
+        HashGrid g = sh_grid(w);
+        for (int v = 0; v < w.V; v++) { Box b = box_empty(); box_add(b, X3(X, v)); sh_register(g, b, v); }
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int f = 0; f < w.F; f++) {
+            std::vector<int> cand;
+            const int J = w.fbody[f];
+            V3 t0 = X3(X, w.face[3 * f]), t1 = X3(X, w.face[3 * f + 1]), t2 = X3(X, w.face[3 * f + 2]);
+            Box fb = box_empty(); box_add(fb, t0); box_add(fb, t1); box_add(fb, t2);
+            sh_query(g, box_inflate(fb, 2 * r), cand);
+            for (int v : cand) {
+                const int I = w.vbody[v];
+                if (I == J) continue;
+                V3 p = X3(X, v);
+                Box vb = box_empty(); box_add(vb, p);
+                if (!overlap(box_inflate(vb, r), fb)) continue;
+                int ty = pt_type(p, t0, t1, t2);
+                if (pt_dist_typed(p, t0, t1, t2, ty) < dh) {
+                    std::vector<int>& o = tpt[thread_id()];
+                    o.push_back(I); o.push_back(v - w.voff[I]); o.push_back(J); o.push_back(f - w.foff[J]);
+                }
+            }
+        }
+    }
+    if (w.P.enable_ee && w.E > 0) {
+        HashGrid g = sh_grid(w);
+        auto ebox = [&](int e) { Box b = box_empty(); box_add(b, X3(X, w.edge[2 * e])); box_add(b, X3(X, w.edge[2 * e + 1])); return b; };
+        for (int e = 0; e < w.E; e++) sh_register(g, ebox(e), e);
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int b = 0; b < w.E; b++) {
+            std::vector<int> cand;
+            const int J = w.ebody[b];
+            const Box bb = ebox(b);
+            sh_query(g, box_inflate(bb, 2 * r), cand);
+            V3 b0 = X3(X, w.edge[2 * b]), b1 = X3(X, w.edge[2 * b + 1]);
+            for (int a : cand) {
+                const int I = w.ebody[a];
+                if (I >= J) continue;                       // each pair once, I < J (col_set.cpp:282)
+                if (!overlap(box_inflate(ebox(a), r), bb)) continue;
+                V3 a0 = X3(X, w.edge[2 * a]), a1 = X3(X, w.edge[2 * a + 1]);
+                int ty = ee_type(a0, a1, b0, b1);
+                if (ee_dist_typed(a0, a1, b0, b1, ty) < dh) {
+                    std::vector<int>& o = tee[thread_id()];
+                    o.push_back(I); o.push_back(a - w.eoff[I]); o.push_back(J); o.push_back(b - w.eoff[J]);
+                }
+            }
+        }
+    }
+    for (int t = 0; t < nt; t++) { pt.insert(pt.end(), tpt[t].begin(), tpt[t].end()); ee.insert(ee.end(), tee[t].begin(), tee[t].end()); }
+}
+
+// CCD step bound of the PT / EE pairs (model/upper_bound.cpp:85-132 PT, :175-224 EE)
+static double sh_ccd_pairs(const World& w, const std::vector<double>& X0, const std::vector<double>& X1) {
+    auto tbox = [&](const int* vs, int n) {
+        Box b = box_empty();
+        for (int c = 0; c < n; c++) { box_add(b, X3(X0, vs[c])); box_add(b, X3(X1, vs[c])); }
+        return b;
+    };
+    double toi = 1.0;
+    if (w.P.enable_pt && w.F > 0) {
+        HashGrid g = sh_grid(w);
+        for (int v = 0; v < w.V; v++) sh_register(g, tbox(&v, 1), v);
+#pragma omp parallel for schedule(dynamic, 64) reduction(min : toi)
+        for (int f = 0; f < w.F; f++) {
+            std::vector<int> cand;
+            const int J = w.fbody[f];
+            const int* fv = &w.face[3 * f];
+            const Box fb = tbox(fv, 3);
+            sh_query(g, fb, cand);
+            for (int v : cand) {
+                if (w.vbody[v] == J) continue;
+                if (!overlap(tbox(&v, 1), fb)) continue;
+                V3 P0[4] = {X3(X0, v), X3(X0, fv[0]), X3(X0, fv[1]), X3(X0, fv[2])};
+                V3 P1[4] = {X3(X1, v), X3(X1, fv[0]), X3(X1, fv[1]), X3(X1, fv[2])};
+                toi = std::min(toi, pt_toi(P0, P1));
+            }
+        }
+    }
+    if (w.P.enable_ee && w.E > 0) {
+        HashGrid g = sh_grid(w);
+        for (int e = 0; e < w.E; e++) sh_register(g, tbox(&w.edge[2 * e], 2), e);
+#pragma omp parallel for schedule(dynamic, 64) reduction(min : toi)
+        for (int b = 0; b < w.E; b++) {
+            std::vector<int> cand;
+            const int J = w.ebody[b];
+            const int* vb = &w.edge[2 * b];
+            const Box bb = tbox(vb, 2);
+            sh_query(g, bb, cand);
+            for (int a : cand) {
+                if (w.ebody[a] >= J) continue;
+                const int* va = &w.edge[2 * a];
+                if (!overlap(tbox(va, 2), bb)) continue;
+                V3 P0[4] = {X3(X0, va[0]), X3(X0, va[1]), X3(X0, vb[0]), X3(X0, vb[1])};
+                V3 P1[4] = {X3(X1, va[0]), X3(X1, va[1]), X3(X1, vb[0]), X3(X1, vb[1])};
+                toi = std::min(toi, ee_toi(P0, P1));
+            }
+        }
+    }
+    return toi;
+}
+
 static void collision_set(const World& w, const std::vector<double>& X, int method,
                           std::vector<int>& pt, std::vector<int>& ee, std::vector<int>& vg) {
     pt.clear(); ee.clear(); vg.clear();
@@ -442,6 +617,11 @@ static void collision_set(const World& w, const std::vector<double>& X, int meth
             double d = dd * dd;
             if (d < w.P.d_hat) { vg.push_back(w.vbody[v]); vg.push_back(v - w.voff[w.vbody[v]]); }
         }
+    if (method == 2) {
+        sh_collision_set(w, X, pt, ee);
+        sort4(pt); sort4(ee);
+        return;
+    }
     std::vector<Box> bb = body_boxes(w, X, nullptr, r);
     std::vector<std::pair<int, int>> pairs;
     if (method == 0) {
@@ -514,6 +694,11 @@ static double ccd_toi(World& w, const std::vector<double>& X0, const std::vector
     *status = ADB_OK;
     if (toi_g < 1e-6) { if (++w.stall_vg > 1) *status = ADB_E_TOI_STALL; } else w.stall_vg = 0;
 
+    if (method == 2) {
+        const double toi_p = sh_ccd_pairs(w, X0, X1);
+        if (toi_p < 1e-6) { if (++w.stall_pt > 1) *status = ADB_E_TOI_STALL; } else w.stall_pt = 0;
+        return std::min(toi_g, toi_p);
+    }
     std::vector<Box> bb = body_boxes(w, X0, &X1, 0.0);
     std::vector<std::pair<int, int>> pairs;
     if (method == 0) { for (int I = 0; I < w.B; I++) for (int J = I + 1; J < w.B; J++) pairs.push_back({I, J}); }
@@ -1012,7 +1197,8 @@ static int step(World& w, adb_step_stats* st) {
     std::vector<double> X, Xs, X1, qv;
     positions(w, w.q.data(), X);
     positions(w, w.q0.data(), Xs);
-    collision_set(w, X, 1, w.pt, w.ee, w.vg);
+    const int set_method = (w.P.iaabb & 1) ? 1 : 2, ccd_method = (w.P.iaabb & 2) ? 1 : 2;
+    collision_set(w, X, set_method, w.pt, w.ee, w.vg);
     int status = ADB_OK;
     std::vector<double> dq(n);
     bool converged = false;
@@ -1034,7 +1220,7 @@ static int step(World& w, adb_step_stats* st) {
             q_plus(w, dq.data(), qv);
             positions(w, qv.data(), X1);
             int cs = ADB_OK;
-            toi = ccd_toi(w, X, X1, 1, &cs);
+            toi = ccd_toi(w, X, X1, ccd_method, &cs);
             if (cs != ADB_OK) { status = cs; w.err = "toi stall"; break; }
         }
         if (toi < 1.0) factor = w.P.backoff;
@@ -1057,7 +1243,7 @@ static int step(World& w, adb_step_stats* st) {
                 for (int i = 0; i < n; i++) dqk[i] = dq[i] * alpha;
                 q_plus(w, dqk.data(), qv);
                 positions(w, qv.data(), X1);
-                collision_set(w, X1, 1, npt, nee, nvg);
+                collision_set(w, X1, set_method, npt, nee, nvg);
                 double E1 = E_barrier_inert(w, qv, X1, npt, nee, nvg) + E_fric(w, X1, Xs);
                 bool wolfe = E1 <= E0 + w.P.c1 * alpha * qdg;
                 alpha /= 2;
@@ -1071,7 +1257,7 @@ static int step(World& w, adb_step_stats* st) {
         } else {
             q_plus(w, dq.data(), qv);
             positions(w, qv.data(), X1);
-            collision_set(w, X1, 1, npt, nee, nvg);
+            collision_set(w, X1, set_method, npt, nee, nvg);
         }
         for (int i = 0; i < n; i++) dq[i] *= alpha;
         for (int i = 0; i < n; i++) w.q[i] += dq[i];

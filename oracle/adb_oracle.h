@@ -26,7 +26,8 @@ void oracle_world_destroy(void* w);
 int  oracle_step(void* w, adb_step_stats* st);
 int  oracle_get_state(void* w, double* q, double* qdot);
 int  oracle_set_state(void* w, const double* q, const double* qdot);
-/* method: 0 = brute force over all primitive pairs, 1 = iAABB body-overlap culling */
+/* method: 0 = brute force over all primitive pairs, 1 = iAABB body-overlap culling,
+ * 2 = spatial hashing (model/spatial_hashing.cpp, model/col_set.cpp, model/upper_bound.cpp) */
 int  oracle_collision_set(void* w, int vtn, const double* dq, int method,
                           int* pt, int pt_cap, int* n_pt, int* ee, int ee_cap, int* n_ee,
                           int* vg, int vg_cap, int* n_vg);

@@ -156,3 +156,51 @@ def test_nan_state_reports_error():
     gw.set_state(q, qd)
     st = gw.lib.adb_step(gw.h, None)
     assert st == -3   # ADB_E_NAN, and the process keeps running (the reference calls exit(1))
+
+
+# Spatial-hash culling (iaabb = 0: contact sets and CCD through the hash grid,
+# model/col_set.cpp, model/upper_bound.cpp) against the oracle's iAABB sets
+# and bounds: the reference grid, a fine grid (primitives span many cells)
+# and an aliasing one (2 bits: cells 0.4 apart share a slot).
+HASH_GRIDS = [dict(sh_dx=0.8, sh_bits=6, sh_min_xyz=-10.0),
+              dict(sh_dx=0.05, sh_bits=6, sh_min_xyz=-10.0),
+              dict(sh_dx=0.1, sh_bits=2, sh_min_xyz=-10.0)]
+
+
+@pytest.mark.parametrize("grid", range(len(HASH_GRIDS)))
+@pytest.mark.parametrize("name", list(SET_SCENES))
+def test_hash_sets_and_ccd_bit_exact(name, grid):
+    _require_gpu()
+    mk, kw = SET_SCENES[name]
+    sc = mk()
+    gw = World(sc, default_params(iaabb=0, **HASH_GRIDS[grid], **kw))
+    ow = O.OracleWorld(sc, default_params(**kw))
+    for vtn, dq in [(1, None), (2, np.random.default_rng(3).normal(0, 0.02, size=(len(sc["mass"]), 12)))]:
+        g = gw.collision_set(vtn, dq)
+        o = ow.collision_set(vtn, dq, method=1)
+        for a, b, what in zip(g, o, ("pt", "ee", "vg")):
+            assert keys(a) == keys(b), (name, grid, vtn, what, len(a), len(b))
+    assert len(o[0]) + len(o[1]) > 0
+    rng = np.random.default_rng(11)
+    hits = 0
+    for scale in (0.01, 0.1, 0.5):
+        dq = rng.normal(0, scale, size=(len(sc["mass"]), 12))
+        tg, to = gw.toi(dq), ow.toi(dq, method=1)
+        assert tg == to, (name, grid, scale, tg, to)
+        hits += tg < 1.0
+    assert hits > 0
+
+
+def test_hash_mode_trajectory_equals_iaabb_mode():
+    # the same pairs reach the same predicates, so the steps are bit-identical
+    _require_gpu()
+    mk, kw = SCENES["spheres"]
+    sc = mk()
+    qs = []
+    for mode in (3, 0):
+        gw = World(sc, default_params(iaabb=mode, sh_dx=0.1, sh_bits=5, **kw))
+        for s in range(8):
+            rc, st = gw.step()
+            assert rc == 0, (mode, s, rc)
+        qs.append(gw.state()[0])
+    assert np.array_equal(qs[0], qs[1])
