@@ -128,6 +128,7 @@ struct ReduceArgs {
     const int4* pt; const int4* ee; const double* mass;
     const double* rec; const double* bodyH; const double* bodyG;
     const int2* vg; const int* vg_start; const double* vgK; const int* voff; const double* xbar;
+    const int* row_ptr; const int* col_idx;
     double* Hval; double* grad;
 };
 // Pass 1: one 256-lane workgroup per chunk of at most RCH contributions of
@@ -136,9 +137,16 @@ struct ReduceArgs {
 constexpr int RW = 4;            // waves per reduction workgroup
 constexpr int RS = 5;            // contributions per wave at once (12 lanes each)
 constexpr int RCH = RW * 3 * RS; // contributions per chunk
-__global__ void k_chunk_count(int nnzb, const int* __restrict__ blk_start, int* __restrict__ nch) {
+// Blocks below the diagonal get no chunks: block (J, I) = block (I, J)^T is
+// written by its upper partner in k_reduce_final, so each off-diagonal pair
+// is reduced once.
+__global__ void k_chunk_count(int nnzb, const unsigned long long* __restrict__ keys, const int* __restrict__ blk_start,
+                              int* __restrict__ nch) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < nnzb) nch[k] = (blk_start[k + 1] - blk_start[k] + RCH - 1) / RCH;
+    if (k >= nnzb) return;
+    const unsigned long long key = keys[blk_start[k]];
+    const bool lower = (key >> 32) > (key & 0xffffffffull);
+    nch[k] = lower ? 0 : (blk_start[k + 1] - blk_start[k] + RCH - 1) / RCH;
 }
 // chunk_off = exclusive scan of nch (nnzb + 1 entries)
 // (also stores the chunk count in counts[3] for the byte accounting)
@@ -174,7 +182,11 @@ __device__ __forceinline__ void combine_waves(Acc156& A, double (*sh)[156]) {
 __global__ __launch_bounds__(64 * RW) void k_reduce_chunks(ReduceArgs a, const int* __restrict__ chunk_off,
                                                            const int* __restrict__ chunk_blk, double* __restrict__ part) {
     __shared__ double P[RW][RS][156];
-    const int ch = blockIdx.x;
+    // XCD-aware order: workgroups are dealt to the 8 XCDs round-robin, so XCD
+    // x gets the contiguous chunk range [x per, (x + 1) per) - the blocks of
+    // the same rows, whose contributions share contact records, meet in one L2
+    const int per = gridDim.x / 8;
+    const int ch = (blockIdx.x % 8) * per + blockIdx.x / 8;
     if (ch >= chunk_off[a.nnzb]) return;
     const int k = chunk_blk[ch];
     const int beg = a.blk_start[k] + (ch - chunk_off[k]) * RCH;
@@ -251,6 +263,7 @@ __global__ __launch_bounds__(64 * RW) void k_reduce_final(ReduceArgs a, const in
     if (k >= a.nnzb) return;
     const unsigned long long key = a.keys[a.blk_start[k]];
     const int row = (int)(key >> 32), col = (int)(key & 0xffffffffull);
+    if (row > col) return;   // written transposed by block (col, row)
     const int w = threadIdx.x / 64, t = threadIdx.x % 64;
     Acc156 A = {{0.0, 0.0, 0.0}, 0.0};
     for (int c = chunk_off[k] + w; c < chunk_off[k + 1]; c += RW) {
@@ -280,6 +293,18 @@ __global__ __launch_bounds__(64 * RW) void k_reduce_final(ReduceArgs a, const in
     }
     for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) a.Hval[144 * (size_t)k + e] = acc[u]; }
     if (row == col && t < 12) a.grad[12 * (size_t)row + t] = gacc;
+    if (row < col) {
+        // the mirror block (col, row) in row col of the BSR structure
+        int lo = a.row_ptr[col], hi = a.row_ptr[col + 1];
+        while (lo < hi) { const int mid = (lo + hi) / 2; if (a.col_idx[mid] < row) lo = mid + 1; else hi = mid; }
+        if (lo < a.row_ptr[col + 1] && a.col_idx[lo] == row) {   // always: the structure is symmetric
+            double* M = a.Hval + 144 * (size_t)lo;
+            for (int u = 0; u < 3; u++) {
+                const int e = t + 64 * u;
+                if (e < 144) M[12 * (e % 12) + e / 12] = acc[u];
+            }
+        }
+    }
 }
 
 // Preconditioner: the explicit inverse of the diagonal sub-matrix of each
@@ -491,19 +516,23 @@ void assemble_system(World& w) {
     a.pt = w.cs.pt.p; a.ee = w.cs.ee.p; a.mass = w.mass.p;
     a.rec = w.crec.p; a.bodyH = w.bodyH.p; a.bodyG = w.bodyG.p;
     a.vg = w.cs.vg.p; a.vg_start = w.vg_start.p; a.vgK = w.vgK.p; a.voff = w.voff.p; a.xbar = w.xbar.p;
+    a.row_ptr = w.row_ptr.p; a.col_idx = w.col_idx.p;
     a.Hval = w.Hval.p; a.grad = w.grad.p;
     {
         // chunk layout of the segments
         w.chunk_n.reserve(nnzb + 1); w.chunk_off.reserve(nnzb + 1);
-        hipLaunchKernelGGL(k_chunk_count, dim3((nnzb + 255) / 256), dim3(256), 0, w.stream, nnzb, w.blk_start.p, w.chunk_n.p);
+        hipLaunchKernelGGL(k_chunk_count, dim3((nnzb + 255) / 256), dim3(256), 0, w.stream, nnzb, w.akeys_s.p, w.blk_start.p,
+                           w.chunk_n.p);
         ADB_HIP(hipMemsetAsync(w.chunk_n.p + nnzb, 0, sizeof(int), w.stream));
         size_t tmp = 0;
         ADB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, w.chunk_n.p, w.chunk_off.p, nnzb + 1, w.stream));
         w.sort_tmp.reserve(tmp + 16);
         ADB_HIP(hipcub::DeviceScan::ExclusiveSum(w.sort_tmp.p, tmp, w.chunk_n.p, w.chunk_off.p, nnzb + 1, w.stream));
-        // each block has at most len / RCH + 1 chunks: launch for the bound, the
-        // kernel reads the exact count
-        const int max_chunks = nnzb + n / RCH;
+        // each diagonal or upper block has at most len / RCH + 1 chunks (the
+        // structure is symmetric with one diagonal block per body, so there are
+        // B + (nnzb - B) / 2 of them): launch for the bound rounded up to the 8
+        // XCDs, the kernel reads the exact count
+        const int max_chunks = (B + (nnzb - B) / 2 + n / RCH + 7) / 8 * 8;
         w.chunk_blk.reserve(max_chunks);
         w.chunk_part.reserve(156 * (size_t)max_chunks);
         hipLaunchKernelGGL(k_chunk_owner, dim3((nnzb + 255) / 256), dim3(256), 0, w.stream, nnzb, w.chunk_off.p, w.chunk_blk.p,

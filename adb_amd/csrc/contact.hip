@@ -31,6 +31,7 @@
 // rotations per round, 9 rounds per sweep over 10 slots, slot 9 idle) on the
 // matrix staged in LDS.
 #include <cstdio>
+#include <algorithm>
 #include "dmath.h"
 #include "world.h"
 
@@ -506,60 +507,86 @@ __device__ __forceinline__ bool contact_ids(const WorldArgs& a, int cid, int& id
     return true;
 }
 
-// Wave-aggregated append: one atomic per wavefront.  Every lane of the
-// wavefront must call it.
-__device__ __forceinline__ int wave_append(bool pred, int* counter) {
-    const unsigned long long m = __ballot(pred);
-    if (m == 0ull) return -1;
-    const int lane = __lane_id();
-    const int leader = __ffsll((long long)m) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(counter, __popcll(m));
-    base = __shfl(base, leader);
-    return pred ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
-}
-
 // Class of an active contact by the points its barrier depends on:
 // 1 = two points (D = 3), 2 = three points (D = 6), 3 = four points (D = 9,
 // also every mollified edge-edge contact).  Inactive contacts (d >= d_hat)
 // get an empty record here.
-__global__ __launch_bounds__(256) void k_contact_classify(WorldArgs a) {
-    const int cid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int total = a.n_pt + a.n_ee;
+__device__ __forceinline__ int classify_one(const WorldArgs& a, int cid) {
+    int id0, id1, id2, id3;
+    const bool is_ee = contact_ids(a, cid, id0, id1, id2, id3);
+    const d3 p0 = ld3(a.X + 3 * (size_t)id0), p1 = ld3(a.X + 3 * (size_t)id1);
+    const d3 p2 = ld3(a.X + 3 * (size_t)id2), p3 = ld3(a.X + 3 * (size_t)id3);
+    int type; double d;
+    if (!is_ee) { type = pt_type(p0, p1, p2, p3); d = pt_dist(p0, p1, p2, p3, type); }
+    else { type = ee_type(p0, p1, p2, p3); d = ee_dist(p0, p1, p2, p3, type); }
+    a.ctype[cid] = type; a.cdist[cid] = d;
     int cls = 0;
-    if (cid < total) {
-        int id0, id1, id2, id3;
-        const bool is_ee = contact_ids(a, cid, id0, id1, id2, id3);
-        const d3 p0 = ld3(a.X + 3 * (size_t)id0), p1 = ld3(a.X + 3 * (size_t)id1);
-        const d3 p2 = ld3(a.X + 3 * (size_t)id2), p3 = ld3(a.X + 3 * (size_t)id3);
-        int type; double d;
-        if (!is_ee) { type = pt_type(p0, p1, p2, p3); d = pt_dist(p0, p1, p2, p3, type); }
-        else { type = ee_type(p0, p1, p2, p3); d = ee_dist(p0, p1, p2, p3, type); }
-        a.ctype[cid] = type; a.cdist[cid] = d;
-        if (d < a.kp.d_hat) {
-            const int kind = is_ee ? ee_map(type).kind : pt_map(type).kind;
-            cls = kind == K_PP ? 1 : (kind == K_PL ? 2 : 3);
-            if (is_ee && cls < 3) {
-                // mollifier active: x = |(p1 - p0) x (p3 - p2)|^2 < eps_x |p0 - p1|^2 |p2 - p3|^2
-                const double eps = a.kp.eps_x * sqn(p0 - p1) * sqn(p2 - p3);
-                const d3 c = cross(p1 - p0, p3 - p2);
-                if (dot(c, c) < eps) cls = 3;
+    if (d < a.kp.d_hat) {
+        const int kind = is_ee ? ee_map(type).kind : pt_map(type).kind;
+        cls = kind == K_PP ? 1 : (kind == K_PL ? 2 : 3);
+        if (is_ee && cls < 3) {
+            // mollifier active: x = |(p1 - p0) x (p3 - p2)|^2 < eps_x |p0 - p1|^2 |p2 - p3|^2
+            const double eps = a.kp.eps_x * sqn(p0 - p1) * sqn(p2 - p3);
+            const d3 c = cross(p1 - p0, p3 - p2);
+            if (dot(c, c) < eps) cls = 3;
+        }
+    } else {
+        a.rec[REC * (size_t)cid + R_D] = 0.0;
+        double* tk = a.tk + 24 * (size_t)cid;
+        for (int i = 0; i < 24; i++) tk[i] = 0.0;
+        a.lam[cid] = 0.0;
+    }
+    a.ccls[cid] = cls;
+    return cls;
+}
+
+// Classification and the per-class lists (class 1 from the front of `list`,
+// class 2 from its back, class 3 in `list3`).  Device-scope atomics on one
+// address serialize (~11 ns each on gfx950), so the slots are claimed per
+// workgroup: each workgroup classifies a tile of rpt x CL_THREADS contacts,
+// counts its classes with LDS atomics (one per wavefront and class), takes
+// one global range per class, and then writes the list entries.
+constexpr int CL_THREADS = 1024;
+constexpr int CL_RPT_MAX = 8;
+__global__ __launch_bounds__(CL_THREADS) void k_contact_classify(WorldArgs a, int rpt) {
+    __shared__ int lcnt[3], lbase[3];
+    const int total = a.n_pt + a.n_ee;
+    const int lane = __lane_id();
+    if (threadIdx.x < 3) lcnt[threadIdx.x] = 0;
+    __syncthreads();
+    int cls_r[CL_RPT_MAX], pos_r[CL_RPT_MAX];
+    const int base = blockIdx.x * CL_THREADS * rpt;
+#pragma unroll
+    for (int r = 0; r < CL_RPT_MAX; r++) {
+        cls_r[r] = 0; pos_r[r] = 0;
+        if (r < rpt) {
+            const int cid = base + r * CL_THREADS + threadIdx.x;
+            const int cls = cid < total ? classify_one(a, cid) : 0;
+            cls_r[r] = cls;
+#pragma unroll
+            for (int k = 1; k <= 3; k++) {
+                const unsigned long long m = __ballot(cls == k);
+                if (m == 0ull) continue;
+                const int leader = __ffsll((long long)m) - 1;
+                int b = 0;
+                if (lane == leader) b = atomicAdd(&lcnt[k - 1], __popcll(m));
+                b = __shfl(b, leader);
+                if (cls == k) pos_r[r] = b + __popcll(m & ((1ull << lane) - 1ull));
             }
-        } else {
-            a.rec[REC * (size_t)cid + R_D] = 0.0;
-            double* tk = a.tk + 24 * (size_t)cid;
-            for (int i = 0; i < 24; i++) tk[i] = 0.0;
-            a.lam[cid] = 0.0;
         }
     }
-    if (cid < total) a.ccls[cid] = cls;
-    const int total_n = total;
-    const int p1 = wave_append(cls == 1, a.counts + 0);
-    const int p2 = wave_append(cls == 2, a.counts + 1);
-    const int p3 = wave_append(cls == 3, a.counts + 2);
-    if (cls == 1) a.list[p1] = cid;
-    else if (cls == 2) a.list[total_n - 1 - p2] = cid;
-    else if (cls == 3) a.list3[p3] = cid;
+    __syncthreads();
+    if (threadIdx.x < 3) lbase[threadIdx.x] = lcnt[threadIdx.x] ? atomicAdd(&a.counts[threadIdx.x], lcnt[threadIdx.x]) : 0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < CL_RPT_MAX; r++) {
+        if (r >= rpt) break;
+        const int cid = base + r * CL_THREADS + threadIdx.x;
+        const int cls = cls_r[r];
+        if (cls == 1) a.list[lbase[0] + pos_r[r]] = cid;
+        else if (cls == 2) a.list[total - 1 - (lbase[1] + pos_r[r])] = cid;
+        else if (cls == 3) a.list3[lbase[2] + pos_r[r]] = cid;
+    }
 }
 
 // Friction of one contact (model/friction.cpp:15-47) with the tangential
@@ -1210,7 +1237,9 @@ void contact_terms(World& w) {
         {
             // ids and 4 points in; type, distance and list slot out
             ProfScope ps(w, PK_CONTACT_CLASSIFY, (double)n * 152.0);
-            hipLaunchKernelGGL(k_contact_classify, dim3((n + 255) / 256), dim3(256), 0, w.stream, a);
+            const int rpt = std::min(CL_RPT_MAX, std::max(1, (n + CL_THREADS * 512 - 1) / (CL_THREADS * 512)));
+            hipLaunchKernelGGL(k_contact_classify, dim3((n + CL_THREADS * rpt - 1) / (CL_THREADS * rpt)), dim3(CL_THREADS), 0,
+                               w.stream, a, rpt);
             ADB_HIP(hipGetLastError());
         }
         const int* snap = prof_snapshot(w, w.ccount.p);

@@ -45,4 +45,11 @@ __device__ __forceinline__ void append_key(bool hit, unsigned long long key, uns
     }
 }
 
+// Contact-set keys are appended to NBK buckets (one counter and one region
+// of `sub` keys each; a workgroup uses bucket blockIdx.x % NBK): device-scope
+// atomics on one address serialize (~11 ns each on gfx950), on 64 addresses
+// they overlap.  The buckets are packed before the sort (cull.hip).
+constexpr int NBK = 64;
+struct BucketList { int cnt[NBK]; int off[NBK]; };
+
 } // namespace adb

@@ -9,12 +9,14 @@
 //      CCD: union of the start and end positions), one workgroup per body;
 //   2. body pairs whose boxes overlap (tiled all-pairs test, wave-aggregated
 //      compaction);
-//   3. one workgroup per body pair: primitives of both bodies that touch the
+//   3. six workgroups per body pair: primitives of both bodies that touch the
 //      pair's overlap box are compacted into LDS lists, then every list pair is
-//      tested (conservative box prefilter, exact FP64 predicate) and hits are
-//      appended with one atomic per wavefront;
+//      tested (conservative box prefilter, exact FP64 predicate); hits gather
+//      in an LDS buffer that goes out with one atomic on one of 64 bucket
+//      counters per flush;
 //   4. vertex-ground candidates by a vertex-parallel scan;
-//   5. radix sort of 64-bit keys gives the canonical order, then decode.
+//   5. the buckets are packed, a radix sort of the 64-bit keys gives the
+//      canonical order, then decode.
 // The box prefilters are conservative, so the sets equal the exact predicate
 // over all primitive pairs (d_PT < d_hat for I != J, d_EE < d_hat for I < J).
 #include <hipcub/hipcub.hpp>
@@ -169,8 +171,9 @@ struct NarrowArgs {
     const double* box;                    // body boxes
     double r; double d_hat;
     int enable_pt, enable_ee;
-    unsigned long long* pt_keys; int pt_cap; unsigned long long* ee_keys; int ee_cap;
-    int* counters;                        // [0] pt, [1] ee
+    unsigned long long* pt_keys; unsigned long long* ee_keys;   // NBK buckets of `sub` keys each
+    int sub;
+    int* bcnt;                            // bucket counters: [0, NBK) pt, [NBK, 2 NBK) ee
     unsigned long long* toi_bits;         // CCD: [1] pt/ee min
 };
 
@@ -233,14 +236,43 @@ __device__ int stage_prims(const NarrowArgs& a, int kind, int* cursor, int p1, c
     return *cnt;
 }
 
+// Keys found by a workgroup gather in LDS (KB of them) and go out in one
+// piece: one atomic on the workgroup's bucket counter per flush.
+constexpr int KB = 512;
+struct KeyBuf { unsigned long long k[KB]; int n; int base; };
+// all threads; kind 0 = PT, 1 = EE
+__device__ void flush_keys(const NarrowArgs& a, int kind, KeyBuf& kb) {
+    __syncthreads();
+    const int n = kb.n;
+    if (n > 0) {
+        const int b = blockIdx.x % NBK;
+        if (threadIdx.x == 0) kb.base = atomicAdd(&a.bcnt[kind * NBK + b], n);
+        __syncthreads();
+        unsigned long long* out = (kind == 0 ? a.pt_keys : a.ee_keys) + (size_t)b * a.sub;
+        const int base = kb.base;
+        for (int i = threadIdx.x; i < n; i += NB_THREADS)
+            if (base + i < a.sub) out[base + i] = kb.k[i];
+        __syncthreads();
+        if (threadIdx.x == 0) kb.n = 0;
+    }
+    __syncthreads();
+}
+
 // exact tests of the queued candidates (LDS index pairs into the staged lists)
+template <int CCD> struct KeySpace { KeyBuf kb; };   // LDS key buffer of the contact-set pass only
+template <> struct KeySpace<1> {};
+
 template <int CCD>
 __device__ void flush_candidates(const NarrowArgs& a, int kind, const int* idA, const int* idB, const int2* q, int* qn,
-                                 double& tmin) {
+                                 double& tmin, KeyBuf* kb) {
     __syncthreads();
     const int n = *qn;
     const int nround = (n + NB_THREADS - 1) / NB_THREADS;
     for (int it = 0; it < nround; it++) {
+        if constexpr (!CCD) {
+            __syncthreads();
+            if (kb->n + NB_THREADS > KB) flush_keys(a, kind, *kb);
+        }
         const int t = it * NB_THREADS + threadIdx.x;
         bool hit = false;
         unsigned long long key = 0;
@@ -280,9 +312,15 @@ __device__ void flush_candidates(const NarrowArgs& a, int kind, const int* idA, 
                 }
             }
         }
-        if (!CCD) {
-            if (kind == 0) append_key(hit, key, a.pt_keys, a.pt_cap, &a.counters[0]);
-            else append_key(hit, key, a.ee_keys, a.ee_cap, &a.counters[1]);
+        if constexpr (!CCD) {
+            const unsigned long long m = __ballot(hit);
+            if (m) {
+                const int lane = __lane_id(), leader = __ffsll((long long)m) - 1;
+                int b = 0;
+                if (lane == leader) b = atomicAdd(&kb->n, __popcll(m));
+                b = __shfl(b, leader);
+                if (hit) kb->k[b + __popcll(m & ((1ull << lane) - 1ull))] = key;
+            }
         }
     }
     __syncthreads();
@@ -302,6 +340,12 @@ __global__ __launch_bounds__(NB_THREADS) void k_pair_narrow(NarrowArgs a) {
     __shared__ int2 q[QCAP];
     __shared__ int cntA, cntB, qn;
     __shared__ double bmin[NB_THREADS / 64];
+    __shared__ KeySpace<CCD> ks;
+    KeyBuf* kb = nullptr;
+    if constexpr (!CCD) {
+        kb = &ks.kb;
+        if (threadIdx.x == 0) kb->n = 0;
+    }
     const int pidx = blockIdx.x;
     if (pidx >= a.n_pairs) return;
     const int2 pr = a.pairs[pidx];
@@ -317,8 +361,8 @@ __global__ __launch_bounds__(NB_THREADS) void k_pair_narrow(NarrowArgs a) {
     // faces of I, 2 + q = edges of I in slice q (of EE_SLICES) vs edges of J
     const int task = blockIdx.y;
     const int pass = task < 2 ? task : 2;
+    const int kind = pass < 2 ? 0 : 1;                            // 0 = PT, 1 = EE
     if (pass < 2 ? a.enable_pt : a.enable_ee) {
-        const int kind = pass < 2 ? 0 : 1;                        // 0 = PT, 1 = EE
         const int bA = pass == 1 ? pr.y : pr.x, bB = pass == 1 ? pr.x : pr.y;
         const int kA = pass < 2 ? 0 : 2, kB = pass < 2 ? 1 : 2;   // primitive kinds
         const int* offA = kA == 0 ? a.voff : a.eoff;
@@ -341,7 +385,7 @@ __global__ __launch_bounds__(NB_THREADS) void k_pair_narrow(NarrowArgs a) {
                 const int total = nA * nB;
                 for (int t0 = 0; t0 < total; t0 += NB_THREADS * PAIRS_PT) {
                     __syncthreads();
-                    if (qn + NB_THREADS * PAIRS_PT > QCAP) flush_candidates<CCD>(a, kind, idA, idB, q, &qn, tmin);
+                    if (qn + NB_THREADS * PAIRS_PT > QCAP) flush_candidates<CCD>(a, kind, idA, idB, q, &qn, tmin, kb);
 #pragma unroll
                     for (int u = 0; u < PAIRS_PT; u++) {
                         const int t = t0 + u * NB_THREADS + threadIdx.x;
@@ -354,10 +398,11 @@ __global__ __launch_bounds__(NB_THREADS) void k_pair_narrow(NarrowArgs a) {
                         }
                     }
                 }
-                flush_candidates<CCD>(a, kind, idA, idB, q, &qn, tmin);
+                flush_candidates<CCD>(a, kind, idA, idB, q, &qn, tmin, kb);
             }
         }
     }
+    if constexpr (!CCD) flush_keys(a, kind, *kb);
     if (CCD) {
         for (int m = 32; m >= 1; m >>= 1) tmin = fmin(tmin, __shfl_xor(tmin, m));
         if (__lane_id() == 0) bmin[threadIdx.x / 64] = tmin;
@@ -424,6 +469,15 @@ static void sort_keys(World& w, unsigned long long* in, unsigned long long* out,
     ADB_HIP(hipcub::DeviceRadixSort::SortKeys(w.sort_tmp.p, tmp, in, out, n, 0, end_bit, w.stream));
 }
 
+// packs bucket b's cnt[b] keys (at src + b sub) to dst + off[b]
+__global__ void k_gather_buckets(BucketList L, int sub, const unsigned long long* __restrict__ src,
+                                 unsigned long long* __restrict__ dst) {
+    const int b = blockIdx.y;
+    const int n = L.cnt[b];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        dst[L.off[b] + i] = src[(size_t)b * sub + i];
+}
+
 static int bits_for(long long n) { int b = 1; while ((1ll << b) <= n) b++; return b; }
 
 // body boxes + pairs; returns the pair count
@@ -462,8 +516,7 @@ static NarrowArgs narrow_args(World& w, int npairs, const double* X0, const doub
     a.X0 = X0; a.X1 = X1; a.box = w.bbox.p; a.r = r; a.d_hat = w.P.d_hat;
     a.enable_pt = w.P.enable_pt && w.F > 0;
     a.enable_ee = w.P.enable_ee && w.E > 0;
-    a.pt_keys = nullptr; a.pt_cap = 0; a.ee_keys = nullptr; a.ee_cap = 0;
-    a.counters = w.counters.p;
+    a.pt_keys = nullptr; a.ee_keys = nullptr; a.sub = 0; a.bcnt = nullptr;
     a.toi_bits = nullptr;
     return a;
 }
@@ -472,18 +525,19 @@ void collision_set(World& w, const double* X, ContactSet& out) {
     const double r = sqrt(w.P.d_hat) * (1.0 + 1e-6) + 1e-12;
     const bool hashed = !(w.P.iaabb & 1);    // spatial hashing (hash.hip) instead of body overlaps
     int npairs = hashed ? 0 : body_pairs(w, X, nullptr, r);
-    w.counters.reserve(16);
+    w.bcnt.reserve(2 * NBK + 1);             // PT buckets, EE buckets, VG counter
     for (int attempt = 0; attempt < 4; attempt++) {
-        size_t pcap = std::max<size_t>(w.keys_a.cap / 2, 1 << 16);
+        const size_t sub = std::max<size_t>(w.keys_a.cap / (2 * NBK), 2048);
+        const size_t pcap = sub * NBK;       // keys_a: PT buckets in [0, pcap), EE buckets in [pcap, 2 pcap)
         w.keys_a.reserve(2 * pcap);
         w.keys_b.reserve(2 * pcap);
-        size_t vcap = w.V + 64;
-        ADB_HIP(hipMemsetAsync(w.counters.p, 0, 4 * sizeof(int), w.stream));
+        const size_t vcap = w.V + 64;
+        w.vg_keys.reserve(vcap);
+        ADB_HIP(hipMemsetAsync(w.bcnt.p, 0, (2 * NBK + 1) * sizeof(int), w.stream));
         NarrowArgs a = narrow_args(w, npairs, X, nullptr, r);
-        a.pt_keys = w.keys_a.p; a.pt_cap = (int)pcap;
-        a.ee_keys = w.keys_a.p + pcap; a.ee_cap = (int)pcap;
+        a.pt_keys = w.keys_a.p; a.ee_keys = w.keys_a.p + pcap; a.sub = (int)sub; a.bcnt = w.bcnt.p;
         if (hashed) {
-            hash_sets(w, X, r, a.pt_keys, a.pt_cap, a.ee_keys, a.ee_cap, w.counters.p);
+            hash_sets(w, X, r, a.pt_keys, a.ee_keys, (int)sub, w.bcnt.p);
         } else if (npairs > 0) {
             ProfScope ps(w, PK_NARROW_SET, 0.0);
             const dim3 grid(npairs, NARROW_TASKS);
@@ -491,38 +545,45 @@ void collision_set(World& w, const double* X, ContactSet& out) {
             else hipLaunchKernelGGL((k_pair_narrow<0, double>), grid, dim3(NB_THREADS), 0, w.stream, a);
             ADB_HIP(hipGetLastError());
         }
-        int h[4];
-        ADB_HIP(hipMemcpyAsync(h, w.counters.p, 4 * sizeof(int), hipMemcpyDeviceToHost, w.stream));
-        ADB_HIP(hipStreamSynchronize(w.stream));
-        // vertex-ground
-        w.vg_keys.reserve(vcap);
-        int nvg = 0;
         if (w.P.enable_ground) {
             ProfScope ps(w, PK_VG_SCAN, 8.0 * w.V);
             hipLaunchKernelGGL(k_vg_scan, dim3((w.V + 255) / 256), dim3(256), 0, w.stream, w.V, X, w.P.ground_y, w.P.d_hat,
-                               w.vg_keys.p, (int)vcap, w.counters.p + 3);
+                               w.vg_keys.p, (int)vcap, w.bcnt.p + 2 * NBK);
             ADB_HIP(hipGetLastError());
-            ADB_HIP(hipMemcpyAsync(&nvg, w.counters.p + 3, sizeof(int), hipMemcpyDeviceToHost, w.stream));
-            ADB_HIP(hipStreamSynchronize(w.stream));
         }
-        if ((size_t)h[0] > pcap || (size_t)h[1] > pcap) {
-            size_t need = std::max(h[0], h[1]);
-            w.keys_a.reserve(2 * need); w.keys_b.reserve(2 * need);
+        int h[2 * NBK + 1];
+        ADB_HIP(hipMemcpyAsync(h, w.bcnt.p, sizeof h, hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipStreamSynchronize(w.stream));
+        const int nvg = w.P.enable_ground ? h[2 * NBK] : 0;
+        int most = 0;
+        for (int i = 0; i < 2 * NBK; i++) most = std::max(most, h[i]);
+        if ((size_t)most > sub) {
+            const size_t need = (size_t)most + most / 4 + 64;
+            w.keys_a.reserve(2 * NBK * need); w.keys_b.reserve(2 * NBK * need);
             continue;
         }
-        const int npt = h[0], nee = h[1];
+        BucketList Lp, Le;
+        int npt = 0, nee = 0;
+        for (int b = 0; b < NBK; b++) {
+            Lp.cnt[b] = h[b]; Lp.off[b] = npt; npt += h[b];
+            Le.cnt[b] = h[NBK + b]; Le.off[b] = nee; nee += h[NBK + b];
+        }
         out.pt.reserve(std::max(npt, 1)); out.ee.reserve(std::max(nee, 1)); out.vg.reserve(std::max(nvg, 1));
-        const int vb = bits_for(w.V), fb = bits_for(w.F), eb = bits_for(w.E);
+        const int vb = bits_for(w.V), eb = bits_for(w.E);
+        // pack the buckets into keys_b, sort back into keys_a, decode
         if (npt > 0) {
-            sort_keys(w, a.pt_keys, w.keys_b.p, npt, 32 + vb);
-            hipLaunchKernelGGL(k_decode4, dim3((npt + 255) / 256), dim3(256), 0, w.stream, npt, w.keys_b.p, w.vbody.p, w.voff.p,
+            hipLaunchKernelGGL(k_gather_buckets, dim3(std::min(64, (most + 255) / 256), NBK), dim3(256), 0, w.stream, Lp, (int)sub,
+                               w.keys_a.p, w.keys_b.p);
+            sort_keys(w, w.keys_b.p, w.keys_a.p, npt, 32 + vb);
+            hipLaunchKernelGGL(k_decode4, dim3((npt + 255) / 256), dim3(256), 0, w.stream, npt, w.keys_a.p, w.vbody.p, w.voff.p,
                                w.fbody.p, w.foff.p, out.pt.p);
             ADB_HIP(hipGetLastError());
         }
-        (void)fb;
         if (nee > 0) {
-            sort_keys(w, a.ee_keys, w.keys_b.p + pcap, nee, 32 + eb);
-            hipLaunchKernelGGL(k_decode4, dim3((nee + 255) / 256), dim3(256), 0, w.stream, nee, w.keys_b.p + pcap, w.ebody.p, w.eoff.p,
+            hipLaunchKernelGGL(k_gather_buckets, dim3(std::min(64, (most + 255) / 256), NBK), dim3(256), 0, w.stream, Le, (int)sub,
+                               w.keys_a.p + pcap, w.keys_b.p + pcap);
+            sort_keys(w, w.keys_b.p + pcap, w.keys_a.p + pcap, nee, 32 + eb);
+            hipLaunchKernelGGL(k_decode4, dim3((nee + 255) / 256), dim3(256), 0, w.stream, nee, w.keys_a.p + pcap, w.ebody.p, w.eoff.p,
                                w.ebody.p, w.eoff.p, out.ee.p);
             ADB_HIP(hipGetLastError());
         }
@@ -550,8 +611,6 @@ double ccd_toi(World& w, const double* X0, const double* X1, int* status) {
     }
     const bool hashed = !(w.P.iaabb & 2);    // spatial hashing (hash.hip) instead of body overlaps
     int npairs = hashed ? 0 : body_pairs(w, X0, X1, 0.0);
-    w.counters.reserve(16);
-    ADB_HIP(hipMemsetAsync(w.counters.p, 0, 4 * sizeof(int), w.stream));
     NarrowArgs a = narrow_args(w, npairs, X0, X1, 0.0);
     a.toi_bits = w.toi_bits.p;
     if (hashed) {

@@ -130,7 +130,7 @@ struct HashQueryArgs {
     const double* X0; const double* X1;      // X1 only for CCD
     const int4* rng; const int* start; const int* ent;
     double r; double d_hat;
-    unsigned long long* keys; int cap; int* counter;   // contact sets
+    unsigned long long* keys; int sub; int* bcnt;      // contact sets: NBK buckets of `sub` keys, their counters
     unsigned long long* toi_bits;                       // CCD: [1] pair minimum
 };
 
@@ -189,8 +189,9 @@ __global__ __launch_bounds__(HB_THREADS) void k_hash_query(HashQueryArgs a) {
                                 hit = pt_dist(p, Q0, Q1, Q2, pt_type(p, Q0, Q1, Q2)) < a.d_hat;
                             }
                             if (hit) {
-                                const int at = atomicAdd(a.counter, 1);
-                                if (at < a.cap) a.keys[at] = ((unsigned long long)e << 32) | (unsigned long long)q;
+                                const int b = blockIdx.x % NBK;
+                                const int at = atomicAdd(&a.bcnt[b], 1);
+                                if (at < a.sub) a.keys[(size_t)b * a.sub + at] = ((unsigned long long)e << 32) | (unsigned long long)q;
                             }
                         } else if (EE) {
                             d3 P0[4] = {ld3(a.X0 + 3 * (size_t)ev[0]), ld3(a.X0 + 3 * (size_t)ev[1]), Q0, Q1};
@@ -261,16 +262,16 @@ static HashQueryArgs query_args(World& w, int nq, const double* X0, const double
     a.X0 = X0; a.X1 = X1;
     a.rng = w.sh_rng.p; a.start = w.sh_start.p; a.ent = w.sh_ent.p;
     a.r = r; a.d_hat = w.P.d_hat;
-    a.keys = nullptr; a.cap = 0; a.counter = nullptr; a.toi_bits = nullptr;
+    a.keys = nullptr; a.sub = 0; a.bcnt = nullptr; a.toi_bits = nullptr;
     return a;
 }
 
-void hash_sets(World& w, const double* X, double r, unsigned long long* pt_keys, int pt_cap, unsigned long long* ee_keys,
-               int ee_cap, int* counters) {
+void hash_sets(World& w, const double* X, double r, unsigned long long* pt_keys, unsigned long long* ee_keys, int sub,
+               int* bcnt) {
     if (w.P.enable_pt && w.F > 0) {
         hash_build(w, false, X, nullptr);
         HashQueryArgs a = query_args(w, w.F, X, nullptr, r);
-        a.keys = pt_keys; a.cap = pt_cap; a.counter = counters;
+        a.keys = pt_keys; a.sub = sub; a.bcnt = bcnt;
         ProfScope ps(w, PK_HASH_QUERY, 0.0);
         hipLaunchKernelGGL((k_hash_query<0, 0>), dim3((w.F + HB_THREADS - 1) / HB_THREADS), dim3(HB_THREADS), 0, w.stream, a);
         ADB_HIP(hipGetLastError());
@@ -278,7 +279,7 @@ void hash_sets(World& w, const double* X, double r, unsigned long long* pt_keys,
     if (w.P.enable_ee && w.E > 0) {
         hash_build(w, true, X, nullptr);
         HashQueryArgs a = query_args(w, w.E, X, nullptr, r);
-        a.keys = ee_keys; a.cap = ee_cap; a.counter = counters + 1;
+        a.keys = ee_keys; a.sub = sub; a.bcnt = bcnt + NBK;
         ProfScope ps(w, PK_HASH_QUERY, 0.0);
         hipLaunchKernelGGL((k_hash_query<1, 0>), dim3((w.E + HB_THREADS - 1) / HB_THREADS), dim3(HB_THREADS), 0, w.stream, a);
         ADB_HIP(hipGetLastError());

@@ -101,6 +101,7 @@ struct World {
     DBuf<double> bsph;               // 4 per body: bounding sphere centre, radius
     DBuf<int2> pairs;                // body pairs (I < J)
     DBuf<int> counters;              // device counters
+    DBuf<int> bcnt;                  // contact-set key bucket counters (2 x 64) + VG counter
     DBuf<unsigned long long> keys_a, keys_b;  // unsorted / sorted keys
     DBuf<unsigned char> sort_tmp;
     DBuf<unsigned long long> toi_bits;
@@ -198,8 +199,8 @@ void transform_positions(World& w, const double* q, double* X);                 
 void q_axpy(World& w, const double* q, const double* dq, double alpha, double* out);     // cull.hip
 void collision_set(World& w, const double* X, ContactSet& out);                          // cull.hip
 double ccd_toi(World& w, const double* X0, const double* X1, int* status);              // cull.hip
-void hash_sets(World& w, const double* X, double r, unsigned long long* pt_keys, int pt_cap,
-               unsigned long long* ee_keys, int ee_cap, int* counters);                  // hash.hip
+void hash_sets(World& w, const double* X, double r, unsigned long long* pt_keys, unsigned long long* ee_keys,
+               int sub, int* bcnt);                                                      // hash.hip
 void hash_ccd(World& w, const double* X0, const double* X1, unsigned long long* toi_bits); // hash.hip
 void contact_terms(World& w);                                                            // contact.hip
 void assemble_system(World& w);                                                          // assemble.hip

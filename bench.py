@@ -9,7 +9,10 @@ kappa = 1e-4, orthogonality kappa = 1e9, Newton tol 1e-2, Armijo c1 = 1e-4,
 CCD backoff 0.8 (the reference's config.json / settings.h values).
 
 A step is one full adb_step (Newton loop with CCD and line search) on
-synthetic input already resident in HBM.  Prints one JSON line.
+synthetic input already resident in HBM; by default 1 warm-up step and 100
+timed steps, during which the bodies fall and pile up (active contacts grow
+from 0 to ~7M, so later steps cost far more than early ones).  Prints one
+JSON line (progress lines go to stderr).
 """
 import argparse
 import json
@@ -36,41 +39,60 @@ def params(precision="fp64"):
                           broad_phase_fp32=1 if precision == "mixed" else 0)
 
 
-def cpu_baseline(sc, p, budget_s):
+def cpu_baseline(sc, p, budget_s, gpu_step_s):
     """The oracle (FP64 C++/OpenMP restatement of the reference path) timed on
-    the host: as many whole steps from the initial state as fit in budget_s."""
+    the host: as many whole steps from the initial state as fit in budget_s.
+    gpu_step_s are the GPU's per-step times from the same initial state (warm-up
+    step first); both sides are compared on steps 1..K-1 of that sample (step 0
+    holds the GPU's first-call set-up)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     threads = O.lib().oracle_set_threads(min(os.cpu_count() or 1, 16))
     w = O.OracleWorld(sc, p)
     t0 = time.perf_counter()
-    steps = 0
+    per = []
     pairs = 0
     while True:
+        t = time.perf_counter()
         rc, st = w.step()
-        steps += 1
+        per.append(time.perf_counter() - t)
         pairs += st.contacts_processed
         el = time.perf_counter() - t0
-        if el >= budget_s or steps >= 50:
+        if el >= budget_s or len(per) >= 50:
             break
-    return {"value": steps / el, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": "first %d steps of the same 2000-icosphere scene from its initial state, %.1f s, "
-                      "oracle FP64 restatement with OpenMP (%d threads); contact-pairs/s %.3g"
-                      % (steps, el, threads, pairs / el)}
+    k = len(per)
+    out = {"value": k / el, "unit": "steps/s", "cores": threads, "kind": "port",
+           "sample": "first %d steps of the same 2000-icosphere scene from its initial state, %.1f s, "
+                     "oracle FP64 restatement with OpenMP (%d threads); contact-pairs/s %.3g"
+                     % (k, el, threads, pairs / el)}
+    m = min(k, len(gpu_step_s))
+    if m > 1:
+        cpu_s, gpu_s = sum(per[1:m]), sum(gpu_step_s[1:m])
+        out["same_sample"] = {"steps": "1-%d" % (m - 1), "cpu_steps_per_s": round((m - 1) / cpu_s, 4),
+                              "gpu_steps_per_s": round((m - 1) / gpu_s, 3), "gpu_over_cpu": round(cpu_s / gpu_s, 1)}
+    return out
 
 
-def run(sc, p, warmup, steps):
-    """warmup untimed steps, then `steps` timed steps; returns (seconds, totals, last stats, world)."""
+def run(sc, p, warmup, steps, label=""):
+    """warmup untimed steps, then `steps` timed steps; returns (seconds, totals,
+    last stats, world, per-step seconds of the warm-up and timed steps).
+    adb_step returns after its stream has drained, so per-step wall times are
+    exact; a progress line goes to stderr every 10 steps."""
     from adb_amd.world import World
     w = World(sc, p)
+    per_step = []
     for _ in range(warmup):
+        t = time.perf_counter()
         w.step()
+        per_step.append(time.perf_counter() - t)
     w.state()   # synchronizes the world's stream
     tot = {"contacts": 0, "newton": 0, "ls": 0, "pcg": 0, "t_ipc": 0.0, "t_solver": 0.0, "t_ccd": 0.0, "t_ls": 0.0}
     last = None
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for i in range(steps):
+        t = time.perf_counter()
         rc, st = w.step()
+        per_step.append(time.perf_counter() - t)
         last = st
         tot["contacts"] += st.contacts_processed
         tot["newton"] += st.newton_iters
@@ -80,8 +102,11 @@ def run(sc, p, warmup, steps):
         tot["t_solver"] += st.t_solver
         tot["t_ccd"] += st.t_ccd
         tot["t_ls"] += st.t_line_search
+        if (i + 1) % 10 == 0:
+            print("%s step %d/%d: %.1f s, %d contacts" % (label, i + 1, steps, time.perf_counter() - t0, st.n_pt + st.n_ee + st.n_vg),
+                  file=sys.stderr, flush=True)
     w.state()
-    return time.perf_counter() - t0, tot, last, w
+    return time.perf_counter() - t0, tot, last, w, per_step
 
 
 def max_vertex_deviation(sc, qa, qb):
@@ -149,15 +174,15 @@ def kernel_stats_md(stats_csv, title, cmd, top=25):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed",
                     help="precision of the reported value; the other one is timed beside it")
     ap.add_argument("--bodies", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the run in the other precision")
-    ap.add_argument("--profile-steps", type=int, default=2,
+    ap.add_argument("--profile-steps", type=int, default=1,
                     help="steps after the timed ones with per-kernel HIP-event timing (roofline figures)")
     ap.add_argument("--profile-json", default="")
     ap.add_argument("--copy-mb", type=float, default=0.0,
@@ -199,7 +224,7 @@ def main():
         del src, dst
     sc = workload(args.bodies)
     p = params(args.precision)
-    elapsed, tot, last, w = run(sc, p, args.warmup, args.steps)
+    elapsed, tot, last, w, per_step = run(sc, p, args.warmup, args.steps, args.precision)
     q_main, _ = w.state()
 
     # per-kernel timing on extra steps (HIP events on the world's stream)
@@ -218,7 +243,7 @@ def main():
     other = None
     if not args.no_compare:
         op = "fp64" if args.precision == "mixed" else "mixed"
-        e2, t2, _, w2 = run(sc, params(op), args.warmup, args.steps)
+        e2, t2, _, w2, _ = run(sc, params(op), args.warmup, args.steps, op)
         q2, _ = w2.state()
         del w2
         other = {"precision": op, "steps_per_s": round(args.steps / e2, 4),
@@ -267,13 +292,16 @@ def main():
         "phase_s": {"ipc": round(tot["t_ipc"], 4), "solver": round(tot["t_solver"], 4),
                     "ccd": round(tot["t_ccd"], 4), "line_search": round(tot["t_ls"], 4)},
         "contacts_last_step": {"pt": last.n_pt, "ee": last.n_ee, "vg": last.n_vg} if last else None,
+        "step_ms": {"first_10": round(1000.0 * sum(per_step[args.warmup:args.warmup + 10]) / max(1, min(10, args.steps)), 2),
+                    "median": round(1000.0 * sorted(per_step[args.warmup:])[args.steps // 2], 2) if args.steps else None,
+                    "max": round(1000.0 * max(per_step[args.warmup:]), 2) if args.steps else None},
         "other_precision": other,
         "top_kernels_ms_profiled_steps": {k: round(v["ms"], 3) for k, v in ranked[:8]},
         "roofline": roofline,
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(sc, p, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(sc, p, args.cpu_seconds, per_step)
     print(json.dumps(line))
 
 
