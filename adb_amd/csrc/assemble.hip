@@ -5,13 +5,18 @@
 // locked scatter of IPC::output_hessian_gradient (model/assemble_matrices.cpp:28-59)
 // and the diagonal += c.hess of ABD::solve (model/abd.cpp:328-341).
 //
-// No atomics: every contribution (4 per PT/EE contact, 1 marker per body) gets
-// a 64-bit key (row << 32 | col); a stable radix sort groups equal keys in a
-// fixed order, unique keys become the blocks of a BSR matrix (12x12 blocks,
-// column-major as the reference's CSC values), and one workgroup per block
-// sums its contributions in sorted order (segmented reduction).  Static
-// bodies (mass < 0) follow the reference: contributions to them are dropped,
-// their diagonal block is I and their gradient 0.
+// No atomics, every sum in a fixed order.  Contacts are radix-sorted by body
+// pair (lo, hi), lo < hi; each pair's contacts are cut into chunks of at most
+// PCH, and one workgroup per chunk reads every contact record once (staged in
+// LDS) and accumulates the chunk's three blocks (lo, lo), (hi, hi), (lo, hi)
+// and two gradients.  Then one workgroup per pair sums its (lo, hi) chunk
+// blocks into the BSR block (lo, hi) and its transpose (hi, lo), and one
+// workgroup per body sums inertia, vertex-ground terms and the diagonal chunk
+// blocks of every pair it belongs to.  The BSR matrix (12x12 blocks,
+// column-major as the reference's CSC values) has, in row b, the pairs
+// (lo, b) sorted by lo, the diagonal, then the pairs (b, hi) sorted by hi.
+// Static bodies (mass < 0) follow the reference: contributions to them are
+// dropped, their diagonal block is I and their gradient 0.
 #include <hipcub/hipcub.hpp>
 #include <algorithm>
 #include <vector>
@@ -68,50 +73,79 @@ __global__ void k_inertia(int B, double dt, double kap, double ga0, double ga1, 
                 }
 }
 
-// contribution keys: B diagonal markers first, then 4 per PT/EE contact
-// value of a contact contribution: slot (bits 0-1), number of Helmert rows
-// nA = D / 3 (bits 2-3), used (bit 4: the block's bodies are dynamic and the
-// contact active), contact id (bits 5+); body markers are -1 - body
-__global__ void k_contrib_keys(int B, int n_pt, int n_ee, const int4* __restrict__ pt, const int4* __restrict__ ee,
-                               const int* __restrict__ ccls, const double* __restrict__ mass,
-                               unsigned long long* __restrict__ keys, int* __restrict__ vals) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nc = n_pt + n_ee;
-    if (i < B) { keys[i] = ((unsigned long long)i << 32) | (unsigned long long)i; vals[i] = -1 - i; return; }
-    i -= B;
-    if (i >= 4 * nc) return;
-    const int c = i / 4, slot = i % 4;
-    int4 t = c < n_pt ? pt[c] : ee[c - n_pt];
-    const int I = t.x, J = t.z;
-    int r, col;
-    switch (slot) {
-    case 0: r = I; col = I; break;
-    case 1: r = J; col = J; break;
-    case 2: r = I; col = J; break;
-    default: r = J; col = I; break;
-    }
-    keys[B + i] = ((unsigned long long)r << 32) | (unsigned long long)col;
-    const bool di = mass[I] > 0.0, dj = mass[J] > 0.0;
-    const bool use = (slot == 0 ? di : (slot == 1 ? dj : (di && dj))) && ccls[c] > 0;
-    vals[B + i] = (c << 5) | ((use ? 1 : 0) << 4) | (ccls[c] << 2) | slot;
+// body-pair key of each contact (lo << bb | hi) and the contact id
+__global__ void k_pair_keys(int n_pt, int n_ee, const int4* __restrict__ pt, const int4* __restrict__ ee, int bb,
+                            unsigned long long* __restrict__ keys, int* __restrict__ vals) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_pt + n_ee) return;
+    const int4 t = c < n_pt ? pt[c] : ee[c - n_pt];
+    const int lo = min(t.x, t.z), hi = max(t.x, t.z);
+    keys[c] = ((unsigned long long)lo << bb) | (unsigned long long)hi;
+    vals[c] = c;
 }
 __global__ void k_flags(int n, const unsigned long long* __restrict__ keys, int* __restrict__ flag) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
 }
-// blk_of = inclusive scan of flags - 1
-__global__ void k_block_meta(int n, const unsigned long long* __restrict__ keys, const int* __restrict__ blk_of,
-                             int* __restrict__ blk_start, int* __restrict__ row_ptr, int* __restrict__ col_idx,
-                             int* __restrict__ diag_blk) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+// seg = inclusive scan of the flags: pair p = seg - 1 starts at its first key
+__global__ void k_pair_meta(int n, int bb, const unsigned long long* __restrict__ keys, const int* __restrict__ seg,
+                            int* __restrict__ pair_start, int* __restrict__ pair_lo, int* __restrict__ pair_hi) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (i == 0 || keys[i] != keys[i - 1]) {
-        const int k = blk_of[i] - 1;
-        const int row = (int)(keys[i] >> 32), col = (int)(keys[i] & 0xffffffffull);
-        blk_start[k] = i;
-        col_idx[k] = col;
-        if (row == col) diag_blk[row] = k;
-        if (i == 0 || (int)(keys[i - 1] >> 32) != row) row_ptr[row] = k;
+    if (i == n - 1) pair_start[seg[i]] = n;   // end of the last pair
+    if (!(i == 0 || keys[i] != keys[i - 1])) return;
+    const int p = seg[i] - 1;
+    pair_start[p] = i;
+    pair_lo[p] = (int)(keys[i] >> bb);
+    pair_hi[p] = (int)(keys[i] & ((1ull << bb) - 1));
+}
+// the pairs keyed by (hi, lo): the lower-triangle side of each row
+__global__ void k_pair_tkeys(int np, int bb, const int* __restrict__ pair_lo, const int* __restrict__ pair_hi,
+                             unsigned long long* __restrict__ tk, int* __restrict__ tv) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    tk[p] = ((unsigned long long)pair_hi[p] << bb) | (unsigned long long)pair_lo[p];
+    tv[p] = p;
+}
+__device__ __forceinline__ int lower_bound_int(const int* v, int n, int x) {
+    int lo = 0, hi = n;
+    while (lo < hi) { const int mid = (lo + hi) / 2; if (v[mid] < x) lo = mid + 1; else hi = mid; }
+    return lo;
+}
+__device__ __forceinline__ int lower_bound_hi(const unsigned long long* tk, int n, int bb, int x) {
+    int lo = 0, hi = n;
+    while (lo < hi) { const int mid = (lo + hi) / 2; if ((int)(tk[mid] >> bb) < x) lo = mid + 1; else hi = mid; }
+    return lo;
+}
+// per body b (b <= B): first pair with lo = b, first transposed pair with hi =
+// b, and the row length (lower pairs + diagonal + upper pairs)
+__global__ void k_body_ranges(int B, int np, int bb, const int* __restrict__ pair_lo, const unsigned long long* __restrict__ tk,
+                              int* __restrict__ up_start, int* __restrict__ low_start, int* __restrict__ rowcnt) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > B) return;
+    const int u0 = lower_bound_int(pair_lo, np, b), l0 = lower_bound_hi(tk, np, bb, b);
+    up_start[b] = u0; low_start[b] = l0;
+    if (b == B) { rowcnt[b] = 0; return; }
+    const int u1 = lower_bound_int(pair_lo, np, b + 1), l1 = lower_bound_hi(tk, np, bb, b + 1);
+    rowcnt[b] = (l1 - l0) + 1 + (u1 - u0);
+}
+// column indices of row b; each pair's block index in its upper and lower row
+__global__ void k_fill_cols(int B, int bb, const int* __restrict__ row_ptr, const int* __restrict__ up_start,
+                            const int* __restrict__ low_start, const int* __restrict__ pair_hi,
+                            const unsigned long long* __restrict__ tk, const int* __restrict__ tv, int* __restrict__ col_idx,
+                            int* __restrict__ diag_blk, int* __restrict__ pair_up, int* __restrict__ pair_low) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    int k = row_ptr[b];
+    for (int i = low_start[b]; i < low_start[b + 1]; i++, k++) {
+        col_idx[k] = (int)(tk[i] & ((1ull << bb) - 1));
+        pair_low[tv[i]] = k;
+    }
+    diag_blk[b] = k;
+    col_idx[k++] = b;
+    for (int p = up_start[b]; p < up_start[b + 1]; p++, k++) {
+        col_idx[k] = pair_hi[p];
+        pair_up[p] = k;
     }
 }
 __global__ void k_vg_start(int B, int n_vg, const int2* __restrict__ vg, int* __restrict__ start) {
@@ -122,189 +156,189 @@ __global__ void k_vg_start(int B, int n_vg, const int2* __restrict__ vg, int* __
     start[b] = lo;
 }
 
-struct ReduceArgs {
-    int nnzb, n_pt, n_ee, B;
-    const unsigned long long* keys; const int* vals; const int* blk_start;
-    const int4* pt; const int4* ee; const double* mass;
-    const double* rec; const double* bodyH; const double* bodyG;
-    const int2* vg; const int* vg_start; const double* vgK; const int* voff; const double* xbar;
-    const int* row_ptr; const int* col_idx;
-    double* Hval; double* grad;
-};
-// Pass 1: one 256-lane workgroup per chunk of at most RCH contributions of
-// one block (long segments - the diagonal block of a body with thousands of
-// contacts - are split so every workgroup has bounded work).
-constexpr int RW = 4;            // waves per reduction workgroup
-constexpr int RS = 5;            // contributions per wave at once (12 lanes each)
-constexpr int RCH = RW * 3 * RS; // contributions per chunk
-// Blocks below the diagonal get no chunks: block (J, I) = block (I, J)^T is
-// written by its upper partner in k_reduce_final, so each off-diagonal pair
-// is reduced once.
-__global__ void k_chunk_count(int nnzb, const unsigned long long* __restrict__ keys, const int* __restrict__ blk_start,
-                              int* __restrict__ nch) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nnzb) return;
-    const unsigned long long key = keys[blk_start[k]];
-    const bool lower = (key >> 32) > (key & 0xffffffffull);
-    nch[k] = lower ? 0 : (blk_start[k + 1] - blk_start[k] + RCH - 1) / RCH;
-}
-// chunk_off = exclusive scan of nch (nnzb + 1 entries)
-// (also stores the chunk count in counts[3] for the byte accounting)
-__global__ void k_chunk_owner(int nnzb, const int* __restrict__ chunk_off, int* __restrict__ chunk_blk, int* __restrict__ counts) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < nnzb) for (int c = chunk_off[k]; c < chunk_off[k + 1]; c++) chunk_blk[c] = k;
-    if (k == 0) counts[3] = chunk_off[nnzb];
-}
-// 144 block entries + 12 gradient entries of one wave, lane t owning t, t + 64, t + 128
-struct Acc156 { double e[3]; double g; };
-// fixed-order combine of the RW wave accumulators into wave 0
-__device__ __forceinline__ void combine_waves(Acc156& A, double (*sh)[156]) {
-    const int w = threadIdx.x / 64, t = threadIdx.x % 64;
-    if (w > 0) {
-        for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) sh[w - 1][e] = A.e[u]; }
-        if (t < 12) sh[w - 1][144 + t] = A.g;
-    }
-    __syncthreads();
-    if (w == 0) {
-        for (int q = 0; q < RW - 1; q++) {
-            for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) A.e[u] += sh[q][e]; }
-            if (t < 12) A.g += sh[q][144 + t];
-        }
-    }
-}
-// Pass 1 body: each wave takes RCH / RW consecutive contributions of the
-// chunk, RS at a time; 12 lanes per contribution, lane (slot, col) owning
-// column col = (be, cc) of the 12x12 block:
+// Chunks: at most PCH contacts of one pair, RSLOT contacts per round, 36
+// lanes per contact: lane (block, column) with block 0 = (lo, lo),
+// 1 = (hi, hi), 2 = (lo, hi) and column col = (be, cc) of the 12x12 block:
 //   t[(a, r)]       = sum_b PK[(3a + r) D + 3b + cc] Z[b][s2][be]
 //   entry (al, r)  += sum_a Z[a][s1][al] t[(a, r)] + G[s1][al] G[s2][be] F[3r + cc]
-// (lane col 0 of each slot also adds the 12 gradient entries).  The RW x RS
-// slot sums are combined in a fixed order.
-__global__ __launch_bounds__(64 * RW) void k_reduce_chunks(ReduceArgs a, const int* __restrict__ chunk_off,
-                                                           const int* __restrict__ chunk_blk, double* __restrict__ part) {
-    __shared__ double P[RW][RS][156];
-    // XCD-aware order: workgroups are dealt to the 8 XCDs round-robin, so XCD
-    // x gets the contiguous chunk range [x per, (x + 1) per) - the blocks of
-    // the same rows, whose contributions share contact records, meet in one L2
+// (s1, s2 = the contact's sides of the block's row and column bodies); the
+// lanes of blocks 0 and 1 also add the gradient entry col of their body.
+constexpr int RSLOT = 7;
+constexpr int RLANES = 36;
+constexpr int PCH = 8 * RSLOT;
+constexpr int PART = 3 * 144 + 24;   // chunk partial: 3 blocks (column-major) + gradients of lo, hi
+constexpr int REC_USED = R_PK + 82;  // doubles of a record that can be in use (even)
+__global__ void k_pair_chunks(int np, const int* __restrict__ pair_start, int* __restrict__ nch) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < np) nch[p] = (pair_start[p + 1] - pair_start[p] + PCH - 1) / PCH;
+}
+// chunk_off = exclusive scan of nch (np + 1 entries); the chunk count also
+// goes to counts[3] for the byte accounting
+__global__ void k_chunk_owner(int np, const int* __restrict__ chunk_off, int* __restrict__ chunk_pair, int* __restrict__ counts) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < np) for (int c = chunk_off[k]; c < chunk_off[k + 1]; c++) chunk_pair[c] = k;
+    if (k == 0) counts[3] = chunk_off[np];
+}
+
+struct PairArgs {
+    int np, n_pt, n_ee, B;
+    const int* cid;                           // contact ids sorted by pair
+    const int* pair_start; const int* pair_lo; const int* pair_hi; const int* pair_up; const int* pair_low;
+    const int* up_start; const int* low_start; const int* tv;
+    const int* chunk_off;
+    const int4* pt; const int4* ee; const int* ccls; const double* mass;
+    const double* rec; const double* bodyH; const double* bodyG;
+    const int2* vg; const int* vg_start; const double* vgK; const int* voff; const double* xbar;
+    const int* diag_blk;
+    double* Hval; double* grad;
+};
+
+__global__ __launch_bounds__(256) void k_pair_reduce(PairArgs a, const int* __restrict__ chunk_pair, double* __restrict__ part) {
+    __shared__ __align__(16) double rec[RSLOT][REC_USED];
+    __shared__ int sinfo[RSLOT][3];           // contact id, side of lo, number of Helmert rows (0: inactive)
+    __shared__ double P[RSLOT][RLANES][13];
+    // XCD-aware order: workgroups are dealt to the 8 XCDs round-robin; XCD x
+    // takes the contiguous chunk range [x per, (x + 1) per)
     const int per = gridDim.x / 8;
     const int ch = (blockIdx.x % 8) * per + blockIdx.x / 8;
-    if (ch >= chunk_off[a.nnzb]) return;
-    const int k = chunk_blk[ch];
-    const int beg = a.blk_start[k] + (ch - chunk_off[k]) * RCH;
-    const int end = min(beg + RCH, a.blk_start[k + 1]);
-    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-    const int slot = l / 12, col = l % 12, be = col / 3, cc = col % 3;
-    const int wb = min(beg + w * (RCH / RW), end), we = min(wb + RCH / RW, end);
+    if (ch >= a.chunk_off[a.np]) return;
+    const int p = chunk_pair[ch];
+    const int beg = a.pair_start[p] + (ch - a.chunk_off[p]) * PCH;
+    const int end = min(beg + PCH, a.pair_start[p + 1]);
+    const int lo = a.pair_lo[p], hi = a.pair_hi[p];
+    const bool dlo = a.mass[lo] > 0.0, dhi = a.mass[hi] > 0.0;
+    const int t = threadIdx.x;
+    const int slot = t / RLANES, rem = t % RLANES, blk = rem / 12, col = rem % 12, be = col / 3, cc = col % 3;
+    const bool use = slot < RSLOT && (blk == 0 ? dlo : (blk == 1 ? dhi : (dlo && dhi)));
     double acc[12], gacc = 0.0;
 #pragma unroll
     for (int i = 0; i < 12; i++) acc[i] = 0.0;
-    for (int r0 = wb; r0 < we; r0 += RS) {
-        const int s = r0 + slot;
-        if (slot < RS && s < we) {
-            const int v = a.vals[s];
-            if (v < 0) {     // diagonal marker: inertia block and gradient of body -1 - v
-                const int b = -1 - v;
+    for (int r0 = beg; r0 < end; r0 += RSLOT) {
+        const int nr = min(RSLOT, end - r0);
+        __syncthreads();                       // the previous round's records are consumed
+        if (t < nr) {
+            const int c = a.cid[r0 + t];
+            const int I = c < a.n_pt ? a.pt[c].x : a.ee[c - a.n_pt].x;
+            sinfo[t][0] = c; sinfo[t][1] = I == lo ? 0 : 1; sinfo[t][2] = a.ccls[c];
+        }
+        __syncthreads();
+        for (int e = t; e < nr * (REC_USED / 2); e += 256) {
+            const int sl = e / (REC_USED / 2), o = e % (REC_USED / 2);
+            const int nA = sinfo[sl][2];
+            if (nA > 0 && 2 * o < R_PK + 9 * nA * nA)
+                reinterpret_cast<double2*>(rec[sl])[o] = reinterpret_cast<const double2*>(a.rec + REC * (size_t)sinfo[sl][0])[o];
+        }
+        __syncthreads();
+        if (use && slot < nr && sinfo[slot][2] > 0) {
+            const double* R = rec[slot];
+            const int nA = sinfo[slot][2], D = 3 * nA;
+            const int slo = sinfo[slot][1], shi = 1 - slo;
+            const int s1 = blk == 1 ? shi : slo, s2 = blk == 0 ? slo : shi;
+            double tt[3][3];
 #pragma unroll
-                for (int i = 0; i < 12; i++) acc[i] += a.bodyH[144 * (size_t)b + 12 * col + i];
-                gacc += a.bodyG[12 * (size_t)b + col];
-            } else if ((v >> 4) & 1) {
-                const int c = v >> 5, cs = v & 3, nA = (v >> 2) & 3, D = 3 * nA;
-                const double* R = a.rec + REC * (size_t)c;
-                {
-                    // cs 0 = (I, I), 1 = (J, J), 2 = (I, J), 3 = (J, I)
-                    const int s1 = (cs == 1 || cs == 3) ? 1 : 0, s2 = (cs == 1 || cs == 2) ? 1 : 0;
-                    double t[3][3];
+            for (int aa = 0; aa < 3; aa++)
 #pragma unroll
-                    for (int aa = 0; aa < 3; aa++)
+                for (int r = 0; r < 3; r++) {
+                    double sum = 0.0;
 #pragma unroll
-                        for (int r = 0; r < 3; r++) {
-                            double sum = 0.0;
-#pragma unroll
-                            for (int bb = 0; bb < 3; bb++)
-                                if (aa < nA && bb < nA) sum += R[R_PK + (3 * aa + r) * D + 3 * bb + cc] * R[R_Z + 8 * bb + 4 * s2 + be];
-                            t[aa][r] = sum;
-                        }
-                    const double g2 = R[R_G + 4 * s2 + be];
-                    const double F0 = R[R_F + cc], F1 = R[R_F + 3 + cc], F2 = R[R_F + 6 + cc];
-#pragma unroll
-                    for (int al = 0; al < 4; al++) {
-                        const double gg = R[R_G + 4 * s1 + al] * g2;
-                        double z[3];
-#pragma unroll
-                        for (int aa = 0; aa < 3; aa++) z[aa] = aa < nA ? R[R_Z + 8 * aa + 4 * s1 + al] : 0.0;
-                        acc[3 * al + 0] += (z[0] * t[0][0] + z[1] * t[1][0] + z[2] * t[2][0]) + gg * F0;
-                        acc[3 * al + 1] += (z[0] * t[0][1] + z[1] * t[1][1] + z[2] * t[2][1]) + gg * F1;
-                        acc[3 * al + 2] += (z[0] * t[0][2] + z[1] * t[1][2] + z[2] * t[2][2]) + gg * F2;
-                    }
-                    if (cs < 2) gacc += R[R_GRAD + 12 * s1 + col];
+                    for (int bb = 0; bb < 3; bb++)
+                        if (aa < nA && bb < nA) sum += R[R_PK + (3 * aa + r) * D + 3 * bb + cc] * R[R_Z + 8 * bb + 4 * s2 + be];
+                    tt[aa][r] = sum;
                 }
+            const double g2 = R[R_G + 4 * s2 + be];
+            const double F0 = R[R_F + cc], F1 = R[R_F + 3 + cc], F2 = R[R_F + 6 + cc];
+#pragma unroll
+            for (int al = 0; al < 4; al++) {
+                const double gg = R[R_G + 4 * s1 + al] * g2;
+                double z[3];
+#pragma unroll
+                for (int aa = 0; aa < 3; aa++) z[aa] = aa < nA ? R[R_Z + 8 * aa + 4 * s1 + al] : 0.0;
+                acc[3 * al + 0] += (z[0] * tt[0][0] + z[1] * tt[1][0] + z[2] * tt[2][0]) + gg * F0;
+                acc[3 * al + 1] += (z[0] * tt[0][1] + z[1] * tt[1][1] + z[2] * tt[2][1]) + gg * F1;
+                acc[3 * al + 2] += (z[0] * tt[0][2] + z[1] * tt[1][2] + z[2] * tt[2][2]) + gg * F2;
             }
+            if (blk < 2) gacc += R[R_GRAD + 12 * s1 + col];
         }
     }
-    if (slot < RS) {
+    if (slot < RSLOT) {
 #pragma unroll
-        for (int i = 0; i < 12; i++) P[w][slot][12 * col + i] = acc[i];
-        P[w][slot][144 + col] = gacc;
+        for (int i = 0; i < 12; i++) P[slot][rem][i] = acc[i];
+        P[slot][rem][12] = gacc;
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < 156; e += 64 * RW) {
+    for (int e = t; e < PART; e += 256) {
+        int ln, row;
+        if (e < 432) { ln = 12 * (e / 144) + (e % 144) / 12; row = e % 12; }
+        else { ln = e - 432; row = 12; }      // gradient of lo (lanes 0-11), of hi (lanes 12-23)
         double sum = 0.0;
-        for (int q = 0; q < RW; q++)
-            for (int u = 0; u < RS; u++) sum += P[q][u][e];
-        part[156 * (size_t)ch + e] = sum;
+        for (int sl = 0; sl < RSLOT; sl++) sum += P[sl][ln][row];
+        part[PART * (size_t)ch + e] = sum;
     }
 }
 
-// Pass 2: one workgroup per block sums its chunk partials (wave w: chunks
-// w, w + 4, ..., then a fixed-order combine), then the body's vertex-ground
-// terms for diagonal blocks.
-__global__ __launch_bounds__(64 * RW) void k_reduce_final(ReduceArgs a, const int* __restrict__ chunk_off, const double* __restrict__ part) {
-    __shared__ double sh[RW - 1][156];
-    const int k = blockIdx.x;
-    if (k >= a.nnzb) return;
-    const unsigned long long key = a.keys[a.blk_start[k]];
-    const int row = (int)(key >> 32), col = (int)(key & 0xffffffffull);
-    if (row > col) return;   // written transposed by block (col, row)
-    const int w = threadIdx.x / 64, t = threadIdx.x % 64;
-    Acc156 A = {{0.0, 0.0, 0.0}, 0.0};
-    for (int c = chunk_off[k] + w; c < chunk_off[k + 1]; c += RW) {
-        const double* P = part + 156 * (size_t)c;
-        for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) A.e[u] += P[e]; }
-        if (t < 12) A.g += P[144 + t];
+// off-diagonal blocks: one 64-lane workgroup per pair, its chunk blocks (lo, hi)
+// summed in order, written to (lo, hi) and transposed to (hi, lo)
+__global__ __launch_bounds__(64) void k_pair_offdiag(PairArgs a, const double* __restrict__ part) {
+    const int p = blockIdx.x;
+    if (p >= a.np) return;
+    const int t = threadIdx.x;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int c = a.chunk_off[p]; c < a.chunk_off[p + 1]; c++) {
+        const double* P = part + PART * (size_t)c + 288;
+        for (int u = 0; u < 3; u++) { const int e = t + 64 * u; if (e < 144) acc[u] += P[e]; }
     }
-    combine_waves(A, sh);
-    if (w != 0) return;
-    double acc[3] = {A.e[0], A.e[1], A.e[2]};
-    double gacc = A.g;
-    if (row == col && a.mass[row] > 0.0) {
+    double* U = a.Hval + 144 * (size_t)a.pair_up[p];
+    double* L = a.Hval + 144 * (size_t)a.pair_low[p];
+    for (int u = 0; u < 3; u++) {
+        const int e = t + 64 * u;
+        if (e < 144) { U[e] = acc[u]; L[12 * (e % 12) + e / 12] = acc[u]; }
+    }
+}
+
+// diagonal blocks and gradients: one 64-lane workgroup per body: inertia,
+// the (lo, lo) chunk blocks of its pairs (b, hi), the (hi, hi) chunk blocks
+// of its pairs (lo, b), each in order, then its vertex-ground terms
+__global__ __launch_bounds__(64) void k_pair_diag(PairArgs a, const double* __restrict__ part) {
+    const int b = blockIdx.x;
+    if (b >= a.B) return;
+    const int t = threadIdx.x;
+    double acc[3], gacc = 0.0;
+    for (int u = 0; u < 3; u++) { const int e = t + 64 * u; acc[u] = e < 144 ? a.bodyH[144 * (size_t)b + e] : 0.0; }
+    if (t < 12) gacc = a.bodyG[12 * (size_t)b + t];
+    // pairs (b, hi): their chunks are contiguous
+    for (int c = a.chunk_off[a.up_start[b]]; c < a.chunk_off[a.up_start[b + 1]]; c++) {
+        const double* P = part + PART * (size_t)c;
+        for (int u = 0; u < 3; u++) { const int e = t + 64 * u; if (e < 144) acc[u] += P[e]; }
+        if (t < 12) gacc += P[432 + t];
+    }
+    // pairs (lo, b)
+    for (int i = a.low_start[b]; i < a.low_start[b + 1]; i++) {
+        const int p = a.tv[i];
+        for (int c = a.chunk_off[p]; c < a.chunk_off[p + 1]; c++) {
+            const double* P = part + PART * (size_t)c;
+            for (int u = 0; u < 3; u++) { const int e = t + 64 * u; if (e < 144) acc[u] += P[144 + e]; }
+            if (t < 12) gacc += P[444 + t];
+        }
+    }
+    if (a.mass[b] > 0.0) {
         // vertex-ground terms of this body: (w w^T) (x) K, gradient w (x) gx
-        for (int s = a.vg_start[row]; s < a.vg_start[row + 1]; s++) {
-            const int v = a.voff[row] + a.vg[s].y;
+        for (int s = a.vg_start[b]; s < a.vg_start[b + 1]; s++) {
+            const int v = a.voff[b] + a.vg[s].y;
             const double w4[4] = {1.0, a.xbar[3 * (size_t)v], a.xbar[3 * (size_t)v + 1], a.xbar[3 * (size_t)v + 2]};
             const double* K = a.vgK + 12 * (size_t)s;
             for (int u = 0; u < 3; u++) {
-                int e = t + 64 * u;
+                const int e = t + 64 * u;
                 if (e < 144) {
-                    int cc = e / 12, rr = e % 12;
+                    const int cc = e / 12, rr = e % 12;
                     acc[u] += w4[rr / 3] * w4[cc / 3] * K[3 * (cc % 3) + (rr % 3)];
                 }
             }
             if (t < 12) gacc += w4[t / 3] * K[9 + t % 3];
         }
     }
-    for (int u = 0; u < 3; u++) { int e = t + 64 * u; if (e < 144) a.Hval[144 * (size_t)k + e] = acc[u]; }
-    if (row == col && t < 12) a.grad[12 * (size_t)row + t] = gacc;
-    if (row < col) {
-        // the mirror block (col, row) in row col of the BSR structure
-        int lo = a.row_ptr[col], hi = a.row_ptr[col + 1];
-        while (lo < hi) { const int mid = (lo + hi) / 2; if (a.col_idx[mid] < row) lo = mid + 1; else hi = mid; }
-        if (lo < a.row_ptr[col + 1] && a.col_idx[lo] == row) {   // always: the structure is symmetric
-            double* M = a.Hval + 144 * (size_t)lo;
-            for (int u = 0; u < 3; u++) {
-                const int e = t + 64 * u;
-                if (e < 144) M[12 * (e % 12) + e / 12] = acc[u];
-            }
-        }
-    }
+    double* H = a.Hval + 144 * (size_t)a.diag_blk[b];
+    for (int u = 0; u < 3; u++) { const int e = t + 64 * u; if (e < 144) H[e] = acc[u]; }
+    if (t < 12) a.grad[12 * (size_t)b + t] = gacc;
 }
 
 // Preconditioner: the explicit inverse of the diagonal sub-matrix of each
@@ -475,84 +509,105 @@ void assemble_system(World& w) {
                        w.q.p, w.q0.p, w.qdot.p, w.mass.p, w.Ic.p, w.bodyG.p, w.bodyH.p);
     ADB_HIP(hipGetLastError());
     const int nc = w.cs.n_pt + w.cs.n_ee;
-    const int n = B + 4 * nc;
-    w.n_contrib = n;
-    w.akeys.reserve(n); w.akeys_s.reserve(n); w.aval.reserve(n); w.aval_s.reserve(n);
-    w.uniq_flag.reserve(n); w.blk_of.reserve(n);
-    hipLaunchKernelGGL(k_contrib_keys, dim3((n + 255) / 256), dim3(256), 0, w.stream, B, w.cs.n_pt, w.cs.n_ee, w.cs.pt.p, w.cs.ee.p,
-                       w.ccls.p, w.mass.p,
-                       w.akeys.p, w.aval.p);
-    ADB_HIP(hipGetLastError());
     int bb = 1; while ((1ll << bb) <= B) bb++;
-    sort_pairs(w, w.akeys.p, w.akeys_s.p, w.aval.p, w.aval_s.p, n, 32 + bb);
-    hipLaunchKernelGGL(k_flags, dim3((n + 255) / 256), dim3(256), 0, w.stream, n, w.akeys_s.p, w.uniq_flag.p);
+    // 1. contacts sorted by body pair; pair segments
+    w.akeys.reserve(std::max(nc, 1)); w.akeys_s.reserve(std::max(nc, 1));
+    w.aval.reserve(std::max(nc, 1)); w.aval_s.reserve(std::max(nc, 1));
+    w.uniq_flag.reserve(std::max(nc, 1)); w.blk_of.reserve(std::max(nc, 1));
+    int np = 0;
+    if (nc > 0) {
+        hipLaunchKernelGGL(k_pair_keys, dim3((nc + 255) / 256), dim3(256), 0, w.stream, w.cs.n_pt, w.cs.n_ee, w.cs.pt.p, w.cs.ee.p, bb,
+                           w.akeys.p, w.aval.p);
+        ADB_HIP(hipGetLastError());
+        sort_pairs(w, w.akeys.p, w.akeys_s.p, w.aval.p, w.aval_s.p, nc, 2 * bb);
+        hipLaunchKernelGGL(k_flags, dim3((nc + 255) / 256), dim3(256), 0, w.stream, nc, w.akeys_s.p, w.uniq_flag.p);
+        ADB_HIP(hipGetLastError());
+        size_t tmp = 0;
+        ADB_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tmp, w.uniq_flag.p, w.blk_of.p, nc, w.stream));
+        w.sort_tmp.reserve(tmp + 16);
+        ADB_HIP(hipcub::DeviceScan::InclusiveSum(w.sort_tmp.p, tmp, w.uniq_flag.p, w.blk_of.p, nc, w.stream));
+        ADB_HIP(hipMemcpyAsync(&np, w.blk_of.p + nc - 1, sizeof(int), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipStreamSynchronize(w.stream));
+    }
+    w.n_pairs = np;
+    w.pair_start.reserve(np + 1); w.pair_lo.reserve(std::max(np, 1)); w.pair_hi.reserve(std::max(np, 1));
+    w.pair_up.reserve(std::max(np, 1)); w.pair_low.reserve(std::max(np, 1));
+    w.tkeys.reserve(std::max(np, 1)); w.tkeys_s.reserve(std::max(np, 1)); w.tval.reserve(std::max(np, 1)); w.tval_s.reserve(std::max(np, 1));
+    if (nc > 0) {
+        hipLaunchKernelGGL(k_pair_meta, dim3((nc + 255) / 256), dim3(256), 0, w.stream, nc, bb, w.akeys_s.p, w.blk_of.p,
+                           w.pair_start.p, w.pair_lo.p, w.pair_hi.p);
+        ADB_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_pair_tkeys, dim3((np + 255) / 256), dim3(256), 0, w.stream, np, bb, w.pair_lo.p, w.pair_hi.p, w.tkeys.p,
+                           w.tval.p);
+        ADB_HIP(hipGetLastError());
+        sort_pairs(w, w.tkeys.p, w.tkeys_s.p, w.tval.p, w.tval_s.p, np, 2 * bb);
+    }
+    if (nc == 0) ADB_HIP(hipMemsetAsync(w.pair_start.p, 0, sizeof(int), w.stream));
+    // 2. BSR structure: row b = pairs (lo, b), diagonal, pairs (b, hi)
+    const int nnzb = B + 2 * np;
+    w.nnzb = nnzb;
+    w.up_start.reserve(B + 1); w.low_start.reserve(B + 1); w.rowcnt.reserve(B + 1);
+    w.row_ptr.reserve(B + 1); w.col_idx.reserve(nnzb); w.diag_blk.reserve(B);
+    w.Hval.reserve(144 * (size_t)nnzb);
+    hipLaunchKernelGGL(k_body_ranges, dim3((B + 1 + 255) / 256), dim3(256), 0, w.stream, B, np, bb, w.pair_lo.p, w.tkeys_s.p,
+                       w.up_start.p, w.low_start.p, w.rowcnt.p);
     ADB_HIP(hipGetLastError());
     {
         size_t tmp = 0;
-        ADB_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tmp, w.uniq_flag.p, w.blk_of.p, n, w.stream));
+        ADB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, w.rowcnt.p, w.row_ptr.p, B + 1, w.stream));
         w.sort_tmp.reserve(tmp + 16);
-        ADB_HIP(hipcub::DeviceScan::InclusiveSum(w.sort_tmp.p, tmp, w.uniq_flag.p, w.blk_of.p, n, w.stream));
+        ADB_HIP(hipcub::DeviceScan::ExclusiveSum(w.sort_tmp.p, tmp, w.rowcnt.p, w.row_ptr.p, B + 1, w.stream));
     }
-    int last = 0;
-    ADB_HIP(hipMemcpyAsync(&last, w.blk_of.p + n - 1, sizeof(int), hipMemcpyDeviceToHost, w.stream));
-    ADB_HIP(hipStreamSynchronize(w.stream));
-    const int nnzb = last;   // inclusive scan: count of unique keys
-    w.nnzb = nnzb;
-    w.blk_start.reserve(nnzb + 1); w.col_idx.reserve(nnzb); w.row_ptr.reserve(B + 1); w.diag_blk.reserve(B);
-    w.Hval.reserve(144 * (size_t)nnzb);
-    // blk_of currently holds (inclusive count); shift to 0-based block ids in k_block_meta via -1
-    hipLaunchKernelGGL(k_block_meta, dim3((n + 255) / 256), dim3(256), 0, w.stream, n, w.akeys_s.p, w.blk_of.p,
-                       w.blk_start.p, w.row_ptr.p, w.col_idx.p, w.diag_blk.p);
+    hipLaunchKernelGGL(k_fill_cols, dim3((B + 255) / 256), dim3(256), 0, w.stream, B, bb, w.row_ptr.p, w.up_start.p, w.low_start.p,
+                       w.pair_hi.p, w.tkeys_s.p, w.tval_s.p, w.col_idx.p, w.diag_blk.p, w.pair_up.p, w.pair_low.p);
     ADB_HIP(hipGetLastError());
-    ADB_HIP(hipMemcpyAsync(w.blk_start.p + nnzb, &w.n_contrib, sizeof(int), hipMemcpyHostToDevice, w.stream));
-    ADB_HIP(hipMemcpyAsync(w.row_ptr.p + B, &w.nnzb, sizeof(int), hipMemcpyHostToDevice, w.stream));
     w.vg_start.reserve(B + 1);
     hipLaunchKernelGGL(k_vg_start, dim3((B + 1 + 255) / 256), dim3(256), 0, w.stream, B, w.cs.n_vg, w.cs.vg.p, w.vg_start.p);
     ADB_HIP(hipGetLastError());
     w.grad.reserve(12 * (size_t)B);
-    ReduceArgs a;
-    a.nnzb = nnzb; a.n_pt = w.cs.n_pt; a.n_ee = w.cs.n_ee; a.B = B;
-    a.keys = w.akeys_s.p; a.vals = w.aval_s.p; a.blk_start = w.blk_start.p;
-    a.pt = w.cs.pt.p; a.ee = w.cs.ee.p; a.mass = w.mass.p;
+    // 3. chunks of at most PCH contacts per pair
+    w.chunk_n.reserve(np + 1); w.chunk_off.reserve(np + 1);
+    if (np > 0)
+        hipLaunchKernelGGL(k_pair_chunks, dim3((np + 255) / 256), dim3(256), 0, w.stream, np, w.pair_start.p, w.chunk_n.p);
+    ADB_HIP(hipMemsetAsync(w.chunk_n.p + np, 0, sizeof(int), w.stream));
+    {
+        size_t tmp = 0;
+        ADB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, w.chunk_n.p, w.chunk_off.p, np + 1, w.stream));
+        w.sort_tmp.reserve(tmp + 16);
+        ADB_HIP(hipcub::DeviceScan::ExclusiveSum(w.sort_tmp.p, tmp, w.chunk_n.p, w.chunk_off.p, np + 1, w.stream));
+    }
+    // each pair has at most len / PCH + 1 chunks: launch for the bound, rounded
+    // up to the 8 XCDs; the kernel reads the exact count
+    const int max_chunks = (np + nc / PCH + 7) / 8 * 8;
+    w.chunk_blk.reserve(std::max(max_chunks, 1));
+    w.chunk_part.reserve(PART * (size_t)std::max(max_chunks, 1));
+    hipLaunchKernelGGL(k_chunk_owner, dim3((np + 255) / 256 + 1), dim3(256), 0, w.stream, np, w.chunk_off.p, w.chunk_blk.p, w.ccount.p);
+    ADB_HIP(hipGetLastError());
+    PairArgs a;
+    a.np = np; a.n_pt = w.cs.n_pt; a.n_ee = w.cs.n_ee; a.B = B;
+    a.cid = w.aval_s.p;
+    a.pair_start = w.pair_start.p; a.pair_lo = w.pair_lo.p; a.pair_hi = w.pair_hi.p; a.pair_up = w.pair_up.p; a.pair_low = w.pair_low.p;
+    a.up_start = w.up_start.p; a.low_start = w.low_start.p; a.tv = w.tval_s.p;
+    a.chunk_off = w.chunk_off.p;
+    a.pt = w.cs.pt.p; a.ee = w.cs.ee.p; a.ccls = w.ccls.p; a.mass = w.mass.p;
     a.rec = w.crec.p; a.bodyH = w.bodyH.p; a.bodyG = w.bodyG.p;
     a.vg = w.cs.vg.p; a.vg_start = w.vg_start.p; a.vgK = w.vgK.p; a.voff = w.voff.p; a.xbar = w.xbar.p;
-    a.row_ptr = w.row_ptr.p; a.col_idx = w.col_idx.p;
+    a.diag_blk = w.diag_blk.p;
     a.Hval = w.Hval.p; a.grad = w.grad.p;
-    {
-        // chunk layout of the segments
-        w.chunk_n.reserve(nnzb + 1); w.chunk_off.reserve(nnzb + 1);
-        hipLaunchKernelGGL(k_chunk_count, dim3((nnzb + 255) / 256), dim3(256), 0, w.stream, nnzb, w.akeys_s.p, w.blk_start.p,
-                           w.chunk_n.p);
-        ADB_HIP(hipMemsetAsync(w.chunk_n.p + nnzb, 0, sizeof(int), w.stream));
-        size_t tmp = 0;
-        ADB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, w.chunk_n.p, w.chunk_off.p, nnzb + 1, w.stream));
-        w.sort_tmp.reserve(tmp + 16);
-        ADB_HIP(hipcub::DeviceScan::ExclusiveSum(w.sort_tmp.p, tmp, w.chunk_n.p, w.chunk_off.p, nnzb + 1, w.stream));
-        // each diagonal or upper block has at most len / RCH + 1 chunks (the
-        // structure is symmetric with one diagonal block per body, so there are
-        // B + (nnzb - B) / 2 of them): launch for the bound rounded up to the 8
-        // XCDs, the kernel reads the exact count
-        const int max_chunks = (B + (nnzb - B) / 2 + n / RCH + 7) / 8 * 8;
-        w.chunk_blk.reserve(max_chunks);
-        w.chunk_part.reserve(156 * (size_t)max_chunks);
-        hipLaunchKernelGGL(k_chunk_owner, dim3((nnzb + 255) / 256), dim3(256), 0, w.stream, nnzb, w.chunk_off.p, w.chunk_blk.p,
-                           w.ccount.p);
-        ADB_HIP(hipGetLastError());
-        if (w.side_pending) {   // records of the 4-point contacts (side stream)
-            ADB_HIP(hipStreamWaitEvent(w.stream, w.ev_side[1], 0));
-            w.side_pending = false;
-        }
-        {
-            // every contact record read once (D = 3: 59 doubles, 6: 94, 9: 147), the
-            // contribution list and inertia terms read, one partial block written per chunk
-            ProfScope ps(w, PK_ASM_REDUCE, 24.0 * n + 1248.0 * B);
-            ps.counts(prof_snapshot(w, w.ccount.p), 472.0, 752.0, 1176.0, 1248.0);
-            hipLaunchKernelGGL(k_reduce_chunks, dim3(max_chunks), dim3(64 * RW), 0, w.stream, a, w.chunk_off.p, w.chunk_blk.p,
-                               w.chunk_part.p);
-        }
-        hipLaunchKernelGGL(k_reduce_final, dim3(nnzb), dim3(64 * RW), 0, w.stream, a, w.chunk_off.p, w.chunk_part.p);
+    if (w.side_pending) {   // records of the 4-point contacts (side stream)
+        ADB_HIP(hipStreamWaitEvent(w.stream, w.ev_side[1], 0));
+        w.side_pending = false;
+    }
+    if (max_chunks > 0) {
+        // every contact record read once (D = 3: 75 doubles, 6: 102, 9: 147), its id,
+        // pair side and class; one partial (3 blocks + 2 gradients) written per chunk
+        ProfScope ps(w, PK_ASM_REDUCE, 24.0 * nc);
+        ps.counts(prof_snapshot(w, w.ccount.p), 600.0, 816.0, 1176.0, 8.0 * PART);
+        hipLaunchKernelGGL(k_pair_reduce, dim3(max_chunks), dim3(256), 0, w.stream, a, w.chunk_blk.p, w.chunk_part.p);
         ADB_HIP(hipGetLastError());
     }
+    if (np > 0) hipLaunchKernelGGL(k_pair_offdiag, dim3(np), dim3(64), 0, w.stream, a, w.chunk_part.p);
+    hipLaunchKernelGGL(k_pair_diag, dim3(B), dim3(64), 0, w.stream, a, w.chunk_part.p);
     ADB_HIP(hipGetLastError());
     build_groups(w);
     {

@@ -127,13 +127,19 @@ struct World {
     DBuf<double> vglam;
     // ---- assembly / BSR
     DBuf<double> bodyH, bodyG;       // inertia terms per body (144 + 12)
-    DBuf<unsigned long long> akeys, akeys_s;   // assembly keys (row<<32|col) per contribution
-    DBuf<int> aval, aval_s;          // contribution id: contact * 4 + slot, or -1 - body
-    DBuf<int> blk_of, uniq_flag;     // per sorted contribution
-    DBuf<int> blk_start;             // first contribution of each block
+    DBuf<unsigned long long> akeys, akeys_s;   // body-pair key per contact
+    DBuf<int> aval, aval_s;          // contact ids (sorted by body pair)
+    DBuf<int> blk_of, uniq_flag;     // per sorted contact: pair segment
+    // body pairs (lo < hi) with contacts, sorted: first contact, bodies, block
+    // index in row lo (upper) and row hi (lower); the pairs keyed (hi, lo)
+    DBuf<int> pair_start, pair_lo, pair_hi, pair_up, pair_low;
+    DBuf<unsigned long long> tkeys, tkeys_s;
+    DBuf<int> tval, tval_s;
+    DBuf<int> up_start, low_start, rowcnt;   // per body: first pair (b, *), first transposed pair (*, b), row length
+    int n_pairs = 0;
     DBuf<int> row_ptr, col_idx, diag_blk, vg_start;
-    DBuf<int> chunk_n, chunk_off, chunk_blk;   // segmented-reduction chunks
-    DBuf<double> chunk_part;                   // 156 per chunk (block + gradient)
+    DBuf<int> chunk_n, chunk_off, chunk_blk;   // per pair: chunk count, first chunk; per chunk: its pair
+    DBuf<double> chunk_part;                   // 456 per chunk (3 blocks + 2 gradients)
     DBuf<double> Hval;               // 144 per block
     DBuf<double> grad;               // 12 B
     // preconditioner groups: connected components of the body graph, split in
@@ -148,7 +154,7 @@ struct World {
     std::vector<int> h_glist;
     std::vector<int> h_rp, h_ci, h_grp_off, h_grp_body;
     std::vector<long long> h_grp_moff;
-    int nnzb = 0, n_contrib = 0;
+    int nnzb = 0;
     // ---- PCG
     DBuf<double> pcg_x, pcg_r, pcg_z, pcg_p, pcg_p2, pcg_Ap, partial, scal;
     DBuf<int> pcg_k;                 // iteration base of the queued PCG batch

@@ -258,7 +258,9 @@ extern "C" int adb_world_create(const adb_scene* s, const adb_params* p, adb_wor
 
 static void free_world(World& w) {
     DBuf<int>* ib[] = {&w.voff, &w.foff, &w.eoff, &w.vbody, &w.fbody, &w.ebody, &w.face, &w.edge, &w.counters,
-                       &w.aval, &w.aval_s, &w.blk_of, &w.uniq_flag, &w.blk_start, &w.row_ptr, &w.col_idx, &w.diag_blk, &w.vg_start,
+                       &w.aval, &w.aval_s, &w.blk_of, &w.uniq_flag, &w.row_ptr, &w.col_idx, &w.diag_blk, &w.vg_start,
+                       &w.pair_start, &w.pair_lo, &w.pair_hi, &w.pair_up, &w.pair_low, &w.tval, &w.tval_s,
+                       &w.up_start, &w.low_start, &w.rowcnt,
                        &w.ctype, &w.ccls, &w.clist, &w.clist3, &w.ccount, &w.chunk_n, &w.chunk_off, &w.chunk_blk,
                        &w.grp_off, &w.grp_body, &w.glist};
     for (auto* b : ib) b->free_();
@@ -268,7 +270,7 @@ static void free_world(World& w) {
     for (auto* b : db) b->free_();
     w.pairs.free_(); w.keys_a.free_(); w.keys_b.free_(); w.sort_tmp.free_(); w.toi_bits.free_(); w.vg_keys.free_();
     w.bcnt.free_(); w.sh_cnt.free_(); w.sh_start.free_(); w.sh_ent.free_(); w.sh_rng.free_(); w.sh_total.free_();
-    w.akeys.free_(); w.akeys_s.free_();
+    w.akeys.free_(); w.akeys_s.free_(); w.tkeys.free_(); w.tkeys_s.free_();
     for (ContactSet* c : {&w.cs, &w.cs_new}) { c->pt.free_(); c->ee.free_(); c->vg.free_(); }
     for (auto& r : w.prof.pending) { w.prof.pool.push_back(r.a); w.prof.pool.push_back(r.b); }
     w.prof.pending.clear();
