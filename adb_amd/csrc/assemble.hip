@@ -342,11 +342,11 @@ __global__ __launch_bounds__(64) void k_pair_diag(PairArgs a, const double* __re
 }
 
 // Preconditioner: the explicit inverse of the diagonal sub-matrix of each
-// body group (a connected component of the contact graph, or a BFS-ordered
-// piece of one with at most GMAX bodies).  On groups that are whole
-// components the preconditioned system decouples exactly, so PCG needs only a
-// few iterations where the contact graph is a union of small clusters; in
-// large clusters it is block-Jacobi with blocks of up to GMAX bodies.
+// body group (at most GMAX bodies, built by heavy-edge aggregation of the
+// contact graph, build_groups).  On groups that are whole components the
+// preconditioned system decouples exactly, so PCG needs only a few
+// iterations where the contact graph is a union of small clusters; in large
+// clusters it is block-Jacobi with blocks of up to GMAX bodies.
 // One workgroup per group: gather the blocks into LDS, Cholesky in place,
 // W = L^-1 column by column (thread per column, stored transposed in the
 // upper triangle), then M^-1 = W^T W.  A group whose matrix is not positive
@@ -443,40 +443,55 @@ __global__ __launch_bounds__(256) void k_group_inv(const int* __restrict__ list,
     }
 }
 
-// Connected components of the block graph (host, from the BSR structure) and
-// their split into groups of at most GMAX bodies in BFS order.
+// Body groups for the preconditioner (host): heavy-edge aggregation of the
+// contact graph.  Body pairs are taken in decreasing order of their contact
+// count (ties by pair index) and their groups merged while the union has at
+// most GMAX bodies, so the strongest couplings land inside the inverted
+// diagonal sub-matrices; every connected component of at most GMAX bodies
+// ends up as one group (the preconditioned system then decouples exactly).
 static void build_groups(World& w) {
-    const int B = w.B, nnzb = w.nnzb;
-    w.h_rp.resize(B + 1); w.h_ci.resize(nnzb);
-    ADB_HIP(hipMemcpyAsync(w.h_rp.data(), w.row_ptr.p, (B + 1) * sizeof(int), hipMemcpyDeviceToHost, w.stream));
-    ADB_HIP(hipMemcpyAsync(w.h_ci.data(), w.col_idx.p, nnzb * sizeof(int), hipMemcpyDeviceToHost, w.stream));
-    ADB_HIP(hipStreamSynchronize(w.stream));
-    std::vector<char> seen(B, 0);
-    std::vector<int> queue;
-    queue.reserve(B);
+    const int B = w.B, np = w.n_pairs;
+    std::vector<int> plo(np), phi(np), ps(np + 1);
+    if (np > 0) {
+        ADB_HIP(hipMemcpyAsync(plo.data(), w.pair_lo.p, np * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipMemcpyAsync(phi.data(), w.pair_hi.p, np * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipMemcpyAsync(ps.data(), w.pair_start.p, (np + 1) * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipStreamSynchronize(w.stream));
+    }
+    // pairs by decreasing contact count, ties by index (counting sort)
+    int wmax = 0;
+    for (int p = 0; p < np; p++) wmax = std::max(wmax, ps[p + 1] - ps[p]);
+    std::vector<int> bucket(wmax + 2, 0), order(np);
+    for (int p = 0; p < np; p++) bucket[wmax - (ps[p + 1] - ps[p]) + 1]++;
+    for (int i = 1; i <= wmax + 1; i++) bucket[i] += bucket[i - 1];
+    for (int p = 0; p < np; p++) order[bucket[wmax - (ps[p + 1] - ps[p])]++] = p;
+    std::vector<int> parent(B), size(B, 1);
+    for (int b = 0; b < B; b++) parent[b] = b;
+    auto find = [&](int x) { while (parent[x] != x) { parent[x] = parent[parent[x]]; x = parent[x]; } return x; };
+    for (int p : order) {
+        int ra = find(plo[p]), rb = find(phi[p]);
+        if (ra == rb || size[ra] + size[rb] > GMAX) continue;
+        if (ra > rb) std::swap(ra, rb);
+        parent[rb] = ra; size[ra] += size[rb];
+    }
+    // groups in order of their smallest body; bodies ascending inside
+    std::vector<int> gid(B, -1), first;
     w.h_grp_off.assign(1, 0);
     w.h_grp_body.clear();
     w.h_grp_moff.assign(1, 0);
-    for (int s = 0; s < B; s++) {
-        if (seen[s]) continue;
-        queue.clear();
-        queue.push_back(s);
-        seen[s] = 1;
-        for (size_t h = 0; h < queue.size(); h++) {
-            const int r = queue[h];
-            for (int k = w.h_rp[r]; k < w.h_rp[r + 1]; k++) {
-                const int c = w.h_ci[k];
-                if (!seen[c]) { seen[c] = 1; queue.push_back(c); }
-            }
-        }
-        for (size_t q = 0; q < queue.size(); q += GMAX) {
-            const int m = (int)std::min<size_t>(GMAX, queue.size() - q);
-            for (int i = 0; i < m; i++) w.h_grp_body.push_back(queue[q + i]);
-            w.h_grp_off.push_back((int)w.h_grp_body.size());
-            w.h_grp_moff.push_back(w.h_grp_moff.back() + (long long)(12 * m) * (12 * m));
-        }
+    std::vector<std::vector<int>> members;
+    for (int b = 0; b < B; b++) {
+        const int r = find(b);
+        if (gid[r] < 0) { gid[r] = (int)members.size(); members.emplace_back(); }
+        members[gid[r]].push_back(b);
     }
-    w.n_groups = (int)w.h_grp_off.size() - 1;
+    for (const auto& m : members) {
+        for (int b : m) w.h_grp_body.push_back(b);
+        w.h_grp_off.push_back((int)w.h_grp_body.size());
+        const long long n = 12 * (long long)m.size();
+        w.h_grp_moff.push_back(w.h_grp_moff.back() + n * n);
+    }
+    w.n_groups = (int)members.size();
     // groups with several bodies first: their inverses take longest
     w.h_glist.clear();
     for (int g = 0; g < w.n_groups; g++) if (w.h_grp_off[g + 1] - w.h_grp_off[g] > 1) w.h_glist.push_back(g);

@@ -142,8 +142,8 @@ struct World {
     DBuf<double> chunk_part;                   // 456 per chunk (3 blocks + 2 gradients)
     DBuf<double> Hval;               // 144 per block
     DBuf<double> grad;               // 12 B
-    // preconditioner groups: connected components of the body graph, split in
-    // BFS order into groups of at most GMAX bodies; per group the dense inverse
+    // preconditioner groups: heavy-edge aggregates of the contact graph of at
+    // most GMAX bodies; per group the dense inverse
     // of its diagonal sub-matrix (n x n, n = 12 bodies, column-major)
     DBuf<int> grp_off, grp_body;     // group g owns grp_body[grp_off[g], grp_off[g + 1])
     DBuf<long long> grp_moff;        // offset of group g's inverse in Ginv
@@ -152,7 +152,7 @@ struct World {
     int n_groups = 0;
     DBuf<int> glist;                 // group ids in launch order: multi-body groups first
     std::vector<int> h_glist;
-    std::vector<int> h_rp, h_ci, h_grp_off, h_grp_body;
+    std::vector<int> h_grp_off, h_grp_body;
     std::vector<long long> h_grp_moff;
     int nnzb = 0;
     // ---- PCG
