@@ -276,49 +276,37 @@ __global__ __launch_bounds__(256) void k_pair_reduce(PairArgs a, const int* __re
     }
 }
 
-// off-diagonal blocks: one 64-lane workgroup per pair, its chunk blocks (lo, hi)
-// summed in order, written to (lo, hi) and transposed to (hi, lo)
-__global__ __launch_bounds__(64) void k_pair_offdiag(PairArgs a, const double* __restrict__ part) {
-    const int p = blockIdx.x;
+// off-diagonal blocks: one thread per entry of each pair's block (lo, hi):
+// the pair's chunk blocks summed in order, written to (lo, hi) and
+// transposed to (hi, lo)
+__global__ void k_pair_offdiag(PairArgs a, const double* __restrict__ part) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = idx / 144, e = idx % 144;
     if (p >= a.np) return;
-    const int t = threadIdx.x;
-    double acc[3] = {0.0, 0.0, 0.0};
-    for (int c = a.chunk_off[p]; c < a.chunk_off[p + 1]; c++) {
-        const double* P = part + PART * (size_t)c + 288;
-        for (int u = 0; u < 3; u++) { const int e = t + 64 * u; if (e < 144) acc[u] += P[e]; }
-    }
-    double* U = a.Hval + 144 * (size_t)a.pair_up[p];
-    double* L = a.Hval + 144 * (size_t)a.pair_low[p];
-    for (int u = 0; u < 3; u++) {
-        const int e = t + 64 * u;
-        if (e < 144) { U[e] = acc[u]; L[12 * (e % 12) + e / 12] = acc[u]; }
-    }
+    double acc = 0.0;
+    for (int c = a.chunk_off[p]; c < a.chunk_off[p + 1]; c++) acc += part[PART * (size_t)c + 288 + e];
+    a.Hval[144 * (size_t)a.pair_up[p] + e] = acc;
+    a.Hval[144 * (size_t)a.pair_low[p] + 12 * (e % 12) + e / 12] = acc;
 }
 
-// diagonal blocks and gradients: one 64-lane workgroup per body: inertia,
-// the (lo, lo) chunk blocks of its pairs (b, hi), the (hi, hi) chunk blocks
-// of its pairs (lo, b), each in order, then its vertex-ground terms
-__global__ __launch_bounds__(64) void k_pair_diag(PairArgs a, const double* __restrict__ part) {
-    const int b = blockIdx.x;
+// diagonal blocks and gradients: one thread per entry (144 block entries, 12
+// gradient entries) of each body: inertia, the (lo, lo) chunk blocks of its
+// pairs (b, hi), the (hi, hi) chunk blocks of its pairs (lo, b), each in
+// order, then its vertex-ground terms
+__global__ void k_pair_diag(PairArgs a, const double* __restrict__ part) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = idx / 156, e = idx % 156;
     if (b >= a.B) return;
-    const int t = threadIdx.x;
-    double acc[3], gacc = 0.0;
-    for (int u = 0; u < 3; u++) { const int e = t + 64 * u; acc[u] = e < 144 ? a.bodyH[144 * (size_t)b + e] : 0.0; }
-    if (t < 12) gacc = a.bodyG[12 * (size_t)b + t];
+    const bool isg = e >= 144;
+    const int ge = e - 144;
+    double acc = isg ? a.bodyG[12 * (size_t)b + ge] : a.bodyH[144 * (size_t)b + e];
+    const int off_up = isg ? 432 + ge : e, off_low = isg ? 444 + ge : 144 + e;
     // pairs (b, hi): their chunks are contiguous
-    for (int c = a.chunk_off[a.up_start[b]]; c < a.chunk_off[a.up_start[b + 1]]; c++) {
-        const double* P = part + PART * (size_t)c;
-        for (int u = 0; u < 3; u++) { const int e = t + 64 * u; if (e < 144) acc[u] += P[e]; }
-        if (t < 12) gacc += P[432 + t];
-    }
+    for (int c = a.chunk_off[a.up_start[b]]; c < a.chunk_off[a.up_start[b + 1]]; c++) acc += part[PART * (size_t)c + off_up];
     // pairs (lo, b)
     for (int i = a.low_start[b]; i < a.low_start[b + 1]; i++) {
         const int p = a.tv[i];
-        for (int c = a.chunk_off[p]; c < a.chunk_off[p + 1]; c++) {
-            const double* P = part + PART * (size_t)c;
-            for (int u = 0; u < 3; u++) { const int e = t + 64 * u; if (e < 144) acc[u] += P[144 + e]; }
-            if (t < 12) gacc += P[444 + t];
-        }
+        for (int c = a.chunk_off[p]; c < a.chunk_off[p + 1]; c++) acc += part[PART * (size_t)c + off_low];
     }
     if (a.mass[b] > 0.0) {
         // vertex-ground terms of this body: (w w^T) (x) K, gradient w (x) gx
@@ -326,19 +314,15 @@ __global__ __launch_bounds__(64) void k_pair_diag(PairArgs a, const double* __re
             const int v = a.voff[b] + a.vg[s].y;
             const double w4[4] = {1.0, a.xbar[3 * (size_t)v], a.xbar[3 * (size_t)v + 1], a.xbar[3 * (size_t)v + 2]};
             const double* K = a.vgK + 12 * (size_t)s;
-            for (int u = 0; u < 3; u++) {
-                const int e = t + 64 * u;
-                if (e < 144) {
-                    const int cc = e / 12, rr = e % 12;
-                    acc[u] += w4[rr / 3] * w4[cc / 3] * K[3 * (cc % 3) + (rr % 3)];
-                }
+            if (isg) acc += w4[ge / 3] * K[9 + ge % 3];
+            else {
+                const int cc = e / 12, rr = e % 12;
+                acc += w4[rr / 3] * w4[cc / 3] * K[3 * (cc % 3) + (rr % 3)];
             }
-            if (t < 12) gacc += w4[t / 3] * K[9 + t % 3];
         }
     }
-    double* H = a.Hval + 144 * (size_t)a.diag_blk[b];
-    for (int u = 0; u < 3; u++) { const int e = t + 64 * u; if (e < 144) H[e] = acc[u]; }
-    if (t < 12) a.grad[12 * (size_t)b + t] = gacc;
+    if (isg) a.grad[12 * (size_t)b + ge] = acc;
+    else a.Hval[144 * (size_t)a.diag_blk[b] + e] = acc;
 }
 
 // Preconditioner: the explicit inverse of the diagonal sub-matrix of each
@@ -621,8 +605,8 @@ void assemble_system(World& w) {
         hipLaunchKernelGGL(k_pair_reduce, dim3(max_chunks), dim3(256), 0, w.stream, a, w.chunk_blk.p, w.chunk_part.p);
         ADB_HIP(hipGetLastError());
     }
-    if (np > 0) hipLaunchKernelGGL(k_pair_offdiag, dim3(np), dim3(64), 0, w.stream, a, w.chunk_part.p);
-    hipLaunchKernelGGL(k_pair_diag, dim3(B), dim3(64), 0, w.stream, a, w.chunk_part.p);
+    if (np > 0) hipLaunchKernelGGL(k_pair_offdiag, dim3((144 * np + 255) / 256), dim3(256), 0, w.stream, a, w.chunk_part.p);
+    hipLaunchKernelGGL(k_pair_diag, dim3((156 * B + 255) / 256), dim3(256), 0, w.stream, a, w.chunk_part.p);
     ADB_HIP(hipGetLastError());
     build_groups(w);
     {

@@ -169,6 +169,7 @@ struct NarrowArgs {
     const int* voff; const int* foff; const int* eoff; const int* face; const int* edge;
     const double* X0; const double* X1;   // X1 only for CCD
     const double* box;                    // body boxes
+    const double* sph;                    // body bounding spheres (centre, radius)
     double r; double d_hat;
     int enable_pt, enable_ee;
     unsigned long long* pt_keys; unsigned long long* ee_keys;   // NBK buckets of `sub` keys each
@@ -207,11 +208,15 @@ __device__ __forceinline__ int prim_verts(const NarrowArgs& a, int kind, int id,
 }
 
 // Stage primitives [*cursor, p1) of one kind into LDS until CAP entries are
-// filled (whole rounds of NB_THREADS); boxes are inflated by `infl`.  Returns
-// the staged count; *cursor advances past the scanned primitives.
+// filled (whole rounds of NB_THREADS); boxes are inflated by `infl`.  A
+// primitive is staged when its box meets the overlap box [xlo, xhi] and comes
+// within sc[3] of the other body's bounding-sphere centre sc (that body's
+// primitives all lie in its ball, swept positions included, so nothing within
+// reach is dropped).  Returns the staged count; *cursor advances past the
+// scanned primitives.
 template <int CCD, class BT, int CAP>
 __device__ int stage_prims(const NarrowArgs& a, int kind, int* cursor, int p1, const double* xlo, const double* xhi,
-                           double infl, BT (*bx)[6], int* ids, int* cnt) {
+                           const double* sc, double infl, BT (*bx)[6], int* ids, int* cnt) {
     __syncthreads();
     if (threadIdx.x == 0) *cnt = 0;
     __syncthreads();
@@ -223,7 +228,12 @@ __device__ int stage_prims(const NarrowArgs& a, int kind, int* cursor, int p1, c
             const int nv = prim_verts(a, kind, p, v);
             double lo[3], hi[3];
             prim_box<CCD>(a.X0, a.X1, v, nv, lo, hi);
-            if (ovl(lo, hi, xlo, xhi)) {
+            double d2 = 0.0;
+            for (int i = 0; i < 3; i++) {
+                const double g = fmax(fmax(lo[i] - sc[i], sc[i] - hi[i]), 0.0);
+                d2 += g * g;
+            }
+            if (ovl(lo, hi, xlo, xhi) && d2 <= sc[3] * sc[3]) {
                 const int k = atomicAdd(cnt, 1);
                 ids[k] = p;
                 for (int i = 0; i < 3; i++) { bx[k][i] = lo_of<BT>(lo[i] - infl); bx[k][3 + i] = hi_of<BT>(hi[i] + infl); }
@@ -357,6 +367,11 @@ __global__ __launch_bounds__(NB_THREADS) void k_pair_narrow(NarrowArgs a) {
     if (threadIdx.x == 0) qn = 0;
     double tmin = 1.0;
     const double infl = CCD ? 0.0 : a.r;
+    // bounding spheres of both bodies, radii grown by the reach of a contact
+    // (1e-9 relative slack covers the rounding of the test)
+    double sx[4], sy[4];
+    for (int k = 0; k < 4; k++) { sx[k] = a.sph[4 * (size_t)pr.x + k]; sy[k] = a.sph[4 * (size_t)pr.y + k]; }
+    sx[3] = (sx[3] + infl) * (1.0 + 1e-9); sy[3] = (sy[3] + infl) * (1.0 + 1e-9);
     // task (blockIdx.y): 0 = vertices of I vs faces of J, 1 = vertices of J vs
     // faces of I, 2 + q = edges of I in slice q (of EE_SLICES) vs edges of J
     const int task = blockIdx.y;
@@ -364,6 +379,8 @@ __global__ __launch_bounds__(NB_THREADS) void k_pair_narrow(NarrowArgs a) {
     const int kind = pass < 2 ? 0 : 1;                            // 0 = PT, 1 = EE
     if (pass < 2 ? a.enable_pt : a.enable_ee) {
         const int bA = pass == 1 ? pr.y : pr.x, bB = pass == 1 ? pr.x : pr.y;
+        const double* sA = pass == 1 ? sy : sx;   // sphere of body A: filters B's primitives
+        const double* sB = pass == 1 ? sx : sy;
         const int kA = pass < 2 ? 0 : 2, kB = pass < 2 ? 1 : 2;   // primitive kinds
         const int* offA = kA == 0 ? a.voff : a.eoff;
         const int* offB = kB == 1 ? a.foff : a.eoff;
@@ -376,12 +393,12 @@ __global__ __launch_bounds__(NB_THREADS) void k_pair_narrow(NarrowArgs a) {
         }
         while (curA < endA) {
             // the A side is inflated by r (vertex or edge of body I), as the all-pairs test
-            const int nA = stage_prims<CCD, BT, CA>(a, kA, &curA, endA, xlo, xhi, infl, bxA, idA, &cntA);
+            const int nA = stage_prims<CCD, BT, CA>(a, kA, &curA, endA, xlo, xhi, sB, infl, bxA, idA, &cntA);
             if (nA == 0) continue;
             int curB = offB[bB];
             const int endB = offB[bB + 1];
             while (curB < endB) {
-                const int nB = stage_prims<CCD, BT, CB>(a, kB, &curB, endB, xlo, xhi, 0.0, bxB, idB, &cntB);
+                const int nB = stage_prims<CCD, BT, CB>(a, kB, &curB, endB, xlo, xhi, sA, 0.0, bxB, idB, &cntB);
                 const int total = nA * nB;
                 for (int t0 = 0; t0 < total; t0 += NB_THREADS * PAIRS_PT) {
                     __syncthreads();
@@ -513,7 +530,7 @@ static NarrowArgs narrow_args(World& w, int npairs, const double* X0, const doub
     NarrowArgs a;
     a.n_pairs = npairs; a.pairs = w.pairs.p;
     a.voff = w.voff.p; a.foff = w.foff.p; a.eoff = w.eoff.p; a.face = w.face.p; a.edge = w.edge.p;
-    a.X0 = X0; a.X1 = X1; a.box = w.bbox.p; a.r = r; a.d_hat = w.P.d_hat;
+    a.X0 = X0; a.X1 = X1; a.box = w.bbox.p; a.sph = w.bsph.p; a.r = r; a.d_hat = w.P.d_hat;
     a.enable_pt = w.P.enable_pt && w.F > 0;
     a.enable_ee = w.P.enable_ee && w.E > 0;
     a.pt_keys = nullptr; a.ee_keys = nullptr; a.sub = 0; a.bcnt = nullptr;
