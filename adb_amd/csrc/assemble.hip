@@ -193,10 +193,14 @@ struct PairArgs {
     double* Hval; double* grad;
 };
 
+// record doubles a thread stages per round (RSLOT records of REC_USED / 2 double2 over 256 threads)
+constexpr int STAGE_PER_T = (RSLOT * (REC_USED / 2) + 255) / 256;
 __global__ __launch_bounds__(256) void k_pair_reduce(PairArgs a, const int* __restrict__ chunk_pair, double* __restrict__ part) {
-    __shared__ __align__(16) double rec[RSLOT][REC_USED];
-    __shared__ int sinfo[RSLOT][3];           // contact id, side of lo, number of Helmert rows (0: inactive)
-    __shared__ double P[RSLOT][RLANES][13];
+    // two record buffers (the next round is loaded while this one is used);
+    // at the end the same space holds the per-slot column sums
+    constexpr int RECBUF = 2 * RSLOT * REC_USED, PSUM = RSLOT * RLANES * 13;
+    __shared__ __align__(16) double lds[RECBUF > PSUM ? RECBUF : PSUM];
+    __shared__ int sc[PCH], sside[PCH], scls[PCH];   // per contact of the chunk: id, side of lo, Helmert rows
     // XCD-aware order: workgroups are dealt to the 8 XCDs round-robin; XCD x
     // takes the contiguous chunk range [x per, (x + 1) per)
     const int per = gridDim.x / 8;
@@ -208,31 +212,51 @@ __global__ __launch_bounds__(256) void k_pair_reduce(PairArgs a, const int* __re
     const int lo = a.pair_lo[p], hi = a.pair_hi[p];
     const bool dlo = a.mass[lo] > 0.0, dhi = a.mass[hi] > 0.0;
     const int t = threadIdx.x;
+    if (t < end - beg) {
+        const int c = a.cid[beg + t];
+        const int I = c < a.n_pt ? a.pt[c].x : a.ee[c - a.n_pt].x;
+        sc[t] = c; sside[t] = I == lo ? 0 : 1; scls[t] = a.ccls[c];
+    }
+    __syncthreads();
     const int slot = t / RLANES, rem = t % RLANES, blk = rem / 12, col = rem % 12, be = col / 3, cc = col % 3;
     const bool use = slot < RSLOT && (blk == 0 ? dlo : (blk == 1 ? dhi : (dlo && dhi)));
+    // loads of round k into registers: element e = t + 256 u of the round's
+    // RSLOT x (REC_USED / 2) double2 (only the doubles its record uses)
+    double2 pre[STAGE_PER_T];
+    auto fetch = [&](int r0) {
+#pragma unroll
+        for (int u = 0; u < STAGE_PER_T; u++) {
+            const int e = t + 256 * u, sl = e / (REC_USED / 2), o = e % (REC_USED / 2), k = r0 - beg + sl;
+            pre[u] = make_double2(0.0, 0.0);
+            if (sl < RSLOT && r0 + sl < end) {
+                const int nA = scls[k];
+                if (nA > 0 && 2 * o < R_PK + 9 * nA * nA)
+                    pre[u] = reinterpret_cast<const double2*>(a.rec + REC * (size_t)sc[k])[o];
+            }
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < STAGE_PER_T; u++) {
+            const int e = t + 256 * u;
+            if (e < RSLOT * (REC_USED / 2)) reinterpret_cast<double2*>(lds + buf * RSLOT * REC_USED)[e] = pre[u];
+        }
+    };
     double acc[12], gacc = 0.0;
 #pragma unroll
     for (int i = 0; i < 12; i++) acc[i] = 0.0;
-    for (int r0 = beg; r0 < end; r0 += RSLOT) {
-        const int nr = min(RSLOT, end - r0);
-        __syncthreads();                       // the previous round's records are consumed
-        if (t < nr) {
-            const int c = a.cid[r0 + t];
-            const int I = c < a.n_pt ? a.pt[c].x : a.ee[c - a.n_pt].x;
-            sinfo[t][0] = c; sinfo[t][1] = I == lo ? 0 : 1; sinfo[t][2] = a.ccls[c];
-        }
-        __syncthreads();
-        for (int e = t; e < nr * (REC_USED / 2); e += 256) {
-            const int sl = e / (REC_USED / 2), o = e % (REC_USED / 2);
-            const int nA = sinfo[sl][2];
-            if (nA > 0 && 2 * o < R_PK + 9 * nA * nA)
-                reinterpret_cast<double2*>(rec[sl])[o] = reinterpret_cast<const double2*>(a.rec + REC * (size_t)sinfo[sl][0])[o];
-        }
-        __syncthreads();
-        if (use && slot < nr && sinfo[slot][2] > 0) {
-            const double* R = rec[slot];
-            const int nA = sinfo[slot][2], D = 3 * nA;
-            const int slo = sinfo[slot][1], shi = 1 - slo;
+    fetch(beg);
+    store(0);
+    __syncthreads();
+    int buf = 0;
+    for (int r0 = beg; r0 < end; r0 += RSLOT, buf ^= 1) {
+        const bool more = r0 + RSLOT < end;
+        if (more) fetch(r0 + RSLOT);
+        const int k = r0 - beg + slot;
+        if (use && r0 + slot < end && scls[k] > 0) {
+            const double* R = lds + (buf * RSLOT + slot) * REC_USED;
+            const int nA = scls[k], D = 3 * nA;
+            const int slo = sside[k], shi = 1 - slo;
             const int s1 = blk == 1 ? shi : slo, s2 = blk == 0 ? slo : shi;
             double tt[3][3];
 #pragma unroll
@@ -259,11 +283,14 @@ __global__ __launch_bounds__(256) void k_pair_reduce(PairArgs a, const int* __re
             }
             if (blk < 2) gacc += R[R_GRAD + 12 * s1 + col];
         }
+        if (more) store(buf ^ 1);
+        __syncthreads();
     }
+    double* P = lds;   // [RSLOT][RLANES][13]
     if (slot < RSLOT) {
 #pragma unroll
-        for (int i = 0; i < 12; i++) P[slot][rem][i] = acc[i];
-        P[slot][rem][12] = gacc;
+        for (int i = 0; i < 12; i++) P[(slot * RLANES + rem) * 13 + i] = acc[i];
+        P[(slot * RLANES + rem) * 13 + 12] = gacc;
     }
     __syncthreads();
     for (int e = t; e < PART; e += 256) {
@@ -271,7 +298,7 @@ __global__ __launch_bounds__(256) void k_pair_reduce(PairArgs a, const int* __re
         if (e < 432) { ln = 12 * (e / 144) + (e % 144) / 12; row = e % 12; }
         else { ln = e - 432; row = 12; }      // gradient of lo (lanes 0-11), of hi (lanes 12-23)
         double sum = 0.0;
-        for (int sl = 0; sl < RSLOT; sl++) sum += P[sl][ln][row];
+        for (int sl = 0; sl < RSLOT; sl++) sum += P[(sl * RLANES + ln) * 13 + row];
         part[PART * (size_t)ch + e] = sum;
     }
 }

@@ -165,7 +165,7 @@ constexpr int NARROW_TASKS = 2 + EE_SLICES;    // workgroups per body pair
 constexpr int QCAP = NB_THREADS * PAIRS_PT * 2; // candidate queue
 
 struct NarrowArgs {
-    int n_pairs; const int2* pairs;
+    const int* n_pairs; int pair_cap; const int2* pairs;   // pair count on the device (may exceed pair_cap)
     const int* voff; const int* foff; const int* eoff; const int* face; const int* edge;
     const double* X0; const double* X1;   // X1 only for CCD
     const double* box;                    // body boxes
@@ -356,69 +356,70 @@ __global__ __launch_bounds__(NB_THREADS) void k_pair_narrow(NarrowArgs a) {
         kb = &ks.kb;
         if (threadIdx.x == 0) kb->n = 0;
     }
-    const int pidx = blockIdx.x;
-    if (pidx >= a.n_pairs) return;
-    const int2 pr = a.pairs[pidx];
-    double xlo[3], xhi[3];
-    for (int k = 0; k < 3; k++) {
-        xlo[k] = fmax(a.box[6 * (size_t)pr.x + k], a.box[6 * (size_t)pr.y + k]);
-        xhi[k] = fmin(a.box[6 * (size_t)pr.x + 3 + k], a.box[6 * (size_t)pr.y + 3 + k]);
-    }
     if (threadIdx.x == 0) qn = 0;
     double tmin = 1.0;
     const double infl = CCD ? 0.0 : a.r;
-    // bounding spheres of both bodies, radii grown by the reach of a contact
-    // (1e-9 relative slack covers the rounding of the test)
-    double sx[4], sy[4];
-    for (int k = 0; k < 4; k++) { sx[k] = a.sph[4 * (size_t)pr.x + k]; sy[k] = a.sph[4 * (size_t)pr.y + k]; }
-    sx[3] = (sx[3] + infl) * (1.0 + 1e-9); sy[3] = (sy[3] + infl) * (1.0 + 1e-9);
     // task (blockIdx.y): 0 = vertices of I vs faces of J, 1 = vertices of J vs
     // faces of I, 2 + q = edges of I in slice q (of EE_SLICES) vs edges of J
     const int task = blockIdx.y;
     const int pass = task < 2 ? task : 2;
     const int kind = pass < 2 ? 0 : 1;                            // 0 = PT, 1 = EE
-    if (pass < 2 ? a.enable_pt : a.enable_ee) {
-        const int bA = pass == 1 ? pr.y : pr.x, bB = pass == 1 ? pr.x : pr.y;
-        const double* sA = pass == 1 ? sy : sx;   // sphere of body A: filters B's primitives
-        const double* sB = pass == 1 ? sx : sy;
-        const int kA = pass < 2 ? 0 : 2, kB = pass < 2 ? 1 : 2;   // primitive kinds
-        const int* offA = kA == 0 ? a.voff : a.eoff;
-        const int* offB = kB == 1 ? a.foff : a.eoff;
-        int curA = offA[bA];
-        int endA = offA[bA + 1];
-        if (pass == 2) {
-            const int len = endA - curA, q = task - 2;
-            const int s0 = curA + (int)((long long)len * q / EE_SLICES), s1 = curA + (int)((long long)len * (q + 1) / EE_SLICES);
-            curA = s0; endA = s1;
+    const int np = min(*a.n_pairs, a.pair_cap);
+    for (int pidx = blockIdx.x; pidx < np; pidx += gridDim.x) {
+        const int2 pr = a.pairs[pidx];
+        double xlo[3], xhi[3];
+        for (int k = 0; k < 3; k++) {
+            xlo[k] = fmax(a.box[6 * (size_t)pr.x + k], a.box[6 * (size_t)pr.y + k]);
+            xhi[k] = fmin(a.box[6 * (size_t)pr.x + 3 + k], a.box[6 * (size_t)pr.y + 3 + k]);
         }
-        while (curA < endA) {
-            // the A side is inflated by r (vertex or edge of body I), as the all-pairs test
-            const int nA = stage_prims<CCD, BT, CA>(a, kA, &curA, endA, xlo, xhi, sB, infl, bxA, idA, &cntA);
-            if (nA == 0) continue;
-            int curB = offB[bB];
-            const int endB = offB[bB + 1];
-            while (curB < endB) {
-                const int nB = stage_prims<CCD, BT, CB>(a, kB, &curB, endB, xlo, xhi, sA, 0.0, bxB, idB, &cntB);
-                const int total = nA * nB;
-                for (int t0 = 0; t0 < total; t0 += NB_THREADS * PAIRS_PT) {
-                    __syncthreads();
-                    if (qn + NB_THREADS * PAIRS_PT > QCAP) flush_candidates<CCD>(a, kind, idA, idB, q, &qn, tmin, kb);
-#pragma unroll
-                    for (int u = 0; u < PAIRS_PT; u++) {
-                        const int t = t0 + u * NB_THREADS + threadIdx.x;
-                        if (t < total) {
-                            const int i = t / nB, j = t % nB;
-                            const bool o = bxA[i][0] <= bxB[j][3] && bxB[j][0] <= bxA[i][3] &&
-                                           bxA[i][1] <= bxB[j][4] && bxB[j][1] <= bxA[i][4] &&
-                                           bxA[i][2] <= bxB[j][5] && bxB[j][2] <= bxA[i][5];
-                            if (o) { const int s = atomicAdd(&qn, 1); q[s] = make_int2(i, j); }
+        // bounding spheres of both bodies, radii grown by the reach of a contact
+        // (1e-9 relative slack covers the rounding of the test)
+        double sx[4], sy[4];
+        for (int k = 0; k < 4; k++) { sx[k] = a.sph[4 * (size_t)pr.x + k]; sy[k] = a.sph[4 * (size_t)pr.y + k]; }
+        sx[3] = (sx[3] + infl) * (1.0 + 1e-9); sy[3] = (sy[3] + infl) * (1.0 + 1e-9);
+        if (pass < 2 ? a.enable_pt : a.enable_ee) {
+            const int bA = pass == 1 ? pr.y : pr.x, bB = pass == 1 ? pr.x : pr.y;
+            const double* sA = pass == 1 ? sy : sx;   // sphere of body A: filters B's primitives
+            const double* sB = pass == 1 ? sx : sy;
+            const int kA = pass < 2 ? 0 : 2, kB = pass < 2 ? 1 : 2;   // primitive kinds
+            const int* offA = kA == 0 ? a.voff : a.eoff;
+            const int* offB = kB == 1 ? a.foff : a.eoff;
+            int curA = offA[bA];
+            int endA = offA[bA + 1];
+            if (pass == 2) {
+                const int len = endA - curA, q = task - 2;
+                const int s0 = curA + (int)((long long)len * q / EE_SLICES), s1 = curA + (int)((long long)len * (q + 1) / EE_SLICES);
+                curA = s0; endA = s1;
+            }
+            while (curA < endA) {
+                // the A side is inflated by r (vertex or edge of body I), as the all-pairs test
+                const int nA = stage_prims<CCD, BT, CA>(a, kA, &curA, endA, xlo, xhi, sB, infl, bxA, idA, &cntA);
+                if (nA == 0) continue;
+                int curB = offB[bB];
+                const int endB = offB[bB + 1];
+                while (curB < endB) {
+                    const int nB = stage_prims<CCD, BT, CB>(a, kB, &curB, endB, xlo, xhi, sA, 0.0, bxB, idB, &cntB);
+                    const int total = nA * nB;
+                    for (int t0 = 0; t0 < total; t0 += NB_THREADS * PAIRS_PT) {
+                        __syncthreads();
+                        if (qn + NB_THREADS * PAIRS_PT > QCAP) flush_candidates<CCD>(a, kind, idA, idB, q, &qn, tmin, kb);
+    #pragma unroll
+                        for (int u = 0; u < PAIRS_PT; u++) {
+                            const int t = t0 + u * NB_THREADS + threadIdx.x;
+                            if (t < total) {
+                                const int i = t / nB, j = t % nB;
+                                const bool o = bxA[i][0] <= bxB[j][3] && bxB[j][0] <= bxA[i][3] &&
+                                               bxA[i][1] <= bxB[j][4] && bxB[j][1] <= bxA[i][4] &&
+                                               bxA[i][2] <= bxB[j][5] && bxB[j][2] <= bxA[i][5];
+                                if (o) { const int s = atomicAdd(&qn, 1); q[s] = make_int2(i, j); }
+                            }
                         }
                     }
+                    flush_candidates<CCD>(a, kind, idA, idB, q, &qn, tmin, kb);
                 }
-                flush_candidates<CCD>(a, kind, idA, idB, q, &qn, tmin, kb);
             }
         }
-    }
+    }   // pairs
     if constexpr (!CCD) flush_keys(a, kind, *kb);
     if (CCD) {
         for (int m = 32; m >= 1; m >>= 1) tmin = fmin(tmin, __shfl_xor(tmin, m));
@@ -497,8 +498,9 @@ __global__ void k_gather_buckets(BucketList L, int sub, const unsigned long long
 
 static int bits_for(long long n) { int b = 1; while ((1ll << b) <= n) b++; return b; }
 
-// body boxes + pairs; returns the pair count
-static int body_pairs(World& w, const double* X0, const double* X1, double infl) {
+// body boxes + pairs; the pair count stays on the device (*cnt, which may
+// exceed the capacity: the caller checks it with its other counters)
+static void body_pairs(World& w, const double* X0, const double* X1, double infl, int* cnt) {
     w.bbox.reserve(6 * (size_t)w.B);
     w.bsph.reserve(4 * (size_t)w.B);
     {
@@ -506,29 +508,25 @@ static int body_pairs(World& w, const double* X0, const double* X1, double infl)
         hipLaunchKernelGGL(k_body_box, dim3(w.B), dim3(NB_THREADS), 0, w.stream, w.voff.p, X0, X1, infl, w.bbox.p, w.bsph.p);
     }
     ADB_HIP(hipGetLastError());
-    int npairs = 0;
-    for (int attempt = 0; attempt < 2; attempt++) {
-        size_t cap = std::max<size_t>(w.pairs.cap, 4096);
-        w.pairs.reserve(cap);
-        ADB_HIP(hipMemsetAsync(w.counters.p + 8, 0, sizeof(int), w.stream));
-        dim3 grid((w.B + 15) / 16, (w.B + 15) / 16), blk(16, 16);
-        {
-            ProfScope ps(w, PK_BODY_PAIRS, 48.0 * w.B);
-            hipLaunchKernelGGL(k_body_pairs, grid, blk, 0, w.stream, w.B, w.bbox.p, w.bsph.p, infl, w.pairs.p, (int)w.pairs.cap,
-                               w.counters.p + 8);
-        }
-        ADB_HIP(hipGetLastError());
-        ADB_HIP(hipMemcpyAsync(&npairs, w.counters.p + 8, sizeof(int), hipMemcpyDeviceToHost, w.stream));
-        ADB_HIP(hipStreamSynchronize(w.stream));
-        if ((size_t)npairs <= w.pairs.cap) break;
-        w.pairs.reserve(npairs);
+    w.pairs.reserve(std::max<size_t>(w.pairs.cap, std::max(4096, 16 * w.B)));
+    ADB_HIP(hipMemsetAsync(cnt, 0, sizeof(int), w.stream));
+    dim3 grid((w.B + 15) / 16, (w.B + 15) / 16), blk(16, 16);
+    {
+        ProfScope ps(w, PK_BODY_PAIRS, 48.0 * w.B);
+        hipLaunchKernelGGL(k_body_pairs, grid, blk, 0, w.stream, w.B, w.bbox.p, w.bsph.p, infl, w.pairs.p, (int)w.pairs.cap, cnt);
     }
-    return npairs;
+    ADB_HIP(hipGetLastError());
 }
 
-static NarrowArgs narrow_args(World& w, int npairs, const double* X0, const double* X1, double r) {
+// workgroups per task of a narrow-phase launch (each loops over the pairs):
+// about the pair count of the previous pass, so most take one pair
+static int narrow_grid(World& w) {
+    return std::max(1, std::min<int>((int)w.pairs.cap, w.pair_hint + w.pair_hint / 8 + 64));
+}
+
+static NarrowArgs narrow_args(World& w, const int* npairs, const double* X0, const double* X1, double r) {
     NarrowArgs a;
-    a.n_pairs = npairs; a.pairs = w.pairs.p;
+    a.n_pairs = npairs; a.pair_cap = (int)w.pairs.cap; a.pairs = w.pairs.p;
     a.voff = w.voff.p; a.foff = w.foff.p; a.eoff = w.eoff.p; a.face = w.face.p; a.edge = w.edge.p;
     a.X0 = X0; a.X1 = X1; a.box = w.bbox.p; a.sph = w.bsph.p; a.r = r; a.d_hat = w.P.d_hat;
     a.enable_pt = w.P.enable_pt && w.F > 0;
@@ -541,8 +539,8 @@ static NarrowArgs narrow_args(World& w, int npairs, const double* X0, const doub
 void collision_set(World& w, const double* X, ContactSet& out) {
     const double r = sqrt(w.P.d_hat) * (1.0 + 1e-6) + 1e-12;
     const bool hashed = !(w.P.iaabb & 1);    // spatial hashing (hash.hip) instead of body overlaps
-    int npairs = hashed ? 0 : body_pairs(w, X, nullptr, r);
-    w.bcnt.reserve(2 * NBK + 1);             // PT buckets, EE buckets, VG counter
+    w.bcnt.reserve(2 * NBK + 2);             // PT buckets, EE buckets, VG counter, body-pair counter
+    int* pcnt = w.bcnt.p + 2 * NBK + 1;
     for (int attempt = 0; attempt < 4; attempt++) {
         const size_t sub = std::max<size_t>(w.keys_a.cap / (2 * NBK), 2048);
         const size_t pcap = sub * NBK;       // keys_a: PT buckets in [0, pcap), EE buckets in [pcap, 2 pcap)
@@ -550,14 +548,15 @@ void collision_set(World& w, const double* X, ContactSet& out) {
         w.keys_b.reserve(2 * pcap);
         const size_t vcap = w.V + 64;
         w.vg_keys.reserve(vcap);
-        ADB_HIP(hipMemsetAsync(w.bcnt.p, 0, (2 * NBK + 1) * sizeof(int), w.stream));
-        NarrowArgs a = narrow_args(w, npairs, X, nullptr, r);
+        ADB_HIP(hipMemsetAsync(w.bcnt.p, 0, (2 * NBK + 2) * sizeof(int), w.stream));
+        if (!hashed) body_pairs(w, X, nullptr, r, pcnt);
+        NarrowArgs a = narrow_args(w, pcnt, X, nullptr, r);
         a.pt_keys = w.keys_a.p; a.ee_keys = w.keys_a.p + pcap; a.sub = (int)sub; a.bcnt = w.bcnt.p;
         if (hashed) {
             hash_sets(w, X, r, a.pt_keys, a.ee_keys, (int)sub, w.bcnt.p);
-        } else if (npairs > 0) {
+        } else {
             ProfScope ps(w, PK_NARROW_SET, 0.0);
-            const dim3 grid(npairs, NARROW_TASKS);
+            const dim3 grid(narrow_grid(w), NARROW_TASKS);
             if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<0, float>), grid, dim3(NB_THREADS), 0, w.stream, a);
             else hipLaunchKernelGGL((k_pair_narrow<0, double>), grid, dim3(NB_THREADS), 0, w.stream, a);
             ADB_HIP(hipGetLastError());
@@ -568,9 +567,14 @@ void collision_set(World& w, const double* X, ContactSet& out) {
                                w.vg_keys.p, (int)vcap, w.bcnt.p + 2 * NBK);
             ADB_HIP(hipGetLastError());
         }
-        int h[2 * NBK + 1];
+        int h[2 * NBK + 2];
         ADB_HIP(hipMemcpyAsync(h, w.bcnt.p, sizeof h, hipMemcpyDeviceToHost, w.stream));
         ADB_HIP(hipStreamSynchronize(w.stream));
+        if (!hashed) w.pair_hint = h[2 * NBK + 1];
+        if (!hashed && (size_t)h[2 * NBK + 1] > w.pairs.cap) {   // more overlapping body pairs than slots
+            w.pairs.reserve(h[2 * NBK + 1]);
+            continue;
+        }
         const int nvg = w.P.enable_ground ? h[2 * NBK] : 0;
         int most = 0;
         for (int i = 0; i < 2 * NBK; i++) most = std::max(most, h[i]);
@@ -627,21 +631,30 @@ double ccd_toi(World& w, const double* X0, const double* X1, int* status) {
         ADB_HIP(hipGetLastError());
     }
     const bool hashed = !(w.P.iaabb & 2);    // spatial hashing (hash.hip) instead of body overlaps
-    int npairs = hashed ? 0 : body_pairs(w, X0, X1, 0.0);
-    NarrowArgs a = narrow_args(w, npairs, X0, X1, 0.0);
-    a.toi_bits = w.toi_bits.p;
-    if (hashed) {
-        hash_ccd(w, X0, X1, w.toi_bits.p);
-    } else if (npairs > 0) {
-        ProfScope ps(w, PK_NARROW_CCD, 0.0);
-        const dim3 grid(npairs, NARROW_TASKS);
-        if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<1, float>), grid, dim3(NB_THREADS), 0, w.stream, a);
-        else hipLaunchKernelGGL((k_pair_narrow<1, double>), grid, dim3(NB_THREADS), 0, w.stream, a);
-        ADB_HIP(hipGetLastError());
-    }
+    w.bcnt.reserve(2 * NBK + 2);
+    int* pcnt = w.bcnt.p + 2 * NBK + 1;
     unsigned long long bits[2];
-    ADB_HIP(hipMemcpyAsync(bits, w.toi_bits.p, sizeof bits, hipMemcpyDeviceToHost, w.stream));
-    ADB_HIP(hipStreamSynchronize(w.stream));
+    for (int attempt = 0; attempt < 2; attempt++) {
+        int np = 0;
+        if (!hashed) body_pairs(w, X0, X1, 0.0, pcnt);
+        NarrowArgs a = narrow_args(w, pcnt, X0, X1, 0.0);
+        a.toi_bits = w.toi_bits.p;
+        if (hashed) {
+            hash_ccd(w, X0, X1, w.toi_bits.p);
+        } else {
+            ProfScope ps(w, PK_NARROW_CCD, 0.0);
+            const dim3 grid(narrow_grid(w), NARROW_TASKS);
+            if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<1, float>), grid, dim3(NB_THREADS), 0, w.stream, a);
+            else hipLaunchKernelGGL((k_pair_narrow<1, double>), grid, dim3(NB_THREADS), 0, w.stream, a);
+            ADB_HIP(hipGetLastError());
+            ADB_HIP(hipMemcpyAsync(&np, pcnt, sizeof(int), hipMemcpyDeviceToHost, w.stream));
+        }
+        ADB_HIP(hipMemcpyAsync(bits, w.toi_bits.p, sizeof bits, hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipStreamSynchronize(w.stream));
+        if (!hashed) w.pair_hint = np;
+        if ((size_t)np <= w.pairs.cap) break;
+        w.pairs.reserve(np);               // more overlapping body pairs than slots: grow, test again
+    }
     const double tg = order_key_inv(bits[0]), tp = order_key_inv(bits[1]);
     // toi < 1e-6 twice in a row is a stall (model/iaabb.cpp:305-314,532-540)
     if (tg < 1e-6) { if (++w.stall_vg > 1) *status = ADB_E_TOI_STALL; } else w.stall_vg = 0;

@@ -150,25 +150,32 @@ __global__ __launch_bounds__(RT) void k_sum2(int G, const double* __restrict__ p
     if (threadIdx.x == 0) out[0] = sh[0];
 }
 
-static double reduce(World& w, const double* a, const double* b, int n) {
-    if (n <= 0) return 0.0;
+// Sum of a[i] (b == null) or a[i] b[i] into w.epart[slot] (slots 0..7), on
+// the world stream without waiting; read_slots fetches the results.
+constexpr int NSLOT = 8;
+static void reduce_async(World& w, const double* a, const double* b, int n, int slot) {
     int G = (n + RT - 1) / RT;
     if (G > 1024) G = 1024;
-    w.epart.reserve(G + 2);
-    hipLaunchKernelGGL(k_sum1, dim3(G), dim3(RT), 0, w.stream, n, G, a, b, w.epart.p + 2);
-    hipLaunchKernelGGL(k_sum2, dim3(1), dim3(RT), 0, w.stream, G, w.epart.p + 2, w.epart.p);
+    w.epart.reserve(NSLOT + std::max(G, 1));
+    if (n <= 0) {
+        ADB_HIP(hipMemsetAsync(w.epart.p + slot, 0, sizeof(double), w.stream));
+        return;
+    }
+    hipLaunchKernelGGL(k_sum1, dim3(G), dim3(RT), 0, w.stream, n, G, a, b, w.epart.p + NSLOT);
+    hipLaunchKernelGGL(k_sum2, dim3(1), dim3(RT), 0, w.stream, G, w.epart.p + NSLOT, w.epart.p + slot);
     ADB_HIP(hipGetLastError());
-    double h;
-    ADB_HIP(hipMemcpyAsync(&h, w.epart.p, sizeof(double), hipMemcpyDeviceToHost, w.stream));
-    ADB_HIP(hipStreamSynchronize(w.stream));
-    return h;
 }
 
-double dot_dev(World& w, const double* a, const double* b, int n) { return reduce(w, a, b, n); }
+void read_slots(World& w, int k, double* out) {
+    ADB_HIP(hipMemcpyAsync(out, w.epart.p, k * sizeof(double), hipMemcpyDeviceToHost, w.stream));
+    ADB_HIP(hipStreamSynchronize(w.stream));
+}
+
+void dot_async(World& w, const double* a, const double* b, int n, int slot) { reduce_async(w, a, b, n, slot); }
 
 static DBuf<double>& items(World& w) { return w.pcg_Ap; }   // scratch reused outside the solve
 
-double energy_barrier_inertia(World& w, const double* qv, const double* X, const ContactSet& cs) {
+void energy_barrier_inertia_async(World& w, const double* qv, const double* X, const ContactSet& cs, int slot) {
     const int n = w.B + cs.n_pt + cs.n_ee + cs.n_vg;
     DBuf<double>& buf = items(w);
     buf.reserve(std::max(n, 12 * w.B));
@@ -184,13 +191,13 @@ double energy_barrier_inertia(World& w, const double* qv, const double* X, const
     ProfScope ps(w, PK_ENERGY, 0.0);
     hipLaunchKernelGGL(k_energy_items, dim3((n + 255) / 256), dim3(256), 0, w.stream, a);
     ADB_HIP(hipGetLastError());
-    return reduce(w, buf.p, nullptr, n);
+    reduce_async(w, buf.p, nullptr, n, slot);
 }
 
-double energy_friction(World& w, const double* X) {
+void energy_friction_async(World& w, const double* X, int slot) {
     const ContactSet& cs = w.cs;
     const int n = cs.n_pt + cs.n_ee + cs.n_vg;
-    if (n == 0) return 0.0;
+    if (n == 0) { reduce_async(w, nullptr, nullptr, 0, slot); return; }
     DBuf<double>& buf = items(w);
     buf.reserve(std::max(n, 12 * w.B));
     FricArgs a;
@@ -204,7 +211,7 @@ double energy_friction(World& w, const double* X) {
     a.out = buf.p;
     hipLaunchKernelGGL(k_fric_items, dim3((n + 255) / 256), dim3(256), 0, w.stream, a);
     ADB_HIP(hipGetLastError());
-    return reduce(w, buf.p, nullptr, n);
+    reduce_async(w, buf.p, nullptr, n, slot);
 }
 
 } // namespace adb

@@ -100,6 +100,7 @@ struct World {
     DBuf<double> bbox;               // 6 per body
     DBuf<double> bsph;               // 4 per body: bounding sphere centre, radius
     DBuf<int2> pairs;                // body pairs (I < J)
+    int pair_hint = 0;               // body-pair count of the last culling pass (narrow-phase grid size)
     DBuf<int> counters;              // device counters
     DBuf<int> bcnt;                  // contact-set key bucket counters (2 x 64) + VG counter
     DBuf<unsigned long long> keys_a, keys_b;  // unsorted / sorted keys
@@ -211,9 +212,11 @@ void hash_ccd(World& w, const double* X0, const double* X1, unsigned long long* 
 void contact_terms(World& w);                                                            // contact.hip
 void assemble_system(World& w);                                                          // assemble.hip
 int pcg_solve(World& w, double* dq_out, int* iters);                                     // pcg.hip
-double energy_barrier_inertia(World& w, const double* qv, const double* X, const ContactSet& cs); // energy.hip
-double energy_friction(World& w, const double* X);                                       // energy.hip
-double dot_dev(World& w, const double* a, const double* b, int n);                       // energy.hip
+// reductions into result slots 0..7 on the world stream; read_slots waits and copies slots [0, k)
+void energy_barrier_inertia_async(World& w, const double* qv, const double* X, const ContactSet& cs, int slot); // energy.hip
+void energy_friction_async(World& w, const double* X, int slot);                        // energy.hip
+void dot_async(World& w, const double* a, const double* b, int n, int slot);            // energy.hip
+void read_slots(World& w, int k, double* out);                                          // energy.hip
 void download_sets(World& w, const ContactSet& cs, std::vector<int>& pt, std::vector<int>& ee, std::vector<int>& vg);
 
 } // namespace adb

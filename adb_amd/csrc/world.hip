@@ -64,7 +64,13 @@ static int step(World& w, adb_step_stats* st) {
         int pit = 0;
         const int prc = pcg_solve(w, w.dq.p, &pit);
         S.pcg_iters += pit;
-        const double nrm2 = prc == ADB_E_NAN ? NAN : dot_dev(w, w.dq.p, w.dq.p, n);
+        // |dq|^2 and |g|^2 with one wait
+        double sl[3];
+        dot_async(w, w.dq.p, w.dq.p, n, 0);
+        dot_async(w, w.grad.p, w.grad.p, n, 1);
+        read_slots(w, 2, sl);
+        const double nrm2 = prc == ADB_E_NAN ? NAN : sl[0];
+        const double gn = std::sqrt(sl[1]);
         const double sup_dq = std::sqrt(nrm2);
         const double t2 = now_s();
         S.t_solver += t2 - t1;
@@ -86,15 +92,21 @@ static int step(World& w, adb_step_stats* st) {
         // Armijo backtracking (model/line_search.cpp:39-136)
         double alpha = 1.0;
         if (w.P.line_search) {
-            const double E0 = energy_barrier_inertia(w, w.q.p, w.X.p, w.cs) + energy_friction(w, w.X.p);
-            const double qdg = dot_dev(w, w.dq.p, w.grad.p, n);
-            const double gn = std::sqrt(dot_dev(w, w.grad.p, w.grad.p, n));
+            energy_barrier_inertia_async(w, w.q.p, w.X.p, w.cs, 0);
+            energy_friction_async(w, w.X.p, 1);
+            dot_async(w, w.dq.p, w.grad.p, n, 2);
+            read_slots(w, 3, sl);
+            const double E0 = sl[0] + sl[1];
+            const double qdg = sl[2];
             int trials = 0;
             while (true) {
                 q_axpy(w, w.q.p, w.dq.p, alpha, w.qtrial.p);
                 transform_positions(w, w.qtrial.p, w.X1.p);
                 collision_set(w, w.X1.p, w.cs_new);
-                const double E1 = energy_barrier_inertia(w, w.qtrial.p, w.X1.p, w.cs_new) + energy_friction(w, w.X1.p);
+                energy_barrier_inertia_async(w, w.qtrial.p, w.X1.p, w.cs_new, 0);
+                energy_friction_async(w, w.X1.p, 1);
+                read_slots(w, 2, sl);
+                const double E1 = sl[0] + sl[1];
                 const bool wolfe = E1 <= E0 + w.P.c1 * alpha * qdg;
                 alpha /= 2;
                 trials++;

@@ -120,35 +120,42 @@ def max_vertex_deviation(sc, qa, qb):
     return float(np.sqrt((dx * dx).sum(1)).max())
 
 
-KERNEL_OF_SCOPE = {"asm_reduce": "k_reduce_chunks", "narrow_set": "k_pair_narrow<0", "block_chol": "k_group_inv",
-                   "contact_group": "k_contact_c3", "contact": "k_contact_small<1>", "spmv": "k_pcg_spmv",
-                   "pcg_update": "k_pcg_update", "contact_classify": "k_contact_classify"}
+# kernels launched once per launch of each profiling scope
+KERNEL_OF_SCOPE = {"asm_reduce": ["k_pair_reduce"], "narrow_set": ["k_pair_narrow<0"], "block_chol": ["k_group_inv"],
+                   "contact_group": ["k_contact_c3"],
+                   "contact": ["k_contact_small<1>", "k_contact_scalars2", "k_contact_pk2"],
+                   "spmv": ["k_pcg_spmv"], "pcg_update": ["k_pcg_update"], "contact_classify": ["k_contact_classify"]}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hbm_traffic_2000_icospheres.json")
 
 
 def pmc_traffic(fetch_csv, write_csv, prof_json, scope, copy_mb):
-    """HBM bytes per launch of one profiled kernel from two rocprofv3 --pmc runs
-    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch) of
-    `bench.py --steps 10 --warmup 2 --no-cpu-baseline --no-compare --copy-mb M
-    --profile-json P`.  Each counter is scaled by what it reads for the plain
-    copy of M MB that the run starts with (the largest copyBuffer dispatch);
-    the dispatches used are the kernel's last N, N = the scope's launches in
-    the profiled steps (from P, whose byte counts give the algorithmic bytes)."""
+    """HBM bytes per launch of one profiling scope from two rocprofv3 --pmc runs
+    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch, counters collected for the
+    scope's kernels and the calibration kernel only) of
+    `bench.py --no-compare --no-cpu-baseline --copy-mb M --profile-json P`.
+    Each counter is scaled by what it reads for the plain device copy of M MB
+    that the run starts with (the largest elementwise dispatch: M MB read, M
+    MB written); for each kernel of the scope the dispatches used are its last
+    N, N = the scope's launches in the profiled steps (from P, whose byte
+    counts give the algorithmic bytes)."""
     import csv
     with open(prof_json) as fh:
         prof = json.load(fh)
     n = prof[scope]["launches"]
-    kname = KERNEL_OF_SCOPE[scope]
-    out = {"kernel_scope": scope, "kernel": kname, "launches": n, "copy_bytes": copy_mb * 1e6,
+    kernels = KERNEL_OF_SCOPE[scope]
+    out = {"kernel_scope": scope, "kernels": kernels, "launches": n, "copy_bytes": copy_mb * 1e6,
            "algorithmic_bytes_per_launch": prof[scope]["bytes"] / n}
     for tag, path in (("fetch", fetch_csv), ("write", write_csv)):
         rows = list(csv.DictReader(open(path)))
-        copies = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]]
-        scale = copy_mb * 1e6 / (1024.0 * max(copies))
-        mine = [float(r["Counter_Value"]) for r in rows if kname in r["Kernel_Name"]][-n:]
-        out[tag + "_kib_raw_avg"] = sum(mine) / len(mine)
+        cal = [float(r["Counter_Value"]) for r in rows if "elementwise" in r["Kernel_Name"] or "copyBuffer" in r["Kernel_Name"]]
+        scale = copy_mb * 1e6 / (1024.0 * max(cal))
+        kib = 0.0
+        for k in kernels:
+            mine = [float(r["Counter_Value"]) for r in rows if k in r["Kernel_Name"]][-n:]
+            kib += sum(mine) / len(mine)
+        out[tag + "_kib_raw_avg"] = kib
         out[tag + "_scale"] = scale
-        out[tag + "_bytes"] = 1024.0 * scale * sum(mine) / len(mine)
+        out[tag + "_bytes"] = 1024.0 * scale * kib
     out["traffic_bytes_per_launch"] = out["fetch_bytes"] + out["write_bytes"]
     return out
 
@@ -186,8 +193,8 @@ def main():
                     help="steps after the timed ones with per-kernel HIP-event timing (roofline figures)")
     ap.add_argument("--profile-json", default="")
     ap.add_argument("--copy-mb", type=float, default=0.0,
-                    help="first run one plain device-to-device copy of this many MB (a known-size kernel "
-                         "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
+                    help="first run one elementwise device kernel that reads and writes this many MB (a "
+                         "known-size kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--pmc-traffic", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "PROFILE_JSON", "SCOPE"),
                     help="no run: summarize two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) into HBM bytes per "
                          "launch of one kernel scope, scaled by the --copy-mb copy; writes --out")
@@ -219,7 +226,7 @@ def main():
         n = int(args.copy_mb * 1e6 / 8)
         src = torch.ones(n, dtype=torch.float64, device="cuda:0")
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.add(src, 1.0, out=dst)      # reads and writes copy_mb MB
         torch.cuda.synchronize()
         del src, dst
     sc = workload(args.bodies)
