@@ -153,46 +153,63 @@ __global__ __launch_bounds__(64 * SP_ROWS) void k_pcg_spmv(int B, const int* __r
     if (threadIdx.x == 0) partA[blockIdx.x] = prod;
 }
 
-__global__ __launch_bounds__(64) void k_pcg_update(int B, const int* __restrict__ kbase, int kloc, int nbS, Groups G,
-                                                   const double* __restrict__ p, const double* __restrict__ y,
-                                                   double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
-                                                   const double* __restrict__ partA, double* __restrict__ partB,
-                                                   double* __restrict__ scal) {
+// UPD_T lanes per group: row i = t % GN of the group, column half t / GN of
+// z = M^-1 r, the two halves combined in order
+constexpr int UPD_T = 2 * GN;
+__global__ __launch_bounds__(UPD_T) void k_pcg_update(int B, const int* __restrict__ kbase, int kloc, int nbS, Groups G,
+                                                      const double* __restrict__ p, const double* __restrict__ y,
+                                                      double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
+                                                      const double* __restrict__ partA, double* __restrict__ partB,
+                                                      double* __restrict__ scal) {
     __shared__ double rs[GN];
+    __shared__ double half2[GN];
+    __shared__ double red[UPD_T / 64][2];
     __shared__ int bod[GN / 12];
     if (scal[S_STOP] != RUN) return;
-    const int gi = blockIdx.x, l = threadIdx.x;
+    const int gi = blockIdx.x, t = threadIdx.x;
+    if (gi > 0 && gi >= *G.count) return;     // workgroup 0 also reports a breakdown
     const int iter = *kbase + kloc;
     const double pAp = wave_sum_array(partA, nbS);
     const double RZ = scal[S_RZ0 + (iter & 1)];
     if (!(pAp > 0.0) || !std::isfinite(pAp)) {
-        if (gi == 0 && l == 0) { scal[S_STOP] = std::isfinite(pAp) ? BREAKDOWN : NONFINITE; scal[S_ITERS] = iter; }
+        if (gi == 0 && t == 0) { scal[S_STOP] = std::isfinite(pAp) ? BREAKDOWN : NONFINITE; scal[S_ITERS] = iter; }
         return;
     }
     if (gi >= *G.count) return;
     const double alpha = RZ / pAp;
     const int m = G.off[gi + 1] - G.off[gi], n = 12 * m;
-    if (l < m) bod[l] = G.body[G.off[gi] + l];
+    if (t < m) bod[t] = G.body[G.off[gi] + t];
     __syncthreads();
-    for (int i = l; i < n; i += 64) {
-        const size_t k = 12 * (size_t)bod[i / 12] + i % 12;
+    if (t < n) {
+        const size_t k = 12 * (size_t)bod[t / 12] + t % 12;
         x[k] += alpha * p[k];
         const double rk = r[k] - alpha * y[k];
         r[k] = rk;
-        rs[i] = rk;
+        rs[t] = rk;
     }
     __syncthreads();
     const double* M = G.Minv + G.moff[gi];
+    const int i = t % GN, h = t / GN, j0 = h * ((n + 1) / 2), j1 = h ? n : (n + 1) / 2;
+    double zi = 0.0;
+    if (i < n)
+        for (int j = j0; j < j1; j++) zi += M[(size_t)j * n + i] * rs[j];
+    if (h == 1 && i < n) half2[i] = zi;
+    __syncthreads();
     double rz = 0.0, rr = 0.0;
-    for (int i = l; i < n; i += 64) {
-        double zi = 0.0;
-        for (int j = 0; j < n; j++) zi += M[(size_t)j * n + i] * rs[j];
+    if (h == 0 && i < n) {
+        zi += half2[i];
         z[12 * (size_t)bod[i / 12] + i % 12] = zi;
-        rz += rs[i] * zi; rr += rs[i] * rs[i];
+        rz = rs[i] * zi; rr = rs[i] * rs[i];
     }
     rz = wave_sum(rz);
     rr = wave_sum(rr);
-    if (l == 0) { partB[gi] = rz; partB[B + gi] = rr; }
+    if (t % 64 == 0) { red[t / 64][0] = rz; red[t / 64][1] = rr; }
+    __syncthreads();
+    if (t == 0) {
+        double a0 = 0.0, a1 = 0.0;
+        for (int w = 0; w < UPD_T / 64; w++) { a0 += red[w][0]; a1 += red[w][1]; }
+        partB[gi] = a0; partB[B + gi] = a1;
+    }
 }
 
 static Groups groups_of(World& w) {
@@ -223,7 +240,7 @@ static void enqueue_batch(World& w, int nb, bool prof) {
         {
             // group inverses read; p, y read; x, r read and written; z written
             ProfScope ps(w, PK_PCG_UPDATE, prof ? 8.0 * (double)w.h_grp_moff.back() + 7.0 * 96.0 * B : 0.0);
-            hipLaunchKernelGGL(k_pcg_update, dim3(B), dim3(64), 0, w.stream, B, w.pcg_k.p, k, nbS, G, pb[(k & 1) ^ 1],
+            hipLaunchKernelGGL(k_pcg_update, dim3(B), dim3(UPD_T), 0, w.stream, B, w.pcg_k.p, k, nbS, G, pb[(k & 1) ^ 1],
                                w.pcg_Ap.p, w.pcg_x.p, w.pcg_r.p, w.pcg_z.p, partA, partB, w.scal.p);
         }
     }
