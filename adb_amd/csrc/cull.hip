@@ -339,8 +339,8 @@ __device__ void flush_candidates(const NarrowArgs& a, int kind, const int* idA, 
 }
 
 template <class BT> struct NarrowCap;
-template <> struct NarrowCap<float> { static constexpr int A = 1024, B = 1024; };
-template <> struct NarrowCap<double> { static constexpr int A = 512, B = 512; };
+template <> struct NarrowCap<float> { static constexpr int A = 512, B = 512; };
+template <> struct NarrowCap<double> { static constexpr int A = 256, B = 256; };
 
 template <int CCD, class BT>
 __global__ __launch_bounds__(NB_THREADS) void k_pair_narrow(NarrowArgs a) {
