@@ -162,7 +162,7 @@ __global__ void k_body_pairs(int B, const double* __restrict__ box, const double
 constexpr int PAIRS_PT = 4;                    // box tests per thread per round
 constexpr int EE_SLICES = 4;                   // workgroups per body pair for the EE pass
 constexpr int NARROW_TASKS = 2 + EE_SLICES;    // workgroups per body pair
-constexpr int QCAP = NB_THREADS * PAIRS_PT * 2; // candidate queue
+constexpr int QCAP = NB_THREADS * PAIRS_PT * 3 / 2; // candidate queue (i << 16 | j: staged indices < 2^16)
 
 struct NarrowArgs {
     const int* n_pairs; int pair_cap; const int2* pairs;   // pair count on the device (may exceed pair_cap)
@@ -273,7 +273,7 @@ template <int CCD> struct KeySpace { KeyBuf kb; };   // LDS key buffer of the co
 template <> struct KeySpace<1> {};
 
 template <int CCD>
-__device__ void flush_candidates(const NarrowArgs& a, int kind, const int* idA, const int* idB, const int2* q, int* qn,
+__device__ void flush_candidates(const NarrowArgs& a, int kind, const int* idA, const int* idB, const int* q, int* qn,
                                  double& tmin, KeyBuf* kb) {
     __syncthreads();
     const int n = *qn;
@@ -287,7 +287,7 @@ __device__ void flush_candidates(const NarrowArgs& a, int kind, const int* idA, 
         bool hit = false;
         unsigned long long key = 0;
         if (t < n) {
-            const int2 c = q[t];
+            const int2 c = make_int2(q[t] >> 16, q[t] & 0xffff);
             const int pa = idA[c.x], pb = idB[c.y];
             if (kind == 0) {
                 const int v = pa, f = pb;
@@ -347,7 +347,7 @@ __global__ __launch_bounds__(NB_THREADS) void k_pair_narrow(NarrowArgs a) {
     constexpr int CA = NarrowCap<BT>::A, CB = NarrowCap<BT>::B;
     __shared__ BT bxA[CA][6], bxB[CB][6];
     __shared__ int idA[CA], idB[CB];
-    __shared__ int2 q[QCAP];
+    __shared__ int q[QCAP];
     __shared__ int cntA, cntB, qn;
     __shared__ double bmin[NB_THREADS / 64];
     __shared__ KeySpace<CCD> ks;
@@ -411,7 +411,7 @@ __global__ __launch_bounds__(NB_THREADS) void k_pair_narrow(NarrowArgs a) {
                                 const bool o = bxA[i][0] <= bxB[j][3] && bxB[j][0] <= bxA[i][3] &&
                                                bxA[i][1] <= bxB[j][4] && bxB[j][1] <= bxA[i][4] &&
                                                bxA[i][2] <= bxB[j][5] && bxB[j][2] <= bxA[i][5];
-                                if (o) { const int s = atomicAdd(&qn, 1); q[s] = make_int2(i, j); }
+                                if (o) { const int s = atomicAdd(&qn, 1); q[s] = (i << 16) | j; }
                             }
                         }
                     }
