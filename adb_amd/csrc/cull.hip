@@ -221,7 +221,11 @@ __device__ int stage_prims(const NarrowArgs& a, int kind, int* cursor, int p1, c
     if (threadIdx.x == 0) *cnt = 0;
     __syncthreads();
     int c = *cursor;
-    while (c < p1 && *cnt + NB_THREADS <= CAP) {
+    while (true) {
+        // every thread reads *cnt before any thread adds to it again
+        const bool go = c < p1 && *cnt + NB_THREADS <= CAP;
+        __syncthreads();
+        if (!go) break;
         const int p = c + threadIdx.x;
         if (p < p1) {
             int v[3];
@@ -280,8 +284,11 @@ __device__ void flush_candidates(const NarrowArgs& a, int kind, const int* idA, 
     const int nround = (n + NB_THREADS - 1) / NB_THREADS;
     for (int it = 0; it < nround; it++) {
         if constexpr (!CCD) {
+            // all threads read kb->n before any thread appends to it again
             __syncthreads();
-            if (kb->n + NB_THREADS > KB) flush_keys(a, kind, *kb);
+            const bool full = kb->n + NB_THREADS > KB;
+            __syncthreads();
+            if (full) flush_keys(a, kind, *kb);
         }
         const int t = it * NB_THREADS + threadIdx.x;
         bool hit = false;
@@ -401,8 +408,11 @@ __global__ __launch_bounds__(NB_THREADS) void k_pair_narrow(NarrowArgs a) {
                     const int nB = stage_prims<CCD, BT, CB>(a, kB, &curB, endB, xlo, xhi, sA, 0.0, bxB, idB, &cntB);
                     const int total = nA * nB;
                     for (int t0 = 0; t0 < total; t0 += NB_THREADS * PAIRS_PT) {
+                        // all threads read qn before any thread appends to the queue again
                         __syncthreads();
-                        if (qn + NB_THREADS * PAIRS_PT > QCAP) flush_candidates<CCD>(a, kind, idA, idB, q, &qn, tmin, kb);
+                        const bool full = qn + NB_THREADS * PAIRS_PT > QCAP;
+                        __syncthreads();
+                        if (full) flush_candidates<CCD>(a, kind, idA, idB, q, &qn, tmin, kb);
     #pragma unroll
                         for (int u = 0; u < PAIRS_PT; u++) {
                             const int t = t0 + u * NB_THREADS + threadIdx.x;
