@@ -532,8 +532,8 @@ __device__ __forceinline__ int classify_one(const WorldArgs& a, int cid) {
         }
     } else {
         a.rec[REC * (size_t)cid + R_D] = 0.0;
-        double* tk = a.tk + 24 * (size_t)cid;
-        for (int i = 0; i < 24; i++) tk[i] = 0.0;
+        double* tk = a.tk + TKN * (size_t)cid;
+        for (int i = 0; i < TKN; i++) tk[i] = 0.0;
         a.lam[cid] = 0.0;
     }
     a.ccls[cid] = cls;
@@ -616,7 +616,7 @@ __device__ __forceinline__ double fric_F(const Fric& f, int i, int j) {
 }
 
 // Record fields shared by the per-thread and per-group paths.
-// Friction part: F, G, the lagged Tk (2 x 12, row-major) and lambda.
+// Friction part: F, G, the lagged basis (gm_k, c0, c1: TKN doubles) and lambda.
 __device__ __forceinline__ void write_fric(const WorldArgs& a, int cid, int nside0, const Fric& f, const double w[4][4]) {
     double* R = a.rec + REC * (size_t)cid;
     const double gmv[4] = {f.gm0, f.gm1, f.gm2, f.gm3};
@@ -634,14 +634,11 @@ __device__ __forceinline__ void write_fric(const WorldArgs& a, int cid, int nsid
                 if ((k < nside0) == (s == 0)) g += gmv[k] * w[k][al];
             R[R_G + 4 * s + al] = g;
         }
-    double* tk = a.tk + 24 * (size_t)cid;
+    double* tk = a.tk + TKN * (size_t)cid;
 #pragma unroll
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < 4; k++) tk[k] = gmv[k];
 #pragma unroll
-        for (int r = 0; r < 3; r++) {
-            tk[3 * k + r] = gmv[k] * cmp(f.c0, r);
-            tk[12 + 3 * k + r] = gmv[k] * cmp(f.c1, r);
-        }
+    for (int r = 0; r < 3; r++) { tk[4 + r] = cmp(f.c0, r); tk[7 + r] = cmp(f.c1, r); }
     a.lam[cid] = f.lam;
 }
 // Body-space gradient from per-point vertex gradients gv_k (friction included).
@@ -1222,7 +1219,7 @@ void contact_terms(World& w) {
     w.crec.reserve(REC * nn);
     w.ctype.reserve(nn); w.cdist.reserve(nn); w.ccls.reserve(nn); w.clist.reserve(nn); w.clist3.reserve(nn);
     w.ccount.reserve(4);
-    w.ctk.reserve(24 * nn);
+    w.ctk.reserve(TKN * nn);
     w.clam.reserve(nn);
     ADB_HIP(hipMemsetAsync(w.ccount.p, 0, 4 * sizeof(int), w.stream));
     if (n > 0) {
@@ -1255,7 +1252,7 @@ void contact_terms(World& w) {
             ADB_HIP(hipStreamWaitEvent(w.stream2, w.ev_side[0], 0));
             {
                 ProfScope ps(w, PK_CONTACT_GROUP, 0.0, w.stream2);
-                ps.counts(snap, 0.0, 0.0, 1720.0, 0.0);
+                ps.counts(snap, 0.0, 0.0, 1608.0, 0.0);
                 const int grid = std::min((n + 63) / 64, 4 * w.num_cu);
                 hipLaunchKernelGGL(k_contact_c3, dim3(grid), dim3(64), 0, w.stream2, a);
                 ADB_HIP(hipGetLastError());
@@ -1265,9 +1262,9 @@ void contact_terms(World& w) {
         }
         {
             // per contact: list, ids, type, distance, X / Xs / xbar of 4 points in;
-            // record (D = 3: 59 doubles, D = 6: 94), Tk and lambda out
+            // record (D = 3: 59 doubles, D = 6: 94), lagged basis (10) and lambda out
             ProfScope ps(w, PK_CONTACT, 0.0);
-            ps.counts(snap, 1016.0, 1296.0, 0.0, 0.0);
+            ps.counts(snap, 904.0, 1184.0, 0.0, 0.0);
             const int grid = std::min((n + 255) / 256, 8 * w.num_cu);
             hipLaunchKernelGGL(k_contact_small<1>, dim3(grid), dim3(256), 0, w.stream, a);
             ADB_HIP(hipGetLastError());

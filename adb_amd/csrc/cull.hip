@@ -489,24 +489,89 @@ __global__ void k_decode2(int n, const unsigned long long* __restrict__ keys, co
     out[i] = make_int2(I, v - voff[I]);
 }
 
-static void sort_keys(World& w, unsigned long long* in, unsigned long long* out, int n, int end_bit) {
-    ProfScope ps(w, PK_SORT, 16.0 * n);
-    size_t tmp = 0;
-    ADB_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, in, out, n, 0, end_bit, w.stream));
-    w.sort_tmp.reserve(tmp + 16);
-    ADB_HIP(hipcub::DeviceRadixSort::SortKeys(w.sort_tmp.p, tmp, in, out, n, 0, end_bit, w.stream));
-}
+// Contact-set keys are unique and sorted by value.  The high part of a key,
+// h = key >> shift, is a vertex or edge id (< nh).  The keys are grouped by
+// g = h >> GS (16 consecutive ids) with a counting sort, then hipCUB's
+// segmented radix sort orders each group on its low shift + GS bits (group
+// sizes vary by two orders of magnitude between the contact patches and the
+// rest, which its size classes absorb).
+// The narrow phase flushes the keys of one body pair and task row together,
+// so consecutive keys mostly share a group: the counting passes take one
+// atomic per run of equal groups in a wavefront, not one per key
+// (device-scope atomics cost ~10 ns each).  Sorting all 43-54 key bits with
+// one radix pass per 8-bit digit took 1.5x longer.
+constexpr int GS = 4;
 
-// packs bucket b's cnt[b] keys (at src + b sub) to dst + off[b]
-__global__ void k_gather_buckets(BucketList L, int sub, const unsigned long long* __restrict__ src,
-                                 unsigned long long* __restrict__ dst) {
+// Lanes [0, nv) of the wavefront hold keys of group g.  Each run of equal g
+// (in lane order) updates ctr[g] by its length with one atomic: added
+// (!FILL), or taken from the run's end (FILL: the run then owns
+// [ctr[g] after, ctr[g] before)).  Returns the lane's slot (FILL).
+template <bool FILL>
+__device__ __forceinline__ int run_atomic(bool valid, unsigned g, int* ctr) {
+    const int lane = __lane_id();
+    const unsigned gp = __shfl_up(g, 1);
+    const bool head = valid && (lane == 0 || gp != g);
+    const unsigned long long mh = __ballot(head);
+    const int nv = __popcll(__ballot(valid));
+    const unsigned long long above = lane == 63 ? 0ull : (mh & (~0ull << (lane + 1)));
+    const int next = above ? __ffsll((long long)above) - 1 : nv;
+    int base = 0;
+    if (head) {
+        if (FILL) base = atomicSub(&ctr[g], next - lane) - (next - lane);
+        else atomicAdd(&ctr[g], next - lane);
+    }
+    const unsigned long long upto = mh & (lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull));
+    const int myhead = upto ? 63 - __clzll(upto) : 0;
+    base = __shfl(base, myhead);
+    return base + lane - myhead;
+}
+// keys of bucket b at src + b * sub; the loop runs whole wavefronts
+template <bool FILL>
+__global__ void k_group_keys(BucketList L, int sub, const unsigned long long* __restrict__ src, int shift, int nh,
+                             int* __restrict__ ctr, unsigned long long* __restrict__ dst) {
     const int b = blockIdx.y;
     const int n = L.cnt[b];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        dst[L.off[b] + i] = src[(size_t)b * sub + i];
+    const int lane = __lane_id();
+    for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += gridDim.x * blockDim.x) {
+        const int i = base + lane;
+        const bool valid = i < n;
+        unsigned long long key = 0;
+        unsigned g = 0;
+        if (valid) {
+            key = src[(size_t)b * sub + i];
+            g = (unsigned)(std::min(key >> shift, (unsigned long long)(nh - 1)) >> GS);
+        }
+        const int pos = run_atomic<FILL>(valid, g, ctr);
+        if (FILL && valid) dst[pos] = key;
+    }
 }
 
-static int bits_for(long long n) { int b = 1; while ((1ll << b) <= n) b++; return b; }
+// Sorts the n keys of L (nb buckets, at most `most` each; high parts < nh)
+// into out; tmp holds n keys.
+static void sort_keys(World& w, const BucketList& L, int nb, int sub, const unsigned long long* src, int n, int most,
+                      int shift, int nh, unsigned long long* tmp, unsigned long long* out) {
+    ProfScope ps(w, PK_SORT, 40.0 * n);
+    const int ng = (nh >> GS) + 1;
+    w.hcnt.reserve((size_t)ng + 1);
+    w.hend.reserve((size_t)ng + 1);
+    ADB_HIP(hipMemsetAsync(w.hcnt.p, 0, (size_t)(ng + 1) * sizeof(int), w.stream));
+    const dim3 grid(std::max(1, std::min(64, (most + 255) / 256)), nb);
+    hipLaunchKernelGGL(k_group_keys<false>, grid, dim3(256), 0, w.stream, L, sub, src, shift, nh, w.hcnt.p, nullptr);
+    ADB_HIP(hipGetLastError());
+    size_t tb = 0;
+    ADB_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tb, w.hcnt.p, w.hend.p, ng + 1, w.stream));
+    w.sort_tmp.reserve(tb + 16);
+    ADB_HIP(hipcub::DeviceScan::InclusiveSum(w.sort_tmp.p, tb, w.hcnt.p, w.hend.p, ng + 1, w.stream));
+    // hend[g] = end of group g; after the fill its start, and hend[ng] = n
+    hipLaunchKernelGGL(k_group_keys<true>, grid, dim3(256), 0, w.stream, L, sub, src, shift, nh, w.hend.p, tmp);
+    ADB_HIP(hipGetLastError());
+    tb = 0;
+    ADB_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, tmp, out, n, ng, w.hend.p, w.hend.p + 1, 0, shift + GS,
+                                                       w.stream));
+    w.sort_tmp.reserve(tb + 16);
+    ADB_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(w.sort_tmp.p, tb, tmp, out, n, ng, w.hend.p, w.hend.p + 1, 0, shift + GS,
+                                                       w.stream));
+}
 
 // body boxes + pairs; the pair count stays on the device (*cnt, which may
 // exceed the capacity: the caller checks it with its other counters)
@@ -554,9 +619,9 @@ void collision_set(World& w, const double* X, ContactSet& out) {
     for (int attempt = 0; attempt < 4; attempt++) {
         const size_t sub = std::max<size_t>(w.keys_a.cap / (2 * NBK), 2048);
         const size_t pcap = sub * NBK;       // keys_a: PT buckets in [0, pcap), EE buckets in [pcap, 2 pcap)
-        w.keys_a.reserve(2 * pcap);
-        w.keys_b.reserve(2 * pcap);
         const size_t vcap = w.V + 64;
+        w.keys_a.reserve(2 * pcap);
+        w.keys_b.reserve(2 * pcap + std::max(pcap, vcap));   // sorted PT, sorted EE, sort scratch / sorted VG
         w.vg_keys.reserve(vcap);
         ADB_HIP(hipMemsetAsync(w.bcnt.p, 0, (2 * NBK + 2) * sizeof(int), w.stream));
         if (!hashed) body_pairs(w, X, nullptr, r, pcnt);
@@ -590,7 +655,7 @@ void collision_set(World& w, const double* X, ContactSet& out) {
         for (int i = 0; i < 2 * NBK; i++) most = std::max(most, h[i]);
         if ((size_t)most > sub) {
             const size_t need = (size_t)most + most / 4 + 64;
-            w.keys_a.reserve(2 * NBK * need); w.keys_b.reserve(2 * NBK * need);
+            w.keys_a.reserve(2 * NBK * need);
             continue;
         }
         BucketList Lp, Le;
@@ -600,27 +665,25 @@ void collision_set(World& w, const double* X, ContactSet& out) {
             Le.cnt[b] = h[NBK + b]; Le.off[b] = nee; nee += h[NBK + b];
         }
         out.pt.reserve(std::max(npt, 1)); out.ee.reserve(std::max(nee, 1)); out.vg.reserve(std::max(nvg, 1));
-        const int vb = bits_for(w.V), eb = bits_for(w.E);
-        // pack the buckets into keys_b, sort back into keys_a, decode
+        // sort the buckets into keys_b (its third part is scratch), decode
         if (npt > 0) {
-            hipLaunchKernelGGL(k_gather_buckets, dim3(std::min(64, (most + 255) / 256), NBK), dim3(256), 0, w.stream, Lp, (int)sub,
-                               w.keys_a.p, w.keys_b.p);
-            sort_keys(w, w.keys_b.p, w.keys_a.p, npt, 32 + vb);
-            hipLaunchKernelGGL(k_decode4, dim3((npt + 255) / 256), dim3(256), 0, w.stream, npt, w.keys_a.p, w.vbody.p, w.voff.p,
+            sort_keys(w, Lp, NBK, (int)sub, w.keys_a.p, npt, most, 32, w.V, w.keys_b.p + 2 * pcap, w.keys_b.p);
+            hipLaunchKernelGGL(k_decode4, dim3((npt + 255) / 256), dim3(256), 0, w.stream, npt, w.keys_b.p, w.vbody.p, w.voff.p,
                                w.fbody.p, w.foff.p, out.pt.p);
             ADB_HIP(hipGetLastError());
         }
         if (nee > 0) {
-            hipLaunchKernelGGL(k_gather_buckets, dim3(std::min(64, (most + 255) / 256), NBK), dim3(256), 0, w.stream, Le, (int)sub,
-                               w.keys_a.p + pcap, w.keys_b.p + pcap);
-            sort_keys(w, w.keys_b.p + pcap, w.keys_a.p + pcap, nee, 32 + eb);
-            hipLaunchKernelGGL(k_decode4, dim3((nee + 255) / 256), dim3(256), 0, w.stream, nee, w.keys_a.p + pcap, w.ebody.p, w.eoff.p,
+            sort_keys(w, Le, NBK, (int)sub, w.keys_a.p + pcap, nee, most, 32, w.E, w.keys_b.p + 2 * pcap, w.keys_b.p + pcap);
+            hipLaunchKernelGGL(k_decode4, dim3((nee + 255) / 256), dim3(256), 0, w.stream, nee, w.keys_b.p + pcap, w.ebody.p, w.eoff.p,
                                w.ebody.p, w.eoff.p, out.ee.p);
             ADB_HIP(hipGetLastError());
         }
         if (nvg > 0) {
-            sort_keys(w, w.vg_keys.p, w.keys_b.p, nvg, vb);
-            hipLaunchKernelGGL(k_decode2, dim3((nvg + 255) / 256), dim3(256), 0, w.stream, nvg, w.keys_b.p, w.vbody.p, w.voff.p, out.vg.p);
+            BucketList Lv;
+            Lv.cnt[0] = nvg; Lv.off[0] = 0;
+            sort_keys(w, Lv, 1, 0, w.vg_keys.p, nvg, nvg, 0, w.V, w.keys_b.p, w.keys_b.p + 2 * pcap);
+            hipLaunchKernelGGL(k_decode2, dim3((nvg + 255) / 256), dim3(256), 0, w.stream, nvg, w.keys_b.p + 2 * pcap, w.vbody.p,
+                               w.voff.p, out.vg.p);
             ADB_HIP(hipGetLastError());
         }
         out.n_pt = npt; out.n_ee = nee; out.n_vg = nvg;

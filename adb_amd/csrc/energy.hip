@@ -101,13 +101,14 @@ __global__ void k_fric_items(FricArgs a) {
                 int ea = a.eoff[c.x] + c.y, eb = a.eoff[c.z] + c.w;
                 ids[0] = a.edge[2 * ea]; ids[1] = a.edge[2 * ea + 1]; ids[2] = a.edge[2 * eb]; ids[3] = a.edge[2 * eb + 1];
             }
-            const double* T = a.tk + 24 * (size_t)i;
+            // lagged basis: the weight of point k times each tangent direction
+            const double* T = a.tk + TKN * (size_t)i;
             double u0 = 0.0, u1 = 0.0;
             for (int k = 0; k < 4; k++)
                 for (int c = 0; c < 3; c++) {
                     double dv = a.X[3 * (size_t)ids[k] + c] - a.Xs[3 * (size_t)ids[k] + c];
-                    u0 += T[3 * k + c] * dv;
-                    u1 += T[12 + 3 * k + c] * dv;
+                    u0 += (T[k] * T[4 + c]) * dv;
+                    u1 += (T[k] * T[7 + c]) * dv;
                 }
             e = a.mu * a.lam[i] * f0_sf(sqrt(u0 * u0 + u1 * u1), a.eps_v);
         }

@@ -51,6 +51,10 @@ template <class T> struct DBuf {
 // PK is the PSD-projected barrier Hessian in Helmert coordinates, F the
 // friction matrix and Z, G the per-side weights of the affine map.
 constexpr int REC = 160;
+// Lagged friction basis of one PT/EE contact: the weights gm_k of its four
+// points and the two tangent directions c0, c1, so that
+// Tk[r][3k + i] = gm_k c_r[i] (products formed where they are used).
+constexpr int TKN = 10;
 constexpr int R_D = 0, R_F = 1, R_G = 10, R_Z = 18, R_GRAD = 42, R_PK = 66;
 
 // Per-kernel HIP-event timing on the world's stream (off unless enabled).
@@ -103,7 +107,8 @@ struct World {
     int pair_hint = 0;               // body-pair count of the last culling pass (narrow-phase grid size)
     DBuf<int> counters;              // device counters
     DBuf<int> bcnt;                  // contact-set key bucket counters (2 x 64) + VG counter
-    DBuf<unsigned long long> keys_a, keys_b;  // unsorted / sorted keys
+    DBuf<unsigned long long> keys_a, keys_b;  // bucketed / sorted keys
+    DBuf<int> hcnt, hend;            // key sort: count and slot range per vertex or edge id
     DBuf<unsigned char> sort_tmp;
     DBuf<unsigned long long> toi_bits;
 
@@ -122,7 +127,7 @@ struct World {
     DBuf<int> clist;                 // class 1 from the front, class 2 from the back
     DBuf<int> clist3;                // class 3 contacts
     DBuf<int> ccount;                // 4 counters: class 1, 2, 3 sizes
-    DBuf<double> ctk;                // 24 per PT/EE contact: Tk (2x12 row-major)
+    DBuf<double> ctk;                // TKN per PT/EE contact: lagged friction basis (see TKN)
     DBuf<double> clam;               // 1 per PT/EE contact
     DBuf<double> vgK;                // 12 per VG contact: K (3x3 col-major) + gx (3)
     DBuf<double> vglam;

@@ -280,7 +280,7 @@ static void free_world(World& w) {
                           &w.crec, &w.cdist, &w.chunk_part, &w.ctk, &w.clam, &w.vgK, &w.vglam, &w.bodyH, &w.bodyG, &w.Hval, &w.grad,
                           &w.Ginv, &w.pcg_x, &w.pcg_r, &w.pcg_z, &w.pcg_p, &w.pcg_p2, &w.pcg_Ap, &w.partial, &w.scal, &w.epart};
     for (auto* b : db) b->free_();
-    w.pairs.free_(); w.keys_a.free_(); w.keys_b.free_(); w.sort_tmp.free_(); w.toi_bits.free_(); w.vg_keys.free_();
+    w.pairs.free_(); w.keys_a.free_(); w.keys_b.free_(); w.hcnt.free_(); w.hend.free_(); w.sort_tmp.free_(); w.toi_bits.free_(); w.vg_keys.free_();
     w.bcnt.free_(); w.sh_cnt.free_(); w.sh_start.free_(); w.sh_ent.free_(); w.sh_rng.free_(); w.sh_total.free_();
     w.akeys.free_(); w.akeys_s.free_(); w.tkeys.free_(); w.tkeys_s.free_();
     for (ContactSet* c : {&w.cs, &w.cs_new}) { c->pt.free_(); c->ee.free_(); c->vg.free_(); }
