@@ -306,12 +306,27 @@ __global__ __launch_bounds__(256) void k_pair_reduce(PairArgs a, const int* __re
 // off-diagonal blocks: one thread per entry of each pair's block (lo, hi):
 // the pair's chunk blocks summed in order, written to (lo, hi) and
 // transposed to (hi, lo)
+// acc += part[PART c + off] for c in [c0, c1), in order; the loads are issued
+// CH_U at a time (a chain of dependent adds would wait for one HBM round trip
+// per chunk)
+constexpr int CH_U = 8;
+__device__ __forceinline__ void add_chunks(double& acc, const double* __restrict__ part, int c0, int c1, int off) {
+    for (int c = c0; c < c1; c += CH_U) {
+        double v[CH_U];
+#pragma unroll
+        for (int u = 0; u < CH_U; u++) v[u] = c + u < c1 ? part[PART * (size_t)(c + u) + off] : 0.0;
+#pragma unroll
+        for (int u = 0; u < CH_U; u++)
+            if (c + u < c1) acc += v[u];
+    }
+}
+
 __global__ void k_pair_offdiag(PairArgs a, const double* __restrict__ part) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     const int p = idx / 144, e = idx % 144;
     if (p >= a.np) return;
     double acc = 0.0;
-    for (int c = a.chunk_off[p]; c < a.chunk_off[p + 1]; c++) acc += part[PART * (size_t)c + 288 + e];
+    add_chunks(acc, part, a.chunk_off[p], a.chunk_off[p + 1], 288 + e);
     a.Hval[144 * (size_t)a.pair_up[p] + e] = acc;
     a.Hval[144 * (size_t)a.pair_low[p] + 12 * (e % 12) + e / 12] = acc;
 }
@@ -329,11 +344,18 @@ __global__ void k_pair_diag(PairArgs a, const double* __restrict__ part) {
     double acc = isg ? a.bodyG[12 * (size_t)b + ge] : a.bodyH[144 * (size_t)b + e];
     const int off_up = isg ? 432 + ge : e, off_low = isg ? 444 + ge : 144 + e;
     // pairs (b, hi): their chunks are contiguous
-    for (int c = a.chunk_off[a.up_start[b]]; c < a.chunk_off[a.up_start[b + 1]]; c++) acc += part[PART * (size_t)c + off_up];
-    // pairs (lo, b)
-    for (int i = a.low_start[b]; i < a.low_start[b + 1]; i++) {
-        const int p = a.tv[i];
-        for (int c = a.chunk_off[p]; c < a.chunk_off[p + 1]; c++) acc += part[PART * (size_t)c + off_low];
+    add_chunks(acc, part, a.chunk_off[a.up_start[b]], a.chunk_off[a.up_start[b + 1]], off_up);
+    // pairs (lo, b), CH_U pairs' chunk ranges fetched at a time
+    for (int i0 = a.low_start[b], i1 = a.low_start[b + 1]; i0 < i1; i0 += CH_U) {
+        int c0[CH_U], c1[CH_U];
+#pragma unroll
+        for (int u = 0; u < CH_U; u++) {
+            const int p = i0 + u < i1 ? a.tv[i0 + u] : 0;
+            c0[u] = i0 + u < i1 ? a.chunk_off[p] : 0;
+            c1[u] = i0 + u < i1 ? a.chunk_off[p + 1] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < CH_U; u++) add_chunks(acc, part, c0[u], c1[u], off_low);
     }
     if (a.mass[b] > 0.0) {
         // vertex-ground terms of this body: (w w^T) (x) K, gradient w (x) gx

@@ -36,12 +36,41 @@ __device__ __forceinline__ double wave_sum(double v) {
     for (int m = 32; m >= 1; m >>= 1) v += __shfl_down(v, m);
     return __shfl(v, 0);
 }
-// fixed-order sum of n values (lane l adds l, l + 64, ... in order)
+// Fixed-order sums of n values (lane l adds l, l + 64, ... in order, then
+// the wave sum).  The loads are issued 8 per lane at a time: a loop of
+// dependent adds would wait for one L2 round trip per value, which for ~1000
+// partials made up most of a PCG iteration.
+constexpr int SUM_U = 8;
 __device__ __forceinline__ double wave_sum_array(const double* v, int n) {
     const int l = threadIdx.x % 64;
     double s = 0.0;
-    for (int i = l; i < n; i += 64) s += v[i];
+    for (int i0 = l; i0 < n; i0 += SUM_U * 64) {
+        double t[SUM_U];
+#pragma unroll
+        for (int u = 0; u < SUM_U; u++) t[u] = i0 + 64 * u < n ? v[i0 + 64 * u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < SUM_U; u++)
+            if (i0 + 64 * u < n) s += t[u];
+    }
     return wave_sum(s);
+}
+__device__ __forceinline__ void wave_sum_array2(const double* a, const double* b, int n, double& ra, double& rb) {
+    const int l = threadIdx.x % 64;
+    double sa = 0.0, sb = 0.0;
+    for (int i0 = l; i0 < n; i0 += SUM_U * 64) {
+        double ta[SUM_U], tb[SUM_U];
+#pragma unroll
+        for (int u = 0; u < SUM_U; u++) {
+            const bool in = i0 + 64 * u < n;
+            ta[u] = in ? a[i0 + 64 * u] : 0.0;
+            tb[u] = in ? b[i0 + 64 * u] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < SUM_U; u++)
+            if (i0 + 64 * u < n) { sa += ta[u]; sb += tb[u]; }
+    }
+    ra = wave_sum(sa);
+    rb = wave_sum(sb);
 }
 // fixed-order workgroup sum (one value per wave combined in wave order)
 __device__ double wg_sum(double v, double* sh) {
@@ -109,8 +138,8 @@ __global__ __launch_bounds__(64 * SP_ROWS) void k_pcg_spmv(int B, const int* __r
     const int iter = *kbase + kloc;
     // scalars of iteration iter - 1 (identical in every wave of every workgroup)
     const int nbU = *gcount;
-    const double RZ = wave_sum_array(partB, nbU);
-    const double RR = wave_sum_array(partB + B, nbU);
+    double RZ, RR;
+    wave_sum_array2(partB, partB + B, nbU, RZ, RR);
     const double bb = iter == 0 ? RR : scal[S_BB];
     int stop = RUN;
     if (!std::isfinite(RR) || !std::isfinite(RZ)) stop = NONFINITE;
@@ -191,8 +220,18 @@ __global__ __launch_bounds__(UPD_T) void k_pcg_update(int B, const int* __restri
     const double* M = G.Minv + G.moff[gi];
     const int i = t % GN, h = t / GN, j0 = h * ((n + 1) / 2), j1 = h ? n : (n + 1) / 2;
     double zi = 0.0;
-    if (i < n)
-        for (int j = j0; j < j1; j++) zi += M[(size_t)j * n + i] * rs[j];
+    if (i < n) {
+        // loads SUM_U at a time, products added in order of j
+        int j = j0;
+        for (; j + SUM_U <= j1; j += SUM_U) {
+            double m[SUM_U];
+#pragma unroll
+            for (int u = 0; u < SUM_U; u++) m[u] = M[(size_t)(j + u) * n + i];
+#pragma unroll
+            for (int u = 0; u < SUM_U; u++) zi += m[u] * rs[j + u];
+        }
+        for (; j < j1; j++) zi += M[(size_t)j * n + i] * rs[j];
+    }
     if (h == 1 && i < n) half2[i] = zi;
     __syncthreads();
     double rz = 0.0, rr = 0.0;
