@@ -450,17 +450,30 @@ __global__ __launch_bounds__(256) void k_group_inv(const int* __restrict__ list,
         }
         return;
     }
-    // W = L^-1: thread c builds column c (W[i][c], i >= c) into A[c][i] (upper) and wd[c]
-    for (int c = t; c < n; c += blockDim.x) {
-        const double wcc = 1.0 / A[c * GLD + c];
-        wd[c] = wcc;
-        for (int i = c + 1; i < n; i++) {
-            double s = A[i * GLD + c] * wcc;
-            for (int k = c + 1; k < i; k++) s += A[i * GLD + k] * A[c * GLD + k];
-            A[c * GLD + i] = -s / A[i * GLD + i];
-        }
+    // W = L^-1 into the upper triangle (W[i][c] at A[c][i], i > c) and wd
+    // (diagonal), right-looking: once row k of W is final, every row i > k
+    // subtracts L[i][k] W[k][c] (c <= k), all (i, c) in parallel; row k + 1 is
+    // then final after its division.  W[i][c] = -(sum_k L[i][k] W[k][c]) /
+    // L[i][i] with the terms in order of k, as a column-by-column forward
+    // substitution, but n steps of parallel updates instead of n^2 / 2
+    // dependent steps of one thread.
+    for (int e = t; e < n; e += blockDim.x) wd[e] = 1.0 / A[e * GLD + e];
+    for (int e = t; e < n * n; e += blockDim.x) {
+        const int c = e / n, i = e % n;
+        if (i > c) A[c * GLD + i] = -0.0;     // -0 - x = -(x): the sign of an all-zero sum matches -(sum)
     }
     __syncthreads();
+    for (int k = 0; k + 1 < n; k++) {
+        const int cols = k + 1, rows = n - 1 - k;
+        for (int e = t; e < rows * cols; e += blockDim.x) {
+            const int i = k + 1 + e / cols, c = e % cols;
+            const double wkc = c == k ? wd[k] : A[c * GLD + k];
+            A[c * GLD + i] -= A[i * GLD + k] * wkc;
+        }
+        __syncthreads();
+        for (int c = t; c <= k; c += blockDim.x) A[c * GLD + k + 1] = A[c * GLD + k + 1] / A[(k + 1) * GLD + k + 1];
+        __syncthreads();
+    }
     // M^-1 = W^T W: (i, j) = sum_{k >= max(i, j)} W[k][i] W[k][j]
     for (int e = t; e < n * n; e += blockDim.x) {
         const int j = e / n, i = e % n;
