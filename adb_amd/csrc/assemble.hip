@@ -380,10 +380,9 @@ __global__ void k_pair_diag(PairArgs a, const double* __restrict__ part) {
 // preconditioned system decouples exactly, so PCG needs only a few
 // iterations where the contact graph is a union of small clusters; in large
 // clusters it is block-Jacobi with blocks of up to GMAX bodies.
-// One workgroup per group: gather the blocks into LDS, Cholesky in place,
-// W = L^-1 column by column (thread per column, stored transposed in the
-// upper triangle), then M^-1 = W^T W.  A group whose matrix is not positive
-// definite falls back to the inverse diagonal.
+// One workgroup per group: gather the blocks into LDS, Cholesky, W = L^-1,
+// M^-1 = W^T W.  A group whose matrix is not positive definite falls back to
+// the inverse diagonal.
 constexpr int GMAX = 8;
 // GM: largest group size of the launch (1 for single bodies, GMAX otherwise);
 // list: the groups of this launch
@@ -392,9 +391,12 @@ __global__ __launch_bounds__(256) void k_group_inv(const int* __restrict__ list,
                                                    const int* __restrict__ grp_body, const long long* __restrict__ grp_moff,
                                                    const int* __restrict__ row_ptr, const int* __restrict__ col_idx,
                                                    const double* __restrict__ Hval, double* __restrict__ Ginv) {
-    constexpr int GN = 12 * GM, GLD = GN + 1;
+    constexpr int GN = 12 * GM, GLD = GN + 1, TT = GN / 16;
+    static_assert(GN == 16 * TT, "16 x 16 tiles");
     __shared__ double A[GN * GLD];
     __shared__ double wd[GN];
+    __shared__ double col[GN];
+    __shared__ double dsh;
     __shared__ int blk[GM][GM];
     __shared__ int bod[GM];
     __shared__ int ok;
@@ -417,27 +419,61 @@ __global__ __launch_bounds__(256) void k_group_inv(const int* __restrict__ list,
         A[i * GLD + j] = k >= 0 ? Hval[144 * (size_t)k + 12 * (j % 12) + (i % 12)] : 0.0;
     }
     __syncthreads();
-    // Cholesky, lower triangle in place
-    for (int j = 0; j < n; j++) {
-        if (t == 0) {
-            const double d = A[j * GLD + j];
-            if (d > 0.0) A[j * GLD + j] = sqrt(d); else ok = 0;
+    // The factorization and the inverse work on 16 x 16 register tiles of
+    // TT x TT entries (thread t owns rows TT (t / 16).., columns TT (t % 16)..),
+    // with one column of L or row of W at a time broadcast through LDS: n
+    // steps of a few barriers each instead of LDS read-modify-write sweeps.
+    // Every entry sees the same operations in the same order as the
+    // column-by-column algorithm (right-looking Cholesky; W = L^-1 with
+    // W[i][c] = -(sum_k L[i][k] W[k][c]) / L[i][i], terms in order of k;
+    // M^-1 entries summed in order of k).
+    const int ti = t / 16, tj = t % 16, r0 = TT * ti, c0 = TT * tj;
+    double T[TT][TT];
+#pragma unroll
+    for (int r = 0; r < TT; r++)
+#pragma unroll
+        for (int c = 0; c < TT; c++) T[r][c] = (r0 + r < n && c0 + c < n) ? A[(r0 + r) * GLD + c0 + c] : 0.0;
+    // Cholesky, lower triangle, column j = TT jt + jj
+    bool bad = false;
+    for (int jt = 0; jt < 16 && TT * jt < n && !bad; jt++) {
+#pragma unroll
+        for (int jj = 0; jj < TT; jj++) {
+            const int j = TT * jt + jj;
+            if (j >= n) break;
+            if (ti == jt && tj == jt) {
+                const double d = T[jj][jj];
+                if (d > 0.0) { T[jj][jj] = sqrt(d); dsh = T[jj][jj]; } else ok = 0;
+            }
+            __syncthreads();
+            if (!ok) { bad = true; break; }
+            const double ljj = dsh;
+            if (tj == jt) {
+#pragma unroll
+                for (int r = 0; r < TT; r++) {
+                    const int i = r0 + r;
+                    if (i > j && i < n) { T[r][jj] = T[r][jj] / ljj; col[i] = T[r][jj]; }
+                }
+            }
+            __syncthreads();
+            if (tj <= ti && c0 + TT - 1 > j) {
+                double li[TT], lk[TT];
+#pragma unroll
+                for (int r = 0; r < TT; r++) li[r] = (r0 + r > j && r0 + r < n) ? col[r0 + r] : 0.0;
+#pragma unroll
+                for (int c = 0; c < TT; c++) lk[c] = (c0 + c > j && c0 + c < n) ? col[c0 + c] : 0.0;
+#pragma unroll
+                for (int r = 0; r < TT; r++)
+#pragma unroll
+                    for (int c = 0; c < TT; c++) {
+                        const int i = r0 + r, k = c0 + c;
+                        if (k > j && k <= i && i < n) T[r][c] -= li[r] * lk[c];
+                    }
+            }
         }
-        __syncthreads();
-        if (!ok) break;
-        const double ljj = A[j * GLD + j];
-        for (int i = j + 1 + t; i < n; i += blockDim.x) A[i * GLD + j] /= ljj;
-        __syncthreads();
-        const int r = n - j - 1;
-        for (int e = t; e < r * r; e += blockDim.x) {
-            const int i = j + 1 + e / r, k = j + 1 + e % r;
-            if (k <= i) A[i * GLD + k] -= A[i * GLD + j] * A[k * GLD + j];
-        }
-        __syncthreads();
     }
     double* out = Ginv + grp_moff[g];
-    if (!ok) {
-        // the Cholesky stopped early; the original diagonal is gone, so take it from Hval
+    if (bad) {
+        // not positive definite: the inverse diagonal of the original matrix (from Hval)
         for (int e = t; e < n * n; e += blockDim.x) {
             const int j = e / n, i = e % n;
             double v = 0.0;
@@ -450,42 +486,102 @@ __global__ __launch_bounds__(256) void k_group_inv(const int* __restrict__ list,
         }
         return;
     }
-    // W = L^-1 into the upper triangle (W[i][c] at A[c][i], i > c) and wd
-    // (diagonal), right-looking: once row k of W is final, every row i > k
-    // subtracts L[i][k] W[k][c] (c <= k), all (i, c) in parallel; row k + 1 is
-    // then final after its division.  W[i][c] = -(sum_k L[i][k] W[k][c]) /
-    // L[i][i] with the terms in order of k, as a column-by-column forward
-    // substitution, but n steps of parallel updates instead of n^2 / 2
-    // dependent steps of one thread.
-    for (int e = t; e < n; e += blockDim.x) wd[e] = 1.0 / A[e * GLD + e];
-    for (int e = t; e < n * n; e += blockDim.x) {
-        const int c = e / n, i = e % n;
-        if (i > c) A[c * GLD + i] = -0.0;     // -0 - x = -(x): the sign of an all-zero sum matches -(sum)
+    // L into LDS (lower triangle), wd = 1 / L[i][i]
+    if (tj <= ti) {
+#pragma unroll
+        for (int r = 0; r < TT; r++)
+#pragma unroll
+            for (int c = 0; c < TT; c++)
+                if (r0 + r < n && c0 + c <= r0 + r) A[(r0 + r) * GLD + c0 + c] = T[r][c];
     }
     __syncthreads();
-    for (int k = 0; k + 1 < n; k++) {
-        const int cols = k + 1, rows = n - 1 - k;
-        for (int e = t; e < rows * cols; e += blockDim.x) {
-            const int i = k + 1 + e / cols, c = e % cols;
-            const double wkc = c == k ? wd[k] : A[c * GLD + k];
-            A[c * GLD + i] -= A[i * GLD + k] * wkc;
+    for (int e = t; e < n; e += blockDim.x) wd[e] = 1.0 / A[e * GLD + e];
+    // W = L^-1, strictly lower part in the tiles (-0 - x = -(x): the sign of
+    // an all-zero sum matches -(sum)), diagonal in wd
+#pragma unroll
+    for (int r = 0; r < TT; r++)
+#pragma unroll
+        for (int c = 0; c < TT; c++) T[r][c] = -0.0;
+    __syncthreads();
+    for (int kt = 0; kt < 16 && TT * kt < n; kt++) {
+#pragma unroll
+        for (int kk = 0; kk < TT; kk++) {
+            const int k = TT * kt + kk;
+            if (k + 1 >= n) break;
+            // row k of W is final: publish it
+            if (ti == kt && tj <= kt) {
+#pragma unroll
+                for (int c = 0; c < TT; c++)
+                    if (c0 + c <= k) col[c0 + c] = c0 + c == k ? wd[k] : T[kk][c];
+            }
+            __syncthreads();
+            if (tj <= ti) {
+                double li[TT], wk[TT];
+#pragma unroll
+                for (int r = 0; r < TT; r++) li[r] = (r0 + r > k && r0 + r < n) ? A[(r0 + r) * GLD + k] : 0.0;
+#pragma unroll
+                for (int c = 0; c < TT; c++) wk[c] = c0 + c <= k ? col[c0 + c] : 0.0;
+#pragma unroll
+                for (int r = 0; r < TT; r++)
+#pragma unroll
+                    for (int c = 0; c < TT; c++)
+                        if (r0 + r > k && r0 + r < n && c0 + c <= k) T[r][c] -= li[r] * wk[c];
+                // row k + 1 is complete: divide by its diagonal
+                const int nr = (kk + 1) % TT;
+                const int nt = kk + 1 == TT ? kt + 1 : kt;
+                if (ti == nt) {
+                    const double lkk = A[(k + 1) * GLD + k + 1];
+#pragma unroll
+                    for (int c = 0; c < TT; c++)
+                        if (c0 + c <= k) T[nr][c] = T[nr][c] / lkk;
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        for (int c = t; c <= k; c += blockDim.x) A[c * GLD + k + 1] = A[c * GLD + k + 1] / A[(k + 1) * GLD + k + 1];
-        __syncthreads();
     }
-    // M^-1 = W^T W: (i, j) = sum_{k >= max(i, j)} W[k][i] W[k][j]
-    for (int e = t; e < n * n; e += blockDim.x) {
-        const int j = e / n, i = e % n;
-        if (i > j) continue;
-        double s = 0.0;
-        for (int k = j; k < n; k++) {
-            const double wki = k == i ? wd[i] : A[i * GLD + k];
-            const double wkj = k == j ? wd[j] : A[j * GLD + k];
-            s += wki * wkj;
+    // W into LDS (strictly lower; the diagonal is wd)
+    if (tj <= ti) {
+#pragma unroll
+        for (int r = 0; r < TT; r++)
+#pragma unroll
+            for (int c = 0; c < TT; c++)
+                if (r0 + r < n && c0 + c < r0 + r) A[(r0 + r) * GLD + c0 + c] = T[r][c];
+    }
+    __syncthreads();
+    // M^-1 = W^T W: (i, j), i <= j, = sum_{k >= j} W[k][i] W[k][j], tiles with ti <= tj
+    if (ti <= tj && r0 < n && c0 < n) {
+#pragma unroll
+        for (int r = 0; r < TT; r++)
+#pragma unroll
+            for (int c = 0; c < TT; c++) T[r][c] = 0.0;
+        for (int k = c0; k < n; k++) {
+            double wi[TT], wj[TT];
+#pragma unroll
+            for (int r = 0; r < TT; r++) {
+                const int i = r0 + r;
+                wi[r] = k == i ? wd[i] : (k > i && i < n ? A[k * GLD + i] : 0.0);
+            }
+#pragma unroll
+            for (int c = 0; c < TT; c++) {
+                const int j = c0 + c;
+                wj[c] = k == j ? wd[j] : (k > j && j < n ? A[k * GLD + j] : 0.0);
+            }
+#pragma unroll
+            for (int r = 0; r < TT; r++)
+#pragma unroll
+                for (int c = 0; c < TT; c++)
+                    if (k >= c0 + c && r0 + r <= c0 + c) T[r][c] += wi[r] * wj[c];
         }
-        out[(size_t)j * n + i] = s;
-        out[(size_t)i * n + j] = s;
+#pragma unroll
+        for (int r = 0; r < TT; r++)
+#pragma unroll
+            for (int c = 0; c < TT; c++) {
+                const int i = r0 + r, j = c0 + c;
+                if (i <= j && j < n) {
+                    out[(size_t)j * n + i] = T[r][c];
+                    out[(size_t)i * n + j] = T[r][c];
+                }
+            }
     }
 }
 
