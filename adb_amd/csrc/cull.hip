@@ -525,12 +525,30 @@ __device__ __forceinline__ int run_atomic(bool valid, unsigned g, int* ctr) {
     base = __shfl(base, myhead);
     return base + lane - myhead;
 }
-// keys of bucket b at src + b * sub; the loop runs whole wavefronts
+// The three contact sets are sorted together: PT keys (v << 32 | f, buckets
+// y < NBK), EE keys (ea << 32 | eb, buckets NBK <= y < 2 NBK; bucket y at
+// buckets + y * sub) and VG keys (v, one flat list, y = 2 NBK).  Their groups
+// follow each other (PT, EE, VG), so the sorted keys are the sorted PT set,
+// then the sorted EE set, then the sorted VG set.
+struct KeySets {
+    BucketList L[2];
+    int sub, nvg, V, E, ng_pt, ng_ee;
+    const unsigned long long* buckets; const unsigned long long* vg;
+};
+// the loop runs whole wavefronts
 template <bool FILL>
-__global__ void k_group_keys(BucketList L, int sub, const unsigned long long* __restrict__ src, int shift, int nh,
-                             int* __restrict__ ctr, unsigned long long* __restrict__ dst) {
-    const int b = blockIdx.y;
-    const int n = L.cnt[b];
+__global__ void k_group_keys(KeySets ks, int* __restrict__ ctr, unsigned long long* __restrict__ dst) {
+    const int y = blockIdx.y;
+    int n, shift, nh, gbase;
+    const unsigned long long* src;
+    if (y < 2 * NBK) {
+        const int set = y / NBK;
+        n = ks.L[set].cnt[y % NBK];
+        src = ks.buckets + (size_t)y * ks.sub;
+        shift = 32; nh = set ? ks.E : ks.V; gbase = set ? ks.ng_pt : 0;
+    } else {
+        n = ks.nvg; src = ks.vg; shift = 0; nh = ks.V; gbase = ks.ng_pt + ks.ng_ee;
+    }
     const int lane = __lane_id();
     for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += gridDim.x * blockDim.x) {
         const int i = base + lane;
@@ -538,38 +556,37 @@ __global__ void k_group_keys(BucketList L, int sub, const unsigned long long* __
         unsigned long long key = 0;
         unsigned g = 0;
         if (valid) {
-            key = src[(size_t)b * sub + i];
-            g = (unsigned)(std::min(key >> shift, (unsigned long long)(nh - 1)) >> GS);
+            key = src[i];
+            g = (unsigned)gbase + (unsigned)(std::min(key >> shift, (unsigned long long)(nh - 1)) >> GS);
         }
         const int pos = run_atomic<FILL>(valid, g, ctr);
         if (FILL && valid) dst[pos] = key;
     }
 }
 
-// Sorts the n keys of L (nb buckets, at most `most` each; high parts < nh)
-// into out; tmp holds n keys.
-static void sort_keys(World& w, const BucketList& L, int nb, int sub, const unsigned long long* src, int n, int most,
-                      int shift, int nh, unsigned long long* tmp, unsigned long long* out) {
+// Sorts the n keys of ks (at most `most` per bucket or list) into out; tmp
+// holds n keys.
+static void sort_keys(World& w, const KeySets& ks, int n, int most, unsigned long long* tmp, unsigned long long* out) {
     ProfScope ps(w, PK_SORT, 40.0 * n);
-    const int ng = (nh >> GS) + 1;
+    const int ng = ks.ng_pt + ks.ng_ee + (w.V >> GS) + 1;
     w.hcnt.reserve((size_t)ng + 1);
     w.hend.reserve((size_t)ng + 1);
     ADB_HIP(hipMemsetAsync(w.hcnt.p, 0, (size_t)(ng + 1) * sizeof(int), w.stream));
-    const dim3 grid(std::max(1, std::min(64, (most + 255) / 256)), nb);
-    hipLaunchKernelGGL(k_group_keys<false>, grid, dim3(256), 0, w.stream, L, sub, src, shift, nh, w.hcnt.p, nullptr);
+    const dim3 grid(std::max(1, std::min(64, (most + 255) / 256)), 2 * NBK + 1);
+    hipLaunchKernelGGL(k_group_keys<false>, grid, dim3(256), 0, w.stream, ks, w.hcnt.p, nullptr);
     ADB_HIP(hipGetLastError());
     size_t tb = 0;
     ADB_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tb, w.hcnt.p, w.hend.p, ng + 1, w.stream));
     w.sort_tmp.reserve(tb + 16);
     ADB_HIP(hipcub::DeviceScan::InclusiveSum(w.sort_tmp.p, tb, w.hcnt.p, w.hend.p, ng + 1, w.stream));
     // hend[g] = end of group g; after the fill its start, and hend[ng] = n
-    hipLaunchKernelGGL(k_group_keys<true>, grid, dim3(256), 0, w.stream, L, sub, src, shift, nh, w.hend.p, tmp);
+    hipLaunchKernelGGL(k_group_keys<true>, grid, dim3(256), 0, w.stream, ks, w.hend.p, tmp);
     ADB_HIP(hipGetLastError());
+    // within a group the keys differ in their low 32 + GS bits (VG keys: GS)
     tb = 0;
-    ADB_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, tmp, out, n, ng, w.hend.p, w.hend.p + 1, 0, shift + GS,
-                                                       w.stream));
+    ADB_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, tmp, out, n, ng, w.hend.p, w.hend.p + 1, 0, 32 + GS, w.stream));
     w.sort_tmp.reserve(tb + 16);
-    ADB_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(w.sort_tmp.p, tb, tmp, out, n, ng, w.hend.p, w.hend.p + 1, 0, shift + GS,
+    ADB_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(w.sort_tmp.p, tb, tmp, out, n, ng, w.hend.p, w.hend.p + 1, 0, 32 + GS,
                                                        w.stream));
 }
 
@@ -621,7 +638,7 @@ void collision_set(World& w, const double* X, ContactSet& out) {
         const size_t pcap = sub * NBK;       // keys_a: PT buckets in [0, pcap), EE buckets in [pcap, 2 pcap)
         const size_t vcap = w.V + 64;
         w.keys_a.reserve(2 * pcap);
-        w.keys_b.reserve(2 * pcap + std::max(pcap, vcap));   // sorted PT, sorted EE, sort scratch / sorted VG
+        w.keys_b.reserve(2 * (2 * pcap + vcap));      // sorted PT, EE, VG; sort scratch
         w.vg_keys.reserve(vcap);
         ADB_HIP(hipMemsetAsync(w.bcnt.p, 0, (2 * NBK + 2) * sizeof(int), w.stream));
         if (!hashed) body_pairs(w, X, nullptr, r, pcnt);
@@ -665,24 +682,29 @@ void collision_set(World& w, const double* X, ContactSet& out) {
             Le.cnt[b] = h[NBK + b]; Le.off[b] = nee; nee += h[NBK + b];
         }
         out.pt.reserve(std::max(npt, 1)); out.ee.reserve(std::max(nee, 1)); out.vg.reserve(std::max(nvg, 1));
-        // sort the buckets into keys_b (its third part is scratch), decode
+        // sort all three sets into keys_b (its second half is scratch), decode
+        const size_t half = 2 * pcap + vcap;
+        const int nall = npt + nee + nvg;
+        if (nall > 0) {
+            KeySets ks;
+            ks.L[0] = Lp; ks.L[1] = Le;
+            ks.sub = (int)sub; ks.nvg = nvg; ks.V = w.V; ks.E = std::max(w.E, 1);
+            ks.ng_pt = (w.V >> GS) + 1; ks.ng_ee = (ks.E >> GS) + 1;
+            ks.buckets = w.keys_a.p; ks.vg = w.vg_keys.p;
+            sort_keys(w, ks, nall, std::max(most, nvg), w.keys_b.p + half, w.keys_b.p);
+        }
         if (npt > 0) {
-            sort_keys(w, Lp, NBK, (int)sub, w.keys_a.p, npt, most, 32, w.V, w.keys_b.p + 2 * pcap, w.keys_b.p);
             hipLaunchKernelGGL(k_decode4, dim3((npt + 255) / 256), dim3(256), 0, w.stream, npt, w.keys_b.p, w.vbody.p, w.voff.p,
                                w.fbody.p, w.foff.p, out.pt.p);
             ADB_HIP(hipGetLastError());
         }
         if (nee > 0) {
-            sort_keys(w, Le, NBK, (int)sub, w.keys_a.p + pcap, nee, most, 32, w.E, w.keys_b.p + 2 * pcap, w.keys_b.p + pcap);
-            hipLaunchKernelGGL(k_decode4, dim3((nee + 255) / 256), dim3(256), 0, w.stream, nee, w.keys_b.p + pcap, w.ebody.p, w.eoff.p,
+            hipLaunchKernelGGL(k_decode4, dim3((nee + 255) / 256), dim3(256), 0, w.stream, nee, w.keys_b.p + npt, w.ebody.p, w.eoff.p,
                                w.ebody.p, w.eoff.p, out.ee.p);
             ADB_HIP(hipGetLastError());
         }
         if (nvg > 0) {
-            BucketList Lv;
-            Lv.cnt[0] = nvg; Lv.off[0] = 0;
-            sort_keys(w, Lv, 1, 0, w.vg_keys.p, nvg, nvg, 0, w.V, w.keys_b.p, w.keys_b.p + 2 * pcap);
-            hipLaunchKernelGGL(k_decode2, dim3((nvg + 255) / 256), dim3(256), 0, w.stream, nvg, w.keys_b.p + 2 * pcap, w.vbody.p,
+            hipLaunchKernelGGL(k_decode2, dim3((nvg + 255) / 256), dim3(256), 0, w.stream, nvg, w.keys_b.p + npt + nee, w.vbody.p,
                                w.voff.p, out.vg.p);
             ADB_HIP(hipGetLastError());
         }
