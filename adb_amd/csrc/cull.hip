@@ -156,7 +156,11 @@ __global__ void k_body_pairs(int B, const double* __restrict__ box, const double
 //   2. box tests: all staged pairs, PAIRS_PT per thread per round, LDS reads
 //      only; overlapping pairs go to an LDS candidate queue;
 //   3. exact tests (FP64, from the positions in HBM) of the queued candidates:
-//      distance type and value against d_hat (sets), or the CCD time of impact.
+//      distance type and value against d_hat (sets, CCD = 0), the CCD time of
+//      impact (CCD = 1), or, over the segment X0 -> X1 (CCD = 2, swept boxes),
+//      the distance at X0 against the reach sqrt(d_hat) plus the largest
+//      displacement of each side: a superset of the set at every point of the
+//      segment, from which the line search's trial sets are filtered.
 // Boxes only cull; the FP64 predicate decides, so the sets equal those of the
 // all-pairs tests (model/thread_local_culling.cpp:209-386 pt/ee_col_set_task).
 constexpr int PAIRS_PT = 4;                    // box tests per thread per round
@@ -167,7 +171,7 @@ constexpr int QCAP = NB_THREADS * PAIRS_PT * 3 / 2; // candidate queue (i << 16 
 struct NarrowArgs {
     const int* n_pairs; int pair_cap; const int2* pairs;   // pair count on the device (may exceed pair_cap)
     const int* voff; const int* foff; const int* eoff; const int* face; const int* edge;
-    const double* X0; const double* X1;   // X1 only for CCD
+    const double* X0; const double* X1;   // X1: CCD and swept candidate sets
     const double* box;                    // body boxes
     const double* sph;                    // body bounding spheres (centre, radius)
     double r; double d_hat;
@@ -283,7 +287,7 @@ __device__ void flush_candidates(const NarrowArgs& a, int kind, const int* idA, 
     const int n = *qn;
     const int nround = (n + NB_THREADS - 1) / NB_THREADS;
     for (int it = 0; it < nround; it++) {
-        if constexpr (!CCD) {
+        if constexpr (CCD != 1) {
             // all threads read kb->n before any thread appends to it again
             __syncthreads();
             const bool full = kb->n + NB_THREADS > KB;
@@ -301,9 +305,19 @@ __device__ void flush_candidates(const NarrowArgs& a, int kind, const int* idA, 
                 const int* fv = a.face + 3 * (size_t)f;
                 const d3 p = ld3(a.X0 + 3 * (size_t)v), t0 = ld3(a.X0 + 3 * (size_t)fv[0]), t1 = ld3(a.X0 + 3 * (size_t)fv[1]),
                          t2 = ld3(a.X0 + 3 * (size_t)fv[2]);
-                if (!CCD) {
+                if (CCD == 0) {
                     const int ty = pt_type(p, t0, t1, t2);
                     hit = pt_dist(p, t0, t1, t2, ty) < a.d_hat;
+                    key = ((unsigned long long)v << 32) | (unsigned long long)f;
+                } else if (CCD == 2) {
+                    // the distance moves by at most the largest displacement of
+                    // each side along X0 -> X1
+                    const double da = sqrt(sqn(ld3(a.X1 + 3 * (size_t)v) - p));
+                    const double db = fmax(fmax(sqrt(sqn(ld3(a.X1 + 3 * (size_t)fv[0]) - t0)), sqrt(sqn(ld3(a.X1 + 3 * (size_t)fv[1]) - t1))),
+                                           sqrt(sqn(ld3(a.X1 + 3 * (size_t)fv[2]) - t2)));
+                    const double reach = (a.r + da + db) * (1.0 + 1e-6) + 1e-9;
+                    const int ty = pt_type(p, t0, t1, t2);
+                    hit = pt_dist(p, t0, t1, t2, ty) < reach * reach;
                     key = ((unsigned long long)v << 32) | (unsigned long long)f;
                 } else {
                     d3 P0[4] = {p, t0, t1, t2};
@@ -317,9 +331,16 @@ __device__ void flush_candidates(const NarrowArgs& a, int kind, const int* idA, 
                 const int* vb = a.edge + 2 * (size_t)eb;
                 const d3 a0 = ld3(a.X0 + 3 * (size_t)va[0]), a1 = ld3(a.X0 + 3 * (size_t)va[1]);
                 const d3 b0 = ld3(a.X0 + 3 * (size_t)vb[0]), b1 = ld3(a.X0 + 3 * (size_t)vb[1]);
-                if (!CCD) {
+                if (CCD == 0) {
                     const int ty = ee_type(a0, a1, b0, b1);
                     hit = ee_dist(a0, a1, b0, b1, ty) < a.d_hat;
+                    key = ((unsigned long long)ea << 32) | (unsigned long long)eb;
+                } else if (CCD == 2) {
+                    const double da = fmax(sqrt(sqn(ld3(a.X1 + 3 * (size_t)va[0]) - a0)), sqrt(sqn(ld3(a.X1 + 3 * (size_t)va[1]) - a1)));
+                    const double db = fmax(sqrt(sqn(ld3(a.X1 + 3 * (size_t)vb[0]) - b0)), sqrt(sqn(ld3(a.X1 + 3 * (size_t)vb[1]) - b1)));
+                    const double reach = (a.r + da + db) * (1.0 + 1e-6) + 1e-9;
+                    const int ty = ee_type(a0, a1, b0, b1);
+                    hit = ee_dist(a0, a1, b0, b1, ty) < reach * reach;
                     key = ((unsigned long long)ea << 32) | (unsigned long long)eb;
                 } else {
                     d3 P0[4] = {a0, a1, b0, b1};
@@ -329,7 +350,7 @@ __device__ void flush_candidates(const NarrowArgs& a, int kind, const int* idA, 
                 }
             }
         }
-        if constexpr (!CCD) {
+        if constexpr (CCD != 1) {
             const unsigned long long m = __ballot(hit);
             if (m) {
                 const int lane = __lane_id(), leader = __ffsll((long long)m) - 1;
@@ -359,13 +380,13 @@ __global__ __launch_bounds__(NB_THREADS) void k_pair_narrow(NarrowArgs a) {
     __shared__ double bmin[NB_THREADS / 64];
     __shared__ KeySpace<CCD> ks;
     KeyBuf* kb = nullptr;
-    if constexpr (!CCD) {
+    if constexpr (CCD != 1) {
         kb = &ks.kb;
         if (threadIdx.x == 0) kb->n = 0;
     }
     if (threadIdx.x == 0) qn = 0;
     double tmin = 1.0;
-    const double infl = CCD ? 0.0 : a.r;
+    const double infl = CCD == 1 ? 0.0 : a.r;
     // task (blockIdx.y): 0 = vertices of I vs faces of J, 1 = vertices of J vs
     // faces of I, 2 + q = edges of I in slice q (of EE_SLICES) vs edges of J
     const int task = blockIdx.y;
@@ -430,8 +451,8 @@ __global__ __launch_bounds__(NB_THREADS) void k_pair_narrow(NarrowArgs a) {
             }
         }
     }   // pairs
-    if constexpr (!CCD) flush_keys(a, kind, *kb);
-    if (CCD) {
+    if constexpr (CCD != 1) flush_keys(a, kind, *kb);
+    if (CCD == 1) {
         for (int m = 32; m >= 1; m >>= 1) tmin = fmin(tmin, __shfl_xor(tmin, m));
         if (__lane_id() == 0) bmin[threadIdx.x / 64] = tmin;
         __syncthreads();
@@ -628,9 +649,13 @@ static NarrowArgs narrow_args(World& w, const int* npairs, const double* X0, con
     return a;
 }
 
-void collision_set(World& w, const double* X, ContactSet& out) {
+// The contact sets at X (X1 == nullptr), or with X1 the PT / EE candidates of
+// the segment X -> X1 (no VG), sorted and decoded into out.
+static void set_pass(World& w, const double* X, const double* X1, ContactSet& out) {
     const double r = sqrt(w.P.d_hat) * (1.0 + 1e-6) + 1e-12;
-    const bool hashed = !(w.P.iaabb & 1);    // spatial hashing (hash.hip) instead of body overlaps
+    const bool swept = X1 != nullptr;
+    const bool hashed = !swept && !(w.P.iaabb & 1);    // spatial hashing (hash.hip) instead of body overlaps
+    const bool ground = w.P.enable_ground && !swept;
     w.bcnt.reserve(2 * NBK + 2);             // PT buckets, EE buckets, VG counter, body-pair counter
     int* pcnt = w.bcnt.p + 2 * NBK + 1;
     for (int attempt = 0; attempt < 4; attempt++) {
@@ -641,19 +666,24 @@ void collision_set(World& w, const double* X, ContactSet& out) {
         w.keys_b.reserve(2 * (2 * pcap + vcap));      // sorted PT, EE, VG; sort scratch
         w.vg_keys.reserve(vcap);
         ADB_HIP(hipMemsetAsync(w.bcnt.p, 0, (2 * NBK + 2) * sizeof(int), w.stream));
-        if (!hashed) body_pairs(w, X, nullptr, r, pcnt);
-        NarrowArgs a = narrow_args(w, pcnt, X, nullptr, r);
+        if (!hashed) body_pairs(w, X, X1, r, pcnt);
+        NarrowArgs a = narrow_args(w, pcnt, X, X1, r);
         a.pt_keys = w.keys_a.p; a.ee_keys = w.keys_a.p + pcap; a.sub = (int)sub; a.bcnt = w.bcnt.p;
         if (hashed) {
             hash_sets(w, X, r, a.pt_keys, a.ee_keys, (int)sub, w.bcnt.p);
         } else {
             ProfScope ps(w, PK_NARROW_SET, 0.0);
             const dim3 grid(narrow_grid(w), NARROW_TASKS);
-            if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<0, float>), grid, dim3(NB_THREADS), 0, w.stream, a);
-            else hipLaunchKernelGGL((k_pair_narrow<0, double>), grid, dim3(NB_THREADS), 0, w.stream, a);
+            if (swept) {
+                if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<2, float>), grid, dim3(NB_THREADS), 0, w.stream, a);
+                else hipLaunchKernelGGL((k_pair_narrow<2, double>), grid, dim3(NB_THREADS), 0, w.stream, a);
+            } else {
+                if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<0, float>), grid, dim3(NB_THREADS), 0, w.stream, a);
+                else hipLaunchKernelGGL((k_pair_narrow<0, double>), grid, dim3(NB_THREADS), 0, w.stream, a);
+            }
             ADB_HIP(hipGetLastError());
         }
-        if (w.P.enable_ground) {
+        if (ground) {
             ProfScope ps(w, PK_VG_SCAN, 8.0 * w.V);
             hipLaunchKernelGGL(k_vg_scan, dim3((w.V + 255) / 256), dim3(256), 0, w.stream, w.V, X, w.P.ground_y, w.P.d_hat,
                                w.vg_keys.p, (int)vcap, w.bcnt.p + 2 * NBK);
@@ -667,7 +697,7 @@ void collision_set(World& w, const double* X, ContactSet& out) {
             w.pairs.reserve(h[2 * NBK + 1]);
             continue;
         }
-        const int nvg = w.P.enable_ground ? h[2 * NBK] : 0;
+        const int nvg = ground ? h[2 * NBK] : 0;
         int most = 0;
         for (int i = 0; i < 2 * NBK; i++) most = std::max(most, h[i]);
         if ((size_t)most > sub) {
@@ -712,6 +742,113 @@ void collision_set(World& w, const double* X, ContactSet& out) {
         return;
     }
     throw AdbError(ADB_E_CAPACITY, "contact-set capacity could not be grown");
+}
+
+void collision_set(World& w, const double* X, ContactSet& out) { set_pass(w, X, nullptr, out); }
+
+void segment_candidates(World& w, const double* X0, const double* X1, ContactSet& cand) { set_pass(w, X0, X1, cand); }
+
+// ---- trial sets from the segment's candidates
+// flags[i] = (candidate i is in the set at X): the distance type and value of
+// the narrow phase's exact test, so the filtered lists equal its sets
+__global__ void k_cand_flags(int n_pt, int n_ee, const int4* __restrict__ pt, const int4* __restrict__ ee,
+                             const int* __restrict__ voff, const int* __restrict__ foff, const int* __restrict__ eoff,
+                             const int* __restrict__ face, const int* __restrict__ edge, const double* __restrict__ X,
+                             double d_hat, char* __restrict__ flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_pt + n_ee) return;
+    bool hit;
+    if (i < n_pt) {
+        const int4 c = pt[i];
+        const int v = voff[c.x] + c.y, f = foff[c.z] + c.w;
+        const int* fv = face + 3 * (size_t)f;
+        const d3 p = ld3(X + 3 * (size_t)v), t0 = ld3(X + 3 * (size_t)fv[0]), t1 = ld3(X + 3 * (size_t)fv[1]),
+                 t2 = ld3(X + 3 * (size_t)fv[2]);
+        hit = pt_dist(p, t0, t1, t2, pt_type(p, t0, t1, t2)) < d_hat;
+    } else {
+        const int4 c = ee[i - n_pt];
+        const int ea = eoff[c.x] + c.y, eb = eoff[c.z] + c.w;
+        const int* va = edge + 2 * (size_t)ea;
+        const int* vb = edge + 2 * (size_t)eb;
+        const d3 a0 = ld3(X + 3 * (size_t)va[0]), a1 = ld3(X + 3 * (size_t)va[1]);
+        const d3 b0 = ld3(X + 3 * (size_t)vb[0]), b1 = ld3(X + 3 * (size_t)vb[1]);
+        hit = ee_dist(a0, a1, b0, b1, ee_type(a0, a1, b0, b1)) < d_hat;
+    }
+    flags[i] = hit ? 1 : 0;
+}
+// VG test of k_vg_scan as a flag per vertex
+__global__ void k_vg_flags(int V, const double* __restrict__ X, double ground_y, double d_hat, char* __restrict__ flags) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const double dd = X[3 * (size_t)v + 1] - ground_y;
+    flags[v] = dd * dd < d_hat ? 1 : 0;
+}
+__global__ void k_vg_of(int n, const int* __restrict__ v, const int* __restrict__ vbody, const int* __restrict__ voff,
+                        int2* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int I = vbody[v[i]];
+    out[i] = make_int2(I, v[i] - voff[I]);
+}
+
+// The sets at X from the candidates of a segment through X (the order of
+// the candidates, sorted, is kept), VG by a scan of all vertices.
+void trial_set(World& w, const double* X, const ContactSet& cand, ContactSet& out) {
+    const int npc = cand.n_pt, nec = cand.n_ee, nc = npc + nec;
+    w.cflags.reserve((size_t)std::max(nc, 1) + w.V);
+    w.vsel.reserve(w.V);
+    w.bcnt.reserve(2 * NBK + 2);
+    out.pt.reserve(std::max(npc, 1)); out.ee.reserve(std::max(nec, 1));
+    int* dn = w.bcnt.p;                      // selected PT, EE, VG counts
+    ADB_HIP(hipMemsetAsync(dn, 0, 3 * sizeof(int), w.stream));
+    {
+        ProfScope ps(w, PK_NARROW_SET, 0.0);
+        if (nc > 0) {
+            hipLaunchKernelGGL(k_cand_flags, dim3((nc + 255) / 256), dim3(256), 0, w.stream, npc, nec, cand.pt.p, cand.ee.p, w.voff.p,
+                               w.foff.p, w.eoff.p, w.face.p, w.edge.p, X, w.P.d_hat, w.cflags.p);
+            ADB_HIP(hipGetLastError());
+        }
+        size_t tb = 0, need = 0;
+        if (npc > 0) {
+            ADB_HIP(hipcub::DeviceSelect::Flagged(nullptr, tb, cand.pt.p, w.cflags.p, out.pt.p, dn, npc, w.stream));
+            need = std::max(need, tb);
+        }
+        if (nec > 0) {
+            ADB_HIP(hipcub::DeviceSelect::Flagged(nullptr, tb, cand.ee.p, w.cflags.p + npc, out.ee.p, dn + 1, nec, w.stream));
+            need = std::max(need, tb);
+        }
+        hipcub::CountingInputIterator<int> vid(0);
+        if (w.P.enable_ground) {
+            ADB_HIP(hipcub::DeviceSelect::Flagged(nullptr, tb, vid, w.cflags.p + nc, w.vsel.p, dn + 2, w.V, w.stream));
+            need = std::max(need, tb);
+        }
+        w.sort_tmp.reserve(need + 16);
+        if (npc > 0) {
+            tb = need;
+            ADB_HIP(hipcub::DeviceSelect::Flagged(w.sort_tmp.p, tb, cand.pt.p, w.cflags.p, out.pt.p, dn, npc, w.stream));
+        }
+        if (nec > 0) {
+            tb = need;
+            ADB_HIP(hipcub::DeviceSelect::Flagged(w.sort_tmp.p, tb, cand.ee.p, w.cflags.p + npc, out.ee.p, dn + 1, nec, w.stream));
+        }
+        if (w.P.enable_ground) {
+            hipLaunchKernelGGL(k_vg_flags, dim3((w.V + 255) / 256), dim3(256), 0, w.stream, w.V, X, w.P.ground_y, w.P.d_hat,
+                               w.cflags.p + nc);
+            ADB_HIP(hipGetLastError());
+            tb = need;
+            ADB_HIP(hipcub::DeviceSelect::Flagged(w.sort_tmp.p, tb, vid, w.cflags.p + nc, w.vsel.p, dn + 2, w.V, w.stream));
+        }
+    }
+    int h[3];
+    ADB_HIP(hipMemcpyAsync(h, dn, sizeof h, hipMemcpyDeviceToHost, w.stream));
+    ADB_HIP(hipStreamSynchronize(w.stream));
+    const int nvg = w.P.enable_ground ? h[2] : 0;
+    out.vg.reserve(std::max(nvg, 1));
+    if (nvg > 0) {
+        hipLaunchKernelGGL(k_vg_of, dim3((nvg + 255) / 256), dim3(256), 0, w.stream, nvg, w.vsel.p, w.vbody.p, w.voff.p, out.vg.p);
+        ADB_HIP(hipGetLastError());
+    }
+    out.n_pt = h[0]; out.n_ee = h[1]; out.n_vg = nvg;
 }
 
 double ccd_toi(World& w, const double* X0, const double* X1, int* status) {

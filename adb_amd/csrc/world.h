@@ -119,6 +119,10 @@ struct World {
     DBuf<int4> sh_rng;
     DBuf<unsigned long long> sh_total;
     ContactSet cs, cs_new;
+    ContactSet cand;                 // line search: candidates of the search segment
+    bool ls_cand = true;             // trial sets filtered from them (ADB_LS_CANDIDATES=0: full passes)
+    DBuf<char> cflags;               // candidate / vertex flags of a trial set
+    DBuf<int> vsel;                  // selected VG vertices
     // ---- contact terms (per PT/EE contact, then per VG contact)
     DBuf<double> crec;               // REC per PT/EE contact: compact body-space record
     DBuf<int> ctype;                 // distance type per contact
@@ -210,6 +214,10 @@ const int* prof_snapshot(World& w, const int* dev_counts);
 void transform_positions(World& w, const double* q, double* X);                         // cull.hip
 void q_axpy(World& w, const double* q, const double* dq, double alpha, double* out);     // cull.hip
 void collision_set(World& w, const double* X, ContactSet& out);                          // cull.hip
+// PT / EE candidates of every set on the segment X0 -> X1 (cull.hip), and a
+// set at a point of that segment filtered from them (VG by a full scan)
+void segment_candidates(World& w, const double* X0, const double* X1, ContactSet& cand);
+void trial_set(World& w, const double* X, const ContactSet& cand, ContactSet& out);
 double ccd_toi(World& w, const double* X0, const double* X1, int* status);              // cull.hip
 void hash_sets(World& w, const double* X, double r, unsigned long long* pt_keys, unsigned long long* ee_keys,
                int sub, int* bcnt);                                                      // hash.hip

@@ -99,10 +99,19 @@ static int step(World& w, adb_step_stats* st) {
             const double E0 = sl[0] + sl[1];
             const double qdg = sl[2];
             int trials = 0;
+            // trial sets from the candidates of the whole search segment q -> q + dq
+            // (vertex positions are affine in q, so every trial point lies on it)
+            const bool cand = w.ls_cand && (w.P.iaabb & 1);
+            if (cand) {
+                q_axpy(w, w.q.p, w.dq.p, 1.0, w.qtrial.p);
+                transform_positions(w, w.qtrial.p, w.X1.p);
+                segment_candidates(w, w.X.p, w.X1.p, w.cand);
+            }
             while (true) {
                 q_axpy(w, w.q.p, w.dq.p, alpha, w.qtrial.p);
                 transform_positions(w, w.qtrial.p, w.X1.p);
-                collision_set(w, w.X1.p, w.cs_new);
+                if (cand) trial_set(w, w.X1.p, w.cand, w.cs_new);
+                else collision_set(w, w.X1.p, w.cs_new);
                 energy_barrier_inertia_async(w, w.qtrial.p, w.X1.p, w.cs_new, 0);
                 energy_friction_async(w, w.X1.p, 1);
                 read_slots(w, 2, sl);
@@ -207,6 +216,7 @@ extern "C" int adb_world_create(const adb_scene* s, const adb_params* p, adb_wor
             throw AdbError(ADB_E_ARG, "spatial hashing needs 1 <= sh_bits <= 8 and a finite sh_dx > 0");
         w.B = s->n_bodies;
         if (const char* g = std::getenv("ADB_PCG_GRAPH")) w.use_graphs = std::strcmp(g, "0") != 0;
+        if (const char* g = std::getenv("ADB_LS_CANDIDATES")) w.ls_cand = std::strcmp(g, "0") != 0;
         w.h_voff.assign(s->vert_offset, s->vert_offset + w.B + 1);
         w.h_foff.assign(s->face_offset, s->face_offset + w.B + 1);
         w.h_eoff.assign(s->edge_offset, s->edge_offset + w.B + 1);
@@ -283,7 +293,8 @@ static void free_world(World& w) {
     w.pairs.free_(); w.keys_a.free_(); w.keys_b.free_(); w.hcnt.free_(); w.hend.free_(); w.sort_tmp.free_(); w.toi_bits.free_(); w.vg_keys.free_();
     w.bcnt.free_(); w.sh_cnt.free_(); w.sh_start.free_(); w.sh_ent.free_(); w.sh_rng.free_(); w.sh_total.free_();
     w.akeys.free_(); w.akeys_s.free_(); w.tkeys.free_(); w.tkeys_s.free_();
-    for (ContactSet* c : {&w.cs, &w.cs_new}) { c->pt.free_(); c->ee.free_(); c->vg.free_(); }
+    for (ContactSet* c : {&w.cs, &w.cs_new, &w.cand}) { c->pt.free_(); c->ee.free_(); c->vg.free_(); }
+    w.cflags.free_(); w.vsel.free_();
     for (auto& r : w.prof.pending) { w.prof.pool.push_back(r.a); w.prof.pool.push_back(r.b); }
     w.prof.pending.clear();
     for (hipEvent_t e : w.prof.pool) (void)hipEventDestroy(e);

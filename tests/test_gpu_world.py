@@ -204,3 +204,25 @@ def test_hash_mode_trajectory_equals_iaabb_mode():
             assert rc == 0, (mode, s, rc)
         qs.append(gw.state()[0])
     assert np.array_equal(qs[0], qs[1])
+
+
+@pytest.mark.parametrize("name,steps", [("cube_stack", 10), ("spheres", 8)])
+def test_line_search_candidate_sets_equal_full_passes(name, steps, monkeypatch):
+    # trial sets filtered from the candidates of the search segment equal the
+    # sets of a full culling pass at each trial point: identical steps
+    _require_gpu()
+    mk, kw = SCENES[name]
+    sc = mk()
+    runs = []
+    for cand in ("1", "0"):
+        monkeypatch.setenv("ADB_LS_CANDIDATES", cand)
+        gw = World(sc, default_params(**kw))
+        counts = []
+        for s in range(steps):
+            rc, st = gw.step()
+            assert rc == 0, (cand, s, rc)
+            counts.append((st.newton_iters, st.ls_trials, st.pcg_iters, st.n_pt, st.n_ee, st.n_vg))
+        runs.append((gw.state()[0], counts))
+    assert runs[0][1] == runs[1][1]
+    assert sum(c[1] for c in runs[0][1]) > 0 and sum(c[3] + c[4] for c in runs[0][1]) > 0
+    assert np.array_equal(runs[0][0], runs[1][0])
