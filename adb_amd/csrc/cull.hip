@@ -749,106 +749,105 @@ void collision_set(World& w, const double* X, ContactSet& out) { set_pass(w, X, 
 void segment_candidates(World& w, const double* X0, const double* X1, ContactSet& cand) { set_pass(w, X0, X1, cand); }
 
 // ---- trial sets from the segment's candidates
-// flags[i] = (candidate i is in the set at X): the distance type and value of
-// the narrow phase's exact test, so the filtered lists equal its sets
-__global__ void k_cand_flags(int n_pt, int n_ee, const int4* __restrict__ pt, const int4* __restrict__ ee,
-                             const int* __restrict__ voff, const int* __restrict__ foff, const int* __restrict__ eoff,
-                             const int* __restrict__ face, const int* __restrict__ edge, const double* __restrict__ X,
-                             double d_hat, char* __restrict__ flags) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_pt + n_ee) return;
-    bool hit;
-    if (i < n_pt) {
-        const int4 c = pt[i];
-        const int v = voff[c.x] + c.y, f = foff[c.z] + c.w;
-        const int* fv = face + 3 * (size_t)f;
-        const d3 p = ld3(X + 3 * (size_t)v), t0 = ld3(X + 3 * (size_t)fv[0]), t1 = ld3(X + 3 * (size_t)fv[1]),
-                 t2 = ld3(X + 3 * (size_t)fv[2]);
-        hit = pt_dist(p, t0, t1, t2, pt_type(p, t0, t1, t2)) < d_hat;
-    } else {
-        const int4 c = ee[i - n_pt];
-        const int ea = eoff[c.x] + c.y, eb = eoff[c.z] + c.w;
-        const int* va = edge + 2 * (size_t)ea;
-        const int* vb = edge + 2 * (size_t)eb;
-        const d3 a0 = ld3(X + 3 * (size_t)va[0]), a1 = ld3(X + 3 * (size_t)va[1]);
-        const d3 b0 = ld3(X + 3 * (size_t)vb[0]), b1 = ld3(X + 3 * (size_t)vb[1]);
-        hit = ee_dist(a0, a1, b0, b1, ee_type(a0, a1, b0, b1)) < d_hat;
+// One launch over three ranges of 256-item blocks: PT candidates, EE
+// candidates, vertices (VG).  Each item is tested as the narrow phase / the
+// VG scan tests it (same distance type and value against d_hat, so the kept
+// lists equal the full pass's sets); pass 1 counts the kept items per block,
+// pass 2 writes them at the block's scanned offset + their rank in the block,
+// which keeps the (sorted) candidate order.
+struct TrialArgs {
+    int n_pt, n_ee, V, nb_pt, nb_ee, nb_v, ground;
+    const int4* pt; const int4* ee;
+    const int* voff; const int* foff; const int* eoff; const int* face; const int* edge; const int* vbody;
+    const double* X; double d_hat, ground_y;
+    int* bcount;                       // kept items per block (pass 1), their exclusive scan (pass 2)
+    int4* out_pt; int4* out_ee; int2* out_vg;
+};
+__device__ __forceinline__ bool trial_keep(const TrialArgs& a, int range, int i) {
+    if (range == 0) {
+        const int4 c = a.pt[i];
+        const int v = a.voff[c.x] + c.y, f = a.foff[c.z] + c.w;
+        const int* fv = a.face + 3 * (size_t)f;
+        const d3 p = ld3(a.X + 3 * (size_t)v), t0 = ld3(a.X + 3 * (size_t)fv[0]), t1 = ld3(a.X + 3 * (size_t)fv[1]),
+                 t2 = ld3(a.X + 3 * (size_t)fv[2]);
+        return pt_dist(p, t0, t1, t2, pt_type(p, t0, t1, t2)) < a.d_hat;
     }
-    flags[i] = hit ? 1 : 0;
+    if (range == 1) {
+        const int4 c = a.ee[i];
+        const int ea = a.eoff[c.x] + c.y, eb = a.eoff[c.z] + c.w;
+        const int* va = a.edge + 2 * (size_t)ea;
+        const int* vb = a.edge + 2 * (size_t)eb;
+        const d3 a0 = ld3(a.X + 3 * (size_t)va[0]), a1 = ld3(a.X + 3 * (size_t)va[1]);
+        const d3 b0 = ld3(a.X + 3 * (size_t)vb[0]), b1 = ld3(a.X + 3 * (size_t)vb[1]);
+        return ee_dist(a0, a1, b0, b1, ee_type(a0, a1, b0, b1)) < a.d_hat;
+    }
+    const double dd = a.X[3 * (size_t)i + 1] - a.ground_y;
+    return dd * dd < a.d_hat;
 }
-// VG test of k_vg_scan as a flag per vertex
-__global__ void k_vg_flags(int V, const double* __restrict__ X, double ground_y, double d_hat, char* __restrict__ flags) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
-    const double dd = X[3 * (size_t)v + 1] - ground_y;
-    flags[v] = dd * dd < d_hat ? 1 : 0;
-}
-__global__ void k_vg_of(int n, const int* __restrict__ v, const int* __restrict__ vbody, const int* __restrict__ voff,
-                        int2* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int I = vbody[v[i]];
-    out[i] = make_int2(I, v[i] - voff[I]);
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_trial_set(TrialArgs a) {
+    __shared__ int wcnt[4];
+    int b = blockIdx.x, range = 0, n = a.n_pt;
+    if (b >= a.nb_pt) { b -= a.nb_pt; range = 1; n = a.n_ee; if (b >= a.nb_ee) { b -= a.nb_ee; range = 2; n = a.V; } }
+    const int i = b * 256 + threadIdx.x;
+    const bool keep = i < n && (range < 2 || a.ground) && trial_keep(a, range, i);
+    const unsigned long long m = __ballot(keep);
+    const int wv = threadIdx.x / 64, lane = __lane_id();
+    if (lane == 0) wcnt[wv] = __popcll(m);
+    __syncthreads();
+    if (!WRITE) {
+        if (threadIdx.x == 0) a.bcount[blockIdx.x] = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        return;
+    }
+    if (!keep) return;
+    int pos = a.bcount[blockIdx.x] + __popcll(m & ((1ull << lane) - 1ull));
+    for (int k = 0; k < wv; k++) pos += wcnt[k];
+    if (range == 0) a.out_pt[pos] = a.pt[i];
+    else if (range == 1) a.out_ee[pos - a.bcount[a.nb_pt]] = a.ee[i];
+    else {
+        const int I = a.vbody[i];
+        a.out_vg[pos - a.bcount[a.nb_pt + a.nb_ee]] = make_int2(I, i - a.voff[I]);
+    }
 }
 
-// The sets at X from the candidates of a segment through X (the order of
-// the candidates, sorted, is kept), VG by a scan of all vertices.
+// The sets at X from the candidates of a segment through X (their sorted
+// order is kept), VG by a scan of all vertices.
 void trial_set(World& w, const double* X, const ContactSet& cand, ContactSet& out) {
-    const int npc = cand.n_pt, nec = cand.n_ee, nc = npc + nec;
-    w.cflags.reserve((size_t)std::max(nc, 1) + w.V);
-    w.vsel.reserve(w.V);
-    w.bcnt.reserve(2 * NBK + 2);
-    out.pt.reserve(std::max(npc, 1)); out.ee.reserve(std::max(nec, 1));
-    int* dn = w.bcnt.p;                      // selected PT, EE, VG counts
-    ADB_HIP(hipMemsetAsync(dn, 0, 3 * sizeof(int), w.stream));
-    {
-        ProfScope ps(w, PK_NARROW_SET, 0.0);
-        if (nc > 0) {
-            hipLaunchKernelGGL(k_cand_flags, dim3((nc + 255) / 256), dim3(256), 0, w.stream, npc, nec, cand.pt.p, cand.ee.p, w.voff.p,
-                               w.foff.p, w.eoff.p, w.face.p, w.edge.p, X, w.P.d_hat, w.cflags.p);
-            ADB_HIP(hipGetLastError());
-        }
-        size_t tb = 0, need = 0;
-        if (npc > 0) {
-            ADB_HIP(hipcub::DeviceSelect::Flagged(nullptr, tb, cand.pt.p, w.cflags.p, out.pt.p, dn, npc, w.stream));
-            need = std::max(need, tb);
-        }
-        if (nec > 0) {
-            ADB_HIP(hipcub::DeviceSelect::Flagged(nullptr, tb, cand.ee.p, w.cflags.p + npc, out.ee.p, dn + 1, nec, w.stream));
-            need = std::max(need, tb);
-        }
-        hipcub::CountingInputIterator<int> vid(0);
-        if (w.P.enable_ground) {
-            ADB_HIP(hipcub::DeviceSelect::Flagged(nullptr, tb, vid, w.cflags.p + nc, w.vsel.p, dn + 2, w.V, w.stream));
-            need = std::max(need, tb);
-        }
-        w.sort_tmp.reserve(need + 16);
-        if (npc > 0) {
-            tb = need;
-            ADB_HIP(hipcub::DeviceSelect::Flagged(w.sort_tmp.p, tb, cand.pt.p, w.cflags.p, out.pt.p, dn, npc, w.stream));
-        }
-        if (nec > 0) {
-            tb = need;
-            ADB_HIP(hipcub::DeviceSelect::Flagged(w.sort_tmp.p, tb, cand.ee.p, w.cflags.p + npc, out.ee.p, dn + 1, nec, w.stream));
-        }
-        if (w.P.enable_ground) {
-            hipLaunchKernelGGL(k_vg_flags, dim3((w.V + 255) / 256), dim3(256), 0, w.stream, w.V, X, w.P.ground_y, w.P.d_hat,
-                               w.cflags.p + nc);
-            ADB_HIP(hipGetLastError());
-            tb = need;
-            ADB_HIP(hipcub::DeviceSelect::Flagged(w.sort_tmp.p, tb, vid, w.cflags.p + nc, w.vsel.p, dn + 2, w.V, w.stream));
-        }
-    }
-    int h[3];
-    ADB_HIP(hipMemcpyAsync(h, dn, sizeof h, hipMemcpyDeviceToHost, w.stream));
-    ADB_HIP(hipStreamSynchronize(w.stream));
-    const int nvg = w.P.enable_ground ? h[2] : 0;
-    out.vg.reserve(std::max(nvg, 1));
-    if (nvg > 0) {
-        hipLaunchKernelGGL(k_vg_of, dim3((nvg + 255) / 256), dim3(256), 0, w.stream, nvg, w.vsel.p, w.vbody.p, w.voff.p, out.vg.p);
+    TrialArgs a;
+    a.n_pt = cand.n_pt; a.n_ee = cand.n_ee; a.V = w.V; a.ground = w.P.enable_ground ? 1 : 0;
+    a.nb_pt = (a.n_pt + 255) / 256; a.nb_ee = (a.n_ee + 255) / 256; a.nb_v = a.ground ? (w.V + 255) / 256 : 0;
+    const int nb = a.nb_pt + a.nb_ee + a.nb_v;
+    a.pt = cand.pt.p; a.ee = cand.ee.p;
+    a.voff = w.voff.p; a.foff = w.foff.p; a.eoff = w.eoff.p; a.face = w.face.p; a.edge = w.edge.p; a.vbody = w.vbody.p;
+    a.X = X; a.d_hat = w.P.d_hat; a.ground_y = w.P.ground_y;
+    // the candidate lists bound the kept PT / EE lists; VG at most one per vertex
+    out.pt.reserve(std::max(a.n_pt, 1)); out.ee.reserve(std::max(a.n_ee, 1)); out.vg.reserve(a.ground ? w.V : 1);
+    a.out_pt = out.pt.p; a.out_ee = out.ee.p; a.out_vg = out.vg.p;
+    w.vsel.reserve(2 * (size_t)nb + 2);
+    int* cnt = w.vsel.p;                      // [0, nb]: block counts; [nb + 1, 2 nb + 2): their exclusive scan
+    int* off = w.vsel.p + nb + 1;
+    a.bcount = cnt;
+    ProfScope ps(w, PK_NARROW_SET, 0.0);
+    if (nb > 0) {
+        hipLaunchKernelGGL(k_trial_set<false>, dim3(nb), dim3(256), 0, w.stream, a);
         ADB_HIP(hipGetLastError());
     }
-    out.n_pt = h[0]; out.n_ee = h[1]; out.n_vg = nvg;
+    ADB_HIP(hipMemsetAsync(cnt + nb, 0, sizeof(int), w.stream));
+    size_t tb = 0;
+    ADB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt, off, nb + 1, w.stream));
+    w.sort_tmp.reserve(tb + 16);
+    ADB_HIP(hipcub::DeviceScan::ExclusiveSum(w.sort_tmp.p, tb, cnt, off, nb + 1, w.stream));
+    a.bcount = off;
+    if (nb > 0) {
+        hipLaunchKernelGGL(k_trial_set<true>, dim3(nb), dim3(256), 0, w.stream, a);
+        ADB_HIP(hipGetLastError());
+    }
+    int h[3];
+    ADB_HIP(hipMemcpyAsync(&h[0], off + a.nb_pt, sizeof(int), hipMemcpyDeviceToHost, w.stream));
+    ADB_HIP(hipMemcpyAsync(&h[1], off + a.nb_pt + a.nb_ee, sizeof(int), hipMemcpyDeviceToHost, w.stream));
+    ADB_HIP(hipMemcpyAsync(&h[2], off + nb, sizeof(int), hipMemcpyDeviceToHost, w.stream));
+    ADB_HIP(hipStreamSynchronize(w.stream));
+    out.n_pt = h[0]; out.n_ee = h[1] - h[0]; out.n_vg = h[2] - h[1];
 }
 
 double ccd_toi(World& w, const double* X0, const double* X1, int* status) {

@@ -121,8 +121,7 @@ struct World {
     ContactSet cs, cs_new;
     ContactSet cand;                 // line search: candidates of the search segment
     bool ls_cand = true;             // trial sets filtered from them (ADB_LS_CANDIDATES=0: full passes)
-    DBuf<char> cflags;               // candidate / vertex flags of a trial set
-    DBuf<int> vsel;                  // selected VG vertices
+    DBuf<int> vsel;                  // trial sets: kept items per block and their scan
     // ---- contact terms (per PT/EE contact, then per VG contact)
     DBuf<double> crec;               // REC per PT/EE contact: compact body-space record
     DBuf<int> ctype;                 // distance type per contact
