@@ -761,6 +761,7 @@ struct TrialArgs {
     const int* voff; const int* foff; const int* eoff; const int* face; const int* edge; const int* vbody;
     const double* X; double d_hat, ground_y;
     int* bcount;                       // kept items per block (pass 1), their exclusive scan (pass 2)
+    unsigned long long* keep;          // pass 1's test results, one bit per item (a word per wavefront)
     int4* out_pt; int4* out_ee; int2* out_vg;
 };
 __device__ __forceinline__ bool trial_keep(const TrialArgs& a, int range, int i) {
@@ -790,15 +791,20 @@ __global__ __launch_bounds__(256) void k_trial_set(TrialArgs a) {
     int b = blockIdx.x, range = 0, n = a.n_pt;
     if (b >= a.nb_pt) { b -= a.nb_pt; range = 1; n = a.n_ee; if (b >= a.nb_ee) { b -= a.nb_ee; range = 2; n = a.V; } }
     const int i = b * 256 + threadIdx.x;
-    const bool keep = i < n && (range < 2 || a.ground) && trial_keep(a, range, i);
-    const unsigned long long m = __ballot(keep);
     const int wv = threadIdx.x / 64, lane = __lane_id();
-    if (lane == 0) wcnt[wv] = __popcll(m);
-    __syncthreads();
+    unsigned long long m;
     if (!WRITE) {
+        const bool keep = i < n && (range < 2 || a.ground) && trial_keep(a, range, i);
+        m = __ballot(keep);
+        if (lane == 0) { wcnt[wv] = __popcll(m); a.keep[4 * (size_t)blockIdx.x + wv] = m; }
+        __syncthreads();
         if (threadIdx.x == 0) a.bcount[blockIdx.x] = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
         return;
     }
+    m = a.keep[4 * (size_t)blockIdx.x + wv];
+    if (lane == 0) wcnt[wv] = __popcll(m);
+    __syncthreads();
+    const bool keep = (m >> lane) & 1ull;
     if (!keep) return;
     int pos = a.bcount[blockIdx.x] + __popcll(m & ((1ull << lane) - 1ull));
     for (int k = 0; k < wv; k++) pos += wcnt[k];
@@ -826,7 +832,8 @@ void trial_set(World& w, const double* X, const ContactSet& cand, ContactSet& ou
     w.vsel.reserve(2 * (size_t)nb + 2);
     int* cnt = w.vsel.p;                      // [0, nb]: block counts; [nb + 1, 2 nb + 2): their exclusive scan
     int* off = w.vsel.p + nb + 1;
-    a.bcount = cnt;
+    w.keepbits.reserve(4 * (size_t)std::max(nb, 1));
+    a.bcount = cnt; a.keep = w.keepbits.p;
     ProfScope ps(w, PK_NARROW_SET, 0.0);
     if (nb > 0) {
         hipLaunchKernelGGL(k_trial_set<false>, dim3(nb), dim3(256), 0, w.stream, a);

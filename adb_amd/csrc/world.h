@@ -122,6 +122,7 @@ struct World {
     ContactSet cand;                 // line search: candidates of the search segment
     bool ls_cand = true;             // trial sets filtered from them (ADB_LS_CANDIDATES=0: full passes)
     DBuf<int> vsel;                  // trial sets: kept items per block and their scan
+    DBuf<unsigned long long> keepbits;   // trial sets: test result bits
     // ---- contact terms (per PT/EE contact, then per VG contact)
     DBuf<double> crec;               // REC per PT/EE contact: compact body-space record
     DBuf<int> ctype;                 // distance type per contact

@@ -294,7 +294,7 @@ static void free_world(World& w) {
     w.bcnt.free_(); w.sh_cnt.free_(); w.sh_start.free_(); w.sh_ent.free_(); w.sh_rng.free_(); w.sh_total.free_();
     w.akeys.free_(); w.akeys_s.free_(); w.tkeys.free_(); w.tkeys_s.free_();
     for (ContactSet* c : {&w.cs, &w.cs_new, &w.cand}) { c->pt.free_(); c->ee.free_(); c->vg.free_(); }
-    w.vsel.free_();
+    w.vsel.free_(); w.keepbits.free_();
     for (auto& r : w.prof.pending) { w.prof.pool.push_back(r.a); w.prof.pool.push_back(r.b); }
     w.prof.pending.clear();
     for (hipEvent_t e : w.prof.pool) (void)hipEventDestroy(e);
