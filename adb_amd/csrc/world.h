@@ -121,6 +121,7 @@ struct World {
     ContactSet cs, cs_new;
     ContactSet cand;                 // line search: candidates of the search segment
     bool ls_cand = true;             // trial sets filtered from them (ADB_LS_CANDIDATES=0: full passes)
+    int ls_prev = 2;                 // trials of the previous line search
     DBuf<int> vsel;                  // trial sets: kept items per block and their scan
     DBuf<unsigned long long> keepbits;   // trial sets: test result bits
     // ---- contact terms (per PT/EE contact, then per VG contact)

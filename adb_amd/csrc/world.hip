@@ -99,9 +99,14 @@ static int step(World& w, adb_step_stats* st) {
             const double E0 = sl[0] + sl[1];
             const double qdg = sl[2];
             int trials = 0;
-            // trial sets from the candidates of the whole search segment q -> q + dq
-            // (vertex positions are affine in q, so every trial point lies on it)
-            const bool cand = w.ls_cand && (w.P.iaabb & 1);
+            // Trial sets from the candidates of the search segment q -> q + alpha dq
+            // (vertex positions are affine in q, so every later trial point lies on
+            // it): one swept pass, then an exact filter per trial.  The swept pass
+            // costs more than one plain pass, so when the previous search took one
+            // trial the first trial gets a plain pass and the candidates are built
+            // only if it is rejected.  Both give the same sets.
+            const bool cand_ok = w.ls_cand && (w.P.iaabb & 1);
+            bool cand = cand_ok && w.ls_prev > 1;
             if (cand) {
                 q_axpy(w, w.q.p, w.dq.p, 1.0, w.qtrial.p);
                 transform_positions(w, w.qtrial.p, w.X1.p);
@@ -110,6 +115,10 @@ static int step(World& w, adb_step_stats* st) {
             while (true) {
                 q_axpy(w, w.q.p, w.dq.p, alpha, w.qtrial.p);
                 transform_positions(w, w.qtrial.p, w.X1.p);
+                if (!cand && cand_ok && trials > 0) {
+                    segment_candidates(w, w.X.p, w.X1.p, w.cand);
+                    cand = true;
+                }
                 if (cand) trial_set(w, w.X1.p, w.cand, w.cs_new);
                 else collision_set(w, w.X1.p, w.cs_new);
                 energy_barrier_inertia_async(w, w.qtrial.p, w.X1.p, w.cs_new, 0);
@@ -123,6 +132,7 @@ static int step(World& w, adb_step_stats* st) {
                 if (trials >= w.P.max_ls_iter) { status = ADB_E_NOT_CONVERGED; w.err = "line search exhausted"; break; }
             }
             S.ls_trials += trials;
+            w.ls_prev = trials;
             alpha *= 2;
             if (status != ADB_OK) break;
         } else {
