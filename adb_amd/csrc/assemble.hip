@@ -195,6 +195,41 @@ struct PairArgs {
 
 // record doubles a thread stages per round (RSLOT records of REC_USED / 2 double2 over 256 threads)
 constexpr int STAGE_PER_T = (RSLOT * (REC_USED / 2) + 255) / 256;
+// One contact's share of a lane's 12 block entries (column (be, cc) of the
+// block (s1, s2)): sum_ab Z[a][s1] Z[b][s2]^T (x) PK_ab + G[s1] G[s2]^T (x) F,
+// with the Helmert row count NA of the contact's class as a constant, so a
+// 2-point contact evaluates 1 of the 9 row pairs (terms in the same order).
+template <int NA>
+__device__ __forceinline__ void pair_terms(const double* R, int s1, int s2, int be, int cc, double (&acc)[12]) {
+    constexpr int D = 3 * NA;
+    double tt[NA][3];
+#pragma unroll
+    for (int aa = 0; aa < NA; aa++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            double sum = 0.0;
+#pragma unroll
+            for (int bb = 0; bb < NA; bb++) sum += R[R_PK + (3 * aa + r) * D + 3 * bb + cc] * R[R_Z + 8 * bb + 4 * s2 + be];
+            tt[aa][r] = sum;
+        }
+    const double g2 = R[R_G + 4 * s2 + be];
+    const double F[3] = {R[R_F + cc], R[R_F + 3 + cc], R[R_F + 6 + cc]};
+#pragma unroll
+    for (int al = 0; al < 4; al++) {
+        const double gg = R[R_G + 4 * s1 + al] * g2;
+        double z[NA];
+#pragma unroll
+        for (int aa = 0; aa < NA; aa++) z[aa] = R[R_Z + 8 * aa + 4 * s1 + al];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double t = z[0] * tt[0][c];
+#pragma unroll
+            for (int aa = 1; aa < NA; aa++) t = t + z[aa] * tt[aa][c];
+            acc[3 * al + c] += t + gg * F[c];
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_pair_reduce(PairArgs a, const int* __restrict__ chunk_pair, double* __restrict__ part) {
     // two record buffers (the next round is loaded while this one is used);
     // at the end the same space holds the per-slot column sums
@@ -255,32 +290,12 @@ __global__ __launch_bounds__(256) void k_pair_reduce(PairArgs a, const int* __re
         const int k = r0 - beg + slot;
         if (use && r0 + slot < end && scls[k] > 0) {
             const double* R = lds + (buf * RSLOT + slot) * REC_USED;
-            const int nA = scls[k], D = 3 * nA;
+            const int nA = scls[k];
             const int slo = sside[k], shi = 1 - slo;
             const int s1 = blk == 1 ? shi : slo, s2 = blk == 0 ? slo : shi;
-            double tt[3][3];
-#pragma unroll
-            for (int aa = 0; aa < 3; aa++)
-#pragma unroll
-                for (int r = 0; r < 3; r++) {
-                    double sum = 0.0;
-#pragma unroll
-                    for (int bb = 0; bb < 3; bb++)
-                        if (aa < nA && bb < nA) sum += R[R_PK + (3 * aa + r) * D + 3 * bb + cc] * R[R_Z + 8 * bb + 4 * s2 + be];
-                    tt[aa][r] = sum;
-                }
-            const double g2 = R[R_G + 4 * s2 + be];
-            const double F0 = R[R_F + cc], F1 = R[R_F + 3 + cc], F2 = R[R_F + 6 + cc];
-#pragma unroll
-            for (int al = 0; al < 4; al++) {
-                const double gg = R[R_G + 4 * s1 + al] * g2;
-                double z[3];
-#pragma unroll
-                for (int aa = 0; aa < 3; aa++) z[aa] = aa < nA ? R[R_Z + 8 * aa + 4 * s1 + al] : 0.0;
-                acc[3 * al + 0] += (z[0] * tt[0][0] + z[1] * tt[1][0] + z[2] * tt[2][0]) + gg * F0;
-                acc[3 * al + 1] += (z[0] * tt[0][1] + z[1] * tt[1][1] + z[2] * tt[2][1]) + gg * F1;
-                acc[3 * al + 2] += (z[0] * tt[0][2] + z[1] * tt[1][2] + z[2] * tt[2][2]) + gg * F2;
-            }
+            if (nA == 1) pair_terms<1>(R, s1, s2, be, cc, acc);
+            else if (nA == 2) pair_terms<2>(R, s1, s2, be, cc, acc);
+            else pair_terms<3>(R, s1, s2, be, cc, acc);
             if (blk < 2) gacc += R[R_GRAD + 12 * s1 + col];
         }
         if (more) store(buf ^ 1);
