@@ -322,8 +322,13 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
     int it = 0, slot = 0;
     bool have_prev = false;
     const int max_it = std::max(w.P.pcg_max_iter, 0);
+    // Near the iteration count of the previous solve (Newton steps change the
+    // system little) the batches shrink to 4 iterations, so the stop test,
+    // read one batch late, leaves few queued iterations that return at once.
+    const int pred = w.pcg_prev_its;
     while (it < max_it) {
-        const int nb = std::min(BATCH, max_it - it);
+        int nb = std::min(BATCH, max_it - it);
+        if (pred > 0 && it + BATCH > pred - 8) nb = std::min(nb, 4);
         if (use_graph && nb == BATCH) ADB_HIP(hipGraphLaunch(w.pcg_graph, w.stream));
         else enqueue_batch(w, nb, w.prof.on);
         it += nb;
@@ -350,7 +355,8 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
     ADB_HIP(hipStreamSynchronize(w.stream));
     const double* h = w.h_scal;
     const int stop = (int)h[S_STOP];
-    if (iters) *iters = stop == RUN ? it : (int)h[S_ITERS];
+    w.pcg_prev_its = stop == RUN ? it : (int)h[S_ITERS];
+    if (iters) *iters = w.pcg_prev_its;
     if (stop == NONFINITE) return ADB_E_NAN;
     ADB_HIP(hipMemcpyAsync(dq_out, w.pcg_x.p, n * sizeof(double), hipMemcpyDeviceToDevice, w.stream));
     if (stop == RUN) return ADB_E_NOT_CONVERGED;
