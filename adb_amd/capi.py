@@ -225,4 +225,4 @@ EXPORTED_SYMBOLS = [
 # kernel ids of adb_profile_read (enum adb_prof_kernel)
 PROF_KERNELS = ["transform", "body_box", "body_pairs", "narrow_set", "vg_scan", "sort", "narrow_ccd", "vg_toi",
                 "contact", "vg_contact", "inertia", "asm_reduce", "block_chol", "spmv", "pcg_update", "pcg_dir",
-                "energy", "contact_classify", "contact_group", "hash_build", "hash_query"]
+                "energy", "contact_classify", "contact_group", "hash_build", "hash_query", "contact_pk"]

@@ -766,8 +766,9 @@ void assemble_system(World& w) {
     a.vg = w.cs.vg.p; a.vg_start = w.vg_start.p; a.vgK = w.vgK.p; a.voff = w.voff.p; a.xbar = w.xbar.p;
     a.diag_blk = w.diag_blk.p;
     a.Hval = w.Hval.p; a.grad = w.grad.p;
-    if (w.side_pending) {   // records of the 4-point contacts (side stream)
+    if (w.side_pending) {   // records of the 4-point contacts and 3-point projections (side streams)
         ADB_HIP(hipStreamWaitEvent(w.stream, w.ev_side[1], 0));
+        ADB_HIP(hipStreamWaitEvent(w.stream, w.ev_pk, 0));
         w.side_pending = false;
     }
     if (max_chunks > 0) {

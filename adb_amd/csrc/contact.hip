@@ -1246,7 +1246,9 @@ void contact_terms(World& w) {
             // the assembly waits for them before its reduction
             if (!w.stream2) {
                 ADB_HIP(hipStreamCreate(&w.stream2));
+                ADB_HIP(hipStreamCreate(&w.stream3));
                 for (hipEvent_t& e : w.ev_side) ADB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                ADB_HIP(hipEventCreateWithFlags(&w.ev_pk, hipEventDisableTiming));
             }
             ADB_HIP(hipEventRecord(w.ev_side[0], w.stream));
             ADB_HIP(hipStreamWaitEvent(w.stream2, w.ev_side[0], 0));
@@ -1258,19 +1260,30 @@ void contact_terms(World& w) {
                 ADB_HIP(hipGetLastError());
             }
             ADB_HIP(hipEventRecord(w.ev_side[1], w.stream2));
+            // the 6x6 projections of the 3-point contacts (VALU-bound) on a third
+            // stream, beside the 2- and 3-point record kernels (bound by their
+            // scattered loads and stores); they write disjoint record fields
+            ADB_HIP(hipStreamWaitEvent(w.stream3, w.ev_side[0], 0));
+            {
+                // per 3-point contact: ids, 4 points, type and distance in; PK (36 doubles) out
+                ProfScope ps(w, PK_CONTACT_PK, 0.0, w.stream3);
+                ps.counts(snap, 0.0, 440.0, 0.0, 0.0);
+                hipLaunchKernelGGL(k_contact_pk2, dim3(std::min((n + 63) / 64, 16 * w.num_cu)), dim3(64), 0, w.stream3, a);
+                ADB_HIP(hipGetLastError());
+            }
+            ADB_HIP(hipEventRecord(w.ev_pk, w.stream3));
             w.side_pending = true;
         }
         {
             // per contact: list, ids, type, distance, X / Xs / xbar of 4 points in;
-            // record (D = 3: 59 doubles, D = 6: 94), lagged basis (10) and lambda out
+            // record (D = 3: 59 doubles; D = 6: 58 here, PK on the third stream),
+            // lagged basis (10) and lambda out
             ProfScope ps(w, PK_CONTACT, 0.0);
-            ps.counts(snap, 904.0, 1184.0, 0.0, 0.0);
+            ps.counts(snap, 904.0, 896.0, 0.0, 0.0);
             const int grid = std::min((n + 255) / 256, 8 * w.num_cu);
             hipLaunchKernelGGL(k_contact_small<1>, dim3(grid), dim3(256), 0, w.stream, a);
             ADB_HIP(hipGetLastError());
             hipLaunchKernelGGL(k_contact_scalars2, dim3(grid), dim3(256), 0, w.stream, a);
-            ADB_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_contact_pk2, dim3(std::min((n + 63) / 64, 16 * w.num_cu)), dim3(64), 0, w.stream, a);
             ADB_HIP(hipGetLastError());
         }
     }

@@ -61,7 +61,7 @@ constexpr int R_D = 0, R_F = 1, R_G = 10, R_Z = 18, R_GRAD = 42, R_PK = 66;
 enum ProfKernel {
     PK_TRANSFORM = 0, PK_BODY_BOX, PK_BODY_PAIRS, PK_NARROW_SET, PK_VG_SCAN, PK_SORT, PK_NARROW_CCD, PK_VG_TOI,
     PK_CONTACT, PK_VG_CONTACT, PK_INERTIA, PK_ASM_REDUCE, PK_BLOCK_CHOL, PK_SPMV, PK_PCG_UPDATE, PK_PCG_DIR,
-    PK_ENERGY, PK_CONTACT_CLASSIFY, PK_CONTACT_GROUP, PK_HASH_BUILD, PK_HASH_QUERY, PK_COUNT
+    PK_ENERGY, PK_CONTACT_CLASSIFY, PK_CONTACT_GROUP, PK_HASH_BUILD, PK_HASH_QUERY, PK_CONTACT_PK, PK_COUNT
 };
 struct Prof {
     bool on = false;
@@ -91,6 +91,8 @@ struct World {
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;   // side stream: 4-point contacts overlap the rest of the stage
     hipEvent_t ev_side[2] = {nullptr, nullptr};   // [0] fork, [1] join
+    hipStream_t stream3 = nullptr;   // third stream: 3-point projections
+    hipEvent_t ev_pk = nullptr;      // its join
     bool side_pending = false;
     std::string err;
 

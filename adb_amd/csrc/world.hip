@@ -322,6 +322,10 @@ static void free_world(World& w) {
     for (hipEvent_t& e : w.ev_side) { if (e) (void)hipEventDestroy(e); e = nullptr; }
     if (w.stream2) (void)hipStreamDestroy(w.stream2);
     w.stream2 = nullptr;
+    if (w.ev_pk) (void)hipEventDestroy(w.ev_pk);
+    w.ev_pk = nullptr;
+    if (w.stream3) (void)hipStreamDestroy(w.stream3);
+    w.stream3 = nullptr;
     if (w.stream) (void)hipStreamDestroy(w.stream);
     w.stream = nullptr;
 }

@@ -121,9 +121,9 @@ def max_vertex_deviation(sc, qa, qb):
 
 
 # kernels launched once per launch of each profiling scope
-KERNEL_OF_SCOPE = {"asm_reduce": ["k_pair_reduce"], "narrow_set": ["k_pair_narrow<0"], "block_chol": ["k_group_inv"],
-                   "contact_group": ["k_contact_c3"],
-                   "contact": ["k_contact_small<1>", "k_contact_scalars2", "k_contact_pk2"],
+KERNEL_OF_SCOPE = {"asm_reduce": ["k_pair_reduce"], "narrow_set": ["k_pair_narrow<0", "k_pair_narrow<2", "k_trial_set"],
+                   "block_chol": ["k_group_inv"], "contact_group": ["k_contact_c3"],
+                   "contact": ["k_contact_small<1>", "k_contact_scalars2"], "contact_pk": ["k_contact_pk2"],
                    "spmv": ["k_pcg_spmv"], "pcg_update": ["k_pcg_update"], "contact_classify": ["k_contact_classify"]}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hbm_traffic_2000_icospheres.json")
 
@@ -274,7 +274,7 @@ def main():
         if os.path.exists(TRAFFIC_JSON):
             with open(TRAFFIC_JSON) as fh:
                 tr = json.load(fh)
-            if tr.get("kernel_scope") == k:
+            if tr.get("kernel_scope") == k and tr.get("kernels") == KERNEL_OF_SCOPE.get(k):
                 roofline["traffic"] = round(tr["traffic_bytes_per_launch"], 0)
                 roofline["traffic_source"] = os.path.relpath(TRAFFIC_JSON, ROOT)
 

@@ -211,7 +211,7 @@ enum adb_prof_kernel {
     ADB_PK_TRANSFORM = 0, ADB_PK_BODY_BOX, ADB_PK_BODY_PAIRS, ADB_PK_NARROW_SET, ADB_PK_VG_SCAN, ADB_PK_SORT,
     ADB_PK_NARROW_CCD, ADB_PK_VG_TOI, ADB_PK_CONTACT, ADB_PK_VG_CONTACT, ADB_PK_INERTIA, ADB_PK_ASM_REDUCE,
     ADB_PK_BLOCK_CHOL, ADB_PK_SPMV, ADB_PK_PCG_UPDATE, ADB_PK_PCG_DIR, ADB_PK_ENERGY,
-    ADB_PK_CONTACT_CLASSIFY, ADB_PK_CONTACT_GROUP, ADB_PK_HASH_BUILD, ADB_PK_HASH_QUERY, ADB_PK_COUNT
+    ADB_PK_CONTACT_CLASSIFY, ADB_PK_CONTACT_GROUP, ADB_PK_HASH_BUILD, ADB_PK_HASH_QUERY, ADB_PK_CONTACT_PK, ADB_PK_COUNT
 };
 int adb_profile_enable(adb_world* w, int on);
 int adb_profile_read(adb_world* w, int kernel, double* total_ms, long long* launches, double* bytes);
