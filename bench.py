@@ -126,6 +126,7 @@ KERNEL_OF_SCOPE = {"asm_reduce": ["k_pair_reduce"], "narrow_set": ["k_pair_narro
                    "contact": ["k_contact_small<1>", "k_contact_scalars2"], "contact_pk": ["k_contact_pk2"],
                    "spmv": ["k_pcg_spmv"], "pcg_update": ["k_pcg_update"], "contact_classify": ["k_contact_classify"]}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hbm_traffic_2000_icospheres.json")
+SIDE_SCOPES = {"contact_group", "contact_pk"}      # timing scopes on the world's side streams
 
 
 def pmc_traffic(fetch_csv, write_csv, prof_json, scope, copy_mb):
@@ -258,9 +259,13 @@ def main():
                  "newton_iters": t2["newton"], "pcg_iters": t2["pcg"]}
 
     steps_per_s = args.steps / elapsed
-    # dominant kernel by device time among those with an algorithmic byte count
+    # dominant kernel by device time among those with an algorithmic byte count,
+    # on the world's main stream: the 4-point contacts and the 3-point
+    # projections run on side streams concurrently with the main stream's
+    # contact kernels, so their event times include sharing the GPU and are
+    # not on the step's critical path (reported beside the roofline)
     ranked = sorted(prof.items(), key=lambda kv: -kv[1]["ms"])
-    dom = next(((k, v) for k, v in ranked if v["bytes"] > 0 and v["launches"] > 0), None)
+    dom = next(((k, v) for k, v in ranked if v["bytes"] > 0 and v["launches"] > 0 and k not in SIDE_SCOPES), None)
     roofline = None
     if dom:
         k, v = dom
@@ -269,7 +274,8 @@ def main():
         gbs = bytes_per_launch / (avg_ms * 1e-3) / 1e9
         roofline = {"kernel": k, "bound": "hbm", "achieved": round(gbs, 1), "peak": MI355X_HBM_GBS,
                     "unit": "GB/s", "frac": round(gbs / MI355X_HBM_GBS, 4), "traffic": None,
-                    "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch, "launches": v["launches"]}
+                    "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch, "launches": v["launches"],
+                    "concurrent_side_stream_ms": {q: round(prof[q]["ms"], 3) for q in sorted(SIDE_SCOPES) if q in prof}}
         # HBM bytes per launch measured by rocprofv3 --pmc on this workload (committed summary)
         if os.path.exists(TRAFFIC_JSON):
             with open(TRAFFIC_JSON) as fh:
