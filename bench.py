@@ -280,7 +280,9 @@ def main():
         if os.path.exists(TRAFFIC_JSON):
             with open(TRAFFIC_JSON) as fh:
                 tr = json.load(fh)
-            if tr.get("kernel_scope") == k and tr.get("kernels") == KERNEL_OF_SCOPE.get(k):
+            # only for the workload it was measured on (same scope, kernels and launches)
+            if (tr.get("kernel_scope") == k and tr.get("kernels") == KERNEL_OF_SCOPE.get(k)
+                    and tr.get("launches") == v["launches"]):
                 roofline["traffic"] = round(tr["traffic_bytes_per_launch"], 0)
                 roofline["traffic_source"] = os.path.relpath(TRAFFIC_JSON, ROOT)
 
