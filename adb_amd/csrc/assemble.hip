@@ -606,15 +606,29 @@ __global__ __launch_bounds__(256) void k_group_inv(const int* __restrict__ list,
 // most GMAX bodies, so the strongest couplings land inside the inverted
 // diagonal sub-matrices; every connected component of at most GMAX bodies
 // ends up as one group (the preconditioned system then decouples exactly).
+// The pair lists go to pinned host memory before the assembly reduction is
+// queued (groups_fetch), so the host builds the groups while it runs.
+static void groups_fetch(World& w) {
+    const int np = w.n_pairs;
+    if (3 * (size_t)np + 1 > w.h_pairs_cap) {
+        if (w.h_pairs) ADB_HIP(hipHostFree(w.h_pairs));
+        w.h_pairs_cap = 3 * (size_t)np + 1 + (size_t)np / 2 + 64;
+        ADB_HIP(hipHostMalloc((void**)&w.h_pairs, w.h_pairs_cap * sizeof(int), hipHostMallocDefault));
+    }
+    if (!w.ev_pairs) ADB_HIP(hipEventCreateWithFlags(&w.ev_pairs, hipEventDisableTiming));
+    if (np > 0) {
+        ADB_HIP(hipMemcpyAsync(w.h_pairs, w.pair_lo.p, np * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipMemcpyAsync(w.h_pairs + np, w.pair_hi.p, np * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipMemcpyAsync(w.h_pairs + 2 * (size_t)np, w.pair_start.p, (np + 1) * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+    }
+    ADB_HIP(hipEventRecord(w.ev_pairs, w.stream));
+}
 static void build_groups(World& w) {
     const int B = w.B, np = w.n_pairs;
-    std::vector<int> plo(np), phi(np), ps(np + 1);
-    if (np > 0) {
-        ADB_HIP(hipMemcpyAsync(plo.data(), w.pair_lo.p, np * sizeof(int), hipMemcpyDeviceToHost, w.stream));
-        ADB_HIP(hipMemcpyAsync(phi.data(), w.pair_hi.p, np * sizeof(int), hipMemcpyDeviceToHost, w.stream));
-        ADB_HIP(hipMemcpyAsync(ps.data(), w.pair_start.p, (np + 1) * sizeof(int), hipMemcpyDeviceToHost, w.stream));
-        ADB_HIP(hipStreamSynchronize(w.stream));
-    }
+    ADB_HIP(hipEventSynchronize(w.ev_pairs));
+    const int* plo = w.h_pairs;
+    const int* phi = w.h_pairs + np;
+    const int* ps = w.h_pairs + 2 * (size_t)np;
     // pairs by decreasing contact count, ties by index (counting sort)
     int wmax = 0;
     for (int p = 0; p < np; p++) wmax = std::max(wmax, ps[p + 1] - ps[p]);
@@ -766,6 +780,7 @@ void assemble_system(World& w) {
     a.vg = w.cs.vg.p; a.vg_start = w.vg_start.p; a.vgK = w.vgK.p; a.voff = w.voff.p; a.xbar = w.xbar.p;
     a.diag_blk = w.diag_blk.p;
     a.Hval = w.Hval.p; a.grad = w.grad.p;
+    groups_fetch(w);
     if (w.side_pending) {   // records of the 4-point contacts and 3-point projections (side streams)
         ADB_HIP(hipStreamWaitEvent(w.stream, w.ev_side[1], 0));
         ADB_HIP(hipStreamWaitEvent(w.stream, w.ev_pk, 0));

@@ -172,6 +172,9 @@ struct World {
     DBuf<double> pcg_x, pcg_r, pcg_z, pcg_p, pcg_p2, pcg_Ap, partial, scal;
     DBuf<int> pcg_k;                 // iteration base of the queued PCG batch
     int pcg_prev_its = 0;            // iterations of the previous solve
+    int* h_pairs = nullptr;          // pinned host copy of the pair lists (lo, hi, start)
+    size_t h_pairs_cap = 0;
+    hipEvent_t ev_pairs = nullptr;   // their copy is done
     bool use_graphs = true;          // ADB_PCG_GRAPH=0 turns graph capture off
     hipGraphExec_t pcg_graph = nullptr;   // one captured PCG batch
     const void* pcg_graph_key[17] = {};   // buffer addresses the graph was captured with

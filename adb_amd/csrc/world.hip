@@ -318,6 +318,10 @@ static void free_world(World& w) {
     w.grp_n.free_();
     if (w.h_scal) (void)hipHostFree(w.h_scal);
     w.h_scal = nullptr;
+    if (w.h_pairs) (void)hipHostFree(w.h_pairs);
+    w.h_pairs = nullptr; w.h_pairs_cap = 0;
+    if (w.ev_pairs) (void)hipEventDestroy(w.ev_pairs);
+    w.ev_pairs = nullptr;
     for (hipEvent_t& e : w.pcg_ev) { if (e) (void)hipEventDestroy(e); e = nullptr; }
     for (hipEvent_t& e : w.ev_side) { if (e) (void)hipEventDestroy(e); e = nullptr; }
     if (w.stream2) (void)hipStreamDestroy(w.stream2);
