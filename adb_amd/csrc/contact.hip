@@ -757,7 +757,9 @@ __device__ __forceinline__ void contact_small(const WorldArgs& a, int cid) {
 // Class 3 contacts, one thread each: the 9x9 Helmert-coordinate matrix
 // K = (C (x) I3) Mred (C^T (x) I3) + mollifier terms (C = h4 S^T, Cm the same
 // for y = p1 - p0, z = p3 - p2) is built one component pair at a time into
-// LDS (element-major, X[e][lane], so lanes never conflict); cyclic Jacobi then
+// LDS (upper triangle only, packed element-major, X[e][lane], so lanes never
+// conflict; 23 KB per 64-lane workgroup lets four fit a CU's 160 KB, one wave
+// per SIMD, which the registers allow); cyclic Jacobi then
 // rotates its packed upper triangle in registers while the eigenvectors
 // accumulate there too.
 __host__ __device__ constexpr int P9(int p, int q) {
@@ -837,7 +839,8 @@ __device__ __forceinline__ void contact_c3(const WorldArgs& a, int cid, double (
 #pragma unroll
             for (int q = 0; q < 3; q++)
 #pragma unroll
-                for (int b = 0; b < 3; b++) V[9 * (3 * q + i) + 3 * b + j][lane] = K[q][b];
+                for (int b = 0; b < 3; b++)
+                    if (3 * q + i <= 3 * b + j) V[P9(3 * q + i, 3 * b + j)][lane] = K[q][b];
         }
         // everything but PK: Z, friction (F, G, Tk, lambda) and the gradient
         const int nside0 = G.is_ee ? 2 : 1;
@@ -880,7 +883,7 @@ __device__ __forceinline__ void contact_c3(const WorldArgs& a, int cid, double (
 #pragma unroll
     for (int p = 0; p < 9; p++)
 #pragma unroll
-        for (int q = p; q < 9; q++) A[P9(p, q)] = V[9 * p + q][lane];
+        for (int q = p; q < 9; q++) A[P9(p, q)] = V[P9(p, q)][lane];
     if (a.kp.psd) {
         double Vr[81];
 #pragma unroll
@@ -947,7 +950,7 @@ __device__ __forceinline__ void contact_c3(const WorldArgs& a, int cid, double (
     }
 }
 __global__ __launch_bounds__(64) void k_contact_c3(WorldArgs a) {
-    __shared__ double V[81][64];
+    __shared__ double V[45][64];
     const int n3 = a.counts[2];
     for (int p = blockIdx.x * 64 + threadIdx.x; p < n3; p += gridDim.x * 64) contact_c3(a, a.list3[p], V, threadIdx.x);
 }
