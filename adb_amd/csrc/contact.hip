@@ -615,15 +615,16 @@ __device__ __forceinline__ double fric_F(const Fric& f, int i, int j) {
     return f.M[0] * c0i * c0j + f.M[1] * (c0i * c1j + c1i * c0j) + f.M[2] * c1i * c1j;
 }
 
-// Record fields shared by the per-thread and per-group paths.
-// Friction part: F, G, the lagged basis (gm_k, c0, c1: TKN doubles) and lambda.
-__device__ __forceinline__ void write_fric(const WorldArgs& a, int cid, int nside0, const Fric& f, const double w[4][4]) {
-    double* R = a.rec + REC * (size_t)cid;
+// Record fields shared by the per-thread and per-group paths, emitted as
+// put(field, value).  Friction part: F and G of the record; the lagged basis
+// (gm_k, c0, c1: TKN doubles); lambda is stored by the caller.
+template <class Put>
+__device__ __forceinline__ void fric_fields(const Fric& f, int nside0, const double w[4][4], Put put) {
     const double gmv[4] = {f.gm0, f.gm1, f.gm2, f.gm3};
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++) R[R_F + 3 * i + j] = fric_F(f, i, j);
+        for (int j = 0; j < 3; j++) put(R_F + 3 * i + j, fric_F(f, i, j));
 #pragma unroll
     for (int s = 0; s < 2; s++)
 #pragma unroll
@@ -632,18 +633,20 @@ __device__ __forceinline__ void write_fric(const WorldArgs& a, int cid, int nsid
 #pragma unroll
             for (int k = 0; k < 4; k++)
                 if ((k < nside0) == (s == 0)) g += gmv[k] * w[k][al];
-            R[R_G + 4 * s + al] = g;
+            put(R_G + 4 * s + al, g);
         }
-    double* tk = a.tk + TKN * (size_t)cid;
+}
+template <class Put>
+__device__ __forceinline__ void tk_fields(const Fric& f, Put put) {
+    const double gmv[4] = {f.gm0, f.gm1, f.gm2, f.gm3};
 #pragma unroll
-    for (int k = 0; k < 4; k++) tk[k] = gmv[k];
+    for (int k = 0; k < 4; k++) put(k, gmv[k]);
 #pragma unroll
-    for (int r = 0; r < 3; r++) { tk[4 + r] = cmp(f.c0, r); tk[7 + r] = cmp(f.c1, r); }
-    a.lam[cid] = f.lam;
+    for (int r = 0; r < 3; r++) { put(4 + r, cmp(f.c0, r)); put(7 + r, cmp(f.c1, r)); }
 }
 // Body-space gradient from per-point vertex gradients gv_k (friction included).
-__device__ __forceinline__ void write_grad(const WorldArgs& a, int cid, int nside0, const double w[4][4], const d3 gv[4]) {
-    double* R = a.rec + REC * (size_t)cid;
+template <class Put>
+__device__ __forceinline__ void grad_fields(int nside0, const double w[4][4], const d3 gv[4], Put put) {
 #pragma unroll
     for (int s = 0; s < 2; s++)
 #pragma unroll
@@ -654,8 +657,53 @@ __device__ __forceinline__ void write_grad(const WorldArgs& a, int cid, int nsid
 #pragma unroll
                 for (int k = 0; k < 4; k++)
                     if ((k < nside0) == (s == 0)) g += w[k][al] * cmp(gv[k], r);
-                R[R_GRAD + 12 * s + 3 * al + r] = g;
+                put(R_GRAD + 12 * s + 3 * al + r, g);
             }
+}
+__device__ __forceinline__ void write_fric(const WorldArgs& a, int cid, int nside0, const Fric& f, const double w[4][4]) {
+    double* R = a.rec + REC * (size_t)cid;
+    fric_fields(f, nside0, w, [&](int k, double v) { R[k] = v; });
+    double* tk = a.tk + TKN * (size_t)cid;
+    tk_fields(f, [&](int k, double v) { tk[k] = v; });
+    a.lam[cid] = f.lam;
+}
+__device__ __forceinline__ void write_grad(const WorldArgs& a, int cid, int nside0, const double w[4][4], const d3 gv[4]) {
+    double* R = a.rec + REC * (size_t)cid;
+    grad_fields(nside0, w, gv, [&](int k, double v) { R[k] = v; });
+}
+
+// Record sections staged per wavefront.  A contact kernel with one thread per
+// record would store each field with 64 lanes on 64 different cache lines;
+// instead lane l puts field k of a section into S[k][l] (element-major, row
+// padded to 65 doubles), and stage_flush stores the section's NF consecutive
+// doubles of each of the wavefront's records with consecutive lanes on
+// consecutive (record, field) pairs: a store instruction then covers a few
+// lines.  Lanes past the end of the contact list compute a duplicate of the
+// last contact (nothing of theirs is stored), so every flush runs with the
+// whole wavefront active.
+constexpr int SNF = 24, SLD = 65;
+struct Stage {
+    double (*S)[SLD];
+    int lane, cid, nvalid;
+};
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// base[stride * cid_r + off + k] = S[k][r] for records r < nvalid, fields k < NF
+template <int NF>
+__device__ __forceinline__ void stage_flush(const Stage& st, double* base, int stride, int off) {
+    static_assert(NF <= SNF, "section larger than the staging rows");
+    wave_lds_sync();
+#pragma unroll 1
+    for (int it = 0; it < NF; it++) {
+        const int t = it * 64 + st.lane;      // < 64 NF, so r < 64
+        const int r = t / NF, k = t - r * NF;
+        const int c = __shfl(st.cid, r);
+        if (r < st.nvalid) base[(size_t)stride * c + off + k] = st.S[k][r];
+    }
+    wave_lds_sync();
 }
 __device__ __forceinline__ void load_w(const double* xbar, int id0, int id1, int id2, int id3, double w[4][4]) {
     const d3 x0 = ld3(xbar + 3 * (size_t)id0), x1 = ld3(xbar + 3 * (size_t)id1);
@@ -674,8 +722,9 @@ __device__ __forceinline__ d3 pick(d3 p0, d3 p1, d3 p2, d3 p3, int k) {
 // Helmert coordinate y = (x_a - x_b) / sqrt(2) the Hessian is
 // K = 4 B1 I + 8 B2 r r^T with eigenvalues 4 B1 (twice, normal to r) and
 // 4 B1 + 8 B2 d (along r).
-template <int CLS>
-__device__ __forceinline__ void contact_small(const WorldArgs& a, int cid) {
+__device__ __forceinline__ void contact_small(const WorldArgs& a, const Stage& st, bool valid) {
+    const int cid = st.cid, l = st.lane;
+    double (*S)[SLD] = st.S;
     int id0, id1, id2, id3;
     const int is_ee = contact_ids(a, cid, id0, id1, id2, id3);
     const d3 P0 = ld3(a.X + 3 * (size_t)id0), P1 = ld3(a.X + 3 * (size_t)id1);
@@ -686,18 +735,23 @@ __device__ __forceinline__ void contact_small(const WorldArgs& a, int cid) {
     const int nside0 = is_ee ? 2 : 1;
     const double B1 = barrier_d1(d, a.kp.d_hat, a.kp.kappa);
     const double B2 = barrier_d2(d, a.kp.d_hat, a.kp.kappa);
-    double* R = a.rec + REC * (size_t)cid;
     const d3 z = mk(0, 0, 0);
-    // friction first: F, G, Tk, lambda written now; kept: gm_k and
-    // ft = fs (u0 c0 + u1 c1), the friction gradient of point k being gm_k ft
+    double w[4][4];
+    load_w(a.xbar, id0, id1, id2, id3, w);
+    // friction first: D, F, G (fields 0..17), the lagged basis and lambda
+    // stored now; kept: gm_k and ft = fs (u0 c0 + u1 c1), the friction
+    // gradient of point k being gm_k ft
     double gmv[4];
     d3 ft = z;
     {
         Fric f;
         friction_eval(a, is_ee, type, d, B1, P0, P1, P2, P3, id0, id1, id2, id3, f);
-        double w[4][4];
-        load_w(a.xbar, id0, id1, id2, id3, w);
-        write_fric(a, cid, nside0, f, w);
+        S[R_D][l] = rec_code(a, cid, 3);
+        fric_fields(f, nside0, w, [&](int k, double v) { S[k][l] = v; });
+        stage_flush<R_Z>(st, a.rec, REC, 0);
+        tk_fields(f, [&](int k, double v) { S[k][l] = v; });
+        stage_flush<TKN>(st, a.tk, TKN, 0);
+        if (valid) a.lam[cid] = f.lam;
         gmv[0] = f.gm0; gmv[1] = f.gm1; gmv[2] = f.gm2; gmv[3] = f.gm3;
         if (f.on) {
             const double t0 = f.fs * f.u0, t1 = f.fs * f.u1;
@@ -705,9 +759,7 @@ __device__ __forceinline__ void contact_small(const WorldArgs& a, int cid) {
         }
     }
     d3 gv[4];
-    gv[0] = z; gv[1] = z; gv[2] = z; gv[3] = z;
-    static_assert(CLS == 1, "3-point contacts: scalars_c2 / pk_c2");
-    double hz[4][2];      // Helmert weight of point slot k in row a (zero off the contact's slots)
+    double hz[4];         // Helmert weight of point slot k (zero off the contact's slots)
     {
         const int ka = mp.plus[0], kb = mp.minus[0];
         const d3 r = pick(P0, P1, P2, P3, ka) - pick(P0, P1, P2, P3, kb);
@@ -719,40 +771,37 @@ __device__ __forceinline__ void contact_small(const WorldArgs& a, int cid) {
 #pragma unroll
                 for (int j = 0; j < 3; j++) {
                     const double rr = cmp(r, i) * cmp(r, j) / d;
-                    R[R_PK + 3 * i + j] = cl * rr + cp * ((i == j ? 1.0 : 0.0) - rr);
+                    S[3 * i + j][l] = cl * rr + cp * ((i == j ? 1.0 : 0.0) - rr);
                 }
         } else {
 #pragma unroll
             for (int i = 0; i < 3; i++)
 #pragma unroll
-                for (int j = 0; j < 3; j++) R[R_PK + 3 * i + j] = (i == j ? lp : 0.0) + 8.0 * B2 * cmp(r, i) * cmp(r, j);
+                for (int j = 0; j < 3; j++) S[3 * i + j][l] = (i == j ? lp : 0.0) + 8.0 * B2 * cmp(r, i) * cmp(r, j);
         }
-        R[R_D] = rec_code(a, cid, 3);
+        stage_flush<9>(st, a.rec, REC, R_PK);
 #pragma unroll
-        for (int k = 0; k < 4; k++) { hz[k][0] = k == ka ? HS2 : (k == kb ? -HS2 : 0.0); hz[k][1] = 0.0; }
+        for (int k = 0; k < 4; k++) hz[k] = k == ka ? HS2 : (k == kb ? -HS2 : 0.0);
         const d3 g = mk(2 * r.x * B1, 2 * r.y * B1, 2 * r.z * B1);
 #pragma unroll
         for (int k = 0; k < 4; k++) gv[k] = k == ka ? g : (k == kb ? mk(-g.x, -g.y, -g.z) : z);
     }
-    double w[4][4];
-    load_w(a.xbar, id0, id1, id2, id3, w);
-    constexpr int NA = CLS == 1 ? 1 : 2;
 #pragma unroll
-    for (int aa = 0; aa < NA; aa++)
+    for (int s = 0; s < 2; s++)
 #pragma unroll
-        for (int s = 0; s < 2; s++)
+        for (int al = 0; al < 4; al++) {
+            double zz = 0.0;
 #pragma unroll
-            for (int al = 0; al < 4; al++) {
-                double zz = 0.0;
-#pragma unroll
-                for (int k = 0; k < 4; k++)
-                    if ((k < nside0) == (s == 0)) zz += hz[k][aa] * w[k][al];
-                R[R_Z + 8 * aa + 4 * s + al] = zz;
-            }
+            for (int k = 0; k < 4; k++)
+                if ((k < nside0) == (s == 0)) zz += hz[k] * w[k][al];
+            S[4 * s + al][l] = zz;
+        }
+    stage_flush<8>(st, a.rec, REC, R_Z);
 #pragma unroll
     for (int k = 0; k < 4; k++)
         gv[k] = mk(gv[k].x + gmv[k] * ft.x, gv[k].y + gmv[k] * ft.y, gv[k].z + gmv[k] * ft.z);
-    write_grad(a, cid, nside0, w, gv);
+    grad_fields(nside0, w, gv, [&](int k, double v) { S[k - R_GRAD][l] = v; });
+    stage_flush<24>(st, a.rec, REC, R_GRAD);
 }
 // Class 3 contacts, one thread each: the 9x9 Helmert-coordinate matrix
 // K = (C (x) I3) Mred (C^T (x) I3) + mollifier terms (C = h4 S^T, Cm the same
@@ -955,8 +1004,10 @@ __global__ __launch_bounds__(64) void k_contact_c3(WorldArgs a) {
     for (int p = blockIdx.x * 64 + threadIdx.x; p < n3; p += gridDim.x * 64) contact_c3(a, a.list3[p], V, threadIdx.x);
 }
 
-// record fields other than PK of a 3-point contact: Z, friction, gradient, D
-__device__ void scalars_c2(const WorldArgs& a, int cid) {
+// record fields other than PK of a 3-point contact: D, friction, gradient, Z
+__device__ __forceinline__ void scalars_c2(const WorldArgs& a, const Stage& st, bool valid) {
+    const int cid = st.cid, l = st.lane;
+    double (*S)[SLD] = st.S;
     int id0, id1, id2, id3;
     const int is_ee = contact_ids(a, cid, id0, id1, id2, id3);
     const d3 P0 = ld3(a.X + 3 * (size_t)id0), P1 = ld3(a.X + 3 * (size_t)id1);
@@ -966,13 +1017,17 @@ __device__ void scalars_c2(const WorldArgs& a, int cid) {
     const RedMap mp = is_ee ? ee_map(type) : pt_map(type);
     const int nside0 = is_ee ? 2 : 1;
     const double B1 = barrier_d1(d, a.kp.d_hat, a.kp.kappa);
-    double* R = a.rec + REC * (size_t)cid;
     const d3 z = mk(0, 0, 0);
     Fric f;
     friction_eval(a, is_ee, type, d, B1, P0, P1, P2, P3, id0, id1, id2, id3, f);
     double w[4][4];
     load_w(a.xbar, id0, id1, id2, id3, w);
-    write_fric(a, cid, nside0, f, w);
+    S[R_D][l] = rec_code(a, cid, 6);
+    fric_fields(f, nside0, w, [&](int k, double v) { S[k][l] = v; });
+    stage_flush<R_Z>(st, a.rec, REC, 0);
+    tk_fields(f, [&](int k, double v) { S[k][l] = v; });
+    stage_flush<TKN>(st, a.tk, TKN, 0);
+    if (valid) a.lam[cid] = f.lam;
     const int ka = mp.plus[0], kb = mp.plus[1], km = mp.minus[0];
     const d3 pm = pick(P0, P1, P2, P3, km);
     RedDist rd;
@@ -992,7 +1047,8 @@ __device__ void scalars_c2(const WorldArgs& a, int cid) {
         const d3 gb = k == ka ? g0 : (k == kb ? g1 : (k == km ? gm : z));
         gv[k] = mk(gb.x + gmv[k] * ft.x, gb.y + gmv[k] * ft.y, gb.z + gmv[k] * ft.z);
     }
-    write_grad(a, cid, nside0, w, gv);
+    grad_fields(nside0, w, gv, [&](int k, double v) { S[k - R_GRAD][l] = v; });
+    stage_flush<24>(st, a.rec, REC, R_GRAD);
     // Helmert rows over the slots (m, a, b): (1, -1, 0) / sqrt 2, (1, 1, -2) / sqrt 6
 #pragma unroll
     for (int aa = 0; aa < 2; aa++)
@@ -1007,9 +1063,9 @@ __device__ void scalars_c2(const WorldArgs& a, int cid) {
                                              : (k == km ? HS6 : (k == ka ? HS6 : (k == kb ? -2.0 * HS6 : 0.0)));
                     if ((k < nside0) == (s == 0)) zz += h * w[k][al];
                 }
-                R[R_Z + 8 * aa + 4 * s + al] = zz;
+                S[8 * aa + 4 * s + al][l] = zz;
             }
-    R[R_D] = rec_code(a, cid, 6);
+    stage_flush<16>(st, a.rec, REC, R_Z);
 }
 
 // packed upper-triangle index of an N x N symmetric matrix
@@ -1142,12 +1198,28 @@ __device__ __forceinline__ void pk_c2(const WorldArgs& a, int cid) {
 }
 
 // record fields other than PK of 3-point contacts, one thread each
-__global__ __launch_bounds__(256) void k_contact_scalars2(WorldArgs a) {
+// one 64-contact slice of the class list per wavefront and pass (block of
+// 256 lanes, four staging areas: 49.9 KB, three blocks per CU as the
+// registers allow); class 1 at list[0, n1), class 2 at list[total - n2, total)
+// (stored from the back)
+template <int CLS>
+__device__ __forceinline__ void contact_slices(const WorldArgs& a) {
+    __shared__ double S[4][SNF][SLD];
     const int total = a.n_pt + a.n_ee;
-    const int cnt = a.counts[1];
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < cnt; p += gridDim.x * blockDim.x)
-        scalars_c2(a, a.list[total - 1 - p]);
+    const int cnt = a.counts[CLS - 1];
+    const int wv = threadIdx.x >> 6;
+    Stage st;
+    st.S = S[wv];
+    st.lane = threadIdx.x & 63;
+    for (int p0 = (blockIdx.x * 4 + wv) * 64; p0 < cnt; p0 += gridDim.x * 256) {
+        const int p = min(p0 + st.lane, cnt - 1);
+        st.nvalid = min(64, cnt - p0);
+        st.cid = a.list[CLS == 1 ? p : total - 1 - p];
+        if (CLS == 1) contact_small(a, st, p0 + st.lane < cnt);
+        else scalars_c2(a, st, p0 + st.lane < cnt);
+    }
 }
+__global__ __launch_bounds__(256) void k_contact_scalars2(WorldArgs a) { contact_slices<2>(a); }
 __global__ __launch_bounds__(64) void k_contact_pk2(WorldArgs a) {
     const int total = a.n_pt + a.n_ee;
     const int cnt = a.counts[1];
@@ -1155,14 +1227,10 @@ __global__ __launch_bounds__(64) void k_contact_pk2(WorldArgs a) {
         pk_c2(a, a.list[total - 1 - p]);
 }
 
-// grid-strided over the class list: class 1 at list[0, n1), class 2 at
-// list[total - n2, total) (stored from the back)
 template <int CLS>
 __global__ __launch_bounds__(256) void k_contact_small(WorldArgs a) {
-    const int total = a.n_pt + a.n_ee;
-    const int cnt = a.counts[CLS - 1];
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < cnt; p += gridDim.x * blockDim.x)
-        contact_small<CLS>(a, a.list[CLS == 1 ? p : total - 1 - p]);
+    static_assert(CLS == 1, "3-point contacts: k_contact_scalars2 / k_contact_pk2");
+    contact_slices<1>(a);
 }
 
 // VG contact: compact body-space form.  The VG block of body I is
