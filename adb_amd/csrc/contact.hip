@@ -16,7 +16,9 @@
 //   class 3 (four points, or any mollified EE, D = 9): 9 x 9 matrix staged
 //     through LDS, cyclic Jacobi in registers (k_contact_c3, on a side stream).
 // Each contact writes a compact body-space record (world.h) that the
-// assembly expands into its four 12 x 12 blocks.
+// assembly expands into its four 12 x 12 blocks; the class 1 and class 2
+// record kernels stage it per wavefront in LDS and store it cooperatively
+// (Stage, stage_flush).
 //
 // PSD projection: every barrier and mollifier term depends on the points only
 // through differences, so the Hessian annihilates rigid translations.  With
