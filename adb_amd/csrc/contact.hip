@@ -24,8 +24,11 @@
 // through differences, so the Hessian annihilates rigid translations.  With
 // Q = h (x) I3, h the Helmert basis of the complement of (1, ..., 1) over the
 // contact's points, H = Q^T (Q H Q^T) Q and the projection onto the PSD cone
-// is Q^T P(Q H Q^T) Q: a D x D symmetric eigenproblem.  As in the reference,
-// a Hessian whose eigenvalues are all >= 0 is returned unchanged.
+// is Q^T P(Q H Q^T) Q: a D x D symmetric eigenproblem.  The kernel API, as in
+// the reference, returns a Hessian whose eigenvalues are all >= 0 unchanged;
+// the world path always rebuilds V diag(max(w, 0)) V^T, equal to it within
+// the Jacobi tolerance (the reference's own test sees the nine rounding-level
+// eigenvalues of the translations).
 //
 // Kernel-level API (k_hess_api, the abdtk functions): one 16-lane group per
 // contact, four contacts per wavefront, lane j < 12 owning column j of the
