@@ -1317,9 +1317,10 @@ void contact_terms(World& w) {
         }
         const int* snap = prof_snapshot(w, w.ccount.p);
         {
-            // 4-point contacts on the side stream: they fill few CUs for long, so
-            // they overlap the 2- and 3-point kernels and the assembly's sort;
-            // the assembly waits for them before its reduction
+            // 4-point contacts on the side stream (four 64-lane workgroups per CU,
+            // one wave per SIMD, FP64-VALU-bound): they overlap the 2- and 3-point
+            // kernels and the assembly's sort; the assembly waits for them
+            // before its reduction
             if (!w.stream2) {
                 ADB_HIP(hipStreamCreate(&w.stream2));
                 ADB_HIP(hipStreamCreate(&w.stream3));
