@@ -696,17 +696,26 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-// base[stride * cid_r + off + k] = S[k][r] for records r < nvalid, fields k < NF
+// base[stride * cid_r + off + k] = S[k][r] for records r < nvalid, fields k < NF,
+// as 16-byte stores of field pairs (the texture address unit's work grows with
+// the number of store instructions, not with the cache lines they cover; off,
+// stride and record bases are even in doubles, so every pair is aligned)
 template <int NF>
 __device__ __forceinline__ void stage_flush(const Stage& st, double* base, int stride, int off) {
-    static_assert(NF <= SNF, "section larger than the staging rows");
+    static_assert(NF <= SNF && NF % 2 == 0, "sections of an even number of fields within the staging rows");
+    constexpr int NP = NF / 2;
     wave_lds_sync();
 #pragma unroll 1
-    for (int it = 0; it < NF; it++) {
-        const int t = it * 64 + st.lane;      // < 64 NF, so r < 64
-        const int r = t / NF, k = t - r * NF;
+    for (int it = 0; it < NP; it++) {
+        const int t = it * 64 + st.lane;      // < 64 NP, so r < 64
+        const int r = t / NP, jp = t - r * NP;
         const int c = __shfl(st.cid, r);
-        if (r < st.nvalid) base[(size_t)stride * c + off + k] = st.S[k][r];
+        if (r < st.nvalid) {
+            double2 v;
+            v.x = st.S[2 * jp][r];
+            v.y = st.S[2 * jp + 1][r];
+            *reinterpret_cast<double2*>(base + (size_t)stride * c + off + 2 * jp) = v;
+        }
     }
     wave_lds_sync();
 }
@@ -784,7 +793,8 @@ __device__ __forceinline__ void contact_small(const WorldArgs& a, const Stage& s
 #pragma unroll
                 for (int j = 0; j < 3; j++) S[3 * i + j][l] = (i == j ? lp : 0.0) + 8.0 * B2 * cmp(r, i) * cmp(r, j);
         }
-        stage_flush<9>(st, a.rec, REC, R_PK);
+        S[9][l] = 0.0;      // pad to a field pair: R_PK + 9 is past a 2-point record
+        stage_flush<10>(st, a.rec, REC, R_PK);
 #pragma unroll
         for (int k = 0; k < 4; k++) hz[k] = k == ka ? HS2 : (k == kb ? -HS2 : 0.0);
         const d3 g = mk(2 * r.x * B1, 2 * r.y * B1, 2 * r.z * B1);
