@@ -20,11 +20,13 @@ ADB_E_TOI_STALL = -4
 ADB_E_CAPACITY = -5
 ADB_E_NOT_CONVERGED = -6
 ADB_E_NO_DEVICE = -7
+ADB_E_SOLVER = -8
 
 STATUS_NAMES = {
     ADB_OK: "ADB_OK", ADB_E_ARG: "ADB_E_ARG", ADB_E_HIP: "ADB_E_HIP", ADB_E_NAN: "ADB_E_NAN",
     ADB_E_TOI_STALL: "ADB_E_TOI_STALL", ADB_E_CAPACITY: "ADB_E_CAPACITY",
     ADB_E_NOT_CONVERGED: "ADB_E_NOT_CONVERGED", ADB_E_NO_DEVICE: "ADB_E_NO_DEVICE",
+    ADB_E_SOLVER: "ADB_E_SOLVER",
 }
 
 _dp = C.POINTER(C.c_double)
@@ -65,6 +67,7 @@ class AdbStepStats(C.Structure):
         ("toi_min", C.c_double),
         ("t_ipc", C.c_double), ("t_solver", C.c_double), ("t_ccd", C.c_double),
         ("t_line_search", C.c_double), ("t_total", C.c_double),
+        ("pcg_not_converged", C.c_int), ("pcg_breakdown", C.c_int),
     ]
 
     def as_dict(self):

@@ -3,7 +3,7 @@
 // Replaces grad_residue_per_body / hess_inertia_per_body (model/mutils.cpp:53-76,
 // model/othogonal_energy.cpp:16-47), gen_empty_sm (model/sparse.cpp:13-57), the
 // locked scatter of IPC::output_hessian_gradient (model/assemble_matrices.cpp:28-59)
-// and the diagonal += c.hess of ABD::solve (model/abd.cpp:328-341).
+// and the diagonal += c.hess of ABD::solve (model/abd.cpp:248-260).
 //
 // No atomics, every sum in a fixed order.  Contacts are radix-sorted by body
 // pair (lo, hi), lo < hi; each pair's contacts are cut into chunks of at most

@@ -2,7 +2,7 @@
 //
 // pt_collision_time / ee_collision_time (model/collision.cpp:299-348, exposed by
 // python/bindings.cpp:81-82) and the distance type + squared distance pair
-// (model/geometry.cpp:101-112, model/abd.cpp:433-434), one entry per lane.
+// (model/geometry.cpp:101-112, model/abd.cpp:353-354), one entry per lane.
 #include "dmath.h"
 #include "world.h"
 

@@ -4,7 +4,7 @@
 // project_to_psd (model/psd.h:9-33), pt_uktk / ee_uktk (model/pt_ipc.cpp:22-107,
 // model/ee_ipc.cpp:21-118), friction (model/friction.cpp:15-47), the J^T H J body
 // mapping (model/pt_ipc.cpp:153-169, model/ee_ipc.cpp:175-192) and the VG term
-// (model/vg_ipc.cpp:12-30, model/abd.cpp:460-481).
+// (model/vg_ipc.cpp:12-30, model/abd.cpp:380-401).
 //
 // World path (contact_terms): k_contact_classify computes each contact's
 // distance type, squared distance and class, the dimension D = 3 n of the

@@ -894,7 +894,7 @@ double ccd_toi(World& w, const double* X0, const double* X1, int* status) {
         w.pairs.reserve(np);               // more overlapping body pairs than slots: grow, test again
     }
     const double tg = order_key_inv(bits[0]), tp = order_key_inv(bits[1]);
-    // toi < 1e-6 twice in a row is a stall (model/iaabb.cpp:305-314,532-540)
+    // toi < 1e-6 twice in a row is a stall (model/iaabb.cpp:305-311,533-537)
     if (tg < 1e-6) { if (++w.stall_vg > 1) *status = ADB_E_TOI_STALL; } else w.stall_vg = 0;
     if (tp < 1e-6) { if (++w.stall_pt > 1) *status = ADB_E_TOI_STALL; } else w.stall_pt = 0;
     return tg < tp ? tg : tp;

@@ -7,7 +7,7 @@
 // implementation that follows the same order.
 //
 // Reference semantics: distance types and distances follow ipc-toolkit
-// v1.2.1 as called at model/geometry.cpp:101-112 and model/abd.cpp:433-434;
+// v1.2.1 as called at model/geometry.cpp:101-112 and model/abd.cpp:353-354;
 // the barrier follows model/barrier.cpp:11-28; CCD follows
 // model/collision.cpp:83-348 with the cubic solver of
 // cyCodeBase/cyPolynomial.h:383-447,587-609,787-880 (host defaults).

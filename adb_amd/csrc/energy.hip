@@ -153,11 +153,11 @@ __global__ __launch_bounds__(RT) void k_sum2(int G, const double* __restrict__ p
 
 // Sum of a[i] (b == null) or a[i] b[i] into w.epart[slot] (slots 0..7), on
 // the world stream without waiting; read_slots fetches the results.
-constexpr int NSLOT = 8;
+constexpr int NSLOT = EPART_NSLOT;
 static void reduce_async(World& w, const double* a, const double* b, int n, int slot) {
     int G = (n + RT - 1) / RT;
-    if (G > 1024) G = 1024;
-    w.epart.reserve(NSLOT + std::max(G, 1));
+    if (G > EPART_MAXG) G = EPART_MAXG;
+    if (w.epart.cap < (size_t)EPART_SLOTS) throw AdbError(ADB_E_ARG, "reduction slots not reserved");
     if (n <= 0) {
         ADB_HIP(hipMemsetAsync(w.epart.p + slot, 0, sizeof(double), w.stream));
         return;

@@ -1,6 +1,6 @@
 // pcg.hip - block-Jacobi preconditioned conjugate gradients on the BSR
 // Hessian, FP64 throughout.  Replaces the direct PardisoLLT / SimplicialLLT
-// solve of ABD::solve (model/abd.cpp:343-371): dq = -H^{-1} g.
+// solve of ABD::solve (model/abd.cpp:263-291): dq = -H^{-1} g.
 //
 // Two launches per iteration, no atomics and no host round trip:
 //   spmv(k)  : every wave first sums the update partials of iteration k - 1
@@ -360,6 +360,7 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
     if (stop == NONFINITE) return ADB_E_NAN;
     ADB_HIP(hipMemcpyAsync(dq_out, w.pcg_x.p, n * sizeof(double), hipMemcpyDeviceToDevice, w.stream));
     if (stop == RUN) return ADB_E_NOT_CONVERGED;
+    if (stop == BREAKDOWN) return ADB_E_SOLVER;
     return ADB_OK;
 }
 

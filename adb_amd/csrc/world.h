@@ -56,6 +56,10 @@ constexpr int REC = 160;
 // Tk[r][3k + i] = gm_k c_r[i] (products formed where they are used).
 constexpr int TKN = 10;
 constexpr int R_D = 0, R_F = 1, R_G = 10, R_Z = 18, R_GRAD = 42, R_PK = 66;
+// Reduction result slots (8) + per-workgroup partials (at most 1024 workgroups):
+// reserved once at world creation, so no reduction queued behind another can
+// see its slots reallocated.
+constexpr int EPART_NSLOT = 8, EPART_MAXG = 1024, EPART_SLOTS = EPART_NSLOT + EPART_MAXG;
 
 // Per-kernel HIP-event timing on the world's stream (off unless enabled).
 enum ProfKernel {

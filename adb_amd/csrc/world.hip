@@ -1,8 +1,8 @@
 // world.hip - host driver of the Newton contact step and the C ABI.
 //
-// adb_step replaces ABD::implicit_euler (model/abd.cpp:110-220) with its
+// adb_step replaces ABD::implicit_euler (model/abd.cpp:30-140) with its
 // stages ABD::ipc, ABD::solve, ABD::ccd and ABD::line_search
-// (model/abd.cpp:222-485, model/line_search.cpp:39-136).  The control flow is
+// (model/abd.cpp:142-404, model/line_search.cpp:39-136).  The control flow is
 // the reference's: q <- q0; contact set at q; repeat { inertia + contact
 // terms, solve, CCD bound with backoff, Armijo backtracking with a fresh
 // contact set per trial, q += alpha dq } until ||dq|| < tol; then
@@ -69,20 +69,29 @@ static int step(World& w, adb_step_stats* st) {
         dot_async(w, w.dq.p, w.dq.p, n, 0);
         dot_async(w, w.grad.p, w.grad.p, n, 1);
         read_slots(w, 2, sl);
+        // a solve that stopped early is counted, not hidden: the direction is
+        // still the last PCG iterate and the line search still guards descent
+        if (prc == ADB_E_NOT_CONVERGED) {
+            S.pcg_not_converged++;
+            w.err = "PCG reached pcg_max_iter at Newton iteration " + std::to_string(iter);
+        } else if (prc == ADB_E_SOLVER) {
+            S.pcg_breakdown++;
+            w.err = "PCG breakdown (system not positive definite) at Newton iteration " + std::to_string(iter);
+        }
         const double nrm2 = prc == ADB_E_NAN ? NAN : sl[0];
         const double gn = std::sqrt(sl[1]);
         const double sup_dq = std::sqrt(nrm2);
         const double t2 = now_s();
         S.t_solver += t2 - t1;
-        if (std::isnan(sup_dq)) { status = ADB_E_NAN; w.err = "solver nan (model/abd.cpp:368-371)"; break; }
-        // CCD step bound (model/abd.cpp:222-276)
+        if (std::isnan(sup_dq)) { status = ADB_E_NAN; w.err = "solver nan (model/abd.cpp:288-291)"; break; }
+        // CCD step bound (model/abd.cpp:142-196)
         double toi = 1.0, factor = 1.0;
         if (w.P.upper_bound) {
             q_axpy(w, w.q.p, w.dq.p, 1.0, w.qtrial.p);
             transform_positions(w, w.qtrial.p, w.X1.p);
             int cs_status = ADB_OK;
             toi = ccd_toi(w, w.X.p, w.X1.p, &cs_status);
-            if (cs_status != ADB_OK) { status = cs_status; w.err = "toi < 1e-6 twice in a row (model/iaabb.cpp:305-314)"; break; }
+            if (cs_status != ADB_OK) { status = cs_status; w.err = "toi < 1e-6 twice in a row (model/iaabb.cpp:305-311,533-537)"; break; }
         }
         if (toi < 1.0) factor = w.P.backoff;
         S.toi_min = std::min(S.toi_min, toi);
@@ -187,7 +196,9 @@ using namespace adb;
     try { __VA_ARGS__ }                                                       \
     catch (const HipError& e) { if (wptr) (wptr)->w.err = e.what; return ADB_E_HIP; } \
     catch (const AdbError& e) { if (wptr) (wptr)->w.err = e.what; return e.code; }    \
-    catch (const std::exception& e) { if (wptr) (wptr)->w.err = e.what(); return ADB_E_ARG; }
+    catch (const std::bad_alloc& e) { if (wptr) (wptr)->w.err = "host allocation failed"; return ADB_E_CAPACITY; } \
+    catch (const std::exception& e) { if (wptr) (wptr)->w.err = e.what(); return ADB_E_ARG; } \
+    catch (...) { if (wptr) (wptr)->w.err = "unknown exception"; return ADB_E_ARG; }
 
 extern "C" void adb_params_default(adb_params* p) {
     std::memset(p, 0, sizeof *p);
@@ -212,6 +223,8 @@ extern "C" int adb_device_count(void) {
     }
     return good;
 }
+
+static void free_world(World& w);
 
 extern "C" int adb_world_create(const adb_scene* s, const adb_params* p, adb_world** out) {
     if (!s || !p || !out || s->n_bodies <= 0) return ADB_E_ARG;
@@ -278,11 +291,16 @@ extern "C" int adb_world_create(const adb_scene* s, const adb_params* p, adb_wor
         ADB_HIP(hipMemcpyAsync(w.q.p, w.q0.p, n * sizeof(double), hipMemcpyDeviceToDevice, w.stream));
         w.X.reserve(3 * (size_t)w.V); w.Xs.reserve(3 * (size_t)w.V); w.X1.reserve(3 * (size_t)w.V);
         w.counters.reserve(16);
+        w.epart.reserve(EPART_SLOTS);
         ADB_HIP(hipStreamSynchronize(w.stream));
     } catch (const HipError& e) {
-        delete W; return ADB_E_HIP;
+        free_world(w); delete W; return ADB_E_HIP;
     } catch (const AdbError& e) {
-        delete W; return e.code;
+        free_world(w); delete W; return e.code;
+    } catch (const std::bad_alloc&) {
+        free_world(w); delete W; return ADB_E_CAPACITY;
+    } catch (...) {
+        free_world(w); delete W; return ADB_E_ARG;
     }
     *out = W;
     return ADB_OK;

@@ -1,6 +1,6 @@
 """Python handle on an adb_world (include/adb_capi.h) - the MI355X Newton
 contact step.  Mirrors the reference's ABD driver (model/abd.h:9-80): one
-`step()` is ABD::implicit_euler (model/abd.cpp:110-220)."""
+`step()` is ABD::implicit_euler (model/abd.cpp:30-140)."""
 import ctypes as C
 
 import numpy as np
@@ -43,7 +43,9 @@ class World:
             raise AdbError(rc, "%s: %s" % (what, self.lib.adb_last_error(self.h).decode()))
         return rc
 
-    def step(self, allow_not_converged=True):
+    def step(self, allow_not_converged=False):
+        """One ABD::implicit_euler step.  Raises AdbError on any status other
+        than ADB_OK (ADB_E_NOT_CONVERGED too, unless allow_not_converged)."""
         st = AdbStepStats()
         rc = self.lib.adb_step(self.h, C.byref(st))
         allow = (-6,) if allow_not_converged else ()

@@ -79,14 +79,15 @@ def run(sc, p, warmup, steps, label=""):
     adb_step returns after its stream has drained, so per-step wall times are
     exact; a progress line goes to stderr every 10 steps."""
     from adb_amd.world import World
-    w = World(sc, p)
+    w = World(sc, p)   # World.step raises on any status but ADB_OK, so no failed step is timed
     per_step = []
     for _ in range(warmup):
         t = time.perf_counter()
         w.step()
         per_step.append(time.perf_counter() - t)
     w.state()   # synchronizes the world's stream
-    tot = {"contacts": 0, "newton": 0, "ls": 0, "pcg": 0, "t_ipc": 0.0, "t_solver": 0.0, "t_ccd": 0.0, "t_ls": 0.0}
+    tot = {"contacts": 0, "newton": 0, "ls": 0, "pcg": 0, "t_ipc": 0.0, "t_solver": 0.0, "t_ccd": 0.0, "t_ls": 0.0,
+           "pcg_not_converged": 0, "pcg_breakdown": 0}
     last = None
     t0 = time.perf_counter()
     for i in range(steps):
@@ -102,6 +103,8 @@ def run(sc, p, warmup, steps, label=""):
         tot["t_solver"] += st.t_solver
         tot["t_ccd"] += st.t_ccd
         tot["t_ls"] += st.t_line_search
+        tot["pcg_not_converged"] += st.pcg_not_converged
+        tot["pcg_breakdown"] += st.pcg_breakdown
         if (i + 1) % 10 == 0:
             print("%s step %d/%d: %.1f s, %d contacts" % (label, i + 1, steps, time.perf_counter() - t0, st.n_pt + st.n_ee + st.n_vg),
                   file=sys.stderr, flush=True)
@@ -304,6 +307,7 @@ def main():
                    "bodies": args.bodies, "precision": args.precision},
         "contact_pairs_per_s": round(tot["contacts"] / elapsed, 1),
         "newton_iters": tot["newton"], "ls_trials": tot["ls"], "pcg_iters": tot["pcg"],
+        "pcg_unfinished": {"max_iter": tot["pcg_not_converged"], "breakdown": tot["pcg_breakdown"]},
         "phase_s": {"ipc": round(tot["t_ipc"], 4), "solver": round(tot["t_solver"], 4),
                     "ccd": round(tot["t_ccd"], 4), "line_search": round(tot["t_ls"], 4)},
         "contacts_last_step": {"pt": last.n_pt, "ee": last.n_ee, "vg": last.n_vg} if last else None,

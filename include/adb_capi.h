@@ -2,7 +2,7 @@
  * adb_capi.h - C ABI of the MI355X-native ADB Newton contact step.
  *
  * This header is the drop-in boundary for the reference's per-timestep contact
- * step (`ABD::implicit_euler` and everything under it, model/abd.cpp:110-220) and
+ * step (`ABD::implicit_euler` and everything under it, model/abd.cpp:30-140) and
  * for the kernel-level functions the reference exposes through its `abdtk`
  * Python module (python/bindings.cpp:15-40,78-82).  Every entry point takes
  * plain pointers and sizes; no C++ or torch types cross the boundary.
@@ -17,7 +17,7 @@
  *   - Enum values equal ipc-toolkit's PointTriangleDistanceType and
  *     EdgeEdgeDistanceType (python/bindings.cpp:54-76).
  *   - No entry point ever exits the process: errors are returned as status codes
- *     (the reference calls exit(1): model/abd.cpp:368-371, model/iaabb.cpp:305-314).
+ *     (the reference calls exit(1): model/abd.cpp:288-291, model/iaabb.cpp:305-311,533-537).
  */
 #ifndef ADB_CAPI_H
 #define ADB_CAPI_H
@@ -30,11 +30,12 @@ extern "C" {
 #define ADB_OK               0
 #define ADB_E_ARG           -1  /* bad argument / size                          */
 #define ADB_E_HIP           -2  /* HIP runtime error                            */
-#define ADB_E_NAN           -3  /* NaN in the Newton direction (abd.cpp:368)    */
+#define ADB_E_NAN           -3  /* NaN in the Newton direction (abd.cpp:288)    */
 #define ADB_E_TOI_STALL     -4  /* toi < 1e-6 twice in a row (iaabb.cpp:305)    */
 #define ADB_E_CAPACITY      -5  /* a fixed-capacity buffer overflowed           */
 #define ADB_E_NOT_CONVERGED -6  /* Newton hit max_iter without ||dq|| < tol     */
 #define ADB_E_NO_DEVICE     -7  /* no usable gfx950 device                      */
+#define ADB_E_SOLVER        -8  /* PCG broke down (system not positive definite) */
 
 /* ---- distance-type enums (ipc-toolkit numbering) ------------------------- */
 enum adb_pt_type { ADB_P_T0 = 0, ADB_P_T1, ADB_P_T2, ADB_P_E0, ADB_P_E1, ADB_P_E2, ADB_P_T, ADB_PT_AUTO };
@@ -111,15 +112,23 @@ typedef struct adb_step_stats {
     long long contacts_processed; /* sum over Newton iterations of active contacts */
     double toi_min;            /* smallest CCD step bound seen                    */
     double t_ipc, t_solver, t_ccd, t_line_search; /* seconds, the reference's 4
-                                   phase buckets (model/abd.h:36-39)              */
+                                   phase buckets (model/abd.h:68, model/abd.cpp:110-121)              */
     double t_total;
+    /* Newton iterations of the step whose linear solve did not finish: PCG hit
+     * pcg_max_iter before ||r|| <= pcg_rtol ||b||, or broke down (r.z <= 0 or
+     * p.Hp <= 0: the assembled system is not positive definite).  The step goes
+     * on with the last PCG iterate, as the line search still guards descent;
+     * adb_last_error names the last such event.  The reference's direct solve
+     * has no counterpart (model/abd.cpp:268-276). */
+    int    pcg_not_converged;
+    int    pcg_breakdown;
 } adb_step_stats;
 
 typedef struct adb_world adb_world;
 
 /* ---- world / step API ----------------------------------------------------
  * Replaces ABD::ABD (model/abd.h:13-14) and ABD::implicit_euler
- * (model/abd.cpp:110-220).  The scene is copied; the caller's arrays are not
+ * (model/abd.cpp:30-140).  The scene is copied; the caller's arrays are not
  * retained.  One world per (thread, device). */
 int  adb_world_create(const adb_scene* scene, const adb_params* params, adb_world** out);
 void adb_world_destroy(adb_world* w);
@@ -147,13 +156,13 @@ int adb_step_size_upper_bound(adb_world* w, const double* dq, double* toi);
 
 /* Contact gradient/Hessian assembly at the current q, with friction lagged on
  * the step's start positions: replaces ABD::ipc + per-body inertia terms +
- * ABD::solve's diagonal assembly (model/abd.cpp:148-160,319-340,385-485).
+ * ABD::solve's diagonal assembly (model/abd.cpp:68-74,248-260,305-404).
  * Writes the assembled global gradient [B*12] and the dense 12B x 12B
  * Hessian (column-major) when hess_dense != NULL (small scenes only). */
 int adb_assemble(adb_world* w, double* grad, double* hess_dense);
 
 /* Newton direction dq = -H^{-1} g at the current q (ABD::solve,
- * model/abd.cpp:309-383) using block-Jacobi PCG. */
+ * model/abd.cpp:229-303) using PCG preconditioned by contact-group inverses. */
 int adb_newton_direction(adb_world* w, double* dq);
 
 /* ---- kernel-level API (abdtk) ---------------------------------------------
@@ -193,7 +202,7 @@ double adb_ee_collision_time(const double x[24]);
 
 /* Distance type + squared distance (ipc-toolkit point_triangle_distance_type /
  * point_triangle_distance and the edge-edge pair, called at
- * model/geometry.cpp:101-112 and model/abd.cpp:433-434), batched on device. */
+ * model/geometry.cpp:101-112 and model/abd.cpp:353-354), batched on device. */
 int adb_pt_distance_batched(int n, const double* pt, int* type, double* dist, void* stream);
 int adb_ee_distance_batched(int n, const double* ee, int* type, double* dist, void* stream);
 
@@ -203,7 +212,7 @@ int adb_device_count(void);
 
 /* ---- measurement hooks -----------------------------------------------------
  * No counterpart in the reference (its timers are the four std::chrono phase
- * buckets of model/abd.h:36-39, reported in adb_step_stats).  Per-kernel
+ * buckets of model/abd.h:68, model/abd.cpp:110-121, reported in adb_step_stats).  Per-kernel
  * HIP-event timing on the world's own stream, for roofline figures:
  * adb_profile_enable(w, 1) clears and starts, adb_profile_read returns the
  * accumulated time, launch count and algorithmic bytes of one kernel. */

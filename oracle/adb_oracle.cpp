@@ -5,7 +5,7 @@
 // cpu_baseline leg of bench.py.  It is never part of the product path.
 //
 // Reference files followed (paths relative to the reference repository):
-//   step / Newton loop ....... model/abd.cpp:110-220, 222-276, 278-307, 309-383, 385-485
+//   step / Newton loop ....... model/abd.cpp:30-140, 142-196, 198-227, 229-303, 305-404
 //   line search / energies ... model/line_search.cpp:39-136, model/energy.cpp:19-86,195-270
 //   inertia / orthogonality .. model/mutils.cpp:53-76,197-201,245-252, model/othogonal_energy.cpp:8-62
 //   contact terms ............ model/core_ipc.cpp:21-83, model/pt_ipc.cpp:22-186,
@@ -824,7 +824,7 @@ static void add_JtHJ(const double Ja[36], const double* H, int ra, int cb, const
 }
 
 // ---------------------------------------------------------------------------
-// Contact terms at the current q (model/abd.cpp:385-485 and the files it calls)
+// Contact terms at the current q (model/abd.cpp:305-404 and the files it calls)
 // ---------------------------------------------------------------------------
 static void contact_pt(const World& w, const std::vector<double>& X, const std::vector<double>& Xs, const int* ij, Contact& c) {
     int I = ij[0], v = w.voff[I] + ij[1], J = ij[2], f = w.foff[J] + ij[3];
@@ -917,7 +917,7 @@ static void contact_ee(const World& w, const std::vector<double>& X, const std::
 
 // ---------------------------------------------------------------------------
 // Block-sparse Cholesky with a minimum-degree body ordering; stands in for
-// MKL PardisoLLT (model/abd.cpp:349-356).
+// MKL PardisoLLT (model/abd.cpp:268-276).
 // ---------------------------------------------------------------------------
 static bool chol12(double* A) {   // in place, lower, column-major; returns false if not PD
     for (int j = 0; j < 12; j++) {
@@ -1023,7 +1023,7 @@ static int sparse_solve(int B, const std::map<std::pair<int, int>, std::vector<d
 }
 
 // ---------------------------------------------------------------------------
-// Assembly at the current q: inertia + contacts + VG (model/abd.cpp:148-160,
+// Assembly at the current q: inertia + contacts + VG (model/abd.cpp:68-74,
 // 319-340, 385-485; model/assemble_matrices.cpp:28-59)
 // ---------------------------------------------------------------------------
 static void assemble(World& w, long long* nactive) {
@@ -1214,7 +1214,7 @@ static int step(World& w, adb_step_stats* st) {
         double sup_dq = std::sqrt(nrm);
         double t2 = now_s(); S.t_solver += t2 - t1;
         if (std::isnan(sup_dq)) { status = ADB_E_NAN; w.err = "solver nan"; break; }
-        // CCD step bound (model/abd.cpp:222-276)
+        // CCD step bound (model/abd.cpp:142-196)
         double toi = 1.0, factor = 1.0;
         if (w.P.upper_bound) {
             q_plus(w, dq.data(), qv);
@@ -1378,6 +1378,70 @@ int oracle_assemble(void* p, double* grad, double* hess) {
                 hess[(size_t)(12 * kv.first.second + c) * n + 12 * kv.first.first + r] = kv.second[c * 12 + r];
     }
     return ADB_OK;
+}
+// Assembly at a Newton iterate q with the step start q0 unchanged: friction is
+// lagged from the contacts at q and measured against the step-start
+// positions, as ABD::ipc inside a step (model/abd.cpp:68-90,305-404).
+int oracle_assemble_at(void* p, const double* qv, double* grad, double* hess) {
+    World& w = *W(p);
+    std::memcpy(w.q.data(), qv, sizeof(double) * 12 * w.B);
+    std::vector<double> X;
+    positions(w, w.q.data(), X);
+    collision_set(w, X, 1, w.pt, w.ee, w.vg);
+    assemble(w, nullptr);
+    int n = 12 * w.B;
+    if (grad) std::memcpy(grad, w.grad.data(), sizeof(double) * n);
+    if (hess) {
+        std::memset(hess, 0, sizeof(double) * (size_t)n * n);
+        for (auto& kv : w.blocks)
+            for (int c = 0; c < 12; c++) for (int r = 0; r < 12; r++)
+                hess[(size_t)(12 * kv.first.second + c) * n + 12 * kv.first.first + r] = kv.second[c * 12 + r];
+    }
+    return ADB_OK;
+}
+// Lagged friction energy (model/energy.cpp:195-270) at q, with the Tk / lambda
+// of the last assembly and the step-start positions of q0.
+int oracle_friction_energy(void* p, const double* qv, double* E) {
+    World& w = *W(p);
+    std::vector<double> X, Xs;
+    positions(w, qv, X);
+    positions(w, w.q0.data(), Xs);
+    *E = E_fric(w, X, Xs);
+    return ADB_OK;
+}
+// ABD::line_search (model/abd.cpp:198-227, model/line_search.cpp:39-136) from
+// the iterate q along dq (already scaled by the CCD bound): assembles at q
+// (gradient and lagged friction), then halves alpha from 1 until the Armijo
+// condition holds.  Returns the accepted alpha, E0, E1 and the trial count.
+int oracle_line_search(void* p, const double* qv, const double* dq, double* alpha_out, double* E0_out, double* E1_out,
+                       int* trials_out) {
+    World& w = *W(p);
+    const int n = 12 * w.B;
+    oracle_assemble_at(p, qv, nullptr, nullptr);
+    std::vector<double> X, Xs, X1, q1(n);
+    positions(w, w.q.data(), X);
+    positions(w, w.q0.data(), Xs);
+    const int set_method = (w.P.iaabb & 1) ? 1 : 2;
+    double E0 = E_barrier_inert(w, w.q, X, w.pt, w.ee, w.vg) + E_fric(w, X, Xs);
+    double qdg = 0, gn = 0;
+    for (int i = 0; i < n; i++) { qdg += dq[i] * w.grad[i]; gn += w.grad[i] * w.grad[i]; }
+    gn = std::sqrt(gn);
+    double alpha = 1.0, E1 = E0;
+    int trials = 0, rc = ADB_OK;
+    std::vector<int> npt, nee, nvg;
+    while (true) {
+        for (int i = 0; i < n; i++) q1[i] = w.q[i] + dq[i] * alpha;
+        positions(w, q1.data(), X1);
+        collision_set(w, X1, set_method, npt, nee, nvg);
+        E1 = E_barrier_inert(w, q1, X1, npt, nee, nvg) + E_fric(w, X1, Xs);
+        bool wolfe = E1 <= E0 + w.P.c1 * alpha * qdg;
+        alpha /= 2;
+        trials++;
+        if (wolfe || !(gn > 1e-3)) break;
+        if (trials >= w.P.max_ls_iter) { rc = ADB_E_NOT_CONVERGED; break; }
+    }
+    *alpha_out = alpha * 2; *E0_out = E0; *E1_out = E1; *trials_out = trials;
+    return rc;
 }
 int oracle_energy(void* p, const double* qv, double* E) {
     World& w = *W(p);

@@ -80,7 +80,7 @@ template <class T> T3<T> tcross(const T3<T>& a, const T3<T>& b) {
 // ---------------------------------------------------------------------------
 // Squared distances (the functions ipc-toolkit v1.2.1 dispatches to from
 // point_triangle_distance / edge_edge_distance; called at
-// model/geometry.cpp:108-109, model/abd.cpp:433-434,
+// model/geometry.cpp:108-109, model/abd.cpp:353-354,
 // model/thread_local_culling.cpp:325-326,375-376).
 // ---------------------------------------------------------------------------
 template <class T> T d_pp(const T3<T>& a, const T3<T>& b) { T3<T> e = sub(a, b); return tdot(e, e); }

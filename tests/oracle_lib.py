@@ -38,6 +38,9 @@ def lib():
         "oracle_assemble": (C.c_int, [vp, _dp, _dp]),
         "oracle_newton_direction": (C.c_int, [vp, _dp]),
         "oracle_energy": (C.c_int, [vp, _dp, _dp]),
+        "oracle_assemble_at": (C.c_int, [vp, _dp, _dp, _dp]),
+        "oracle_friction_energy": (C.c_int, [vp, _dp, _dp]),
+        "oracle_line_search": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp, _ip]),
         "oracle_set_threads": (C.c_int, [C.c_int]),
         "oracle_pt_distance": (None, [_dp, _ip, _dp]),
         "oracle_ee_distance": (None, [_dp, _ip, _dp]),
@@ -294,3 +297,23 @@ class OracleWorld:
         dq = np.zeros(12 * self.B)
         rc = lib().oracle_newton_direction(self.h, dptr(dq))
         return rc, dq
+
+    def assemble_at(self, q, dense=True):
+        n = 12 * self.B
+        g = np.zeros(n)
+        H = np.zeros(n * n) if dense else None
+        qa = _d(q).reshape(-1)
+        lib().oracle_assemble_at(self.h, dptr(qa), dptr(g), dptr(H) if dense else None)
+        return g, (H.reshape(n, n).T if dense else None)
+
+    def friction_energy(self, q):
+        e = C.c_double()
+        qa = _d(q).reshape(-1)
+        lib().oracle_friction_energy(self.h, dptr(qa), C.byref(e))
+        return e.value
+
+    def line_search(self, q, dq):
+        a, e0, e1, t = C.c_double(), C.c_double(), C.c_double(), C.c_int()
+        qa, da = _d(q).reshape(-1), _d(dq).reshape(-1)
+        rc = lib().oracle_line_search(self.h, dptr(qa), dptr(da), C.byref(a), C.byref(e0), C.byref(e1), C.byref(t))
+        return rc, a.value, e0.value, e1.value, t.value

@@ -1,5 +1,5 @@
 """Pins for the oracle's distance functions (the ipc-toolkit v1.2.1 calls at
-model/geometry.cpp:101-112, model/abd.cpp:433-434, model/core_ipc.cpp:30-31,62-67;
+model/geometry.cpp:101-112, model/abd.cpp:353-354, model/core_ipc.cpp:30-31,62-67;
 ipc-toolkit is not vendored by the reference and cannot be fetched here).
 
 * Typed squared distances agree with textbook closest-point routines
