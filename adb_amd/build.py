@@ -28,6 +28,27 @@ def _compile(src):
     return obj
 
 
+def build_pybind(verbose=True):
+    """The reference's `abdtk` module as a pybind11 extension over libadb.so,
+    in adb_amd/pyext/ (import it with that folder on sys.path: `import abdtk`).
+    Host code only (g++); the kernels it calls are libadb.so's."""
+    import sysconfig
+    import pybind11
+    out_dir = os.path.join(HERE, "pyext")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "abdtk" + sysconfig.get_config_var("EXT_SUFFIX"))
+    cmd = ["g++", "-O2", "-shared", "-fPIC", "-std=c++17", "-I", pybind11.get_include(),
+           "-I", sysconfig.get_paths()["include"], os.path.join(CSRC, "abdtk_pybind.cpp"),
+           "-L", HERE, "-ladb", "-Wl,-rpath,$ORIGIN/..", "-o", out + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("pybind11 module build failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
+    os.replace(out + ".tmp", out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
@@ -41,6 +62,7 @@ def build(verbose=True):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_pybind(verbose)
     return LIB
 
 

@@ -193,6 +193,9 @@ def _declare(lib):
         "adb_step_size_upper_bound": (C.c_int, [vp, _dp, _dp]),
         "adb_assemble": (C.c_int, [vp, _dp, _dp]),
         "adb_newton_direction": (C.c_int, [vp, _dp]),
+        "adb_contact_terms": (C.c_int, [vp, _dp, _dp, _dp]),
+        "adb_adjoint_solve": (C.c_int, [vp, _dp, _dp, _ip]),
+        "adb_line_search": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp, _ip]),
         "adb_ipc_hess_pt_12x12_batched": (C.c_int, [C.POINTER(AdbKernelConfig), C.c_int, vp, vp, vp, vp, vp, vp]),
         "adb_ipc_hess_ee_12x12_batched": (C.c_int, [C.POINTER(AdbKernelConfig), C.c_int, vp, vp, vp, vp, vp, vp, vp]),
         "adb_pt_collision_time_batched": (C.c_int, [C.c_int, vp, vp, vp]),
@@ -204,6 +207,7 @@ def _declare(lib):
         "adb_pt_distance_batched": (C.c_int, [C.c_int, vp, vp, vp, vp]),
         "adb_ee_distance_batched": (C.c_int, [C.c_int, vp, vp, vp, vp]),
         "adb_device_count": (C.c_int, []),
+        "adb_collision_time_noalloc": (C.c_int, [C.c_int, C.c_int, _dp, _dp]),
         "adb_profile_enable": (C.c_int, [vp, C.c_int]),
         "adb_profile_read": (C.c_int, [vp, C.c_int, _dp, C.POINTER(C.c_longlong), _dp]),
     }
@@ -217,12 +221,13 @@ def _declare(lib):
 EXPORTED_SYMBOLS = [
     "adb_params_default", "adb_world_create", "adb_world_destroy", "adb_step", "adb_get_state",
     "adb_set_state", "adb_last_error", "adb_collision_set", "adb_step_size_upper_bound",
-    "adb_assemble", "adb_newton_direction", "adb_kernel_config_default",
+    "adb_assemble", "adb_newton_direction", "adb_contact_terms", "adb_line_search", "adb_adjoint_solve",
+    "adb_kernel_config_default",
     "adb_ipc_hess_pt_12x12_batched", "adb_ipc_hess_ee_12x12_batched",
     "adb_pt_collision_time_batched", "adb_ee_collision_time_batched",
     "adb_ipc_hess_pt_12x12", "adb_ipc_hess_ee_12x12", "adb_pt_collision_time",
     "adb_ee_collision_time", "adb_pt_distance_batched", "adb_ee_distance_batched",
-    "adb_device_count", "adb_profile_enable", "adb_profile_read",
+    "adb_device_count", "adb_profile_enable", "adb_profile_read", "adb_collision_time_noalloc",
 ]
 
 # kernel ids of adb_profile_read (enum adb_prof_kernel)

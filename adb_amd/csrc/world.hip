@@ -41,6 +41,59 @@ static void scale(World& w, double* v, int n, double s) {
     ADB_HIP(hipGetLastError());
 }
 
+// Armijo backtracking (model/line_search.cpp:39-136) from w.q along w.dq (the
+// direction already scaled by the CCD bound).  On entry w.X and w.cs hold the
+// positions and the contact set at q, w.grad the gradient there and the
+// lagged friction terms are those of q; gn = ||grad||.  On return w.qtrial,
+// w.X1 and w.cs_new hold the accepted point q + alpha dq.
+static int line_search(World& w, double gn, double* alpha_out, int* trials_out, double* E0_out, double* E1_out) {
+    const int n = 12 * w.B;
+    double sl[3];
+    energy_barrier_inertia_async(w, w.q.p, w.X.p, w.cs, 0);
+    energy_friction_async(w, w.X.p, 1);
+    dot_async(w, w.dq.p, w.grad.p, n, 2);
+    read_slots(w, 3, sl);
+    const double E0 = sl[0] + sl[1];
+    const double qdg = sl[2];
+    double alpha = 1.0, E1 = E0;
+    int trials = 0, status = ADB_OK;
+    // Trial sets from the candidates of the search segment q -> q + alpha dq
+    // (vertex positions are affine in q, so every later trial point lies on
+    // it): one swept pass, then an exact filter per trial.  The swept pass
+    // costs more than one plain pass, so when the previous search took one
+    // trial the first trial gets a plain pass and the candidates are built
+    // only if it is rejected.  Both give the same sets.
+    const bool cand_ok = w.ls_cand && (w.P.iaabb & 1);
+    bool cand = cand_ok && w.ls_prev > 1;
+    if (cand) {
+        q_axpy(w, w.q.p, w.dq.p, 1.0, w.qtrial.p);
+        transform_positions(w, w.qtrial.p, w.X1.p);
+        segment_candidates(w, w.X.p, w.X1.p, w.cand);
+    }
+    while (true) {
+        q_axpy(w, w.q.p, w.dq.p, alpha, w.qtrial.p);
+        transform_positions(w, w.qtrial.p, w.X1.p);
+        if (!cand && cand_ok && trials > 0) {
+            segment_candidates(w, w.X.p, w.X1.p, w.cand);
+            cand = true;
+        }
+        if (cand) trial_set(w, w.X1.p, w.cand, w.cs_new);
+        else collision_set(w, w.X1.p, w.cs_new);
+        energy_barrier_inertia_async(w, w.qtrial.p, w.X1.p, w.cs_new, 0);
+        energy_friction_async(w, w.X1.p, 1);
+        read_slots(w, 2, sl);
+        E1 = sl[0] + sl[1];
+        const bool wolfe = E1 <= E0 + w.P.c1 * alpha * qdg;
+        alpha /= 2;
+        trials++;
+        if (wolfe || !(gn > 1e-3)) break;
+        if (trials >= w.P.max_ls_iter) { status = ADB_E_NOT_CONVERGED; w.err = "line search exhausted"; break; }
+    }
+    w.ls_prev = trials;
+    *alpha_out = alpha * 2; *trials_out = trials; *E0_out = E0; *E1_out = E1;
+    return status;
+}
+
 static int step(World& w, adb_step_stats* st) {
     adb_step_stats S;
     std::memset(&S, 0, sizeof S);
@@ -101,48 +154,10 @@ static int step(World& w, adb_step_stats* st) {
         // Armijo backtracking (model/line_search.cpp:39-136)
         double alpha = 1.0;
         if (w.P.line_search) {
-            energy_barrier_inertia_async(w, w.q.p, w.X.p, w.cs, 0);
-            energy_friction_async(w, w.X.p, 1);
-            dot_async(w, w.dq.p, w.grad.p, n, 2);
-            read_slots(w, 3, sl);
-            const double E0 = sl[0] + sl[1];
-            const double qdg = sl[2];
             int trials = 0;
-            // Trial sets from the candidates of the search segment q -> q + alpha dq
-            // (vertex positions are affine in q, so every later trial point lies on
-            // it): one swept pass, then an exact filter per trial.  The swept pass
-            // costs more than one plain pass, so when the previous search took one
-            // trial the first trial gets a plain pass and the candidates are built
-            // only if it is rejected.  Both give the same sets.
-            const bool cand_ok = w.ls_cand && (w.P.iaabb & 1);
-            bool cand = cand_ok && w.ls_prev > 1;
-            if (cand) {
-                q_axpy(w, w.q.p, w.dq.p, 1.0, w.qtrial.p);
-                transform_positions(w, w.qtrial.p, w.X1.p);
-                segment_candidates(w, w.X.p, w.X1.p, w.cand);
-            }
-            while (true) {
-                q_axpy(w, w.q.p, w.dq.p, alpha, w.qtrial.p);
-                transform_positions(w, w.qtrial.p, w.X1.p);
-                if (!cand && cand_ok && trials > 0) {
-                    segment_candidates(w, w.X.p, w.X1.p, w.cand);
-                    cand = true;
-                }
-                if (cand) trial_set(w, w.X1.p, w.cand, w.cs_new);
-                else collision_set(w, w.X1.p, w.cs_new);
-                energy_barrier_inertia_async(w, w.qtrial.p, w.X1.p, w.cs_new, 0);
-                energy_friction_async(w, w.X1.p, 1);
-                read_slots(w, 2, sl);
-                const double E1 = sl[0] + sl[1];
-                const bool wolfe = E1 <= E0 + w.P.c1 * alpha * qdg;
-                alpha /= 2;
-                trials++;
-                if (wolfe || !(gn > 1e-3)) break;
-                if (trials >= w.P.max_ls_iter) { status = ADB_E_NOT_CONVERGED; w.err = "line search exhausted"; break; }
-            }
+            double E0, E1;
+            status = line_search(w, gn, &alpha, &trials, &E0, &E1);
             S.ls_trials += trials;
-            w.ls_prev = trials;
-            alpha *= 2;
             if (status != ADB_OK) break;
         } else {
             q_axpy(w, w.q.p, w.dq.p, 1.0, w.qtrial.p);
@@ -490,11 +505,8 @@ extern "C" int adb_step_size_upper_bound(adb_world* W, const double* dq, double*
     })
 }
 
-extern "C" int adb_assemble(adb_world* W, double* grad, double* hess) {
-    if (!W) return ADB_E_ARG;
-    ADB_TRY(W, {
-        World& w = W->w;
-        assemble_now(w);
+// the assembled gradient and (when hess != null) the dense Hessian to the host
+static void read_system(World& w, double* grad, double* hess) {
         const int n = 12 * w.B;
         if (grad) ADB_HIP(hipMemcpyAsync(grad, w.grad.p, n * sizeof(double), hipMemcpyDeviceToHost, w.stream));
         if (hess) {
@@ -512,8 +524,93 @@ extern "C" int adb_assemble(adb_world* W, double* grad, double* hess) {
                             hess[(size_t)(12 * ci[k] + c) * n + 12 * r + i] = hv[144 * (size_t)k + 12 * c + i];
         }
         ADB_HIP(hipStreamSynchronize(w.stream));
+}
+
+// The contact set, contact terms and assembly at the Newton iterate q of a
+// step that started at the world's state q0 (uploaded to w.q; w.X / w.Xs
+// hold the positions at q and q0).
+static void terms_at(World& w, const double* q_host) {
+    const size_t n = 12 * (size_t)w.B;
+    ADB_HIP(hipMemcpyAsync(w.q.p, q_host, n * sizeof(double), hipMemcpyHostToDevice, w.stream));
+    transform_positions(w, w.q.p, w.X.p);
+    transform_positions(w, w.q0.p, w.Xs.p);
+    collision_set(w, w.X.p, w.cs);
+    contact_terms(w);
+    assemble_system(w);
+}
+
+extern "C" int adb_assemble(adb_world* W, double* grad, double* hess) {
+    if (!W) return ADB_E_ARG;
+    ADB_TRY(W, {
+        World& w = W->w;
+        assemble_now(w);
+        read_system(w, grad, hess);
         return ADB_OK;
     })
+}
+
+extern "C" int adb_contact_terms(adb_world* W, const double* q, double* grad, double* hess) {
+    if (!W || !q) return ADB_E_ARG;
+    ADB_TRY(W, {
+        World& w = W->w;
+        terms_at(w, q);
+        read_system(w, grad, hess);
+        return ADB_OK;
+    })
+}
+
+extern "C" int adb_line_search(adb_world* W, const double* q, const double* dq, double* alpha, double* E0, double* E1,
+                               int* trials) {
+    if (!W || !q || !dq) return ADB_E_ARG;
+    ADB_TRY(W, {
+        World& w = W->w;
+        const int n = 12 * w.B;
+        terms_at(w, q);
+        ADB_HIP(hipMemcpyAsync(w.dq.p, dq, n * sizeof(double), hipMemcpyHostToDevice, w.stream));
+        double sl[1];
+        dot_async(w, w.grad.p, w.grad.p, n, 0);
+        read_slots(w, 1, sl);
+        double a = 1.0, e0 = 0.0, e1 = 0.0;
+        int t = 0;
+        const int rc = line_search(w, std::sqrt(sl[0]), &a, &t, &e0, &e1);
+        if (alpha) *alpha = a;
+        if (E0) *E0 = e0;
+        if (E1) *E1 = e1;
+        if (trials) *trials = t;
+        return rc;
+    })
+}
+
+extern "C" int adb_adjoint_solve(adb_world* W, const double* rhs, double* x, int* iters) {
+    if (!W || !rhs || !x) return ADB_E_ARG;
+    World& w = W->w;
+    // the lagged friction terms depend on the step's start through Tk and
+    // lambda; the adjoint covers the frictionless step
+    if (w.P.mu != 0.0 && (w.P.pt_fric || w.P.ee_fric || w.P.vg_fric)) {
+        w.err = "adb_adjoint_solve differentiates frictionless steps only (mu = 0)";
+        return ADB_E_ARG;
+    }
+    const int psd = w.P.psd;
+    w.P.psd = 0;                      // exact second derivatives, not their PSD projections
+    int rc = ADB_OK;
+    try {
+        const int n = 12 * w.B;
+        assemble_now(w);
+        // pcg_solve returns -H^{-1} grad: solve with grad = -rhs
+        ADB_HIP(hipMemcpyAsync(w.grad.p, rhs, n * sizeof(double), hipMemcpyHostToDevice, w.stream));
+        scale(w, w.grad.p, n, -1.0);
+        int it = 0;
+        rc = pcg_solve(w, w.dq.p, &it);
+        if (iters) *iters = it;
+        ADB_HIP(hipMemcpyAsync(x, w.dq.p, n * sizeof(double), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipStreamSynchronize(w.stream));
+        if (rc == ADB_E_NOT_CONVERGED) w.err = "adjoint PCG reached pcg_max_iter";
+        if (rc == ADB_E_SOLVER) w.err = "adjoint PCG breakdown: the Hessian at the state is not positive definite";
+    } catch (const HipError& e) { w.err = e.what; rc = ADB_E_HIP; }
+    catch (const AdbError& e) { w.err = e.what; rc = e.code; }
+    catch (...) { w.err = "unknown exception"; rc = ADB_E_ARG; }
+    w.P.psd = psd;
+    return rc;
 }
 
 extern "C" int adb_newton_direction(adb_world* W, double* dq) {

@@ -65,7 +65,7 @@ def quat_to_mat(w, x, y, z):
 
 
 def euler_rotation(a, b, c):
-    """rotation(a, b, c) of model/geometry.cpp:178-194."""
+    """rotation(a, b, c) of model/geometry.cpp:248-264."""
     s1, s2, s3 = math.sin(a), math.sin(b), math.sin(c)
     c1, c2, c3 = math.cos(a), math.cos(b), math.cos(c)
     return np.array([
@@ -178,6 +178,24 @@ def build_scene(bodies):
         "edges": np.concatenate(es).reshape(-1, 2), "mass": np.array(mass), "Ic": np.array(Ic),
         "q0": np.array(q0).reshape(-1, 12), "dqdt": np.array(dq).reshape(-1, 12),
     }
+
+
+def subset(scene, bodies, q=None, qdot=None):
+    """The scene restricted to the listed bodies (in that order), with their
+    rest meshes, mass and Ic; q / qdot (B x 12, optional) replace q0 / dqdt."""
+    bodies = [int(b) for b in bodies]
+    vo, fo, eo = scene["vert_offset"], scene["face_offset"], scene["edge_offset"]
+    q = scene["q0"] if q is None else np.asarray(q).reshape(-1, 12)
+    qdot = scene["dqdt"] if qdot is None else np.asarray(qdot).reshape(-1, 12)
+    out = []
+    for b in bodies:
+        out.append(dict(verts=scene["rest_verts"][vo[b]:vo[b + 1]], tris=scene["faces"][fo[b]:fo[b + 1]],
+                        edges=scene["edges"][eo[b]:eo[b + 1]], p=q[b, :3], mass=float(scene["mass"][b]),
+                        Ic=float(scene["Ic"][b])))
+    sc = build_scene(out)
+    sc["q0"] = np.array(q[bodies], dtype=np.float64).reshape(-1, 12)
+    sc["dqdt"] = np.array(qdot[bodies], dtype=np.float64).reshape(-1, 12)
+    return sc
 
 
 def scene_counts(scene):
@@ -318,35 +336,8 @@ def batched_scenes(n_scenes=256, seed=3, nmin=8, nmax=200, spacing=20.0):
     return build_scene(flat), [len(sc) for sc in scenes]
 
 
-def load_reference_scene_json(path_or_list):
-    """Cube scenes in the reference's JSON format ({p, p_dot, omega, theta|euler,
-    mass?}), as customize() reads them (test_cases/spinning_cube.cpp:56-127)."""
-    import json
-    if isinstance(path_or_list, str):
-        with open(path_or_list) as fh:
-            data = json.load(fh)
-    else:
-        data = path_or_list
-    v, f, e = cube_mesh()
-    bodies = []
-    for it in data:
-        if "obj" in it:
-            raise ValueError("OBJ bodies are not supported by this loader")
-        omega = it["omega"]
-        if "euler" in it:
-            ds = it["euler"]
-            R = euler_rotation(ds[2] / 180.0 * math.pi, ds[1] / 180.0 * math.pi, ds[0] / 180.0 * math.pi)
-        else:
-            th = it["theta"] / 180.0 * math.pi
-            s, c = math.sin(th), math.cos(th)
-            R = np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
-        m = float(it.get("mass", 1000.0))
-        # dqdt[1..3] = skew(omega) (model/geometry.h:41-47)
-        a0, a1, a2 = omega
-        sk = [-np.array([0.0, -a2, a1]), -np.array([a2, 0.0, -a0]), -np.array([-a1, a0, 0.0])]
-        bodies.append(dict(verts=v, tris=f, edges=e, R=R, p=it["p"], v=it["p_dot"], mass=m, Ic=m / 12.0,
-                           _dA=sk))
-    sc = build_scene(bodies)
-    for b, body in enumerate(bodies):
-        sc["dqdt"][b, 3:] = np.concatenate(body["_dA"])
-    return sc
+def load_reference_scene_json(path_or_list, obj_dir=None, meshes=None):
+    """Scenes in the reference's JSON format (test_cases/spinning_cube.cpp:56-127);
+    see adb_amd.scene_io.load_scene_json."""
+    from .scene_io import load_scene_json
+    return load_scene_json(path_or_list, obj_dir=obj_dir, meshes=meshes)

@@ -92,6 +92,35 @@ class World:
         self._check(self.lib.adb_assemble(self.h, dptr(g), dptr(H) if dense else None), "adb_assemble")
         return g, (H.reshape(n, n).T if dense else None)
 
+    def contact_terms(self, q, dense=True):
+        """Gradient and dense Hessian at the Newton iterate q of a step that
+        starts at the current state (friction lagged from the contacts at q)."""
+        n = 12 * self.B
+        g = np.zeros(n)
+        H = np.zeros(n * n) if dense else None
+        qa = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        self._check(self.lib.adb_contact_terms(self.h, dptr(qa), dptr(g), dptr(H) if dense else None),
+                    "adb_contact_terms")
+        return g, (H.reshape(n, n).T if dense else None)
+
+    def line_search(self, q, dq):
+        """(status, alpha, E0, E1, trials) of the Armijo search from q along dq."""
+        a, e0, e1, t = C.c_double(), C.c_double(), C.c_double(), C.c_int()
+        qa = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        da = np.ascontiguousarray(dq, dtype=np.float64).reshape(-1)
+        rc = self.lib.adb_line_search(self.h, dptr(qa), dptr(da), C.byref(a), C.byref(e0), C.byref(e1), C.byref(t))
+        self._check(rc, "adb_line_search", allow=(-6,))
+        return rc, a.value, e0.value, e1.value, t.value
+
+    def adjoint_solve(self, rhs):
+        """x = H^{-1} rhs with the exact Hessian of the last step's incremental
+        potential at the current state (frictionless worlds)."""
+        r = np.ascontiguousarray(rhs, dtype=np.float64).reshape(-1)
+        x = np.zeros(12 * self.B)
+        it = C.c_int()
+        self._check(self.lib.adb_adjoint_solve(self.h, dptr(r), dptr(x), C.byref(it)), "adb_adjoint_solve")
+        return x
+
     def profile_enable(self, on=True):
         self._check(self.lib.adb_profile_enable(self.h, 1 if on else 0), "adb_profile_enable")
 

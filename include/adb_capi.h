@@ -161,9 +161,38 @@ int adb_step_size_upper_bound(adb_world* w, const double* dq, double* toi);
  * Hessian (column-major) when hess_dense != NULL (small scenes only). */
 int adb_assemble(adb_world* w, double* grad, double* hess_dense);
 
+/* ABD::ipc plus the per-body inertia terms at a Newton iterate q ([B*12])
+ * of a step that starts at the world's state (model/abd.cpp:68-90,305-404):
+ * the contact set at q, the barrier terms, and friction lagged from the
+ * contacts at q against the step-start positions.  Writes the gradient and,
+ * when hess_dense != NULL, the dense Hessian as adb_assemble. */
+int adb_contact_terms(adb_world* w, const double* q, double* grad, double* hess_dense);
+
+/* ABD::line_search (model/abd.cpp:198-227, model/line_search.cpp:39-136) from
+ * the iterate q along dq (both [B*12], dq already scaled by the CCD bound):
+ * contact terms at q as adb_contact_terms, then alpha = 1, 1/2, 1/4, ... until
+ * E(q + alpha dq) <= E(q) + c1 alpha dq.g, each trial with the contact set at
+ * its own point.  Returns the accepted alpha, the energies E(q) and
+ * E(q + alpha dq), and the number of trials; ADB_E_NOT_CONVERGED when
+ * max_ls_iter trials fail. */
+int adb_line_search(adb_world* w, const double* q, const double* dq, double* alpha, double* E0, double* E1,
+                    int* trials);
+
 /* Newton direction dq = -H^{-1} g at the current q (ABD::solve,
  * model/abd.cpp:229-303) using PCG preconditioned by contact-group inverses. */
 int adb_newton_direction(adb_world* w, double* dq);
+
+/* Adjoint solve of one converged step (implicit differentiation), for the
+ * differentiable step: x = H^{-1} rhs, with H the exact (not PSD-projected)
+ * Hessian of the step's incremental potential at the world's current state,
+ * i.e. the end of the last adb_step: M + dt^2 Hess(E_ortho) + Hess(barrier)
+ * (model/mutils.cpp:53-76, model/core_ipc.cpp:21-83).  The step's optimality
+ * condition grad E(q*; q_tilde) = 0 with q_tilde = q0 + dt qdot + dt^2 g
+ * (AffineBody::q_tile, model/mutils.cpp:245-252) gives
+ * d(q*) / d(q_tilde) = H^{-1} M.  Frictionless worlds only (ADB_E_ARG when mu != 0
+ * and friction is on); solved by the step's PCG to pcg_rtol; iters may be
+ * NULL.  No counterpart in the reference, which has no adjoint. */
+int adb_adjoint_solve(adb_world* w, const double* rhs, double* x, int* iters);
 
 /* ---- kernel-level API (abdtk) ---------------------------------------------
  * Batched device versions: all pointers are DEVICE pointers, n entries,
@@ -199,6 +228,20 @@ int adb_ipc_hess_ee_12x12(const adb_kernel_config* cfg, const double ee[12], int
                           double dist, double H[144], double g[12], double* mollifier);
 double adb_pt_collision_time(const double x[24]);
 double adb_ee_collision_time(const double x[24]);
+
+/* No counterpart in the reference: a variant of adb_pt_collision_time /
+ * adb_ee_collision_time for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  is_ee selects the edge-edge form; x holds n entries of 8
+ * points (24 doubles, 4 at t0 then 4 at t1) in HOST memory and toi receives n
+ * times of impact in host memory.  It allocates nothing: the inputs and
+ * outputs are staged through a fixed device array of the library (at most
+ * 4096 entries per call, about 800 KB per device, held until the process
+ * ends).  n > 4096 returns ADB_E_CAPACITY, bad arguments ADB_E_ARG.  Calls in
+ * one process are serialized by a lock; the array is shared by every call on a
+ * device, so callers that bypass the lock (another copy of the library in the
+ * same process) would race on it. */
+int adb_collision_time_noalloc(int is_ee, int n, const double* x, double* toi);
 
 /* Distance type + squared distance (ipc-toolkit point_triangle_distance_type /
  * point_triangle_distance and the edge-edge pair, called at

@@ -1443,6 +1443,16 @@ int oracle_line_search(void* p, const double* qv, const double* dq, double* alph
     *alpha_out = alpha * 2; *E0_out = E0; *E1_out = E1; *trials_out = trials;
     return rc;
 }
+// x = H^{-1} rhs with the exact (unprojected) Hessian at the current state:
+// the reference gradient of the differentiable step's adjoint.
+int oracle_adjoint_solve(void* p, const double* rhs, double* x) {
+    World& w = *W(p);
+    const int psd = w.P.psd;
+    w.P.psd = 0;
+    oracle_assemble(p, nullptr, nullptr);
+    w.P.psd = psd;
+    return sparse_solve(w.B, w.blocks, rhs, x);
+}
 int oracle_energy(void* p, const double* qv, double* E) {
     World& w = *W(p);
     std::vector<double> q1(qv, qv + 12 * w.B), X;

@@ -37,6 +37,7 @@ int  oracle_newton_direction(void* w, double* dq);
 /* incremental potential without friction at q: inertia + orthogonality + barriers */
 int  oracle_energy(void* w, const double* q, double* E);
 int  oracle_assemble_at(void* w, const double* q, double* grad, double* hess);
+int  oracle_adjoint_solve(void* w, const double* rhs, double* x);
 int  oracle_friction_energy(void* w, const double* q, double* E);
 int  oracle_line_search(void* w, const double* q, const double* dq, double* alpha, double* E0, double* E1, int* trials);
 int  oracle_set_threads(int n);

@@ -39,6 +39,7 @@ def lib():
         "oracle_newton_direction": (C.c_int, [vp, _dp]),
         "oracle_energy": (C.c_int, [vp, _dp, _dp]),
         "oracle_assemble_at": (C.c_int, [vp, _dp, _dp, _dp]),
+        "oracle_adjoint_solve": (C.c_int, [vp, _dp, _dp]),
         "oracle_friction_energy": (C.c_int, [vp, _dp, _dp]),
         "oracle_line_search": (C.c_int, [vp, _dp, _dp, _dp, _dp, _dp, _ip]),
         "oracle_set_threads": (C.c_int, [C.c_int]),
@@ -260,15 +261,20 @@ class OracleWorld:
         qd = None if qdot is None else _d(qdot).reshape(-1)
         lib().oracle_set_state(self.h, dptr(q), dptr(qd) if qd is not None else None)
 
-    def collision_set(self, vtn=1, dq=None, method=1, cap=2_000_000):
-        pt = np.zeros(4 * cap, dtype=np.int32)
-        ee = np.zeros(4 * cap, dtype=np.int32)
-        vg = np.zeros(2 * cap, dtype=np.int32)
-        npt, nee, nvg = C.c_int(), C.c_int(), C.c_int()
-        dqa = None if dq is None else _d(dq).reshape(-1)
-        rc = lib().oracle_collision_set(self.h, vtn, dptr(dqa) if dqa is not None else None, method,
-                                        iptr(pt), cap, C.byref(npt), iptr(ee), cap, C.byref(nee),
-                                        iptr(vg), cap, C.byref(nvg))
+    def collision_set(self, vtn=1, dq=None, method=1, cap=1 << 20):
+        while True:
+            pt = np.zeros(4 * cap, dtype=np.int32)
+            ee = np.zeros(4 * cap, dtype=np.int32)
+            vg = np.zeros(2 * cap, dtype=np.int32)
+            npt, nee, nvg = C.c_int(), C.c_int(), C.c_int()
+            dqa = None if dq is None else _d(dq).reshape(-1)
+            rc = lib().oracle_collision_set(self.h, vtn, dptr(dqa) if dqa is not None else None, method,
+                                            iptr(pt), cap, C.byref(npt), iptr(ee), cap, C.byref(nee),
+                                            iptr(vg), cap, C.byref(nvg))
+            if rc == -5:      # ADB_E_CAPACITY: the true counts were written
+                cap = max(npt.value, nee.value, nvg.value) + 16
+                continue
+            break
         if rc != 0:
             raise RuntimeError("oracle_collision_set: %d" % rc)
         return (pt[:4 * npt.value].reshape(-1, 4).copy(), ee[:4 * nee.value].reshape(-1, 4).copy(),
@@ -305,6 +311,14 @@ class OracleWorld:
         qa = _d(q).reshape(-1)
         lib().oracle_assemble_at(self.h, dptr(qa), dptr(g), dptr(H) if dense else None)
         return g, (H.reshape(n, n).T if dense else None)
+
+    def adjoint_solve(self, rhs):
+        x = np.zeros(12 * self.B)
+        r = _d(rhs).reshape(-1)
+        rc = lib().oracle_adjoint_solve(self.h, dptr(r), dptr(x))
+        if rc != 0:
+            raise RuntimeError("oracle_adjoint_solve: %d" % rc)
+        return x
 
     def friction_energy(self, q):
         e = C.c_double()

@@ -1,0 +1,171 @@
+"""GPU parity of the stage entry points and of the error paths, through the C ABI.
+
+* adb_contact_terms at a sliding Newton iterate (q != q0, so the lagged
+  point-triangle / edge-edge / vertex-ground friction acts) against the
+  oracle's assembly at the same iterate: gradient and Hessian within 1e-9 of
+  the largest entry.
+* adb_line_search against the oracle's: the same trial count and alpha,
+  energies within 1e-12 relative.
+* Error paths (the reference exits instead, model/abd.cpp:288-291,
+  model/iaabb.cpp:305-311,533-537): a body below the ground moving down stalls
+  the CCD bound (ADB_E_TOI_STALL) as in the oracle; max_iter = 1 returns
+  ADB_E_NOT_CONVERGED with the state advanced exactly as the oracle's; a
+  one-iteration PCG limit is counted in the step statistics; a culling pass
+  whose body pairs and keys outgrow the buffers of the previous pass grows
+  them and repeats, with sets still bit-exact; caller capacities too small
+  return ADB_E_CAPACITY with the true counts.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from adb_amd import capi, scenes
+from adb_amd.capi import default_params
+from adb_amd.world import AdbError, World
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _require_gpu():
+    assert torch.cuda.is_available(), "GPU tests need an MI355X"
+
+
+def stacked_pair(top_size, angle_deg, gap):
+    v0, f0, e0 = scenes.cube_mesh()
+    v1, f1, e1 = scenes.cube_mesh((top_size, top_size, top_size))
+    c, s = math.cos(math.radians(angle_deg)), math.sin(math.radians(angle_deg))
+    R = np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])
+    return scenes.build_scene([dict(verts=v0, tris=f0, edges=e0, p=(0.0, 0.0, 0.0)),
+                               dict(verts=v1, tris=f1, edges=e1, R=R, p=(0.05, 0.5 + top_size / 2 + gap, -0.03))])
+
+
+def sliding_cases():
+    gap = 0.5 * math.sqrt(1e-3)
+    out = []
+    for size, ang in ((0.5, 30.0), (1.0, 45.0)):
+        sc = stacked_pair(size, ang, gap)
+        sc["q0"][0, 1] = -0.5 + 0.5 + gap     # the lower cube rests near the ground: VG terms too
+        sc["q0"][1, 1] += gap
+        q = sc["q0"].copy()
+        q[1, 0] += 3e-3
+        q[1, 2] -= 1.5e-3
+        q[0, 0] -= 2e-3                       # both slide: relative and ground friction
+        out.append((sc, q))
+    sc = scenes.cube_stack(4, seed=3, rotate=True)
+    q = sc["q0"].copy()
+    q[:, :3] += np.random.default_rng(4).normal(0, 2e-3, size=(4, 3))
+    out.append((sc, q))
+    return out
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_contact_terms_at_sliding_iterate(case):
+    _require_gpu()
+    sc, q = sliding_cases()[case]
+    for mu in (0.5, 0.0):
+        p = default_params(d_hat=1e-3, kappa=1e-4, mu=mu, dt=0.01)
+        gw, ow = World(sc, p), O.OracleWorld(sc, p)
+        gg, Hg = gw.contact_terms(q)
+        go, Ho = ow.assemble_at(q)
+        np.testing.assert_allclose(gg, go, rtol=0, atol=1e-9 * np.abs(go).max())
+        np.testing.assert_allclose(Hg, Ho, rtol=0, atol=1e-9 * np.abs(Ho).max())
+    pt, ee, vg = ow.collision_set(vtn=2, dq=q - sc["q0"])
+    assert len(pt) + len(ee) > 0
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_line_search_matches_oracle(case):
+    _require_gpu()
+    sc, q = sliding_cases()[case]
+    p = default_params(d_hat=1e-3, kappa=1e-4, mu=0.5, dt=0.01)
+    gw, ow = World(sc, p), O.OracleWorld(sc, p)
+    go, _ = ow.assemble_at(q)
+    rng = np.random.default_rng(case)
+    seen = set()
+    for scale in (1e-5, 1e-3, 3e-2):
+        dq = -go * scale / np.abs(go).max() + rng.normal(0, 1e-2 * scale, size=go.shape)
+        rg, ag, e0g, e1g, tg = gw.line_search(q, dq)
+        ro, ao, e0o, e1o, to = ow.line_search(q, dq)
+        assert rg == ro == 0
+        assert (tg, ag) == (to, ao), (scale, tg, to, ag, ao)
+        assert e0g == pytest.approx(e0o, rel=1e-12) and e1g == pytest.approx(e1o, rel=1e-12)
+        seen.add(tg)
+    assert max(seen) > 1
+
+
+def test_toi_stall_is_reported():
+    _require_gpu()
+    sc = scenes.cube_stack(1, seed=1, jitter=0.0, y0=-0.02)    # bottom face 2 cm below the ground
+    sc["dqdt"][0, 1] = -1.0
+    p = default_params(d_hat=1e-3)
+    gw, ow = World(sc, p), O.OracleWorld(sc, p)
+    ro, _ = ow.step()
+    with pytest.raises(AdbError) as e:
+        gw.step()
+    assert e.value.code == capi.ADB_E_TOI_STALL == ro
+
+
+def test_max_iter_one_not_converged_like_oracle():
+    _require_gpu()
+    sc = scenes.cube_stack(3, seed=1)
+    p = default_params(d_hat=1e-3, mu=0.5, max_iter=1, tol=1e-12)
+    gw, ow = World(sc, p), O.OracleWorld(sc, p)
+    for s in range(5):
+        rg, sg = gw.step(allow_not_converged=True)
+        ro, so = ow.step()
+        assert rg == ro == capi.ADB_E_NOT_CONVERGED and sg.newton_iters == so.newton_iters == 1
+        np.testing.assert_allclose(gw.state()[0], ow.state()[0], rtol=0, atol=1e-10)
+        np.testing.assert_allclose(gw.state()[1], ow.state()[1], rtol=0, atol=1e-8)
+    with pytest.raises(AdbError):
+        gw.step()                              # strict by default
+
+
+def test_pcg_iteration_limit_is_counted():
+    _require_gpu()
+    sc = scenes.cube_stack(6, seed=1, spacing=1.0 + 0.5 * math.sqrt(1e-3))
+    p = default_params(d_hat=1e-3, mu=0.5, pcg_max_iter=1)
+    gw = World(sc, p)
+    counted = 0
+    for s in range(5):
+        rc, st = gw.step(allow_not_converged=True)
+        counted += st.pcg_not_converged
+    assert counted > 0
+    assert "PCG" in gw.lib.adb_last_error(gw.h).decode()
+
+
+def test_culling_buffers_grow_and_retry():
+    _require_gpu()
+    sc = scenes.random_cubes(60, seed=8, box=6.0)           # sparse: few pairs, small buffers
+    p = default_params(d_hat=0.05)
+    gw, ow = World(sc, p), O.OracleWorld(sc, p)
+    sparse = gw.collision_set()
+    q = sc["q0"].copy()
+    q[:, :3] *= 0.2                                          # then packed: many more pairs and keys
+    gw.set_state(q)
+    ow.set_state(q)
+    for a, b, what in zip(gw.collision_set(), ow.collision_set(), ("pt", "ee", "vg")):
+        assert a.tolist() == b.tolist(), what
+    dense = ow.collision_set()
+    assert len(dense[0]) + len(dense[1]) > 20 * max(1, len(sparse[0]) + len(sparse[1]))
+    dq = np.random.default_rng(2).normal(0, 0.05, size=q.shape)
+    assert gw.toi(dq) == ow.toi(dq)
+
+
+def test_caller_capacity_too_small():
+    _require_gpu()
+    sc = scenes.cube_stack(3, seed=1)
+    gw = World(sc, default_params(d_hat=1e-3))
+    import ctypes as C
+    pt = np.zeros(4, dtype=np.int32)
+    ee = np.zeros(4, dtype=np.int32)
+    vg = np.zeros(2, dtype=np.int32)
+    n = [C.c_int(), C.c_int(), C.c_int()]
+    rc = gw.lib.adb_collision_set(gw.h, 1, None, capi.iptr(pt), 1, C.byref(n[0]), capi.iptr(ee), 1, C.byref(n[1]),
+                                  capi.iptr(vg), 1, C.byref(n[2]))
+    full = gw.collision_set()
+    assert rc == capi.ADB_E_CAPACITY
+    assert [x.value for x in n] == [len(full[0]), len(full[1]), len(full[2])]
