@@ -52,19 +52,22 @@ def test_reference_example_and_random_contacts(abdtk):
     assert torch.cuda.is_available(), "GPU tests need an MI355X"
     arr = np.array([[0.4360, 0.0259, 0.5497], [0.4353, 0.4204, 0.3303],
                     [0.2046, 0.6193, 0.2997], [0.2668, 0.6211, 0.5291]])
-    t, d = O.pt_distance(arr.reshape(-1))
-    assert t == 6
-    # the example's squared point-plane distance is far beyond d_hat; scale the
-    # configuration into the barrier band as well
     cfg = kernel_config()
+    # as the example: type P_T with the squared point-plane distance (there
+    # ipctk.point_plane_distance); the example's distance is beyond d_hat, so
+    # the configuration is also scaled into the barrier band
     for scale in (1.0, 0.01):
         P = arr * scale
-        t, d = O.pt_distance(P.reshape(-1))
+        n = np.cross(P[2] - P[1], P[3] - P[1])
+        d = float(np.dot(P[0] - P[1], n) ** 2 / np.dot(n, n))
         H, g = abdtk.ipc_hess_pt_12x12([P[i] for i in range(4)], list(range(4)), abdtk.P_T, d)
-        Hc, gc = ct.ipc_hess_pt_12x12(P, None, t, d)
+        Hc, gc = ct.ipc_hess_pt_12x12(P, None, 6, d)
         assert np.array_equal(H, Hc) and np.array_equal(g, gc)
-        Ho, go = O.ipc_hess_pt(cfg, P.reshape(-1), t, d)
+        Ho, go = O.ipc_hess_pt(cfg, P.reshape(-1), 6, d)
+        if d < cfg.d_hat:
+            assert np.abs(Ho).max() > 0
         np.testing.assert_allclose(H, Ho, rtol=0, atol=1e-9 * max(1e-300, np.abs(Ho).max()))
+        np.testing.assert_allclose(g, go, rtol=0, atol=1e-9 * max(1e-300, np.abs(go).max()))
     rng = np.random.default_rng(3)
     for _ in range(50):
         P = rng.uniform(-1, 1, size=(4, 3)) * 0.01

@@ -126,7 +126,9 @@ def test_max_iter_one_not_converged_like_oracle():
 
 def test_pcg_iteration_limit_is_counted():
     _require_gpu()
-    sc = scenes.cube_stack(6, seed=1, spacing=1.0 + 0.5 * math.sqrt(1e-3))
+    # 12 touching cubes: one contact component larger than a preconditioner
+    # group (8 bodies), so one PCG iteration cannot solve the system
+    sc = scenes.cube_stack(12, seed=1, spacing=1.0 + 0.5 * math.sqrt(1e-3))
     p = default_params(d_hat=1e-3, mu=0.5, pcg_max_iter=1)
     gw = World(sc, p)
     counted = 0
