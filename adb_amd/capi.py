@@ -185,6 +185,8 @@ def _declare(lib):
         "adb_kernel_config_default": (None, [C.POINTER(AdbKernelConfig)]),
         "adb_world_create": (C.c_int, [C.POINTER(AdbScene), C.POINTER(AdbParams), C.POINTER(vp)]),
         "adb_world_destroy": (None, [vp]),
+        "adb_world_create_batched": (C.c_int, [C.POINTER(AdbScene), _ip, C.c_int, C.POINTER(AdbParams), C.POINTER(vp)]),
+        "adb_batch_stats": (C.c_int, [vp, _ip, _ip, _ip, _ip]),
         "adb_step": (C.c_int, [vp, C.POINTER(AdbStepStats)]),
         "adb_get_state": (C.c_int, [vp, _dp, _dp]),
         "adb_set_state": (C.c_int, [vp, _dp, _dp]),
@@ -219,7 +221,8 @@ def _declare(lib):
 
 # Every symbol include/adb_capi.h declares (checked by tests/test_capi_symbols.py).
 EXPORTED_SYMBOLS = [
-    "adb_params_default", "adb_world_create", "adb_world_destroy", "adb_step", "adb_get_state",
+    "adb_params_default", "adb_world_create", "adb_world_create_batched", "adb_batch_stats", "adb_world_destroy",
+    "adb_step", "adb_get_state",
     "adb_set_state", "adb_last_error", "adb_collision_set", "adb_step_size_upper_bound",
     "adb_assemble", "adb_newton_direction", "adb_contact_terms", "adb_line_search", "adb_adjoint_solve",
     "adb_kernel_config_default",

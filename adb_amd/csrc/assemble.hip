@@ -663,6 +663,19 @@ static void build_groups(World& w) {
         w.h_grp_moff.push_back(w.h_grp_moff.back() + n * n);
     }
     w.n_groups = (int)members.size();
+    if (w.S > 1) {
+        // groups are numbered in order of their smallest body and never span two
+        // scenes (no body pair crosses scenes): scene s owns groups
+        // [h_sc_goff[s], h_sc_goff[s + 1])
+        w.h_sc_goff.assign(w.S + 1, w.n_groups);
+        int g = 0;
+        for (int sc = 0; sc < w.S; sc++) {
+            while (g < w.n_groups && members[g][0] < w.h_scene_off[sc]) g++;
+            w.h_sc_goff[sc] = g;
+        }
+        w.sc_goff.reserve(w.S + 1);
+        ADB_HIP(hipMemcpyAsync(w.sc_goff.p, w.h_sc_goff.data(), (w.S + 1) * sizeof(int), hipMemcpyHostToDevice, w.stream));
+    }
     // groups with several bodies first: their inverses take longest
     w.h_glist.clear();
     for (int g = 0; g < w.n_groups; g++) if (w.h_grp_off[g + 1] - w.h_grp_off[g] > 1) w.h_glist.push_back(g);

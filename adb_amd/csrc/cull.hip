@@ -57,6 +57,26 @@ void q_axpy(World& w, const double* q, const double* dq, double alpha, double* o
     hipLaunchKernelGGL(k_axpy, dim3((n + 255) / 256), dim3(256), 0, w.stream, n, q, dq, alpha, out);
     ADB_HIP(hipGetLastError());
 }
+// per-scene factors: out = q + f[scene] dq; v *= f[scene]
+__global__ void k_axpy_scenes(int n, const int* __restrict__ scene_of, const double* __restrict__ q,
+                              const double* __restrict__ dq, const double* __restrict__ f, double* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = q[i] + dq[i] * f[scene_of[i / 12]];
+}
+__global__ void k_scale_scenes(int n, const int* __restrict__ scene_of, double* __restrict__ v, const double* __restrict__ f) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = v[i] * f[scene_of[i / 12]];
+}
+void q_axpy_scenes(World& w, const double* q, const double* dq, const double* f, double* out) {
+    int n = 12 * w.B;
+    hipLaunchKernelGGL(k_axpy_scenes, dim3((n + 255) / 256), dim3(256), 0, w.stream, n, w.scene_of.p, q, dq, f, out);
+    ADB_HIP(hipGetLastError());
+}
+void scale_scenes(World& w, double* v, const double* f) {
+    int n = 12 * w.B;
+    hipLaunchKernelGGL(k_scale_scenes, dim3((n + 255) / 256), dim3(256), 0, w.stream, n, w.scene_of.p, v, f);
+    ADB_HIP(hipGetLastError());
+}
 
 // ------------------------------------------------------------------ body boxes
 // one workgroup per body; X1 != null: union of both position sets
@@ -146,6 +166,56 @@ __global__ void k_body_pairs(int B, const double* __restrict__ box, const double
     }
 }
 
+// outward rounding of doubles to FP32
+__device__ __forceinline__ float round_down(double x) { float f = (float)x; return (double)f > x ? nextafterf(f, -INFINITY) : f; }
+__device__ __forceinline__ float round_up(double x) { float f = (float)x; return (double)f < x ? nextafterf(f, INFINITY) : f; }
+
+// ------------------------------------------------------------------ sort and sweep
+// Bodies sorted by (scene, box min x rounded down to FP32); each sorted body
+// scans the following ones of its scene while their min x does not pass its
+// max x (rounded up), and tests those with the same box and sphere tests as
+// k_body_pairs: the same pairs, without testing all B^2 / 2.
+__device__ __forceinline__ unsigned int f32_order(float x) {
+    unsigned int b = __float_as_uint(x);
+    return (b >> 31) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float f32_from_order(unsigned int k) {
+    unsigned int b = (k >> 31) ? (k & 0x7fffffffu) : ~k;
+    return __uint_as_float(b);
+}
+__global__ void k_sweep_keys(int B, const int* __restrict__ scene_of, const double* __restrict__ box,
+                             unsigned long long* __restrict__ keys, int* __restrict__ ids) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const float lo = round_down(box[6 * (size_t)b]);
+    keys[b] = ((unsigned long long)(scene_of ? scene_of[b] : 0) << 32) | f32_order(lo);
+    ids[b] = b;
+}
+__global__ void k_sweep_pairs(int B, const unsigned long long* __restrict__ keys, const int* __restrict__ ids,
+                              const double* __restrict__ box, const double* __restrict__ sph, double infl,
+                              int2* __restrict__ pairs, int cap, int* __restrict__ counter) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B) return;
+    const int I = ids[i];
+    const unsigned int scene = (unsigned int)(keys[i] >> 32);
+    const double* a = box + 6 * (size_t)I;
+    const float hx = round_up(a[3]);
+    const double* sa = sph + 4 * (size_t)I;
+    for (int j = i + 1; j < B; j++) {
+        const unsigned long long kj = keys[j];
+        if ((unsigned int)(kj >> 32) != scene || f32_from_order((unsigned int)kj) > hx) break;
+        const int J = ids[j];
+        const double* bb = box + 6 * (size_t)J;
+        if (!ovl(a, a + 3, bb, bb + 3)) continue;
+        const double* sb = sph + 4 * (size_t)J;
+        const double dx = sa[0] - sb[0], dy = sa[1] - sb[1], dz = sa[2] - sb[2];
+        const double lim = (sa[3] + sb[3] + infl) * (1.0 + 1e-9);
+        if (dx * dx + dy * dy + dz * dz > lim * lim) continue;
+        const int k = atomicAdd(counter, 1);
+        if (k < cap) pairs[k] = make_int2(min(I, J), max(I, J));
+    }
+}
+
 // ------------------------------------------------------------------ narrow phase
 // One workgroup per overlapping body pair, three passes (vertices of I vs
 // faces of J, vertices of J vs faces of I, edges of I vs edges of J):
@@ -179,7 +249,8 @@ struct NarrowArgs {
     unsigned long long* pt_keys; unsigned long long* ee_keys;   // NBK buckets of `sub` keys each
     int sub;
     int* bcnt;                            // bucket counters: [0, NBK) pt, [NBK, 2 NBK) ee
-    unsigned long long* toi_bits;         // CCD: [1] pt/ee min
+    unsigned long long* toi_bits;         // CCD: [1] pt/ee min; per scene [2 s + 1] when scene_of
+    const int* scene_of;                  // batched worlds: body -> scene (CCD minimum per scene)
 };
 
 template <int CCD>
@@ -194,9 +265,7 @@ __device__ __forceinline__ void prim_box(const double* X0, const double* X1, con
         }
     }
 }
-// outward rounding to the box type
-__device__ __forceinline__ float round_down(double x) { float f = (float)x; return (double)f > x ? nextafterf(f, -INFINITY) : f; }
-__device__ __forceinline__ float round_up(double x) { float f = (float)x; return (double)f < x ? nextafterf(f, INFINITY) : f; }
+// outward rounding to the box type: round_down / round_up (above)
 template <class BT> __device__ __forceinline__ BT lo_of(double x);
 template <class BT> __device__ __forceinline__ BT hi_of(double x);
 template <> __device__ __forceinline__ float lo_of<float>(double x) { return round_down(x); }
@@ -450,9 +519,22 @@ __global__ __launch_bounds__(NB_THREADS) void k_pair_narrow(NarrowArgs a) {
                 }
             }
         }
+        if (CCD == 1 && a.scene_of) {
+            // batched world: this pair's minimum goes to its scene's slot
+            for (int m = 32; m >= 1; m >>= 1) tmin = fmin(tmin, __shfl_xor(tmin, m));
+            __syncthreads();
+            if (__lane_id() == 0) bmin[threadIdx.x / 64] = tmin;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double t = bmin[0];
+                for (int i = 1; i < NB_THREADS / 64; i++) t = fmin(t, bmin[i]);
+                if (t < 1.0) atomicMin(&a.toi_bits[2 * a.scene_of[pr.x] + 1], order_key(t));
+            }
+            tmin = 1.0;
+        }
     }   // pairs
     if constexpr (CCD != 1) flush_keys(a, kind, *kb);
-    if (CCD == 1) {
+    if (CCD == 1 && !a.scene_of) {
         for (int m = 32; m >= 1; m >>= 1) tmin = fmin(tmin, __shfl_xor(tmin, m));
         if (__lane_id() == 0) bmin[threadIdx.x / 64] = tmin;
         __syncthreads();
@@ -477,7 +559,7 @@ __global__ void k_vg_scan(int V, const double* __restrict__ X, double ground_y, 
 }
 // vertex-ground CCD (model/collision.cpp:83-105)
 __global__ void k_vg_toi(int V, const double* __restrict__ X0, const double* __restrict__ X1, double ground_y,
-                         unsigned long long* toi_bits) {
+                         unsigned long long* toi_bits, const int* __restrict__ vbody, const int* __restrict__ scene_of) {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     double t = 1.0;
     if (v < V) {
@@ -486,6 +568,10 @@ __global__ void k_vg_toi(int V, const double* __restrict__ X0, const double* __r
             double d1 = X0[3 * (size_t)v + 1] - ground_y;
             t = fmin(1.0, d1 / (d1 - d2));
         }
+    }
+    if (scene_of) {   // batched world: each vertex's scene slot (rare: only vertices that reach the ground)
+        if (t < 1.0) atomicMin(&toi_bits[2 * scene_of[vbody[v]]], order_key(t));
+        return;
     }
     for (int m = 32; m >= 1; m >>= 1) t = fmin(t, __shfl_xor(t, m));
     if (__lane_id() == 0 && t < 1.0) atomicMin(&toi_bits[0], order_key(t));
@@ -623,6 +709,24 @@ static void body_pairs(World& w, const double* X0, const double* X1, double infl
     ADB_HIP(hipGetLastError());
     w.pairs.reserve(std::max<size_t>(w.pairs.cap, std::max(4096, 16 * w.B)));
     ADB_HIP(hipMemsetAsync(cnt, 0, sizeof(int), w.stream));
+    if (w.sweep) {
+        const int B = w.B;
+        w.swk.reserve(B); w.swk_s.reserve(B); w.swi.reserve(B); w.swi_s.reserve(B);
+        ProfScope ps(w, PK_BODY_PAIRS, 48.0 * w.B);
+        hipLaunchKernelGGL(k_sweep_keys, dim3((B + 255) / 256), dim3(256), 0, w.stream, B,
+                           w.S > 1 ? (const int*)w.scene_of.p : (const int*)nullptr, w.bbox.p, w.swk.p, w.swi.p);
+        ADB_HIP(hipGetLastError());
+        int hi = 32;
+        while (hi < 64 && (1ll << (hi - 32)) <= w.S) hi++;
+        size_t tb = 0;
+        ADB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, w.swk.p, w.swk_s.p, w.swi.p, w.swi_s.p, B, 0, hi, w.stream));
+        w.sort_tmp.reserve(tb + 16);
+        ADB_HIP(hipcub::DeviceRadixSort::SortPairs(w.sort_tmp.p, tb, w.swk.p, w.swk_s.p, w.swi.p, w.swi_s.p, B, 0, hi, w.stream));
+        hipLaunchKernelGGL(k_sweep_pairs, dim3((B + 127) / 128), dim3(128), 0, w.stream, B, w.swk_s.p, w.swi_s.p, w.bbox.p,
+                           w.bsph.p, infl, w.pairs.p, (int)w.pairs.cap, cnt);
+        ADB_HIP(hipGetLastError());
+        return;
+    }
     dim3 grid((w.B + 15) / 16, (w.B + 15) / 16), blk(16, 16);
     {
         ProfScope ps(w, PK_BODY_PAIRS, 48.0 * w.B);
@@ -646,6 +750,7 @@ static NarrowArgs narrow_args(World& w, const int* npairs, const double* X0, con
     a.enable_ee = w.P.enable_ee && w.E > 0;
     a.pt_keys = nullptr; a.ee_keys = nullptr; a.sub = 0; a.bcnt = nullptr;
     a.toi_bits = nullptr;
+    a.scene_of = nullptr;
     return a;
 }
 
@@ -865,7 +970,8 @@ double ccd_toi(World& w, const double* X0, const double* X1, int* status) {
     ADB_HIP(hipMemcpyAsync(w.toi_bits.p, init, sizeof init, hipMemcpyHostToDevice, w.stream));
     if (w.P.enable_ground) {
         ProfScope ps(w, PK_VG_TOI, 16.0 * w.V);
-        hipLaunchKernelGGL(k_vg_toi, dim3((w.V + 255) / 256), dim3(256), 0, w.stream, w.V, X0, X1, w.P.ground_y, w.toi_bits.p);
+        hipLaunchKernelGGL(k_vg_toi, dim3((w.V + 255) / 256), dim3(256), 0, w.stream, w.V, X0, X1, w.P.ground_y, w.toi_bits.p,
+                           w.vbody.p, (const int*)nullptr);
         ADB_HIP(hipGetLastError());
     }
     const bool hashed = !(w.P.iaabb & 2);    // spatial hashing (hash.hip) instead of body overlaps
@@ -898,6 +1004,56 @@ double ccd_toi(World& w, const double* X0, const double* X1, int* status) {
     if (tg < 1e-6) { if (++w.stall_vg > 1) *status = ADB_E_TOI_STALL; } else w.stall_vg = 0;
     if (tp < 1e-6) { if (++w.stall_pt > 1) *status = ADB_E_TOI_STALL; } else w.stall_pt = 0;
     return tg < tp ? tg : tp;
+}
+
+// Per-scene CCD bounds of a batched world (slots 2 s: vertex-ground, 2 s + 1:
+// PT / EE), stalls counted per scene as ccd_toi counts them for one scene.
+void ccd_toi_scenes(World& w, const double* X0, const double* X1, double* toi, int* status) {
+    const int S = w.S;
+    w.toi_bits.reserve(2 * (size_t)S);
+    std::vector<unsigned long long> bits(2 * (size_t)S, order_key_host(1.0));
+    ADB_HIP(hipMemcpyAsync(w.toi_bits.p, bits.data(), bits.size() * sizeof(unsigned long long), hipMemcpyHostToDevice, w.stream));
+    if (w.P.enable_ground) {
+        ProfScope ps(w, PK_VG_TOI, 16.0 * w.V);
+        hipLaunchKernelGGL(k_vg_toi, dim3((w.V + 255) / 256), dim3(256), 0, w.stream, w.V, X0, X1, w.P.ground_y, w.toi_bits.p,
+                           w.vbody.p, w.scene_of.p);
+        ADB_HIP(hipGetLastError());
+    }
+    w.bcnt.reserve(2 * NBK + 2);
+    int* pcnt = w.bcnt.p + 2 * NBK + 1;
+    for (int attempt = 0; attempt < 2; attempt++) {
+        int np = 0;
+        body_pairs(w, X0, X1, 0.0, pcnt);
+        NarrowArgs a = narrow_args(w, pcnt, X0, X1, 0.0);
+        a.toi_bits = w.toi_bits.p;
+        a.scene_of = w.scene_of.p;
+        {
+            ProfScope ps(w, PK_NARROW_CCD, 0.0);
+            const dim3 grid(narrow_grid(w), NARROW_TASKS);
+            if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<1, float>), grid, dim3(NB_THREADS), 0, w.stream, a);
+            else hipLaunchKernelGGL((k_pair_narrow<1, double>), grid, dim3(NB_THREADS), 0, w.stream, a);
+            ADB_HIP(hipGetLastError());
+        }
+        ADB_HIP(hipMemcpyAsync(&np, pcnt, sizeof(int), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipMemcpyAsync(bits.data(), w.toi_bits.p, bits.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipStreamSynchronize(w.stream));
+        w.pair_hint = np;
+        if ((size_t)np <= w.pairs.cap) break;
+        w.pairs.reserve(np);
+        std::fill(bits.begin(), bits.end(), order_key_host(1.0));
+        ADB_HIP(hipMemcpyAsync(w.toi_bits.p, bits.data(), bits.size() * sizeof(unsigned long long), hipMemcpyHostToDevice, w.stream));
+        if (w.P.enable_ground)
+            hipLaunchKernelGGL(k_vg_toi, dim3((w.V + 255) / 256), dim3(256), 0, w.stream, w.V, X0, X1, w.P.ground_y, w.toi_bits.p,
+                               w.vbody.p, w.scene_of.p);
+    }
+    w.h_stall_pt.resize(S, 0); w.h_stall_vg.resize(S, 0);
+    for (int s = 0; s < S; s++) {
+        const double tg = order_key_inv(bits[2 * s]), tp = order_key_inv(bits[2 * s + 1]);
+        status[s] = ADB_OK;
+        if (tg < 1e-6) { if (++w.h_stall_vg[s] > 1) status[s] = ADB_E_TOI_STALL; } else w.h_stall_vg[s] = 0;
+        if (tp < 1e-6) { if (++w.h_stall_pt[s] > 1) status[s] = ADB_E_TOI_STALL; } else w.h_stall_pt[s] = 0;
+        toi[s] = tg < tp ? tg : tp;
+    }
 }
 
 } // namespace adb

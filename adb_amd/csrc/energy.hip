@@ -215,4 +215,125 @@ void energy_friction_async(World& w, const double* X, int slot) {
     reduce_async(w, buf.p, nullptr, n, slot);
 }
 
+// ---------------------------------------------------------------- per scene
+// Sums over the ranges r < R of scene s: [base_r + mult_r off_r[s],
+// base_r + mult_r off_r[s + 1]) of a[i] (b == null) or a[i] b[i]; one
+// workgroup per scene, terms in a fixed order (strided, then a tree).
+struct SceneRanges { int R; const int* off[4]; long long base[4]; int mult[4]; };
+__global__ __launch_bounds__(RT) void k_scene_sum(int S, SceneRanges rg, const double* __restrict__ a,
+                                                  const double* __restrict__ b, double* __restrict__ out, int accumulate) {
+    __shared__ double sh[RT];
+    const int s = blockIdx.x;
+    double acc = 0.0;
+    for (int r = 0; r < rg.R; r++) {
+        const long long lo = rg.base[r] + (long long)rg.mult[r] * rg.off[r][s];
+        const long long hi = rg.base[r] + (long long)rg.mult[r] * rg.off[r][s + 1];
+        for (long long i = lo + threadIdx.x; i < hi; i += RT) acc += b ? a[i] * b[i] : a[i];
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int m = RT / 2; m >= 1; m >>= 1) {
+        if ((int)threadIdx.x < m) sh[threadIdx.x] += sh[threadIdx.x + m];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[s] = accumulate ? out[s] + sh[0] : sh[0];
+}
+// first entry of each scene in the sorted PT, EE and VG lists (entries are
+// sorted by their first body, bodies by scene): seg[k (S + 1) + s], k = list
+__global__ void k_scene_ranges(int S, const int* __restrict__ scene_off, int n_pt, const int4* __restrict__ pt, int n_ee,
+                               const int4* __restrict__ ee, int n_vg, const int2* __restrict__ vg, int* __restrict__ seg) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * (S + 1)) return;
+    const int k = t / (S + 1), s = t % (S + 1);
+    const int n = k == 0 ? n_pt : (k == 1 ? n_ee : n_vg);
+    if (s == S) { seg[t] = n; return; }
+    const int b0 = scene_off[s];
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        const int I = k == 0 ? pt[mid].x : (k == 1 ? ee[mid].x : vg[mid].x);
+        if (I < b0) lo = mid + 1; else hi = mid;
+    }
+    seg[t] = lo;
+}
+
+static void scene_ranges(World& w, const ContactSet& cs) {
+    const int S = w.S;
+    w.seg.reserve(3 * (size_t)(S + 1));
+    hipLaunchKernelGGL(k_scene_ranges, dim3((3 * (S + 1) + 255) / 256), dim3(256), 0, w.stream, S, w.scene_off.p, cs.n_pt, cs.pt.p,
+                       cs.n_ee, cs.ee.p, cs.n_vg, cs.vg.p, w.seg.p);
+    ADB_HIP(hipGetLastError());
+}
+
+void dot_scenes_async(World& w, const double* a, const double* b, int slot) {
+    SceneRanges rg;
+    rg.R = 1; rg.off[0] = w.scene_off.p; rg.base[0] = 0; rg.mult[0] = 12;
+    w.sred.reserve((size_t)SRED_N * w.S);
+    hipLaunchKernelGGL(k_scene_sum, dim3(w.S), dim3(RT), 0, w.stream, w.S, rg, a, b, w.sred.p + (size_t)slot * w.S, 0);
+    ADB_HIP(hipGetLastError());
+}
+
+// E_barrier_inert + E_fric per scene (model/energy.cpp:19-86,195-270)
+void energy_scenes_async(World& w, const double* qv, const double* X, const ContactSet& cs, int slot) {
+    const int S = w.S;
+    w.sred.reserve((size_t)SRED_N * S);
+    double* out = w.sred.p + (size_t)slot * S;
+    scene_ranges(w, cs);
+    const int* seg = w.seg.p;
+    DBuf<double>& buf = items(w);
+    const int n = w.B + cs.n_pt + cs.n_ee + cs.n_vg;
+    buf.reserve(std::max(n, 12 * w.B));
+    {
+        EnergyArgs a;
+        a.B = w.B; a.n_pt = cs.n_pt; a.n_ee = cs.n_ee; a.n_vg = cs.n_vg;
+        a.dt = w.P.dt; a.kap_o = w.P.kappa_ortho; a.d_hat = w.P.d_hat; a.kappa = w.P.kappa; a.eps_x = w.P.eps_x;
+        a.ground_y = w.P.ground_y; a.g0 = w.P.gravity[0]; a.g1 = w.P.gravity[1]; a.g2 = w.P.gravity[2];
+        a.en_pt = w.P.enable_pt; a.en_ee = w.P.enable_ee; a.en_g = w.P.enable_ground;
+        a.qv = qv; a.q0 = w.q0.p; a.qdot = w.qdot.p; a.mass = w.mass.p; a.Ic = w.Ic.p; a.X = X;
+        a.pt = cs.pt.p; a.ee = cs.ee.p; a.vg = cs.vg.p;
+        a.voff = w.voff.p; a.foff = w.foff.p; a.eoff = w.eoff.p; a.face = w.face.p; a.edge = w.edge.p;
+        a.out = buf.p;
+        ProfScope ps(w, PK_ENERGY, 0.0);
+        hipLaunchKernelGGL(k_energy_items, dim3((n + 255) / 256), dim3(256), 0, w.stream, a);
+        ADB_HIP(hipGetLastError());
+        SceneRanges rg;
+        rg.R = 4;
+        rg.off[0] = w.scene_off.p; rg.base[0] = 0; rg.mult[0] = 1;
+        rg.off[1] = seg; rg.base[1] = w.B; rg.mult[1] = 1;
+        rg.off[2] = seg + (S + 1); rg.base[2] = (long long)w.B + cs.n_pt; rg.mult[2] = 1;
+        rg.off[3] = seg + 2 * (S + 1); rg.base[3] = (long long)w.B + cs.n_pt + cs.n_ee; rg.mult[3] = 1;
+        hipLaunchKernelGGL(k_scene_sum, dim3(S), dim3(RT), 0, w.stream, S, rg, buf.p, (const double*)nullptr, out, 0);
+        ADB_HIP(hipGetLastError());
+    }
+    // lagged friction on the step's contact set w.cs (its own ranges)
+    const ContactSet& fc = w.cs;
+    const int nf = fc.n_pt + fc.n_ee + fc.n_vg;
+    if (nf == 0) return;
+    scene_ranges(w, fc);
+    FricArgs a;
+    a.n_pt = fc.n_pt; a.n_ee = fc.n_ee; a.n_vg = fc.n_vg;
+    a.mu = w.P.mu; a.eps_v = w.P.dt * w.P.evh;
+    a.pt_f = w.P.pt_fric; a.ee_f = w.P.ee_fric; a.vg_f = w.P.vg_fric;
+    a.en_pt = w.P.enable_pt; a.en_ee = w.P.enable_ee; a.en_g = w.P.enable_ground;
+    a.X = X; a.Xs = w.Xs.p; a.pt = fc.pt.p; a.ee = fc.ee.p; a.vg = fc.vg.p;
+    a.voff = w.voff.p; a.foff = w.foff.p; a.eoff = w.eoff.p; a.face = w.face.p; a.edge = w.edge.p;
+    a.tk = w.ctk.p; a.lam = w.clam.p; a.vglam = w.vglam.p;
+    a.out = buf.p;
+    hipLaunchKernelGGL(k_fric_items, dim3((nf + 255) / 256), dim3(256), 0, w.stream, a);
+    ADB_HIP(hipGetLastError());
+    SceneRanges rg;
+    rg.R = 3;
+    rg.off[0] = seg; rg.base[0] = 0; rg.mult[0] = 1;
+    rg.off[1] = seg + (S + 1); rg.base[1] = fc.n_pt; rg.mult[1] = 1;
+    rg.off[2] = seg + 2 * (S + 1); rg.base[2] = (long long)fc.n_pt + fc.n_ee; rg.mult[2] = 1;
+    hipLaunchKernelGGL(k_scene_sum, dim3(S), dim3(RT), 0, w.stream, S, rg, buf.p, (const double*)nullptr, out, 1);
+    ADB_HIP(hipGetLastError());
+}
+
+void read_scene_sums(World& w, int nslots) {
+    if (!w.h_sred) ADB_HIP(hipHostMalloc((void**)&w.h_sred, (size_t)SRED_N * w.S * sizeof(double), hipHostMallocDefault));
+    ADB_HIP(hipMemcpyAsync(w.h_sred, w.sred.p, (size_t)nslots * w.S * sizeof(double), hipMemcpyDeviceToHost, w.stream));
+    ADB_HIP(hipStreamSynchronize(w.stream));
+}
+
 } // namespace adb

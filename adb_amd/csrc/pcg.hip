@@ -121,6 +121,30 @@ __global__ __launch_bounds__(64) void k_pcg_init(int B, Groups G, const double* 
     rr = wave_sum(rr);
     if (l == 0) { partB[gi] = rz; partB[B + gi] = rr; }
 }
+// Batched worlds: each scene's residual test, |r_s|^2 <= rtol^2 |b_s|^2, from
+// the group partials (groups are numbered scene by scene); first = the
+// partials of iteration 0 (|b_s|^2 = |r_s|^2 then).  One workgroup; brun = 1
+// while some scene is above its tolerance.
+__global__ __launch_bounds__(1024) void k_pcg_scene_stop(int S, int B, const int* __restrict__ sc_goff, double rtol2, int first,
+                                                         const double* __restrict__ partB, double* __restrict__ sbb,
+                                                         const double* __restrict__ scal, int* __restrict__ brun) {
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    if (scal[S_STOP] == RUN) {
+        int mine = 0;
+        for (int s = threadIdx.x; s < S; s += blockDim.x) {
+            double rr = 0.0;
+            for (int g = sc_goff[s]; g < sc_goff[s + 1]; g++) rr += partB[B + g];
+            if (first) sbb[s] = rr;
+            if (!(rr <= rtol2 * sbb[s])) mine = 1;
+        }
+        if (mine) any = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *brun = any;
+}
+
 // iteration base of the next batch
 __global__ void k_pcg_advance(int* kbase, int n) { if (threadIdx.x == 0 && blockIdx.x == 0) *kbase += n; }
 
@@ -131,7 +155,7 @@ __global__ __launch_bounds__(64 * SP_ROWS) void k_pcg_spmv(int B, const int* __r
                                                            const double* __restrict__ z, const double* __restrict__ pold,
                                                            double* __restrict__ pnew, double* __restrict__ y,
                                                            const double* __restrict__ partB, double* __restrict__ partA,
-                                                           double* __restrict__ scal) {
+                                                           double* __restrict__ scal, const int* __restrict__ brun) {
     __shared__ double sh[SP_ROWS];
     __shared__ double acc_sh[SP_ROWS][5][12];
     if (scal[S_STOP] != RUN) return;
@@ -143,7 +167,7 @@ __global__ __launch_bounds__(64 * SP_ROWS) void k_pcg_spmv(int B, const int* __r
     const double bb = iter == 0 ? RR : scal[S_BB];
     int stop = RUN;
     if (!std::isfinite(RR) || !std::isfinite(RZ)) stop = NONFINITE;
-    else if (RR <= rtol2 * bb) stop = CONVERGED;
+    else if (brun ? *brun == 0 : RR <= rtol2 * bb) stop = CONVERGED;   // batched: every scene's own test
     else if (!(RZ > 0.0)) stop = BREAKDOWN;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (iter == 0) scal[S_BB] = RR;
@@ -274,7 +298,8 @@ static void enqueue_batch(World& w, int nb, bool prof) {
             // H blocks, their column indices, z and p_old read, p_new and y written
             ProfScope ps(w, PK_SPMV, prof ? 1152.0 * w.nnzb + 4.0 * w.nnzb + 4.0 * 96.0 * B : 0.0);
             hipLaunchKernelGGL(k_pcg_spmv, dim3(nbS), dim3(64 * SP_ROWS), 0, w.stream, B, w.pcg_k.p, k, rtol2, w.grp_n.p, w.row_ptr.p,
-                               w.col_idx.p, w.Hval.p, w.pcg_z.p, pb[k & 1], pb[(k & 1) ^ 1], w.pcg_Ap.p, partB, partA, w.scal.p);
+                               w.col_idx.p, w.Hval.p, w.pcg_z.p, pb[k & 1], pb[(k & 1) ^ 1], w.pcg_Ap.p, partB, partA, w.scal.p,
+                               w.S > 1 ? (const int*)w.pcg_brun.p : (const int*)nullptr);
         }
         {
             // group inverses read; p, y read; x, r read and written; z written
@@ -282,6 +307,9 @@ static void enqueue_batch(World& w, int nb, bool prof) {
             hipLaunchKernelGGL(k_pcg_update, dim3(B), dim3(UPD_T), 0, w.stream, B, w.pcg_k.p, k, nbS, G, pb[(k & 1) ^ 1],
                                w.pcg_Ap.p, w.pcg_x.p, w.pcg_r.p, w.pcg_z.p, partA, partB, w.scal.p);
         }
+        if (w.S > 1)
+            hipLaunchKernelGGL(k_pcg_scene_stop, dim3(1), dim3(1024), 0, w.stream, w.S, B, w.sc_goff.p, rtol2, 0, partB, w.pcg_sbb.p,
+                               w.scal.p, w.pcg_brun.p);
     }
     hipLaunchKernelGGL(k_pcg_advance, dim3(1), dim3(64), 0, w.stream, w.pcg_k.p, nb);
 }
@@ -298,6 +326,12 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
     hipLaunchKernelGGL(k_pcg_init, dim3(B), dim3(64), 0, w.stream, B, groups_of(w), w.grad.p, w.pcg_x.p, w.pcg_r.p, w.pcg_z.p,
                        w.pcg_p.p, partB, w.scal.p, w.pcg_k.p);
     ADB_HIP(hipGetLastError());
+    if (w.S > 1) {
+        w.pcg_sbb.reserve(w.S); w.pcg_brun.reserve(1);
+        hipLaunchKernelGGL(k_pcg_scene_stop, dim3(1), dim3(1024), 0, w.stream, w.S, B, w.sc_goff.p,
+                           w.P.pcg_rtol * w.P.pcg_rtol, 1, partB, w.pcg_sbb.p, w.scal.p, w.pcg_brun.p);
+        ADB_HIP(hipGetLastError());
+    }
     // the batch graph is captured once per set of buffer addresses (not while
     // per-kernel profiling is on, nor with ADB_PCG_GRAPH=0 for tools that do
     // not support graph capture)
@@ -305,9 +339,10 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
     if (use_graph) {
         const void* key[] = {w.row_ptr.p, w.col_idx.p, w.Hval.p, w.Ginv.p, w.pcg_x.p, w.pcg_r.p, w.pcg_z.p, w.pcg_p.p,
                              w.pcg_p2.p, w.pcg_Ap.p, w.partial.p, w.scal.p, w.pcg_k.p, w.grp_off.p, w.grp_body.p, w.grp_moff.p,
-                             w.grp_n.p};
+                             w.grp_n.p, w.sc_goff.p, w.pcg_sbb.p, w.pcg_brun.p};
+        static_assert(sizeof key / sizeof key[0] == World::PCG_GRAPH_KEYS, "graph key size");
         bool same = w.pcg_graph != nullptr;
-        for (int i = 0; i < 17; i++) same = same && w.pcg_graph_key[i] == key[i];
+        for (int i = 0; i < World::PCG_GRAPH_KEYS; i++) same = same && w.pcg_graph_key[i] == key[i];
         if (!same) {
             if (w.pcg_graph) { ADB_HIP(hipGraphExecDestroy(w.pcg_graph)); w.pcg_graph = nullptr; }
             hipGraph_t g;
@@ -316,7 +351,7 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
             ADB_HIP(hipStreamEndCapture(w.stream, &g));
             ADB_HIP(hipGraphInstantiate(&w.pcg_graph, g, nullptr, nullptr, 0));
             ADB_HIP(hipGraphDestroy(g));
-            for (int i = 0; i < 17; i++) w.pcg_graph_key[i] = key[i];
+            for (int i = 0; i < World::PCG_GRAPH_KEYS; i++) w.pcg_graph_key[i] = key[i];
         }
     }
     int it = 0, slot = 0;
@@ -348,7 +383,8 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
         const double rtol2 = w.P.pcg_rtol * w.P.pcg_rtol;
         double* partA = w.partial.p;
         hipLaunchKernelGGL(k_pcg_spmv, dim3(nbS), dim3(64 * SP_ROWS), 0, w.stream, B, w.pcg_k.p, 0, rtol2, w.grp_n.p, w.row_ptr.p,
-                           w.col_idx.p, w.Hval.p, w.pcg_z.p, w.pcg_p.p, w.pcg_p2.p, w.pcg_Ap.p, partB, partA, w.scal.p);
+                           w.col_idx.p, w.Hval.p, w.pcg_z.p, w.pcg_p.p, w.pcg_p2.p, w.pcg_Ap.p, partB, partA, w.scal.p,
+                           w.S > 1 ? (const int*)w.pcg_brun.p : (const int*)nullptr);
     }
     ADB_HIP(hipGetLastError());
     ADB_HIP(hipMemcpyAsync(w.h_scal, w.scal.p, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, w.stream));

@@ -181,7 +181,8 @@ struct World {
     hipEvent_t ev_pairs = nullptr;   // their copy is done
     bool use_graphs = true;          // ADB_PCG_GRAPH=0 turns graph capture off
     hipGraphExec_t pcg_graph = nullptr;   // one captured PCG batch
-    const void* pcg_graph_key[17] = {};   // buffer addresses the graph was captured with
+    static constexpr int PCG_GRAPH_KEYS = 20;
+    const void* pcg_graph_key[PCG_GRAPH_KEYS] = {};   // buffer addresses the graph was captured with
     double* h_scal = nullptr;        // pinned host copies of the PCG scalars (2 x 8)
     hipEvent_t pcg_ev[2] = {nullptr, nullptr};
     // ---- energies
@@ -192,7 +193,28 @@ struct World {
     int stall_pt = 0, stall_vg = 0;
     int num_cu = 256;
     Prof prof;
+    // ---- independent scenes (adb_world_create_batched; S == 1 otherwise).
+    // Bodies of scene s are [h_scene_off[s], h_scene_off[s + 1]); bodies of
+    // different scenes never form a pair.
+    int S = 1;
+    std::vector<int> h_scene_off;
+    DBuf<int> scene_off, scene_of;        // device copies; body -> scene
+    bool sweep = false;                   // body pairs by sort and sweep (large or batched worlds)
+    DBuf<unsigned long long> swk, swk_s;  // sweep keys (scene, box min x), sorted
+    DBuf<int> swi, swi_s;                 // body ids, sorted with the keys
+    DBuf<int> seg;                        // per contact list, per scene: first entry (3 x (S + 1))
+    DBuf<double> sred;                    // per-scene sums (SRED_N x S)
+    DBuf<double> sfac;                    // per-scene factor (CCD scale, line-search alpha)
+    DBuf<int> sc_goff;                    // per scene: first preconditioner group (S + 1)
+    std::vector<int> h_sc_goff;
+    DBuf<double> pcg_sbb;                 // per scene: |b|^2 of the current solve
+    DBuf<int> pcg_brun;                   // 1 while some scene's residual is above its tolerance
+    double* h_sred = nullptr;             // pinned host copy of sred
+    std::vector<int> h_stall_pt, h_stall_vg;
+    // per-scene results of the last step
+    std::vector<int> sc_newton, sc_ls, sc_status, sc_contacts;
 };
+constexpr int SRED_N = 4;   // per-scene sum slots
 
 // Brackets a kernel launch with events when profiling is on; `bytes` is the
 // launch's algorithmic HBM traffic.
@@ -230,6 +252,16 @@ void collision_set(World& w, const double* X, ContactSet& out);                 
 void segment_candidates(World& w, const double* X0, const double* X1, ContactSet& cand);
 void trial_set(World& w, const double* X, const ContactSet& cand, ContactSet& out);
 double ccd_toi(World& w, const double* X0, const double* X1, int* status);              // cull.hip
+// per-scene CCD bounds of a batched world into toi[S] (host), stalls per scene into status[S]
+void ccd_toi_scenes(World& w, const double* X0, const double* X1, double* toi, int* status);   // cull.hip
+// out = q + f[scene(body)] dq (f: per-scene factors on the device)
+void q_axpy_scenes(World& w, const double* q, const double* dq, const double* f, double* out); // cull.hip
+void scale_scenes(World& w, double* v, const double* f);                                     // cull.hip
+// per-scene sums into w.sred[slot * S + s]: sum of a[i] (b[i]) over scene s's body
+// rows (a, b: 12 per body), or its energy items (barrier + inertia and friction) at q, X
+void dot_scenes_async(World& w, const double* a, const double* b, int slot);                 // energy.hip
+void energy_scenes_async(World& w, const double* qv, const double* X, const ContactSet& cs, int slot); // energy.hip
+void read_scene_sums(World& w, int nslots);                                                 // energy.hip: into w.h_sred
 void hash_sets(World& w, const double* X, double r, unsigned long long* pt_keys, unsigned long long* ee_keys,
                int sub, int* bcnt);                                                      // hash.hip
 void hash_ccd(World& w, const double* X0, const double* X1, unsigned long long* toi_bits); // hash.hip

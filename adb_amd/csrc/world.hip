@@ -28,6 +28,12 @@ __global__ void k_velocity(int n, double dt, const double* __restrict__ q, doubl
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) { qdot[i] = (q[i] - q0[i]) / dt; q0[i] = q[i]; }
 }
+// batched worlds: scenes whose step failed (keep[scene] == 0) keep their state
+__global__ void k_velocity_scenes(int n, double dt, const int* __restrict__ scene_of, const double* __restrict__ keep,
+                                  const double* __restrict__ q, double* __restrict__ q0, double* __restrict__ qdot) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && keep[scene_of[i / 12]] != 0.0) { qdot[i] = (q[i] - q0[i]) / dt; q0[i] = q[i]; }
+}
 
 static double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
 
@@ -184,6 +190,183 @@ static int step(World& w, adb_step_stats* st) {
     return status;
 }
 
+// One step of every scene of a batched world (adb_world_create_batched).  The
+// control flow of step() per scene: each scene has its own Newton iteration
+// count, convergence test on its own ||dq||, CCD bound and backoff, and
+// Armijo search with its own alpha, energies and gradient; a scene that has
+// converged keeps its state (its gradient is zeroed, so its PCG rows stay at
+// dq = 0) while the others iterate.  Every stage is one launch sequence over
+// all scenes; per-scene scalars are segmented sums over each scene's bodies
+// and contacts.
+static int step_batched(World& w, adb_step_stats* st) {
+    adb_step_stats S_;
+    std::memset(&S_, 0, sizeof S_);
+    S_.toi_min = 1.0;
+    const double t_start = now_s();
+    const int S = w.S, n = 12 * w.B;
+    std::vector<char> active(S, 1);
+    std::vector<int> newton(S, 0), lst(S, 0), status(S, ADB_OK);
+    std::vector<double> fac(S), alpha(S), toi(S), E0(S), qdg(S), gn(S), sup(S);
+    std::vector<int> cst(S);
+    w.sfac.reserve(S);
+    auto put_factors = [&](const std::vector<double>& f) {
+        ADB_HIP(hipMemcpyAsync(w.sfac.p, f.data(), S * sizeof(double), hipMemcpyHostToDevice, w.stream));
+    };
+    ADB_HIP(hipMemcpyAsync(w.q.p, w.q0.p, n * sizeof(double), hipMemcpyDeviceToDevice, w.stream));
+    transform_positions(w, w.q.p, w.X.p);
+    transform_positions(w, w.q0.p, w.Xs.p);
+    collision_set(w, w.X.p, w.cs);
+    int iters = 0;
+    for (int iter = 0; iter < w.P.max_iter; iter++) {
+        bool any = false;
+        for (int s = 0; s < S; s++) any = any || active[s];
+        if (!any) break;
+        iters++;
+        const double t0 = now_s();
+        contact_terms(w);
+        assemble_system(w);
+        S_.contacts_processed += (long long)w.cs.n_pt + w.cs.n_ee + w.cs.n_vg;
+        // scenes that have converged: zero gradient, so dq = 0 on their rows
+        for (int s = 0; s < S; s++) fac[s] = active[s] ? 1.0 : 0.0;
+        put_factors(fac);
+        scale_scenes(w, w.grad.p, w.sfac.p);
+        ADB_HIP(hipStreamSynchronize(w.stream));
+        const double t1 = now_s();
+        S_.t_ipc += t1 - t0;
+        int pit = 0;
+        const int prc = pcg_solve(w, w.dq.p, &pit);
+        S_.pcg_iters += pit;
+        if (prc == ADB_E_NOT_CONVERGED) { S_.pcg_not_converged++; w.err = "PCG reached pcg_max_iter"; }
+        else if (prc == ADB_E_SOLVER) { S_.pcg_breakdown++; w.err = "PCG breakdown (system not positive definite)"; }
+        dot_scenes_async(w, w.dq.p, w.dq.p, 0);
+        dot_scenes_async(w, w.grad.p, w.grad.p, 1);
+        read_scene_sums(w, 2);
+        for (int s = 0; s < S; s++) {
+            sup[s] = prc == ADB_E_NAN ? NAN : std::sqrt(w.h_sred[s]);
+            gn[s] = std::sqrt(w.h_sred[S + s]);
+            if (active[s] && std::isnan(sup[s])) { status[s] = ADB_E_NAN; active[s] = 0; }
+        }
+        const double t2 = now_s();
+        S_.t_solver += t2 - t1;
+        // CCD bounds (model/abd.cpp:142-196), per scene
+        for (int s = 0; s < S; s++) toi[s] = 1.0;
+        if (w.P.upper_bound) {
+            q_axpy(w, w.q.p, w.dq.p, 1.0, w.qtrial.p);
+            transform_positions(w, w.qtrial.p, w.X1.p);
+            ccd_toi_scenes(w, w.X.p, w.X1.p, toi.data(), cst.data());
+            for (int s = 0; s < S; s++)
+                if (active[s] && cst[s] != ADB_OK) { status[s] = cst[s]; active[s] = 0; }
+        }
+        for (int s = 0; s < S; s++) {
+            fac[s] = active[s] ? (toi[s] < 1.0 ? w.P.backoff : 1.0) * toi[s] : 0.0;
+            if (active[s]) S_.toi_min = std::min(S_.toi_min, toi[s]);
+        }
+        put_factors(fac);
+        scale_scenes(w, w.dq.p, w.sfac.p);
+        const double t3 = now_s();
+        S_.t_ccd += t3 - t2;
+        // Armijo backtracking (model/line_search.cpp:39-136), per scene
+        std::vector<char> done(S), exhausted(S, 0);
+        for (int s = 0; s < S; s++) { alpha[s] = active[s] ? 1.0 : 0.0; done[s] = !active[s]; }
+        if (w.P.line_search) {
+            energy_scenes_async(w, w.q.p, w.X.p, w.cs, 0);
+            dot_scenes_async(w, w.dq.p, w.grad.p, 1);
+            read_scene_sums(w, 2);
+            for (int s = 0; s < S; s++) { E0[s] = w.h_sred[s]; qdg[s] = w.h_sred[S + s]; }
+            const bool cand_ok = w.ls_cand;
+            bool cand = false;
+            if (cand_ok) {
+                q_axpy(w, w.q.p, w.dq.p, 1.0, w.qtrial.p);
+                transform_positions(w, w.qtrial.p, w.X1.p);
+                segment_candidates(w, w.X.p, w.X1.p, w.cand);
+                cand = true;
+            }
+            std::vector<int> trials(S, 0);
+            while (true) {
+                put_factors(alpha);
+                q_axpy_scenes(w, w.q.p, w.dq.p, w.sfac.p, w.qtrial.p);
+                transform_positions(w, w.qtrial.p, w.X1.p);
+                if (cand) trial_set(w, w.X1.p, w.cand, w.cs_new);
+                else collision_set(w, w.X1.p, w.cs_new);
+                energy_scenes_async(w, w.qtrial.p, w.X1.p, w.cs_new, 0);
+                read_scene_sums(w, 1);
+                bool all = true;
+                for (int s = 0; s < S; s++) {
+                    if (done[s]) continue;
+                    const double E1 = w.h_sred[s];
+                    trials[s]++;
+                    if (E1 <= E0[s] + w.P.c1 * alpha[s] * qdg[s] || !(gn[s] > 1e-3)) { done[s] = 1; continue; }
+                    if (trials[s] >= w.P.max_ls_iter) {
+                        // as step(): the scene stops at the iterate before this search
+                        status[s] = ADB_E_NOT_CONVERGED; done[s] = 1; active[s] = 0; exhausted[s] = 1;
+                        continue;
+                    }
+                    alpha[s] /= 2;
+                    all = false;
+                }
+                if (all) break;
+            }
+            for (int s = 0; s < S; s++) { lst[s] += trials[s]; S_.ls_trials += trials[s]; }
+            bool redo = false;
+            for (int s = 0; s < S; s++)
+                if (exhausted[s] && alpha[s] != 0.0) { alpha[s] = 0.0; redo = true; }
+            if (redo) {
+                put_factors(alpha);
+                q_axpy_scenes(w, w.q.p, w.dq.p, w.sfac.p, w.qtrial.p);
+                transform_positions(w, w.qtrial.p, w.X1.p);
+                collision_set(w, w.X1.p, w.cs_new);
+            }
+        } else {
+            put_factors(alpha);
+            q_axpy_scenes(w, w.q.p, w.dq.p, w.sfac.p, w.qtrial.p);
+            transform_positions(w, w.qtrial.p, w.X1.p);
+            collision_set(w, w.X1.p, w.cs_new);
+        }
+        std::swap(w.q, w.qtrial);
+        std::swap(w.X, w.X1);
+        std::swap(w.cs, w.cs_new);
+        S_.t_line_search += now_s() - t3;
+        for (int s = 0; s < S; s++) {
+            if (!active[s]) continue;
+            newton[s]++;
+            if (sup[s] < w.P.tol) active[s] = 0;
+        }
+    }
+    for (int s = 0; s < S; s++)
+        if (active[s] && status[s] == ADB_OK) status[s] = ADB_E_NOT_CONVERGED;
+    // velocity update (model/abd.cpp:128-134) for the scenes whose step ended
+    // with ADB_OK or ADB_E_NOT_CONVERGED, as step()
+    for (int s = 0; s < S; s++) fac[s] = (status[s] == ADB_OK || status[s] == ADB_E_NOT_CONVERGED) ? 1.0 : 0.0;
+    put_factors(fac);
+    hipLaunchKernelGGL(k_velocity_scenes, dim3((n + 255) / 256), dim3(256), 0, w.stream, n, w.P.dt, w.scene_of.p, w.sfac.p,
+                       w.q.p, w.q0.p, w.qdot.p);
+    ADB_HIP(hipGetLastError());
+    // contacts per scene at the end of the step
+    {
+        std::vector<int> pt, ee, vg;
+        download_sets(w, w.cs, pt, ee, vg);
+        std::vector<int> cnt(S, 0);
+        auto scene_of_body = [&](int b) {
+            return (int)(std::upper_bound(w.h_scene_off.begin(), w.h_scene_off.end(), b) - w.h_scene_off.begin()) - 1;
+        };
+        for (size_t i = 0; i < pt.size(); i += 4) cnt[scene_of_body(pt[i])]++;
+        for (size_t i = 0; i < ee.size(); i += 4) cnt[scene_of_body(ee[i])]++;
+        for (size_t i = 0; i < vg.size(); i += 2) cnt[scene_of_body(vg[i])]++;
+        w.sc_contacts = cnt;
+    }
+    ADB_HIP(hipStreamSynchronize(w.stream));
+    w.sc_newton = newton; w.sc_ls = lst; w.sc_status = status;
+    S_.newton_iters = *std::max_element(newton.begin(), newton.end());
+    S_.n_pt = w.cs.n_pt; S_.n_ee = w.cs.n_ee; S_.n_vg = w.cs.n_vg;
+    S_.t_total = now_s() - t_start;
+    if (st) *st = S_;
+    int rc = ADB_OK;
+    for (int s = 0; s < S; s++)
+        if (status[s] != ADB_OK && (rc == ADB_OK || rc == ADB_E_NOT_CONVERGED)) rc = status[s];
+    (void)iters;
+    return rc;
+}
+
 void download_sets(World& w, const ContactSet& cs, std::vector<int>& pt, std::vector<int>& ee, std::vector<int>& vg) {
     pt.resize(4 * (size_t)cs.n_pt); ee.resize(4 * (size_t)cs.n_ee); vg.resize(2 * (size_t)cs.n_vg);
     if (cs.n_pt) ADB_HIP(hipMemcpyAsync(pt.data(), cs.pt.p, pt.size() * sizeof(int), hipMemcpyDeviceToHost, w.stream));
@@ -259,6 +442,9 @@ extern "C" int adb_world_create(const adb_scene* s, const adb_params* p, adb_wor
         w.h_foff.assign(s->face_offset, s->face_offset + w.B + 1);
         w.h_eoff.assign(s->edge_offset, s->edge_offset + w.B + 1);
         w.V = w.h_voff[w.B]; w.F = w.h_foff[w.B]; w.E = w.h_eoff[w.B];
+        // body pairs by sort and sweep above a few thousand bodies (the
+        // all-pairs test costs B^2 / 2); both give the same pairs
+        w.sweep = w.B > 4096;
         if (w.V <= 0) throw AdbError(ADB_E_ARG, "scene has no vertices");
         ADB_HIP(hipStreamCreate(&w.stream));
         hipDeviceProp_t prop;
@@ -338,6 +524,10 @@ static void free_world(World& w) {
     w.akeys.free_(); w.akeys_s.free_(); w.tkeys.free_(); w.tkeys_s.free_();
     for (ContactSet* c : {&w.cs, &w.cs_new, &w.cand}) { c->pt.free_(); c->ee.free_(); c->vg.free_(); }
     w.vsel.free_(); w.keepbits.free_();
+    w.scene_off.free_(); w.scene_of.free_(); w.swk.free_(); w.swk_s.free_(); w.swi.free_(); w.swi_s.free_();
+    w.seg.free_(); w.sred.free_(); w.sfac.free_(); w.sc_goff.free_(); w.pcg_sbb.free_(); w.pcg_brun.free_();
+    if (w.h_sred) (void)hipHostFree(w.h_sred);
+    w.h_sred = nullptr;
     for (auto& r : w.prof.pending) { w.prof.pool.push_back(r.a); w.prof.pool.push_back(r.b); }
     w.prof.pending.clear();
     for (hipEvent_t e : w.prof.pool) (void)hipEventDestroy(e);
@@ -375,7 +565,52 @@ extern "C" void adb_world_destroy(adb_world* W) {
 
 extern "C" int adb_step(adb_world* W, adb_step_stats* st) {
     if (!W) return ADB_E_ARG;
-    ADB_TRY(W, { return step(W->w, st); })
+    ADB_TRY(W, { return W->w.S > 1 ? step_batched(W->w, st) : step(W->w, st); })
+}
+
+extern "C" int adb_world_create_batched(const adb_scene* s, const int* scene_off, int n_scenes, const adb_params* p,
+                                        adb_world** out) {
+    if (!s || !scene_off || !p || !out || n_scenes <= 0 || scene_off[0] != 0 || scene_off[n_scenes] != s->n_bodies)
+        return ADB_E_ARG;
+    for (int i = 0; i < n_scenes; i++)
+        if (scene_off[i + 1] < scene_off[i]) return ADB_E_ARG;
+    if ((p->iaabb & 3) != 3) return ADB_E_ARG;      // contact sets and CCD by iAABB body overlaps
+    int rc = adb_world_create(s, p, out);
+    if (rc != ADB_OK) return rc;
+    World& w = (*out)->w;
+    try {
+        w.S = n_scenes;
+        w.sweep = true;
+        w.h_scene_off.assign(scene_off, scene_off + n_scenes + 1);
+        std::vector<int> of(w.B);
+        for (int sc = 0; sc < n_scenes; sc++)
+            for (int b = scene_off[sc]; b < scene_off[sc + 1]; b++) of[b] = sc;
+        upload(w, w.scene_off, w.h_scene_off.data(), n_scenes + 1);
+        upload(w, w.scene_of, of.data(), w.B);
+        w.sred.reserve((size_t)SRED_N * n_scenes);
+        w.sfac.reserve(n_scenes);
+        w.pcg_sbb.reserve(n_scenes); w.pcg_brun.reserve(1);
+        w.sc_goff.reserve(n_scenes + 1);
+        ADB_HIP(hipStreamSynchronize(w.stream));
+    } catch (const HipError& e) {
+        adb_world_destroy(*out); *out = nullptr; return ADB_E_HIP;
+    } catch (...) {
+        adb_world_destroy(*out); *out = nullptr; return ADB_E_ARG;
+    }
+    return ADB_OK;
+}
+
+extern "C" int adb_batch_stats(adb_world* W, int* newton_iters, int* ls_trials, int* status, int* contacts) {
+    if (!W) return ADB_E_ARG;
+    World& w = W->w;
+    const size_t S = w.sc_newton.size();
+    for (size_t i = 0; i < S; i++) {
+        if (newton_iters) newton_iters[i] = w.sc_newton[i];
+        if (ls_trials) ls_trials[i] = w.sc_ls[i];
+        if (status) status[i] = w.sc_status[i];
+        if (contacts) contacts[i] = w.sc_contacts[i];
+    }
+    return S == (size_t)w.S ? ADB_OK : ADB_E_ARG;
 }
 
 extern "C" int adb_get_state(adb_world* W, double* q, double* qdot) {

@@ -16,16 +16,32 @@ class AdbError(RuntimeError):
 
 
 class World:
-    def __init__(self, scene, params=None):
+    def __init__(self, scene, params=None, scene_offsets=None):
+        """One world over `scene`; with scene_offsets (n_scenes + 1 body
+        offsets) the bodies form independent scenes stepped together
+        (adb_world_create_batched)."""
         self.lib = load_library()
         self.params = params if params is not None else default_params()
         self.arrays = SceneArrays(scene)
         self.B = len(scene["mass"])
         h = C.c_void_p()
-        rc = self.lib.adb_world_create(C.byref(self.arrays.struct), C.byref(self.params), C.byref(h))
+        if scene_offsets is None:
+            self.n_scenes = 1
+            rc = self.lib.adb_world_create(C.byref(self.arrays.struct), C.byref(self.params), C.byref(h))
+        else:
+            self._soff = np.ascontiguousarray(scene_offsets, dtype=np.int32)
+            self.n_scenes = len(self._soff) - 1
+            rc = self.lib.adb_world_create_batched(C.byref(self.arrays.struct), iptr(self._soff), self.n_scenes,
+                                                   C.byref(self.params), C.byref(h))
         if rc != ADB_OK:
             raise AdbError(rc, "adb_world_create failed")
         self.h = h
+
+    def batch_stats(self):
+        """Per-scene (newton_iters, ls_trials, status, contacts) of the last step."""
+        out = [np.zeros(self.n_scenes, dtype=np.int32) for _ in range(4)]
+        self._check(self.lib.adb_batch_stats(self.h, *[iptr(a) for a in out]), "adb_batch_stats")
+        return tuple(out)
 
     def close(self):
         if getattr(self, "h", None) is not None:

@@ -137,6 +137,27 @@ int  adb_get_state(adb_world* w, double* q, double* qdot);               /* [B*1
 int  adb_set_state(adb_world* w, const double* q, const double* qdot);
 const char* adb_last_error(adb_world* w);
 
+/* ---- batched independent scenes ------------------------------------------
+ * Many independent scenes (each with its own bodies, contacts, Newton loop,
+ * CCD bound, line search and convergence test, as if each were stepped by its
+ * own ABD::implicit_euler, model/abd.cpp:30-140) in one world: every stage
+ * runs once per Newton iteration over the bodies and contacts of all scenes
+ * still iterating, with per-scene scalars on the device.  `scene` holds the
+ * bodies of all scenes, scene by scene; scene_offsets[s] .. scene_offsets[s+1]
+ * are the bodies of scene s (n_scenes + 1 entries, scene_offsets[0] = 0,
+ * nondecreasing, scene_offsets[n_scenes] = n_bodies).  Bodies of different
+ * scenes never interact, whatever their positions.  Contact sets by iAABB
+ * only (params->iaabb must be 3).  adb_step steps every scene once; the step's
+ * statistics are summed over the scenes (newton_iters: the largest per-scene
+ * count) and adb_batch_stats gives them per scene. */
+int adb_world_create_batched(const adb_scene* scene, const int* scene_offsets, int n_scenes,
+                             const adb_params* params, adb_world** out);
+/* Per-scene results of the last adb_step (arrays of n_scenes entries, any may
+ * be NULL): Newton iterations, line-search trials, status (ADB_OK or an
+ * ADB_E_* code for that scene), and the scene's contacts at the end of the
+ * step (PT + EE + VG). */
+int adb_batch_stats(adb_world* w, int* newton_iters, int* ls_trials, int* status, int* contacts);
+
 /* ---- stage entry points (parity tests) -----------------------------------
  * adb_collision_set: replaces IAABB::iaabb_brute_force(vtn = 1 or 2)
  * (model/iaabb.cpp:545-568).  vtn = 1 culls at the current q, vtn = 2 at
