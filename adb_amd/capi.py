@@ -212,6 +212,7 @@ def _declare(lib):
         "adb_collision_time_noalloc": (C.c_int, [C.c_int, C.c_int, _dp, _dp]),
         "adb_profile_enable": (C.c_int, [vp, C.c_int]),
         "adb_profile_read": (C.c_int, [vp, C.c_int, _dp, C.POINTER(C.c_longlong), _dp]),
+        "adb_profile_flops": (C.c_int, [vp, C.c_int, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -230,7 +231,7 @@ EXPORTED_SYMBOLS = [
     "adb_pt_collision_time_batched", "adb_ee_collision_time_batched",
     "adb_ipc_hess_pt_12x12", "adb_ipc_hess_ee_12x12", "adb_pt_collision_time",
     "adb_ee_collision_time", "adb_pt_distance_batched", "adb_ee_distance_batched",
-    "adb_device_count", "adb_profile_enable", "adb_profile_read", "adb_collision_time_noalloc",
+    "adb_device_count", "adb_profile_enable", "adb_profile_read", "adb_profile_flops", "adb_collision_time_noalloc",
 ]
 
 # kernel ids of adb_profile_read (enum adb_prof_kernel)

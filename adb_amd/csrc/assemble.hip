@@ -399,19 +399,30 @@ __global__ void k_pair_diag(PairArgs a, const double* __restrict__ part) {
 // M^-1 = W^T W.  A group whose matrix is not positive definite falls back to
 // the inverse diagonal.
 constexpr int GMAX = 8;
-// GM: largest group size of the launch (1 for single bodies, GMAX otherwise);
-// list: the groups of this launch
+// GM: largest group size of the launch; list: the groups of this launch.
+//
+// Blocked by bodies (12 x 12 blocks, m <= GM blocks per side), every step one
+// launch-wide phase between barriers:
+//   Cholesky, for each block column kb: the diagonal block by one wavefront
+//     (lane j owns column j in registers, columns of L broadcast by
+//     shuffles), the panel below it by forward substitution (one thread per
+//     row), the trailing lower triangle by 4 x 4 register tiles;
+//   W = L^-1: the diagonal blocks' inverses (one lane per column), then block
+//     rows ib = 1 .. m-1 in order: T = sum_jb L(ib, jb) W(jb, kb) for all
+//     kb < ib at once, W(ib, kb) = -W(ib, ib) T;
+//   M^-1 = W^T W on 4 x 4 register tiles of the upper triangle.
+// W is kept transposed in the strict upper triangle of the LDS matrix (L in
+// the lower one), its diagonal (1 / L_ii) in wd.  About 3 m + 2 m barriers per
+// group instead of a few per column.
 template <int GM>
 __global__ __launch_bounds__(256) void k_group_inv(const int* __restrict__ list, const int* __restrict__ grp_off,
                                                    const int* __restrict__ grp_body, const long long* __restrict__ grp_moff,
                                                    const int* __restrict__ row_ptr, const int* __restrict__ col_idx,
                                                    const double* __restrict__ Hval, double* __restrict__ Ginv) {
-    constexpr int GN = 12 * GM, GLD = GN + 1, TT = GN / 16;
-    static_assert(GN == 16 * TT, "16 x 16 tiles");
+    constexpr int GN = 12 * GM, GLD = GN + 1;
     __shared__ double A[GN * GLD];
+    __shared__ double Tm[(GM > 1 ? GM - 1 : 1) * 144];
     __shared__ double wd[GN];
-    __shared__ double col[GN];
-    __shared__ double dsh;
     __shared__ int blk[GM][GM];
     __shared__ int bod[GM];
     __shared__ int ok;
@@ -429,65 +440,104 @@ __global__ __launch_bounds__(256) void k_group_inv(const int* __restrict__ list,
     }
     __syncthreads();
     for (int e = t; e < n * n; e += blockDim.x) {
-        const int j = e / n, i = e % n;       // A[i][j], column-major in the block
+        const int j = e / n, i = e % n;       // A[i][j] row-major in LDS; Hval blocks column-major
         const int k = blk[i / 12][j / 12];
         A[i * GLD + j] = k >= 0 ? Hval[144 * (size_t)k + 12 * (j % 12) + (i % 12)] : 0.0;
     }
     __syncthreads();
-    // The factorization and the inverse work on 16 x 16 register tiles of
-    // TT x TT entries (thread t owns rows TT (t / 16).., columns TT (t % 16)..),
-    // with one column of L or row of W at a time broadcast through LDS: n
-    // steps of a few barriers each instead of LDS read-modify-write sweeps.
-    // Every entry sees the same operations in the same order as the
-    // column-by-column algorithm (right-looking Cholesky; W = L^-1 with
-    // W[i][c] = -(sum_k L[i][k] W[k][c]) / L[i][i], terms in order of k;
-    // M^-1 entries summed in order of k).
-    const int ti = t / 16, tj = t % 16, r0 = TT * ti, c0 = TT * tj;
-    double T[TT][TT];
+    // ---------------------------------------------------------------- Cholesky
+    for (int kb = 0; kb < m; kb++) {
+        const int c0 = 12 * kb;
+        if (t < 64) {
+            const int j = t;
+            double col[12];
 #pragma unroll
-    for (int r = 0; r < TT; r++)
+            for (int i = 0; i < 12; i++) col[i] = j < 12 ? A[(c0 + i) * GLD + c0 + j] : 0.0;
+            bool good = true;
 #pragma unroll
-        for (int c = 0; c < TT; c++) T[r][c] = (r0 + r < n && c0 + c < n) ? A[(r0 + r) * GLD + c0 + c] : 0.0;
-    // Cholesky, lower triangle, column j = TT jt + jj
-    bool bad = false;
-    for (int jt = 0; jt < 16 && TT * jt < n && !bad; jt++) {
+            for (int k = 0; k < 12; k++) {
+                const double dkk = __shfl(col[k], k);
+                if (!(dkk > 0.0)) { good = false; break; }
+                const double lkk = sqrt(dkk);
+                if (j == k) {
+                    col[k] = lkk;
 #pragma unroll
-        for (int jj = 0; jj < TT; jj++) {
-            const int j = TT * jt + jj;
-            if (j >= n) break;
-            if (ti == jt && tj == jt) {
-                const double d = T[jj][jj];
-                if (d > 0.0) { T[jj][jj] = sqrt(d); dsh = T[jj][jj]; } else ok = 0;
-            }
-            __syncthreads();
-            if (!ok) { bad = true; break; }
-            const double ljj = dsh;
-            if (tj == jt) {
+                    for (int i = k + 1; i < 12; i++) col[i] = col[i] / lkk;
+                }
+                double lk[12];
 #pragma unroll
-                for (int r = 0; r < TT; r++) {
-                    const int i = r0 + r;
-                    if (i > j && i < n) { T[r][jj] = T[r][jj] / ljj; col[i] = T[r][jj]; }
+                for (int i = 0; i < 12; i++) lk[i] = __shfl(col[i], k);
+                double ljk = 0.0;
+#pragma unroll
+                for (int i = 0; i < 12; i++) ljk = i == j ? lk[i] : ljk;
+                if (j > k && j < 12) {
+#pragma unroll
+                    for (int i = k + 1; i < 12; i++)
+                        if (i >= j) col[i] -= lk[i] * ljk;
                 }
             }
-            __syncthreads();
-            if (tj <= ti && c0 + TT - 1 > j) {
-                double li[TT], lk[TT];
+            if (!good) { if (t == 0) ok = 0; }
+            else if (j < 12) {
 #pragma unroll
-                for (int r = 0; r < TT; r++) li[r] = (r0 + r > j && r0 + r < n) ? col[r0 + r] : 0.0;
-#pragma unroll
-                for (int c = 0; c < TT; c++) lk[c] = (c0 + c > j && c0 + c < n) ? col[c0 + c] : 0.0;
-#pragma unroll
-                for (int r = 0; r < TT; r++)
-#pragma unroll
-                    for (int c = 0; c < TT; c++) {
-                        const int i = r0 + r, k = c0 + c;
-                        if (k > j && k <= i && i < n) T[r][c] -= li[r] * lk[c];
-                    }
+                for (int i = 0; i < 12; i++)
+                    if (i >= j) A[(c0 + i) * GLD + c0 + j] = col[i];
             }
         }
+        __syncthreads();
+        if (!ok) break;
+        const int rows = n - c0 - 12;
+        if (rows <= 0) break;
+        // panel: L(r, c0 + j) = (A(r, c0 + j) - sum_{k < j} L(r, c0 + k) L(c0 + j, c0 + k)) / L(c0 + j, c0 + j)
+        for (int rr = t; rr < rows; rr += blockDim.x) {
+            const int r = c0 + 12 + rr;
+            double x[12];
+#pragma unroll
+            for (int jj = 0; jj < 12; jj++) {
+                double v = A[r * GLD + c0 + jj];
+#pragma unroll
+                for (int k = 0; k < jj; k++) v -= x[k] * A[(c0 + jj) * GLD + c0 + k];
+                x[jj] = v / A[(c0 + jj) * GLD + c0 + jj];
+            }
+#pragma unroll
+            for (int jj = 0; jj < 12; jj++) A[r * GLD + c0 + jj] = x[jj];
+        }
+        __syncthreads();
+        // trailing lower triangle: A(i, j) -= sum_k L(i, c0 + k) L(j, c0 + k), j <= i, 4 x 4 tiles
+        const int nt = (rows + 3) / 4, ntiles = nt * (nt + 1) / 2;
+        for (int tile = t; tile < ntiles; tile += blockDim.x) {
+            int ti = 0;
+            while ((ti + 1) * (ti + 2) / 2 <= tile) ti++;
+            const int tj = tile - ti * (ti + 1) / 2;
+            const int i0 = c0 + 12 + 4 * ti, j0 = c0 + 12 + 4 * tj;
+            double acc[4][4];
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
+#pragma unroll
+            for (int k = 0; k < 12; k++) {
+                double li[4], lj[4];
+#pragma unroll
+                for (int a = 0; a < 4; a++) li[a] = i0 + a < n ? A[(i0 + a) * GLD + c0 + k] : 0.0;
+#pragma unroll
+                for (int b = 0; b < 4; b++) lj[b] = j0 + b < n ? A[(j0 + b) * GLD + c0 + k] : 0.0;
+#pragma unroll
+                for (int a = 0; a < 4; a++)
+#pragma unroll
+                    for (int b = 0; b < 4; b++) acc[a][b] += li[a] * lj[b];
+            }
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const int i = i0 + a, jj = j0 + b;
+                    if (i < n && jj <= i) A[i * GLD + jj] -= acc[a][b];
+                }
+        }
+        __syncthreads();
     }
     double* out = Ginv + grp_moff[g];
-    if (bad) {
+    if (!ok) {
         // not positive definite: the inverse diagonal of the original matrix (from Hval)
         for (int e = t; e < n * n; e += blockDim.x) {
             const int j = e / n, i = e % n;
@@ -501,103 +551,97 @@ __global__ __launch_bounds__(256) void k_group_inv(const int* __restrict__ list,
         }
         return;
     }
-    // L into LDS (lower triangle), wd = 1 / L[i][i]
-    if (tj <= ti) {
+    // ---------------------------------------------------------------- W = L^-1
+    // W(i, k), i > k, is stored at A[k][i]; W(i, i) = wd[i]
+    // diagonal blocks: lane (wave v, lane j < 12) solves L_bb w = e_j for b = v, v + 4, ...
+    {
+        const int v = t / 64, j = t % 64;
+        for (int b = v; b < m; b += (int)blockDim.x / 64) {
+            if (j < 12) {
+                const int c0 = 12 * b;
+                double wv[12];
 #pragma unroll
-        for (int r = 0; r < TT; r++)
+                for (int i = 0; i < 12; i++) {
+                    double s = i == j ? 1.0 : 0.0;
 #pragma unroll
-            for (int c = 0; c < TT; c++)
-                if (r0 + r < n && c0 + c <= r0 + r) A[(r0 + r) * GLD + c0 + c] = T[r][c];
-    }
-    __syncthreads();
-    for (int e = t; e < n; e += blockDim.x) wd[e] = 1.0 / A[e * GLD + e];
-    // W = L^-1, strictly lower part in the tiles (-0 - x = -(x): the sign of
-    // an all-zero sum matches -(sum)), diagonal in wd
-#pragma unroll
-    for (int r = 0; r < TT; r++)
-#pragma unroll
-        for (int c = 0; c < TT; c++) T[r][c] = -0.0;
-    __syncthreads();
-    for (int kt = 0; kt < 16 && TT * kt < n; kt++) {
-#pragma unroll
-        for (int kk = 0; kk < TT; kk++) {
-            const int k = TT * kt + kk;
-            if (k + 1 >= n) break;
-            // row k of W is final: publish it
-            if (ti == kt && tj <= kt) {
-#pragma unroll
-                for (int c = 0; c < TT; c++)
-                    if (c0 + c <= k) col[c0 + c] = c0 + c == k ? wd[k] : T[kk][c];
-            }
-            __syncthreads();
-            if (tj <= ti) {
-                double li[TT], wk[TT];
-#pragma unroll
-                for (int r = 0; r < TT; r++) li[r] = (r0 + r > k && r0 + r < n) ? A[(r0 + r) * GLD + k] : 0.0;
-#pragma unroll
-                for (int c = 0; c < TT; c++) wk[c] = c0 + c <= k ? col[c0 + c] : 0.0;
-#pragma unroll
-                for (int r = 0; r < TT; r++)
-#pragma unroll
-                    for (int c = 0; c < TT; c++)
-                        if (r0 + r > k && r0 + r < n && c0 + c <= k) T[r][c] -= li[r] * wk[c];
-                // row k + 1 is complete: divide by its diagonal
-                const int nr = (kk + 1) % TT;
-                const int nt = kk + 1 == TT ? kt + 1 : kt;
-                if (ti == nt) {
-                    const double lkk = A[(k + 1) * GLD + k + 1];
-#pragma unroll
-                    for (int c = 0; c < TT; c++)
-                        if (c0 + c <= k) T[nr][c] = T[nr][c] / lkk;
+                    for (int k = 0; k < i; k++)
+                        if (k >= j) s -= A[(c0 + i) * GLD + c0 + k] * wv[k];
+                    wv[i] = i >= j ? s / A[(c0 + i) * GLD + c0 + i] : 0.0;
                 }
+                wd[c0 + j] = wv[j];
+#pragma unroll
+                for (int i = 0; i < 12; i++)
+                    if (i > j) A[(c0 + j) * GLD + c0 + i] = wv[i];
             }
-            __syncthreads();
         }
     }
-    // W into LDS (strictly lower; the diagonal is wd)
-    if (tj <= ti) {
-#pragma unroll
-        for (int r = 0; r < TT; r++)
-#pragma unroll
-            for (int c = 0; c < TT; c++)
-                if (r0 + r < n && c0 + c < r0 + r) A[(r0 + r) * GLD + c0 + c] = T[r][c];
-    }
     __syncthreads();
-    // M^-1 = W^T W: (i, j), i <= j, = sum_{k >= j} W[k][i] W[k][j], tiles with ti <= tj
-    if (ti <= tj && r0 < n && c0 < n) {
-#pragma unroll
-        for (int r = 0; r < TT; r++)
-#pragma unroll
-            for (int c = 0; c < TT; c++) T[r][c] = 0.0;
-        for (int k = c0; k < n; k++) {
-            double wi[TT], wj[TT];
-#pragma unroll
-            for (int r = 0; r < TT; r++) {
-                const int i = r0 + r;
-                wi[r] = k == i ? wd[i] : (k > i && i < n ? A[k * GLD + i] : 0.0);
-            }
-#pragma unroll
-            for (int c = 0; c < TT; c++) {
-                const int j = c0 + c;
-                wj[c] = k == j ? wd[j] : (k > j && j < n ? A[k * GLD + j] : 0.0);
-            }
-#pragma unroll
-            for (int r = 0; r < TT; r++)
-#pragma unroll
-                for (int c = 0; c < TT; c++)
-                    if (k >= c0 + c && r0 + r <= c0 + c) T[r][c] += wi[r] * wj[c];
+    // W(k, c) of an entry known to be in the lower triangle (k >= c)
+#define WLO(k, c) ((k) == (c) ? wd[(k)] : A[(c) * GLD + (k)])
+    for (int ib = 1; ib < m; ib++) {
+        // T(kb)(r, c) = sum_{jb = kb}^{ib - 1} sum_s L(12 ib + r, 12 jb + s) W(12 jb + s, 12 kb + c)
+        for (int e = t; e < ib * 144; e += blockDim.x) {
+            const int kb = e / 144, r = (e % 144) / 12, c = e % 12;
+            const int row = 12 * ib + r, cc = 12 * kb + c;
+            double sum = 0.0;
+            for (int q = cc; q < 12 * ib; q++) sum += A[row * GLD + q] * WLO(q, cc);
+            Tm[e] = sum;
         }
-#pragma unroll
-        for (int r = 0; r < TT; r++)
-#pragma unroll
-            for (int c = 0; c < TT; c++) {
-                const int i = r0 + r, j = c0 + c;
-                if (i <= j && j < n) {
-                    out[(size_t)j * n + i] = T[r][c];
-                    out[(size_t)i * n + j] = T[r][c];
-                }
-            }
+        __syncthreads();
+        // W(ib, kb)(r, c) = -sum_{s <= r} W(12 ib + r, 12 ib + s) T(kb)(s, c)
+        for (int e = t; e < ib * 144; e += blockDim.x) {
+            const int kb = e / 144, r = (e % 144) / 12, c = e % 12;
+            const int row = 12 * ib + r;
+            double sum = 0.0;
+            for (int q = 0; q <= r; q++) sum += WLO(row, 12 * ib + q) * Tm[kb * 144 + 12 * q + c];
+            A[(12 * kb + c) * GLD + row] = -sum;
+        }
+        __syncthreads();
     }
+    // ---------------------------------------------------------------- M^-1 = W^T W
+    // (i, j), i <= j: sum_{k >= j} W(k, i) W(k, j), 4 x 4 tiles of the upper triangle
+    {
+        const int nt = (n + 3) / 4, ntiles = nt * (nt + 1) / 2;
+        for (int tile = t; tile < ntiles; tile += blockDim.x) {
+            int tj = 0;
+            while ((tj + 1) * (tj + 2) / 2 <= tile) tj++;
+            const int ti = tile - tj * (tj + 1) / 2;    // ti <= tj
+            const int i0 = 4 * ti, j0 = 4 * tj;
+            double acc[4][4];
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
+            for (int k = j0; k < n; k++) {
+                double wi[4], wj[4];
+#pragma unroll
+                for (int a = 0; a < 4; a++) {
+                    const int i = i0 + a;
+                    wi[a] = (i < n && k >= i) ? WLO(k, i) : 0.0;
+                }
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const int jj = j0 + b;
+                    wj[b] = (jj < n && k >= jj) ? WLO(k, jj) : 0.0;
+                }
+#pragma unroll
+                for (int a = 0; a < 4; a++)
+#pragma unroll
+                    for (int b = 0; b < 4; b++) acc[a][b] += wi[a] * wj[b];
+            }
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const int i = i0 + a, jj = j0 + b;
+                    if (i <= jj && jj < n) {
+                        out[(size_t)jj * n + i] = acc[a][b];
+                        out[(size_t)i * n + jj] = acc[a][b];
+                    }
+                }
+        }
+    }
+#undef WLO
 }
 
 // Body groups for the preconditioner (host): heavy-edge aggregation of the
@@ -690,6 +734,28 @@ static void build_groups(World& w) {
                            w.stream));
     // the host vectors stay alive (members of w) until the copies have run
     w.Ginv.reserve((size_t)w.h_grp_moff.back());
+}
+
+// FP64 operations of k_group_inv for a group of m bodies (n = 12 m), counted
+// from its loops: a multiply-add is 2, a divide or square root 1.
+static double group_inv_flops(int m) {
+    const int n = 12 * m;
+    double f = 0.0;
+    for (int kb = 0; kb < m; kb++) {
+        f += 650.0;                                   // 12 x 12 diagonal Cholesky
+        const double rows = n - 12.0 * (kb + 1);
+        f += rows * 144.0;                            // panel forward substitution
+        f += rows * (rows + 1.0) / 2.0 * 25.0;        // trailing update: 12 products + 1 subtraction per entry
+    }
+    f += 650.0 * m;                                   // inverses of the diagonal blocks
+    for (int ib = 1; ib < m; ib++)
+        for (int kb = 0; kb < ib; kb++)
+            for (int c = 0; c < 12; c++) {
+                f += 12.0 * 2.0 * (12.0 * ib - (12.0 * kb + c));   // T(kb): 12 rows
+                for (int r = 0; r < 12; r++) f += 2.0 * (r + 1);   // W(ib, kb) = -W(ib, ib) T
+            }
+    for (int j = 0; j < n; j++) f += (j + 1.0) * (n - j) * 2.0;   // W^T W, upper triangle
+    return f;
 }
 
 template <class K, class V>
@@ -813,6 +879,11 @@ void assemble_system(World& w) {
     build_groups(w);
     {
         ProfScope ps(w, PK_BLOCK_CHOL, 0.0);
+        if (w.prof.on) {
+            double fl = 0.0;
+            for (int g = 0; g < w.n_groups; g++) fl += group_inv_flops(w.h_grp_off[g + 1] - w.h_grp_off[g]);
+            ps.flops = fl;
+        }
         hipLaunchKernelGGL(k_group_inv<GMAX>, dim3(w.n_groups), dim3(256), 0, w.stream, w.glist.p, w.grp_off.p, w.grp_body.p,
                            w.grp_moff.p, w.row_ptr.p, w.col_idx.p, w.Hval.p, w.Ginv.p);
         ADB_HIP(hipGetLastError());

@@ -71,13 +71,14 @@ struct Prof {
     bool on = false;
     // bytes of a launch = bytes + sum_i per[i] * snap[i], where snap points to
     // a pinned host copy of device counters taken on the stream before the launch
-    struct Rec { int k; hipEvent_t a, b; double bytes; const int* snap; double per[4]; };
+    struct Rec { int k; hipEvent_t a, b; double bytes; const int* snap; double per[4]; double flops; };
     std::vector<Rec> pending;
     std::vector<hipEvent_t> pool;
     int* snap = nullptr;             // pinned host ring of counter snapshots
     int snap_cap = 0, snap_used = 0;
     double ms[PK_COUNT] = {0};
     double bytes[PK_COUNT] = {0};
+    double flops[PK_COUNT] = {0};     // counted FP64 operations (FP64-VALU-bound kernels)
     long long count[PK_COUNT] = {0};
 };
 
@@ -220,6 +221,7 @@ constexpr int SRED_N = 4;   // per-scene sum slots
 // launch's algorithmic HBM traffic.
 struct ProfScope {
     World& w; int k; double bytes; hipEvent_t a = nullptr; hipStream_t st;
+    double flops = 0.0;               // the launch's counted FP64 operations, when known
     const int* snap = nullptr; double per[4] = {0, 0, 0, 0};
     void counts(const int* s, double p0, double p1, double p2, double p3) { snap = s; per[0] = p0; per[1] = p1; per[2] = p2; per[3] = p3; }
     ProfScope(World& w_, int k_, double bytes_, hipStream_t s = nullptr) : w(w_), k(k_), bytes(bytes_), st(s ? s : w_.stream) {
@@ -231,7 +233,7 @@ struct ProfScope {
         if (!w.prof.on || !a) return;
         hipEvent_t b = get();
         (void)hipEventRecord(b, st);
-        w.prof.pending.push_back({k, a, b, bytes, snap, {per[0], per[1], per[2], per[3]}});
+        w.prof.pending.push_back({k, a, b, bytes, snap, {per[0], per[1], per[2], per[3]}, flops});
     }
     hipEvent_t get() {
         if (!w.prof.pool.empty()) { hipEvent_t e = w.prof.pool.back(); w.prof.pool.pop_back(); return e; }

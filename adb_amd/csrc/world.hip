@@ -653,6 +653,7 @@ void prof_flush(World& w) {
         double bytes = r.bytes;
         if (r.snap) for (int i = 0; i < 4; i++) bytes += r.per[i] * (double)r.snap[i];
         w.prof.bytes[r.k] += bytes;
+        w.prof.flops[r.k] += r.flops;
         w.prof.count[r.k] += 1;
         w.prof.pool.push_back(r.a);
         w.prof.pool.push_back(r.b);
@@ -680,7 +681,17 @@ extern "C" int adb_profile_enable(adb_world* W, int on) {
         World& w = W->w;
         prof_flush(w);
         w.prof.on = on != 0;
-        for (int k = 0; k < PK_COUNT; k++) { w.prof.ms[k] = 0.0; w.prof.bytes[k] = 0.0; w.prof.count[k] = 0; }
+        for (int k = 0; k < PK_COUNT; k++) { w.prof.ms[k] = 0.0; w.prof.bytes[k] = 0.0; w.prof.flops[k] = 0.0; w.prof.count[k] = 0; }
+        return ADB_OK;
+    })
+}
+
+extern "C" int adb_profile_flops(adb_world* W, int kernel, double* flops) {
+    if (!W || kernel < 0 || kernel >= PK_COUNT || !flops) return ADB_E_ARG;
+    ADB_TRY(W, {
+        World& w = W->w;
+        prof_flush(w);
+        *flops = w.prof.flops[kernel];
         return ADB_OK;
     })
 }

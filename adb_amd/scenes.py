@@ -312,14 +312,19 @@ def card_lattice(n=10000, seed=5, size=(0.1, 0.002, 0.15), tilt_deg=12.0, pitch=
     return build_scene(bodies)
 
 
-def batched_scenes(n_scenes=256, seed=3, nmin=8, nmax=200, spacing=20.0):
-    """Independent scenes of U(nmin, nmax) cubes or subdiv-2 icospheres placed far
-    apart on a grid (so they never interact) above a shared ground plane."""
+def batched_scenes(n_scenes=256, seed=3, nmin=8, nmax=200, spacing=20.0, pitch=2.0, ground_y=-0.5):
+    """Independent scenes of U(nmin, nmax) bodies, each a randomly rotated unit
+    cube or a subdivision-2 icosphere (162 vertices, radius 0.5) with
+    probability 1/2, stacked on a grid of `pitch` (more than a rotated cube's
+    diagonal, sqrt(3)) above the ground, the lowest layer clear of it by
+    0.034; the scenes sit `spacing` apart (batched stepping keeps them apart
+    whatever their positions)."""
     rng = PCG32(seed)
     cv, cf, ce = cube_mesh()
     sv, sf, se = icosphere_mesh(2, 0.5)
     scenes = []
     side = int(math.ceil(math.sqrt(n_scenes)))
+    y0 = ground_y + 0.5 * math.sqrt(3.0) + 0.034
     for s in range(n_scenes):
         nb = int(nmin + (nmax - nmin + 1) * rng.uniform())
         ox, oz = (s % side) * spacing, (s // side) * spacing
@@ -329,7 +334,7 @@ def batched_scenes(n_scenes=256, seed=3, nmin=8, nmax=200, spacing=20.0):
             a, b_, c = k % cols, (k // cols) % cols, k // (cols * cols)
             use_sphere = rng.uniform() < 0.5
             vv, ff, ee = (sv, sf, se) if use_sphere else (cv, cf, ce)
-            p = (ox + 1.3 * a + rng.uniform(-0.05, 0.05), 0.2 + 1.3 * c, oz + 1.3 * b_ + rng.uniform(-0.05, 0.05))
+            p = (ox + pitch * a + rng.uniform(-0.05, 0.05), y0 + pitch * c, oz + pitch * b_ + rng.uniform(-0.05, 0.05))
             bodies.append(dict(verts=vv, tris=ff, edges=ee, R=rng.rotation(), p=p))
         scenes.append(bodies)
     flat = [b for sc in scenes for b in sc]

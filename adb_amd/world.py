@@ -146,7 +146,9 @@ class World:
         for k, name in enumerate(PROF_KERNELS):
             ms, n, b = C.c_double(), C.c_longlong(), C.c_double()
             self._check(self.lib.adb_profile_read(self.h, k, C.byref(ms), C.byref(n), C.byref(b)), "adb_profile_read")
-            out[name] = {"ms": ms.value, "launches": n.value, "bytes": b.value}
+            fl = C.c_double()
+            self._check(self.lib.adb_profile_flops(self.h, k, C.byref(fl)), "adb_profile_flops")
+            out[name] = {"ms": ms.value, "launches": n.value, "bytes": b.value, "flops": fl.value}
         return out
 
     def newton_direction(self):

@@ -47,7 +47,10 @@ def cpu_baseline(sc, p, budget_s, gpu_step_s):
     holds the GPU's first-call set-up)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    threads = O.lib().oracle_set_threads(min(os.cpu_count() or 1, 16))
+    # the host share of one GPU on the benchmark machines is 16 cores; more
+    # threads than that would time other jobs' cores
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = O.lib().oracle_set_threads(min(nproc, 16))
     w = O.OracleWorld(sc, p)
     t0 = time.perf_counter()
     per = []
@@ -61,7 +64,14 @@ def cpu_baseline(sc, p, budget_s, gpu_step_s):
         if el >= budget_s or len(per) >= 50:
             break
     k = len(per)
-    out = {"value": k / el, "unit": "steps/s", "cores": threads, "kind": "port",
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((l.split(":", 1)[1].strip() for l in fh if l.startswith("model name")), "")
+    except OSError:
+        pass
+    out = {"value": k / el, "unit": "steps/s", "cores": threads, "kind": "port", "cpu_model": model,
+           "nproc": nproc, "threads_note": "OpenMP threads = min(nproc, 16): the host share of one GPU",
            "sample": "first %d steps of the same 2000-icosphere scene from its initial state, %.1f s, "
                      "oracle FP64 restatement with OpenMP (%d threads); contact-pairs/s %.3g"
                      % (k, el, threads, pairs / el)}
@@ -130,6 +140,23 @@ KERNEL_OF_SCOPE = {"asm_reduce": ["k_pair_reduce"], "narrow_set": ["k_pair_narro
                    "spmv": ["k_pcg_spmv"], "pcg_update": ["k_pcg_update"], "contact_classify": ["k_contact_classify"]}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hbm_traffic_2000_icospheres.json")
 SIDE_SCOPES = {"contact_group", "contact_pk"}      # timing scopes on the world's side streams
+# FP64 operations per launch measured with rocprofv3 counters (committed
+# summary), for the FP64-VALU-bound scopes whose count the host does not know
+FP64_OPS_JSON = os.path.join(ROOT, "profiles", "fp64_ops_2000_icospheres.json")
+FP64_OPS_OF_SCOPE = {"contact_group", "contact_pk", "narrow_set", "narrow_ccd"}
+
+
+def measured_fp64_ops(scope, launches):
+    """FP64 operations of `launches` launches of the scope from the committed
+    counter summary, when it was measured on this workload and launch count."""
+    if not os.path.exists(FP64_OPS_JSON):
+        return 0.0
+    with open(FP64_OPS_JSON) as fh:
+        d = json.load(fh)
+    e = d.get("scopes", {}).get(scope)
+    if not e or e.get("launches") != launches:
+        return 0.0
+    return e["fp64_ops_per_launch"] * launches
 
 
 def pmc_traffic(fetch_csv, write_csv, prof_json, scope, copy_mb):
@@ -262,23 +289,37 @@ def main():
                  "newton_iters": t2["newton"], "pcg_iters": t2["pcg"]}
 
     steps_per_s = args.steps / elapsed
-    # dominant kernel by device time among those with an algorithmic byte count,
-    # on the world's main stream: the 4-point contacts and the 3-point
-    # projections run on side streams concurrently with the main stream's
-    # contact kernels, so their event times include sharing the GPU and are
-    # not on the step's critical path (reported beside the roofline)
+    # dominant kernel scope: the most device time in the profiled steps (side
+    # stream scopes included; their event times include sharing the GPU with
+    # the main stream's kernels).  FP64-VALU-bound scopes with a counted
+    # operation total report FP64 throughput against 78.6 TFLOP/s (MI355X's
+    # FP64 peak, the same for vector and matrix); the others report their
+    # algorithmic HBM bytes against 8 TB/s.
     ranked = sorted(prof.items(), key=lambda kv: -kv[1]["ms"])
-    dom = next(((k, v) for k, v in ranked if v["bytes"] > 0 and v["launches"] > 0 and k not in SIDE_SCOPES), None)
+    dom = next(((k, v) for k, v in ranked if v["launches"] > 0), None)
     roofline = None
     if dom:
         k, v = dom
         avg_ms = v["ms"] / v["launches"]
-        bytes_per_launch = v["bytes"] / v["launches"]
-        gbs = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-        roofline = {"kernel": k, "bound": "hbm", "achieved": round(gbs, 1), "peak": MI355X_HBM_GBS,
-                    "unit": "GB/s", "frac": round(gbs / MI355X_HBM_GBS, 4), "traffic": None,
-                    "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch, "launches": v["launches"],
-                    "concurrent_side_stream_ms": {q: round(prof[q]["ms"], 3) for q in sorted(SIDE_SCOPES) if q in prof}}
+        side = {q: round(prof[q]["ms"], 3) for q in sorted(SIDE_SCOPES) if q in prof}
+        flops = v.get("flops", 0.0)
+        if flops <= 0.0 and k in FP64_OPS_OF_SCOPE:
+            flops = measured_fp64_ops(k, v["launches"])
+        if flops > 0.0:
+            per = flops / v["launches"]
+            tf = per / (avg_ms * 1e-3) / 1e12
+            roofline = {"kernel": k, "bound": "mfma", "pipe": "fp64 VALU", "achieved": round(tf, 3),
+                        "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(tf / MI355X_FP64_TFLOPS, 4),
+                        "traffic": None, "avg_launch_ms": avg_ms, "flops_per_launch": per, "launches": v["launches"],
+                        "flops_source": "counted" if v.get("flops", 0.0) > 0.0 else "rocprofv3 FP64 instruction counters",
+                        "on_side_stream": k in SIDE_SCOPES, "concurrent_side_stream_ms": side}
+        else:
+            bytes_per_launch = v["bytes"] / v["launches"]
+            gbs = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+            roofline = {"kernel": k, "bound": "hbm", "achieved": round(gbs, 1), "peak": MI355X_HBM_GBS,
+                        "unit": "GB/s", "frac": round(gbs / MI355X_HBM_GBS, 4), "traffic": None,
+                        "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch, "launches": v["launches"],
+                        "on_side_stream": k in SIDE_SCOPES, "concurrent_side_stream_ms": side}
         # HBM bytes per launch measured by rocprofv3 --pmc on this workload (committed summary)
         if os.path.exists(TRAFFIC_JSON):
             with open(TRAFFIC_JSON) as fh:

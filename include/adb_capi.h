@@ -288,6 +288,10 @@ enum adb_prof_kernel {
 };
 int adb_profile_enable(adb_world* w, int on);
 int adb_profile_read(adb_world* w, int kernel, double* total_ms, long long* launches, double* bytes);
+/* Counted FP64 operations of the kernel's launches since adb_profile_enable,
+ * for the FP64-VALU-bound kernels whose operation count the host knows
+ * exactly (the preconditioner's group inverses); 0 for the others. */
+int adb_profile_flops(adb_world* w, int kernel, double* flops);
 
 #ifdef __cplusplus
 }

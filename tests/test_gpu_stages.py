@@ -171,3 +171,20 @@ def test_caller_capacity_too_small():
     full = gw.collision_set()
     assert rc == capi.ADB_E_CAPACITY
     assert [x.value for x in n] == [len(full[0]), len(full[1]), len(full[2])]
+
+
+@pytest.mark.parametrize("n,exact", [(8, True), (12, False)])
+def test_group_inverse_is_exact_preconditioner(n, exact):
+    # every contact component of at most 8 bodies is one preconditioner group,
+    # whose inverse is then exact: PCG needs one iteration per Newton
+    # iteration (a wrong inverse would still converge, only slower)
+    _require_gpu()
+    sc = scenes.cube_stack(n, seed=1, spacing=1.0 + 0.5 * math.sqrt(1e-3))
+    gw = World(sc, default_params(d_hat=1e-3, mu=0.5))
+    newton = pcg = 0
+    for s in range(3):
+        rc, st = gw.step()
+        newton += st.newton_iters
+        pcg += st.pcg_iters
+    assert st.n_pt + st.n_ee > 0
+    assert (pcg == newton) == exact, (pcg, newton)
