@@ -74,6 +74,13 @@ class AdbStepStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class AdbTuning(C.Structure):
+    _fields_ = [("narrow_per_pair", C.c_double), ("ee_slices", C.c_int), ("pcg_batch", C.c_int)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class AdbKernelConfig(C.Structure):
     _fields_ = [("d_hat", C.c_double), ("kappa", C.c_double), ("eps_x", C.c_double), ("psd", C.c_int)]
 
@@ -213,6 +220,8 @@ def _declare(lib):
         "adb_profile_enable": (C.c_int, [vp, C.c_int]),
         "adb_profile_read": (C.c_int, [vp, C.c_int, _dp, C.POINTER(C.c_longlong), _dp]),
         "adb_profile_flops": (C.c_int, [vp, C.c_int, _dp]),
+        "adb_get_tuning": (C.c_int, [vp, C.POINTER(AdbTuning)]),
+        "adb_set_tuning": (C.c_int, [vp, C.POINTER(AdbTuning)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -231,7 +240,7 @@ EXPORTED_SYMBOLS = [
     "adb_pt_collision_time_batched", "adb_ee_collision_time_batched",
     "adb_ipc_hess_pt_12x12", "adb_ipc_hess_ee_12x12", "adb_pt_collision_time",
     "adb_ee_collision_time", "adb_pt_distance_batched", "adb_ee_distance_batched",
-    "adb_device_count", "adb_profile_enable", "adb_profile_read", "adb_profile_flops", "adb_collision_time_noalloc",
+    "adb_device_count", "adb_profile_enable", "adb_profile_read", "adb_profile_flops", "adb_get_tuning", "adb_set_tuning", "adb_collision_time_noalloc",
 ]
 
 # kernel ids of adb_profile_read (enum adb_prof_kernel)

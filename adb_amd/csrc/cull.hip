@@ -234,8 +234,7 @@ __global__ void k_sweep_pairs(int B, const unsigned long long* __restrict__ keys
 // Boxes only cull; the FP64 predicate decides, so the sets equal those of the
 // all-pairs tests (model/thread_local_culling.cpp:209-386 pt/ee_col_set_task).
 constexpr int PAIRS_PT = 4;                    // box tests per thread per round
-constexpr int EE_SLICES = 4;                   // workgroups per body pair for the EE pass
-constexpr int NARROW_TASKS = 2 + EE_SLICES;    // workgroups per body pair
+// workgroups per body pair: 2 PT passes + the EE pass in ee_slices slices (tuning)
 constexpr int QCAP = NB_THREADS * PAIRS_PT * 3 / 2; // candidate queue (i << 16 | j: staged indices < 2^16)
 
 struct NarrowArgs {
@@ -251,6 +250,7 @@ struct NarrowArgs {
     int* bcnt;                            // bucket counters: [0, NBK) pt, [NBK, 2 NBK) ee
     unsigned long long* toi_bits;         // CCD: [1] pt/ee min; per scene [2 s + 1] when scene_of
     const int* scene_of;                  // batched worlds: body -> scene (CCD minimum per scene)
+    int ee_slices;                        // workgroups per body pair of the EE pass
 };
 
 template <int CCD>
@@ -457,7 +457,7 @@ __global__ __launch_bounds__(NB_THREADS) void k_pair_narrow(NarrowArgs a) {
     double tmin = 1.0;
     const double infl = CCD == 1 ? 0.0 : a.r;
     // task (blockIdx.y): 0 = vertices of I vs faces of J, 1 = vertices of J vs
-    // faces of I, 2 + q = edges of I in slice q (of EE_SLICES) vs edges of J
+    // faces of I, 2 + q = edges of I in slice q (of a.ee_slices) vs edges of J
     const int task = blockIdx.y;
     const int pass = task < 2 ? task : 2;
     const int kind = pass < 2 ? 0 : 1;                            // 0 = PT, 1 = EE
@@ -485,7 +485,7 @@ __global__ __launch_bounds__(NB_THREADS) void k_pair_narrow(NarrowArgs a) {
             int endA = offA[bA + 1];
             if (pass == 2) {
                 const int len = endA - curA, q = task - 2;
-                const int s0 = curA + (int)((long long)len * q / EE_SLICES), s1 = curA + (int)((long long)len * (q + 1) / EE_SLICES);
+                const int s0 = curA + (int)((long long)len * q / a.ee_slices), s1 = curA + (int)((long long)len * (q + 1) / a.ee_slices);
                 curA = s0; endA = s1;
             }
             while (curA < endA) {
@@ -738,7 +738,8 @@ static void body_pairs(World& w, const double* X0, const double* X1, double infl
 // workgroups per task of a narrow-phase launch (each loops over the pairs):
 // about the pair count of the previous pass, so most take one pair
 static int narrow_grid(World& w) {
-    return std::max(1, std::min<int>((int)w.pairs.cap, w.pair_hint + w.pair_hint / 8 + 64));
+    const double g = w.pair_hint * w.tune.narrow_per_pair + 64.0;
+    return std::max(1, (int)std::min<double>((double)w.pairs.cap, g));
 }
 
 static NarrowArgs narrow_args(World& w, const int* npairs, const double* X0, const double* X1, double r) {
@@ -751,6 +752,7 @@ static NarrowArgs narrow_args(World& w, const int* npairs, const double* X0, con
     a.pt_keys = nullptr; a.ee_keys = nullptr; a.sub = 0; a.bcnt = nullptr;
     a.toi_bits = nullptr;
     a.scene_of = nullptr;
+    a.ee_slices = w.tune.ee_slices;
     return a;
 }
 
@@ -778,7 +780,7 @@ static void set_pass(World& w, const double* X, const double* X1, ContactSet& ou
             hash_sets(w, X, r, a.pt_keys, a.ee_keys, (int)sub, w.bcnt.p);
         } else {
             ProfScope ps(w, PK_NARROW_SET, 0.0);
-            const dim3 grid(narrow_grid(w), NARROW_TASKS);
+            const dim3 grid(narrow_grid(w), 2 + w.tune.ee_slices);
             if (swept) {
                 if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<2, float>), grid, dim3(NB_THREADS), 0, w.stream, a);
                 else hipLaunchKernelGGL((k_pair_narrow<2, double>), grid, dim3(NB_THREADS), 0, w.stream, a);
@@ -987,7 +989,7 @@ double ccd_toi(World& w, const double* X0, const double* X1, int* status) {
             hash_ccd(w, X0, X1, w.toi_bits.p);
         } else {
             ProfScope ps(w, PK_NARROW_CCD, 0.0);
-            const dim3 grid(narrow_grid(w), NARROW_TASKS);
+            const dim3 grid(narrow_grid(w), 2 + w.tune.ee_slices);
             if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<1, float>), grid, dim3(NB_THREADS), 0, w.stream, a);
             else hipLaunchKernelGGL((k_pair_narrow<1, double>), grid, dim3(NB_THREADS), 0, w.stream, a);
             ADB_HIP(hipGetLastError());
@@ -1029,7 +1031,7 @@ void ccd_toi_scenes(World& w, const double* X0, const double* X1, double* toi, i
         a.scene_of = w.scene_of.p;
         {
             ProfScope ps(w, PK_NARROW_CCD, 0.0);
-            const dim3 grid(narrow_grid(w), NARROW_TASKS);
+            const dim3 grid(narrow_grid(w), 2 + w.tune.ee_slices);
             if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<1, float>), grid, dim3(NB_THREADS), 0, w.stream, a);
             else hipLaunchKernelGGL((k_pair_narrow<1, double>), grid, dim3(NB_THREADS), 0, w.stream, a);
             ADB_HIP(hipGetLastError());

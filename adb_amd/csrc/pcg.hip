@@ -29,7 +29,7 @@ enum { S_STOP = 0, S_ITERS, S_BB, S_RZ0, S_RZ1, S_COUNT };
 enum { RUN = 0, CONVERGED = 1, BREAKDOWN = 2, NONFINITE = 3 };
 
 constexpr int SP_ROWS = 4;       // block rows per spmv workgroup (one 64-lane team each)
-constexpr int BATCH = 16;        // iterations queued between host checks
+// iterations queued between host checks: w.tune.pcg_batch (even)
 
 // fixed-order wave sum, broadcast from lane 0
 __device__ __forceinline__ double wave_sum(double v) {
@@ -281,8 +281,8 @@ static Groups groups_of(World& w) {
     return G;
 }
 
-// One batch = BATCH iterations; kernel k of the batch computes iteration
-// *kbase + k, and the last node advances *kbase.  Because BATCH is even, the
+// One batch = pcg_batch iterations; kernel k of the batch computes iteration
+// *kbase + k, and the last node advances *kbase.  Because the batch is even, the
 // p-buffer parity of local iteration k is k & 1 in every batch, so one
 // captured graph serves every batch.
 static void enqueue_batch(World& w, int nb, bool prof) {
@@ -341,7 +341,8 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
                              w.pcg_p2.p, w.pcg_Ap.p, w.partial.p, w.scal.p, w.pcg_k.p, w.grp_off.p, w.grp_body.p, w.grp_moff.p,
                              w.grp_n.p, w.sc_goff.p, w.pcg_sbb.p, w.pcg_brun.p};
         static_assert(sizeof key / sizeof key[0] == World::PCG_GRAPH_KEYS, "graph key size");
-        bool same = w.pcg_graph != nullptr;
+        const int BATCH = w.tune.pcg_batch;
+        bool same = w.pcg_graph != nullptr && w.pcg_graph_batch == BATCH;
         for (int i = 0; i < World::PCG_GRAPH_KEYS; i++) same = same && w.pcg_graph_key[i] == key[i];
         if (!same) {
             if (w.pcg_graph) { ADB_HIP(hipGraphExecDestroy(w.pcg_graph)); w.pcg_graph = nullptr; }
@@ -352,8 +353,10 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
             ADB_HIP(hipGraphInstantiate(&w.pcg_graph, g, nullptr, nullptr, 0));
             ADB_HIP(hipGraphDestroy(g));
             for (int i = 0; i < World::PCG_GRAPH_KEYS; i++) w.pcg_graph_key[i] = key[i];
+            w.pcg_graph_batch = BATCH;
         }
     }
+    const int BATCH = w.tune.pcg_batch;
     int it = 0, slot = 0;
     bool have_prev = false;
     const int max_it = std::max(w.P.pcg_max_iter, 0);

@@ -214,6 +214,10 @@ struct World {
     std::vector<int> h_stall_pt, h_stall_vg;
     // per-scene results of the last step
     std::vector<int> sc_newton, sc_ls, sc_status, sc_contacts;
+    // launch shapes that change how fast a step runs, never its results
+    // (adb_set_tuning; adb_amd/autotune.py picks them per workload)
+    adb_tuning tune = {1.125, 4, 16};
+    int pcg_graph_batch = 0;              // PCG batch size the graph was captured with
 };
 constexpr int SRED_N = 4;   // per-scene sum slots
 

@@ -686,6 +686,23 @@ extern "C" int adb_profile_enable(adb_world* W, int on) {
     })
 }
 
+extern "C" int adb_get_tuning(adb_world* W, adb_tuning* t) {
+    if (!W || !t) return ADB_E_ARG;
+    *t = W->w.tune;
+    return ADB_OK;
+}
+
+extern "C" int adb_set_tuning(adb_world* W, const adb_tuning* t) {
+    if (!W || !t) return ADB_E_ARG;
+    if (!(t->narrow_per_pair >= 0.125 && t->narrow_per_pair <= 8.0) || t->ee_slices < 1 || t->ee_slices > 16 ||
+        t->pcg_batch < 2 || t->pcg_batch > 64 || (t->pcg_batch & 1)) {
+        W->w.err = "tuning out of range";
+        return ADB_E_ARG;
+    }
+    W->w.tune = *t;
+    return ADB_OK;
+}
+
 extern "C" int adb_profile_flops(adb_world* W, int kernel, double* flops) {
     if (!W || kernel < 0 || kernel >= PK_COUNT || !flops) return ADB_E_ARG;
     ADB_TRY(W, {

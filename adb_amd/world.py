@@ -37,6 +37,24 @@ class World:
             raise AdbError(rc, "adb_world_create failed")
         self.h = h
 
+    def tuning(self):
+        from .capi import AdbTuning
+        t = AdbTuning()
+        self._check(self.lib.adb_get_tuning(self.h, C.byref(t)), "adb_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, narrow_per_pair=None, ee_slices=None, pcg_batch=None):
+        from .capi import AdbTuning
+        t = AdbTuning()
+        self._check(self.lib.adb_get_tuning(self.h, C.byref(t)), "adb_get_tuning")
+        if narrow_per_pair is not None:
+            t.narrow_per_pair = narrow_per_pair
+        if ee_slices is not None:
+            t.ee_slices = ee_slices
+        if pcg_batch is not None:
+            t.pcg_batch = pcg_batch
+        self._check(self.lib.adb_set_tuning(self.h, C.byref(t)), "adb_set_tuning")
+
     def batch_stats(self):
         """Per-scene (newton_iters, ls_trials, status, contacts) of the last step."""
         out = [np.zeros(self.n_scenes, dtype=np.int32) for _ in range(4)]

@@ -83,11 +83,13 @@ def cpu_baseline(sc, p, budget_s, gpu_step_s):
     return out
 
 
-def run(sc, p, warmup, steps, label=""):
+def run(sc, p, warmup, steps, label="", tune=None):
     """warmup untimed steps, then `steps` timed steps; returns (seconds, totals,
     last stats, world, per-step seconds of the warm-up and timed steps).
     adb_step returns after its stream has drained, so per-step wall times are
-    exact; a progress line goes to stderr every 10 steps."""
+    exact; a progress line goes to stderr every 10 steps.  tune: "auto" runs
+    the autotuner on the warmed-up state (outside the timed region; the
+    launch shapes it picks never change results), a dict applies a tuning."""
     from adb_amd.world import World
     w = World(sc, p)   # World.step raises on any status but ADB_OK, so no failed step is timed
     per_step = []
@@ -95,6 +97,14 @@ def run(sc, p, warmup, steps, label=""):
         t = time.perf_counter()
         w.step()
         per_step.append(time.perf_counter() - t)
+    w.tuning_source = "default"
+    if tune == "auto":
+        from adb_amd import autotune
+        _, w.tuning_source = autotune.autotune(w, key=autotune.workload_key(sc, p, "gfx950"), probe_steps=2,
+                                               log=lambda m: print(label, m, file=sys.stderr, flush=True))
+    elif isinstance(tune, dict):
+        w.set_tuning(**tune)
+        w.tuning_source = "given"
     w.state()   # synchronizes the world's stream
     tot = {"contacts": 0, "newton": 0, "ls": 0, "pcg": 0, "t_ipc": 0.0, "t_solver": 0.0, "t_ccd": 0.0, "t_ls": 0.0,
            "pcg_not_converged": 0, "pcg_breakdown": 0}
@@ -220,6 +230,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the run in the other precision")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--profile-steps", type=int, default=1,
                     help="steps after the timed ones with per-kernel HIP-event timing (roofline figures)")
     ap.add_argument("--profile-json", default="")
@@ -262,8 +273,10 @@ def main():
         del src, dst
     sc = workload(args.bodies)
     p = params(args.precision)
-    elapsed, tot, last, w, per_step = run(sc, p, args.warmup, args.steps, args.precision)
+    elapsed, tot, last, w, per_step = run(sc, p, args.warmup, args.steps, args.precision,
+                                          tune=None if args.no_autotune else "auto")
     q_main, _ = w.state()
+    tuning = dict(w.tuning(), source=w.tuning_source)
 
     # per-kernel timing on extra steps (HIP events on the world's stream)
     prof = {}
@@ -281,7 +294,8 @@ def main():
     other = None
     if not args.no_compare:
         op = "fp64" if args.precision == "mixed" else "mixed"
-        e2, t2, _, w2, _ = run(sc, params(op), args.warmup, args.steps, op)
+        e2, t2, _, w2, _ = run(sc, params(op), args.warmup, args.steps, op,
+                               tune={k: v for k, v in tuning.items() if k != "source"})
         q2, _ = w2.state()
         del w2
         other = {"precision": op, "steps_per_s": round(args.steps / e2, 4),
@@ -356,6 +370,7 @@ def main():
                     "median": round(1000.0 * sorted(per_step[args.warmup:])[args.steps // 2], 2) if args.steps else None,
                     "max": round(1000.0 * max(per_step[args.warmup:]), 2) if args.steps else None},
         "other_precision": other,
+        "tuning": tuning,
         "top_kernels_ms_profiled_steps": {k: round(v["ms"], 3) for k, v in ranked[:8]},
         "roofline": roofline,
         "cpu_baseline": None,

@@ -288,6 +288,24 @@ enum adb_prof_kernel {
 };
 int adb_profile_enable(adb_world* w, int on);
 int adb_profile_read(adb_world* w, int kernel, double* total_ms, long long* launches, double* bytes);
+/* ---- launch tuning ----------------------------------------------------------
+ * No counterpart in the reference (its only knob is the BVH threshold
+ * `thres`, config.json:68).  Launch shapes of a world that change how fast a
+ * step runs but never its results (contact sets are sorted, sums are taken in
+ * a fixed order whatever the grid): workgroups of the narrow phase per
+ * overlapping body pair and pass (narrow_per_pair, 0.125 .. 8), workgroups per
+ * body pair of its edge-edge pass (ee_slices, 1 .. 16), PCG iterations per
+ * queued batch between host checks (pcg_batch, even, 2 .. 64).  The
+ * autotuner (adb_amd/autotune.py) times candidates on a workload's own
+ * states and caches the choice per workload. */
+typedef struct adb_tuning {
+    double narrow_per_pair;
+    int    ee_slices;
+    int    pcg_batch;
+} adb_tuning;
+int adb_get_tuning(adb_world* w, adb_tuning* t);
+int adb_set_tuning(adb_world* w, const adb_tuning* t);   /* ADB_E_ARG outside the ranges */
+
 /* Counted FP64 operations of the kernel's launches since adb_profile_enable,
  * for the FP64-VALU-bound kernels whose operation count the host knows
  * exactly (the preconditioner's group inverses); 0 for the others. */

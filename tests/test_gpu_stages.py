@@ -188,3 +188,30 @@ def test_group_inverse_is_exact_preconditioner(n, exact):
         pcg += st.pcg_iters
     assert st.n_pt + st.n_ee > 0
     assert (pcg == newton) == exact, (pcg, newton)
+
+
+def test_tunings_give_identical_steps():
+    # the launch shapes of adb_tuning change speed only: every tuning gives the
+    # same steps bit for bit (single-scene and batched worlds)
+    _require_gpu()
+    from adb_amd import autotune
+    sc = scenes.sphere_pack(12, seed=5, subdiv=2, rmin=0.2, rmax=0.3, half=0.8, ytop=1.5, gap=0.02, vsigma=0.3)
+    p = default_params(d_hat=4e-3, mu=0.3)
+    tunings = [dict(narrow_per_pair=1.125, ee_slices=4, pcg_batch=16), dict(narrow_per_pair=0.125, ee_slices=1, pcg_batch=2),
+               dict(narrow_per_pair=8.0, ee_slices=16, pcg_batch=64), dict(narrow_per_pair=0.5, ee_slices=8, pcg_batch=8)]
+    states = []
+    for t in tunings:
+        gw = World(sc, p)
+        gw.set_tuning(**t)
+        for s in range(6):
+            gw.step()
+        states.append(gw.state())
+    for q, qd in states[1:]:
+        assert np.array_equal(q, states[0][0]) and np.array_equal(qd, states[0][1])
+    gw = World(sc, p)
+    with pytest.raises(AdbError):
+        gw.set_tuning(pcg_batch=3)
+    t, src = autotune.autotune(gw, key=None, probe_steps=1)
+    assert src == "timed" and t["pcg_batch"] in autotune.CANDIDATES["pcg_batch"]
+    q0 = gw.state()[0]
+    assert np.array_equal(q0, sc["q0"])           # the tuner restored the state
