@@ -721,9 +721,18 @@ static void build_groups(World& w) {
         ADB_HIP(hipMemcpyAsync(w.sc_goff.p, w.h_sc_goff.data(), (w.S + 1) * sizeof(int), hipMemcpyHostToDevice, w.stream));
     }
     // groups with several bodies first: their inverses take longest
+    // launch lists by size class (the inverse kernel's LDS is sized for the
+    // class, so small groups share a CU): 5..GMAX bodies, 2..4, single bodies
     w.h_glist.clear();
-    for (int g = 0; g < w.n_groups; g++) if (w.h_grp_off[g + 1] - w.h_grp_off[g] > 1) w.h_glist.push_back(g);
+    for (int g = 0; g < w.n_groups; g++) if (w.h_grp_off[g + 1] - w.h_grp_off[g] > 4) w.h_glist.push_back(g);
+    w.n_glist[0] = (int)w.h_glist.size();
+    for (int g = 0; g < w.n_groups; g++) {
+        const int m = w.h_grp_off[g + 1] - w.h_grp_off[g];
+        if (m > 1 && m <= 4) w.h_glist.push_back(g);
+    }
+    w.n_glist[1] = (int)w.h_glist.size() - w.n_glist[0];
     for (int g = 0; g < w.n_groups; g++) if (w.h_grp_off[g + 1] - w.h_grp_off[g] == 1) w.h_glist.push_back(g);
+    w.n_glist[2] = (int)w.h_glist.size() - w.n_glist[0] - w.n_glist[1];
     w.glist.reserve(std::max(w.n_groups, 1));
     ADB_HIP(hipMemcpyAsync(w.glist.p, w.h_glist.data(), w.n_groups * sizeof(int), hipMemcpyHostToDevice, w.stream));
     w.grp_off.reserve(w.n_groups + 1); w.grp_body.reserve(B); w.grp_moff.reserve(w.n_groups + 1); w.grp_n.reserve(1);
@@ -884,8 +893,17 @@ void assemble_system(World& w) {
             for (int g = 0; g < w.n_groups; g++) fl += group_inv_flops(w.h_grp_off[g + 1] - w.h_grp_off[g]);
             ps.flops = fl;
         }
-        hipLaunchKernelGGL(k_group_inv<GMAX>, dim3(w.n_groups), dim3(256), 0, w.stream, w.glist.p, w.grp_off.p, w.grp_body.p,
-                           w.grp_moff.p, w.row_ptr.p, w.col_idx.p, w.Hval.p, w.Ginv.p);
+        static_assert(GMAX == 8, "size classes 5..8, 2..4, 1");
+        const int* L = w.glist.p;
+        if (w.n_glist[0] > 0)
+            hipLaunchKernelGGL(k_group_inv<8>, dim3(w.n_glist[0]), dim3(256), 0, w.stream, L, w.grp_off.p, w.grp_body.p,
+                               w.grp_moff.p, w.row_ptr.p, w.col_idx.p, w.Hval.p, w.Ginv.p);
+        if (w.n_glist[1] > 0)
+            hipLaunchKernelGGL(k_group_inv<4>, dim3(w.n_glist[1]), dim3(256), 0, w.stream, L + w.n_glist[0], w.grp_off.p,
+                               w.grp_body.p, w.grp_moff.p, w.row_ptr.p, w.col_idx.p, w.Hval.p, w.Ginv.p);
+        if (w.n_glist[2] > 0)
+            hipLaunchKernelGGL(k_group_inv<1>, dim3(w.n_glist[2]), dim3(64), 0, w.stream, L + w.n_glist[0] + w.n_glist[1],
+                               w.grp_off.p, w.grp_body.p, w.grp_moff.p, w.row_ptr.p, w.col_idx.p, w.Hval.p, w.Ginv.p);
         ADB_HIP(hipGetLastError());
     }
 }

@@ -11,6 +11,9 @@
 // the barrier follows model/barrier.cpp:11-28; CCD follows
 // model/collision.cpp:83-348 with the cubic solver of
 // cyCodeBase/cyPolynomial.h:383-447,587-609,787-880 (host defaults).
+// The cubic root finder below restates cyCodeBase's CubicRoots / FindClosed
+// (Copyright (c) 2022, Cem Yuksel; MIT license, full notice in
+// THIRD_PARTY_NOTICES.md at the repository root).
 #pragma once
 #include <hip/hip_runtime.h>
 

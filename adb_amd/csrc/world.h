@@ -170,6 +170,7 @@ struct World {
     int n_groups = 0;
     DBuf<int> glist;                 // group ids in launch order: multi-body groups first
     std::vector<int> h_glist;
+    int n_glist[3] = {0, 0, 0};      // groups of 5..8, 2..4 and 1 bodies, in that order in glist
     std::vector<int> h_grp_off, h_grp_body;
     std::vector<long long> h_grp_moff;
     int nnzb = 0;

@@ -241,7 +241,8 @@ inline double f1_over_x(double x, double eps) {
 // CCD (model/collision.cpp:83-348, cyCodeBase/cyPolynomial.h:69,117,383-447,
 // 587-609,787-880).  Root finding restates Yuksel's cubic solver with the
 // host defaults the reference uses (xError = 6e-7, Newton-first FindClosed,
-// boundError = false).
+// boundError = false).  cyCodeBase: Copyright (c) 2022, Cem Yuksel; MIT
+// license, full notice in THIRD_PARTY_NOTICES.md at the repository root.
 // ---------------------------------------------------------------------------
 inline double poly3(const double c[4], double x) { double r = c[3]; r = r * x + c[2]; r = r * x + c[1]; r = r * x + c[0]; return r; }
 inline double poly2(const double c[3], double x) { double r = c[2]; r = r * x + c[1]; r = r * x + c[0]; return r; }

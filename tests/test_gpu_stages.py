@@ -215,3 +215,33 @@ def test_tunings_give_identical_steps():
     assert src == "timed" and t["pcg_batch"] in autotune.CANDIDATES["pcg_batch"]
     q0 = gw.state()[0]
     assert np.array_equal(q0, sc["q0"])           # the tuner restored the state
+
+
+@pytest.mark.parametrize("case", ["spheres", "cubes", "chain"])
+def test_unprojected_assembly_matches_oracle(case):
+    # psd = 0: the exact contact Hessians (the adjoint's), every contact class
+    _require_gpu()
+    if case == "spheres":
+        sc = scenes.sphere_pack(12, seed=5, subdiv=2, rmin=0.2, rmax=0.3, half=0.8, ytop=1.5, gap=0.02, vsigma=0.3)
+        kw = dict(d_hat=4e-3, mu=0.3)
+    elif case == "cubes":
+        sc = scenes.cube_stack(4, seed=3, rotate=True)
+        kw = dict(d_hat=1e-3, mu=0.5)
+    else:
+        sc = scenes.torus_chain(6, seed=4)
+        kw = dict(d_hat=1e-5, mu=0.0, enable_ground=0)
+    p = default_params(kappa=1e-4, dt=0.01, **kw)
+    gw = World(sc, p)
+    for s in range(8):                      # into contact
+        gw.step()
+    q, qd = gw.state()
+    p0 = default_params(kappa=1e-4, dt=0.01, psd=0, **kw)
+    g0w, ow = World(sc, p0), O.OracleWorld(sc, p0)
+    g0w.set_state(q, qd)
+    ow.set_state(q, qd)
+    gg, Hg = g0w.assemble(dense=True)
+    go, Ho = ow.assemble(dense=True)
+    pt, ee, _ = ow.collision_set()
+    assert len(pt) + len(ee) > 0
+    np.testing.assert_allclose(gg, go, rtol=0, atol=1e-9 * np.abs(go).max())
+    np.testing.assert_allclose(Hg, Ho, rtol=0, atol=1e-9 * np.abs(Ho).max())
