@@ -25,52 +25,58 @@
 
 namespace adb {
 
-// inertia + orthogonality per body (H column-major)
+// inertia + orthogonality per body (H column-major): one thread per Hessian
+// entry (B x 144 threads, consecutive threads on consecutive entries of one
+// body, so the stores coalesce); entries 0..11 of a body also write its
+// gradient.  Every entry sees the operations of the plain per-body loop in
+// the same order: the diagonal mass term, then + h 4 kappa dt^2 for the
+// blocks of a1..a3.
 __global__ void k_inertia(int B, double dt, double kap, double ga0, double ga1, double ga2, const double* __restrict__ q,
                           const double* __restrict__ q0, const double* __restrict__ qdot, const double* __restrict__ mass,
                           const double* __restrict__ Ic, double* __restrict__ bodyG, double* __restrict__ bodyH) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= B) return;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 144ll * B) return;
+    const int b = (int)(t / 144), e = (int)(t % 144);
     double* G = bodyG + 12 * (size_t)b;
     double* H = bodyH + 144 * (size_t)b;
+    const int col = e / 12, row = e % 12;
     if (mass[b] < 0.0) {
-        for (int i = 0; i < 12; i++) G[i] = 0.0;
-        for (int i = 0; i < 144; i++) H[i] = (i % 13 == 0) ? 1.0 : 0.0;
+        if (e < 12) G[e] = 0.0;
+        H[e] = (e % 13 == 0) ? 1.0 : 0.0;
         return;
     }
     const double* Q = q + 12 * (size_t)b;
-    d3 a[4] = {ld3(Q), ld3(Q + 3), ld3(Q + 6), ld3(Q + 9)};
-    double go[12];
-    go[0] = go[1] = go[2] = 0.0;
-    for (int i = 1; i < 4; i++) {
-        double s0 = 0, s1 = 0, s2 = 0;
-        for (int j = 1; j < 4; j++) {
-            double s = dot(a[i], a[j]) - (i == j ? 1.0 : 0.0);
-            s0 += s * a[j].x; s1 += s * a[j].y; s2 += s * a[j].z;
-        }
-        go[3 * i] = 4 * kap * s0; go[3 * i + 1] = 4 * kap * s1; go[3 * i + 2] = 4 * kap * s2;
-    }
+    const d3 a[4] = {ld3(Q), ld3(Q + 3), ld3(Q + 6), ld3(Q + 9)};
     const double m = mass[b], ic = Ic[b];
-    for (int i = 0; i < 12; i++) {
-        double qt = q0[12 * (size_t)b + i] + dt * qdot[12 * (size_t)b + i];
-        if (i < 3) qt += dt * dt * (i == 0 ? ga0 : (i == 1 ? ga1 : ga2));
-        G[i] = dt * dt * go[i] + (i < 3 ? m : ic) * (Q[i] - qt);
+    if (e < 12) {
+        double go = 0.0;
+        const int i = e / 3, r = e % 3;
+        if (i > 0) {
+            double s0 = 0, s1 = 0, s2 = 0;
+            for (int j = 1; j < 4; j++) {
+                double s = dot(a[i], a[j]) - (i == j ? 1.0 : 0.0);
+                s0 += s * a[j].x; s1 += s * a[j].y; s2 += s * a[j].z;
+            }
+            go = 4 * kap * (r == 0 ? s0 : (r == 1 ? s1 : s2));
+        }
+        double qt = q0[12 * (size_t)b + e] + dt * qdot[12 * (size_t)b + e];
+        if (e < 3) qt += dt * dt * (e == 0 ? ga0 : (e == 1 ? ga1 : ga2));
+        G[e] = dt * dt * go + (e < 3 ? m : ic) * (Q[e] - qt);
     }
-    for (int i = 0; i < 144; i++) H[i] = 0.0;
-    for (int i = 0; i < 12; i++) H[13 * i] = i < 3 ? m : ic;
-    for (int i = 1; i < 4; i++)
-        for (int j = 1; j < 4; j++)
-            for (int r = 0; r < 3; r++)
-                for (int c = 0; c < 3; c++) {
-                    double h;
-                    if (i == j) {
-                        h = 2 * cmp(a[i], r) * cmp(a[i], c) + (r == c ? dot(a[i], a[i]) - 1 : 0.0);
-                        for (int k = 1; k < 4; k++) if (k != i) h += cmp(a[k], r) * cmp(a[k], c);
-                    } else {
-                        h = (r == c ? dot(a[j], a[i]) : 0.0) + cmp(a[j], r) * cmp(a[i], c);
-                    }
-                    H[(3 * j + c) * 12 + 3 * i + r] += h * 4 * kap * dt * dt;
-                }
+    double hv = row == col ? (row < 3 ? m : ic) : 0.0;
+    // H[(3 j + c) * 12 + 3 i + r]
+    const int j = col / 3, c = col % 3, i = row / 3, r = row % 3;
+    if (i >= 1 && j >= 1) {
+        double h;
+        if (i == j) {
+            h = 2 * cmp(a[i], r) * cmp(a[i], c) + (r == c ? dot(a[i], a[i]) - 1 : 0.0);
+            for (int k = 1; k < 4; k++) if (k != i) h += cmp(a[k], r) * cmp(a[k], c);
+        } else {
+            h = (r == c ? dot(a[j], a[i]) : 0.0) + cmp(a[j], r) * cmp(a[i], c);
+        }
+        hv += h * 4 * kap * dt * dt;
+    }
+    H[e] = hv;
 }
 
 // body-pair key of each contact (lo << bb | hi) and the contact id
@@ -395,253 +401,196 @@ __global__ void k_pair_diag(PairArgs a, const double* __restrict__ part) {
 // preconditioned system decouples exactly, so PCG needs only a few
 // iterations where the contact graph is a union of small clusters; in large
 // clusters it is block-Jacobi with blocks of up to GMAX bodies.
-// One workgroup per group: gather the blocks into LDS, Cholesky, W = L^-1,
-// M^-1 = W^T W.  A group whose matrix is not positive definite falls back to
-// the inverse diagonal.
-constexpr int GMAX = 8;
-// GM: largest group size of the launch; list: the groups of this launch.
 //
-// Blocked by bodies (12 x 12 blocks, m <= GM blocks per side), every step one
-// launch-wide phase between barriers:
-//   Cholesky, for each block column kb: the diagonal block by one wavefront
-//     (lane j owns column j in registers, columns of L broadcast by
-//     shuffles), the panel below it by forward substitution (one thread per
-//     row), the trailing lower triangle by 4 x 4 register tiles;
-//   W = L^-1: the diagonal blocks' inverses (one lane per column), then block
-//     rows ib = 1 .. m-1 in order: T = sum_jb L(ib, jb) W(jb, kb) for all
-//     kb < ib at once, W(ib, kb) = -W(ib, ib) T;
-//   M^-1 = W^T W on 4 x 4 register tiles of the upper triangle.
-// W is kept transposed in the strict upper triangle of the LDS matrix (L in
-// the lower one), its diagonal (1 / L_ii) in wd.  About 3 m + 2 m barriers per
-// group instead of a few per column.
-template <int GM>
-__global__ __launch_bounds__(256) void k_group_inv(const int* __restrict__ list, const int* __restrict__ grp_off,
-                                                   const int* __restrict__ grp_body, const long long* __restrict__ grp_moff,
-                                                   const int* __restrict__ row_ptr, const int* __restrict__ col_idx,
-                                                   const double* __restrict__ Hval, double* __restrict__ Ginv) {
-    constexpr int GN = 12 * GM, GLD = GN + 1;
-    __shared__ double A[GN * GLD];
-    __shared__ double Tm[(GM > 1 ? GM - 1 : 1) * 144];
-    __shared__ double wd[GN];
-    __shared__ int blk[GM][GM];
-    __shared__ int bod[GM];
-    __shared__ int ok;
-    const int g = list[blockIdx.x], t = threadIdx.x;
-    const int m = grp_off[g + 1] - grp_off[g], n = 12 * m;
-    if (t < m) bod[t] = grp_body[grp_off[g] + t];
-    if (t == 0) ok = 1;
-    __syncthreads();
-    for (int e = t; e < m * m; e += blockDim.x) {
+// The inverse is formed by the symmetric sweep operator (Goodnight, "A
+// tutorial on the SWEEP operator", The American Statistician 33(3), 1979):
+// sweeping pivot k of a symmetric A with d = a_kk replaces
+//     a_kk <- -1/d,   a_ik <- a_ik / d,   a_ij <- a_ij - a_ik a_kj / d  (i, j != k),
+// and after every pivot A holds -A^-1.  For a symmetric positive definite A
+// the pivots are the Schur complements (the squares of the Cholesky
+// diagonal), all positive; a pivot that is not marks the group as not
+// positive definite, which then falls back to the inverse of its diagonal.
+// Every pivot step is one rank-1 update of the whole matrix, so the group's
+// threads work in parallel at every step, with one synchronisation per step:
+// the latency of a 96 x 96 group is n steps, where a factor-then-invert
+// sequence waits on a chain of panel and triangular solves.
+//
+// Only the upper triangle is swept (the matrix stays symmetric): a thread
+// owns one T x T tile (bi <= bj) in registers.  Column k (= row k) is read
+// from an LDS vector c written at the end of the previous step by the
+// owners of its entries; double-buffered, so one barrier per step.
+// Size classes (one launch, classes by block range, largest first):
+//   5..8 bodies: one group per 256-lane workgroup, 6 x 6 tiles (136 of 16 x 16);
+//   2..4 bodies: one group per wavefront, 6 x 6 tiles (36 of 8 x 8);
+//   1 body:      four groups per wavefront, 16 lanes each, 3 x 3 tiles (10 of 4 x 4).
+constexpr int GMAX = 8;
+
+__device__ __forceinline__ void sweep_sync(bool block) {
+    if (block) {
+        __syncthreads();
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+// upper-triangle tile of a team lane: tiles enumerated row by row, (bi, bj), bi <= bj < nt
+__device__ __forceinline__ bool upper_tile(int l, int nt, int& bi, int& bj) {
+    int r = 0, rem = l;
+    while (r < nt && rem >= nt - r) { rem -= nt - r; r++; }
+    bi = r; bj = r + rem;
+    return r < nt;
+}
+// Sweep one group of m bodies (n = 12 m) with a team of lanes; c: 2 x NMAX
+// doubles of LDS, blk: m x m block indices (BSR position or -1).
+template <int T, int NMAX>
+__device__ __forceinline__ void group_sweep(int lane, int m, const int* blk, const double* __restrict__ Hval,
+                                            double* __restrict__ out, double* c, bool block_sync) {
+    constexpr int NT = NMAX / T;
+    const int n = 12 * m, nt = n / T;
+    int bi, bj;
+    const bool own = upper_tile(lane, NT, bi, bj) && bj < nt;
+    const int i0 = T * bi, j0 = T * bj;
+    double a[T][T];
+    if (own) {
+        // Hval blocks are column-major 12 x 12; a T x T tile lies in one block
+        const int k = blk[(i0 / 12) * m + j0 / 12];
+#pragma unroll
+        for (int y = 0; y < T; y++)
+#pragma unroll
+            for (int x = 0; x < T; x++)
+                a[x][y] = k >= 0 ? Hval[144 * (size_t)k + 12 * ((j0 + y) % 12) + (i0 + x) % 12] : 0.0;
+        // column 0 (= row 0): entries (0, j) of tile row 0
+        if (bi == 0)
+#pragma unroll
+            for (int y = 0; y < T; y++) c[j0 + y] = a[0][y];
+    }
+    sweep_sync(block_sync);
+    bool pd = true;
+    for (int k = 0; k < n; k++) {
+        const double* ck = c + (k & 1) * NMAX;
+        double* cn = c + ((k + 1) & 1) * NMAX;
+        const double d = ck[k];
+        if (!(d > 0.0) || !(d < INFINITY)) { pd = false; break; }      // uniform: every lane reads the same d
+        const double inv = 1.0 / d;
+        if (own) {
+            double ci[T], u[T];
+#pragma unroll
+            for (int x = 0; x < T; x++) ci[x] = ck[i0 + x];
+#pragma unroll
+            for (int y = 0; y < T; y++) u[y] = ck[j0 + y] * inv;
+#pragma unroll
+            for (int x = 0; x < T; x++)
+#pragma unroll
+                for (int y = 0; y < T; y++) {
+                    const int i = i0 + x, j = j0 + y;
+                    double v = a[x][y] - ci[x] * u[y];
+                    if (i == k || j == k) v = (i == j) ? -inv : (i == k ? u[y] : ci[x] * inv);
+                    a[x][y] = v;
+                }
+            // column k + 1 for the next step: (i, k + 1), i <= k + 1, and (k + 1, j), j >= k + 1
+            // (static tile indices only: a runtime index would put the tile in scratch memory)
+            const int k1 = k + 1;
+            if (k1 < n && ((k1 >= j0 && k1 < j0 + T) || (k1 >= i0 && k1 < i0 + T)))
+#pragma unroll
+                for (int x = 0; x < T; x++)
+#pragma unroll
+                    for (int y = 0; y < T; y++) {
+                        const int i = i0 + x, j = j0 + y;
+                        if (j == k1 && i <= k1) cn[i] = a[x][y];
+                        else if (i == k1 && j > k1) cn[j] = a[x][y];
+                    }
+        }
+        sweep_sync(block_sync);
+    }
+    if (!pd) {
+        // not positive definite: the inverse diagonal of the original matrix (from Hval)
+        if (own)
+#pragma unroll
+            for (int x = 0; x < T; x++)
+#pragma unroll
+                for (int y = 0; y < T; y++) {
+                    const int i = i0 + x, j = j0 + y;
+                    double v = 0.0;
+                    if (i == j) {
+                        const int k = blk[(i / 12) * m + i / 12];
+                        const double dd = k >= 0 ? Hval[144 * (size_t)k + 13 * (i % 12)] : 1.0;
+                        v = 1.0 / (dd > 0.0 ? dd : 1.0);
+                    }
+                    out[(size_t)j * n + i] = v;
+                    out[(size_t)i * n + j] = v;
+                }
+        return;
+    }
+    // A = -M^-1 (upper triangle); store both triangles, column-major
+    if (own)
+#pragma unroll
+        for (int x = 0; x < T; x++)
+#pragma unroll
+            for (int y = 0; y < T; y++) {
+                const int i = i0 + x, j = j0 + y;
+                if (i <= j) {
+                    out[(size_t)j * n + i] = -a[x][y];
+                    out[(size_t)i * n + j] = -a[x][y];
+                }
+            }
+}
+// block index of every (body a, body b) pair of a group, row-major m x m
+__device__ __forceinline__ void group_blocks(int lane, int nl, int m, const int* bod, const int* __restrict__ row_ptr,
+                                             const int* __restrict__ col_idx, int* blk) {
+    for (int e = lane; e < m * m; e += nl) {
         const int ia = e / m, ib = e % m;
         int k = -1;
         for (int q = row_ptr[bod[ia]]; q < row_ptr[bod[ia] + 1]; q++)
             if (col_idx[q] == bod[ib]) { k = q; break; }
-        blk[ia][ib] = k;
+        blk[e] = k;
     }
-    __syncthreads();
-    for (int e = t; e < n * n; e += blockDim.x) {
-        const int j = e / n, i = e % n;       // A[i][j] row-major in LDS; Hval blocks column-major
-        const int k = blk[i / 12][j / 12];
-        A[i * GLD + j] = k >= 0 ? Hval[144 * (size_t)k + 12 * (j % 12) + (i % 12)] : 0.0;
-    }
-    __syncthreads();
-    // ---------------------------------------------------------------- Cholesky
-    for (int kb = 0; kb < m; kb++) {
-        const int c0 = 12 * kb;
-        if (t < 64) {
-            const int j = t;
-            double col[12];
-#pragma unroll
-            for (int i = 0; i < 12; i++) col[i] = j < 12 ? A[(c0 + i) * GLD + c0 + j] : 0.0;
-            bool good = true;
-#pragma unroll
-            for (int k = 0; k < 12; k++) {
-                const double dkk = __shfl(col[k], k);
-                if (!(dkk > 0.0)) { good = false; break; }
-                const double lkk = sqrt(dkk);
-                if (j == k) {
-                    col[k] = lkk;
-#pragma unroll
-                    for (int i = k + 1; i < 12; i++) col[i] = col[i] / lkk;
-                }
-                double lk[12];
-#pragma unroll
-                for (int i = 0; i < 12; i++) lk[i] = __shfl(col[i], k);
-                double ljk = 0.0;
-#pragma unroll
-                for (int i = 0; i < 12; i++) ljk = i == j ? lk[i] : ljk;
-                if (j > k && j < 12) {
-#pragma unroll
-                    for (int i = k + 1; i < 12; i++)
-                        if (i >= j) col[i] -= lk[i] * ljk;
-                }
-            }
-            if (!good) { if (t == 0) ok = 0; }
-            else if (j < 12) {
-#pragma unroll
-                for (int i = 0; i < 12; i++)
-                    if (i >= j) A[(c0 + i) * GLD + c0 + j] = col[i];
-            }
-        }
+}
+__global__ __launch_bounds__(256) void k_group_inv(const int* __restrict__ list, int nb8, int n4, int n1,
+                                                   const int* __restrict__ grp_off, const int* __restrict__ grp_body,
+                                                   const long long* __restrict__ grp_moff, const int* __restrict__ row_ptr,
+                                                   const int* __restrict__ col_idx, const double* __restrict__ Hval,
+                                                   double* __restrict__ Ginv) {
+    __shared__ double c[4][2 * 96];     // class 8: c[0] (2 x 96); class 4: c[wave] (2 x 48); class 1: c[wave] (4 x 2 x 12)
+    __shared__ int blk[4][64];
+    __shared__ int bod[4][8];
+    const int t = threadIdx.x, wv = t / 64, ln = t % 64;
+    const int nb4 = (n4 + 3) / 4;
+    if ((int)blockIdx.x < nb8) {
+        const int g = list[blockIdx.x];
+        const int m = grp_off[g + 1] - grp_off[g];
+        if (t < m) bod[0][t] = grp_body[grp_off[g] + t];
         __syncthreads();
-        if (!ok) break;
-        const int rows = n - c0 - 12;
-        if (rows <= 0) break;
-        // panel: L(r, c0 + j) = (A(r, c0 + j) - sum_{k < j} L(r, c0 + k) L(c0 + j, c0 + k)) / L(c0 + j, c0 + j)
-        for (int rr = t; rr < rows; rr += blockDim.x) {
-            const int r = c0 + 12 + rr;
-            double x[12];
-#pragma unroll
-            for (int jj = 0; jj < 12; jj++) {
-                double v = A[r * GLD + c0 + jj];
-#pragma unroll
-                for (int k = 0; k < jj; k++) v -= x[k] * A[(c0 + jj) * GLD + c0 + k];
-                x[jj] = v / A[(c0 + jj) * GLD + c0 + jj];
-            }
-#pragma unroll
-            for (int jj = 0; jj < 12; jj++) A[r * GLD + c0 + jj] = x[jj];
-        }
+        group_blocks(t, 256, m, bod[0], row_ptr, col_idx, blk[0]);
         __syncthreads();
-        // trailing lower triangle: A(i, j) -= sum_k L(i, c0 + k) L(j, c0 + k), j <= i, 4 x 4 tiles
-        const int nt = (rows + 3) / 4, ntiles = nt * (nt + 1) / 2;
-        for (int tile = t; tile < ntiles; tile += blockDim.x) {
-            int ti = 0;
-            while ((ti + 1) * (ti + 2) / 2 <= tile) ti++;
-            const int tj = tile - ti * (ti + 1) / 2;
-            const int i0 = c0 + 12 + 4 * ti, j0 = c0 + 12 + 4 * tj;
-            double acc[4][4];
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
-#pragma unroll
-            for (int k = 0; k < 12; k++) {
-                double li[4], lj[4];
-#pragma unroll
-                for (int a = 0; a < 4; a++) li[a] = i0 + a < n ? A[(i0 + a) * GLD + c0 + k] : 0.0;
-#pragma unroll
-                for (int b = 0; b < 4; b++) lj[b] = j0 + b < n ? A[(j0 + b) * GLD + c0 + k] : 0.0;
-#pragma unroll
-                for (int a = 0; a < 4; a++)
-#pragma unroll
-                    for (int b = 0; b < 4; b++) acc[a][b] += li[a] * lj[b];
-            }
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int b = 0; b < 4; b++) {
-                    const int i = i0 + a, jj = j0 + b;
-                    if (i < n && jj <= i) A[i * GLD + jj] -= acc[a][b];
-                }
+        group_sweep<6, 96>(t, m, blk[0], Hval, Ginv + grp_moff[g], c[0], true);
+    } else if ((int)blockIdx.x < nb8 + nb4) {
+        const int q = 4 * ((int)blockIdx.x - nb8) + wv;          // one group per wavefront
+        if (q >= n4) return;                                       // whole wavefront: no block barrier below
+        const int g = list[nb8 + q];
+        const int m = grp_off[g + 1] - grp_off[g];
+        if (ln < m) bod[wv][ln] = grp_body[grp_off[g] + ln];
+        sweep_sync(false);
+        group_blocks(ln, 64, m, bod[wv], row_ptr, col_idx, blk[wv]);
+        sweep_sync(false);
+        group_sweep<6, 48>(ln, m, blk[wv], Hval, Ginv + grp_moff[g], c[wv], false);
+    } else {
+        const int q = 16 * ((int)blockIdx.x - nb8 - nb4) + 4 * wv + ln / 16;   // four groups per wavefront
+        const int sub = ln / 16, sl = ln % 16;
+        const bool have = q < n1;
+        // lanes of absent groups still take part in the wavefront's syncs
+        const int g = have ? list[nb8 + n4 + q] : -1;
+        if (have && sl == 0) {
+            bod[wv][sub] = grp_body[grp_off[g]];
+            int k = -1;
+            const int b = grp_body[grp_off[g]];
+            for (int p = row_ptr[b]; p < row_ptr[b + 1]; p++)
+                if (col_idx[p] == b) { k = p; break; }
+            blk[wv][sub] = k;
         }
-        __syncthreads();
+        sweep_sync(false);
+        if (have) group_sweep<3, 12>(sl, 1, &blk[wv][sub], Hval, Ginv + grp_moff[g], &c[wv][sub * 24], false);
     }
-    double* out = Ginv + grp_moff[g];
-    if (!ok) {
-        // not positive definite: the inverse diagonal of the original matrix (from Hval)
-        for (int e = t; e < n * n; e += blockDim.x) {
-            const int j = e / n, i = e % n;
-            double v = 0.0;
-            if (i == j) {
-                const int k = blk[i / 12][i / 12];
-                const double d = k >= 0 ? Hval[144 * (size_t)k + 13 * (i % 12)] : 1.0;
-                v = 1.0 / (d > 0.0 ? d : 1.0);
-            }
-            out[(size_t)j * n + i] = v;
-        }
-        return;
-    }
-    // ---------------------------------------------------------------- W = L^-1
-    // W(i, k), i > k, is stored at A[k][i]; W(i, i) = wd[i]
-    // diagonal blocks: lane (wave v, lane j < 12) solves L_bb w = e_j for b = v, v + 4, ...
-    {
-        const int v = t / 64, j = t % 64;
-        for (int b = v; b < m; b += (int)blockDim.x / 64) {
-            if (j < 12) {
-                const int c0 = 12 * b;
-                double wv[12];
-#pragma unroll
-                for (int i = 0; i < 12; i++) {
-                    double s = i == j ? 1.0 : 0.0;
-#pragma unroll
-                    for (int k = 0; k < i; k++)
-                        if (k >= j) s -= A[(c0 + i) * GLD + c0 + k] * wv[k];
-                    wv[i] = i >= j ? s / A[(c0 + i) * GLD + c0 + i] : 0.0;
-                }
-                wd[c0 + j] = wv[j];
-#pragma unroll
-                for (int i = 0; i < 12; i++)
-                    if (i > j) A[(c0 + j) * GLD + c0 + i] = wv[i];
-            }
-        }
-    }
-    __syncthreads();
-    // W(k, c) of an entry known to be in the lower triangle (k >= c)
-#define WLO(k, c) ((k) == (c) ? wd[(k)] : A[(c) * GLD + (k)])
-    for (int ib = 1; ib < m; ib++) {
-        // T(kb)(r, c) = sum_{jb = kb}^{ib - 1} sum_s L(12 ib + r, 12 jb + s) W(12 jb + s, 12 kb + c)
-        for (int e = t; e < ib * 144; e += blockDim.x) {
-            const int kb = e / 144, r = (e % 144) / 12, c = e % 12;
-            const int row = 12 * ib + r, cc = 12 * kb + c;
-            double sum = 0.0;
-            for (int q = cc; q < 12 * ib; q++) sum += A[row * GLD + q] * WLO(q, cc);
-            Tm[e] = sum;
-        }
-        __syncthreads();
-        // W(ib, kb)(r, c) = -sum_{s <= r} W(12 ib + r, 12 ib + s) T(kb)(s, c)
-        for (int e = t; e < ib * 144; e += blockDim.x) {
-            const int kb = e / 144, r = (e % 144) / 12, c = e % 12;
-            const int row = 12 * ib + r;
-            double sum = 0.0;
-            for (int q = 0; q <= r; q++) sum += WLO(row, 12 * ib + q) * Tm[kb * 144 + 12 * q + c];
-            A[(12 * kb + c) * GLD + row] = -sum;
-        }
-        __syncthreads();
-    }
-    // ---------------------------------------------------------------- M^-1 = W^T W
-    // (i, j), i <= j: sum_{k >= j} W(k, i) W(k, j), 4 x 4 tiles of the upper triangle
-    {
-        const int nt = (n + 3) / 4, ntiles = nt * (nt + 1) / 2;
-        for (int tile = t; tile < ntiles; tile += blockDim.x) {
-            int tj = 0;
-            while ((tj + 1) * (tj + 2) / 2 <= tile) tj++;
-            const int ti = tile - tj * (tj + 1) / 2;    // ti <= tj
-            const int i0 = 4 * ti, j0 = 4 * tj;
-            double acc[4][4];
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
-            for (int k = j0; k < n; k++) {
-                double wi[4], wj[4];
-#pragma unroll
-                for (int a = 0; a < 4; a++) {
-                    const int i = i0 + a;
-                    wi[a] = (i < n && k >= i) ? WLO(k, i) : 0.0;
-                }
-#pragma unroll
-                for (int b = 0; b < 4; b++) {
-                    const int jj = j0 + b;
-                    wj[b] = (jj < n && k >= jj) ? WLO(k, jj) : 0.0;
-                }
-#pragma unroll
-                for (int a = 0; a < 4; a++)
-#pragma unroll
-                    for (int b = 0; b < 4; b++) acc[a][b] += wi[a] * wj[b];
-            }
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int b = 0; b < 4; b++) {
-                    const int i = i0 + a, jj = j0 + b;
-                    if (i <= jj && jj < n) {
-                        out[(size_t)jj * n + i] = acc[a][b];
-                        out[(size_t)i * n + jj] = acc[a][b];
-                    }
-                }
-        }
-    }
-#undef WLO
 }
 
 // Body groups for the preconditioner (host): heavy-edge aggregation of the
@@ -720,9 +669,9 @@ static void build_groups(World& w) {
         w.sc_goff.reserve(w.S + 1);
         ADB_HIP(hipMemcpyAsync(w.sc_goff.p, w.h_sc_goff.data(), (w.S + 1) * sizeof(int), hipMemcpyHostToDevice, w.stream));
     }
-    // groups with several bodies first: their inverses take longest
-    // launch lists by size class (the inverse kernel's LDS is sized for the
-    // class, so small groups share a CU): 5..GMAX bodies, 2..4, single bodies
+    // group lists by size class, largest first (their inverses take longest;
+    // the inverse kernel gives each class its own team width): 5..GMAX
+    // bodies, 2..4, single bodies
     w.h_glist.clear();
     for (int g = 0; g < w.n_groups; g++) if (w.h_grp_off[g + 1] - w.h_grp_off[g] > 4) w.h_glist.push_back(g);
     w.n_glist[0] = (int)w.h_glist.size();
@@ -746,25 +695,12 @@ static void build_groups(World& w) {
 }
 
 // FP64 operations of k_group_inv for a group of m bodies (n = 12 m), counted
-// from its loops: a multiply-add is 2, a divide or square root 1.
+// from the sweep: per pivot step one reciprocal, n products u_j = a_kj / d,
+// 2 per upper entry off row and column k (product, subtraction), 1 per entry
+// of row and column k but the pivot: n^2 + n per step.
 static double group_inv_flops(int m) {
-    const int n = 12 * m;
-    double f = 0.0;
-    for (int kb = 0; kb < m; kb++) {
-        f += 650.0;                                   // 12 x 12 diagonal Cholesky
-        const double rows = n - 12.0 * (kb + 1);
-        f += rows * 144.0;                            // panel forward substitution
-        f += rows * (rows + 1.0) / 2.0 * 25.0;        // trailing update: 12 products + 1 subtraction per entry
-    }
-    f += 650.0 * m;                                   // inverses of the diagonal blocks
-    for (int ib = 1; ib < m; ib++)
-        for (int kb = 0; kb < ib; kb++)
-            for (int c = 0; c < 12; c++) {
-                f += 12.0 * 2.0 * (12.0 * ib - (12.0 * kb + c));   // T(kb): 12 rows
-                for (int r = 0; r < 12; r++) f += 2.0 * (r + 1);   // W(ib, kb) = -W(ib, ib) T
-            }
-    for (int j = 0; j < n; j++) f += (j + 1.0) * (n - j) * 2.0;   // W^T W, upper triangle
-    return f;
+    const double n = 12.0 * m;
+    return n * (n * n + n);
 }
 
 template <class K, class V>
@@ -779,7 +715,7 @@ void assemble_system(World& w) {
     const int B = w.B;
     w.bodyG.reserve(12 * (size_t)B);
     w.bodyH.reserve(144 * (size_t)B);
-    hipLaunchKernelGGL(k_inertia, dim3((B + 127) / 128), dim3(128), 0, w.stream, B, w.P.dt, w.P.kappa_ortho, w.P.gravity[0], w.P.gravity[1], w.P.gravity[2],
+    hipLaunchKernelGGL(k_inertia, dim3((144 * B + 255) / 256), dim3(256), 0, w.stream, B, w.P.dt, w.P.kappa_ortho, w.P.gravity[0], w.P.gravity[1], w.P.gravity[2],
                        w.q.p, w.q0.p, w.qdot.p, w.mass.p, w.Ic.p, w.bodyG.p, w.bodyH.p);
     ADB_HIP(hipGetLastError());
     const int nc = w.cs.n_pt + w.cs.n_ee;
@@ -894,16 +830,11 @@ void assemble_system(World& w) {
             ps.flops = fl;
         }
         static_assert(GMAX == 8, "size classes 5..8, 2..4, 1");
-        const int* L = w.glist.p;
-        if (w.n_glist[0] > 0)
-            hipLaunchKernelGGL(k_group_inv<8>, dim3(w.n_glist[0]), dim3(256), 0, w.stream, L, w.grp_off.p, w.grp_body.p,
-                               w.grp_moff.p, w.row_ptr.p, w.col_idx.p, w.Hval.p, w.Ginv.p);
-        if (w.n_glist[1] > 0)
-            hipLaunchKernelGGL(k_group_inv<4>, dim3(w.n_glist[1]), dim3(256), 0, w.stream, L + w.n_glist[0], w.grp_off.p,
-                               w.grp_body.p, w.grp_moff.p, w.row_ptr.p, w.col_idx.p, w.Hval.p, w.Ginv.p);
-        if (w.n_glist[2] > 0)
-            hipLaunchKernelGGL(k_group_inv<1>, dim3(w.n_glist[2]), dim3(64), 0, w.stream, L + w.n_glist[0] + w.n_glist[1],
-                               w.grp_off.p, w.grp_body.p, w.grp_moff.p, w.row_ptr.p, w.col_idx.p, w.Hval.p, w.Ginv.p);
+        const int nb = w.n_glist[0] + (w.n_glist[1] + 3) / 4 + (w.n_glist[2] + 15) / 16;
+        if (nb > 0)
+            hipLaunchKernelGGL(k_group_inv, dim3(nb), dim3(256), 0, w.stream, w.glist.p, w.n_glist[0], w.n_glist[1],
+                               w.n_glist[2], w.grp_off.p, w.grp_body.p, w.grp_moff.p, w.row_ptr.p, w.col_idx.p, w.Hval.p,
+                               w.Ginv.p);
         ADB_HIP(hipGetLastError());
     }
 }

@@ -145,6 +145,7 @@ def max_vertex_deviation(sc, qa, qb):
 
 # kernels launched once per launch of each profiling scope
 KERNEL_OF_SCOPE = {"asm_reduce": ["k_pair_reduce"], "narrow_set": ["k_pair_narrow<0", "k_pair_narrow<2", "k_trial_set"],
+                   "narrow_ccd": ["k_pair_narrow<1"],
                    "block_chol": ["k_group_inv"], "contact_group": ["k_contact_c3"],
                    "contact": ["k_contact_small<1>", "k_contact_scalars2"], "contact_pk": ["k_contact_pk2"],
                    "spmv": ["k_pcg_spmv"], "pcg_update": ["k_pcg_update"], "contact_classify": ["k_contact_classify"]}
@@ -201,6 +202,39 @@ def pmc_traffic(fetch_csv, write_csv, prof_json, scope, copy_mb):
     return out
 
 
+def pmc_fp64(counter_csv, prof_json, scopes):
+    """FP64 operations per launch of FP64-VALU-bound scopes from one rocprofv3
+    --pmc run with SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 (wave instructions
+    per dispatch) of `bench.py --no-compare --no-cpu-baseline --profile-json
+    P`: ops = 64 (ADD + MUL + TRANS) + 128 FMA per dispatch (every lane of
+    every issued instruction, so lanes masked off count too: an upper bound on
+    the useful operations), averaged over the last N dispatches of each
+    kernel, N = the scope's launches in the profiled steps of P."""
+    import csv
+    with open(prof_json) as fh:
+        prof = json.load(fh)
+    rows = list(csv.DictReader(open(counter_csv)))
+    out = {"source": "rocprofv3 --pmc SQ_INSTS_VALU_ADD_F64 SQ_INSTS_VALU_MUL_F64 SQ_INSTS_VALU_FMA_F64 "
+                      "SQ_INSTS_VALU_TRANS_F64", "scopes": {}}
+    weight = {"SQ_INSTS_VALU_ADD_F64": 64.0, "SQ_INSTS_VALU_MUL_F64": 64.0, "SQ_INSTS_VALU_TRANS_F64": 64.0,
+              "SQ_INSTS_VALU_FMA_F64": 128.0}
+    for scope in scopes:
+        n = prof[scope]["launches"]
+        if n <= 0:
+            continue
+        total = 0.0
+        for k in KERNEL_OF_SCOPE[scope]:
+            per = {}
+            for r in rows:
+                if k in r["Kernel_Name"] and r["Counter_Name"] in weight:
+                    per.setdefault(r["Dispatch_Id"], 0.0)
+                    per[r["Dispatch_Id"]] += weight[r["Counter_Name"]] * float(r["Counter_Value"])
+            vals = [per[d] for d in sorted(per, key=int)][-n:]
+            total += sum(vals) / max(1, len(vals))
+        out["scopes"][scope] = {"kernels": KERNEL_OF_SCOPE[scope], "launches": n, "fp64_ops_per_launch": total}
+    return out
+
+
 def kernel_stats_md(stats_csv, title, cmd, top=25):
     """Markdown table of a rocprofv3 --stats kernel summary (CSV)."""
     import csv
@@ -240,12 +274,22 @@ def main():
     ap.add_argument("--pmc-traffic", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "PROFILE_JSON", "SCOPE"),
                     help="no run: summarize two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) into HBM bytes per "
                          "launch of one kernel scope, scaled by the --copy-mb copy; writes --out")
+    ap.add_argument("--pmc-fp64", nargs=2, metavar=("COUNTER_CSV", "PROFILE_JSON"),
+                    help="no run: summarize a rocprofv3 --pmc run of the FP64 instruction counters into FP64 "
+                         "operations per launch of the FP64-VALU-bound scopes; writes --out")
     ap.add_argument("--kernel-stats", nargs=3, metavar=("STATS_CSV", "TITLE", "CMD"),
                     help="no run: write a markdown table of a rocprofv3 --stats kernel summary to --out")
     ap.add_argument("--out", default="")
     args = ap.parse_args()
     if args.pmc_traffic:
         res = pmc_traffic(*args.pmc_traffic, args.copy_mb)
+        if args.out:
+            with open(args.out, "w") as fh:
+                json.dump(res, fh, indent=1)
+        print(json.dumps(res))
+        return
+    if args.pmc_fp64:
+        res = pmc_fp64(args.pmc_fp64[0], args.pmc_fp64[1], sorted(FP64_OPS_OF_SCOPE | {"block_chol"}))
         if args.out:
             with open(args.out, "w") as fh:
                 json.dump(res, fh, indent=1)

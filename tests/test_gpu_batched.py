@@ -6,9 +6,9 @@
   steps; and against each scene stepped alone on the GPU.
 * The 256-scene workload (U(8, 200) bodies per scene, cubes or 162-vertex
   icospheres, dropped onto the ground, FP32 broad-phase boxes): 50 steps with
-  every scene converging; 3 sampled scenes against their own single-scene
-  worlds (Newton counts equal every step, states within 1e-5 of the extent:
-  the PCG iterates differ at the solver tolerance and the piles grow that).
+  every scene converging and every vertex above the ground at the end; 3
+  sampled scenes against their own single-scene worlds over the first 20
+  steps (Newton counts equal, states within 1e-8 of the extent).
 * Bodies of different scenes never interact even when the scenes overlap in
   space: two copies of one scene at the same place step exactly like one.
 """
@@ -90,20 +90,30 @@ def test_256_scenes_50_steps():
     bw = World(full, p, scene_offsets=off)
     sample = [int(np.argmin(counts)), int(np.argmax(counts)), 128]
     alone = {s: World(parts[s], p) for s in sample}
+    ground = p.ground_y
     for step in range(50):
         rc, st = bw.step()
         newton, ls, status, contacts = bw.batch_stats()
         assert np.all(status == 0), step
         q, _ = bw.state()
+        assert np.isfinite(q).all()
+        if step >= 20:
+            continue
         for s in sample:
             rs, ss = alone[s].step()
             assert newton[s] == ss.newton_iters, (step, s)
             sc = parts[s]
             ext = float(np.ptp(world_vertices(sc, sc["q0"]), axis=0).max())
-            # the batched PCG stops when every scene's residual is below its
-            # tolerance, a single-scene PCG when its own is: iterates differ at
-            # the solver tolerance, which 50 steps of piling grow (Newton
-            # tolerance 1e-2) but which never change a Newton count
+            # The batched PCG runs one Krylov space over the union of the
+            # scenes and stops when every scene's residual is below its
+            # tolerance; a single-scene PCG stops on its own.  Iterates differ
+            # at the solver tolerance (1e-12 relative), and the pile grows
+            # that: ~1e-8 of the extent by step 30, then a contact that enters
+            # in one run a Newton iteration before the other makes the runs
+            # part.  The first 20 steps are compared; all 50 must succeed.
             dev = np.abs(world_vertices(sc, q[off[s]:off[s + 1]]) - world_vertices(sc, alone[s].state()[0])).max()
-            assert dev <= 1e-5 * ext, (step, s, dev)
+            assert dev <= 1e-8 * ext, (step, s, dev)
+    for s in range(len(counts)):
+        sc = parts[s]
+        assert world_vertices(sc, q[off[s]:off[s + 1]])[:, 1].min() > ground
     assert contacts.sum() > 1000 and sum(counts) == bw.B
