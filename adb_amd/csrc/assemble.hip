@@ -402,30 +402,32 @@ __global__ void k_pair_diag(PairArgs a, const double* __restrict__ part) {
 // iterations where the contact graph is a union of small clusters; in large
 // clusters it is block-Jacobi with blocks of up to GMAX bodies.
 //
-// The inverse is formed by the symmetric sweep operator (Goodnight, "A
-// tutorial on the SWEEP operator", The American Statistician 33(3), 1979):
-// sweeping pivot k of a symmetric A with d = a_kk replaces
-//     a_kk <- -1/d,   a_ik <- a_ik / d,   a_ij <- a_ij - a_ik a_kj / d  (i, j != k),
-// and after every pivot A holds -A^-1.  For a symmetric positive definite A
-// the pivots are the Schur complements (the squares of the Cholesky
-// diagonal), all positive; a pivot that is not marks the group as not
-// positive definite, which then falls back to the inverse of its diagonal.
-// Every pivot step is one rank-1 update of the whole matrix, so the group's
-// threads work in parallel at every step, with one synchronisation per step:
-// the latency of a 96 x 96 group is n steps, where a factor-then-invert
-// sequence waits on a chain of panel and triangular solves.
-//
-// Only the upper triangle is swept (the matrix stays symmetric): a thread
-// owns one T x T tile (bi <= bj) in registers.  Column k (= row k) is read
-// from an LDS vector c written at the end of the previous step by the
-// owners of its entries; double-buffered, so one barrier per step.
-// Size classes (one launch, classes by block range, largest first):
-//   5..8 bodies: one group per 256-lane workgroup, 6 x 6 tiles (136 of 16 x 16);
-//   2..4 bodies: one group per wavefront, 6 x 6 tiles (36 of 8 x 8);
-//   1 body:      four groups per wavefront, 16 lanes each, 3 x 3 tiles (10 of 4 x 4).
+// The inverse is formed by the block sweep operator (the sweep operator of
+// Goodnight, "A tutorial on the SWEEP operator", The American Statistician
+// 33(3), 1979, taken one 12 x 12 body block at a time).  Sweeping block K of
+// a symmetric A with D = A_KK replaces
+//     A_KK <- -D^-1,  A_iK <- A_iK D^-1,  A_Kj <- D^-1 A_Kj,
+//     A_ij <- A_ij - A_iK D^-1 A_Kj                    (i, j outside K),
+// and after every block A holds -A^-1.  For a symmetric positive definite A
+// every D is a Schur complement, itself positive definite; a non-positive
+// pivot while inverting D (itself a 12-step scalar sweep) marks the group
+// as not positive definite, which then falls back to the inverse of its
+// diagonal.  A group of m bodies takes m block steps, each a rank-12 update
+// of the whole upper triangle in which every lane of the group works:
+// a lane owns one 6 x 6 tile (a 6 x 6 tile lies in one body block) and
+// updates it with 12 fused multiply-adds per entry from the column panel
+// P = A_:K and W = P D^-1, both in LDS (transposed, so a tile's 6 rows or
+// columns are 6 consecutive doubles).  Size classes, one launch, classes by
+// block range, largest first: 5..8 bodies one group per 256-lane workgroup
+// (136 tiles of 16 x 16), 1..4 bodies one group per wavefront (36 tiles of
+// 8 x 8, wavefront-level synchronisation).
 constexpr int GMAX = 8;
+constexpr int GI_LD = 97;            // padded row length of the transposed panels (class 5..8)
+constexpr int GI_LD4 = 49;           // the same for groups of up to 4 bodies
+// LDS of one group: PT[12][LD], WT[12][LD], D[12][13]
+constexpr int GI_LDS8 = 24 * GI_LD + 156, GI_LDS4 = 24 * GI_LD4 + 156;
 
-__device__ __forceinline__ void sweep_sync(bool block) {
+__device__ __forceinline__ void gi_sync(bool block) {
     if (block) {
         __syncthreads();
     } else {
@@ -434,74 +436,131 @@ __device__ __forceinline__ void sweep_sync(bool block) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
-// upper-triangle tile of a team lane: tiles enumerated row by row, (bi, bj), bi <= bj < nt
+// upper-triangle tile of a lane: tiles enumerated row by row, (bi, bj), bi <= bj < nt
 __device__ __forceinline__ bool upper_tile(int l, int nt, int& bi, int& bj) {
     int r = 0, rem = l;
     while (r < nt && rem >= nt - r) { rem -= nt - r; r++; }
     bi = r; bj = r + rem;
     return r < nt;
 }
-// Sweep one group of m bodies (n = 12 m) with a team of lanes; c: 2 x NMAX
-// doubles of LDS, blk: m x m block indices (BSR position or -1).
-template <int T, int NMAX>
-__device__ __forceinline__ void group_sweep(int lane, int m, const int* blk, const double* __restrict__ Hval,
-                                            double* __restrict__ out, double* c, bool block_sync) {
-    constexpr int NT = NMAX / T;
+// In-place scalar sweep of the 12 x 12 symmetric D (LDS, row length 13) by
+// the team's NL lanes; afterwards D holds -D^-1.  Returns false (on every
+// lane) when a pivot is not positive.
+template <int NL>
+__device__ __forceinline__ bool sweep12(double* D, int lane, bool block_sync) {
+    constexpr int S = (144 + NL - 1) / NL;
+    for (int k = 0; k < 12; k++) {
+        const double d = D[13 * k + k];
+        if (!(d > 0.0) || !(d < INFINITY)) return false;       // the same d on every lane
+        const double inv = 1.0 / d;
+        double v[S];
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const int e = lane + NL * s, i = e / 12, j = e % 12;
+            v[s] = 0.0;
+            if (e < 144) {
+                const double ci = D[13 * i + k], cj = D[13 * k + j];
+                double x = D[13 * i + j] - ci * (cj * inv);
+                if (i == k || j == k) x = (i == j) ? -inv : (i == k ? cj * inv : ci * inv);
+                v[s] = x;
+            }
+        }
+        gi_sync(block_sync);
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            const int e = lane + NL * s;
+            if (e < 144) D[13 * (e / 12) + e % 12] = v[s];
+        }
+        gi_sync(block_sync);
+    }
+    return true;
+}
+// Block sweep of one group of m bodies (n = 12 m <= NMAX) by a team of NL
+// lanes.  sm: LDS of the group (PT, WT, D); blk: m x m BSR block positions (-1: zero).
+template <int NMAX, int NL, int LD>
+__device__ __forceinline__ void group_bsweep(int lane, int m, const int* blk, const double* __restrict__ Hval,
+                                             double* __restrict__ out, double* sm, bool block_sync) {
+    constexpr int T = 6, NT = NMAX / T;
+    double* PT = sm;                 // PT[r][i] = A[i][K0 + r]
+    double* WT = sm + 12 * LD;       // WT[c][i] = (P D^-1)[i][c]
+    double* D = sm + 24 * LD;        // 12 x 12, row length 13
     const int n = 12 * m, nt = n / T;
     int bi, bj;
     const bool own = upper_tile(lane, NT, bi, bj) && bj < nt;
-    const int i0 = T * bi, j0 = T * bj;
+    const int i0 = T * bi, j0 = T * bj, brow = bi / 2, bcol = bj / 2;
     double a[T][T];
     if (own) {
-        // Hval blocks are column-major 12 x 12; a T x T tile lies in one block
-        const int k = blk[(i0 / 12) * m + j0 / 12];
+        // Hval blocks are column-major 12 x 12
+        const int k = blk[brow * m + bcol];
 #pragma unroll
         for (int y = 0; y < T; y++)
 #pragma unroll
             for (int x = 0; x < T; x++)
                 a[x][y] = k >= 0 ? Hval[144 * (size_t)k + 12 * ((j0 + y) % 12) + (i0 + x) % 12] : 0.0;
-        // column 0 (= row 0): entries (0, j) of tile row 0
-        if (bi == 0)
-#pragma unroll
-            for (int y = 0; y < T; y++) c[j0 + y] = a[0][y];
     }
-    sweep_sync(block_sync);
     bool pd = true;
-    for (int k = 0; k < n; k++) {
-        const double* ck = c + (k & 1) * NMAX;
-        double* cn = c + ((k + 1) & 1) * NMAX;
-        const double d = ck[k];
-        if (!(d > 0.0) || !(d < INFINITY)) { pd = false; break; }      // uniform: every lane reads the same d
-        const double inv = 1.0 / d;
-        if (own) {
-            double ci[T], u[T];
-#pragma unroll
-            for (int x = 0; x < T; x++) ci[x] = ck[i0 + x];
-#pragma unroll
-            for (int y = 0; y < T; y++) u[y] = ck[j0 + y] * inv;
+    for (int kb = 0; kb < m; kb++) {
+        const int K0 = 12 * kb;
+        // the column panel A[:, K] from the upper triangle (and its mirror)
+        if (own && (bcol == kb || brow == kb)) {
 #pragma unroll
             for (int x = 0; x < T; x++)
 #pragma unroll
                 for (int y = 0; y < T; y++) {
                     const int i = i0 + x, j = j0 + y;
-                    double v = a[x][y] - ci[x] * u[y];
-                    if (i == k || j == k) v = (i == j) ? -inv : (i == k ? u[y] : ci[x] * inv);
-                    a[x][y] = v;
+                    if (i <= j) {
+                        if (bcol == kb) PT[(j - K0) * LD + i] = a[x][y];
+                        if (brow == kb) PT[(i - K0) * LD + j] = a[x][y];
+                    }
                 }
-            // column k + 1 for the next step: (i, k + 1), i <= k + 1, and (k + 1, j), j >= k + 1
-            // (static tile indices only: a runtime index would put the tile in scratch memory)
-            const int k1 = k + 1;
-            if (k1 < n && ((k1 >= j0 && k1 < j0 + T) || (k1 >= i0 && k1 < i0 + T)))
+        }
+        gi_sync(block_sync);
+        // D = the panel's rows in K; -D^-1 by the scalar sweep
+        for (int e = lane; e < 144; e += NL) D[13 * (e / 12) + e % 12] = PT[(e % 12) * LD + K0 + e / 12];
+        gi_sync(block_sync);
+        if (!sweep12<NL>(D, lane, block_sync)) { pd = false; break; }
+        // W = P D^-1 = -P (D swept), rows outside K (rows in K are not used)
+        for (int e = lane; e < 12 * n; e += NL) {
+            const int i = e / 12, c = e % 12;
+            double s = 0.0;
+#pragma unroll
+            for (int r = 0; r < 12; r++) s = fma(PT[r * LD + i], D[13 * r + c], s);
+            WT[c * LD + i] = -s;
+        }
+        gi_sync(block_sync);
+        if (own) {
+            if (brow != kb && bcol != kb) {
+                // A_ij -= sum_r W[i][r] P[j][r]
+#pragma unroll 2
+                for (int r = 0; r < 12; r++) {
+                    double wv[T], pv[T];
+#pragma unroll
+                    for (int x = 0; x < T; x++) wv[x] = WT[r * LD + i0 + x];
+#pragma unroll
+                    for (int y = 0; y < T; y++) pv[y] = PT[r * LD + j0 + y];
+#pragma unroll
+                    for (int x = 0; x < T; x++)
+#pragma unroll
+                        for (int y = 0; y < T; y++) a[x][y] = fma(-wv[x], pv[y], a[x][y]);
+                }
+            } else if (brow == kb && bcol == kb) {
 #pragma unroll
                 for (int x = 0; x < T; x++)
 #pragma unroll
-                    for (int y = 0; y < T; y++) {
-                        const int i = i0 + x, j = j0 + y;
-                        if (j == k1 && i <= k1) cn[i] = a[x][y];
-                        else if (i == k1 && j > k1) cn[j] = a[x][y];
-                    }
+                    for (int y = 0; y < T; y++) a[x][y] = D[13 * (i0 + x - K0) + j0 + y - K0];     // -D^-1
+            } else if (bcol == kb) {
+#pragma unroll
+                for (int x = 0; x < T; x++)
+#pragma unroll
+                    for (int y = 0; y < T; y++) a[x][y] = WT[(j0 + y - K0) * LD + i0 + x];       // (A_iK D^-1)
+            } else {
+#pragma unroll
+                for (int x = 0; x < T; x++)
+#pragma unroll
+                    for (int y = 0; y < T; y++) a[x][y] = WT[(i0 + x - K0) * LD + j0 + y];       // (D^-1 A_Kj)
+            }
         }
-        sweep_sync(block_sync);
+        gi_sync(block_sync);
     }
     if (!pd) {
         // not positive definite: the inverse diagonal of the original matrix (from Hval)
@@ -546,16 +605,16 @@ __device__ __forceinline__ void group_blocks(int lane, int nl, int m, const int*
         blk[e] = k;
     }
 }
-__global__ __launch_bounds__(256) void k_group_inv(const int* __restrict__ list, int nb8, int n4, int n1,
+__global__ __launch_bounds__(256) void k_group_inv(const int* __restrict__ list, int nb8, int n4,
                                                    const int* __restrict__ grp_off, const int* __restrict__ grp_body,
                                                    const long long* __restrict__ grp_moff, const int* __restrict__ row_ptr,
                                                    const int* __restrict__ col_idx, const double* __restrict__ Hval,
                                                    double* __restrict__ Ginv) {
-    __shared__ double c[4][2 * 96];     // class 8: c[0] (2 x 96); class 4: c[wave] (2 x 48); class 1: c[wave] (4 x 2 x 12)
+    static_assert(4 * GI_LDS4 >= GI_LDS8, "LDS of one 5..8-body group within that of four small groups");
+    __shared__ double sm[4 * GI_LDS4];
     __shared__ int blk[4][64];
     __shared__ int bod[4][8];
     const int t = threadIdx.x, wv = t / 64, ln = t % 64;
-    const int nb4 = (n4 + 3) / 4;
     if ((int)blockIdx.x < nb8) {
         const int g = list[blockIdx.x];
         const int m = grp_off[g + 1] - grp_off[g];
@@ -563,33 +622,17 @@ __global__ __launch_bounds__(256) void k_group_inv(const int* __restrict__ list,
         __syncthreads();
         group_blocks(t, 256, m, bod[0], row_ptr, col_idx, blk[0]);
         __syncthreads();
-        group_sweep<6, 96>(t, m, blk[0], Hval, Ginv + grp_moff[g], c[0], true);
-    } else if ((int)blockIdx.x < nb8 + nb4) {
-        const int q = 4 * ((int)blockIdx.x - nb8) + wv;          // one group per wavefront
+        group_bsweep<96, 256, GI_LD>(t, m, blk[0], Hval, Ginv + grp_moff[g], sm, true);
+    } else {
+        const int q = 4 * ((int)blockIdx.x - nb8) + wv;           // one group per wavefront
         if (q >= n4) return;                                       // whole wavefront: no block barrier below
         const int g = list[nb8 + q];
         const int m = grp_off[g + 1] - grp_off[g];
         if (ln < m) bod[wv][ln] = grp_body[grp_off[g] + ln];
-        sweep_sync(false);
+        gi_sync(false);
         group_blocks(ln, 64, m, bod[wv], row_ptr, col_idx, blk[wv]);
-        sweep_sync(false);
-        group_sweep<6, 48>(ln, m, blk[wv], Hval, Ginv + grp_moff[g], c[wv], false);
-    } else {
-        const int q = 16 * ((int)blockIdx.x - nb8 - nb4) + 4 * wv + ln / 16;   // four groups per wavefront
-        const int sub = ln / 16, sl = ln % 16;
-        const bool have = q < n1;
-        // lanes of absent groups still take part in the wavefront's syncs
-        const int g = have ? list[nb8 + n4 + q] : -1;
-        if (have && sl == 0) {
-            bod[wv][sub] = grp_body[grp_off[g]];
-            int k = -1;
-            const int b = grp_body[grp_off[g]];
-            for (int p = row_ptr[b]; p < row_ptr[b + 1]; p++)
-                if (col_idx[p] == b) { k = p; break; }
-            blk[wv][sub] = k;
-        }
-        sweep_sync(false);
-        if (have) group_sweep<3, 12>(sl, 1, &blk[wv][sub], Hval, Ginv + grp_moff[g], &c[wv][sub * 24], false);
+        gi_sync(false);
+        group_bsweep<48, 64, GI_LD4>(ln, m, blk[wv], Hval, Ginv + grp_moff[g], sm + wv * GI_LDS4, false);
     }
 }
 
@@ -695,12 +738,22 @@ static void build_groups(World& w) {
 }
 
 // FP64 operations of k_group_inv for a group of m bodies (n = 12 m), counted
-// from the sweep: per pivot step one reciprocal, n products u_j = a_kj / d,
-// 2 per upper entry off row and column k (product, subtraction), 1 per entry
-// of row and column k but the pivot: n^2 + n per step.
+// from the block sweep (a fused multiply-add is 2): per block step the 12 x 12
+// scalar sweep (12 steps of 144 entries, 3 operations each, and a
+// reciprocal), W = P D^-1 (12 n entries of 12 multiply-adds) and the rank-12
+// update of the upper 6 x 6 tiles outside block row and column K (12
+// multiply-adds per entry; diagonal tiles whole).
 static double group_inv_flops(int m) {
-    const double n = 12.0 * m;
-    return n * (n * n + n);
+    const int nt = 2 * m;
+    double f = 0.0;
+    for (int kb = 0; kb < m; kb++) {
+        f += 12.0 * (144.0 * 3.0 + 1.0);
+        f += 12.0 * (12.0 * m) * 24.0;
+        for (int bi = 0; bi < nt; bi++)
+            for (int bj = bi; bj < nt; bj++)
+                if (bi / 2 != kb && bj / 2 != kb) f += 36.0 * 24.0;
+    }
+    return f;
 }
 
 template <class K, class V>
@@ -829,12 +882,12 @@ void assemble_system(World& w) {
             for (int g = 0; g < w.n_groups; g++) fl += group_inv_flops(w.h_grp_off[g + 1] - w.h_grp_off[g]);
             ps.flops = fl;
         }
-        static_assert(GMAX == 8, "size classes 5..8, 2..4, 1");
-        const int nb = w.n_glist[0] + (w.n_glist[1] + 3) / 4 + (w.n_glist[2] + 15) / 16;
+        static_assert(GMAX == 8, "size classes 5..8 and 1..4");
+        const int n4 = w.n_glist[1] + w.n_glist[2];
+        const int nb = w.n_glist[0] + (n4 + 3) / 4;
         if (nb > 0)
-            hipLaunchKernelGGL(k_group_inv, dim3(nb), dim3(256), 0, w.stream, w.glist.p, w.n_glist[0], w.n_glist[1],
-                               w.n_glist[2], w.grp_off.p, w.grp_body.p, w.grp_moff.p, w.row_ptr.p, w.col_idx.p, w.Hval.p,
-                               w.Ginv.p);
+            hipLaunchKernelGGL(k_group_inv, dim3(nb), dim3(256), 0, w.stream, w.glist.p, w.n_glist[0], n4, w.grp_off.p,
+                               w.grp_body.p, w.grp_moff.p, w.row_ptr.p, w.col_idx.p, w.Hval.p, w.Ginv.p);
         ADB_HIP(hipGetLastError());
     }
 }
