@@ -379,16 +379,24 @@ __global__ void k_pair_diag(PairArgs a, const double* __restrict__ part) {
         for (int u = 0; u < CH_U; u++) add_chunks(acc, part, c0[u], c1[u], off_low);
     }
     if (a.mass[b] > 0.0) {
-        // vertex-ground terms of this body: (w w^T) (x) K, gradient w (x) gx
-        for (int s = a.vg_start[b]; s < a.vg_start[b + 1]; s++) {
-            const int v = a.voff[b] + a.vg[s].y;
-            const double w4[4] = {1.0, a.xbar[3 * (size_t)v], a.xbar[3 * (size_t)v + 1], a.xbar[3 * (size_t)v + 2]};
-            const double* K = a.vgK + 12 * (size_t)s;
-            if (isg) acc += w4[ge / 3] * K[9 + ge % 3];
-            else {
-                const int cc = e / 12, rr = e % 12;
-                acc += w4[rr / 3] * w4[cc / 3] * K[3 * (cc % 3) + (rr % 3)];
+        // vertex-ground terms of this body: (w w^T) (x) K, gradient w (x) gx;
+        // the loads of CH_U contacts are issued together, the adds stay in order
+        const int cc = e / 12, rr = e % 12;
+        const int ka = isg ? ge / 3 : rr / 3, kb = isg ? 0 : cc / 3;
+        const int kk = isg ? 9 + ge % 3 : 3 * (cc % 3) + (rr % 3);
+        for (int s0 = a.vg_start[b], s1 = a.vg_start[b + 1]; s0 < s1; s0 += CH_U) {
+            double wa[CH_U], wb[CH_U], kv[CH_U];
+#pragma unroll
+            for (int u = 0; u < CH_U; u++) {
+                const int s = s0 + u < s1 ? s0 + u : s0;
+                const size_t v = (size_t)a.voff[b] + a.vg[s].y;
+                wa[u] = ka == 0 ? 1.0 : a.xbar[3 * v + ka - 1];
+                wb[u] = kb == 0 ? 1.0 : a.xbar[3 * v + kb - 1];
+                kv[u] = a.vgK[12 * (size_t)s + kk];
             }
+#pragma unroll
+            for (int u = 0; u < CH_U; u++)
+                if (s0 + u < s1) acc += isg ? wa[u] * kv[u] : wa[u] * wb[u] * kv[u];
         }
     }
     if (isg) a.grad[12 * (size_t)b + ge] = acc;

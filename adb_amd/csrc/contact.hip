@@ -69,6 +69,35 @@ constexpr double JACOBI_TOL = 1e-24;
 // such entries add at most 36e-28 |A|_F^2 to the off-diagonal sum, far below
 // the stop test, and late sweeps then cost little
 constexpr double JACOBI_SKIP = 1e-14;
+// Reciprocal and reciprocal square root from the hardware approximations,
+// each refined by two Newton steps (each step doubles the correct bits).
+// The Jacobi rotation parameters need no correct rounding, and this chain is
+// several times shorter than IEEE division and square root.
+__device__ __forceinline__ double rcp_r(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0); y = fma(y, e, y);
+    e = fma(-x, y, 1.0); return fma(y, e, y);
+}
+__device__ __forceinline__ double rsq_r(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    double e = fma(-h * y, y, 0.5); y = fma(y, e, y);
+    e = fma(-h * y, y, 0.5); return fma(y, e, y);
+}
+// Jacobi rotation annihilating a_pq (Golub and Van Loan, Algorithm 8.5.1):
+// tau = (a_qq - a_pp) / (2 a_pq), t = sign(tau) / (|tau| + sqrt(1 + tau^2)),
+// c = 1 / sqrt(1 + t^2), s = t c; after it a_pp' = a_pp - t a_pq, a_qq' = a_qq + t a_pq.
+__device__ __forceinline__ void jacobi_cs(double app, double aqq, double apq, double& c, double& s, double& t) {
+    const double tau = (aqq - app) * (0.5 * rcp_r(apq));
+    if (fabs(tau) > 1e150) {
+        t = 0.5 * rcp_r(tau);
+    } else {
+        const double x = fma(tau, tau, 1.0);
+        t = (tau >= 0 ? 1.0 : -1.0) * rcp_r(fma(x, rsq_r(x), fabs(tau)));      // x rsq(x) = sqrt(x)
+    }
+    c = rsq_r(fma(t, t, 1.0));
+    s = t * c;
+}
 
 struct GroupLDS {
     double H[12][LDH];
@@ -152,12 +181,8 @@ __device__ __forceinline__ void group_psd(GroupLDS& S, int lane, bool on) {
                     const double apq = S.A[p][q];
                     if (apq != 0.0) {
                         const double app = S.A[p][p], aqq = S.A[q][q];
-                        const double tau = (aqq - app) / (2.0 * apq);
                         double t;
-                        if (fabs(tau) > 1e150) t = 0.5 / tau;
-                        else t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                        c = 1.0 / sqrt(1.0 + t * t);
-                        s = t * c;
+                        jacobi_cs(app, aqq, apq, c, s, t);
                     }
                 }
                 S.cs[lane][0] = c; S.cs[lane][1] = s;
@@ -170,9 +195,9 @@ __device__ __forceinline__ void group_psd(GroupLDS& S, int lane, bool on) {
                     if (q < 9) {
                         const double c = S.cs[k][0], s = S.cs[k][1];
                         const double ap = S.A[lane][p], aq = S.A[lane][q];
-                        S.A[lane][p] = c * ap - s * aq; S.A[lane][q] = s * ap + c * aq;
+                        S.A[lane][p] = fma(c, ap, -s * aq); S.A[lane][q] = fma(s, ap, c * aq);
                         const double vp = S.V[lane][p], vq = S.V[lane][q];
-                        S.V[lane][p] = c * vp - s * vq; S.V[lane][q] = s * vp + c * vq;
+                        S.V[lane][p] = fma(c, vp, -s * vq); S.V[lane][q] = fma(s, vp, c * vq);
                     }
                 }
             }
@@ -184,7 +209,7 @@ __device__ __forceinline__ void group_psd(GroupLDS& S, int lane, bool on) {
                     if (q < 9) {
                         const double c = S.cs[k][0], s = S.cs[k][1];
                         const double ap = S.A[p][lane], aq = S.A[q][lane];
-                        S.A[p][lane] = c * ap - s * aq; S.A[q][lane] = s * ap + c * aq;
+                        S.A[p][lane] = fma(c, ap, -s * aq); S.A[q][lane] = fma(s, ap, c * aq);
                     }
                 }
             }
@@ -818,27 +843,33 @@ __device__ __forceinline__ void contact_small(const WorldArgs& a, const Stage& s
     grad_fields(nside0, w, gv, [&](int k, double v) { S[k - R_GRAD][l] = v; });
     stage_flush<24>(st, a.rec, REC, R_GRAD);
 }
-// Class 3 contacts, one thread each: the 9x9 Helmert-coordinate matrix
-// K = (C (x) I3) Mred (C^T (x) I3) + mollifier terms (C = h4 S^T, Cm the same
-// for y = p1 - p0, z = p3 - p2) is built one component pair at a time into
-// LDS (upper triangle only, packed element-major, X[e][lane], so lanes never
-// conflict; 23 KB per 64-lane workgroup lets four fit a CU's 160 KB, one wave
-// per SIMD, which the registers allow); cyclic Jacobi then
-// rotates its packed upper triangle in registers while the eigenvectors
-// accumulate there too.
+// Class 3 contacts (four points: point-plane, edge-edge interior, every
+// mollified edge-edge) in two launches.
+// k_contact_c3: ten threads per contact.  Thread role r < 9 builds the entries
+// of the 9x9 Helmert-coordinate matrix K = (C (x) I3) Mred (C^T (x) I3) +
+// mollifier terms (C = h4 S^T, Cm the same for y = p1 - p0, z = p3 - p2) of
+// one component pair (i, j) = (r / 3, r % 3) and stores them with their
+// mirror images (K exactly symmetric) in the record's PK field; role 9
+// writes the record's other fields (Z, friction, gradient, class code).
+// k_contact_psd9: one thread per contact projects K in place, cyclic Jacobi
+// on the packed upper triangle in registers with the eigenvectors beside it
+// (a scheduling barrier after each rotation keeps the live ranges to one
+// rotation).  Split this way the geometry's registers stay out of the Jacobi
+// kernel, and the matrix of a contact is built by nine threads at once.
 __host__ __device__ constexpr int P9(int p, int q) {
     return p <= q ? p * 9 - p * (p - 1) / 2 + (q - p) : q * 9 - q * (q - 1) / 2 + (p - q);
 }
-__device__ __forceinline__ void contact_c3(const WorldArgs& a, int cid, double (*V)[64], int lane) {
+constexpr int C3_ROLES = 10;
+__device__ __forceinline__ void contact_c3(const WorldArgs& a, int cid, int role) {
     double* R = a.rec + REC * (size_t)cid;
-    {
-        Geo G;
-        int id0, id1, id2, id3;
-        G.is_ee = contact_ids(a, cid, id0, id1, id2, id3);
-        G.p0 = ld3(a.X + 3 * (size_t)id0); G.p1 = ld3(a.X + 3 * (size_t)id1);
-        G.p2 = ld3(a.X + 3 * (size_t)id2); G.p3 = ld3(a.X + 3 * (size_t)id3);
-        G.type = a.ctype[cid]; G.d = a.cdist[cid];
-        geo_init(G, a.kp);
+    Geo G;
+    int id0, id1, id2, id3;
+    G.is_ee = contact_ids(a, cid, id0, id1, id2, id3);
+    G.p0 = ld3(a.X + 3 * (size_t)id0); G.p1 = ld3(a.X + 3 * (size_t)id1);
+    G.p2 = ld3(a.X + 3 * (size_t)id2); G.p3 = ld3(a.X + 3 * (size_t)id3);
+    G.type = a.ctype[cid]; G.d = a.cdist[cid];
+    geo_init(G, a.kp);
+    if (role < 9) {
         double C[3][3], Cm[3][2];
 #pragma unroll
         for (int q = 0; q < 3; q++) {
@@ -852,10 +883,8 @@ __device__ __forceinline__ void contact_c3(const WorldArgs& a, int cid, double (
             Cm[q][0] = kHel[q][1] - kHel[q][0];
             Cm[q][1] = kHel[q][3] - kHel[q][2];
         }
-        // K[(q, i), (b, j)] for one component pair (i, j) per iteration, into V (LDS)
-#pragma unroll 1
-        for (int ij = 0; ij < 9; ij++) {
-            const int i = ij / 3, j = ij % 3;
+        {
+            const int i = role / 3, j = role % 3;
             double K[3][3];
 #pragma unroll
             for (int q = 0; q < 3; q++)
@@ -904,51 +933,65 @@ __device__ __forceinline__ void contact_c3(const WorldArgs& a, int cid, double (
             for (int q = 0; q < 3; q++)
 #pragma unroll
                 for (int b = 0; b < 3; b++)
-                    if (3 * q + i <= 3 * b + j) V[P9(3 * q + i, 3 * b + j)][lane] = K[q][b];
+                    if (3 * q + i <= 3 * b + j) {
+                        // entry (3q + i, 3b + j) of the upper triangle and its mirror; the
+                        // lower-triangle entries of this pair are upper ones of pair (j, i)
+                        R[R_PK + 9 * (3 * q + i) + 3 * b + j] = K[q][b];
+                        R[R_PK + 9 * (3 * b + j) + 3 * q + i] = K[q][b];
+                    }
         }
-        // everything but PK: Z, friction (F, G, Tk, lambda) and the gradient
-        const int nside0 = G.is_ee ? 2 : 1;
-        double w[4][4];
-        load_w(a.xbar, id0, id1, id2, id3, w);
-#pragma unroll
-        for (int q = 0; q < 3; q++)
-#pragma unroll
-            for (int s = 0; s < 2; s++)
-#pragma unroll
-                for (int al = 0; al < 4; al++) {
-                    double zz = 0.0;
-#pragma unroll
-                    for (int k = 0; k < 4; k++)
-                        if ((k < nside0) == (s == 0)) zz += kHel[q][k] * w[k][al];
-                    R[R_Z + 8 * q + 4 * s + al] = zz;
-                }
-        Fric f;
-        friction_eval(a, G.is_ee, G.type, G.d, G.B1, G.p0, G.p1, G.p2, G.p3, id0, id1, id2, id3, f);
-        write_fric(a, cid, nside0, f, w);
-        const double gmv[4] = {f.gm0, f.gm1, f.gm2, f.gm3};
-        d3 gv[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            double g3[3];
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                double g = barrier_grad(G, 3 * k + c);
-                if (f.on) g += f.fs * (gmv[k] * cmp(f.c0, c) * f.u0 + gmv[k] * cmp(f.c1, c) * f.u1);
-                g3[c] = g;
-            }
-            gv[k] = mk(g3[0], g3[1], g3[2]);
-        }
-        write_grad(a, cid, nside0, w, gv);
-        R[R_D] = rec_code(a, cid, 9);
+        return;
     }
+    // everything but PK: Z, friction (F, G, Tk, lambda) and the gradient
+    const int nside0 = G.is_ee ? 2 : 1;
+    double w[4][4];
+    load_w(a.xbar, id0, id1, id2, id3, w);
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+            for (int al = 0; al < 4; al++) {
+                double zz = 0.0;
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if ((k < nside0) == (s == 0)) zz += kHel[q][k] * w[k][al];
+                R[R_Z + 8 * q + 4 * s + al] = zz;
+            }
+    Fric f;
+    friction_eval(a, G.is_ee, G.type, G.d, G.B1, G.p0, G.p1, G.p2, G.p3, id0, id1, id2, id3, f);
+    write_fric(a, cid, nside0, f, w);
+    const double gmv[4] = {f.gm0, f.gm1, f.gm2, f.gm3};
+    d3 gv[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        double g3[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double g = barrier_grad(G, 3 * k + c);
+            if (f.on) g += f.fs * (gmv[k] * cmp(f.c0, c) * f.u0 + gmv[k] * cmp(f.c1, c) * f.u1);
+            g3[c] = g;
+        }
+        gv[k] = mk(g3[0], g3[1], g3[2]);
+    }
+    write_grad(a, cid, nside0, w, gv);
+    R[R_D] = rec_code(a, cid, 9);
+}
+__global__ __launch_bounds__(256) void k_contact_c3(WorldArgs a) {
+    const long long n = (long long)C3_ROLES * a.counts[2];
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        contact_c3(a, a.list3[t / C3_ROLES], (int)(t % C3_ROLES));
+}
+__device__ __forceinline__ void psd9(const WorldArgs& a, int cid) {
+    double* R = a.rec + REC * (size_t)cid;
     // PK: packed K and V in registers (static indices); a scheduling barrier
     // after each rotation keeps the live ranges to one rotation
     double A[45];
 #pragma unroll
     for (int p = 0; p < 9; p++)
 #pragma unroll
-        for (int q = p; q < 9; q++) A[P9(p, q)] = V[P9(p, q)][lane];
-    if (a.kp.psd) {
+        for (int q = p; q < 9; q++) A[P9(p, q)] = R[R_PK + 9 * p + q];
+    {
         double Vr[81];
 #pragma unroll
         for (int e = 0; e < 81; e++) Vr[e] = (e % 10 == 0) ? 1.0 : 0.0;
@@ -970,24 +1013,23 @@ __device__ __forceinline__ void contact_c3(const WorldArgs& a, int cid, double (
                     const double apq = A[P9(p, q)];
                     if (fabs(apq) > thr) {
                         const double app = A[P9(p, p)], aqq = A[P9(q, q)];
-                        const double tau = (aqq - app) / (2.0 * apq);
-                        const double t = fabs(tau) > 1e150 ? 0.5 / tau : (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                        const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+                        double c, s, t;
+                        jacobi_cs(app, aqq, apq, c, s, t);
 #pragma unroll
                         for (int k = 0; k < 9; k++) {
                             if (k == p || k == q) continue;
                             const double akp = A[P9(k, p)], akq = A[P9(k, q)];
-                            A[P9(k, p)] = c * akp - s * akq;
-                            A[P9(k, q)] = s * akp + c * akq;
+                            A[P9(k, p)] = fma(c, akp, -s * akq);
+                            A[P9(k, q)] = fma(s, akp, c * akq);
                         }
-                        A[P9(p, p)] = c * c * app - 2.0 * c * s * apq + s * s * aqq;
-                        A[P9(q, q)] = s * s * app + 2.0 * c * s * apq + c * c * aqq;
+                        A[P9(p, p)] = fma(-t, apq, app);
+                        A[P9(q, q)] = fma(t, apq, aqq);
                         A[P9(p, q)] = 0.0;
 #pragma unroll
                         for (int k = 0; k < 9; k++) {
                             const double vkp = Vr[9 * k + p], vkq = Vr[9 * k + q];
-                            Vr[9 * k + p] = c * vkp - s * vkq;
-                            Vr[9 * k + q] = s * vkp + c * vkq;
+                            Vr[9 * k + p] = fma(c, vkp, -s * vkq);
+                            Vr[9 * k + q] = fma(s, vkp, c * vkq);
                         }
                     }
                     __builtin_amdgcn_sched_barrier(0);
@@ -1006,17 +1048,11 @@ __device__ __forceinline__ void contact_c3(const WorldArgs& a, int cid, double (
                 R[R_PK + 9 * p + q] = s;
                 R[R_PK + 9 * q + p] = s;
             }
-    } else {
-#pragma unroll
-        for (int p = 0; p < 9; p++)
-#pragma unroll
-            for (int q = p; q < 9; q++) { R[R_PK + 9 * p + q] = A[P9(p, q)]; R[R_PK + 9 * q + p] = A[P9(p, q)]; }
     }
 }
-__global__ __launch_bounds__(64) void k_contact_c3(WorldArgs a) {
-    __shared__ double V[45][64];
+__global__ __launch_bounds__(64) void k_contact_psd9(WorldArgs a) {
     const int n3 = a.counts[2];
-    for (int p = blockIdx.x * 64 + threadIdx.x; p < n3; p += gridDim.x * 64) contact_c3(a, a.list3[p], V, threadIdx.x);
+    for (int p = blockIdx.x * 64 + threadIdx.x; p < n3; p += gridDim.x * 64) psd9(a, a.list3[p]);
 }
 
 // record fields other than PK of a 3-point contact: D, friction, gradient, Z
@@ -1113,24 +1149,23 @@ __device__ __forceinline__ void jacobi_packed(double (&A)[N * (N + 1) / 2], doub
                 const double apq = A[PK_<N>(p, q)];
                 if (fabs(apq) > thr) {
                     const double app = A[PK_<N>(p, p)], aqq = A[PK_<N>(q, q)];
-                    const double tau = (aqq - app) / (2.0 * apq);
-                    const double t = fabs(tau) > 1e150 ? 0.5 / tau : (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                    const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+                    double c, s, t;
+                    jacobi_cs(app, aqq, apq, c, s, t);
 #pragma unroll
                     for (int k = 0; k < N; k++) {
                         if (k == p || k == q) continue;
                         const double akp = A[PK_<N>(k, p)], akq = A[PK_<N>(k, q)];
-                        A[PK_<N>(k, p)] = c * akp - s * akq;
-                        A[PK_<N>(k, q)] = s * akp + c * akq;
+                        A[PK_<N>(k, p)] = fma(c, akp, -s * akq);
+                        A[PK_<N>(k, q)] = fma(s, akp, c * akq);
                     }
-                    A[PK_<N>(p, p)] = c * c * app - 2.0 * c * s * apq + s * s * aqq;
-                    A[PK_<N>(q, q)] = s * s * app + 2.0 * c * s * apq + c * c * aqq;
+                    A[PK_<N>(p, p)] = fma(-t, apq, app);
+                    A[PK_<N>(q, q)] = fma(t, apq, aqq);
                     A[PK_<N>(p, q)] = 0.0;
 #pragma unroll
                     for (int k = 0; k < N; k++) {
                         const double vkp = V[N * k + p], vkq = V[N * k + q];
-                        V[N * k + p] = c * vkp - s * vkq;
-                        V[N * k + q] = s * vkp + c * vkq;
+                        V[N * k + p] = fma(c, vkp, -s * vkq);
+                        V[N * k + q] = fma(s, vkp, c * vkq);
                     }
                 }
                 __builtin_amdgcn_sched_barrier(0);
@@ -1342,8 +1377,12 @@ void contact_terms(World& w) {
             {
                 ProfScope ps(w, PK_CONTACT_GROUP, 0.0, w.stream2);
                 ps.counts(snap, 0.0, 0.0, 1608.0, 0.0);
-                const int grid = std::min((n + 63) / 64, 4 * w.num_cu);
-                hipLaunchKernelGGL(k_contact_c3, dim3(grid), dim3(64), 0, w.stream2, a);
+                // the class count is on the device: grids sized for the worst case, grid-stride loops
+                hipLaunchKernelGGL(k_contact_c3, dim3(std::min((C3_ROLES * n + 255) / 256, 16 * w.num_cu)), dim3(256), 0,
+                                   w.stream2, a);
+                ADB_HIP(hipGetLastError());
+                if (a.kp.psd)
+                    hipLaunchKernelGGL(k_contact_psd9, dim3(std::min((n + 63) / 64, 4 * w.num_cu)), dim3(64), 0, w.stream2, a);
                 ADB_HIP(hipGetLastError());
             }
             ADB_HIP(hipEventRecord(w.ev_side[1], w.stream2));

@@ -171,6 +171,10 @@ void read_slots(World& w, int k, double* out) {
     ADB_HIP(hipMemcpyAsync(out, w.epart.p, k * sizeof(double), hipMemcpyDeviceToHost, w.stream));
     ADB_HIP(hipStreamSynchronize(w.stream));
 }
+void copy_slots_async(World& w, int first, int k) {
+    if (!w.h_slots) ADB_HIP(hipHostMalloc((void**)&w.h_slots, NSLOT * sizeof(double), hipHostMallocDefault));
+    ADB_HIP(hipMemcpyAsync(w.h_slots + first, w.epart.p + first, k * sizeof(double), hipMemcpyDeviceToHost, w.stream));
+}
 
 void dot_async(World& w, const double* a, const double* b, int n, int slot) { reduce_async(w, a, b, n, slot); }
 

@@ -360,11 +360,36 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
     int it = 0, slot = 0;
     bool have_prev = false;
     const int max_it = std::max(w.P.pcg_max_iter, 0);
+    // stop test after the last queued update: the spmv of the next iteration
+    // tests it first and returns at once when stopped; if not, it does that
+    // iteration's spmv, which the next batch's first spmv repeats identically
+    // (same iteration, same p-buffer parity: batches are even)
+    auto stop_test = [&]() {
+        const double rtol2 = w.P.pcg_rtol * w.P.pcg_rtol;
+        double* partA = w.partial.p;
+        hipLaunchKernelGGL(k_pcg_spmv, dim3(nbS), dim3(64 * SP_ROWS), 0, w.stream, B, w.pcg_k.p, 0, rtol2, w.grp_n.p, w.row_ptr.p,
+                           w.col_idx.p, w.Hval.p, w.pcg_z.p, w.pcg_p.p, w.pcg_p2.p, w.pcg_Ap.p, partB, partA, w.scal.p,
+                           w.S > 1 ? (const int*)w.pcg_brun.p : (const int*)nullptr);
+        ADB_HIP(hipGetLastError());
+    };
     // Near the iteration count of the previous solve (Newton steps change the
     // system little) the batches shrink to 4 iterations, so the stop test,
     // read one batch late, leaves few queued iterations that return at once.
     const int pred = w.pcg_prev_its;
-    while (it < max_it) {
+    bool done = false;
+    if (pred > 0 && pred <= 4 && max_it > 0) {
+        // a short previous solve (the light phase of a scene, where groups are
+        // whole components): queue that many iterations (rounded up to even)
+        // and test at once, instead of queueing batches ahead of the test
+        const int nb0 = std::min(pred + (pred & 1), max_it);
+        enqueue_batch(w, nb0, w.prof.on);
+        it += nb0;
+        stop_test();
+        ADB_HIP(hipMemcpyAsync(w.h_scal, w.scal.p, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipStreamSynchronize(w.stream));
+        done = w.h_scal[S_STOP] != RUN;
+    }
+    while (!done && it < max_it) {
         int nb = std::min(BATCH, max_it - it);
         if (pred > 0 && it + BATCH > pred - 8) nb = std::min(nb, 4);
         if (use_graph && nb == BATCH) ADB_HIP(hipGraphLaunch(w.pcg_graph, w.stream));
@@ -381,17 +406,11 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
         have_prev = true;
         slot ^= 1;
     }
-    // stop test after the last queued update (returns at once when stopped earlier)
-    {
-        const double rtol2 = w.P.pcg_rtol * w.P.pcg_rtol;
-        double* partA = w.partial.p;
-        hipLaunchKernelGGL(k_pcg_spmv, dim3(nbS), dim3(64 * SP_ROWS), 0, w.stream, B, w.pcg_k.p, 0, rtol2, w.grp_n.p, w.row_ptr.p,
-                           w.col_idx.p, w.Hval.p, w.pcg_z.p, w.pcg_p.p, w.pcg_p2.p, w.pcg_Ap.p, partB, partA, w.scal.p,
-                           w.S > 1 ? (const int*)w.pcg_brun.p : (const int*)nullptr);
+    if (!done) {
+        stop_test();
+        ADB_HIP(hipMemcpyAsync(w.h_scal, w.scal.p, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipStreamSynchronize(w.stream));
     }
-    ADB_HIP(hipGetLastError());
-    ADB_HIP(hipMemcpyAsync(w.h_scal, w.scal.p, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, w.stream));
-    ADB_HIP(hipStreamSynchronize(w.stream));
     const double* h = w.h_scal;
     const int stop = (int)h[S_STOP];
     w.pcg_prev_its = stop == RUN ? it : (int)h[S_ITERS];

@@ -129,6 +129,11 @@ struct World {
     ContactSet cand;                 // line search: candidates of the search segment
     bool ls_cand = true;             // trial sets filtered from them (ADB_LS_CANDIDATES=0: full passes)
     int ls_prev = 2;                 // trials of the previous line search
+    // barrier + inertia energy at the point the last line search accepted (the
+    // next Newton iteration's q, positions and contact set), valid within a step
+    bool e0_valid = false;
+    double e0_bi = 0.0;
+    double* h_slots = nullptr;       // pinned host copies of reduction slots copied ahead of a wait
     DBuf<int> vsel;                  // trial sets: kept items per block and their scan
     DBuf<unsigned long long> keepbits;   // trial sets: test result bits
     // ---- contact terms (per PT/EE contact, then per VG contact)
@@ -280,6 +285,7 @@ void energy_barrier_inertia_async(World& w, const double* qv, const double* X, c
 void energy_friction_async(World& w, const double* X, int slot);                        // energy.hip
 void dot_async(World& w, const double* a, const double* b, int n, int slot);            // energy.hip
 void read_slots(World& w, int k, double* out);                                          // energy.hip
+void copy_slots_async(World& w, int first, int k);    // energy.hip: slots [first, first + k) into w.h_slots, no wait
 void download_sets(World& w, const ContactSet& cs, std::vector<int>& pt, std::vector<int>& ee, std::vector<int>& vg);
 
 } // namespace adb

@@ -54,14 +54,18 @@ static void scale(World& w, double* v, int n, double s) {
 // w.X1 and w.cs_new hold the accepted point q + alpha dq.
 static int line_search(World& w, double gn, double* alpha_out, int* trials_out, double* E0_out, double* E1_out) {
     const int n = 12 * w.B;
-    double sl[3];
-    energy_barrier_inertia_async(w, w.q.p, w.X.p, w.cs, 0);
-    energy_friction_async(w, w.X.p, 1);
+    double sl[7];
+    // E0 = barrier + inertia + friction at q, and dq . grad, queued now and read
+    // with the first trial's energies (one wait).  The barrier + inertia part is
+    // that of the point the previous line search accepted when there was one:
+    // the same q, positions and contact set, so the same value bit for bit.
+    // The friction part is recomputed: its lagged terms are this iteration's.
+    const bool have_bi = w.e0_valid;
+    if (!have_bi) energy_barrier_inertia_async(w, w.q.p, w.X.p, w.cs, 5);
+    energy_friction_async(w, w.X.p, 6);
     dot_async(w, w.dq.p, w.grad.p, n, 2);
-    read_slots(w, 3, sl);
-    const double E0 = sl[0] + sl[1];
-    const double qdg = sl[2];
-    double alpha = 1.0, E1 = E0;
+    double E0 = 0.0, qdg = 0.0, E1 = 0.0, alpha = 1.0, e1_bi = 0.0;
+    bool have_e0 = false;
     int trials = 0, status = ADB_OK;
     // Trial sets from the candidates of the search segment q -> q + alpha dq
     // (vertex positions are affine in q, so every later trial point lies on
@@ -87,7 +91,13 @@ static int line_search(World& w, double gn, double* alpha_out, int* trials_out, 
         else collision_set(w, w.X1.p, w.cs_new);
         energy_barrier_inertia_async(w, w.qtrial.p, w.X1.p, w.cs_new, 0);
         energy_friction_async(w, w.X1.p, 1);
-        read_slots(w, 2, sl);
+        read_slots(w, have_e0 ? 2 : 7, sl);
+        if (!have_e0) {
+            E0 = (have_bi ? w.e0_bi : sl[5]) + sl[6];
+            qdg = sl[2];
+            have_e0 = true;
+        }
+        e1_bi = sl[0];
         E1 = sl[0] + sl[1];
         const bool wolfe = E1 <= E0 + w.P.c1 * alpha * qdg;
         alpha /= 2;
@@ -96,6 +106,8 @@ static int line_search(World& w, double gn, double* alpha_out, int* trials_out, 
         if (trials >= w.P.max_ls_iter) { status = ADB_E_NOT_CONVERGED; w.err = "line search exhausted"; break; }
     }
     w.ls_prev = trials;
+    w.e0_valid = status == ADB_OK;
+    w.e0_bi = e1_bi;
     *alpha_out = alpha * 2; *trials_out = trials; *E0_out = E0; *E1_out = E1;
     return status;
 }
@@ -110,6 +122,7 @@ static int step(World& w, adb_step_stats* st) {
     transform_positions(w, w.q.p, w.X.p);
     transform_positions(w, w.q0.p, w.Xs.p);
     collision_set(w, w.X.p, w.cs);
+    w.e0_valid = false;
     int status = ADB_OK;
     bool converged = false;
     for (int iter = 0; iter < w.P.max_iter; iter++) {
@@ -123,11 +136,24 @@ static int step(World& w, adb_step_stats* st) {
         int pit = 0;
         const int prc = pcg_solve(w, w.dq.p, &pit);
         S.pcg_iters += pit;
-        // |dq|^2 and |g|^2 with one wait
-        double sl[3];
-        dot_async(w, w.dq.p, w.dq.p, n, 0);
-        dot_async(w, w.grad.p, w.grad.p, n, 1);
-        read_slots(w, 2, sl);
+        const double t2 = now_s();
+        S.t_solver += t2 - t1;
+        // |dq|^2 and |g|^2 (slots 3, 4), copied to the host ahead of the CCD
+        // bound's wait and read after it (one wait for both)
+        dot_async(w, w.dq.p, w.dq.p, n, 3);
+        dot_async(w, w.grad.p, w.grad.p, n, 4);
+        copy_slots_async(w, 3, 2);
+        double toi = 1.0, factor = 1.0;
+        int cs_status = ADB_OK;
+        if (w.P.upper_bound) {
+            // CCD step bound (model/abd.cpp:142-196)
+            q_axpy(w, w.q.p, w.dq.p, 1.0, w.qtrial.p);
+            transform_positions(w, w.qtrial.p, w.X1.p);
+            toi = ccd_toi(w, w.X.p, w.X1.p, &cs_status);
+        } else {
+            ADB_HIP(hipStreamSynchronize(w.stream));
+        }
+        const double sl[2] = {w.h_slots[3], w.h_slots[4]};
         // a solve that stopped early is counted, not hidden: the direction is
         // still the last PCG iterate and the line search still guards descent
         if (prc == ADB_E_NOT_CONVERGED) {
@@ -140,18 +166,8 @@ static int step(World& w, adb_step_stats* st) {
         const double nrm2 = prc == ADB_E_NAN ? NAN : sl[0];
         const double gn = std::sqrt(sl[1]);
         const double sup_dq = std::sqrt(nrm2);
-        const double t2 = now_s();
-        S.t_solver += t2 - t1;
         if (std::isnan(sup_dq)) { status = ADB_E_NAN; w.err = "solver nan (model/abd.cpp:288-291)"; break; }
-        // CCD step bound (model/abd.cpp:142-196)
-        double toi = 1.0, factor = 1.0;
-        if (w.P.upper_bound) {
-            q_axpy(w, w.q.p, w.dq.p, 1.0, w.qtrial.p);
-            transform_positions(w, w.qtrial.p, w.X1.p);
-            int cs_status = ADB_OK;
-            toi = ccd_toi(w, w.X.p, w.X1.p, &cs_status);
-            if (cs_status != ADB_OK) { status = cs_status; w.err = "toi < 1e-6 twice in a row (model/iaabb.cpp:305-311,533-537)"; break; }
-        }
+        if (cs_status != ADB_OK) { status = cs_status; w.err = "toi < 1e-6 twice in a row (model/iaabb.cpp:305-311,533-537)"; break; }
         if (toi < 1.0) factor = w.P.backoff;
         S.toi_min = std::min(S.toi_min, toi);
         scale(w, w.dq.p, n, factor * toi);
@@ -169,6 +185,7 @@ static int step(World& w, adb_step_stats* st) {
             q_axpy(w, w.q.p, w.dq.p, 1.0, w.qtrial.p);
             transform_positions(w, w.qtrial.p, w.X1.p);
             collision_set(w, w.X1.p, w.cs_new);
+            w.e0_valid = false;
         }
         // q += alpha dq is exactly the last trial's q + dq * alpha
         std::swap(w.q, w.qtrial);
@@ -541,6 +558,8 @@ static void free_world(World& w) {
     w.grp_n.free_();
     if (w.h_scal) (void)hipHostFree(w.h_scal);
     w.h_scal = nullptr;
+    if (w.h_slots) (void)hipHostFree(w.h_slots);
+    w.h_slots = nullptr;
     if (w.h_pairs) (void)hipHostFree(w.h_pairs);
     w.h_pairs = nullptr; w.h_pairs_cap = 0;
     if (w.ev_pairs) (void)hipEventDestroy(w.ev_pairs);
@@ -835,7 +854,9 @@ extern "C" int adb_line_search(adb_world* W, const double* q, const double* dq, 
         read_slots(w, 1, sl);
         double a = 1.0, e0 = 0.0, e1 = 0.0;
         int t = 0;
+        w.e0_valid = false;
         const int rc = line_search(w, std::sqrt(sl[0]), &a, &t, &e0, &e1);
+        w.e0_valid = false;
         if (alpha) *alpha = a;
         if (E0) *E0 = e0;
         if (E1) *E1 = e1;
