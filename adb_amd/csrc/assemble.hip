@@ -18,7 +18,9 @@
 // Static bodies (mass < 0) follow the reference: contributions to them are
 // dropped, their diagonal block is I and their gradient 0.
 #include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 #include <algorithm>
+#include <cstring>
 #include <vector>
 #include "dmath.h"
 #include "world.h"
@@ -733,14 +735,37 @@ static void build_groups(World& w) {
     w.n_glist[1] = (int)w.h_glist.size() - w.n_glist[0];
     for (int g = 0; g < w.n_groups; g++) if (w.h_grp_off[g + 1] - w.h_grp_off[g] == 1) w.h_glist.push_back(g);
     w.n_glist[2] = (int)w.h_glist.size() - w.n_glist[0] - w.n_glist[1];
-    w.glist.reserve(std::max(w.n_groups, 1));
-    ADB_HIP(hipMemcpyAsync(w.glist.p, w.h_glist.data(), w.n_groups * sizeof(int), hipMemcpyHostToDevice, w.stream));
-    w.grp_off.reserve(w.n_groups + 1); w.grp_body.reserve(B); w.grp_moff.reserve(w.n_groups + 1); w.grp_n.reserve(1);
-    ADB_HIP(hipMemcpyAsync(w.grp_n.p, &w.n_groups, sizeof(int), hipMemcpyHostToDevice, w.stream));
-    ADB_HIP(hipMemcpyAsync(w.grp_off.p, w.h_grp_off.data(), (w.n_groups + 1) * sizeof(int), hipMemcpyHostToDevice, w.stream));
-    ADB_HIP(hipMemcpyAsync(w.grp_body.p, w.h_grp_body.data(), B * sizeof(int), hipMemcpyHostToDevice, w.stream));
-    ADB_HIP(hipMemcpyAsync(w.grp_moff.p, w.h_grp_moff.data(), (w.n_groups + 1) * sizeof(long long), hipMemcpyHostToDevice,
-                           w.stream));
+    // one upload of all group tables (pinned staging; the previous upload from
+    // it has completed before it is rewritten).  Offsets depend on B only (at
+    // most B groups), so the tables' device addresses stay the same from one
+    // Newton iteration to the next (the captured PCG batch is keyed on them).
+    const int G = w.n_groups;
+    const size_t nll = (size_t)B + 1, nint = (size_t)(B + 1) + B + B + 1, total = nll + (nint + 1) / 2;
+    if (!w.ev_grp) ADB_HIP(hipEventCreateWithFlags(&w.ev_grp, hipEventDisableTiming));
+    ADB_HIP(hipEventSynchronize(w.ev_grp));
+    if (total > w.h_grp_pack_cap) {
+        if (w.h_grp_pack) ADB_HIP(hipHostFree(w.h_grp_pack));
+        w.h_grp_pack_cap = total + total / 2 + 64;
+        ADB_HIP(hipHostMalloc((void**)&w.h_grp_pack, w.h_grp_pack_cap * sizeof(long long), hipHostMallocDefault));
+        std::memset(w.h_grp_pack, 0, w.h_grp_pack_cap * sizeof(long long));
+    }
+    w.grp_pack.reserve(total);
+    {
+        long long* hm = w.h_grp_pack;
+        int* hi = reinterpret_cast<int*>(w.h_grp_pack + nll);
+        std::memcpy(hm, w.h_grp_moff.data(), (size_t)(G + 1) * sizeof(long long));
+        std::memcpy(hi, w.h_grp_off.data(), (size_t)(G + 1) * sizeof(int));
+        std::memcpy(hi + B + 1, w.h_grp_body.data(), (size_t)B * sizeof(int));
+        std::memcpy(hi + 2 * B + 1, w.h_glist.data(), (size_t)G * sizeof(int));
+        hi[3 * B + 1] = G;
+    }
+    w.grp_moff = w.grp_pack.p;
+    w.grp_off = reinterpret_cast<int*>(w.grp_pack.p + nll);
+    w.grp_body = w.grp_off + B + 1;
+    w.glist = w.grp_body + B;
+    w.grp_n = w.glist + B;
+    ADB_HIP(hipMemcpyAsync(w.grp_pack.p, w.h_grp_pack, total * sizeof(long long), hipMemcpyHostToDevice, w.stream));
+    ADB_HIP(hipEventRecord(w.ev_grp, w.stream));
     // the host vectors stay alive (members of w) until the copies have run
     w.Ginv.reserve((size_t)w.h_grp_moff.back());
 }
@@ -767,9 +792,15 @@ static double group_inv_flops(int m) {
 template <class K, class V>
 static void sort_pairs(World& w, K* kin, K* kout, V* vin, V* vout, int n, int end_bit) {
     size_t tmp = 0;
-    ADB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kin, kout, vin, vout, n, 0, end_bit, w.stream));
+    // rocPRIM's radix sort with the merge-sort limit at 0: onesweep passes
+    // (8-bit digits, ceil(end_bit / 8) of them) at every size.  Below its default
+    // limit of 2^20 keys rocPRIM sorts blocks and merges them, log2(n / block)
+    // passes of two kernels each (16 launches, ~80 us for 2e5 contacts, against
+    // ~5 launches here); both sorts are stable, so the order is the same.
+    using OneSweep = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config, rocprim::default_config, 0>;
+    ADB_HIP(rocprim::radix_sort_pairs<OneSweep>(nullptr, tmp, kin, kout, vin, vout, n, 0, end_bit, w.stream));
     w.sort_tmp.reserve(tmp + 16);
-    ADB_HIP(hipcub::DeviceRadixSort::SortPairs(w.sort_tmp.p, tmp, kin, kout, vin, vout, n, 0, end_bit, w.stream));
+    ADB_HIP(rocprim::radix_sort_pairs<OneSweep>(w.sort_tmp.p, tmp, kin, kout, vin, vout, n, 0, end_bit, w.stream));
 }
 
 void assemble_system(World& w) {
@@ -894,8 +925,8 @@ void assemble_system(World& w) {
         const int n4 = w.n_glist[1] + w.n_glist[2];
         const int nb = w.n_glist[0] + (n4 + 3) / 4;
         if (nb > 0)
-            hipLaunchKernelGGL(k_group_inv, dim3(nb), dim3(256), 0, w.stream, w.glist.p, w.n_glist[0], n4, w.grp_off.p,
-                               w.grp_body.p, w.grp_moff.p, w.row_ptr.p, w.col_idx.p, w.Hval.p, w.Ginv.p);
+            hipLaunchKernelGGL(k_group_inv, dim3(nb), dim3(256), 0, w.stream, w.glist, w.n_glist[0], n4, w.grp_off,
+                               w.grp_body, w.grp_moff, w.row_ptr.p, w.col_idx.p, w.Hval.p, w.Ginv.p);
         ADB_HIP(hipGetLastError());
     }
 }

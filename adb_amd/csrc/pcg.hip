@@ -277,7 +277,7 @@ __global__ __launch_bounds__(UPD_T) void k_pcg_update(int B, const int* __restri
 
 static Groups groups_of(World& w) {
     Groups G;
-    G.off = w.grp_off.p; G.body = w.grp_body.p; G.moff = w.grp_moff.p; G.Minv = w.Ginv.p; G.count = w.grp_n.p;
+    G.off = w.grp_off; G.body = w.grp_body; G.moff = w.grp_moff; G.Minv = w.Ginv.p; G.count = w.grp_n;
     return G;
 }
 
@@ -297,7 +297,7 @@ static void enqueue_batch(World& w, int nb, bool prof) {
         {
             // H blocks, their column indices, z and p_old read, p_new and y written
             ProfScope ps(w, PK_SPMV, prof ? 1152.0 * w.nnzb + 4.0 * w.nnzb + 4.0 * 96.0 * B : 0.0);
-            hipLaunchKernelGGL(k_pcg_spmv, dim3(nbS), dim3(64 * SP_ROWS), 0, w.stream, B, w.pcg_k.p, k, rtol2, w.grp_n.p, w.row_ptr.p,
+            hipLaunchKernelGGL(k_pcg_spmv, dim3(nbS), dim3(64 * SP_ROWS), 0, w.stream, B, w.pcg_k.p, k, rtol2, w.grp_n, w.row_ptr.p,
                                w.col_idx.p, w.Hval.p, w.pcg_z.p, pb[k & 1], pb[(k & 1) ^ 1], w.pcg_Ap.p, partB, partA, w.scal.p,
                                w.S > 1 ? (const int*)w.pcg_brun.p : (const int*)nullptr);
         }
@@ -338,8 +338,8 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
     const bool use_graph = !w.prof.on && w.use_graphs;
     if (use_graph) {
         const void* key[] = {w.row_ptr.p, w.col_idx.p, w.Hval.p, w.Ginv.p, w.pcg_x.p, w.pcg_r.p, w.pcg_z.p, w.pcg_p.p,
-                             w.pcg_p2.p, w.pcg_Ap.p, w.partial.p, w.scal.p, w.pcg_k.p, w.grp_off.p, w.grp_body.p, w.grp_moff.p,
-                             w.grp_n.p, w.sc_goff.p, w.pcg_sbb.p, w.pcg_brun.p};
+                             w.pcg_p2.p, w.pcg_Ap.p, w.partial.p, w.scal.p, w.pcg_k.p, w.grp_off, w.grp_body, w.grp_moff,
+                             w.grp_n, w.sc_goff.p, w.pcg_sbb.p, w.pcg_brun.p};
         static_assert(sizeof key / sizeof key[0] == World::PCG_GRAPH_KEYS, "graph key size");
         const int BATCH = w.tune.pcg_batch;
         bool same = w.pcg_graph != nullptr && w.pcg_graph_batch == BATCH;
@@ -367,7 +367,7 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
     auto stop_test = [&]() {
         const double rtol2 = w.P.pcg_rtol * w.P.pcg_rtol;
         double* partA = w.partial.p;
-        hipLaunchKernelGGL(k_pcg_spmv, dim3(nbS), dim3(64 * SP_ROWS), 0, w.stream, B, w.pcg_k.p, 0, rtol2, w.grp_n.p, w.row_ptr.p,
+        hipLaunchKernelGGL(k_pcg_spmv, dim3(nbS), dim3(64 * SP_ROWS), 0, w.stream, B, w.pcg_k.p, 0, rtol2, w.grp_n, w.row_ptr.p,
                            w.col_idx.p, w.Hval.p, w.pcg_z.p, w.pcg_p.p, w.pcg_p2.p, w.pcg_Ap.p, partB, partA, w.scal.p,
                            w.S > 1 ? (const int*)w.pcg_brun.p : (const int*)nullptr);
         ADB_HIP(hipGetLastError());

@@ -168,12 +168,18 @@ struct World {
     // preconditioner groups: heavy-edge aggregates of the contact graph of at
     // most GMAX bodies; per group the dense inverse
     // of its diagonal sub-matrix (n x n, n = 12 bodies, column-major)
-    DBuf<int> grp_off, grp_body;     // group g owns grp_body[grp_off[g], grp_off[g + 1])
-    DBuf<long long> grp_moff;        // offset of group g's inverse in Ginv
-    DBuf<int> grp_n;                 // number of groups (device copy)
+    // the group tables live in one device buffer, uploaded with one copy from a
+    // pinned staging buffer: grp_moff (offset of group g's inverse in Ginv),
+    // grp_off / grp_body (group g owns grp_body[grp_off[g], grp_off[g + 1])),
+    // glist (group ids in launch order: multi-body groups first), grp_n (count)
+    DBuf<long long> grp_pack;
+    long long* h_grp_pack = nullptr;
+    size_t h_grp_pack_cap = 0;
+    hipEvent_t ev_grp = nullptr;     // the last upload from the staging buffer
+    long long* grp_moff = nullptr;
+    int *grp_off = nullptr, *grp_body = nullptr, *glist = nullptr, *grp_n = nullptr;
     DBuf<double> Ginv;
     int n_groups = 0;
-    DBuf<int> glist;                 // group ids in launch order: multi-body groups first
     std::vector<int> h_glist;
     int n_glist[3] = {0, 0, 0};      // groups of 5..8, 2..4 and 1 bodies, in that order in glist
     std::vector<int> h_grp_off, h_grp_body;

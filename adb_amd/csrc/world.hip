@@ -529,8 +529,7 @@ static void free_world(World& w) {
                        &w.aval, &w.aval_s, &w.blk_of, &w.uniq_flag, &w.row_ptr, &w.col_idx, &w.diag_blk, &w.vg_start,
                        &w.pair_start, &w.pair_lo, &w.pair_hi, &w.pair_up, &w.pair_low, &w.tval, &w.tval_s,
                        &w.up_start, &w.low_start, &w.rowcnt,
-                       &w.ctype, &w.ccls, &w.clist, &w.clist3, &w.ccount, &w.chunk_n, &w.chunk_off, &w.chunk_blk,
-                       &w.grp_off, &w.grp_body, &w.glist};
+                       &w.ctype, &w.ccls, &w.clist, &w.clist3, &w.ccount, &w.chunk_n, &w.chunk_off, &w.chunk_blk};
     for (auto* b : ib) b->free_();
     DBuf<double>* db[] = {&w.xbar, &w.mass, &w.Ic, &w.q, &w.q0, &w.qdot, &w.dq, &w.qtrial, &w.X, &w.Xs, &w.X1, &w.bbox, &w.bsph,
                           &w.crec, &w.cdist, &w.chunk_part, &w.ctk, &w.clam, &w.vgK, &w.vglam, &w.bodyH, &w.bodyG, &w.Hval, &w.grad,
@@ -554,8 +553,12 @@ static void free_world(World& w) {
     if (w.pcg_graph) (void)hipGraphExecDestroy(w.pcg_graph);
     w.pcg_graph = nullptr;
     w.pcg_k.free_();
-    w.grp_moff.free_();
-    w.grp_n.free_();
+    w.grp_pack.free_();
+    w.grp_moff = nullptr; w.grp_off = w.grp_body = w.glist = w.grp_n = nullptr;
+    if (w.h_grp_pack) (void)hipHostFree(w.h_grp_pack);
+    w.h_grp_pack = nullptr; w.h_grp_pack_cap = 0;
+    if (w.ev_grp) (void)hipEventDestroy(w.ev_grp);
+    w.ev_grp = nullptr;
     if (w.h_scal) (void)hipHostFree(w.h_scal);
     w.h_scal = nullptr;
     if (w.h_slots) (void)hipHostFree(w.h_slots);
