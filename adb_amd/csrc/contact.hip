@@ -13,8 +13,10 @@
 //   class 1 (two points, D = 3): closed-form projection (k_contact_small<1>);
 //   class 2 (three points, D = 6): record fields (k_contact_scalars2), then
 //     cyclic Jacobi on the packed 6 x 6 matrix in registers (k_contact_pk2);
-//   class 3 (four points, or any mollified EE, D = 9): 9 x 9 matrix staged
-//     through LDS, cyclic Jacobi in registers (k_contact_c3, on a side stream).
+//   class 3 (four points, or any mollified EE, D = 9): the 9 x 9 matrix into
+//     the record, then cyclic Jacobi in registers (k_contact_c3 and
+//     k_contact_psd9, on a side stream; nine threads per contact build the
+//     matrix while the class is small).
 // Each contact writes a compact body-space record (world.h) that the
 // assembly expands into its four 12 x 12 blocks; the class 1 and class 2
 // record kernels stage it per wavefront in LDS and store it cooperatively
@@ -860,89 +862,107 @@ __host__ __device__ constexpr int P9(int p, int q) {
     return p <= q ? p * 9 - p * (p - 1) / 2 + (q - p) : q * 9 - q * (q - 1) / 2 + (p - q);
 }
 constexpr int C3_ROLES = 10;
-__device__ __forceinline__ void contact_c3(const WorldArgs& a, int cid, int role) {
-    double* R = a.rec + REC * (size_t)cid;
+// below this many 4-point contacts the matrix is built by nine threads per
+// contact (ten with the record fields) and projected by k_contact_psd9: few
+// contacts leave most of the GPU idle, and the dependent chain per thread is
+// what counts.  At or above it, one thread per contact does everything (no
+// geometry recomputed per thread: the GPU is full and the total work counts).
+// Both paths compute every value with the same operations: identical records.
+constexpr int C3_SPLIT_MAX = 32768;
+struct C3Ctx {
     Geo G;
     int id0, id1, id2, id3;
-    G.is_ee = contact_ids(a, cid, id0, id1, id2, id3);
-    G.p0 = ld3(a.X + 3 * (size_t)id0); G.p1 = ld3(a.X + 3 * (size_t)id1);
-    G.p2 = ld3(a.X + 3 * (size_t)id2); G.p3 = ld3(a.X + 3 * (size_t)id3);
+    double* R;
+};
+__device__ __forceinline__ void c3_setup(const WorldArgs& a, int cid, C3Ctx& x) {
+    x.R = a.rec + REC * (size_t)cid;
+    Geo& G = x.G;
+    G.is_ee = contact_ids(a, cid, x.id0, x.id1, x.id2, x.id3);
+    G.p0 = ld3(a.X + 3 * (size_t)x.id0); G.p1 = ld3(a.X + 3 * (size_t)x.id1);
+    G.p2 = ld3(a.X + 3 * (size_t)x.id2); G.p3 = ld3(a.X + 3 * (size_t)x.id3);
     G.type = a.ctype[cid]; G.d = a.cdist[cid];
     geo_init(G, a.kp);
-    if (role < 9) {
-        double C[3][3], Cm[3][2];
+}
+// the entries of K for component pair role = (i, j), into the PK field
+__device__ __forceinline__ void c3_kbuild(const C3Ctx& x, int role) {
+    const Geo& G = x.G;
+    double* R = x.R;
+    double C[3][3], Cm[3][2];
 #pragma unroll
-        for (int q = 0; q < 3; q++) {
+    for (int q = 0; q < 3; q++) {
 #pragma unroll
-            for (int m = 0; m < 3; m++) {
-                double c = 0.0;
+        for (int m = 0; m < 3; m++) {
+            double c = 0.0;
 #pragma unroll
-                for (int k = 0; k < 4; k++) c += kHel[q][k] * red_sign(G.map, m, k);
-                C[q][m] = m < G.map.nr ? c : 0.0;
-            }
-            Cm[q][0] = kHel[q][1] - kHel[q][0];
-            Cm[q][1] = kHel[q][3] - kHel[q][2];
+            for (int k = 0; k < 4; k++) c += kHel[q][k] * red_sign(G.map, m, k);
+            C[q][m] = m < G.map.nr ? c : 0.0;
         }
-        {
-            const int i = role / 3, j = role % 3;
-            double K[3][3];
-#pragma unroll
-            for (int q = 0; q < 3; q++)
-#pragma unroll
-                for (int b = 0; b < 3; b++) K[q][b] = 0.0;
-#pragma unroll
-            for (int m = 0; m < 3; m++)
-#pragma unroll
-                for (int n = 0; n < 3; n++) {
-                    if (m < G.map.nr && n < G.map.nr) {
-                        const double Mv = G.B1 * G.rd.hess(m, i, n, j) + G.B2 * cmp(G.rd.g[m], i) * cmp(G.rd.g[n], j);
-#pragma unroll
-                        for (int q = 0; q < 3; q++)
-#pragma unroll
-                            for (int b = 0; b < 3; b++) K[q][b] += C[q][m] * C[b][n] * Mv;
-                    }
-                }
-            if (G.is_ee) {
-                double gdi[3], gdj[3], gxi[3], gxj[3];
-#pragma unroll
-                for (int q = 0; q < 3; q++) {
-                    double si = 0.0, sj = 0.0;
-#pragma unroll
-                    for (int m = 0; m < 3; m++)
-                        if (m < G.map.nr) { si += C[q][m] * cmp(G.rd.g[m], i); sj += C[q][m] * cmp(G.rd.g[m], j); }
-                    gdi[q] = si; gdj[q] = sj;
-                    gxi[q] = G.m2 != 0.0 ? Cm[q][0] * G.mc.grad(0, i) + Cm[q][1] * G.mc.grad(1, i) : 0.0;
-                    gxj[q] = G.m2 != 0.0 ? Cm[q][0] * G.mc.grad(0, j) + Cm[q][1] * G.mc.grad(1, j) : 0.0;
-                }
-#pragma unroll
-                for (int q = 0; q < 3; q++)
-#pragma unroll
-                    for (int b = 0; b < 3; b++) {
-                        double hxk = 0.0;
-                        if (G.m2 != 0.0) {
-#pragma unroll
-                            for (int u = 0; u < 2; u++)
-#pragma unroll
-                                for (int v = 0; v < 2; v++) hxk += Cm[q][u] * Cm[b][v] * G.mc.hess(u, i, v, j);
-                        }
-                        const double gmi = G.m1 * gxi[q], gmj = G.m1 * gxj[b];
-                        K[q][b] = G.m * K[q][b] + G.B1 * (gmi * gdj[b] + gdi[q] * gmj) + G.B0 * (G.m2 * gxi[q] * gxj[b] + G.m1 * hxk);
-                    }
-            }
-#pragma unroll
-            for (int q = 0; q < 3; q++)
-#pragma unroll
-                for (int b = 0; b < 3; b++)
-                    if (3 * q + i <= 3 * b + j) {
-                        // entry (3q + i, 3b + j) of the upper triangle and its mirror; the
-                        // lower-triangle entries of this pair are upper ones of pair (j, i)
-                        R[R_PK + 9 * (3 * q + i) + 3 * b + j] = K[q][b];
-                        R[R_PK + 9 * (3 * b + j) + 3 * q + i] = K[q][b];
-                    }
-        }
-        return;
+        Cm[q][0] = kHel[q][1] - kHel[q][0];
+        Cm[q][1] = kHel[q][3] - kHel[q][2];
     }
-    // everything but PK: Z, friction (F, G, Tk, lambda) and the gradient
+    {
+        const int i = role / 3, j = role % 3;
+        double K[3][3];
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) K[q][b] = 0.0;
+#pragma unroll
+        for (int m = 0; m < 3; m++)
+#pragma unroll
+            for (int n = 0; n < 3; n++) {
+                if (m < G.map.nr && n < G.map.nr) {
+                    const double Mv = G.B1 * G.rd.hess(m, i, n, j) + G.B2 * cmp(G.rd.g[m], i) * cmp(G.rd.g[n], j);
+#pragma unroll
+                    for (int q = 0; q < 3; q++)
+#pragma unroll
+                        for (int b = 0; b < 3; b++) K[q][b] += C[q][m] * C[b][n] * Mv;
+                }
+            }
+        if (G.is_ee) {
+            double gdi[3], gdj[3], gxi[3], gxj[3];
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                double si = 0.0, sj = 0.0;
+#pragma unroll
+                for (int m = 0; m < 3; m++)
+                    if (m < G.map.nr) { si += C[q][m] * cmp(G.rd.g[m], i); sj += C[q][m] * cmp(G.rd.g[m], j); }
+                gdi[q] = si; gdj[q] = sj;
+                gxi[q] = G.m2 != 0.0 ? Cm[q][0] * G.mc.grad(0, i) + Cm[q][1] * G.mc.grad(1, i) : 0.0;
+                gxj[q] = G.m2 != 0.0 ? Cm[q][0] * G.mc.grad(0, j) + Cm[q][1] * G.mc.grad(1, j) : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) {
+                    double hxk = 0.0;
+                    if (G.m2 != 0.0) {
+#pragma unroll
+                        for (int u = 0; u < 2; u++)
+#pragma unroll
+                            for (int v = 0; v < 2; v++) hxk += Cm[q][u] * Cm[b][v] * G.mc.hess(u, i, v, j);
+                    }
+                    const double gmi = G.m1 * gxi[q], gmj = G.m1 * gxj[b];
+                    K[q][b] = G.m * K[q][b] + G.B1 * (gmi * gdj[b] + gdi[q] * gmj) + G.B0 * (G.m2 * gxi[q] * gxj[b] + G.m1 * hxk);
+                }
+        }
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+#pragma unroll
+            for (int b = 0; b < 3; b++)
+                if (3 * q + i <= 3 * b + j) {
+                    // entry (3q + i, 3b + j) of the upper triangle and its mirror; the
+                    // lower-triangle entries of this pair are upper ones of pair (j, i)
+                    R[R_PK + 9 * (3 * q + i) + 3 * b + j] = K[q][b];
+                    R[R_PK + 9 * (3 * b + j) + 3 * q + i] = K[q][b];
+                }
+    }
+}
+// everything but PK: Z, friction (F, G, Tk, lambda), the gradient, the class code
+__device__ __forceinline__ void c3_fields(const WorldArgs& a, int cid, const C3Ctx& x) {
+    const Geo& G = x.G;
+    double* R = x.R;
+    const int id0 = x.id0, id1 = x.id1, id2 = x.id2, id3 = x.id3;
     const int nside0 = G.is_ee ? 2 : 1;
     double w[4][4];
     load_w(a.xbar, id0, id1, id2, id3, w);
@@ -976,11 +996,6 @@ __device__ __forceinline__ void contact_c3(const WorldArgs& a, int cid, int role
     }
     write_grad(a, cid, nside0, w, gv);
     R[R_D] = rec_code(a, cid, 9);
-}
-__global__ __launch_bounds__(256) void k_contact_c3(WorldArgs a) {
-    const long long n = (long long)C3_ROLES * a.counts[2];
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
-        contact_c3(a, a.list3[t / C3_ROLES], (int)(t % C3_ROLES));
 }
 __device__ __forceinline__ void psd9(const WorldArgs& a, int cid) {
     double* R = a.rec + REC * (size_t)cid;
@@ -1050,8 +1065,32 @@ __device__ __forceinline__ void psd9(const WorldArgs& a, int cid) {
             }
     }
 }
+__global__ __launch_bounds__(256) void k_contact_c3(WorldArgs a) {
+    const int n3 = a.counts[2];
+    if (n3 < C3_SPLIT_MAX) {
+        const long long n = (long long)C3_ROLES * n3;
+        for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+            const int cid = a.list3[t / C3_ROLES], role = (int)(t % C3_ROLES);
+            C3Ctx x;
+            c3_setup(a, cid, x);
+            if (role < 9) c3_kbuild(x, role);
+            else c3_fields(a, cid, x);
+        }
+    } else {
+        for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n3; p += gridDim.x * blockDim.x) {
+            const int cid = a.list3[p];
+            C3Ctx x;
+            c3_setup(a, cid, x);
+#pragma unroll 1
+            for (int role = 0; role < 9; role++) c3_kbuild(x, role);
+            c3_fields(a, cid, x);
+            if (a.kp.psd) psd9(a, cid);
+        }
+    }
+}
 __global__ __launch_bounds__(64) void k_contact_psd9(WorldArgs a) {
     const int n3 = a.counts[2];
+    if (n3 >= C3_SPLIT_MAX) return;                   // projected by k_contact_c3
     for (int p = blockIdx.x * 64 + threadIdx.x; p < n3; p += gridDim.x * 64) psd9(a, a.list3[p]);
 }
 

@@ -146,7 +146,7 @@ def max_vertex_deviation(sc, qa, qb):
 # kernels launched once per launch of each profiling scope
 KERNEL_OF_SCOPE = {"asm_reduce": ["k_pair_reduce"], "narrow_set": ["k_pair_narrow<0", "k_pair_narrow<2", "k_trial_set"],
                    "narrow_ccd": ["k_pair_narrow<1"],
-                   "block_chol": ["k_group_inv"], "contact_group": ["k_contact_c3"],
+                   "block_chol": ["k_group_inv"], "contact_group": ["k_contact_c3", "k_contact_psd9"],
                    "contact": ["k_contact_small<1>", "k_contact_scalars2"], "contact_pk": ["k_contact_pk2"],
                    "spmv": ["k_pcg_spmv"], "pcg_update": ["k_pcg_update"], "contact_classify": ["k_contact_classify"]}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hbm_traffic_2000_icospheres.json")
@@ -253,6 +253,44 @@ def kernel_stats_md(stats_csv, title, cmd, top=25):
     return "\n".join(lines) + "\n"
 
 
+def replay_main(args):
+    """One step of the workload from a saved state, replayed: every replay
+    starts from the same q, qdot, so the timings compare code, not
+    trajectories (the pile's Newton counts depend on rounding anywhere)."""
+    import numpy as np
+    from adb_amd.world import World
+    sc = workload(args.bodies)
+    p = params(args.precision)
+    d = np.load(args.replay)
+    w = World(sc, p)
+    times, st = [], None
+    for i in range(args.warmup + args.steps):
+        w.set_state(d["q"], d["qdot"])
+        t = time.perf_counter()
+        rc, st = w.step()
+        if i >= args.warmup:
+            times.append(time.perf_counter() - t)
+    prof = {}
+    if args.profile_steps > 0:
+        w.profile_enable(True)
+        w.set_state(d["q"], d["qdot"])
+        w.step()
+        prof = w.profile_read()
+        w.profile_enable(False)
+        if args.profile_json:
+            with open(args.profile_json, "w") as fh:
+                json.dump(prof, fh, indent=1)
+    ms = 1000.0 * sum(times) / max(1, len(times))
+    ranked = sorted(prof.items(), key=lambda kv: -kv[1]["ms"])
+    print(json.dumps({"metric": "ms per replayed step", "value": round(ms, 3), "unit": "ms", "higher_is_better": False,
+                      "steps": args.steps, "warmup": args.warmup,
+                      "config": {"workload": "2000 icospheres (642 v / 1280 t / 1920 e each), one step replayed from %s"
+                                             % os.path.basename(args.replay), "precision": args.precision},
+                      "newton_iters": st.newton_iters, "pcg_iters": st.pcg_iters, "ls_trials": st.ls_trials,
+                      "contacts": st.n_pt + st.n_ee + st.n_vg, "min_ms": round(1000.0 * min(times), 3),
+                      "top_scopes_ms": {k: [round(v["ms"], 3), v["launches"]] for k, v in ranked[:10]}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -279,6 +317,9 @@ def main():
                          "operations per launch of the FP64-VALU-bound scopes; writes --out")
     ap.add_argument("--kernel-stats", nargs=3, metavar=("STATS_CSV", "TITLE", "CMD"),
                     help="no run: write a markdown table of a rocprofv3 --stats kernel summary to --out")
+    ap.add_argument("--replay", default="", metavar="NPZ",
+                    help="time single steps replayed from a saved state (q, qdot arrays of the same workload, e.g. "
+                         "tests/golden/c2_pile_step70.npz): --warmup untimed and --steps timed replays, one profiled")
     ap.add_argument("--out", default="")
     args = ap.parse_args()
     if args.pmc_traffic:
@@ -315,6 +356,9 @@ def main():
         torch.add(src, 1.0, out=dst)      # reads and writes copy_mb MB
         torch.cuda.synchronize()
         del src, dst
+    if args.replay:
+        replay_main(args)
+        return
     sc = workload(args.bodies)
     p = params(args.precision)
     elapsed, tot, last, w, per_step = run(sc, p, args.warmup, args.steps, args.precision,
