@@ -215,7 +215,10 @@ def pmc_fp64(counter_csv, prof_json, scopes):
         prof = json.load(fh)
     rows = list(csv.DictReader(open(counter_csv)))
     out = {"source": "rocprofv3 --pmc SQ_INSTS_VALU_ADD_F64 SQ_INSTS_VALU_MUL_F64 SQ_INSTS_VALU_FMA_F64 "
-                      "SQ_INSTS_VALU_TRANS_F64", "scopes": {}}
+                      "SQ_INSTS_VALU_TRANS_F64", "scopes": {},
+           "counting": "64 operations per issued ADD / MUL / TRANS wave instruction, 128 per FMA, averaged over the "
+                       "scope's launches in the profiled step after the timed ones: every lane of every issued "
+                       "instruction, an upper bound on the useful operations"}
     weight = {"SQ_INSTS_VALU_ADD_F64": 64.0, "SQ_INSTS_VALU_MUL_F64": 64.0, "SQ_INSTS_VALU_TRANS_F64": 64.0,
               "SQ_INSTS_VALU_FMA_F64": 128.0}
     for scope in scopes:
