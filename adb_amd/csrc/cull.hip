@@ -758,6 +758,10 @@ static NarrowArgs narrow_args(World& w, const int* npairs, const double* X0, con
 
 // The contact sets at X (X1 == nullptr), or with X1 the PT / EE candidates of
 // the segment X -> X1 (no VG), sorted and decoded into out.
+static int* small_host(World& w) {
+    if (!w.h_small) ADB_HIP(hipHostMalloc((void**)&w.h_small, 256 * sizeof(int), hipHostMallocDefault));
+    return w.h_small;
+}
 static void set_pass(World& w, const double* X, const double* X1, ContactSet& out) {
     const double r = sqrt(w.P.d_hat) * (1.0 + 1e-6) + 1e-12;
     const bool swept = X1 != nullptr;
@@ -796,8 +800,9 @@ static void set_pass(World& w, const double* X, const double* X1, ContactSet& ou
                                w.vg_keys.p, (int)vcap, w.bcnt.p + 2 * NBK);
             ADB_HIP(hipGetLastError());
         }
-        int h[2 * NBK + 2];
-        ADB_HIP(hipMemcpyAsync(h, w.bcnt.p, sizeof h, hipMemcpyDeviceToHost, w.stream));
+        static_assert(2 * NBK + 2 <= 256, "bucket counters fit the pinned scratch");
+        int* h = small_host(w);
+        ADB_HIP(hipMemcpyAsync(h, w.bcnt.p, (2 * NBK + 2) * sizeof(int), hipMemcpyDeviceToHost, w.stream));
         ADB_HIP(hipStreamSynchronize(w.stream));
         if (!hashed) w.pair_hint = h[2 * NBK + 1];
         if (!hashed && (size_t)h[2 * NBK + 1] > w.pairs.cap) {   // more overlapping body pairs than slots
@@ -925,7 +930,12 @@ __global__ __launch_bounds__(256) void k_trial_set(TrialArgs a) {
 
 // The sets at X from the candidates of a segment through X (their sorted
 // order is kept), VG by a scan of all vertices.
+__global__ void k_counts3(const int* __restrict__ off, int i0, int i1, int i2, int* __restrict__ dst) {
+    if (threadIdx.x == 0) { dst[0] = off[i0]; dst[1] = off[i1]; dst[2] = off[i2]; }
+}
 void trial_set(World& w, const double* X, const ContactSet& cand, ContactSet& out) {
+    w.tcount.reserve(4);
+    if (!w.h_tcount) ADB_HIP(hipHostMalloc((void**)&w.h_tcount, 4 * sizeof(int), hipHostMallocDefault));
     TrialArgs a;
     a.n_pt = cand.n_pt; a.n_ee = cand.n_ee; a.V = w.V; a.ground = w.P.enable_ground ? 1 : 0;
     a.nb_pt = (a.n_pt + 255) / 256; a.nb_ee = (a.n_ee + 255) / 256; a.nb_v = a.ground ? (w.V + 255) / 256 : 0;
@@ -956,11 +966,12 @@ void trial_set(World& w, const double* X, const ContactSet& cand, ContactSet& ou
         hipLaunchKernelGGL(k_trial_set<true>, dim3(nb), dim3(256), 0, w.stream, a);
         ADB_HIP(hipGetLastError());
     }
-    int h[3];
-    ADB_HIP(hipMemcpyAsync(&h[0], off + a.nb_pt, sizeof(int), hipMemcpyDeviceToHost, w.stream));
-    ADB_HIP(hipMemcpyAsync(&h[1], off + a.nb_pt + a.nb_ee, sizeof(int), hipMemcpyDeviceToHost, w.stream));
-    ADB_HIP(hipMemcpyAsync(&h[2], off + nb, sizeof(int), hipMemcpyDeviceToHost, w.stream));
+    // the three counts gathered on the device, one read into pinned memory
+    hipLaunchKernelGGL(k_counts3, dim3(1), dim3(64), 0, w.stream, off, a.nb_pt, a.nb_pt + a.nb_ee, nb, w.tcount.p);
+    ADB_HIP(hipGetLastError());
+    ADB_HIP(hipMemcpyAsync(w.h_tcount, w.tcount.p, 3 * sizeof(int), hipMemcpyDeviceToHost, w.stream));
     ADB_HIP(hipStreamSynchronize(w.stream));
+    const int* h = w.h_tcount;
     out.n_pt = h[0]; out.n_ee = h[1] - h[0]; out.n_vg = h[2] - h[1];
 }
 
@@ -979,9 +990,11 @@ double ccd_toi(World& w, const double* X0, const double* X1, int* status) {
     const bool hashed = !(w.P.iaabb & 2);    // spatial hashing (hash.hip) instead of body overlaps
     w.bcnt.reserve(2 * NBK + 2);
     int* pcnt = w.bcnt.p + 2 * NBK + 1;
-    unsigned long long bits[2];
+    int* hs = small_host(w);                  // [0]: pair count; [2, 6): the two 64-bit bounds
+    unsigned long long* bits = reinterpret_cast<unsigned long long*>(hs + 2);
     for (int attempt = 0; attempt < 2; attempt++) {
         int np = 0;
+        hs[0] = 0;
         if (!hashed) body_pairs(w, X0, X1, 0.0, pcnt);
         NarrowArgs a = narrow_args(w, pcnt, X0, X1, 0.0);
         a.toi_bits = w.toi_bits.p;
@@ -993,10 +1006,11 @@ double ccd_toi(World& w, const double* X0, const double* X1, int* status) {
             if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<1, float>), grid, dim3(NB_THREADS), 0, w.stream, a);
             else hipLaunchKernelGGL((k_pair_narrow<1, double>), grid, dim3(NB_THREADS), 0, w.stream, a);
             ADB_HIP(hipGetLastError());
-            ADB_HIP(hipMemcpyAsync(&np, pcnt, sizeof(int), hipMemcpyDeviceToHost, w.stream));
+            ADB_HIP(hipMemcpyAsync(hs, pcnt, sizeof(int), hipMemcpyDeviceToHost, w.stream));
         }
-        ADB_HIP(hipMemcpyAsync(bits, w.toi_bits.p, sizeof bits, hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipMemcpyAsync(bits, w.toi_bits.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, w.stream));
         ADB_HIP(hipStreamSynchronize(w.stream));
+        np = hs[0];
         if (!hashed) w.pair_hint = np;
         if ((size_t)np <= w.pairs.cap) break;
         w.pairs.reserve(np);               // more overlapping body pairs than slots: grow, test again

@@ -126,6 +126,9 @@ struct World {
     DBuf<int4> sh_rng;
     DBuf<unsigned long long> sh_total;
     ContactSet cs, cs_new;
+    DBuf<int> tcount;                // trial-set counts (PT, PT + EE, all), gathered for one read
+    int* h_tcount = nullptr;         // pinned host copy
+    int* h_small = nullptr;          // pinned host scratch (256 ints) for the small synchronous reads of the culling passes
     ContactSet cand;                 // line search: candidates of the search segment
     bool ls_cand = true;             // trial sets filtered from them (ADB_LS_CANDIDATES=0: full passes)
     int ls_prev = 2;                 // trials of the previous line search

@@ -539,6 +539,11 @@ static void free_world(World& w) {
     w.bcnt.free_(); w.sh_cnt.free_(); w.sh_start.free_(); w.sh_ent.free_(); w.sh_rng.free_(); w.sh_total.free_();
     w.akeys.free_(); w.akeys_s.free_(); w.tkeys.free_(); w.tkeys_s.free_();
     for (ContactSet* c : {&w.cs, &w.cs_new, &w.cand}) { c->pt.free_(); c->ee.free_(); c->vg.free_(); }
+    w.tcount.free_();
+    if (w.h_tcount) (void)hipHostFree(w.h_tcount);
+    w.h_tcount = nullptr;
+    if (w.h_small) (void)hipHostFree(w.h_small);
+    w.h_small = nullptr;
     w.vsel.free_(); w.keepbits.free_();
     w.scene_off.free_(); w.scene_of.free_(); w.swk.free_(); w.swk_s.free_(); w.swi.free_(); w.swi_s.free_();
     w.seg.free_(); w.sred.free_(); w.sfac.free_(); w.sc_goff.free_(); w.pcg_sbb.free_(); w.pcg_brun.free_();
