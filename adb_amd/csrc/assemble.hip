@@ -671,7 +671,7 @@ static void groups_fetch(World& w) {
 }
 static void build_groups(World& w) {
     const int B = w.B, np = w.n_pairs;
-    ADB_HIP(hipEventSynchronize(w.ev_pairs));
+    wait_event(w, w.ev_pairs);
     const int* plo = w.h_pairs;
     const int* phi = w.h_pairs + np;
     const int* ps = w.h_pairs + 2 * (size_t)np;
@@ -742,7 +742,7 @@ static void build_groups(World& w) {
     const int G = w.n_groups;
     const size_t nll = (size_t)B + 1, nint = (size_t)(B + 1) + B + B + 1, total = nll + (nint + 1) / 2;
     if (!w.ev_grp) ADB_HIP(hipEventCreateWithFlags(&w.ev_grp, hipEventDisableTiming));
-    ADB_HIP(hipEventSynchronize(w.ev_grp));
+    wait_event(w, w.ev_grp);
     if (total > w.h_grp_pack_cap) {
         if (w.h_grp_pack) ADB_HIP(hipHostFree(w.h_grp_pack));
         w.h_grp_pack_cap = total + total / 2 + 64;
@@ -829,7 +829,7 @@ void assemble_system(World& w) {
         w.sort_tmp.reserve(tmp + 16);
         ADB_HIP(hipcub::DeviceScan::InclusiveSum(w.sort_tmp.p, tmp, w.uniq_flag.p, w.blk_of.p, nc, w.stream));
         ADB_HIP(hipMemcpyAsync(&np, w.blk_of.p + nc - 1, sizeof(int), hipMemcpyDeviceToHost, w.stream));
-        ADB_HIP(hipStreamSynchronize(w.stream));
+        wait_stream(w);
     }
     w.n_pairs = np;
     w.pair_start.reserve(np + 1); w.pair_lo.reserve(std::max(np, 1)); w.pair_hi.reserve(std::max(np, 1));

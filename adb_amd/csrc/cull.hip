@@ -803,7 +803,7 @@ static void set_pass(World& w, const double* X, const double* X1, ContactSet& ou
         static_assert(2 * NBK + 2 <= 256, "bucket counters fit the pinned scratch");
         int* h = small_host(w);
         ADB_HIP(hipMemcpyAsync(h, w.bcnt.p, (2 * NBK + 2) * sizeof(int), hipMemcpyDeviceToHost, w.stream));
-        ADB_HIP(hipStreamSynchronize(w.stream));
+        wait_stream(w);
         if (!hashed) w.pair_hint = h[2 * NBK + 1];
         if (!hashed && (size_t)h[2 * NBK + 1] > w.pairs.cap) {   // more overlapping body pairs than slots
             w.pairs.reserve(h[2 * NBK + 1]);
@@ -970,7 +970,7 @@ void trial_set(World& w, const double* X, const ContactSet& cand, ContactSet& ou
     hipLaunchKernelGGL(k_counts3, dim3(1), dim3(64), 0, w.stream, off, a.nb_pt, a.nb_pt + a.nb_ee, nb, w.tcount.p);
     ADB_HIP(hipGetLastError());
     ADB_HIP(hipMemcpyAsync(w.h_tcount, w.tcount.p, 3 * sizeof(int), hipMemcpyDeviceToHost, w.stream));
-    ADB_HIP(hipStreamSynchronize(w.stream));
+    wait_stream(w);
     const int* h = w.h_tcount;
     out.n_pt = h[0]; out.n_ee = h[1] - h[0]; out.n_vg = h[2] - h[1];
 }
@@ -1009,7 +1009,7 @@ double ccd_toi(World& w, const double* X0, const double* X1, int* status) {
             ADB_HIP(hipMemcpyAsync(hs, pcnt, sizeof(int), hipMemcpyDeviceToHost, w.stream));
         }
         ADB_HIP(hipMemcpyAsync(bits, w.toi_bits.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, w.stream));
-        ADB_HIP(hipStreamSynchronize(w.stream));
+        wait_stream(w);
         np = hs[0];
         if (!hashed) w.pair_hint = np;
         if ((size_t)np <= w.pairs.cap) break;
@@ -1052,7 +1052,7 @@ void ccd_toi_scenes(World& w, const double* X0, const double* X1, double* toi, i
         }
         ADB_HIP(hipMemcpyAsync(&np, pcnt, sizeof(int), hipMemcpyDeviceToHost, w.stream));
         ADB_HIP(hipMemcpyAsync(bits.data(), w.toi_bits.p, bits.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, w.stream));
-        ADB_HIP(hipStreamSynchronize(w.stream));
+        wait_stream(w);
         w.pair_hint = np;
         if ((size_t)np <= w.pairs.cap) break;
         w.pairs.reserve(np);

@@ -169,7 +169,7 @@ static void reduce_async(World& w, const double* a, const double* b, int n, int 
 
 void read_slots(World& w, int k, double* out) {
     ADB_HIP(hipMemcpyAsync(out, w.epart.p, k * sizeof(double), hipMemcpyDeviceToHost, w.stream));
-    ADB_HIP(hipStreamSynchronize(w.stream));
+    wait_stream(w);
 }
 void copy_slots_async(World& w, int first, int k) {
     if (!w.h_slots) ADB_HIP(hipHostMalloc((void**)&w.h_slots, NSLOT * sizeof(double), hipHostMallocDefault));
@@ -337,7 +337,7 @@ void energy_scenes_async(World& w, const double* qv, const double* X, const Cont
 void read_scene_sums(World& w, int nslots) {
     if (!w.h_sred) ADB_HIP(hipHostMalloc((void**)&w.h_sred, (size_t)SRED_N * w.S * sizeof(double), hipHostMallocDefault));
     ADB_HIP(hipMemcpyAsync(w.h_sred, w.sred.p, (size_t)nslots * w.S * sizeof(double), hipMemcpyDeviceToHost, w.stream));
-    ADB_HIP(hipStreamSynchronize(w.stream));
+    wait_stream(w);
 }
 
 } // namespace adb

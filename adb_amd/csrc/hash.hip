@@ -238,7 +238,7 @@ static void hash_build(World& w, bool edges, const double* X0, const double* X1)
     }
     unsigned long long total = 0;
     ADB_HIP(hipMemcpyAsync(&total, w.sh_total.p, sizeof total, hipMemcpyDeviceToHost, w.stream));
-    ADB_HIP(hipStreamSynchronize(w.stream));
+    wait_stream(w);
     // the slot counts are left set: a failed build must not leave them for the next
     if (total > (unsigned long long)INT_MAX - 1) {
         ADB_HIP(hipMemsetAsync(w.sh_cnt.p, 0, (nslot + 1) * sizeof(int), w.stream));

@@ -386,7 +386,7 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
         it += nb0;
         stop_test();
         ADB_HIP(hipMemcpyAsync(w.h_scal, w.scal.p, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, w.stream));
-        ADB_HIP(hipStreamSynchronize(w.stream));
+        wait_stream(w);
         done = w.h_scal[S_STOP] != RUN;
     }
     while (!done && it < max_it) {
@@ -400,7 +400,7 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
         ADB_HIP(hipEventRecord(w.pcg_ev[slot], w.stream));
         if (have_prev) {
             // the batch before this one: stopped already?
-            ADB_HIP(hipEventSynchronize(w.pcg_ev[slot ^ 1]));
+            wait_event(w, w.pcg_ev[slot ^ 1]);
             if (w.h_scal[S_COUNT * (slot ^ 1) + S_STOP] != RUN) break;
         }
         have_prev = true;
@@ -409,7 +409,7 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
     if (!done) {
         stop_test();
         ADB_HIP(hipMemcpyAsync(w.h_scal, w.scal.p, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, w.stream));
-        ADB_HIP(hipStreamSynchronize(w.stream));
+        wait_stream(w);
     }
     const double* h = w.h_scal;
     const int stop = (int)h[S_STOP];

@@ -130,6 +130,7 @@ struct World {
     int* h_tcount = nullptr;         // pinned host copy
     int* h_small = nullptr;          // pinned host scratch (256 ints) for the small synchronous reads of the culling passes
     ContactSet cand;                 // line search: candidates of the search segment
+    bool spin_wait = true;           // host waits spin on the query (ADB_SPIN_WAIT=0: blocking calls)
     bool ls_cand = true;             // trial sets filtered from them (ADB_LS_CANDIDATES=0: full passes)
     int ls_prev = 2;                 // trials of the previous line search
     // barrier + inertia energy at the point the last line search accepted (the
@@ -260,6 +261,11 @@ struct ProfScope {
     }
 };
 void prof_flush(World& w);
+// Host waits for the world's stream or an event on it: a spin on the query
+// (ADB_SPIN_WAIT=1, default) answers as soon as the work is done, where the
+// blocking call may first put the thread to sleep; both report errors.
+void wait_stream(World& w);
+void wait_event(World& w, hipEvent_t e);
 // Pinned host copy of 4 device counters (enqueued on the world stream) when
 // profiling is on, else nullptr.
 const int* prof_snapshot(World& w, const int* dev_counts);

@@ -47,6 +47,19 @@ static void scale(World& w, double* v, int n, double s) {
     ADB_HIP(hipGetLastError());
 }
 
+void wait_stream(World& w) {
+    if (!w.spin_wait) { ADB_HIP(hipStreamSynchronize(w.stream)); return; }
+    hipError_t e;
+    while ((e = hipStreamQuery(w.stream)) == hipErrorNotReady) {}
+    ADB_HIP(e);
+}
+void wait_event(World& w, hipEvent_t ev) {
+    if (!w.spin_wait) { ADB_HIP(hipEventSynchronize(ev)); return; }
+    hipError_t e;
+    while ((e = hipEventQuery(ev)) == hipErrorNotReady) {}
+    ADB_HIP(e);
+}
+
 // Armijo backtracking (model/line_search.cpp:39-136) from w.q along w.dq (the
 // direction already scaled by the CCD bound).  On entry w.X and w.cs hold the
 // positions and the contact set at q, w.grad the gradient there and the
@@ -130,7 +143,7 @@ static int step(World& w, adb_step_stats* st) {
         contact_terms(w);
         assemble_system(w);
         S.contacts_processed += (long long)w.cs.n_pt + w.cs.n_ee + w.cs.n_vg;
-        ADB_HIP(hipStreamSynchronize(w.stream));
+        wait_stream(w);
         const double t1 = now_s();
         S.t_ipc += t1 - t0;
         int pit = 0;
@@ -151,7 +164,7 @@ static int step(World& w, adb_step_stats* st) {
             transform_positions(w, w.qtrial.p, w.X1.p);
             toi = ccd_toi(w, w.X.p, w.X1.p, &cs_status);
         } else {
-            ADB_HIP(hipStreamSynchronize(w.stream));
+            wait_stream(w);
         }
         const double sl[2] = {w.h_slots[3], w.h_slots[4]};
         // a solve that stopped early is counted, not hidden: the direction is
@@ -200,7 +213,7 @@ static int step(World& w, adb_step_stats* st) {
         hipLaunchKernelGGL(k_velocity, dim3((n + 255) / 256), dim3(256), 0, w.stream, n, w.P.dt, w.q.p, w.q0.p, w.qdot.p);
         ADB_HIP(hipGetLastError());
     }
-    ADB_HIP(hipStreamSynchronize(w.stream));
+    wait_stream(w);
     S.n_pt = w.cs.n_pt; S.n_ee = w.cs.n_ee; S.n_vg = w.cs.n_vg;
     S.t_total = now_s() - t_start;
     if (st) *st = S;
@@ -247,7 +260,7 @@ static int step_batched(World& w, adb_step_stats* st) {
         for (int s = 0; s < S; s++) fac[s] = active[s] ? 1.0 : 0.0;
         put_factors(fac);
         scale_scenes(w, w.grad.p, w.sfac.p);
-        ADB_HIP(hipStreamSynchronize(w.stream));
+        wait_stream(w);
         const double t1 = now_s();
         S_.t_ipc += t1 - t0;
         int pit = 0;
@@ -371,7 +384,7 @@ static int step_batched(World& w, adb_step_stats* st) {
         for (size_t i = 0; i < vg.size(); i += 2) cnt[scene_of_body(vg[i])]++;
         w.sc_contacts = cnt;
     }
-    ADB_HIP(hipStreamSynchronize(w.stream));
+    wait_stream(w);
     w.sc_newton = newton; w.sc_ls = lst; w.sc_status = status;
     S_.newton_iters = *std::max_element(newton.begin(), newton.end());
     S_.n_pt = w.cs.n_pt; S_.n_ee = w.cs.n_ee; S_.n_vg = w.cs.n_vg;
@@ -389,7 +402,7 @@ void download_sets(World& w, const ContactSet& cs, std::vector<int>& pt, std::ve
     if (cs.n_pt) ADB_HIP(hipMemcpyAsync(pt.data(), cs.pt.p, pt.size() * sizeof(int), hipMemcpyDeviceToHost, w.stream));
     if (cs.n_ee) ADB_HIP(hipMemcpyAsync(ee.data(), cs.ee.p, ee.size() * sizeof(int), hipMemcpyDeviceToHost, w.stream));
     if (cs.n_vg) ADB_HIP(hipMemcpyAsync(vg.data(), cs.vg.p, vg.size() * sizeof(int), hipMemcpyDeviceToHost, w.stream));
-    ADB_HIP(hipStreamSynchronize(w.stream));
+    wait_stream(w);
 }
 
 // assemble at the current q (= q0 between steps)
@@ -455,6 +468,7 @@ extern "C" int adb_world_create(const adb_scene* s, const adb_params* p, adb_wor
         w.B = s->n_bodies;
         if (const char* g = std::getenv("ADB_PCG_GRAPH")) w.use_graphs = std::strcmp(g, "0") != 0;
         if (const char* g = std::getenv("ADB_LS_CANDIDATES")) w.ls_cand = std::strcmp(g, "0") != 0;
+        if (const char* g = std::getenv("ADB_SPIN_WAIT")) w.spin_wait = std::strcmp(g, "0") != 0;
         w.h_voff.assign(s->vert_offset, s->vert_offset + w.B + 1);
         w.h_foff.assign(s->face_offset, s->face_offset + w.B + 1);
         w.h_eoff.assign(s->edge_offset, s->edge_offset + w.B + 1);
@@ -510,7 +524,7 @@ extern "C" int adb_world_create(const adb_scene* s, const adb_params* p, adb_wor
         w.X.reserve(3 * (size_t)w.V); w.Xs.reserve(3 * (size_t)w.V); w.X1.reserve(3 * (size_t)w.V);
         w.counters.reserve(16);
         w.epart.reserve(EPART_SLOTS);
-        ADB_HIP(hipStreamSynchronize(w.stream));
+        wait_stream(w);
     } catch (const HipError& e) {
         free_world(w); delete W; return ADB_E_HIP;
     } catch (const AdbError& e) {
@@ -618,7 +632,7 @@ extern "C" int adb_world_create_batched(const adb_scene* s, const int* scene_off
         w.sfac.reserve(n_scenes);
         w.pcg_sbb.reserve(n_scenes); w.pcg_brun.reserve(1);
         w.sc_goff.reserve(n_scenes + 1);
-        ADB_HIP(hipStreamSynchronize(w.stream));
+        wait_stream(w);
     } catch (const HipError& e) {
         adb_world_destroy(*out); *out = nullptr; return ADB_E_HIP;
     } catch (...) {
@@ -647,7 +661,7 @@ extern "C" int adb_get_state(adb_world* W, double* q, double* qdot) {
         const size_t n = 12 * (size_t)w.B * sizeof(double);
         if (q) ADB_HIP(hipMemcpyAsync(q, w.q0.p, n, hipMemcpyDeviceToHost, w.stream));
         if (qdot) ADB_HIP(hipMemcpyAsync(qdot, w.qdot.p, n, hipMemcpyDeviceToHost, w.stream));
-        ADB_HIP(hipStreamSynchronize(w.stream));
+        wait_stream(w);
         return ADB_OK;
     })
 }
@@ -662,7 +676,7 @@ extern "C" int adb_set_state(adb_world* W, const double* q, const double* qdot) 
             ADB_HIP(hipMemcpyAsync(w.q.p, q, n, hipMemcpyHostToDevice, w.stream));
         }
         if (qdot) ADB_HIP(hipMemcpyAsync(w.qdot.p, qdot, n, hipMemcpyHostToDevice, w.stream));
-        ADB_HIP(hipStreamSynchronize(w.stream));
+        wait_stream(w);
         return ADB_OK;
     })
 }
@@ -672,7 +686,7 @@ extern "C" const char* adb_last_error(adb_world* W) { return W ? W->w.err.c_str(
 namespace adb {
 void prof_flush(World& w) {
     if (w.prof.pending.empty()) return;
-    ADB_HIP(hipStreamSynchronize(w.stream));
+    wait_stream(w);
     for (auto& r : w.prof.pending) {
         float ms = 0.0f;
         ADB_HIP(hipEventElapsedTime(&ms, r.a, r.b));
@@ -805,7 +819,7 @@ static void read_system(World& w, double* grad, double* hess) {
             ADB_HIP(hipMemcpyAsync(rp.data(), w.row_ptr.p, rp.size() * sizeof(int), hipMemcpyDeviceToHost, w.stream));
             ADB_HIP(hipMemcpyAsync(ci.data(), w.col_idx.p, ci.size() * sizeof(int), hipMemcpyDeviceToHost, w.stream));
             ADB_HIP(hipMemcpyAsync(hv.data(), w.Hval.p, hv.size() * sizeof(double), hipMemcpyDeviceToHost, w.stream));
-            ADB_HIP(hipStreamSynchronize(w.stream));
+            wait_stream(w);
             std::memset(hess, 0, sizeof(double) * (size_t)n * n);
             for (int r = 0; r < w.B; r++)
                 for (int k = rp[r]; k < rp[r + 1]; k++)
@@ -813,7 +827,7 @@ static void read_system(World& w, double* grad, double* hess) {
                         for (int i = 0; i < 12; i++)
                             hess[(size_t)(12 * ci[k] + c) * n + 12 * r + i] = hv[144 * (size_t)k + 12 * c + i];
         }
-        ADB_HIP(hipStreamSynchronize(w.stream));
+        wait_stream(w);
 }
 
 // The contact set, contact terms and assembly at the Newton iterate q of a
@@ -895,7 +909,7 @@ extern "C" int adb_adjoint_solve(adb_world* W, const double* rhs, double* x, int
         rc = pcg_solve(w, w.dq.p, &it);
         if (iters) *iters = it;
         ADB_HIP(hipMemcpyAsync(x, w.dq.p, n * sizeof(double), hipMemcpyDeviceToHost, w.stream));
-        ADB_HIP(hipStreamSynchronize(w.stream));
+        wait_stream(w);
         if (rc == ADB_E_NOT_CONVERGED) w.err = "adjoint PCG reached pcg_max_iter";
         if (rc == ADB_E_SOLVER) w.err = "adjoint PCG breakdown: the Hessian at the state is not positive definite";
     } catch (const HipError& e) { w.err = e.what; rc = ADB_E_HIP; }
@@ -913,7 +927,7 @@ extern "C" int adb_newton_direction(adb_world* W, double* dq) {
         int it = 0;
         int rc = pcg_solve(w, w.dq.p, &it);
         ADB_HIP(hipMemcpyAsync(dq, w.dq.p, 12 * (size_t)w.B * sizeof(double), hipMemcpyDeviceToHost, w.stream));
-        ADB_HIP(hipStreamSynchronize(w.stream));
+        wait_stream(w);
         return rc;
     })
 }
