@@ -18,7 +18,6 @@
 // Static bodies (mass < 0) follow the reference: contributions to them are
 // dropped, their diagonal block is I and their gradient 0.
 #include <hipcub/hipcub.hpp>
-#include <rocprim/device/device_radix_sort.hpp>
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -792,15 +791,9 @@ static double group_inv_flops(int m) {
 template <class K, class V>
 static void sort_pairs(World& w, K* kin, K* kout, V* vin, V* vout, int n, int end_bit) {
     size_t tmp = 0;
-    // rocPRIM's radix sort with the merge-sort limit at 0: onesweep passes
-    // (8-bit digits, ceil(end_bit / 8) of them) at every size.  Below its default
-    // limit of 2^20 keys rocPRIM sorts blocks and merges them, log2(n / block)
-    // passes of two kernels each (16 launches, ~80 us for 2e5 contacts, against
-    // ~5 launches here); both sorts are stable, so the order is the same.
-    using OneSweep = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config, rocprim::default_config, 0>;
-    ADB_HIP(rocprim::radix_sort_pairs<OneSweep>(nullptr, tmp, kin, kout, vin, vout, n, 0, end_bit, w.stream));
+    ADB_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, kin, kout, vin, vout, n, 0, end_bit, w.stream));
     w.sort_tmp.reserve(tmp + 16);
-    ADB_HIP(rocprim::radix_sort_pairs<OneSweep>(w.sort_tmp.p, tmp, kin, kout, vin, vout, n, 0, end_bit, w.stream));
+    ADB_HIP(hipcub::DeviceRadixSort::SortPairs(w.sort_tmp.p, tmp, kin, kout, vin, vout, n, 0, end_bit, w.stream));
 }
 
 void assemble_system(World& w) {
