@@ -15,8 +15,9 @@
 //      in an LDS buffer that goes out with one atomic on one of 64 bucket
 //      counters per flush;
 //   4. vertex-ground candidates by a vertex-parallel scan;
-//   5. the buckets are packed, a radix sort of the 64-bit keys gives the
-//      canonical order, then decode.
+//   5. the keys are grouped by their high bits (counting sort), each key is
+//      placed by its rank inside its group (the canonical sorted order), then
+//      decoded.
 // The box prefilters are conservative, so the sets equal the exact predicate
 // over all primitive pairs (d_PT < d_hat for I != J, d_EE < d_hat for I < J).
 #include <hipcub/hipcub.hpp>
@@ -47,6 +48,34 @@ __global__ void k_axpy(int n, const double* __restrict__ q, const double* __rest
     if (i < n) out[i] = q[i] + (dq ? dq[i] * alpha : 0.0);
 }
 
+// q' = q + alpha dq (dq null: q' = q) written to qout (thread i < 12 B writes
+// entry i, as k_axpy computes it), and the vertex positions at q' to X (and
+// X2 if given); each vertex thread forms its body's q' itself.
+__global__ void k_transform_q(int V, int n12, const int* __restrict__ vbody, const double* __restrict__ xbar,
+                              const double* __restrict__ q, const double* __restrict__ dq, double alpha, double* __restrict__ qout,
+                              double* __restrict__ X, double* __restrict__ X2) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n12 && qout) qout[v] = dq ? q[v] + dq[v] * alpha : q[v];
+    if (v >= V) return;
+    const size_t b = 12 * (size_t)vbody[v];
+    double Q[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) Q[k] = dq ? q[b + k] + dq[b + k] * alpha : q[b + k];
+    const double x0 = xbar[3 * (size_t)v], x1 = xbar[3 * (size_t)v + 1], x2 = xbar[3 * (size_t)v + 2];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const double x = ((Q[3 + i] * x0 + Q[6 + i] * x1) + Q[9 + i] * x2) + Q[i];
+        X[3 * (size_t)v + i] = x;
+        if (X2) X2[3 * (size_t)v + i] = x;
+    }
+}
+void transform_step(World& w, const double* q, const double* dq, double alpha, double* qout, double* X, double* X2) {
+    ProfScope ps(w, PK_TRANSFORM, (X2 ? 76.0 : 52.0) * w.V + (dq ? 288.0 : 192.0) * w.B);
+    const int n = std::max(w.V, 12 * w.B);
+    hipLaunchKernelGGL(k_transform_q, dim3((n + 255) / 256), dim3(256), 0, w.stream, w.V, 12 * w.B, w.vbody.p, w.xbar.p, q, dq, alpha,
+                       qout, X, X2);
+    ADB_HIP(hipGetLastError());
+}
 void transform_positions(World& w, const double* q, double* X) {
     ProfScope ps(w, PK_TRANSFORM, 52.0 * w.V + 96.0 * w.B);   // xbar + vbody in, X out, q
     hipLaunchKernelGGL(k_transform, dim3((w.V + 255) / 256), dim3(256), 0, w.stream, w.V, w.vbody.p, w.xbar.p, q, X);
@@ -82,10 +111,18 @@ void scale_scenes(World& w, double* v, const double* f) {
 // one workgroup per body; X1 != null: union of both position sets
 // Body AABB (inflated by `infl`) and bounding sphere about the box centre,
 // both from the vertex positions at X0 (and X1: the swept set, as the
-// segments x0 -> x1 stay inside the ball of their end points).
+// segments x0 -> x1 stay inside the ball of their end points).  Workgroup 0
+// also clears the pass's counters zero[0, nzero) and, with toi, sets the two
+// step bounds toi[0, 2) to toi_one (the kernels after it in the stream
+// count and take minima there).
 __global__ void k_body_box(const int* __restrict__ voff, const double* __restrict__ X0, const double* __restrict__ X1,
-                           double infl, double* __restrict__ box, double* __restrict__ sph) {
+                           double infl, double* __restrict__ box, double* __restrict__ sph, int* __restrict__ zero, int nzero,
+                           unsigned long long* __restrict__ toi, unsigned long long toi_one) {
     const int b = blockIdx.x;
+    if (b == 0) {
+        for (int i = threadIdx.x; i < nzero; i += blockDim.x) zero[i] = 0;
+        if (toi && threadIdx.x < 2) toi[threadIdx.x] = toi_one;
+    }
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (int v = voff[b] + threadIdx.x; v < voff[b + 1]; v += blockDim.x) {
         for (int k = 0; k < 3; k++) {
@@ -578,36 +615,22 @@ __global__ void k_vg_toi(int V, const double* __restrict__ X0, const double* __r
 }
 
 // ------------------------------------------------------------------ decode
-__global__ void k_decode4(int n, const unsigned long long* __restrict__ keys, const int* __restrict__ bodyA,
-                          const int* __restrict__ offA, const int* __restrict__ bodyB, const int* __restrict__ offB, int4* out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    unsigned long long k = keys[i];
-    int a = (int)(k >> 32), b = (int)(k & 0xffffffffull);
-    int I = bodyA[a], J = bodyB[b];
-    out[i] = make_int4(I, a - offA[I], J, b - offB[J]);
-}
-__global__ void k_decode2(int n, const unsigned long long* __restrict__ keys, const int* __restrict__ vbody,
-                          const int* __restrict__ voff, int2* out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int v = (int)keys[i];
-    int I = vbody[v];
-    out[i] = make_int2(I, v - voff[I]);
-}
-
 // Contact-set keys are unique and sorted by value.  The high part of a key,
 // h = key >> shift, is a vertex or edge id (< nh).  The keys are grouped by
-// g = h >> GS (16 consecutive ids) with a counting sort, then hipCUB's
-// segmented radix sort orders each group on its low shift + GS bits (group
-// sizes vary by two orders of magnitude between the contact patches and the
-// rest, which its size classes absorb).
+// g = h >> gs (2^gs consecutive ids) with a counting sort; then, up to
+// RANK_MAX keys, every key goes to its group's start plus the number of
+// smaller keys in its group (k_group_rank: one thread per key, a loop over
+// its group, groups of 4 ids), above it hipCUB's segmented radix sort orders
+// groups of 16 ids (ranking is quadratic in the group size, and the contact
+// patches of a dense pile make groups of hundreds of keys; the radix sort
+// slows down with more, smaller segments).
 // The narrow phase flushes the keys of one body pair and task row together,
 // so consecutive keys mostly share a group: the counting passes take one
 // atomic per run of equal groups in a wavefront, not one per key
 // (device-scope atomics cost ~10 ns each).  Sorting all 43-54 key bits with
 // one radix pass per 8-bit digit took 1.5x longer.
-constexpr int GS = 4;
+constexpr int GS_RANK = 2, GS_RADIX = 4;
+constexpr int RANK_MAX = 1 << 20;
 
 // Lanes [0, nv) of the wavefront hold keys of group g.  Each run of equal g
 // (in lane order) updates ctr[g] by its length with one atomic: added
@@ -639,13 +662,20 @@ __device__ __forceinline__ int run_atomic(bool valid, unsigned g, int* ctr) {
 // then the sorted EE set, then the sorted VG set.
 struct KeySets {
     BucketList L[2];
-    int sub, nvg, V, E, ng_pt, ng_ee;
+    int sub, nvg, V, E, gs, ng_pt, ng_ee;
     const unsigned long long* buckets; const unsigned long long* vg;
 };
-// the loop runs whole wavefronts
+// the loop runs whole wavefronts.  The fill pass also clears the counts of
+// the count pass (clear[0, nclear), read by the scan before it) for the next
+// sort.
 template <bool FILL>
-__global__ void k_group_keys(KeySets ks, int* __restrict__ ctr, unsigned long long* __restrict__ dst) {
+__global__ void k_group_keys(KeySets ks, int* __restrict__ ctr, unsigned long long* __restrict__ dst, int* __restrict__ clear,
+                             int nclear) {
     const int y = blockIdx.y;
+    if (FILL) {
+        const int nt = gridDim.x * gridDim.y * blockDim.x;
+        for (int i = (blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x; i < nclear; i += nt) clear[i] = 0;
+    }
     int n, shift, nh, gbase;
     const unsigned long long* src;
     if (y < 2 * NBK) {
@@ -664,51 +694,124 @@ __global__ void k_group_keys(KeySets ks, int* __restrict__ ctr, unsigned long lo
         unsigned g = 0;
         if (valid) {
             key = src[i];
-            g = (unsigned)gbase + (unsigned)(std::min(key >> shift, (unsigned long long)(nh - 1)) >> GS);
+            g = (unsigned)gbase + (unsigned)(std::min(key >> shift, (unsigned long long)(nh - 1)) >> ks.gs);
         }
         const int pos = run_atomic<FILL>(valid, g, ctr);
         if (FILL && valid) dst[pos] = key;
     }
 }
 
-// Sorts the n keys of ks (at most `most` per bucket or list) into out; tmp
-// holds n keys.
-static void sort_keys(World& w, const KeySets& ks, int n, int most, unsigned long long* tmp, unsigned long long* out) {
+// Keys [0, npt) of tmp are PT keys, [npt, npt + nee) EE keys and the rest VG
+// keys, grouped as k_group_keys<true> placed them (group g in [start[g],
+// start[g + 1]), in no order inside).  The keys of a set are unique, so a
+// key's rank among the keys of its group is its offset in sorted order; the
+// key is decoded there into the contact set (body, local index) entries.
+struct SetOut {
+    const int* vbody; const int* voff; const int* fbody; const int* foff; const int* ebody; const int* eoff;
+    int4* pt; int4* ee; int2* vg;
+};
+__global__ __launch_bounds__(256) void k_group_rank(KeySets ks, int npt, int nee, int n, const int* __restrict__ start,
+                                                    const unsigned long long* __restrict__ tmp, SetOut o) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long key = tmp[i];
+    int shift, nh, gbase;
+    if (i < npt) { shift = 32; nh = ks.V; gbase = 0; }
+    else if (i < npt + nee) { shift = 32; nh = ks.E; gbase = ks.ng_pt; }
+    else { shift = 0; nh = ks.V; gbase = ks.ng_pt + ks.ng_ee; }
+    const unsigned g = (unsigned)gbase + (unsigned)(std::min(key >> shift, (unsigned long long)(nh - 1)) >> ks.gs);
+    const int s = start[g], e = start[g + 1];
+    int r = 0;
+#pragma unroll 4
+    for (int j = s; j < e; j++) r += tmp[j] < key ? 1 : 0;
+    const int pos = s + r;
+    const int a = (int)(key >> 32), b = (int)(key & 0xffffffffull);
+    if (i < npt) {
+        const int I = o.vbody[a], J = o.fbody[b];
+        o.pt[pos] = make_int4(I, a - o.voff[I], J, b - o.foff[J]);
+    } else if (i < npt + nee) {
+        const int I = o.ebody[a], J = o.ebody[b];
+        o.ee[pos - npt] = make_int4(I, a - o.eoff[I], J, b - o.eoff[J]);
+    } else {
+        const int I = o.vbody[b];
+        o.vg[pos - npt - nee] = make_int2(I, b - o.voff[I]);
+    }
+}
+
+// Sorted keys [0, npt + nee + nvg) decoded into the three lists.
+__global__ void k_decode_all(int npt, int nee, int n, const unsigned long long* __restrict__ keys, SetOut o) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long key = keys[i];
+    const int a = (int)(key >> 32), b = (int)(key & 0xffffffffull);
+    if (i < npt) {
+        const int I = o.vbody[a], J = o.fbody[b];
+        o.pt[i] = make_int4(I, a - o.voff[I], J, b - o.foff[J]);
+    } else if (i < npt + nee) {
+        const int I = o.ebody[a], J = o.ebody[b];
+        o.ee[i - npt] = make_int4(I, a - o.eoff[I], J, b - o.eoff[J]);
+    } else {
+        const int I = o.vbody[b];
+        o.vg[i - npt - nee] = make_int2(I, b - o.voff[I]);
+    }
+}
+// Sorts the n keys of ks (npt PT, nee EE, then VG; at most `most` per bucket
+// or list) and decodes them into out's lists; tmp and sorted (radix path)
+// hold n keys each.
+static void sort_keys(World& w, const KeySets& ks, int npt, int nee, int n, int most, unsigned long long* tmp,
+                      unsigned long long* sorted, ContactSet& out) {
     ProfScope ps(w, PK_SORT, 40.0 * n);
-    const int ng = ks.ng_pt + ks.ng_ee + (w.V >> GS) + 1;
-    w.hcnt.reserve((size_t)ng + 1);
+    const int ng = ks.ng_pt + ks.ng_ee + (w.V >> ks.gs) + 1;
+    // the counts are zero between sorts (cleared by the fill pass), once set here
+    if (w.hcnt.cap < (size_t)ng + 1) {
+        w.hcnt.reserve((size_t)ng + 1);
+        ADB_HIP(hipMemsetAsync(w.hcnt.p, 0, w.hcnt.cap * sizeof(int), w.stream));
+    }
     w.hend.reserve((size_t)ng + 1);
-    ADB_HIP(hipMemsetAsync(w.hcnt.p, 0, (size_t)(ng + 1) * sizeof(int), w.stream));
     const dim3 grid(std::max(1, std::min(64, (most + 255) / 256)), 2 * NBK + 1);
-    hipLaunchKernelGGL(k_group_keys<false>, grid, dim3(256), 0, w.stream, ks, w.hcnt.p, nullptr);
+    hipLaunchKernelGGL(k_group_keys<false>, grid, dim3(256), 0, w.stream, ks, w.hcnt.p, nullptr, nullptr, 0);
     ADB_HIP(hipGetLastError());
     size_t tb = 0;
     ADB_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tb, w.hcnt.p, w.hend.p, ng + 1, w.stream));
     w.sort_tmp.reserve(tb + 16);
     ADB_HIP(hipcub::DeviceScan::InclusiveSum(w.sort_tmp.p, tb, w.hcnt.p, w.hend.p, ng + 1, w.stream));
     // hend[g] = end of group g; after the fill its start, and hend[ng] = n
-    hipLaunchKernelGGL(k_group_keys<true>, grid, dim3(256), 0, w.stream, ks, w.hend.p, tmp);
+    hipLaunchKernelGGL(k_group_keys<true>, grid, dim3(256), 0, w.stream, ks, w.hend.p, tmp, w.hcnt.p, ng + 1);
     ADB_HIP(hipGetLastError());
-    // within a group the keys differ in their low 32 + GS bits (VG keys: GS)
+    SetOut o;
+    o.vbody = w.vbody.p; o.voff = w.voff.p; o.fbody = w.fbody.p; o.foff = w.foff.p; o.ebody = w.ebody.p; o.eoff = w.eoff.p;
+    o.pt = out.pt.p; o.ee = out.ee.p; o.vg = out.vg.p;
+    if (ks.gs == GS_RANK) {
+        hipLaunchKernelGGL(k_group_rank, dim3((n + 255) / 256), dim3(256), 0, w.stream, ks, npt, nee, n, w.hend.p, tmp, o);
+        ADB_HIP(hipGetLastError());
+        return;
+    }
+    // within a group the keys differ in their low 32 + gs bits (VG keys: gs)
     tb = 0;
-    ADB_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, tmp, out, n, ng, w.hend.p, w.hend.p + 1, 0, 32 + GS, w.stream));
-    w.sort_tmp.reserve(tb + 16);
-    ADB_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(w.sort_tmp.p, tb, tmp, out, n, ng, w.hend.p, w.hend.p + 1, 0, 32 + GS,
+    ADB_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, tb, tmp, sorted, n, ng, w.hend.p, w.hend.p + 1, 0, 32 + ks.gs,
                                                        w.stream));
+    w.sort_tmp.reserve(tb + 16);
+    ADB_HIP(hipcub::DeviceSegmentedRadixSort::SortKeys(w.sort_tmp.p, tb, tmp, sorted, n, ng, w.hend.p, w.hend.p + 1, 0, 32 + ks.gs,
+                                                       w.stream));
+    hipLaunchKernelGGL(k_decode_all, dim3((n + 255) / 256), dim3(256), 0, w.stream, npt, nee, n, sorted, o);
+    ADB_HIP(hipGetLastError());
 }
 
 // body boxes + pairs; the pair count stays on the device (*cnt, which may
-// exceed the capacity: the caller checks it with its other counters)
-static void body_pairs(World& w, const double* X0, const double* X1, double infl, int* cnt) {
+// exceed the capacity: the caller checks it with its other counters).  The
+// box kernel clears zero[0, nzero) (which holds *cnt) and sets toi (if given).
+static void body_pairs(World& w, const double* X0, const double* X1, double infl, int* cnt, int* zero, int nzero,
+                       unsigned long long* toi = nullptr) {
     w.bbox.reserve(6 * (size_t)w.B);
     w.bsph.reserve(4 * (size_t)w.B);
     {
         ProfScope ps(w, PK_BODY_BOX, (X1 ? 48.0 : 24.0) * w.V + 52.0 * w.B);
-        hipLaunchKernelGGL(k_body_box, dim3(w.B), dim3(NB_THREADS), 0, w.stream, w.voff.p, X0, X1, infl, w.bbox.p, w.bsph.p);
+        hipLaunchKernelGGL(k_body_box, dim3(w.B), dim3(NB_THREADS), 0, w.stream, w.voff.p, X0, X1, infl, w.bbox.p, w.bsph.p,
+                           zero, nzero, toi, order_key_host(1.0));
     }
     ADB_HIP(hipGetLastError());
+    if (cnt < zero || cnt >= zero + nzero) throw AdbError(ADB_E_ARG, "body-pair counter outside the cleared range");
     w.pairs.reserve(std::max<size_t>(w.pairs.cap, std::max(4096, 16 * w.B)));
-    ADB_HIP(hipMemsetAsync(cnt, 0, sizeof(int), w.stream));
     if (w.sweep) {
         const int B = w.B;
         w.swk.reserve(B); w.swk_s.reserve(B); w.swi.reserve(B); w.swi_s.reserve(B);
@@ -776,8 +879,8 @@ static void set_pass(World& w, const double* X, const double* X1, ContactSet& ou
         w.keys_a.reserve(2 * pcap);
         w.keys_b.reserve(2 * (2 * pcap + vcap));      // sorted PT, EE, VG; sort scratch
         w.vg_keys.reserve(vcap);
-        ADB_HIP(hipMemsetAsync(w.bcnt.p, 0, (2 * NBK + 2) * sizeof(int), w.stream));
-        if (!hashed) body_pairs(w, X, X1, r, pcnt);
+        if (hashed) ADB_HIP(hipMemsetAsync(w.bcnt.p, 0, (2 * NBK + 2) * sizeof(int), w.stream));
+        else body_pairs(w, X, X1, r, pcnt, w.bcnt.p, 2 * NBK + 2);   // also clears the bucket counters
         NarrowArgs a = narrow_args(w, pcnt, X, X1, r);
         a.pt_keys = w.keys_a.p; a.ee_keys = w.keys_a.p + pcap; a.sub = (int)sub; a.bcnt = w.bcnt.p;
         if (hashed) {
@@ -824,31 +927,18 @@ static void set_pass(World& w, const double* X, const double* X1, ContactSet& ou
             Le.cnt[b] = h[NBK + b]; Le.off[b] = nee; nee += h[NBK + b];
         }
         out.pt.reserve(std::max(npt, 1)); out.ee.reserve(std::max(nee, 1)); out.vg.reserve(std::max(nvg, 1));
-        // sort all three sets into keys_b (its second half is scratch), decode
+        // sort all three sets (grouped in keys_b's second half, sorted in its
+        // first half when radix-sorted) and decode them into out
         const size_t half = 2 * pcap + vcap;
         const int nall = npt + nee + nvg;
         if (nall > 0) {
             KeySets ks;
             ks.L[0] = Lp; ks.L[1] = Le;
             ks.sub = (int)sub; ks.nvg = nvg; ks.V = w.V; ks.E = std::max(w.E, 1);
-            ks.ng_pt = (w.V >> GS) + 1; ks.ng_ee = (ks.E >> GS) + 1;
+            ks.gs = nall <= RANK_MAX ? GS_RANK : GS_RADIX;
+            ks.ng_pt = (w.V >> ks.gs) + 1; ks.ng_ee = (ks.E >> ks.gs) + 1;
             ks.buckets = w.keys_a.p; ks.vg = w.vg_keys.p;
-            sort_keys(w, ks, nall, std::max(most, nvg), w.keys_b.p + half, w.keys_b.p);
-        }
-        if (npt > 0) {
-            hipLaunchKernelGGL(k_decode4, dim3((npt + 255) / 256), dim3(256), 0, w.stream, npt, w.keys_b.p, w.vbody.p, w.voff.p,
-                               w.fbody.p, w.foff.p, out.pt.p);
-            ADB_HIP(hipGetLastError());
-        }
-        if (nee > 0) {
-            hipLaunchKernelGGL(k_decode4, dim3((nee + 255) / 256), dim3(256), 0, w.stream, nee, w.keys_b.p + npt, w.ebody.p, w.eoff.p,
-                               w.ebody.p, w.eoff.p, out.ee.p);
-            ADB_HIP(hipGetLastError());
-        }
-        if (nvg > 0) {
-            hipLaunchKernelGGL(k_decode2, dim3((nvg + 255) / 256), dim3(256), 0, w.stream, nvg, w.keys_b.p + npt + nee, w.vbody.p,
-                               w.voff.p, out.vg.p);
-            ADB_HIP(hipGetLastError());
+            sort_keys(w, ks, npt, nee, nall, std::max(most, nvg), w.keys_b.p + half, w.keys_b.p, out);
         }
         out.n_pt = npt; out.n_ee = nee; out.n_vg = nvg;
         return;
@@ -977,25 +1067,28 @@ void trial_set(World& w, const double* X, const ContactSet& cand, ContactSet& ou
 
 double ccd_toi(World& w, const double* X0, const double* X1, int* status) {
     *status = ADB_OK;
-    w.toi_bits.reserve(2);
-    const unsigned long long one = order_key_host(1.0);
-    unsigned long long init[2] = {one, one};
-    ADB_HIP(hipMemcpyAsync(w.toi_bits.p, init, sizeof init, hipMemcpyHostToDevice, w.stream));
-    if (w.P.enable_ground) {
-        ProfScope ps(w, PK_VG_TOI, 16.0 * w.V);
-        hipLaunchKernelGGL(k_vg_toi, dim3((w.V + 255) / 256), dim3(256), 0, w.stream, w.V, X0, X1, w.P.ground_y, w.toi_bits.p,
-                           w.vbody.p, (const int*)nullptr);
-        ADB_HIP(hipGetLastError());
-    }
+    // toi_bits: the two bounds, then the body-pair counter (one read for all)
+    w.toi_bits.reserve(3);
     const bool hashed = !(w.P.iaabb & 2);    // spatial hashing (hash.hip) instead of body overlaps
-    w.bcnt.reserve(2 * NBK + 2);
-    int* pcnt = w.bcnt.p + 2 * NBK + 1;
-    int* hs = small_host(w);                  // [0]: pair count; [2, 6): the two 64-bit bounds
+    int* pcnt = reinterpret_cast<int*>(w.toi_bits.p + 2);
+    int* hs = small_host(w);                  // [2, 6): the two 64-bit bounds; [6]: pair count
     unsigned long long* bits = reinterpret_cast<unsigned long long*>(hs + 2);
     for (int attempt = 0; attempt < 2; attempt++) {
         int np = 0;
-        hs[0] = 0;
-        if (!hashed) body_pairs(w, X0, X1, 0.0, pcnt);
+        // the bounds start at 1 (set by the box kernel, or copied when hashing)
+        if (!hashed) {
+            body_pairs(w, X0, X1, 0.0, pcnt, pcnt, 1, w.toi_bits.p);
+        } else {
+            const unsigned long long one = order_key_host(1.0);
+            const unsigned long long init[2] = {one, one};
+            ADB_HIP(hipMemcpyAsync(w.toi_bits.p, init, sizeof init, hipMemcpyHostToDevice, w.stream));
+        }
+        if (w.P.enable_ground) {
+            ProfScope ps(w, PK_VG_TOI, 16.0 * w.V);
+            hipLaunchKernelGGL(k_vg_toi, dim3((w.V + 255) / 256), dim3(256), 0, w.stream, w.V, X0, X1, w.P.ground_y, w.toi_bits.p,
+                               w.vbody.p, (const int*)nullptr);
+            ADB_HIP(hipGetLastError());
+        }
         NarrowArgs a = narrow_args(w, pcnt, X0, X1, 0.0);
         a.toi_bits = w.toi_bits.p;
         if (hashed) {
@@ -1006,11 +1099,10 @@ double ccd_toi(World& w, const double* X0, const double* X1, int* status) {
             if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<1, float>), grid, dim3(NB_THREADS), 0, w.stream, a);
             else hipLaunchKernelGGL((k_pair_narrow<1, double>), grid, dim3(NB_THREADS), 0, w.stream, a);
             ADB_HIP(hipGetLastError());
-            ADB_HIP(hipMemcpyAsync(hs, pcnt, sizeof(int), hipMemcpyDeviceToHost, w.stream));
         }
-        ADB_HIP(hipMemcpyAsync(bits, w.toi_bits.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipMemcpyAsync(bits, w.toi_bits.p, (hashed ? 2 : 3) * sizeof(unsigned long long), hipMemcpyDeviceToHost, w.stream));
         wait_stream(w);
-        np = hs[0];
+        np = hashed ? 0 : hs[6];
         if (!hashed) w.pair_hint = np;
         if ((size_t)np <= w.pairs.cap) break;
         w.pairs.reserve(np);               // more overlapping body pairs than slots: grow, test again
@@ -1039,7 +1131,7 @@ void ccd_toi_scenes(World& w, const double* X0, const double* X1, double* toi, i
     int* pcnt = w.bcnt.p + 2 * NBK + 1;
     for (int attempt = 0; attempt < 2; attempt++) {
         int np = 0;
-        body_pairs(w, X0, X1, 0.0, pcnt);
+        body_pairs(w, X0, X1, 0.0, pcnt, pcnt, 1);
         NarrowArgs a = narrow_args(w, pcnt, X0, X1, 0.0);
         a.toi_bits = w.toi_bits.p;
         a.scene_of = w.scene_of.p;

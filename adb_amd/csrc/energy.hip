@@ -20,10 +20,9 @@ struct EnergyArgs {
     double* out;
 };
 
-__global__ void k_energy_items(EnergyArgs a) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = a.B + a.n_pt + a.n_ee + a.n_vg;
-    if (i >= n) return;
+// energy of item i < B + n_pt + n_ee + n_vg: body i (inertia and
+// orthogonality), then the PT, EE and VG barriers
+__device__ __forceinline__ double energy_item(const EnergyArgs& a, int i) {
     double e = 0.0;
     if (i < a.B) {
         if (a.mass[i] >= 0.0) {
@@ -70,7 +69,13 @@ __global__ void k_energy_items(EnergyArgs a) {
             if (d < a.d_hat) e = barrier(d, a.d_hat, a.kappa);
         }
     }
-    a.out[i] = e;
+    return e;
+}
+__global__ void k_energy_items(EnergyArgs a) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = a.B + a.n_pt + a.n_ee + a.n_vg;
+    if (i >= n) return;
+    a.out[i] = energy_item(a, i);
 }
 
 struct FricArgs {
@@ -83,10 +88,8 @@ struct FricArgs {
     const double* tk; const double* lam; const double* vglam;
     double* out;
 };
-__global__ void k_fric_items(FricArgs a) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = a.n_pt + a.n_ee + a.n_vg;
-    if (i >= n) return;
+// friction energy of item i < n_pt + n_ee + n_vg (lagged terms)
+__device__ __forceinline__ double fric_item(const FricArgs& a, int i) {
     double e = 0.0;
     if (i < a.n_pt + a.n_ee) {
         const bool is_ee = i >= a.n_pt;
@@ -120,7 +123,13 @@ __global__ void k_fric_items(FricArgs a) {
         double uz = a.X[3 * (size_t)v + 2] - a.Xs[3 * (size_t)v + 2];
         e = a.mu * a.vglam[k] * f0_sf(sqrt(ux * ux + uz * uz), a.eps_v);
     }
-    a.out[i] = e;
+    return e;
+}
+__global__ void k_fric_items(FricArgs a) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = a.n_pt + a.n_ee + a.n_vg;
+    if (i >= n) return;
+    a.out[i] = fric_item(a, i);
 }
 
 // two-level fixed-shape reduction
@@ -149,6 +158,43 @@ __global__ __launch_bounds__(RT) void k_sum2(int G, const double* __restrict__ p
         __syncthreads();
     }
     if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// The barrier + inertia items (blocks [0, G1)) and the friction items
+// (blocks [G1, gridDim.x)) of one evaluation in one launch, one item per
+// thread, each block's tree sum written to part1 / part2: the partials that
+// k_sum1 forms over the item buffers when no thread holds two items.
+__global__ __launch_bounds__(RT) void k_energy_parts(EnergyArgs ea, int G1, FricArgs fa, double* __restrict__ part1,
+                                                     double* __restrict__ part2) {
+    __shared__ double sh[RT];
+    const bool fr = (int)blockIdx.x >= G1;
+    const int i = (fr ? (int)blockIdx.x - G1 : (int)blockIdx.x) * RT + threadIdx.x;
+    double s = 0.0;
+    if (!fr) { if (i < ea.B + ea.n_pt + ea.n_ee + ea.n_vg) s += energy_item(ea, i); }
+    else if (i < fa.n_pt + fa.n_ee + fa.n_vg) s += fric_item(fa, i);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int m = RT / 2; m >= 1; m >>= 1) {
+        if ((int)threadIdx.x < m) sh[threadIdx.x] += sh[threadIdx.x + m];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) (fr ? part2[blockIdx.x - G1] : part1[blockIdx.x]) = sh[0];
+}
+// k_sum2 over two partial lists at once (block 0: list 1, block 1: list 2)
+__global__ __launch_bounds__(RT) void k_sum2x(int G1, const double* __restrict__ part1, double* __restrict__ out1, int G2,
+                                             const double* __restrict__ part2, double* __restrict__ out2) {
+    __shared__ double sh[RT];
+    const int G = blockIdx.x ? G2 : G1;
+    const double* part = blockIdx.x ? part2 : part1;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < G; i += RT) s += part[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int m = RT / 2; m >= 1; m >>= 1) {
+        if ((int)threadIdx.x < m) sh[threadIdx.x] += sh[threadIdx.x + m];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) (blockIdx.x ? out2 : out1)[0] = sh[0];
 }
 
 // Sum of a[i] (b == null) or a[i] b[i] into w.epart[slot] (slots 0..7), on
@@ -180,10 +226,7 @@ void dot_async(World& w, const double* a, const double* b, int n, int slot) { re
 
 static DBuf<double>& items(World& w) { return w.pcg_Ap; }   // scratch reused outside the solve
 
-void energy_barrier_inertia_async(World& w, const double* qv, const double* X, const ContactSet& cs, int slot) {
-    const int n = w.B + cs.n_pt + cs.n_ee + cs.n_vg;
-    DBuf<double>& buf = items(w);
-    buf.reserve(std::max(n, 12 * w.B));
+static EnergyArgs energy_args(World& w, const double* qv, const double* X, const ContactSet& cs) {
     EnergyArgs a;
     a.B = w.B; a.n_pt = cs.n_pt; a.n_ee = cs.n_ee; a.n_vg = cs.n_vg;
     a.dt = w.P.dt; a.kap_o = w.P.kappa_ortho; a.d_hat = w.P.d_hat; a.kappa = w.P.kappa; a.eps_x = w.P.eps_x;
@@ -192,6 +235,28 @@ void energy_barrier_inertia_async(World& w, const double* qv, const double* X, c
     a.qv = qv; a.q0 = w.q0.p; a.qdot = w.qdot.p; a.mass = w.mass.p; a.Ic = w.Ic.p; a.X = X;
     a.pt = cs.pt.p; a.ee = cs.ee.p; a.vg = cs.vg.p;
     a.voff = w.voff.p; a.foff = w.foff.p; a.eoff = w.eoff.p; a.face = w.face.p; a.edge = w.edge.p;
+    a.out = nullptr;
+    return a;
+}
+static FricArgs fric_args(World& w, const double* X) {
+    const ContactSet& cs = w.cs;
+    FricArgs a;
+    a.n_pt = cs.n_pt; a.n_ee = cs.n_ee; a.n_vg = cs.n_vg;
+    a.mu = w.P.mu; a.eps_v = w.P.dt * w.P.evh;
+    a.pt_f = w.P.pt_fric; a.ee_f = w.P.ee_fric; a.vg_f = w.P.vg_fric;
+    a.en_pt = w.P.enable_pt; a.en_ee = w.P.enable_ee; a.en_g = w.P.enable_ground;
+    a.X = X; a.Xs = w.Xs.p; a.pt = cs.pt.p; a.ee = cs.ee.p; a.vg = cs.vg.p;
+    a.voff = w.voff.p; a.foff = w.foff.p; a.eoff = w.eoff.p; a.face = w.face.p; a.edge = w.edge.p;
+    a.tk = w.ctk.p; a.lam = w.clam.p; a.vglam = w.vglam.p;
+    a.out = nullptr;
+    return a;
+}
+
+void energy_barrier_inertia_async(World& w, const double* qv, const double* X, const ContactSet& cs, int slot) {
+    const int n = w.B + cs.n_pt + cs.n_ee + cs.n_vg;
+    DBuf<double>& buf = items(w);
+    buf.reserve(std::max(n, 12 * w.B));
+    EnergyArgs a = energy_args(w, qv, X, cs);
     a.out = buf.p;
     ProfScope ps(w, PK_ENERGY, 0.0);
     hipLaunchKernelGGL(k_energy_items, dim3((n + 255) / 256), dim3(256), 0, w.stream, a);
@@ -205,18 +270,42 @@ void energy_friction_async(World& w, const double* X, int slot) {
     if (n == 0) { reduce_async(w, nullptr, nullptr, 0, slot); return; }
     DBuf<double>& buf = items(w);
     buf.reserve(std::max(n, 12 * w.B));
-    FricArgs a;
-    a.n_pt = cs.n_pt; a.n_ee = cs.n_ee; a.n_vg = cs.n_vg;
-    a.mu = w.P.mu; a.eps_v = w.P.dt * w.P.evh;
-    a.pt_f = w.P.pt_fric; a.ee_f = w.P.ee_fric; a.vg_f = w.P.vg_fric;
-    a.en_pt = w.P.enable_pt; a.en_ee = w.P.enable_ee; a.en_g = w.P.enable_ground;
-    a.X = X; a.Xs = w.Xs.p; a.pt = cs.pt.p; a.ee = cs.ee.p; a.vg = cs.vg.p;
-    a.voff = w.voff.p; a.foff = w.foff.p; a.eoff = w.eoff.p; a.face = w.face.p; a.edge = w.edge.p;
-    a.tk = w.ctk.p; a.lam = w.clam.p; a.vglam = w.vglam.p;
+    FricArgs a = fric_args(w, X);
     a.out = buf.p;
     hipLaunchKernelGGL(k_fric_items, dim3((n + 255) / 256), dim3(256), 0, w.stream, a);
     ADB_HIP(hipGetLastError());
     reduce_async(w, buf.p, nullptr, n, slot);
+}
+
+// Barrier + inertia energy at (qv, X) over *cs into slot_bi (cs null: not
+// evaluated) and friction energy at X (lagged terms of w.cs) into slot_f.
+// When every item list fits one item per thread in EPART_MAXG blocks, two
+// launches (k_energy_parts, k_sum2x) with the operations of the per-item
+// buffers and reduce_async; else those.
+void energy_async(World& w, const double* qv, const double* X, const ContactSet* cs, int slot_bi, int slot_f) {
+    const int n1 = cs ? w.B + cs->n_pt + cs->n_ee + cs->n_vg : 0;
+    const int n2 = w.cs.n_pt + w.cs.n_ee + w.cs.n_vg;
+    const int G1 = (n1 + RT - 1) / RT, G2 = (n2 + RT - 1) / RT;
+    if (G1 > EPART_MAXG || G2 > EPART_MAXG) {
+        if (cs) energy_barrier_inertia_async(w, qv, X, *cs, slot_bi);
+        energy_friction_async(w, X, slot_f);
+        return;
+    }
+    if (w.epart.cap < (size_t)EPART_SLOTS) throw AdbError(ADB_E_ARG, "reduction slots not reserved");
+    EnergyArgs ea = cs ? energy_args(w, qv, X, *cs) : EnergyArgs{};
+    const FricArgs fa = fric_args(w, X);
+    double* part1 = w.epart.p + NSLOT;
+    double* part2 = w.epart.p + NSLOT + EPART_MAXG;
+    {
+        ProfScope ps(w, PK_ENERGY, 0.0);
+        if (G1 + G2 > 0)
+            hipLaunchKernelGGL(k_energy_parts, dim3(G1 + G2), dim3(RT), 0, w.stream, ea, G1, fa, part1, part2);
+        ADB_HIP(hipGetLastError());
+    }
+    // block 0 sums list 1 (or, without cs, list 2 into slot_f)
+    if (cs) hipLaunchKernelGGL(k_sum2x, dim3(2), dim3(RT), 0, w.stream, G1, part1, w.epart.p + slot_bi, G2, part2, w.epart.p + slot_f);
+    else hipLaunchKernelGGL(k_sum2x, dim3(1), dim3(RT), 0, w.stream, G2, part2, w.epart.p + slot_f, 0, part2, w.epart.p + slot_f);
+    ADB_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------- per scene

@@ -305,7 +305,7 @@ static void enqueue_batch(World& w, int nb, bool prof) {
             // group inverses read; p, y read; x, r read and written; z written
             ProfScope ps(w, PK_PCG_UPDATE, prof ? 8.0 * (double)w.h_grp_moff.back() + 7.0 * 96.0 * B : 0.0);
             hipLaunchKernelGGL(k_pcg_update, dim3(B), dim3(UPD_T), 0, w.stream, B, w.pcg_k.p, k, nbS, G, pb[(k & 1) ^ 1],
-                               w.pcg_Ap.p, w.pcg_x.p, w.pcg_r.p, w.pcg_z.p, partA, partB, w.scal.p);
+                               w.pcg_Ap.p, w.pcg_xo, w.pcg_r.p, w.pcg_z.p, partA, partB, w.scal.p);
         }
         if (w.S > 1)
             hipLaunchKernelGGL(k_pcg_scene_stop, dim3(1), dim3(1024), 0, w.stream, w.S, B, w.sc_goff.p, rtol2, 0, partB, w.pcg_sbb.p,
@@ -317,13 +317,15 @@ static void enqueue_batch(World& w, int nb, bool prof) {
 int pcg_solve(World& w, double* dq_out, int* iters) {
     const int B = w.B, n = 12 * B;
     const int nbS = (B + SP_ROWS - 1) / SP_ROWS;
-    w.pcg_x.reserve(n); w.pcg_r.reserve(n); w.pcg_z.reserve(n); w.pcg_p.reserve(n); w.pcg_p2.reserve(n); w.pcg_Ap.reserve(n);
+    // the iterate x lives in the caller's output (12 B doubles)
+    w.pcg_xo = dq_out;
+    w.pcg_r.reserve(n); w.pcg_z.reserve(n); w.pcg_p.reserve(n); w.pcg_p2.reserve(n); w.pcg_Ap.reserve(n);
     w.partial.reserve((size_t)nbS + 2 * (size_t)B); w.scal.reserve(S_COUNT); w.pcg_k.reserve(1);
     double* partB = w.partial.p + nbS;
     if (!w.h_scal) ADB_HIP(hipHostMalloc((void**)&w.h_scal, 2 * S_COUNT * sizeof(double), hipHostMallocDefault));
     for (hipEvent_t& e : w.pcg_ev)
         if (!e) ADB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    hipLaunchKernelGGL(k_pcg_init, dim3(B), dim3(64), 0, w.stream, B, groups_of(w), w.grad.p, w.pcg_x.p, w.pcg_r.p, w.pcg_z.p,
+    hipLaunchKernelGGL(k_pcg_init, dim3(B), dim3(64), 0, w.stream, B, groups_of(w), w.grad.p, w.pcg_xo, w.pcg_r.p, w.pcg_z.p,
                        w.pcg_p.p, partB, w.scal.p, w.pcg_k.p);
     ADB_HIP(hipGetLastError());
     if (w.S > 1) {
@@ -337,7 +339,7 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
     // not support graph capture)
     const bool use_graph = !w.prof.on && w.use_graphs;
     if (use_graph) {
-        const void* key[] = {w.row_ptr.p, w.col_idx.p, w.Hval.p, w.Ginv.p, w.pcg_x.p, w.pcg_r.p, w.pcg_z.p, w.pcg_p.p,
+        const void* key[] = {w.row_ptr.p, w.col_idx.p, w.Hval.p, w.Ginv.p, w.pcg_xo, w.pcg_r.p, w.pcg_z.p, w.pcg_p.p,
                              w.pcg_p2.p, w.pcg_Ap.p, w.partial.p, w.scal.p, w.pcg_k.p, w.grp_off, w.grp_body, w.grp_moff,
                              w.grp_n, w.sc_goff.p, w.pcg_sbb.p, w.pcg_brun.p};
         static_assert(sizeof key / sizeof key[0] == World::PCG_GRAPH_KEYS, "graph key size");
@@ -416,7 +418,6 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
     w.pcg_prev_its = stop == RUN ? it : (int)h[S_ITERS];
     if (iters) *iters = w.pcg_prev_its;
     if (stop == NONFINITE) return ADB_E_NAN;
-    ADB_HIP(hipMemcpyAsync(dq_out, w.pcg_x.p, n * sizeof(double), hipMemcpyDeviceToDevice, w.stream));
     if (stop == RUN) return ADB_E_NOT_CONVERGED;
     if (stop == BREAKDOWN) return ADB_E_SOLVER;
     return ADB_OK;

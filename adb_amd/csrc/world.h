@@ -59,7 +59,7 @@ constexpr int R_D = 0, R_F = 1, R_G = 10, R_Z = 18, R_GRAD = 42, R_PK = 66;
 // Reduction result slots (8) + per-workgroup partials (at most 1024 workgroups):
 // reserved once at world creation, so no reduction queued behind another can
 // see its slots reallocated.
-constexpr int EPART_NSLOT = 8, EPART_MAXG = 1024, EPART_SLOTS = EPART_NSLOT + EPART_MAXG;
+constexpr int EPART_NSLOT = 8, EPART_MAXG = 1024, EPART_SLOTS = EPART_NSLOT + 2 * EPART_MAXG;
 
 // Per-kernel HIP-event timing on the world's stream (off unless enabled).
 enum ProfKernel {
@@ -190,7 +190,8 @@ struct World {
     std::vector<long long> h_grp_moff;
     int nnzb = 0;
     // ---- PCG
-    DBuf<double> pcg_x, pcg_r, pcg_z, pcg_p, pcg_p2, pcg_Ap, partial, scal;
+    DBuf<double> pcg_r, pcg_z, pcg_p, pcg_p2, pcg_Ap, partial, scal;
+    double* pcg_xo = nullptr;        // the solve's iterate: the caller's output buffer
     DBuf<int> pcg_k;                 // iteration base of the queued PCG batch
     int pcg_prev_its = 0;            // iterations of the previous solve
     int* h_pairs = nullptr;          // pinned host copy of the pair lists (lo, hi, start)
@@ -273,6 +274,8 @@ const int* prof_snapshot(World& w, const int* dev_counts);
 // ---- stage launchers (implemented in the .hip files)
 void transform_positions(World& w, const double* q, double* X);                         // cull.hip
 void q_axpy(World& w, const double* q, const double* dq, double alpha, double* out);     // cull.hip
+void transform_step(World& w, const double* q, const double* dq, double alpha, double* qout, double* X,
+                    double* X2 = nullptr);                                                  // cull.hip
 void collision_set(World& w, const double* X, ContactSet& out);                          // cull.hip
 // PT / EE candidates of every set on the segment X0 -> X1 (cull.hip), and a
 // set at a point of that segment filtered from them (VG by a full scan)
@@ -298,6 +301,7 @@ int pcg_solve(World& w, double* dq_out, int* iters);                            
 // reductions into result slots 0..7 on the world stream; read_slots waits and copies slots [0, k)
 void energy_barrier_inertia_async(World& w, const double* qv, const double* X, const ContactSet& cs, int slot); // energy.hip
 void energy_friction_async(World& w, const double* X, int slot);                        // energy.hip
+void energy_async(World& w, const double* qv, const double* X, const ContactSet* cs, int slot_bi, int slot_f); // energy.hip
 void dot_async(World& w, const double* a, const double* b, int n, int slot);            // energy.hip
 void read_slots(World& w, int k, double* out);                                          // energy.hip
 void copy_slots_async(World& w, int first, int k);    // energy.hip: slots [first, first + k) into w.h_slots, no wait

@@ -74,8 +74,7 @@ static int line_search(World& w, double gn, double* alpha_out, int* trials_out, 
     // the same q, positions and contact set, so the same value bit for bit.
     // The friction part is recomputed: its lagged terms are this iteration's.
     const bool have_bi = w.e0_valid;
-    if (!have_bi) energy_barrier_inertia_async(w, w.q.p, w.X.p, w.cs, 5);
-    energy_friction_async(w, w.X.p, 6);
+    energy_async(w, w.q.p, w.X.p, have_bi ? nullptr : &w.cs, 5, 6);
     dot_async(w, w.dq.p, w.grad.p, n, 2);
     double E0 = 0.0, qdg = 0.0, E1 = 0.0, alpha = 1.0, e1_bi = 0.0;
     bool have_e0 = false;
@@ -89,21 +88,18 @@ static int line_search(World& w, double gn, double* alpha_out, int* trials_out, 
     const bool cand_ok = w.ls_cand && (w.P.iaabb & 1);
     bool cand = cand_ok && w.ls_prev > 1;
     if (cand) {
-        q_axpy(w, w.q.p, w.dq.p, 1.0, w.qtrial.p);
-        transform_positions(w, w.qtrial.p, w.X1.p);
+        transform_step(w, w.q.p, w.dq.p, 1.0, w.qtrial.p, w.X1.p);
         segment_candidates(w, w.X.p, w.X1.p, w.cand);
     }
     while (true) {
-        q_axpy(w, w.q.p, w.dq.p, alpha, w.qtrial.p);
-        transform_positions(w, w.qtrial.p, w.X1.p);
+        transform_step(w, w.q.p, w.dq.p, alpha, w.qtrial.p, w.X1.p);
         if (!cand && cand_ok && trials > 0) {
             segment_candidates(w, w.X.p, w.X1.p, w.cand);
             cand = true;
         }
         if (cand) trial_set(w, w.X1.p, w.cand, w.cs_new);
         else collision_set(w, w.X1.p, w.cs_new);
-        energy_barrier_inertia_async(w, w.qtrial.p, w.X1.p, w.cs_new, 0);
-        energy_friction_async(w, w.X1.p, 1);
+        energy_async(w, w.qtrial.p, w.X1.p, &w.cs_new, 0, 1);
         read_slots(w, have_e0 ? 2 : 7, sl);
         if (!have_e0) {
             E0 = (have_bi ? w.e0_bi : sl[5]) + sl[6];
@@ -131,9 +127,8 @@ static int step(World& w, adb_step_stats* st) {
     S.toi_min = 1.0;
     const double t_start = now_s();
     const int n = 12 * w.B;
-    ADB_HIP(hipMemcpyAsync(w.q.p, w.q0.p, n * sizeof(double), hipMemcpyDeviceToDevice, w.stream));
-    transform_positions(w, w.q.p, w.X.p);
-    transform_positions(w, w.q0.p, w.Xs.p);
+    // q = q0, and X = Xs = the positions at q0, in one launch
+    transform_step(w, w.q0.p, nullptr, 0.0, w.q.p, w.X.p, w.Xs.p);
     collision_set(w, w.X.p, w.cs);
     w.e0_valid = false;
     int status = ADB_OK;
@@ -160,8 +155,7 @@ static int step(World& w, adb_step_stats* st) {
         int cs_status = ADB_OK;
         if (w.P.upper_bound) {
             // CCD step bound (model/abd.cpp:142-196)
-            q_axpy(w, w.q.p, w.dq.p, 1.0, w.qtrial.p);
-            transform_positions(w, w.qtrial.p, w.X1.p);
+            transform_step(w, w.q.p, w.dq.p, 1.0, w.qtrial.p, w.X1.p);
             toi = ccd_toi(w, w.X.p, w.X1.p, &cs_status);
         } else {
             wait_stream(w);
@@ -195,8 +189,7 @@ static int step(World& w, adb_step_stats* st) {
             S.ls_trials += trials;
             if (status != ADB_OK) break;
         } else {
-            q_axpy(w, w.q.p, w.dq.p, 1.0, w.qtrial.p);
-            transform_positions(w, w.qtrial.p, w.X1.p);
+            transform_step(w, w.q.p, w.dq.p, 1.0, w.qtrial.p, w.X1.p);
             collision_set(w, w.X1.p, w.cs_new);
             w.e0_valid = false;
         }
@@ -547,7 +540,7 @@ static void free_world(World& w) {
     for (auto* b : ib) b->free_();
     DBuf<double>* db[] = {&w.xbar, &w.mass, &w.Ic, &w.q, &w.q0, &w.qdot, &w.dq, &w.qtrial, &w.X, &w.Xs, &w.X1, &w.bbox, &w.bsph,
                           &w.crec, &w.cdist, &w.chunk_part, &w.ctk, &w.clam, &w.vgK, &w.vglam, &w.bodyH, &w.bodyG, &w.Hval, &w.grad,
-                          &w.Ginv, &w.pcg_x, &w.pcg_r, &w.pcg_z, &w.pcg_p, &w.pcg_p2, &w.pcg_Ap, &w.partial, &w.scal, &w.epart};
+                          &w.Ginv, &w.pcg_r, &w.pcg_z, &w.pcg_p, &w.pcg_p2, &w.pcg_Ap, &w.partial, &w.scal, &w.epart};
     for (auto* b : db) b->free_();
     w.pairs.free_(); w.keys_a.free_(); w.keys_b.free_(); w.hcnt.free_(); w.hend.free_(); w.sort_tmp.free_(); w.toi_bits.free_(); w.vg_keys.free_();
     w.bcnt.free_(); w.sh_cnt.free_(); w.sh_start.free_(); w.sh_ent.free_(); w.sh_rng.free_(); w.sh_total.free_();
