@@ -50,17 +50,37 @@ __global__ void k_axpy(int n, const double* __restrict__ q, const double* __rest
 
 // q' = q + alpha dq (dq null: q' = q) written to qout (thread i < 12 B writes
 // entry i, as k_axpy computes it), and the vertex positions at q' to X (and
-// X2 if given); each vertex thread forms its body's q' itself.
-__global__ void k_transform_q(int V, int n12, const int* __restrict__ vbody, const double* __restrict__ xbar,
-                              const double* __restrict__ q, const double* __restrict__ dq, double alpha, double* __restrict__ qout,
-                              double* __restrict__ X, double* __restrict__ X2) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+// X2 if given).  A workgroup's 256 vertices belong to consecutive bodies: when
+// there are at most TQ_BODIES of them, their q' is formed once into LDS,
+// else every vertex thread forms its body's q'.
+constexpr int TQ_BODIES = 32;
+__global__ __launch_bounds__(256) void k_transform_q(int V, int n12, const int* __restrict__ vbody, const double* __restrict__ xbar,
+                                                     const double* __restrict__ q, const double* __restrict__ dq, double alpha,
+                                                     double* __restrict__ qout, double* __restrict__ X, double* __restrict__ X2) {
+    __shared__ double sq[12 * TQ_BODIES];
+    const int v0 = blockIdx.x * 256, v = v0 + threadIdx.x;
     if (v < n12 && qout) qout[v] = dq ? q[v] + dq[v] * alpha : q[v];
+    if (v0 >= V) return;
+    const int b0 = vbody[v0], nb = vbody[min(v0 + 255, V - 1)] - b0 + 1;
+    const bool staged = nb <= TQ_BODIES;
+    if (staged) {
+        for (int t = threadIdx.x; t < 12 * nb; t += 256) {
+            const size_t k = 12 * (size_t)b0 + t;
+            sq[t] = dq ? q[k] + dq[k] * alpha : q[k];
+        }
+        __syncthreads();
+    }
     if (v >= V) return;
-    const size_t b = 12 * (size_t)vbody[v];
+    const int bv = vbody[v];
+    const size_t b = 12 * (size_t)bv;
     double Q[12];
+    if (staged) {
 #pragma unroll
-    for (int k = 0; k < 12; k++) Q[k] = dq ? q[b + k] + dq[b + k] * alpha : q[b + k];
+        for (int k = 0; k < 12; k++) Q[k] = sq[12 * (bv - b0) + k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 12; k++) Q[k] = dq ? q[b + k] + dq[b + k] * alpha : q[b + k];
+    }
     const double x0 = xbar[3 * (size_t)v], x1 = xbar[3 * (size_t)v + 1], x2 = xbar[3 * (size_t)v + 2];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
@@ -839,9 +859,11 @@ static void body_pairs(World& w, const double* X0, const double* X1, double infl
 }
 
 // workgroups per task of a narrow-phase launch (each loops over the pairs):
-// about the pair count of the previous pass, so most take one pair
-static int narrow_grid(World& w) {
-    const double g = w.pair_hint * w.tune.narrow_per_pair + 64.0;
+// about the pair count of the previous pass of the same kind (contact set,
+// CCD, segment candidates: their inflations and sweeps give very different
+// counts), so most take one pair
+static int narrow_grid(World& w, int kind) {
+    const double g = w.pair_hint[kind] * w.tune.narrow_per_pair + 64.0;
     return std::max(1, (int)std::min<double>((double)w.pairs.cap, g));
 }
 
@@ -887,7 +909,7 @@ static void set_pass(World& w, const double* X, const double* X1, ContactSet& ou
             hash_sets(w, X, r, a.pt_keys, a.ee_keys, (int)sub, w.bcnt.p);
         } else {
             ProfScope ps(w, PK_NARROW_SET, 0.0);
-            const dim3 grid(narrow_grid(w), 2 + w.tune.ee_slices);
+            const dim3 grid(narrow_grid(w, swept ? 2 : 0), 2 + w.tune.ee_slices);
             if (swept) {
                 if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<2, float>), grid, dim3(NB_THREADS), 0, w.stream, a);
                 else hipLaunchKernelGGL((k_pair_narrow<2, double>), grid, dim3(NB_THREADS), 0, w.stream, a);
@@ -907,7 +929,7 @@ static void set_pass(World& w, const double* X, const double* X1, ContactSet& ou
         int* h = small_host(w);
         ADB_HIP(hipMemcpyAsync(h, w.bcnt.p, (2 * NBK + 2) * sizeof(int), hipMemcpyDeviceToHost, w.stream));
         wait_stream(w);
-        if (!hashed) w.pair_hint = h[2 * NBK + 1];
+        if (!hashed) w.pair_hint[swept ? 2 : 0] = h[2 * NBK + 1];
         if (!hashed && (size_t)h[2 * NBK + 1] > w.pairs.cap) {   // more overlapping body pairs than slots
             w.pairs.reserve(h[2 * NBK + 1]);
             continue;
@@ -1095,7 +1117,7 @@ double ccd_toi(World& w, const double* X0, const double* X1, int* status) {
             hash_ccd(w, X0, X1, w.toi_bits.p);
         } else {
             ProfScope ps(w, PK_NARROW_CCD, 0.0);
-            const dim3 grid(narrow_grid(w), 2 + w.tune.ee_slices);
+            const dim3 grid(narrow_grid(w, 1), 2 + w.tune.ee_slices);
             if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<1, float>), grid, dim3(NB_THREADS), 0, w.stream, a);
             else hipLaunchKernelGGL((k_pair_narrow<1, double>), grid, dim3(NB_THREADS), 0, w.stream, a);
             ADB_HIP(hipGetLastError());
@@ -1103,7 +1125,7 @@ double ccd_toi(World& w, const double* X0, const double* X1, int* status) {
         ADB_HIP(hipMemcpyAsync(bits, w.toi_bits.p, (hashed ? 2 : 3) * sizeof(unsigned long long), hipMemcpyDeviceToHost, w.stream));
         wait_stream(w);
         np = hashed ? 0 : hs[6];
-        if (!hashed) w.pair_hint = np;
+        if (!hashed) w.pair_hint[1] = np;
         if ((size_t)np <= w.pairs.cap) break;
         w.pairs.reserve(np);               // more overlapping body pairs than slots: grow, test again
     }
@@ -1137,7 +1159,7 @@ void ccd_toi_scenes(World& w, const double* X0, const double* X1, double* toi, i
         a.scene_of = w.scene_of.p;
         {
             ProfScope ps(w, PK_NARROW_CCD, 0.0);
-            const dim3 grid(narrow_grid(w), 2 + w.tune.ee_slices);
+            const dim3 grid(narrow_grid(w, 1), 2 + w.tune.ee_slices);
             if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<1, float>), grid, dim3(NB_THREADS), 0, w.stream, a);
             else hipLaunchKernelGGL((k_pair_narrow<1, double>), grid, dim3(NB_THREADS), 0, w.stream, a);
             ADB_HIP(hipGetLastError());
@@ -1145,7 +1167,7 @@ void ccd_toi_scenes(World& w, const double* X0, const double* X1, double* toi, i
         ADB_HIP(hipMemcpyAsync(&np, pcnt, sizeof(int), hipMemcpyDeviceToHost, w.stream));
         ADB_HIP(hipMemcpyAsync(bits.data(), w.toi_bits.p, bits.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, w.stream));
         wait_stream(w);
-        w.pair_hint = np;
+        w.pair_hint[1] = np;
         if ((size_t)np <= w.pairs.cap) break;
         w.pairs.reserve(np);
         std::fill(bits.begin(), bits.end(), order_key_host(1.0));

@@ -111,7 +111,8 @@ struct World {
     DBuf<double> bbox;               // 6 per body
     DBuf<double> bsph;               // 4 per body: bounding sphere centre, radius
     DBuf<int2> pairs;                // body pairs (I < J)
-    int pair_hint = 0;               // body-pair count of the last culling pass (narrow-phase grid size)
+    int pair_hint[3] = {0, 0, 0};    // body-pair count of the last culling pass of each kind (narrow-phase grid
+                                     // size): contact set, CCD, segment candidates
     DBuf<int> counters;              // device counters
     DBuf<int> bcnt;                  // contact-set key bucket counters (2 x 64) + VG counter
     DBuf<unsigned long long> keys_a, keys_b;  // bucketed / sorted keys
