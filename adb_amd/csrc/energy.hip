@@ -160,32 +160,57 @@ __global__ __launch_bounds__(RT) void k_sum2(int G, const double* __restrict__ p
     if (threadIdx.x == 0) out[0] = sh[0];
 }
 
-// The barrier + inertia items (blocks [0, G1)) and the friction items
-// (blocks [G1, gridDim.x)) of one evaluation in one launch, one item per
-// thread, each block's tree sum written to part1 / part2: the partials that
-// k_sum1 forms over the item buffers when no thread holds two items.
-__global__ __launch_bounds__(RT) void k_energy_parts(EnergyArgs ea, int G1, FricArgs fa, double* __restrict__ part1,
-                                                     double* __restrict__ part2) {
+// k_sum1 over two inputs at once (blockIdx.y selects a1 b1 -> part1 or
+// a2 b2 -> part2)
+__global__ __launch_bounds__(RT) void k_sum1x(int n, int G, const double* __restrict__ a1, const double* __restrict__ b1,
+                                             double* __restrict__ part1, const double* __restrict__ a2,
+                                             const double* __restrict__ b2, double* __restrict__ part2) {
     __shared__ double sh[RT];
-    const bool fr = (int)blockIdx.x >= G1;
-    const int i = (fr ? (int)blockIdx.x - G1 : (int)blockIdx.x) * RT + threadIdx.x;
+    const double* a = blockIdx.y ? a2 : a1;
+    const double* b = blockIdx.y ? b2 : b1;
     double s = 0.0;
-    if (!fr) { if (i < ea.B + ea.n_pt + ea.n_ee + ea.n_vg) s += energy_item(ea, i); }
-    else if (i < fa.n_pt + fa.n_ee + fa.n_vg) s += fric_item(fa, i);
+    for (long long i = (long long)blockIdx.x * RT + threadIdx.x; i < n; i += (long long)G * RT) s += b ? a[i] * b[i] : a[i];
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int m = RT / 2; m >= 1; m >>= 1) {
         if ((int)threadIdx.x < m) sh[threadIdx.x] += sh[threadIdx.x + m];
         __syncthreads();
     }
-    if (threadIdx.x == 0) (fr ? part2[blockIdx.x - G1] : part1[blockIdx.x]) = sh[0];
+    if (threadIdx.x == 0) (blockIdx.y ? part2 : part1)[blockIdx.x] = sh[0];
 }
-// k_sum2 over two partial lists at once (block 0: list 1, block 1: list 2)
-__global__ __launch_bounds__(RT) void k_sum2x(int G1, const double* __restrict__ part1, double* __restrict__ out1, int G2,
-                                             const double* __restrict__ part2, double* __restrict__ out2) {
+
+// The barrier + inertia items (blocks [0, G1)), the friction items (blocks
+// [G1, G1 + G2)) and the products da[i] db[i], i < dn (blocks from G1 + G2)
+// of one evaluation in one launch, one item per thread, each block's tree
+// sum written to part1 / part2 / part3: the partials that k_sum1 forms over
+// the item buffers (and over da, db) when no thread holds two items.
+__global__ __launch_bounds__(RT) void k_energy_parts(EnergyArgs ea, int G1, FricArgs fa, int G2, const double* __restrict__ da,
+                                                     const double* __restrict__ db, int dn, double* __restrict__ part1,
+                                                     double* __restrict__ part2, double* __restrict__ part3) {
     __shared__ double sh[RT];
-    const int G = blockIdx.x ? G2 : G1;
-    const double* part = blockIdx.x ? part2 : part1;
+    const int blk = blockIdx.x;
+    const int list = blk < G1 ? 0 : (blk < G1 + G2 ? 1 : 2);
+    const int lb = list == 0 ? blk : (list == 1 ? blk - G1 : blk - G1 - G2);
+    const int i = lb * RT + threadIdx.x;
+    double s = 0.0;
+    if (list == 0) { if (i < ea.B + ea.n_pt + ea.n_ee + ea.n_vg) s += energy_item(ea, i); }
+    else if (list == 1) { if (i < fa.n_pt + fa.n_ee + fa.n_vg) s += fric_item(fa, i); }
+    else if (i < dn) s += da[i] * db[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int m = RT / 2; m >= 1; m >>= 1) {
+        if ((int)threadIdx.x < m) sh[threadIdx.x] += sh[threadIdx.x + m];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) (list == 0 ? part1 : (list == 1 ? part2 : part3))[lb] = sh[0];
+}
+// k_sum2 over up to three partial lists at once (block l: list l)
+struct PartLists { int G[3]; const double* part[3]; double* out[3]; };
+__global__ __launch_bounds__(RT) void k_sum2x(PartLists L) {
+    __shared__ double sh[RT];
+    const int l = blockIdx.x;
+    const int G = L.G[l];
+    const double* part = L.part[l];
     double s = 0.0;
     for (int i = threadIdx.x; i < G; i += RT) s += part[i];
     sh[threadIdx.x] = s;
@@ -194,7 +219,7 @@ __global__ __launch_bounds__(RT) void k_sum2x(int G1, const double* __restrict__
         if ((int)threadIdx.x < m) sh[threadIdx.x] += sh[threadIdx.x + m];
         __syncthreads();
     }
-    if (threadIdx.x == 0) (blockIdx.x ? out2 : out1)[0] = sh[0];
+    if (threadIdx.x == 0) L.out[l][0] = sh[0];
 }
 
 // Sum of a[i] (b == null) or a[i] b[i] into w.epart[slot] (slots 0..7), on
@@ -223,6 +248,23 @@ void copy_slots_async(World& w, int first, int k) {
 }
 
 void dot_async(World& w, const double* a, const double* b, int n, int slot) { reduce_async(w, a, b, n, slot); }
+// a1 . b1 into slot1 and a2 . b2 into slot2 in two launches, each sum as
+// dot_async forms it
+void dot2_async(World& w, const double* a1, const double* b1, int slot1, const double* a2, const double* b2, int slot2, int n) {
+    if (n <= 0) { dot_async(w, a1, b1, n, slot1); dot_async(w, a2, b2, n, slot2); return; }
+    int G = (n + RT - 1) / RT;
+    if (G > EPART_MAXG) G = EPART_MAXG;
+    if (w.epart.cap < (size_t)EPART_SLOTS) throw AdbError(ADB_E_ARG, "reduction slots not reserved");
+    double* part1 = w.epart.p + NSLOT;
+    double* part2 = w.epart.p + NSLOT + EPART_MAXG;
+    hipLaunchKernelGGL(k_sum1x, dim3(G, 2), dim3(RT), 0, w.stream, n, G, a1, b1, part1, a2, b2, part2);
+    PartLists L;
+    L.G[0] = G; L.part[0] = part1; L.out[0] = w.epart.p + slot1;
+    L.G[1] = G; L.part[1] = part2; L.out[1] = w.epart.p + slot2;
+    L.G[2] = 0; L.part[2] = part2; L.out[2] = nullptr;
+    hipLaunchKernelGGL(k_sum2x, dim3(2), dim3(RT), 0, w.stream, L);
+    ADB_HIP(hipGetLastError());
+}
 
 static DBuf<double>& items(World& w) { return w.pcg_Ap; }   // scratch reused outside the solve
 
@@ -278,33 +320,41 @@ void energy_friction_async(World& w, const double* X, int slot) {
 }
 
 // Barrier + inertia energy at (qv, X) over *cs into slot_bi (cs null: not
-// evaluated) and friction energy at X (lagged terms of w.cs) into slot_f.
-// When every item list fits one item per thread in EPART_MAXG blocks, two
-// launches (k_energy_parts, k_sum2x) with the operations of the per-item
-// buffers and reduce_async; else those.
-void energy_async(World& w, const double* qv, const double* X, const ContactSet* cs, int slot_bi, int slot_f) {
+// evaluated), friction energy at X (lagged terms of w.cs) into slot_f and,
+// with da, da . db (dn entries) into slot_d.  When every list fits one item
+// per thread in EPART_MAXG blocks, two launches (k_energy_parts, k_sum2x)
+// with the operations of the per-item buffers and reduce_async; else those.
+void energy_async(World& w, const double* qv, const double* X, const ContactSet* cs, int slot_bi, int slot_f,
+                  const double* da, const double* db, int dn, int slot_d) {
     const int n1 = cs ? w.B + cs->n_pt + cs->n_ee + cs->n_vg : 0;
     const int n2 = w.cs.n_pt + w.cs.n_ee + w.cs.n_vg;
-    const int G1 = (n1 + RT - 1) / RT, G2 = (n2 + RT - 1) / RT;
-    if (G1 > EPART_MAXG || G2 > EPART_MAXG) {
+    const int n3 = da ? dn : 0;
+    const int G1 = (n1 + RT - 1) / RT, G2 = (n2 + RT - 1) / RT, G3 = (n3 + RT - 1) / RT;
+    if (G1 > EPART_MAXG || G2 > EPART_MAXG || G3 > EPART_MAXG) {
         if (cs) energy_barrier_inertia_async(w, qv, X, *cs, slot_bi);
         energy_friction_async(w, X, slot_f);
+        if (da) dot_async(w, da, db, dn, slot_d);
         return;
     }
     if (w.epart.cap < (size_t)EPART_SLOTS) throw AdbError(ADB_E_ARG, "reduction slots not reserved");
     EnergyArgs ea = cs ? energy_args(w, qv, X, *cs) : EnergyArgs{};
     const FricArgs fa = fric_args(w, X);
-    double* part1 = w.epart.p + NSLOT;
-    double* part2 = w.epart.p + NSLOT + EPART_MAXG;
+    double* part[3] = {w.epart.p + NSLOT, w.epart.p + NSLOT + EPART_MAXG, w.epart.p + NSLOT + 2 * EPART_MAXG};
     {
         ProfScope ps(w, PK_ENERGY, 0.0);
-        if (G1 + G2 > 0)
-            hipLaunchKernelGGL(k_energy_parts, dim3(G1 + G2), dim3(RT), 0, w.stream, ea, G1, fa, part1, part2);
+        if (G1 + G2 + G3 > 0)
+            hipLaunchKernelGGL(k_energy_parts, dim3(G1 + G2 + G3), dim3(RT), 0, w.stream, ea, G1, fa, G2, da, db, n3, part[0], part[1],
+                               part[2]);
         ADB_HIP(hipGetLastError());
     }
-    // block 0 sums list 1 (or, without cs, list 2 into slot_f)
-    if (cs) hipLaunchKernelGGL(k_sum2x, dim3(2), dim3(RT), 0, w.stream, G1, part1, w.epart.p + slot_bi, G2, part2, w.epart.p + slot_f);
-    else hipLaunchKernelGGL(k_sum2x, dim3(1), dim3(RT), 0, w.stream, G2, part2, w.epart.p + slot_f, 0, part2, w.epart.p + slot_f);
+    // the lists that are evaluated, in order (an empty list sums to 0)
+    PartLists L;
+    int nl = 0;
+    if (cs) { L.G[nl] = G1; L.part[nl] = part[0]; L.out[nl] = w.epart.p + slot_bi; nl++; }
+    L.G[nl] = G2; L.part[nl] = part[1]; L.out[nl] = w.epart.p + slot_f; nl++;
+    if (da) { L.G[nl] = G3; L.part[nl] = part[2]; L.out[nl] = w.epart.p + slot_d; nl++; }
+    for (int l = nl; l < 3; l++) { L.G[l] = 0; L.part[l] = part[0]; L.out[l] = nullptr; }
+    hipLaunchKernelGGL(k_sum2x, dim3(nl), dim3(RT), 0, w.stream, L);
     ADB_HIP(hipGetLastError());
 }
 

@@ -59,7 +59,7 @@ constexpr int R_D = 0, R_F = 1, R_G = 10, R_Z = 18, R_GRAD = 42, R_PK = 66;
 // Reduction result slots (8) + per-workgroup partials (at most 1024 workgroups):
 // reserved once at world creation, so no reduction queued behind another can
 // see its slots reallocated.
-constexpr int EPART_NSLOT = 8, EPART_MAXG = 1024, EPART_SLOTS = EPART_NSLOT + 2 * EPART_MAXG;
+constexpr int EPART_NSLOT = 8, EPART_MAXG = 1024, EPART_SLOTS = EPART_NSLOT + 3 * EPART_MAXG;
 
 // Per-kernel HIP-event timing on the world's stream (off unless enabled).
 enum ProfKernel {
@@ -302,8 +302,11 @@ int pcg_solve(World& w, double* dq_out, int* iters);                            
 // reductions into result slots 0..7 on the world stream; read_slots waits and copies slots [0, k)
 void energy_barrier_inertia_async(World& w, const double* qv, const double* X, const ContactSet& cs, int slot); // energy.hip
 void energy_friction_async(World& w, const double* X, int slot);                        // energy.hip
-void energy_async(World& w, const double* qv, const double* X, const ContactSet* cs, int slot_bi, int slot_f); // energy.hip
+void energy_async(World& w, const double* qv, const double* X, const ContactSet* cs, int slot_bi, int slot_f,
+                  const double* da = nullptr, const double* db = nullptr, int dn = 0, int slot_d = 0);   // energy.hip
 void dot_async(World& w, const double* a, const double* b, int n, int slot);            // energy.hip
+void dot2_async(World& w, const double* a1, const double* b1, int slot1, const double* a2, const double* b2, int slot2,
+                int n);                                                                    // energy.hip
 void read_slots(World& w, int k, double* out);                                          // energy.hip
 void copy_slots_async(World& w, int first, int k);    // energy.hip: slots [first, first + k) into w.h_slots, no wait
 void download_sets(World& w, const ContactSet& cs, std::vector<int>& pt, std::vector<int>& ee, std::vector<int>& vg);

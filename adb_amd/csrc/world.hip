@@ -74,8 +74,7 @@ static int line_search(World& w, double gn, double* alpha_out, int* trials_out, 
     // the same q, positions and contact set, so the same value bit for bit.
     // The friction part is recomputed: its lagged terms are this iteration's.
     const bool have_bi = w.e0_valid;
-    energy_async(w, w.q.p, w.X.p, have_bi ? nullptr : &w.cs, 5, 6);
-    dot_async(w, w.dq.p, w.grad.p, n, 2);
+    energy_async(w, w.q.p, w.X.p, have_bi ? nullptr : &w.cs, 5, 6, w.dq.p, w.grad.p, n, 2);
     double E0 = 0.0, qdg = 0.0, E1 = 0.0, alpha = 1.0, e1_bi = 0.0;
     bool have_e0 = false;
     int trials = 0, status = ADB_OK;
@@ -148,8 +147,7 @@ static int step(World& w, adb_step_stats* st) {
         S.t_solver += t2 - t1;
         // |dq|^2 and |g|^2 (slots 3, 4), copied to the host ahead of the CCD
         // bound's wait and read after it (one wait for both)
-        dot_async(w, w.dq.p, w.dq.p, n, 3);
-        dot_async(w, w.grad.p, w.grad.p, n, 4);
+        dot2_async(w, w.dq.p, w.dq.p, 3, w.grad.p, w.grad.p, 4, n);
         copy_slots_async(w, 3, 2);
         double toi = 1.0, factor = 1.0;
         int cs_status = ADB_OK;
