@@ -1087,53 +1087,67 @@ void trial_set(World& w, const double* X, const ContactSet& cand, ContactSet& ou
     out.n_pt = h[0]; out.n_ee = h[1] - h[0]; out.n_vg = h[2] - h[1];
 }
 
-double ccd_toi(World& w, const double* X0, const double* X1, int* status) {
-    *status = ADB_OK;
-    // toi_bits: the two bounds, then the body-pair counter (one read for all)
+// The CCD step bound of X0 -> X1 (model/abd.cpp:142-196) in two parts: the
+// launches and the copy of the bounds and the body-pair count to pinned
+// memory (ccd_queue, no wait), then the wait, a repeat with grown pair slots
+// when they overflowed, and the stall test (ccd_finish).  toi_bits holds the
+// two bounds, then the body-pair counter (one read for all).
+static bool ccd_hashed(const World& w) { return !(w.P.iaabb & 2); }   // spatial hashing (hash.hip) instead of body overlaps
+void ccd_queue(World& w, const double* X0, const double* X1) {
     w.toi_bits.reserve(3);
-    const bool hashed = !(w.P.iaabb & 2);    // spatial hashing (hash.hip) instead of body overlaps
+    const bool hashed = ccd_hashed(w);
     int* pcnt = reinterpret_cast<int*>(w.toi_bits.p + 2);
     int* hs = small_host(w);                  // [2, 6): the two 64-bit bounds; [6]: pair count
     unsigned long long* bits = reinterpret_cast<unsigned long long*>(hs + 2);
-    for (int attempt = 0; attempt < 2; attempt++) {
-        int np = 0;
-        // the bounds start at 1 (set by the box kernel, or copied when hashing)
-        if (!hashed) {
-            body_pairs(w, X0, X1, 0.0, pcnt, pcnt, 1, w.toi_bits.p);
-        } else {
-            const unsigned long long one = order_key_host(1.0);
-            const unsigned long long init[2] = {one, one};
-            ADB_HIP(hipMemcpyAsync(w.toi_bits.p, init, sizeof init, hipMemcpyHostToDevice, w.stream));
-        }
-        if (w.P.enable_ground) {
-            ProfScope ps(w, PK_VG_TOI, 16.0 * w.V);
-            hipLaunchKernelGGL(k_vg_toi, dim3((w.V + 255) / 256), dim3(256), 0, w.stream, w.V, X0, X1, w.P.ground_y, w.toi_bits.p,
-                               w.vbody.p, (const int*)nullptr);
-            ADB_HIP(hipGetLastError());
-        }
-        NarrowArgs a = narrow_args(w, pcnt, X0, X1, 0.0);
-        a.toi_bits = w.toi_bits.p;
-        if (hashed) {
-            hash_ccd(w, X0, X1, w.toi_bits.p);
-        } else {
-            ProfScope ps(w, PK_NARROW_CCD, 0.0);
-            const dim3 grid(narrow_grid(w, 1), 2 + w.tune.ee_slices);
-            if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<1, float>), grid, dim3(NB_THREADS), 0, w.stream, a);
-            else hipLaunchKernelGGL((k_pair_narrow<1, double>), grid, dim3(NB_THREADS), 0, w.stream, a);
-            ADB_HIP(hipGetLastError());
-        }
-        ADB_HIP(hipMemcpyAsync(bits, w.toi_bits.p, (hashed ? 2 : 3) * sizeof(unsigned long long), hipMemcpyDeviceToHost, w.stream));
+    // the bounds start at 1 (set by the box kernel, or copied when hashing)
+    if (!hashed) {
+        body_pairs(w, X0, X1, 0.0, pcnt, pcnt, 1, w.toi_bits.p);
+    } else {
+        const unsigned long long one = order_key_host(1.0);
+        const unsigned long long init[2] = {one, one};
+        ADB_HIP(hipMemcpyAsync(w.toi_bits.p, init, sizeof init, hipMemcpyHostToDevice, w.stream));
+    }
+    if (w.P.enable_ground) {
+        ProfScope ps(w, PK_VG_TOI, 16.0 * w.V);
+        hipLaunchKernelGGL(k_vg_toi, dim3((w.V + 255) / 256), dim3(256), 0, w.stream, w.V, X0, X1, w.P.ground_y, w.toi_bits.p,
+                           w.vbody.p, (const int*)nullptr);
+        ADB_HIP(hipGetLastError());
+    }
+    NarrowArgs a = narrow_args(w, pcnt, X0, X1, 0.0);
+    a.toi_bits = w.toi_bits.p;
+    if (hashed) {
+        hash_ccd(w, X0, X1, w.toi_bits.p);
+    } else {
+        ProfScope ps(w, PK_NARROW_CCD, 0.0);
+        const dim3 grid(narrow_grid(w, 1), 2 + w.tune.ee_slices);
+        if (w.P.broad_phase_fp32) hipLaunchKernelGGL((k_pair_narrow<1, float>), grid, dim3(NB_THREADS), 0, w.stream, a);
+        else hipLaunchKernelGGL((k_pair_narrow<1, double>), grid, dim3(NB_THREADS), 0, w.stream, a);
+        ADB_HIP(hipGetLastError());
+    }
+    ADB_HIP(hipMemcpyAsync(bits, w.toi_bits.p, (hashed ? 2 : 3) * sizeof(unsigned long long), hipMemcpyDeviceToHost, w.stream));
+}
+double ccd_finish(World& w, const double* X0, const double* X1, int* status) {
+    *status = ADB_OK;
+    const bool hashed = ccd_hashed(w);
+    int* hs = small_host(w);
+    const unsigned long long* bits = reinterpret_cast<const unsigned long long*>(hs + 2);
+    for (int attempt = 0;; attempt++) {
         wait_stream(w);
-        np = hashed ? 0 : hs[6];
+        const int np = hashed ? 0 : hs[6];
         if (!hashed) w.pair_hint[1] = np;
-        if ((size_t)np <= w.pairs.cap) break;
+        if ((size_t)np <= w.pairs.cap || attempt == 1) break;
         w.pairs.reserve(np);               // more overlapping body pairs than slots: grow, test again
+        ccd_queue(w, X0, X1);
     }
     const double tg = order_key_inv(bits[0]), tp = order_key_inv(bits[1]);
     // toi < 1e-6 twice in a row is a stall (model/iaabb.cpp:305-311,533-537)
     if (tg < 1e-6) { if (++w.stall_vg > 1) *status = ADB_E_TOI_STALL; } else w.stall_vg = 0;
     if (tp < 1e-6) { if (++w.stall_pt > 1) *status = ADB_E_TOI_STALL; } else w.stall_pt = 0;
     return tg < tp ? tg : tp;
+}
+double ccd_toi(World& w, const double* X0, const double* X1, int* status) {
+    ccd_queue(w, X0, X1);
+    return ccd_finish(w, X0, X1, status);
 }
 
 // Per-scene CCD bounds of a batched world (slots 2 s: vertex-ground, 2 s + 1:

@@ -18,6 +18,7 @@
 // returns at once, and the host queues iterations in batches, checking the
 // flag of the previous batch while the current one runs.
 #include <cmath>
+#include <functional>
 #include "dmath.h"
 #include "world.h"
 
@@ -314,7 +315,8 @@ static void enqueue_batch(World& w, int nb, bool prof) {
     hipLaunchKernelGGL(k_pcg_advance, dim3(1), dim3(64), 0, w.stream, w.pcg_k.p, nb);
 }
 
-int pcg_solve(World& w, double* dq_out, int* iters) {
+int pcg_solve(World& w, double* dq_out, int* iters, const std::function<void()>& after, bool* after_valid) {
+    if (after_valid) *after_valid = false;
     const int B = w.B, n = 12 * B;
     const int nbS = (B + SP_ROWS - 1) / SP_ROWS;
     // the iterate x lives in the caller's output (12 B doubles)
@@ -388,8 +390,12 @@ int pcg_solve(World& w, double* dq_out, int* iters) {
         it += nb0;
         stop_test();
         ADB_HIP(hipMemcpyAsync(w.h_scal, w.scal.p, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, w.stream));
+        // the caller's next work (on x as it stands) is queued ahead of the
+        // wait; it is valid when these iterations finished the solve
+        if (after) after();
         wait_stream(w);
         done = w.h_scal[S_STOP] != RUN;
+        if (after_valid) *after_valid = done && after;
     }
     while (!done && it < max_it) {
         int nb = std::min(BATCH, max_it - it);

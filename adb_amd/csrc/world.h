@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 #include "../../include/adb_capi.h"
@@ -283,6 +284,8 @@ void collision_set(World& w, const double* X, ContactSet& out);                 
 void segment_candidates(World& w, const double* X0, const double* X1, ContactSet& cand);
 void trial_set(World& w, const double* X, const ContactSet& cand, ContactSet& out);
 double ccd_toi(World& w, const double* X0, const double* X1, int* status);              // cull.hip
+void ccd_queue(World& w, const double* X0, const double* X1);                            // cull.hip
+double ccd_finish(World& w, const double* X0, const double* X1, int* status);           // cull.hip
 // per-scene CCD bounds of a batched world into toi[S] (host), stalls per scene into status[S]
 void ccd_toi_scenes(World& w, const double* X0, const double* X1, double* toi, int* status);   // cull.hip
 // out = q + f[scene(body)] dq (f: per-scene factors on the device)
@@ -298,7 +301,11 @@ void hash_sets(World& w, const double* X, double r, unsigned long long* pt_keys,
 void hash_ccd(World& w, const double* X0, const double* X1, unsigned long long* toi_bits); // hash.hip
 void contact_terms(World& w);                                                            // contact.hip
 void assemble_system(World& w);                                                          // assemble.hip
-int pcg_solve(World& w, double* dq_out, int* iters);                                     // pcg.hip
+// `after` (if given) queues work behind a short solve's stop test, before
+// its wait; *after_valid tells whether that solve was then complete (else
+// the work ran on an unfinished x and must be queued again)
+int pcg_solve(World& w, double* dq_out, int* iters, const std::function<void()>& after = {},
+              bool* after_valid = nullptr);                                                 // pcg.hip
 // reductions into result slots 0..7 on the world stream; read_slots waits and copies slots [0, k)
 void energy_barrier_inertia_async(World& w, const double* qv, const double* X, const ContactSet& cs, int slot); // energy.hip
 void energy_friction_async(World& w, const double* X, int slot);                        // energy.hip

@@ -140,24 +140,29 @@ static int step(World& w, adb_step_stats* st) {
         wait_stream(w);
         const double t1 = now_s();
         S.t_ipc += t1 - t0;
+        // |dq|^2 and |g|^2 (slots 3, 4) copied to the host and the CCD step
+        // bound (model/abd.cpp:142-196) queued: read in one wait.  Behind a
+        // short solve they are queued ahead of its stop test's wait (and
+        // queued again if that solve turns out unfinished).
+        auto queue_bound = [&]() {
+            dot2_async(w, w.dq.p, w.dq.p, 3, w.grad.p, w.grad.p, 4, n);
+            copy_slots_async(w, 3, 2);
+            if (w.P.upper_bound) {
+                transform_step(w, w.q.p, w.dq.p, 1.0, w.qtrial.p, w.X1.p);
+                ccd_queue(w, w.X.p, w.X1.p);
+            }
+        };
         int pit = 0;
-        const int prc = pcg_solve(w, w.dq.p, &pit);
+        bool queued = false;
+        const int prc = pcg_solve(w, w.dq.p, &pit, queue_bound, &queued);
         S.pcg_iters += pit;
         const double t2 = now_s();
         S.t_solver += t2 - t1;
-        // |dq|^2 and |g|^2 (slots 3, 4), copied to the host ahead of the CCD
-        // bound's wait and read after it (one wait for both)
-        dot2_async(w, w.dq.p, w.dq.p, 3, w.grad.p, w.grad.p, 4, n);
-        copy_slots_async(w, 3, 2);
+        if (!queued) queue_bound();
         double toi = 1.0, factor = 1.0;
         int cs_status = ADB_OK;
-        if (w.P.upper_bound) {
-            // CCD step bound (model/abd.cpp:142-196)
-            transform_step(w, w.q.p, w.dq.p, 1.0, w.qtrial.p, w.X1.p);
-            toi = ccd_toi(w, w.X.p, w.X1.p, &cs_status);
-        } else {
-            wait_stream(w);
-        }
+        if (w.P.upper_bound) toi = ccd_finish(w, w.X.p, w.X1.p, &cs_status);
+        else wait_stream(w);
         const double sl[2] = {w.h_slots[3], w.h_slots[4]};
         // a solve that stopped early is counted, not hidden: the direction is
         // still the last PCG iterate and the line search still guards descent
