@@ -94,24 +94,30 @@ __global__ void k_flags(int n, const unsigned long long* __restrict__ keys, int*
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
 }
+// The pair count: *npd when the host sized the launches by an upper bound np
+// (the pair count stays on the device; clamped to the bound, which the host
+// checks when it reads the count), else np.
+__device__ __forceinline__ int pair_count(int np, const int* npd) { return npd ? min(*npd, np) : np; }
 // seg = inclusive scan of the flags: pair p = seg - 1 starts at its first key
-__global__ void k_pair_meta(int n, int bb, const unsigned long long* __restrict__ keys, const int* __restrict__ seg,
-                            int* __restrict__ pair_start, int* __restrict__ pair_lo, int* __restrict__ pair_hi) {
+// (pairs past cap, the lists' length, are not written).  Each pair also gets
+// its key by (hi, lo) (the lower-triangle side of each row) in tk, tv; slots
+// [np, cap) of lists sized by a bound (npd: the count on the device) get keys
+// above every pair's (all ones), so the sort leaves them at the end.
+__global__ void k_pair_meta(int n, int bb, int cap, const int* __restrict__ npd, const unsigned long long* __restrict__ keys,
+                            const int* __restrict__ seg, int* __restrict__ pair_start, int* __restrict__ pair_lo,
+                            int* __restrict__ pair_hi, unsigned long long* __restrict__ tk, int* __restrict__ tv) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (i == n - 1) pair_start[seg[i]] = n;   // end of the last pair
+    if (i == n - 1) pair_start[min(seg[i], cap)] = n;   // end of the last pair (or of the list)
+    if (npd && i < cap && i >= *npd) { tk[i] = ~0ull; tv[i] = i; }
     if (!(i == 0 || keys[i] != keys[i - 1])) return;
     const int p = seg[i] - 1;
+    if (p >= cap) return;
+    const int lo = (int)(keys[i] >> bb), hi = (int)(keys[i] & ((1ull << bb) - 1));
     pair_start[p] = i;
-    pair_lo[p] = (int)(keys[i] >> bb);
-    pair_hi[p] = (int)(keys[i] & ((1ull << bb) - 1));
-}
-// the pairs keyed by (hi, lo): the lower-triangle side of each row
-__global__ void k_pair_tkeys(int np, int bb, const int* __restrict__ pair_lo, const int* __restrict__ pair_hi,
-                             unsigned long long* __restrict__ tk, int* __restrict__ tv) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    tk[p] = ((unsigned long long)pair_hi[p] << bb) | (unsigned long long)pair_lo[p];
+    pair_lo[p] = lo;
+    pair_hi[p] = hi;
+    tk[p] = ((unsigned long long)hi << bb) | (unsigned long long)lo;
     tv[p] = p;
 }
 __device__ __forceinline__ int lower_bound_int(const int* v, int n, int x) {
@@ -126,10 +132,12 @@ __device__ __forceinline__ int lower_bound_hi(const unsigned long long* tk, int 
 }
 // per body b (b <= B): first pair with lo = b, first transposed pair with hi =
 // b, and the row length (lower pairs + diagonal + upper pairs)
-__global__ void k_body_ranges(int B, int np, int bb, const int* __restrict__ pair_lo, const unsigned long long* __restrict__ tk,
-                              int* __restrict__ up_start, int* __restrict__ low_start, int* __restrict__ rowcnt) {
+__global__ void k_body_ranges(int B, int npb, const int* __restrict__ npd, int bb, const int* __restrict__ pair_lo,
+                              const unsigned long long* __restrict__ tk, int* __restrict__ up_start, int* __restrict__ low_start,
+                              int* __restrict__ rowcnt) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b > B) return;
+    const int np = pair_count(npb, npd);
     const int u0 = lower_bound_int(pair_lo, np, b), l0 = lower_bound_hi(tk, np, bb, b);
     up_start[b] = u0; low_start[b] = l0;
     if (b == B) { rowcnt[b] = 0; return; }
@@ -172,23 +180,103 @@ __global__ void k_vg_start(int B, int n_vg, const int2* __restrict__ vg, int* __
 // lanes of blocks 0 and 1 also add the gradient entry col of their body.
 constexpr int RSLOT = 7;
 constexpr int RLANES = 36;
-constexpr int PCH = 8 * RSLOT;
+constexpr int PCH = 16 * RSLOT;   // 112: at step 70 of the icosphere pile 4 % faster than 56 (8 rounds); 28 and 14 slower
 constexpr int PART = 3 * 144 + 24;   // chunk partial: 3 blocks (column-major) + gradients of lo, hi
 constexpr int REC_USED = R_PK + 82;  // doubles of a record that can be in use (even)
-__global__ void k_pair_chunks(int np, const int* __restrict__ pair_start, int* __restrict__ nch) {
+// chunks per pair; 0 for the slots [np, npb] of a launch sized by the bound
+__global__ void k_pair_chunks(int npb, const int* __restrict__ npd, const int* __restrict__ pair_start, int* __restrict__ nch) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < np) nch[p] = (pair_start[p + 1] - pair_start[p] + PCH - 1) / PCH;
+    if (p > npb) return;
+    const int np = pair_count(npb, npd);
+    nch[p] = p < np ? (pair_start[p + 1] - pair_start[p] + PCH - 1) / PCH : 0;
 }
-// chunk_off = exclusive scan of nch (np + 1 entries); the chunk count also
+// chunk_off = exclusive scan of nch (npb + 1 entries); the chunk count also
 // goes to counts[3] for the byte accounting
-__global__ void k_chunk_owner(int np, const int* __restrict__ chunk_off, int* __restrict__ chunk_pair, int* __restrict__ counts) {
+__global__ void k_chunk_owner(int npb, const int* __restrict__ npd, const int* __restrict__ chunk_off, int* __restrict__ chunk_pair,
+                              int* __restrict__ counts) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int np = pair_count(npb, npd);
     if (k < np) for (int c = chunk_off[k]; c < chunk_off[k + 1]; c++) chunk_pair[c] = k;
     if (k == 0) counts[3] = chunk_off[np];
 }
 
+// ---- the block-row structure, vertex-ground starts, chunk table and the
+// host's copy of the pair lists in one workgroup (the launches of the kernels
+// above, which stay for large B + pair counts, each cost more than their work
+// in the light phase of a scene)
+constexpr int STRUCT_T = 1024, STRUCT_MAX = 1 << 16;
+// exclusive scan of in[0, n) into out[0, n], out[n] = the total, by one
+// workgroup of STRUCT_T threads (contiguous ranges per thread); returns the total
+__device__ int block_scan_excl(const int* in, int* out, int n, int* sh) {
+    const int t = threadIdx.x, per = (n + STRUCT_T - 1) / STRUCT_T;
+    const int i0 = min(n, t * per), i1 = min(n, i0 + per);
+    int s = 0;
+    for (int i = i0; i < i1; i++) s += in[i];
+    int incl = s;
+    for (int d = 1; d < 64; d <<= 1) { const int v = __shfl_up(incl, d); if (__lane_id() >= d) incl += v; }
+    if (__lane_id() == 63) sh[t / 64] = incl;
+    __syncthreads();
+    int off = incl - s, total = 0;
+    for (int w = 0; w < STRUCT_T / 64; w++) { if (w < t / 64) off += sh[w]; total += sh[w]; }
+    for (int i = i0; i < i1; i++) { out[i] = off; off += in[i]; }
+    if (t == 0) out[n] = total;
+    __syncthreads();
+    return total;
+}
+struct StructArgs {
+    int B, npb, bb, n_vg;
+    const int* npd;
+    const int* pair_lo; const int* pair_hi; const int* pair_start; const unsigned long long* tk; const int* tv; const int2* vg;
+    int* up_start; int* low_start; int* rowcnt; int* row_ptr; int* col_idx; int* diag_blk; int* pair_up; int* pair_low;
+    int* vg_start; int* chunk_n; int* chunk_off; int* chunk_pair; int* counts;
+    int* host_pack;   // [pair count, npb pair_lo, npb pair_hi, npb + 1 pair_start]
+};
+__global__ __launch_bounds__(STRUCT_T) void k_structure(StructArgs a) {
+    __shared__ int sh[STRUCT_T / 64];
+    const int np = pair_count(a.npb, a.npd), t = threadIdx.x;
+    // k_body_ranges and k_vg_start
+    for (int b = t; b <= a.B; b += STRUCT_T) {
+        a.up_start[b] = lower_bound_int(a.pair_lo, np, b);
+        a.low_start[b] = lower_bound_hi(a.tk, np, a.bb, b);
+        int lo = 0, hi = a.n_vg;
+        while (lo < hi) { const int mid = (lo + hi) / 2; if (a.vg[mid].x < b) lo = mid + 1; else hi = mid; }
+        a.vg_start[b] = lo;
+    }
+    __syncthreads();
+    for (int b = t; b < a.B; b += STRUCT_T)
+        a.rowcnt[b] = (a.low_start[b + 1] - a.low_start[b]) + 1 + (a.up_start[b + 1] - a.up_start[b]);
+    __syncthreads();
+    block_scan_excl(a.rowcnt, a.row_ptr, a.B, sh);
+    // k_fill_cols
+    for (int b = t; b < a.B; b += STRUCT_T) {
+        int k = a.row_ptr[b];
+        for (int i = a.low_start[b]; i < a.low_start[b + 1]; i++, k++) {
+            a.col_idx[k] = (int)(a.tk[i] & ((1ull << a.bb) - 1));
+            a.pair_low[a.tv[i]] = k;
+        }
+        a.diag_blk[b] = k;
+        a.col_idx[k++] = b;
+        for (int p = a.up_start[b]; p < a.up_start[b + 1]; p++, k++) {
+            a.col_idx[k] = a.pair_hi[p];
+            a.pair_up[p] = k;
+        }
+    }
+    // k_pair_chunks, the scan, k_chunk_owner
+    for (int p = t; p < np; p += STRUCT_T) a.chunk_n[p] = (a.pair_start[p + 1] - a.pair_start[p] + PCH - 1) / PCH;
+    __syncthreads();
+    const int nch = block_scan_excl(a.chunk_n, a.chunk_off, np, sh);
+    for (int p = t; p < np; p += STRUCT_T)
+        for (int c = a.chunk_off[p]; c < a.chunk_off[p + 1]; c++) a.chunk_pair[c] = p;
+    if (t == 0) a.counts[3] = nch;
+    // the pair lists for the host's group building, one copy
+    if (t == 0) a.host_pack[0] = np;
+    for (int p = t; p < np; p += STRUCT_T) { a.host_pack[1 + p] = a.pair_lo[p]; a.host_pack[1 + a.npb + p] = a.pair_hi[p]; }
+    for (int p = t; p <= np; p += STRUCT_T) a.host_pack[1 + 2 * a.npb + p] = a.pair_start[p];
+}
+
 struct PairArgs {
     int np, n_pt, n_ee, B;
+    const int* npd;                           // the pair count on the device (np is then a bound), or null
     const int* cid;                           // contact ids sorted by pair
     const int* pair_start; const int* pair_lo; const int* pair_hi; const int* pair_up; const int* pair_low;
     const int* up_start; const int* low_start; const int* tv;
@@ -247,7 +335,7 @@ __global__ __launch_bounds__(256) void k_pair_reduce(PairArgs a, const int* __re
     // takes the contiguous chunk range [x per, (x + 1) per)
     const int per = gridDim.x / 8;
     const int ch = (blockIdx.x % 8) * per + blockIdx.x / 8;
-    if (ch >= a.chunk_off[a.np]) return;
+    if (ch >= a.chunk_off[pair_count(a.np, a.npd)]) return;
     const int p = chunk_pair[ch];
     const int beg = a.pair_start[p] + (ch - a.chunk_off[p]) * PCH;
     const int end = min(beg + PCH, a.pair_start[p + 1]);
@@ -346,7 +434,7 @@ __device__ __forceinline__ void add_chunks(double& acc, const double* __restrict
 __global__ void k_pair_offdiag(PairArgs a, const double* __restrict__ part) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     const int p = idx / 144, e = idx % 144;
-    if (p >= a.np) return;
+    if (p >= pair_count(a.np, a.npd)) return;
     double acc = 0.0;
     add_chunks(acc, part, a.chunk_off[p], a.chunk_off[p + 1], 288 + e);
     a.Hval[144 * (size_t)a.pair_up[p] + e] = acc;
@@ -653,27 +741,42 @@ __global__ __launch_bounds__(256) void k_group_inv(const int* __restrict__ list,
 // ends up as one group (the preconditioned system then decouples exactly).
 // The pair lists go to pinned host memory before the assembly reduction is
 // queued (groups_fetch), so the host builds the groups while it runs.
-static void groups_fetch(World& w) {
-    const int np = w.n_pairs;
-    if (3 * (size_t)np + 1 > w.h_pairs_cap) {
+// h_pairs = [pair count, npb pair_lo, npb pair_hi, npb + 1 pair_start]: the
+// first npb entries of each list (npb = the pair count, or a bound of it
+// with npd the count on the device); build_groups reads the count with them.
+static void groups_fetch(World& w, int npb, const int* npd, const int* pack) {
+    if (3 * (size_t)npb + 2 > w.h_pairs_cap) {
         if (w.h_pairs) ADB_HIP(hipHostFree(w.h_pairs));
-        w.h_pairs_cap = 3 * (size_t)np + 1 + (size_t)np / 2 + 64;
+        w.h_pairs_cap = 3 * (size_t)npb + 2 + (size_t)npb / 2 + 64;
         ADB_HIP(hipHostMalloc((void**)&w.h_pairs, w.h_pairs_cap * sizeof(int), hipHostMallocDefault));
     }
     if (!w.ev_pairs) ADB_HIP(hipEventCreateWithFlags(&w.ev_pairs, hipEventDisableTiming));
-    if (np > 0) {
-        ADB_HIP(hipMemcpyAsync(w.h_pairs, w.pair_lo.p, np * sizeof(int), hipMemcpyDeviceToHost, w.stream));
-        ADB_HIP(hipMemcpyAsync(w.h_pairs + np, w.pair_hi.p, np * sizeof(int), hipMemcpyDeviceToHost, w.stream));
-        ADB_HIP(hipMemcpyAsync(w.h_pairs + 2 * (size_t)np, w.pair_start.p, (np + 1) * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+    w.h_pairs_n = npb;
+    if (pack) {
+        // the same layout, packed on the device (k_structure): one copy
+        ADB_HIP(hipMemcpyAsync(w.h_pairs, pack, (3 * (size_t)npb + 2) * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+    } else if (npd) {
+        ADB_HIP(hipMemcpyAsync(w.h_pairs, npd, sizeof(int), hipMemcpyDeviceToHost, w.stream));
+    } else {
+        w.h_pairs[0] = npb;
+    }
+    if (!pack && npb > 0) {
+        ADB_HIP(hipMemcpyAsync(w.h_pairs + 1, w.pair_lo.p, npb * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipMemcpyAsync(w.h_pairs + 1 + npb, w.pair_hi.p, npb * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipMemcpyAsync(w.h_pairs + 1 + 2 * (size_t)npb, w.pair_start.p, (npb + 1) * sizeof(int), hipMemcpyDeviceToHost,
+                               w.stream));
     }
     ADB_HIP(hipEventRecord(w.ev_pairs, w.stream));
 }
 static void build_groups(World& w) {
-    const int B = w.B, np = w.n_pairs;
     wait_event(w, w.ev_pairs);
-    const int* plo = w.h_pairs;
-    const int* phi = w.h_pairs + np;
-    const int* ps = w.h_pairs + 2 * (size_t)np;
+    const int B = w.B, npb = w.h_pairs_n, np = w.h_pairs[0];
+    if (np > npb) throw AdbError(ADB_E_ARG, "body pairs with contacts exceed the culling pass's pair count");
+    w.n_pairs = np;
+    w.nnzb = B + 2 * np;
+    const int* plo = w.h_pairs + 1;
+    const int* phi = w.h_pairs + 1 + npb;
+    const int* ps = w.h_pairs + 1 + 2 * (size_t)npb;
     // pairs by decreasing contact count, ties by index (counting sort)
     int wmax = 0;
     for (int p = 0; p < np; p++) wmax = std::max(wmax, ps[p + 1] - ps[p]);
@@ -828,19 +931,28 @@ void assemble_system(World& w) {
         ADB_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tmp, w.uniq_flag.p, w.blk_of.p, nc, w.stream));
         w.sort_tmp.reserve(tmp + 16);
         ADB_HIP(hipcub::DeviceScan::InclusiveSum(w.sort_tmp.p, tmp, w.uniq_flag.p, w.blk_of.p, nc, w.stream));
-        ADB_HIP(hipMemcpyAsync(&np, w.blk_of.p + nc - 1, sizeof(int), hipMemcpyDeviceToHost, w.stream));
-        wait_stream(w);
     }
-    w.n_pairs = np;
+    // The pair count np = blk_of[nc - 1] stays on the device when the culling
+    // pass that made the set bounded it (its overlapping body pairs): every
+    // launch below is sized by that bound and reads np there; the host gets
+    // np with the pair lists (build_groups).  Else it is read here.
+    const int* npd = nullptr;
+    if (nc > 0 && w.cs.np_bound >= 0) {
+        npd = w.blk_of.p + nc - 1;
+        np = std::min(w.cs.np_bound, nc);
+    } else if (nc > 0) {
+        if (!w.h_np) ADB_HIP(hipHostMalloc((void**)&w.h_np, sizeof(int), hipHostMallocDefault));
+        ADB_HIP(hipMemcpyAsync(w.h_np, w.blk_of.p + nc - 1, sizeof(int), hipMemcpyDeviceToHost, w.stream));
+        wait_stream(w);
+        np = *w.h_np;
+    }
+    // np: the pair count or its bound (npd set) from here on
     w.pair_start.reserve(np + 1); w.pair_lo.reserve(std::max(np, 1)); w.pair_hi.reserve(std::max(np, 1));
     w.pair_up.reserve(std::max(np, 1)); w.pair_low.reserve(std::max(np, 1));
     w.tkeys.reserve(std::max(np, 1)); w.tkeys_s.reserve(std::max(np, 1)); w.tval.reserve(std::max(np, 1)); w.tval_s.reserve(std::max(np, 1));
     if (nc > 0) {
-        hipLaunchKernelGGL(k_pair_meta, dim3((nc + 255) / 256), dim3(256), 0, w.stream, nc, bb, w.akeys_s.p, w.blk_of.p,
-                           w.pair_start.p, w.pair_lo.p, w.pair_hi.p);
-        ADB_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_pair_tkeys, dim3((np + 255) / 256), dim3(256), 0, w.stream, np, bb, w.pair_lo.p, w.pair_hi.p, w.tkeys.p,
-                           w.tval.p);
+        hipLaunchKernelGGL(k_pair_meta, dim3((nc + 255) / 256), dim3(256), 0, w.stream, nc, bb, np, npd, w.akeys_s.p, w.blk_of.p,
+                           w.pair_start.p, w.pair_lo.p, w.pair_hi.p, w.tkeys.p, w.tval.p);
         ADB_HIP(hipGetLastError());
         sort_pairs(w, w.tkeys.p, w.tkeys_s.p, w.tval.p, w.tval_s.p, np, 2 * bb);
     }
@@ -851,42 +963,58 @@ void assemble_system(World& w) {
     w.up_start.reserve(B + 1); w.low_start.reserve(B + 1); w.rowcnt.reserve(B + 1);
     w.row_ptr.reserve(B + 1); w.col_idx.reserve(nnzb); w.diag_blk.reserve(B);
     w.Hval.reserve(144 * (size_t)nnzb);
-    hipLaunchKernelGGL(k_body_ranges, dim3((B + 1 + 255) / 256), dim3(256), 0, w.stream, B, np, bb, w.pair_lo.p, w.tkeys_s.p,
-                       w.up_start.p, w.low_start.p, w.rowcnt.p);
-    ADB_HIP(hipGetLastError());
-    {
-        size_t tmp = 0;
-        ADB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, w.rowcnt.p, w.row_ptr.p, B + 1, w.stream));
-        w.sort_tmp.reserve(tmp + 16);
-        ADB_HIP(hipcub::DeviceScan::ExclusiveSum(w.sort_tmp.p, tmp, w.rowcnt.p, w.row_ptr.p, B + 1, w.stream));
-    }
-    hipLaunchKernelGGL(k_fill_cols, dim3((B + 255) / 256), dim3(256), 0, w.stream, B, bb, w.row_ptr.p, w.up_start.p, w.low_start.p,
-                       w.pair_hi.p, w.tkeys_s.p, w.tval_s.p, w.col_idx.p, w.diag_blk.p, w.pair_up.p, w.pair_low.p);
-    ADB_HIP(hipGetLastError());
     w.vg_start.reserve(B + 1);
-    hipLaunchKernelGGL(k_vg_start, dim3((B + 1 + 255) / 256), dim3(256), 0, w.stream, B, w.cs.n_vg, w.cs.vg.p, w.vg_start.p);
-    ADB_HIP(hipGetLastError());
     w.grad.reserve(12 * (size_t)B);
-    // 3. chunks of at most PCH contacts per pair
     w.chunk_n.reserve(np + 1); w.chunk_off.reserve(np + 1);
-    if (np > 0)
-        hipLaunchKernelGGL(k_pair_chunks, dim3((np + 255) / 256), dim3(256), 0, w.stream, np, w.pair_start.p, w.chunk_n.p);
-    ADB_HIP(hipMemsetAsync(w.chunk_n.p + np, 0, sizeof(int), w.stream));
-    {
-        size_t tmp = 0;
-        ADB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, w.chunk_n.p, w.chunk_off.p, np + 1, w.stream));
-        w.sort_tmp.reserve(tmp + 16);
-        ADB_HIP(hipcub::DeviceScan::ExclusiveSum(w.sort_tmp.p, tmp, w.chunk_n.p, w.chunk_off.p, np + 1, w.stream));
-    }
     // each pair has at most len / PCH + 1 chunks: launch for the bound, rounded
     // up to the 8 XCDs; the kernel reads the exact count
     const int max_chunks = (np + nc / PCH + 7) / 8 * 8;
     w.chunk_blk.reserve(std::max(max_chunks, 1));
     w.chunk_part.reserve(PART * (size_t)std::max(max_chunks, 1));
-    hipLaunchKernelGGL(k_chunk_owner, dim3((np + 255) / 256 + 1), dim3(256), 0, w.stream, np, w.chunk_off.p, w.chunk_blk.p, w.ccount.p);
-    ADB_HIP(hipGetLastError());
+    const bool one_wg = B + np <= STRUCT_MAX;
+    if (one_wg) {
+        // 3. block rows, vertex-ground starts and chunks in one workgroup
+        w.pair_pack.reserve(3 * (size_t)np + 2);
+        StructArgs sa;
+        sa.B = B; sa.npb = np; sa.bb = bb; sa.n_vg = w.cs.n_vg; sa.npd = npd;
+        sa.pair_lo = w.pair_lo.p; sa.pair_hi = w.pair_hi.p; sa.pair_start = w.pair_start.p; sa.tk = w.tkeys_s.p; sa.tv = w.tval_s.p;
+        sa.vg = w.cs.vg.p;
+        sa.up_start = w.up_start.p; sa.low_start = w.low_start.p; sa.rowcnt = w.rowcnt.p; sa.row_ptr = w.row_ptr.p;
+        sa.col_idx = w.col_idx.p; sa.diag_blk = w.diag_blk.p; sa.pair_up = w.pair_up.p; sa.pair_low = w.pair_low.p;
+        sa.vg_start = w.vg_start.p; sa.chunk_n = w.chunk_n.p; sa.chunk_off = w.chunk_off.p; sa.chunk_pair = w.chunk_blk.p;
+        sa.counts = w.ccount.p; sa.host_pack = w.pair_pack.p;
+        hipLaunchKernelGGL(k_structure, dim3(1), dim3(STRUCT_T), 0, w.stream, sa);
+        ADB_HIP(hipGetLastError());
+    } else {
+        hipLaunchKernelGGL(k_body_ranges, dim3((B + 1 + 255) / 256), dim3(256), 0, w.stream, B, np, npd, bb, w.pair_lo.p,
+                           w.tkeys_s.p, w.up_start.p, w.low_start.p, w.rowcnt.p);
+        ADB_HIP(hipGetLastError());
+        {
+            size_t tmp = 0;
+            ADB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, w.rowcnt.p, w.row_ptr.p, B + 1, w.stream));
+            w.sort_tmp.reserve(tmp + 16);
+            ADB_HIP(hipcub::DeviceScan::ExclusiveSum(w.sort_tmp.p, tmp, w.rowcnt.p, w.row_ptr.p, B + 1, w.stream));
+        }
+        hipLaunchKernelGGL(k_fill_cols, dim3((B + 255) / 256), dim3(256), 0, w.stream, B, bb, w.row_ptr.p, w.up_start.p,
+                           w.low_start.p, w.pair_hi.p, w.tkeys_s.p, w.tval_s.p, w.col_idx.p, w.diag_blk.p, w.pair_up.p, w.pair_low.p);
+        ADB_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_vg_start, dim3((B + 1 + 255) / 256), dim3(256), 0, w.stream, B, w.cs.n_vg, w.cs.vg.p, w.vg_start.p);
+        ADB_HIP(hipGetLastError());
+        // 3. chunks of at most PCH contacts per pair
+        hipLaunchKernelGGL(k_pair_chunks, dim3((np + 1 + 255) / 256), dim3(256), 0, w.stream, np, npd, w.pair_start.p, w.chunk_n.p);
+        ADB_HIP(hipGetLastError());
+        {
+            size_t tmp = 0;
+            ADB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, w.chunk_n.p, w.chunk_off.p, np + 1, w.stream));
+            w.sort_tmp.reserve(tmp + 16);
+            ADB_HIP(hipcub::DeviceScan::ExclusiveSum(w.sort_tmp.p, tmp, w.chunk_n.p, w.chunk_off.p, np + 1, w.stream));
+        }
+        hipLaunchKernelGGL(k_chunk_owner, dim3((np + 255) / 256 + 1), dim3(256), 0, w.stream, np, npd, w.chunk_off.p, w.chunk_blk.p,
+                           w.ccount.p);
+        ADB_HIP(hipGetLastError());
+    }
     PairArgs a;
-    a.np = np; a.n_pt = w.cs.n_pt; a.n_ee = w.cs.n_ee; a.B = B;
+    a.np = np; a.npd = npd; a.n_pt = w.cs.n_pt; a.n_ee = w.cs.n_ee; a.B = B;
     a.cid = w.aval_s.p;
     a.pair_start = w.pair_start.p; a.pair_lo = w.pair_lo.p; a.pair_hi = w.pair_hi.p; a.pair_up = w.pair_up.p; a.pair_low = w.pair_low.p;
     a.up_start = w.up_start.p; a.low_start = w.low_start.p; a.tv = w.tval_s.p;
@@ -896,7 +1024,7 @@ void assemble_system(World& w) {
     a.vg = w.cs.vg.p; a.vg_start = w.vg_start.p; a.vgK = w.vgK.p; a.voff = w.voff.p; a.xbar = w.xbar.p;
     a.diag_blk = w.diag_blk.p;
     a.Hval = w.Hval.p; a.grad = w.grad.p;
-    groups_fetch(w);
+    groups_fetch(w, np, npd, one_wg ? w.pair_pack.p : nullptr);
     if (w.side_pending) {   // records of the 4-point contacts and 3-point projections (side streams)
         ADB_HIP(hipStreamWaitEvent(w.stream, w.ev_side[1], 0));
         ADB_HIP(hipStreamWaitEvent(w.stream, w.ev_pk, 0));

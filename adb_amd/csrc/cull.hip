@@ -963,6 +963,8 @@ static void set_pass(World& w, const double* X, const double* X1, ContactSet& ou
             sort_keys(w, ks, npt, nee, nall, std::max(most, nvg), w.keys_b.p + half, w.keys_b.p, out);
         }
         out.n_pt = npt; out.n_ee = nee; out.n_vg = nvg;
+        // PT / EE contacts only come from the overlapping body pairs of the pass
+        out.np_bound = hashed ? -1 : h[2 * NBK + 1];
         return;
     }
     throw AdbError(ADB_E_CAPACITY, "contact-set capacity could not be grown");
@@ -1085,6 +1087,7 @@ void trial_set(World& w, const double* X, const ContactSet& cand, ContactSet& ou
     wait_stream(w);
     const int* h = w.h_tcount;
     out.n_pt = h[0]; out.n_ee = h[1] - h[0]; out.n_vg = h[2] - h[1];
+    out.np_bound = cand.np_bound;   // a subset of the candidates
 }
 
 // The CCD step bound of X0 -> X1 (model/abd.cpp:142-196) in two parts: the

@@ -88,6 +88,7 @@ struct ContactSet {
     DBuf<int4> pt, ee;      // (I, v, J, f), (I, ei, J, ej)
     DBuf<int2> vg;          // (I, v)
     int n_pt = 0, n_ee = 0, n_vg = 0;
+    int np_bound = -1;      // bound of the body pairs with PT / EE contacts (the culling pass's overlapping pairs), or -1
 };
 
 struct World {
@@ -159,6 +160,7 @@ struct World {
     DBuf<unsigned long long> akeys, akeys_s;   // body-pair key per contact
     DBuf<int> aval, aval_s;          // contact ids (sorted by body pair)
     DBuf<int> blk_of, uniq_flag;     // per sorted contact: pair segment
+    DBuf<int> pair_pack;             // pair count and lists packed for one copy to h_pairs
     // body pairs (lo < hi) with contacts, sorted: first contact, bodies, block
     // index in row lo (upper) and row hi (lower); the pairs keyed (hi, lo)
     DBuf<int> pair_start, pair_lo, pair_hi, pair_up, pair_low;
@@ -196,8 +198,10 @@ struct World {
     double* pcg_xo = nullptr;        // the solve's iterate: the caller's output buffer
     DBuf<int> pcg_k;                 // iteration base of the queued PCG batch
     int pcg_prev_its = 0;            // iterations of the previous solve
-    int* h_pairs = nullptr;          // pinned host copy of the pair lists (lo, hi, start)
+    int* h_pairs = nullptr;          // pinned host copy of the pair count and lists (lo, hi, start)
     size_t h_pairs_cap = 0;
+    int h_pairs_n = 0;               // entries per list in h_pairs (the pair count or its bound)
+    int* h_np = nullptr;             // pinned host copy of the pair count (read when unbounded)
     hipEvent_t ev_pairs = nullptr;   // their copy is done
     bool use_graphs = true;          // ADB_PCG_GRAPH=0 turns graph capture off
     hipGraphExec_t pcg_graph = nullptr;   // one captured PCG batch

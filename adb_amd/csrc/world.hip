@@ -539,7 +539,8 @@ static void free_world(World& w) {
                        &w.aval, &w.aval_s, &w.blk_of, &w.uniq_flag, &w.row_ptr, &w.col_idx, &w.diag_blk, &w.vg_start,
                        &w.pair_start, &w.pair_lo, &w.pair_hi, &w.pair_up, &w.pair_low, &w.tval, &w.tval_s,
                        &w.up_start, &w.low_start, &w.rowcnt,
-                       &w.ctype, &w.ccls, &w.clist, &w.clist3, &w.ccount, &w.chunk_n, &w.chunk_off, &w.chunk_blk};
+                       &w.ctype, &w.ccls, &w.clist, &w.clist3, &w.ccount, &w.chunk_n, &w.chunk_off, &w.chunk_blk,
+                       &w.pair_pack};
     for (auto* b : ib) b->free_();
     DBuf<double>* db[] = {&w.xbar, &w.mass, &w.Ic, &w.q, &w.q0, &w.qdot, &w.dq, &w.qtrial, &w.X, &w.Xs, &w.X1, &w.bbox, &w.bsph,
                           &w.crec, &w.cdist, &w.chunk_part, &w.ctk, &w.clam, &w.vgK, &w.vglam, &w.bodyH, &w.bodyG, &w.Hval, &w.grad,
@@ -580,6 +581,8 @@ static void free_world(World& w) {
     w.h_slots = nullptr;
     if (w.h_pairs) (void)hipHostFree(w.h_pairs);
     w.h_pairs = nullptr; w.h_pairs_cap = 0;
+    if (w.h_np) (void)hipHostFree(w.h_np);
+    w.h_np = nullptr;
     if (w.ev_pairs) (void)hipEventDestroy(w.ev_pairs);
     w.ev_pairs = nullptr;
     for (hipEvent_t& e : w.pcg_ev) { if (e) (void)hipEventDestroy(e); e = nullptr; }
