@@ -744,7 +744,7 @@ __global__ __launch_bounds__(256) void k_group_inv(const int* __restrict__ list,
 // h_pairs = [pair count, npb pair_lo, npb pair_hi, npb + 1 pair_start]: the
 // first npb entries of each list (npb = the pair count, or a bound of it
 // with npd the count on the device); build_groups reads the count with them.
-static void groups_fetch(World& w, int npb, const int* npd, const int* pack) {
+static void groups_fetch(World& w, int npb, const int* npd, const int* pack, hipStream_t st) {
     if (3 * (size_t)npb + 2 > w.h_pairs_cap) {
         if (w.h_pairs) ADB_HIP(hipHostFree(w.h_pairs));
         w.h_pairs_cap = 3 * (size_t)npb + 2 + (size_t)npb / 2 + 64;
@@ -754,19 +754,19 @@ static void groups_fetch(World& w, int npb, const int* npd, const int* pack) {
     w.h_pairs_n = npb;
     if (pack) {
         // the same layout, packed on the device (k_structure): one copy
-        ADB_HIP(hipMemcpyAsync(w.h_pairs, pack, (3 * (size_t)npb + 2) * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipMemcpyAsync(w.h_pairs, pack, (3 * (size_t)npb + 2) * sizeof(int), hipMemcpyDeviceToHost, st));
     } else if (npd) {
-        ADB_HIP(hipMemcpyAsync(w.h_pairs, npd, sizeof(int), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipMemcpyAsync(w.h_pairs, npd, sizeof(int), hipMemcpyDeviceToHost, st));
     } else {
         w.h_pairs[0] = npb;
     }
     if (!pack && npb > 0) {
-        ADB_HIP(hipMemcpyAsync(w.h_pairs + 1, w.pair_lo.p, npb * sizeof(int), hipMemcpyDeviceToHost, w.stream));
-        ADB_HIP(hipMemcpyAsync(w.h_pairs + 1 + npb, w.pair_hi.p, npb * sizeof(int), hipMemcpyDeviceToHost, w.stream));
+        ADB_HIP(hipMemcpyAsync(w.h_pairs + 1, w.pair_lo.p, npb * sizeof(int), hipMemcpyDeviceToHost, st));
+        ADB_HIP(hipMemcpyAsync(w.h_pairs + 1 + npb, w.pair_hi.p, npb * sizeof(int), hipMemcpyDeviceToHost, st));
         ADB_HIP(hipMemcpyAsync(w.h_pairs + 1 + 2 * (size_t)npb, w.pair_start.p, (npb + 1) * sizeof(int), hipMemcpyDeviceToHost,
-                               w.stream));
+                               st));
     }
-    ADB_HIP(hipEventRecord(w.ev_pairs, w.stream));
+    ADB_HIP(hipEventRecord(w.ev_pairs, st));
 }
 static void build_groups(World& w) {
     wait_event(w, w.ev_pairs);
@@ -898,21 +898,28 @@ static double group_inv_flops(int m) {
 using PairSortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::merge_sort_config<512, 512, 8>,
                                                   rocprim::default_config, 400000>;
 template <class K, class V>
-static void sort_pairs(World& w, K* kin, K* kout, V* vin, V* vout, int n, int end_bit) {
+static void sort_pairs(World& w, K* kin, K* kout, V* vin, V* vout, int n, int end_bit, hipStream_t st) {
     size_t tmp = 0;
-    ADB_HIP(rocprim::radix_sort_pairs<PairSortConfig>(nullptr, tmp, kin, kout, vin, vout, (size_t)n, 0, end_bit, w.stream));
+    ADB_HIP(rocprim::radix_sort_pairs<PairSortConfig>(nullptr, tmp, kin, kout, vin, vout, (size_t)n, 0, end_bit, st));
     w.sort_tmp.reserve(tmp + 16);
-    ADB_HIP(rocprim::radix_sort_pairs<PairSortConfig>((void*)w.sort_tmp.p, tmp, kin, kout, vin, vout, (size_t)n, 0, end_bit,
-                                                      w.stream));
+    ADB_HIP(rocprim::radix_sort_pairs<PairSortConfig>((void*)w.sort_tmp.p, tmp, kin, kout, vin, vout, (size_t)n, 0, end_bit, st));
 }
 
-void assemble_system(World& w) {
+// Queues the pair structure of w.cs on stream4, from the fork point ev_asm_fork
+// (recorded by the caller where the contact set was final when forked, else
+// here): contacts sorted by body pair, pair lists, BSR block rows, the chunk
+// table and the host's copy of the pair lists; its plan (sizes, the device
+// pair count, the set's serial) goes to w.ap for assemble_system.
+void assemble_structure(World& w, bool forked) {
     const int B = w.B;
-    w.bodyG.reserve(12 * (size_t)B);
-    w.bodyH.reserve(144 * (size_t)B);
-    hipLaunchKernelGGL(k_inertia, dim3((144 * B + 255) / 256), dim3(256), 0, w.stream, B, w.P.dt, w.P.kappa_ortho, w.P.gravity[0], w.P.gravity[1], w.P.gravity[2],
-                       w.q.p, w.q0.p, w.qdot.p, w.mass.p, w.Ic.p, w.bodyG.p, w.bodyH.p);
-    ADB_HIP(hipGetLastError());
+    if (!w.stream4) {
+        ADB_HIP(hipStreamCreate(&w.stream4));
+        ADB_HIP(hipEventCreateWithFlags(&w.ev_asm_join, hipEventDisableTiming));
+    }
+    if (!w.ev_asm_fork) ADB_HIP(hipEventCreateWithFlags(&w.ev_asm_fork, hipEventDisableTiming));
+    if (!forked) ADB_HIP(hipEventRecord(w.ev_asm_fork, w.stream));
+    const hipStream_t st = w.stream4;
+    ADB_HIP(hipStreamWaitEvent(st, w.ev_asm_fork, 0));
     const int nc = w.cs.n_pt + w.cs.n_ee;
     int bb = 1; while ((1ll << bb) <= B) bb++;
     // 1. contacts sorted by body pair; pair segments
@@ -921,16 +928,16 @@ void assemble_system(World& w) {
     w.uniq_flag.reserve(std::max(nc, 1)); w.blk_of.reserve(std::max(nc, 1));
     int np = 0;
     if (nc > 0) {
-        hipLaunchKernelGGL(k_pair_keys, dim3((nc + 255) / 256), dim3(256), 0, w.stream, w.cs.n_pt, w.cs.n_ee, w.cs.pt.p, w.cs.ee.p, bb,
+        hipLaunchKernelGGL(k_pair_keys, dim3((nc + 255) / 256), dim3(256), 0, st, w.cs.n_pt, w.cs.n_ee, w.cs.pt.p, w.cs.ee.p, bb,
                            w.akeys.p, w.aval.p);
         ADB_HIP(hipGetLastError());
-        sort_pairs(w, w.akeys.p, w.akeys_s.p, w.aval.p, w.aval_s.p, nc, 2 * bb);
-        hipLaunchKernelGGL(k_flags, dim3((nc + 255) / 256), dim3(256), 0, w.stream, nc, w.akeys_s.p, w.uniq_flag.p);
+        sort_pairs(w, w.akeys.p, w.akeys_s.p, w.aval.p, w.aval_s.p, nc, 2 * bb, st);
+        hipLaunchKernelGGL(k_flags, dim3((nc + 255) / 256), dim3(256), 0, st, nc, w.akeys_s.p, w.uniq_flag.p);
         ADB_HIP(hipGetLastError());
         size_t tmp = 0;
-        ADB_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tmp, w.uniq_flag.p, w.blk_of.p, nc, w.stream));
+        ADB_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tmp, w.uniq_flag.p, w.blk_of.p, nc, st));
         w.sort_tmp.reserve(tmp + 16);
-        ADB_HIP(hipcub::DeviceScan::InclusiveSum(w.sort_tmp.p, tmp, w.uniq_flag.p, w.blk_of.p, nc, w.stream));
+        ADB_HIP(hipcub::DeviceScan::InclusiveSum(w.sort_tmp.p, tmp, w.uniq_flag.p, w.blk_of.p, nc, st));
     }
     // The pair count np = blk_of[nc - 1] stays on the device when the culling
     // pass that made the set bounded it (its overlapping body pairs): every
@@ -942,8 +949,8 @@ void assemble_system(World& w) {
         np = std::min(w.cs.np_bound, nc);
     } else if (nc > 0) {
         if (!w.h_np) ADB_HIP(hipHostMalloc((void**)&w.h_np, sizeof(int), hipHostMallocDefault));
-        ADB_HIP(hipMemcpyAsync(w.h_np, w.blk_of.p + nc - 1, sizeof(int), hipMemcpyDeviceToHost, w.stream));
-        wait_stream(w);
+        ADB_HIP(hipMemcpyAsync(w.h_np, w.blk_of.p + nc - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+        wait_stream(w, st);
         np = *w.h_np;
     }
     // np: the pair count or its bound (npd set) from here on
@@ -951,12 +958,12 @@ void assemble_system(World& w) {
     w.pair_up.reserve(std::max(np, 1)); w.pair_low.reserve(std::max(np, 1));
     w.tkeys.reserve(std::max(np, 1)); w.tkeys_s.reserve(std::max(np, 1)); w.tval.reserve(std::max(np, 1)); w.tval_s.reserve(std::max(np, 1));
     if (nc > 0) {
-        hipLaunchKernelGGL(k_pair_meta, dim3((nc + 255) / 256), dim3(256), 0, w.stream, nc, bb, np, npd, w.akeys_s.p, w.blk_of.p,
+        hipLaunchKernelGGL(k_pair_meta, dim3((nc + 255) / 256), dim3(256), 0, st, nc, bb, np, npd, w.akeys_s.p, w.blk_of.p,
                            w.pair_start.p, w.pair_lo.p, w.pair_hi.p, w.tkeys.p, w.tval.p);
         ADB_HIP(hipGetLastError());
-        sort_pairs(w, w.tkeys.p, w.tkeys_s.p, w.tval.p, w.tval_s.p, np, 2 * bb);
+        sort_pairs(w, w.tkeys.p, w.tkeys_s.p, w.tval.p, w.tval_s.p, np, 2 * bb, st);
     }
-    if (nc == 0) ADB_HIP(hipMemsetAsync(w.pair_start.p, 0, sizeof(int), w.stream));
+    if (nc == 0) ADB_HIP(hipMemsetAsync(w.pair_start.p, 0, sizeof(int), st));
     // 2. BSR structure: row b = pairs (lo, b), diagonal, pairs (b, hi)
     const int nnzb = B + 2 * np;
     w.nnzb = nnzb;
@@ -983,36 +990,55 @@ void assemble_system(World& w) {
         sa.col_idx = w.col_idx.p; sa.diag_blk = w.diag_blk.p; sa.pair_up = w.pair_up.p; sa.pair_low = w.pair_low.p;
         sa.vg_start = w.vg_start.p; sa.chunk_n = w.chunk_n.p; sa.chunk_off = w.chunk_off.p; sa.chunk_pair = w.chunk_blk.p;
         sa.counts = w.ccount.p; sa.host_pack = w.pair_pack.p;
-        hipLaunchKernelGGL(k_structure, dim3(1), dim3(STRUCT_T), 0, w.stream, sa);
+        hipLaunchKernelGGL(k_structure, dim3(1), dim3(STRUCT_T), 0, st, sa);
         ADB_HIP(hipGetLastError());
     } else {
-        hipLaunchKernelGGL(k_body_ranges, dim3((B + 1 + 255) / 256), dim3(256), 0, w.stream, B, np, npd, bb, w.pair_lo.p,
+        hipLaunchKernelGGL(k_body_ranges, dim3((B + 1 + 255) / 256), dim3(256), 0, st, B, np, npd, bb, w.pair_lo.p,
                            w.tkeys_s.p, w.up_start.p, w.low_start.p, w.rowcnt.p);
         ADB_HIP(hipGetLastError());
         {
             size_t tmp = 0;
-            ADB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, w.rowcnt.p, w.row_ptr.p, B + 1, w.stream));
+            ADB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, w.rowcnt.p, w.row_ptr.p, B + 1, st));
             w.sort_tmp.reserve(tmp + 16);
-            ADB_HIP(hipcub::DeviceScan::ExclusiveSum(w.sort_tmp.p, tmp, w.rowcnt.p, w.row_ptr.p, B + 1, w.stream));
+            ADB_HIP(hipcub::DeviceScan::ExclusiveSum(w.sort_tmp.p, tmp, w.rowcnt.p, w.row_ptr.p, B + 1, st));
         }
-        hipLaunchKernelGGL(k_fill_cols, dim3((B + 255) / 256), dim3(256), 0, w.stream, B, bb, w.row_ptr.p, w.up_start.p,
+        hipLaunchKernelGGL(k_fill_cols, dim3((B + 255) / 256), dim3(256), 0, st, B, bb, w.row_ptr.p, w.up_start.p,
                            w.low_start.p, w.pair_hi.p, w.tkeys_s.p, w.tval_s.p, w.col_idx.p, w.diag_blk.p, w.pair_up.p, w.pair_low.p);
         ADB_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_vg_start, dim3((B + 1 + 255) / 256), dim3(256), 0, w.stream, B, w.cs.n_vg, w.cs.vg.p, w.vg_start.p);
+        hipLaunchKernelGGL(k_vg_start, dim3((B + 1 + 255) / 256), dim3(256), 0, st, B, w.cs.n_vg, w.cs.vg.p, w.vg_start.p);
         ADB_HIP(hipGetLastError());
         // 3. chunks of at most PCH contacts per pair
-        hipLaunchKernelGGL(k_pair_chunks, dim3((np + 1 + 255) / 256), dim3(256), 0, w.stream, np, npd, w.pair_start.p, w.chunk_n.p);
+        hipLaunchKernelGGL(k_pair_chunks, dim3((np + 1 + 255) / 256), dim3(256), 0, st, np, npd, w.pair_start.p, w.chunk_n.p);
         ADB_HIP(hipGetLastError());
         {
             size_t tmp = 0;
-            ADB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, w.chunk_n.p, w.chunk_off.p, np + 1, w.stream));
+            ADB_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, w.chunk_n.p, w.chunk_off.p, np + 1, st));
             w.sort_tmp.reserve(tmp + 16);
-            ADB_HIP(hipcub::DeviceScan::ExclusiveSum(w.sort_tmp.p, tmp, w.chunk_n.p, w.chunk_off.p, np + 1, w.stream));
+            ADB_HIP(hipcub::DeviceScan::ExclusiveSum(w.sort_tmp.p, tmp, w.chunk_n.p, w.chunk_off.p, np + 1, st));
         }
-        hipLaunchKernelGGL(k_chunk_owner, dim3((np + 255) / 256 + 1), dim3(256), 0, w.stream, np, npd, w.chunk_off.p, w.chunk_blk.p,
+        hipLaunchKernelGGL(k_chunk_owner, dim3((np + 255) / 256 + 1), dim3(256), 0, st, np, npd, w.chunk_off.p, w.chunk_blk.p,
                            w.ccount.p);
         ADB_HIP(hipGetLastError());
     }
+    groups_fetch(w, np, npd, one_wg ? w.pair_pack.p : nullptr, st);
+    ADB_HIP(hipEventRecord(w.ev_asm_join, st));
+    w.ap.pending = true; w.ap.serial = w.cs.serial;
+    w.ap.nc = nc; w.ap.np = np; w.ap.npd = npd; w.ap.max_chunks = max_chunks;
+}
+
+// The values: inertia terms, then (after the pair structure and the contact
+// records of the side streams) the chunk reduction, the off-diagonal and
+// diagonal blocks and gradients, the host's groups and their inverses.
+void assemble_system(World& w) {
+    if (!w.ap.pending || w.ap.serial != w.cs.serial) assemble_structure(w, false);
+    w.ap.pending = false;
+    const int B = w.B, nc = w.ap.nc, np = w.ap.np, max_chunks = w.ap.max_chunks;
+    const int* npd = w.ap.npd;
+    w.bodyG.reserve(12 * (size_t)B);
+    w.bodyH.reserve(144 * (size_t)B);
+    hipLaunchKernelGGL(k_inertia, dim3((144 * B + 255) / 256), dim3(256), 0, w.stream, B, w.P.dt, w.P.kappa_ortho, w.P.gravity[0],
+                       w.P.gravity[1], w.P.gravity[2], w.q.p, w.q0.p, w.qdot.p, w.mass.p, w.Ic.p, w.bodyG.p, w.bodyH.p);
+    ADB_HIP(hipGetLastError());
     PairArgs a;
     a.np = np; a.npd = npd; a.n_pt = w.cs.n_pt; a.n_ee = w.cs.n_ee; a.B = B;
     a.cid = w.aval_s.p;
@@ -1024,7 +1050,7 @@ void assemble_system(World& w) {
     a.vg = w.cs.vg.p; a.vg_start = w.vg_start.p; a.vgK = w.vgK.p; a.voff = w.voff.p; a.xbar = w.xbar.p;
     a.diag_blk = w.diag_blk.p;
     a.Hval = w.Hval.p; a.grad = w.grad.p;
-    groups_fetch(w, np, npd, one_wg ? w.pair_pack.p : nullptr);
+    ADB_HIP(hipStreamWaitEvent(w.stream, w.ev_asm_join, 0));   // the pair structure
     if (w.side_pending) {   // records of the 4-point contacts and 3-point projections (side streams)
         ADB_HIP(hipStreamWaitEvent(w.stream, w.ev_side[1], 0));
         ADB_HIP(hipStreamWaitEvent(w.stream, w.ev_pk, 0));

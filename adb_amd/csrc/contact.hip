@@ -1381,7 +1381,13 @@ void contact_terms(World& w) {
     w.ccount.reserve(4);
     w.ctk.reserve(TKN * nn);
     w.clam.reserve(nn);
-    ADB_HIP(hipMemsetAsync(w.ccount.p, 0, 4 * sizeof(int), w.stream));
+    // the contact set is final here: the assembly's pair structure, which
+    // needs nothing else, forks from this point onto its own stream (queued
+    // below, after the contact kernels)
+    if (!w.ev_asm_fork) ADB_HIP(hipEventCreateWithFlags(&w.ev_asm_fork, hipEventDisableTiming));
+    ADB_HIP(hipEventRecord(w.ev_asm_fork, w.stream));
+    // class counts [0, 3) (entry 3, the chunk count, belongs to the assembly)
+    ADB_HIP(hipMemsetAsync(w.ccount.p, 0, 3 * sizeof(int), w.stream));
     if (n > 0) {
         WorldArgs a;
         a.kp = make_kp(w.P);
@@ -1464,6 +1470,7 @@ void contact_terms(World& w) {
         hipLaunchKernelGGL(k_vg_world, dim3((nv + 255) / 256), dim3(256), 0, w.stream, v);
         ADB_HIP(hipGetLastError());
     }
+    assemble_structure(w, true);
 }
 
 } // namespace adb

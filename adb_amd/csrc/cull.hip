@@ -965,6 +965,7 @@ static void set_pass(World& w, const double* X, const double* X1, ContactSet& ou
         out.n_pt = npt; out.n_ee = nee; out.n_vg = nvg;
         // PT / EE contacts only come from the overlapping body pairs of the pass
         out.np_bound = hashed ? -1 : h[2 * NBK + 1];
+        out.serial = ++w.cs_serial;
         return;
     }
     throw AdbError(ADB_E_CAPACITY, "contact-set capacity could not be grown");
@@ -1088,6 +1089,7 @@ void trial_set(World& w, const double* X, const ContactSet& cand, ContactSet& ou
     const int* h = w.h_tcount;
     out.n_pt = h[0]; out.n_ee = h[1] - h[0]; out.n_vg = h[2] - h[1];
     out.np_bound = cand.np_bound;   // a subset of the candidates
+    out.serial = ++w.cs_serial;
 }
 
 // The CCD step bound of X0 -> X1 (model/abd.cpp:142-196) in two parts: the

@@ -89,6 +89,7 @@ struct ContactSet {
     DBuf<int2> vg;          // (I, v)
     int n_pt = 0, n_ee = 0, n_vg = 0;
     int np_bound = -1;      // bound of the body pairs with PT / EE contacts (the culling pass's overlapping pairs), or -1
+    long long serial = 0;   // distinct per built set (World::cs_serial)
 };
 
 struct World {
@@ -101,6 +102,17 @@ struct World {
     hipStream_t stream3 = nullptr;   // third stream: 3-point projections
     hipEvent_t ev_pk = nullptr;      // its join
     bool side_pending = false;
+    // the assembly's pair structure (sort by body pair, block rows, chunks) on
+    // a fourth stream, forked where the contact set is final (ev_asm_fork) and
+    // joined before the assembly reduction (ev_asm_join)
+    hipStream_t stream4 = nullptr;
+    hipEvent_t ev_asm_fork = nullptr, ev_asm_join = nullptr;
+    // the queued pair structure's plan: the contact set's serial, contacts,
+    // pair count (or its bound with npd the count on the device), chunk launch size
+    struct AsmPlan {
+        bool pending = false; long long serial = -1; int nc = 0, np = 0, max_chunks = 0; const int* npd = nullptr;
+    } ap;
+    long long cs_serial = 0;         // serial of the last contact set built
     std::string err;
 
     // ---- scene (device)
@@ -272,6 +284,7 @@ void prof_flush(World& w);
 // (ADB_SPIN_WAIT=1, default) answers as soon as the work is done, where the
 // blocking call may first put the thread to sleep; both report errors.
 void wait_stream(World& w);
+void wait_stream(World& w, hipStream_t s);
 void wait_event(World& w, hipEvent_t e);
 // Pinned host copy of 4 device counters (enqueued on the world stream) when
 // profiling is on, else nullptr.
@@ -304,6 +317,7 @@ void hash_sets(World& w, const double* X, double r, unsigned long long* pt_keys,
                int sub, int* bcnt);                                                      // hash.hip
 void hash_ccd(World& w, const double* X0, const double* X1, unsigned long long* toi_bits); // hash.hip
 void contact_terms(World& w);                                                            // contact.hip
+void assemble_structure(World& w, bool forked);                                         // assemble.hip
 void assemble_system(World& w);                                                          // assemble.hip
 // `after` (if given) queues work behind a short solve's stop test, before
 // its wait; *after_valid tells whether that solve was then complete (else

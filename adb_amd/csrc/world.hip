@@ -53,6 +53,12 @@ void wait_stream(World& w) {
     while ((e = hipStreamQuery(w.stream)) == hipErrorNotReady) {}
     ADB_HIP(e);
 }
+void wait_stream(World& w, hipStream_t s) {
+    if (!w.spin_wait) { ADB_HIP(hipStreamSynchronize(s)); return; }
+    hipError_t e;
+    while ((e = hipStreamQuery(s)) == hipErrorNotReady) {}
+    ADB_HIP(e);
+}
 void wait_event(World& w, hipEvent_t ev) {
     if (!w.spin_wait) { ADB_HIP(hipEventSynchronize(ev)); return; }
     hipError_t e;
@@ -593,6 +599,9 @@ static void free_world(World& w) {
     w.ev_pk = nullptr;
     if (w.stream3) (void)hipStreamDestroy(w.stream3);
     w.stream3 = nullptr;
+    for (hipEvent_t* e : {&w.ev_asm_fork, &w.ev_asm_join}) { if (*e) (void)hipEventDestroy(*e); *e = nullptr; }
+    if (w.stream4) (void)hipStreamDestroy(w.stream4);
+    w.stream4 = nullptr;
     if (w.stream) (void)hipStreamDestroy(w.stream);
     w.stream = nullptr;
 }
