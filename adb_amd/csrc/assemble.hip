@@ -944,7 +944,7 @@ void assemble_structure(World& w, bool forked) {
     // launch below is sized by that bound and reads np there; the host gets
     // np with the pair lists (build_groups).  Else it is read here.
     const int* npd = nullptr;
-    if (nc > 0 && w.cs.np_bound >= 0) {
+    if (nc > 0 && w.cs.np_bound >= 0 && !w.general_paths) {
         npd = w.blk_of.p + nc - 1;
         np = std::min(w.cs.np_bound, nc);
     } else if (nc > 0) {
@@ -978,7 +978,7 @@ void assemble_structure(World& w, bool forked) {
     const int max_chunks = (np + nc / PCH + 7) / 8 * 8;
     w.chunk_blk.reserve(std::max(max_chunks, 1));
     w.chunk_part.reserve(PART * (size_t)std::max(max_chunks, 1));
-    const bool one_wg = B + np <= STRUCT_MAX;
+    const bool one_wg = B + np <= STRUCT_MAX && !w.general_paths;
     if (one_wg) {
         // 3. block rows, vertex-ground starts and chunks in one workgroup
         w.pair_pack.reserve(3 * (size_t)np + 2);

@@ -957,7 +957,7 @@ static void set_pass(World& w, const double* X, const double* X1, ContactSet& ou
             KeySets ks;
             ks.L[0] = Lp; ks.L[1] = Le;
             ks.sub = (int)sub; ks.nvg = nvg; ks.V = w.V; ks.E = std::max(w.E, 1);
-            ks.gs = nall <= RANK_MAX ? GS_RANK : GS_RADIX;
+            ks.gs = nall <= RANK_MAX && !w.general_paths ? GS_RANK : GS_RADIX;
             ks.ng_pt = (w.V >> ks.gs) + 1; ks.ng_ee = (ks.E >> ks.gs) + 1;
             ks.buckets = w.keys_a.p; ks.vg = w.vg_keys.p;
             sort_keys(w, ks, npt, nee, nall, std::max(most, nvg), w.keys_b.p + half, w.keys_b.p, out);

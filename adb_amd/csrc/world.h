@@ -146,6 +146,10 @@ struct World {
     int* h_small = nullptr;          // pinned host scratch (256 ints) for the small synchronous reads of the culling passes
     ContactSet cand;                 // line search: candidates of the search segment
     bool spin_wait = true;           // host waits spin on the query (ADB_SPIN_WAIT=0: blocking calls)
+    // the paths taken above a size at every size (tests): the contact-set keys
+    // radix-sorted, the assembly's pair count read by the host, its block
+    // rows and chunk table from the multi-launch kernels
+    bool general_paths = false;
     bool ls_cand = true;             // trial sets filtered from them (ADB_LS_CANDIDATES=0: full passes)
     int ls_prev = 2;                 // trials of the previous line search
     // barrier + inertia energy at the point the last line search accepted (the

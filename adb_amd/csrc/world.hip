@@ -471,6 +471,7 @@ extern "C" int adb_world_create(const adb_scene* s, const adb_params* p, adb_wor
         if (const char* g = std::getenv("ADB_PCG_GRAPH")) w.use_graphs = std::strcmp(g, "0") != 0;
         if (const char* g = std::getenv("ADB_LS_CANDIDATES")) w.ls_cand = std::strcmp(g, "0") != 0;
         if (const char* g = std::getenv("ADB_SPIN_WAIT")) w.spin_wait = std::strcmp(g, "0") != 0;
+        if (const char* g = std::getenv("ADB_GENERAL_PATHS")) w.general_paths = std::strcmp(g, "0") != 0;
         w.h_voff.assign(s->vert_offset, s->vert_offset + w.B + 1);
         w.h_foff.assign(s->face_offset, s->face_offset + w.B + 1);
         w.h_eoff.assign(s->edge_offset, s->edge_offset + w.B + 1);

@@ -206,6 +206,30 @@ def test_hash_mode_trajectory_equals_iaabb_mode():
     assert np.array_equal(qs[0], qs[1])
 
 
+@pytest.mark.parametrize("name,steps", [("cube_stack", 8), ("spheres", 6)])
+def test_general_paths_give_identical_steps(name, steps, monkeypatch):
+    # the paths taken above a size (contact-set keys radix-sorted instead of
+    # ranked, the assembly's pair count read by the host instead of bounded,
+    # its block rows and chunk table from the multi-launch kernels instead of
+    # one workgroup) give the same steps bit for bit
+    _require_gpu()
+    mk, kw = SCENES[name]
+    sc = mk()
+    runs = []
+    for general in ("0", "1"):
+        monkeypatch.setenv("ADB_GENERAL_PATHS", general)
+        gw = World(sc, default_params(**kw))
+        counts = []
+        for s in range(steps):
+            rc, st = gw.step()
+            assert rc == 0, (general, s, rc)
+            counts.append((st.newton_iters, st.ls_trials, st.pcg_iters, st.n_pt, st.n_ee, st.n_vg))
+        runs.append((gw.state(), counts))
+    assert runs[0][1] == runs[1][1]
+    assert sum(c[3] + c[4] for c in runs[0][1]) > 0
+    assert np.array_equal(runs[0][0][0], runs[1][0][0]) and np.array_equal(runs[0][0][1], runs[1][0][1])
+
+
 @pytest.mark.parametrize("name,steps", [("cube_stack", 10), ("spheres", 8)])
 def test_line_search_candidate_sets_equal_full_passes(name, steps, monkeypatch):
     # trial sets filtered from the candidates of the search segment equal the
