@@ -14,9 +14,9 @@
 //   class 2 (three points, D = 6): record fields (k_contact_scalars2), then
 //     cyclic Jacobi on the packed 6 x 6 matrix in registers (k_contact_pk2);
 //   class 3 (four points, or any mollified EE, D = 9): the 9 x 9 matrix into
-//     the record, then cyclic Jacobi in registers (k_contact_c3 and
-//     k_contact_psd9, on a side stream; nine threads per contact build the
-//     matrix while the class is small).
+//     the record, then cyclic Jacobi in registers (k_contact_c3r and
+//     k_contact_psd9, or k_contact_c3f, on a side stream; nine threads per
+//     contact build the matrix while the class is small).
 // Each contact writes a compact body-space record (world.h) that the
 // assembly expands into its four 12 x 12 blocks; the class 1 and class 2
 // record kernels stage it per wavefront in LDS and store it cooperatively
@@ -847,7 +847,7 @@ __device__ __forceinline__ void contact_small(const WorldArgs& a, const Stage& s
 }
 // Class 3 contacts (four points: point-plane, edge-edge interior, every
 // mollified edge-edge) in two launches.
-// k_contact_c3: ten threads per contact.  Thread role r < 9 builds the entries
+// k_contact_c3r: ten threads per contact.  Thread role r < 9 builds the entries
 // of the 9x9 Helmert-coordinate matrix K = (C (x) I3) Mred (C^T (x) I3) +
 // mollifier terms (C = h4 S^T, Cm the same for y = p1 - p0, z = p3 - p2) of
 // one component pair (i, j) = (r / 3, r % 3) and stores them with their
@@ -1065,18 +1065,26 @@ __device__ __forceinline__ void psd9(const WorldArgs& a, int cid) {
             }
     }
 }
-__global__ __launch_bounds__(256) void k_contact_c3(WorldArgs a) {
+// The two 4-point paths are separate kernels (both launched; the one the
+// device count does not choose returns at once), so the role threads of the
+// split path get the registers of their own code: the fused path's Jacobi
+// holds ~360 (one wavefront per SIMD), the roles far fewer.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_contact_c3r(WorldArgs a) {
     const int n3 = a.counts[2];
-    if (n3 < C3_SPLIT_MAX) {
-        const long long n = (long long)C3_ROLES * n3;
-        for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
-            const int cid = a.list3[t / C3_ROLES], role = (int)(t % C3_ROLES);
-            C3Ctx x;
-            c3_setup(a, cid, x);
-            if (role < 9) c3_kbuild(x, role);
-            else c3_fields(a, cid, x);
-        }
-    } else {
+    if (n3 >= C3_SPLIT_MAX) return;                   // k_contact_c3f
+    const long long n = (long long)C3_ROLES * n3;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        const int cid = a.list3[t / C3_ROLES], role = (int)(t % C3_ROLES);
+        C3Ctx x;
+        c3_setup(a, cid, x);
+        if (role < 9) c3_kbuild(x, role);
+        else c3_fields(a, cid, x);
+    }
+}
+__global__ __launch_bounds__(256) void k_contact_c3f(WorldArgs a) {
+    const int n3 = a.counts[2];
+    if (n3 < C3_SPLIT_MAX) return;                    // k_contact_c3r + k_contact_psd9
+    {
         for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n3; p += gridDim.x * blockDim.x) {
             const int cid = a.list3[p];
             C3Ctx x;
@@ -1090,7 +1098,7 @@ __global__ __launch_bounds__(256) void k_contact_c3(WorldArgs a) {
 }
 __global__ __launch_bounds__(64) void k_contact_psd9(WorldArgs a) {
     const int n3 = a.counts[2];
-    if (n3 >= C3_SPLIT_MAX) return;                   // projected by k_contact_c3
+    if (n3 >= C3_SPLIT_MAX) return;                   // projected by k_contact_c3f
     for (int p = blockIdx.x * 64 + threadIdx.x; p < n3; p += gridDim.x * 64) psd9(a, a.list3[p]);
 }
 
@@ -1423,8 +1431,9 @@ void contact_terms(World& w) {
                 ProfScope ps(w, PK_CONTACT_GROUP, 0.0, w.stream2);
                 ps.counts(snap, 0.0, 0.0, 1608.0, 0.0);
                 // the class count is on the device: grids sized for the worst case, grid-stride loops
-                hipLaunchKernelGGL(k_contact_c3, dim3(std::min((C3_ROLES * n + 255) / 256, 16 * w.num_cu)), dim3(256), 0,
+                hipLaunchKernelGGL(k_contact_c3r, dim3(std::min((C3_ROLES * n + 255) / 256, 16 * w.num_cu)), dim3(256), 0,
                                    w.stream2, a);
+                hipLaunchKernelGGL(k_contact_c3f, dim3(std::min((n + 255) / 256, 16 * w.num_cu)), dim3(256), 0, w.stream2, a);
                 ADB_HIP(hipGetLastError());
                 if (a.kp.psd)
                     hipLaunchKernelGGL(k_contact_psd9, dim3(std::min((n + 63) / 64, 4 * w.num_cu)), dim3(64), 0, w.stream2, a);
