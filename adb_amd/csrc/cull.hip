@@ -190,6 +190,71 @@ __global__ void k_body_box(const int* __restrict__ voff, const double* __restric
     }
 }
 
+// The same boxes and spheres with one wavefront per body (four bodies per
+// workgroup), for bodies of at most 64 * BOX_VC vertices: each lane keeps its
+// positions in registers between the box pass and the sphere pass, and both
+// reductions stay inside the wavefront (no LDS, no workgroup barrier).  The
+// results equal k_body_box's bit for bit: the minima, maxima and squared
+// distances come from the same expressions, and a minimum or maximum does not
+// depend on the order it is taken in.
+constexpr int BOX_VC = 16;
+template <bool SWEPT>
+__global__ __launch_bounds__(256) void k_body_box_w(int B, const int* __restrict__ voff, const double* __restrict__ X0,
+                                                    const double* __restrict__ X1, double infl, double* __restrict__ box,
+                                                    double* __restrict__ sph, int* __restrict__ zero, int nzero,
+                                                    unsigned long long* __restrict__ toi, unsigned long long toi_one) {
+    if (blockIdx.x == 0) {
+        for (int i = threadIdx.x; i < nzero; i += blockDim.x) zero[i] = 0;
+        if (toi && threadIdx.x < 2) toi[threadIdx.x] = toi_one;
+    }
+    const int b = blockIdx.x * 4 + threadIdx.x / 64, ln = threadIdx.x % 64;
+    if (b >= B) return;
+    const int v0 = voff[b], nv = voff[b + 1] - v0;
+    double x[BOX_VC][3], y[SWEPT ? BOX_VC : 1][3];
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+#pragma unroll
+    for (int i = 0; i < BOX_VC; i++) {
+        const int t = ln + 64 * i;
+        if (t < nv) {
+            const size_t o = 3 * (size_t)(v0 + t);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                x[i][k] = X0[o + k];
+                lo[k] = fmin(lo[k], x[i][k]); hi[k] = fmax(hi[k], x[i][k]);
+                if (SWEPT) {
+                    y[i][k] = X1[o + k];
+                    lo[k] = fmin(lo[k], y[i][k]); hi[k] = fmax(hi[k], y[i][k]);
+                }
+            }
+        }
+    }
+    double cen[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        for (int m = 32; m >= 1; m >>= 1) { lo[k] = fmin(lo[k], __shfl_xor(lo[k], m)); hi[k] = fmax(hi[k], __shfl_xor(hi[k], m)); }
+        cen[k] = 0.5 * (lo[k] + hi[k]);
+    }
+    if (ln < 3) {
+        const double l = ln == 0 ? lo[0] : ln == 1 ? lo[1] : lo[2], h = ln == 0 ? hi[0] : ln == 1 ? hi[1] : hi[2];
+        box[6 * (size_t)b + ln] = l - infl;
+        box[6 * (size_t)b + 3 + ln] = h + infl;
+    }
+    double r2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < BOX_VC; i++) {
+        if (ln + 64 * i < nv) {
+            const double dx = x[i][0] - cen[0], dy = x[i][1] - cen[1], dz = x[i][2] - cen[2];
+            r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+            if (SWEPT) {
+                const double ex = y[i][0] - cen[0], ey = y[i][1] - cen[1], ez = y[i][2] - cen[2];
+                r2 = fmax(r2, ex * ex + ey * ey + ez * ez);
+            }
+        }
+    }
+    for (int m = 32; m >= 1; m >>= 1) r2 = fmax(r2, __shfl_xor(r2, m));
+    if (ln < 4) sph[4 * (size_t)b + ln] = ln < 3 ? cen[ln == 0 ? 0 : ln == 1 ? 1 : 2] : sqrt(r2) * (1.0 + 1e-12) + 1e-300;
+}
+
 // ------------------------------------------------------------------ body pairs
 // tiled all-pairs: thread (I, J) over a 2-D grid, only I < J, wave-aggregated append
 // body pairs whose (inflated) boxes overlap and whose bounding spheres are
@@ -826,8 +891,14 @@ static void body_pairs(World& w, const double* X0, const double* X1, double infl
     w.bsph.reserve(4 * (size_t)w.B);
     {
         ProfScope ps(w, PK_BODY_BOX, (X1 ? 48.0 : 24.0) * w.V + 52.0 * w.B);
-        hipLaunchKernelGGL(k_body_box, dim3(w.B), dim3(NB_THREADS), 0, w.stream, w.voff.p, X0, X1, infl, w.bbox.p, w.bsph.p,
-                           zero, nzero, toi, order_key_host(1.0));
+        if (w.max_vpb <= 64 * BOX_VC && !w.general_paths) {
+            auto kern = X1 ? k_body_box_w<true> : k_body_box_w<false>;
+            hipLaunchKernelGGL(kern, dim3((w.B + 3) / 4), dim3(256), 0, w.stream, w.B, w.voff.p, X0, X1, infl, w.bbox.p,
+                               w.bsph.p, zero, nzero, toi, order_key_host(1.0));
+        } else {
+            hipLaunchKernelGGL(k_body_box, dim3(w.B), dim3(NB_THREADS), 0, w.stream, w.voff.p, X0, X1, infl, w.bbox.p,
+                               w.bsph.p, zero, nzero, toi, order_key_host(1.0));
+        }
     }
     ADB_HIP(hipGetLastError());
     if (cnt < zero || cnt >= zero + nzero) throw AdbError(ADB_E_ARG, "body-pair counter outside the cleared range");

@@ -148,7 +148,8 @@ struct World {
     bool spin_wait = true;           // host waits spin on the query (ADB_SPIN_WAIT=0: blocking calls)
     // the paths taken above a size at every size (tests): the contact-set keys
     // radix-sorted, the assembly's pair count read by the host, its block
-    // rows and chunk table from the multi-launch kernels
+    // rows and chunk table from the multi-launch kernels, the body boxes one
+    // workgroup per body
     bool general_paths = false;
     bool ls_cand = true;             // trial sets filtered from them (ADB_LS_CANDIDATES=0: full passes)
     int ls_prev = 2;                 // trials of the previous line search

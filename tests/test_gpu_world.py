@@ -211,7 +211,8 @@ def test_general_paths_give_identical_steps(name, steps, monkeypatch):
     # the paths taken above a size (contact-set keys radix-sorted instead of
     # ranked, the assembly's pair count read by the host instead of bounded,
     # its block rows and chunk table from the multi-launch kernels instead of
-    # one workgroup) give the same steps bit for bit
+    # one workgroup, the body boxes from one workgroup per body instead of one
+    # wavefront) give the same steps bit for bit
     _require_gpu()
     mk, kw = SCENES[name]
     sc = mk()
