@@ -19,6 +19,7 @@
 // dropped, their diagonal block is I and their gradient 0.
 #include <hipcub/hipcub.hpp>
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <vector>
 #include "dmath.h"
@@ -223,6 +224,26 @@ __device__ int block_scan_excl(const int* in, int* out, int n, int* sh) {
     __syncthreads();
     return total;
 }
+// in-place suffix minimum of v[0, n) by one workgroup of STRUCT_T threads
+// (contiguous ranges per thread): v[i] = min(v[i], ..., v[n - 1])
+__device__ void block_suffix_min(int* v, int n, int* sh) {
+    const int t = threadIdx.x, per = (n + STRUCT_T - 1) / STRUCT_T;
+    const int i0 = min(n, t * per), i1 = min(n, i0 + per);
+    int m = INT_MAX;
+    for (int i = i0; i < i1; i++) m = min(m, v[i]);
+    int suf = m;   // minimum over this lane and the lanes after it
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_down(suf, d);
+        if (__lane_id() + d < 64) suf = min(suf, o);
+    }
+    if (__lane_id() == 0) sh[t / 64] = suf;
+    __syncthreads();
+    int right = __shfl_down(suf, 1);   // minimum over the ranges after this thread's
+    if (__lane_id() == 63) right = INT_MAX;
+    for (int wv = t / 64 + 1; wv < STRUCT_T / 64; wv++) right = min(right, sh[wv]);
+    for (int i = i1 - 1; i >= i0; i--) { right = min(right, v[i]); v[i] = right; }
+    __syncthreads();
+}
 struct StructArgs {
     int B, npb, bb, n_vg;
     const int* npd;
@@ -234,15 +255,26 @@ struct StructArgs {
 __global__ __launch_bounds__(STRUCT_T) void k_structure(StructArgs a) {
     __shared__ int sh[STRUCT_T / 64];
     const int np = pair_count(a.npb, a.npd), t = threadIdx.x;
-    // k_body_ranges and k_vg_start
-    for (int b = t; b <= a.B; b += STRUCT_T) {
-        a.up_start[b] = lower_bound_int(a.pair_lo, np, b);
-        a.low_start[b] = lower_bound_hi(a.tk, np, a.bb, b);
-        int lo = 0, hi = a.n_vg;
-        while (lo < hi) { const int mid = (lo + hi) / 2; if (a.vg[mid].x < b) lo = mid + 1; else hi = mid; }
-        a.vg_start[b] = lo;
+    // k_body_ranges and k_vg_start without their binary searches (chains of
+    // dependent loads): each body's first entry in a list sorted by body is
+    // marked, the other bodies get the list length, and a suffix minimum over
+    // the bodies gives every body the first entry at or after it
+    for (int b = t; b <= a.B; b += STRUCT_T) { a.up_start[b] = np; a.low_start[b] = np; a.vg_start[b] = a.n_vg; }
+    __syncthreads();
+    for (int i = t; i < np; i += STRUCT_T) {
+        const int k = a.pair_lo[i];
+        if (i == 0 || a.pair_lo[i - 1] != k) a.up_start[k] = i;
+        const int h = (int)(a.tk[i] >> a.bb);
+        if (i == 0 || (int)(a.tk[i - 1] >> a.bb) != h) a.low_start[h] = i;
+    }
+    for (int i = t; i < a.n_vg; i += STRUCT_T) {
+        const int k = a.vg[i].x;
+        if (i == 0 || a.vg[i - 1].x != k) a.vg_start[k] = i;
     }
     __syncthreads();
+    block_suffix_min(a.up_start, a.B + 1, sh);
+    block_suffix_min(a.low_start, a.B + 1, sh);
+    block_suffix_min(a.vg_start, a.B + 1, sh);
     for (int b = t; b < a.B; b += STRUCT_T)
         a.rowcnt[b] = (a.low_start[b + 1] - a.low_start[b]) + 1 + (a.up_start[b + 1] - a.up_start[b]);
     __syncthreads();
@@ -780,11 +812,17 @@ static void build_groups(World& w) {
     // pairs by decreasing contact count, ties by index (counting sort)
     int wmax = 0;
     for (int p = 0; p < np; p++) wmax = std::max(wmax, ps[p + 1] - ps[p]);
-    std::vector<int> bucket(wmax + 2, 0), order(np);
+    std::vector<int>& bucket = w.hg_bucket;
+    std::vector<int>& order = w.hg_order;
+    bucket.assign(wmax + 2, 0);
+    order.resize(np);
     for (int p = 0; p < np; p++) bucket[wmax - (ps[p + 1] - ps[p]) + 1]++;
     for (int i = 1; i <= wmax + 1; i++) bucket[i] += bucket[i - 1];
     for (int p = 0; p < np; p++) order[bucket[wmax - (ps[p + 1] - ps[p])]++] = p;
-    std::vector<int> parent(B), size(B, 1);
+    std::vector<int>& parent = w.hg_parent;
+    std::vector<int>& size = w.hg_size;
+    parent.resize(B);
+    size.assign(B, 1);
     for (int b = 0; b < B; b++) parent[b] = b;
     auto find = [&](int x) { while (parent[x] != x) { parent[x] = parent[parent[x]]; x = parent[x]; } return x; };
     for (int p : order) {
@@ -793,24 +831,31 @@ static void build_groups(World& w) {
         if (ra > rb) std::swap(ra, rb);
         parent[rb] = ra; size[ra] += size[rb];
     }
-    // groups in order of their smallest body; bodies ascending inside
-    std::vector<int> gid(B, -1), first;
-    w.h_grp_off.assign(1, 0);
-    w.h_grp_body.clear();
-    w.h_grp_moff.assign(1, 0);
-    std::vector<std::vector<int>> members;
+    // groups in order of their smallest body; bodies ascending inside: group
+    // numbers by first appearance, then a counting sort of the bodies
+    std::vector<int>& gid = w.hg_gid;    // group of each root
+    std::vector<int>& bg = w.hg_bg;      // group of each body
+    gid.assign(B, -1);
+    bg.resize(B);
+    int G = 0;
     for (int b = 0; b < B; b++) {
         const int r = find(b);
-        if (gid[r] < 0) { gid[r] = (int)members.size(); members.emplace_back(); }
-        members[gid[r]].push_back(b);
+        if (gid[r] < 0) gid[r] = G++;
+        bg[b] = gid[r];
     }
-    for (const auto& m : members) {
-        for (int b : m) w.h_grp_body.push_back(b);
-        w.h_grp_off.push_back((int)w.h_grp_body.size());
-        const long long n = 12 * (long long)m.size();
-        w.h_grp_moff.push_back(w.h_grp_moff.back() + n * n);
+    w.h_grp_off.assign(G + 1, 0);
+    for (int b = 0; b < B; b++) w.h_grp_off[bg[b] + 1]++;
+    for (int g = 0; g < G; g++) w.h_grp_off[g + 1] += w.h_grp_off[g];
+    w.h_grp_body.resize(B);
+    gid.assign(w.h_grp_off.begin(), w.h_grp_off.end() - 1);   // fill positions
+    for (int b = 0; b < B; b++) w.h_grp_body[gid[bg[b]]++] = b;
+    w.h_grp_moff.resize(G + 1);
+    w.h_grp_moff[0] = 0;
+    for (int g = 0; g < G; g++) {
+        const long long n = 12 * (long long)(w.h_grp_off[g + 1] - w.h_grp_off[g]);
+        w.h_grp_moff[g + 1] = w.h_grp_moff[g] + n * n;
     }
-    w.n_groups = (int)members.size();
+    w.n_groups = G;
     if (w.S > 1) {
         // groups are numbered in order of their smallest body and never span two
         // scenes (no body pair crosses scenes): scene s owns groups
@@ -818,7 +863,7 @@ static void build_groups(World& w) {
         w.h_sc_goff.assign(w.S + 1, w.n_groups);
         int g = 0;
         for (int sc = 0; sc < w.S; sc++) {
-            while (g < w.n_groups && members[g][0] < w.h_scene_off[sc]) g++;
+            while (g < w.n_groups && w.h_grp_body[w.h_grp_off[g]] < w.h_scene_off[sc]) g++;
             w.h_sc_goff[sc] = g;
         }
         w.sc_goff.reserve(w.S + 1);
@@ -841,7 +886,6 @@ static void build_groups(World& w) {
     // it has completed before it is rewritten).  Offsets depend on B only (at
     // most B groups), so the tables' device addresses stay the same from one
     // Newton iteration to the next (the captured PCG batch is keyed on them).
-    const int G = w.n_groups;
     const size_t nll = (size_t)B + 1, nint = (size_t)(B + 1) + B + B + 1, total = nll + (nint + 1) / 2;
     if (!w.ev_grp) ADB_HIP(hipEventCreateWithFlags(&w.ev_grp, hipEventDisableTiming));
     wait_event(w, w.ev_grp);

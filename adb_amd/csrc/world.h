@@ -209,6 +209,8 @@ struct World {
     int n_glist[3] = {0, 0, 0};      // groups of 5..8, 2..4 and 1 bodies, in that order in glist
     std::vector<int> h_grp_off, h_grp_body;
     std::vector<long long> h_grp_moff;
+    // host scratch of the group building (kept to avoid allocations per call)
+    std::vector<int> hg_bucket, hg_order, hg_parent, hg_size, hg_gid, hg_bg;
     int nnzb = 0;
     // ---- PCG
     DBuf<double> pcg_r, pcg_z, pcg_p, pcg_p2, pcg_Ap, partial, scal;
@@ -220,6 +222,7 @@ struct World {
     int h_pairs_n = 0;               // entries per list in h_pairs (the pair count or its bound)
     int* h_np = nullptr;             // pinned host copy of the pair count (read when unbounded)
     hipEvent_t ev_pairs = nullptr;   // their copy is done
+    hipEvent_t ev_phase[2] = {nullptr, nullptr};   // contact terms + assembly on the stream: start, end
     bool use_graphs = true;          // ADB_PCG_GRAPH=0 turns graph capture off
     hipGraphExec_t pcg_graph = nullptr;   // one captured PCG batch
     static constexpr int PCG_GRAPH_KEYS = 20;

@@ -112,7 +112,11 @@ typedef struct adb_step_stats {
     long long contacts_processed; /* sum over Newton iterations of active contacts */
     double toi_min;            /* smallest CCD step bound seen                    */
     double t_ipc, t_solver, t_ccd, t_line_search; /* seconds, the reference's 4
-                                   phase buckets (model/abd.h:68, model/abd.cpp:110-121)              */
+                                   phase buckets (model/abd.h:68, model/abd.cpp:110-121);
+                                   t_ipc is the device time of the contact terms and
+                                   the assembly (HIP events: the solve is queued
+                                   behind them without a host wait), t_solver the
+                                   rest of the host time to the solve's end      */
     double t_total;
     /* Newton iterations of the step whose linear solve did not finish: PCG hit
      * pcg_max_iter before ||r|| <= pcg_rtol ||b||, or broke down (r.z <= 0 or
