@@ -1468,8 +1468,11 @@ void contact_terms(World& w) {
         }
     }
     const int nv = w.cs.n_vg;
-    w.vgK.reserve(12 * (size_t)std::max(nv, 1));
-    w.vglam.reserve(std::max(nv, 1));
+    // at least V / 8 entries: the count grows step by step while a pile
+    // settles, and each regrowth frees a buffer (a device-wide wait)
+    const size_t nvc = std::max<size_t>(std::max(nv, 1), (size_t)w.V / 8);
+    w.vgK.reserve(12 * nvc);
+    w.vglam.reserve(nvc);
     if (nv > 0) {
         VgArgs v;
         v.kp = make_kp(w.P); v.n = nv; v.ground_y = w.P.ground_y; v.vg_fric = w.P.vg_fric;

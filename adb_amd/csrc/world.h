@@ -27,13 +27,15 @@ struct AdbError {
     AdbError(int c, std::string w) : code(c), what(std::move(w)) {}
 };
 
-// Grow-only device buffer.
+// Grow-only device buffer.  A regrowth frees the old buffer, which waits for
+// the whole device, so each takes half again what is asked for: counts that
+// grow step by step (a settling pile) regrow rarely.
 template <class T> struct DBuf {
     T* p = nullptr; size_t cap = 0;
     void reserve(size_t n) {
         if (n <= cap) return;
         if (p) ADB_HIP(hipFree(p));
-        size_t c = n + n / 4 + 64;
+        size_t c = n + n / 2 + 64;
         ADB_HIP(hipMalloc((void**)&p, c * sizeof(T)));
         cap = c;
     }
