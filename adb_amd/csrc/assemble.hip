@@ -938,8 +938,12 @@ static double group_inv_flops(int m) {
 // Stable radix sort of (key, value) pairs on bits [0, end_bit) with rocPRIM:
 // its block sort + merge passes with 4096-item blocks up to 400k items (at
 // 1.9e5 body-pair keys 57 us against 65 with its default 1024-item blocks),
-// onesweep above (at 6e5 keys 91 us against 134-155 for the merge passes).
-using PairSortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::merge_sort_config<512, 512, 8>,
+// the merges odd-even over that whole range (one launch per pass; its default
+// switches to two-launch merge-path passes above ~2e5 items: 2.15e5 keys 84 ->
+// 60 us, 4e5 keys 113 -> 83 us), onesweep above (at 6e5 keys 91 us against
+// 134-155 for the merge passes).
+using PairSortConfig = rocprim::radix_sort_config<rocprim::default_config,
+                                                  rocprim::merge_sort_config<512, 512, 8, 128, 128, 4, 400000>,
                                                   rocprim::default_config, 400000>;
 template <class K, class V>
 static void sort_pairs(World& w, K* kin, K* kout, V* vin, V* vout, int n, int end_bit, hipStream_t st) {
