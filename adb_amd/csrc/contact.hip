@@ -86,16 +86,29 @@ __device__ __forceinline__ double rsq_r(double x) {
     double e = fma(-h * y, y, 0.5); y = fma(y, e, y);
     e = fma(-h * y, y, 0.5); return fma(y, e, y);
 }
+// The same with one Newton step: where only the rotation angle depends on
+// the value, an error there leaves a residual of that relative size in the
+// annihilated entry (removed by the next sweep), not a non-orthogonal rotation.
+__device__ __forceinline__ double rcp_r1(double x) {
+    const double y = __builtin_amdgcn_rcp(x);
+    return fma(y, fma(-x, y, 1.0), y);
+}
+__device__ __forceinline__ double rsq_r1(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    return fma(y, fma(-(0.5 * x) * y, y, 0.5), y);
+}
 // Jacobi rotation annihilating a_pq (Golub and Van Loan, Algorithm 8.5.1):
 // tau = (a_qq - a_pp) / (2 a_pq), t = sign(tau) / (|tau| + sqrt(1 + tau^2)),
 // c = 1 / sqrt(1 + t^2), s = t c; after it a_pp' = a_pp - t a_pq, a_qq' = a_qq + t a_pq.
+// t sets only the angle (one Newton step each); c keeps two, so that
+// c^2 + s^2 = 1 to rounding and the accumulated rotations stay orthogonal.
 __device__ __forceinline__ void jacobi_cs(double app, double aqq, double apq, double& c, double& s, double& t) {
-    const double tau = (aqq - app) * (0.5 * rcp_r(apq));
+    const double tau = (aqq - app) * (0.5 * rcp_r1(apq));
     if (fabs(tau) > 1e150) {
-        t = 0.5 * rcp_r(tau);
+        t = 0.5 * rcp_r1(tau);
     } else {
         const double x = fma(tau, tau, 1.0);
-        t = (tau >= 0 ? 1.0 : -1.0) * rcp_r(fma(x, rsq_r(x), fabs(tau)));      // x rsq(x) = sqrt(x)
+        t = (tau >= 0 ? 1.0 : -1.0) * rcp_r1(fma(x, rsq_r1(x), fabs(tau)));      // x rsq(x) = sqrt(x)
     }
     c = rsq_r(fma(t, t, 1.0));
     s = t * c;
