@@ -252,53 +252,74 @@ struct StructArgs {
     int* vg_start; int* chunk_n; int* chunk_off; int* chunk_pair; int* counts;
     int* host_pack;   // [pair count, npb pair_lo, npb pair_hi, npb + 1 pair_start]
 };
+// body arrays of at most STRUCT_LB entries (B + 1) and chunk arrays of at most
+// 5 STRUCT_LB / 2 entries (np + 1) are staged in LDS (40 KB, reused): the
+// phases between barriers then read their inputs from LDS instead of global
+// memory, and the outputs are written once
+constexpr int STRUCT_LB = 2048;
 __global__ __launch_bounds__(STRUCT_T) void k_structure(StructArgs a) {
     __shared__ int sh[STRUCT_T / 64];
-    const int np = pair_count(a.npb, a.npd), t = threadIdx.x;
+    __shared__ int lb[5 * STRUCT_LB];
+    const int np = pair_count(a.npb, a.npd), t = threadIdx.x, B = a.B;
+    const bool lb_on = B + 1 <= STRUCT_LB;
+    int* up = lb_on ? lb : a.up_start;
+    int* low = lb_on ? lb + STRUCT_LB : a.low_start;
+    int* vgs = lb_on ? lb + 2 * STRUCT_LB : a.vg_start;
+    int* rc = lb_on ? lb + 3 * STRUCT_LB : a.rowcnt;
+    int* rp = lb_on ? lb + 4 * STRUCT_LB : a.row_ptr;
     // k_body_ranges and k_vg_start without their binary searches (chains of
     // dependent loads): each body's first entry in a list sorted by body is
     // marked, the other bodies get the list length, and a suffix minimum over
     // the bodies gives every body the first entry at or after it
-    for (int b = t; b <= a.B; b += STRUCT_T) { a.up_start[b] = np; a.low_start[b] = np; a.vg_start[b] = a.n_vg; }
+    for (int b = t; b <= B; b += STRUCT_T) { up[b] = np; low[b] = np; vgs[b] = a.n_vg; }
     __syncthreads();
     for (int i = t; i < np; i += STRUCT_T) {
         const int k = a.pair_lo[i];
-        if (i == 0 || a.pair_lo[i - 1] != k) a.up_start[k] = i;
+        if (i == 0 || a.pair_lo[i - 1] != k) up[k] = i;
         const int h = (int)(a.tk[i] >> a.bb);
-        if (i == 0 || (int)(a.tk[i - 1] >> a.bb) != h) a.low_start[h] = i;
+        if (i == 0 || (int)(a.tk[i - 1] >> a.bb) != h) low[h] = i;
     }
     for (int i = t; i < a.n_vg; i += STRUCT_T) {
         const int k = a.vg[i].x;
-        if (i == 0 || a.vg[i - 1].x != k) a.vg_start[k] = i;
+        if (i == 0 || a.vg[i - 1].x != k) vgs[k] = i;
     }
     __syncthreads();
-    block_suffix_min(a.up_start, a.B + 1, sh);
-    block_suffix_min(a.low_start, a.B + 1, sh);
-    block_suffix_min(a.vg_start, a.B + 1, sh);
-    for (int b = t; b < a.B; b += STRUCT_T)
-        a.rowcnt[b] = (a.low_start[b + 1] - a.low_start[b]) + 1 + (a.up_start[b + 1] - a.up_start[b]);
+    block_suffix_min(up, B + 1, sh);
+    block_suffix_min(low, B + 1, sh);
+    block_suffix_min(vgs, B + 1, sh);
+    for (int b = t; b < B; b += STRUCT_T) rc[b] = (low[b + 1] - low[b]) + 1 + (up[b + 1] - up[b]);
     __syncthreads();
-    block_scan_excl(a.rowcnt, a.row_ptr, a.B, sh);
+    block_scan_excl(rc, rp, B, sh);
+    if (lb_on)
+        for (int b = t; b <= B; b += STRUCT_T) {
+            a.up_start[b] = up[b]; a.low_start[b] = low[b]; a.vg_start[b] = vgs[b]; a.row_ptr[b] = rp[b];
+        }
     // k_fill_cols
-    for (int b = t; b < a.B; b += STRUCT_T) {
-        int k = a.row_ptr[b];
-        for (int i = a.low_start[b]; i < a.low_start[b + 1]; i++, k++) {
+    for (int b = t; b < B; b += STRUCT_T) {
+        int k = rp[b];
+        for (int i = low[b]; i < low[b + 1]; i++, k++) {
             a.col_idx[k] = (int)(a.tk[i] & ((1ull << a.bb) - 1));
             a.pair_low[a.tv[i]] = k;
         }
         a.diag_blk[b] = k;
         a.col_idx[k++] = b;
-        for (int p = a.up_start[b]; p < a.up_start[b + 1]; p++, k++) {
+        for (int p = up[b]; p < up[b + 1]; p++, k++) {
             a.col_idx[k] = a.pair_hi[p];
             a.pair_up[p] = k;
         }
     }
+    __syncthreads();   // the LDS is reused below
     // k_pair_chunks, the scan, k_chunk_owner
-    for (int p = t; p < np; p += STRUCT_T) a.chunk_n[p] = (a.pair_start[p + 1] - a.pair_start[p] + PCH - 1) / PCH;
+    const bool lc_on = 2 * (np + 1) <= 5 * STRUCT_LB;
+    int* cn = lc_on ? lb : a.chunk_n;
+    int* co = lc_on ? lb + np + 1 : a.chunk_off;
+    for (int p = t; p < np; p += STRUCT_T) cn[p] = (a.pair_start[p + 1] - a.pair_start[p] + PCH - 1) / PCH;
     __syncthreads();
-    const int nch = block_scan_excl(a.chunk_n, a.chunk_off, np, sh);
+    const int nch = block_scan_excl(cn, co, np, sh);
     for (int p = t; p < np; p += STRUCT_T)
-        for (int c = a.chunk_off[p]; c < a.chunk_off[p + 1]; c++) a.chunk_pair[c] = p;
+        for (int c = co[p]; c < co[p + 1]; c++) a.chunk_pair[c] = p;
+    if (lc_on)
+        for (int p = t; p <= np; p += STRUCT_T) a.chunk_off[p] = co[p];
     if (t == 0) a.counts[3] = nch;
     // the pair lists for the host's group building, one copy
     if (t == 0) a.host_pack[0] = np;
