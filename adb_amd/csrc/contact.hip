@@ -14,9 +14,9 @@
 //   class 2 (three points, D = 6): record fields (k_contact_scalars2), then
 //     cyclic Jacobi on the packed 6 x 6 matrix in registers (k_contact_pk2);
 //   class 3 (four points, or any mollified EE, D = 9): the 9 x 9 matrix into
-//     the record, then cyclic Jacobi in registers (k_contact_c3r and
-//     k_contact_psd9, or k_contact_c3f, on a side stream; nine threads per
-//     contact build the matrix while the class is small).
+//     the record, then cyclic Jacobi in registers (k_contact_c3r, nine
+//     threads per contact building the matrix and a tenth the other record
+//     fields, then k_contact_psd9, one thread per contact, on a side stream).
 // Each contact writes a compact body-space record (world.h) that the
 // assembly expands into its four 12 x 12 blocks; the class 1 and class 2
 // record kernels stage it per wavefront in LDS and store it cooperatively
@@ -875,13 +875,6 @@ __host__ __device__ constexpr int P9(int p, int q) {
     return p <= q ? p * 9 - p * (p - 1) / 2 + (q - p) : q * 9 - q * (q - 1) / 2 + (p - q);
 }
 constexpr int C3_ROLES = 10;
-// below this many 4-point contacts the matrix is built by nine threads per
-// contact (ten with the record fields) and projected by k_contact_psd9: few
-// contacts leave most of the GPU idle, and the dependent chain per thread is
-// what counts.  At or above it, one thread per contact does everything (no
-// geometry recomputed per thread: the GPU is full and the total work counts).
-// Both paths compute every value with the same operations: identical records.
-constexpr int C3_SPLIT_MAX = 32768;
 struct C3Ctx {
     Geo G;
     int id0, id1, id2, id3;
@@ -1078,13 +1071,11 @@ __device__ __forceinline__ void psd9(const WorldArgs& a, int cid) {
             }
     }
 }
-// The two 4-point paths are separate kernels (both launched; the one the
-// device count does not choose returns at once), so the role threads of the
-// split path get the registers of their own code: the fused path's Jacobi
-// holds ~360 (one wavefront per SIMD), the roles far fewer.
+// The roles and the Jacobi are separate kernels, so the role threads get the
+// registers of their own code (two wavefronts per SIMD) while the Jacobi
+// holds ~280 (one).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_contact_c3r(WorldArgs a) {
     const int n3 = a.counts[2];
-    if (n3 >= C3_SPLIT_MAX) return;                   // k_contact_c3f
     const long long n = (long long)C3_ROLES * n3;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
         const int cid = a.list3[t / C3_ROLES], role = (int)(t % C3_ROLES);
@@ -1094,24 +1085,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         else c3_fields(a, cid, x);
     }
 }
-__global__ __launch_bounds__(256) void k_contact_c3f(WorldArgs a) {
-    const int n3 = a.counts[2];
-    if (n3 < C3_SPLIT_MAX) return;                    // k_contact_c3r + k_contact_psd9
-    {
-        for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n3; p += gridDim.x * blockDim.x) {
-            const int cid = a.list3[p];
-            C3Ctx x;
-            c3_setup(a, cid, x);
-#pragma unroll 1
-            for (int role = 0; role < 9; role++) c3_kbuild(x, role);
-            c3_fields(a, cid, x);
-            if (a.kp.psd) psd9(a, cid);
-        }
-    }
-}
 __global__ __launch_bounds__(64) void k_contact_psd9(WorldArgs a) {
     const int n3 = a.counts[2];
-    if (n3 >= C3_SPLIT_MAX) return;                   // projected by k_contact_c3f
     for (int p = blockIdx.x * 64 + threadIdx.x; p < n3; p += gridDim.x * 64) psd9(a, a.list3[p]);
 }
 
@@ -1446,7 +1421,6 @@ void contact_terms(World& w) {
                 // the class count is on the device: grids sized for the worst case, grid-stride loops
                 hipLaunchKernelGGL(k_contact_c3r, dim3(std::min((C3_ROLES * n + 255) / 256, 16 * w.num_cu)), dim3(256), 0,
                                    w.stream2, a);
-                hipLaunchKernelGGL(k_contact_c3f, dim3(std::min((n + 255) / 256, 16 * w.num_cu)), dim3(256), 0, w.stream2, a);
                 ADB_HIP(hipGetLastError());
                 if (a.kp.psd)
                     hipLaunchKernelGGL(k_contact_psd9, dim3(std::min((n + 63) / 64, 4 * w.num_cu)), dim3(64), 0, w.stream2, a);

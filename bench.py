@@ -146,7 +146,7 @@ def max_vertex_deviation(sc, qa, qb):
 # kernels launched once per launch of each profiling scope
 KERNEL_OF_SCOPE = {"asm_reduce": ["k_pair_reduce"], "narrow_set": ["k_pair_narrow<0", "k_pair_narrow<2", "k_trial_set"],
                    "narrow_ccd": ["k_pair_narrow<1"],
-                   "block_chol": ["k_group_inv"], "contact_group": ["k_contact_c3r", "k_contact_c3f", "k_contact_psd9"],
+                   "block_chol": ["k_group_inv"], "contact_group": ["k_contact_c3r", "k_contact_psd9"],
                    "contact": ["k_contact_small<1>", "k_contact_scalars2"], "contact_pk": ["k_contact_pk2"],
                    "spmv": ["k_pcg_spmv"], "pcg_update": ["k_pcg_update"], "contact_classify": ["k_contact_classify"]}
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "hbm_traffic_2000_icospheres.json")
