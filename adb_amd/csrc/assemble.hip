@@ -91,10 +91,12 @@ __global__ void k_pair_keys(int n_pt, int n_ee, const int4* __restrict__ pt, con
     keys[c] = ((unsigned long long)lo << bb) | (unsigned long long)hi;
     vals[c] = c;
 }
-__global__ void k_flags(int n, const unsigned long long* __restrict__ keys, int* __restrict__ flag) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
-}
+// 1 where a run of equal keys starts (read by the scan through an iterator:
+// the flags are never stored)
+struct FirstOfRun {
+    const unsigned long long* k;
+    __host__ __device__ int operator()(int i) const { return (i == 0 || k[i] != k[i - 1]) ? 1 : 0; }
+};
 // The pair count: *npd when the host sized the launches by an upper bound np
 // (the pair count stays on the device; clamped to the bound, which the host
 // checks when it reads the count), else np.
@@ -994,19 +996,20 @@ void assemble_structure(World& w, bool forked) {
     // 1. contacts sorted by body pair; pair segments
     w.akeys.reserve(std::max(nc, 1)); w.akeys_s.reserve(std::max(nc, 1));
     w.aval.reserve(std::max(nc, 1)); w.aval_s.reserve(std::max(nc, 1));
-    w.uniq_flag.reserve(std::max(nc, 1)); w.blk_of.reserve(std::max(nc, 1));
+    w.blk_of.reserve(std::max(nc, 1));
     int np = 0;
     if (nc > 0) {
         hipLaunchKernelGGL(k_pair_keys, dim3((nc + 255) / 256), dim3(256), 0, st, w.cs.n_pt, w.cs.n_ee, w.cs.pt.p, w.cs.ee.p, bb,
                            w.akeys.p, w.aval.p);
         ADB_HIP(hipGetLastError());
         sort_pairs(w, w.akeys.p, w.akeys_s.p, w.aval.p, w.aval_s.p, nc, 2 * bb, st);
-        hipLaunchKernelGGL(k_flags, dim3((nc + 255) / 256), dim3(256), 0, st, nc, w.akeys_s.p, w.uniq_flag.p);
-        ADB_HIP(hipGetLastError());
+        // pair segment of each sorted contact: inclusive scan of the run starts
+        const hipcub::TransformInputIterator<int, FirstOfRun, hipcub::CountingInputIterator<int>> starts(
+            hipcub::CountingInputIterator<int>(0), FirstOfRun{w.akeys_s.p});
         size_t tmp = 0;
-        ADB_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tmp, w.uniq_flag.p, w.blk_of.p, nc, st));
+        ADB_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tmp, starts, w.blk_of.p, nc, st));
         w.sort_tmp.reserve(tmp + 16);
-        ADB_HIP(hipcub::DeviceScan::InclusiveSum(w.sort_tmp.p, tmp, w.uniq_flag.p, w.blk_of.p, nc, st));
+        ADB_HIP(hipcub::DeviceScan::InclusiveSum(w.sort_tmp.p, tmp, starts, w.blk_of.p, nc, st));
     }
     // The pair count np = blk_of[nc - 1] stays on the device when the culling
     // pass that made the set bounded it (its overlapping body pairs): every

@@ -178,7 +178,7 @@ struct World {
     DBuf<double> bodyH, bodyG;       // inertia terms per body (144 + 12)
     DBuf<unsigned long long> akeys, akeys_s;   // body-pair key per contact
     DBuf<int> aval, aval_s;          // contact ids (sorted by body pair)
-    DBuf<int> blk_of, uniq_flag;     // per sorted contact: pair segment
+    DBuf<int> blk_of;                // per sorted contact: pair segment
     DBuf<int> pair_pack;             // pair count and lists packed for one copy to h_pairs
     // body pairs (lo < hi) with contacts, sorted: first contact, bodies, block
     // index in row lo (upper) and row hi (lower); the pairs keyed (hi, lo)

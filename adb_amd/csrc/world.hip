@@ -554,7 +554,7 @@ extern "C" int adb_world_create(const adb_scene* s, const adb_params* p, adb_wor
 
 static void free_world(World& w) {
     DBuf<int>* ib[] = {&w.voff, &w.foff, &w.eoff, &w.vbody, &w.fbody, &w.ebody, &w.face, &w.edge, &w.counters,
-                       &w.aval, &w.aval_s, &w.blk_of, &w.uniq_flag, &w.row_ptr, &w.col_idx, &w.diag_blk, &w.vg_start,
+                       &w.aval, &w.aval_s, &w.blk_of, &w.row_ptr, &w.col_idx, &w.diag_blk, &w.vg_start,
                        &w.pair_start, &w.pair_lo, &w.pair_hi, &w.pair_up, &w.pair_low, &w.tval, &w.tval_s,
                        &w.up_start, &w.low_start, &w.rowcnt,
                        &w.ctype, &w.ccls, &w.clist, &w.clist3, &w.ccount, &w.chunk_n, &w.chunk_off, &w.chunk_blk,
